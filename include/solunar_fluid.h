@@ -1,0 +1,231 @@
+/*
+ * solunar_fluid.h: C ABI of the MI355X (gfx950) SPH particle-fluid step.
+ *
+ * One handle (`sfl_world`) stands for one reference `FluidWorld`
+ * (src/fluid/mod.rs:310-339). Every call returns an `int32_t` status whose
+ * non-zero values map one to one onto the reference's `PhysicsError`
+ * variants (src/error.rs:5-22), plus SFL_DEVICE_ERROR and SFL_UNSUPPORTED.
+ *
+ * Contracts
+ *  - Buffers are caller owned host memory. They are copied in or out during
+ *    the call; no pointer is kept.
+ *  - A world owns one HIP stream on the device named in its settings. A
+ *    handle may move between threads but must not be used by two at once
+ *    (the reference's `&mut self`).
+ *  - Bad input changes nothing (src/fluid/mod.rs:915-972).
+ *  - Vectors are three packed floats (x, y, z); arrays of them are
+ *    interleaved xyz, 12 bytes per element.
+ */
+#ifndef SOLUNAR_FLUID_H
+#define SOLUNAR_FLUID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: the PhysicsError variants in declaration order
+ * (src/error.rs:5-22), offset by one so zero is success. */
+enum sfl_status {
+  SFL_OK = 0,
+  SFL_INVALID_HANDLE = 1,
+  SFL_INVALID_FRAME = 2,
+  SFL_FRAME_TOO_DEEP = 3,
+  SFL_NON_FINITE = 4, /* detail string from sfl_error_detail */
+  SFL_INVALID_SHAPE = 5,
+  SFL_INVALID_MATERIAL = 6,
+  SFL_INVALID_SETTINGS = 7,
+  SFL_NOT_DYNAMIC = 8,
+  SFL_NOT_KINEMATIC = 9,
+  SFL_OUT_OF_RANGE = 10,
+  SFL_DISPATCH_INCOMPLETE = 11,
+  SFL_INVALID_JOINT = 12,
+  SFL_BODY_HAS_JOINTS = 13,
+  SFL_SHAPE_IN_USE = 14,
+  SFL_TERRAIN_BUSY = 15,
+  SFL_FLUID_FULL = 16,
+  /* No reference counterpart: a HIP call failed (detail names it). */
+  SFL_DEVICE_ERROR = 100,
+  /* No reference counterpart: a feature the device path does not carry. */
+  SFL_UNSUPPORTED = 101
+};
+
+enum sfl_solver { SFL_WCSPH = 0, SFL_DFSPH = 1 };
+
+typedef struct sfl_vec3 {
+  float x, y, z;
+} sfl_vec3;
+
+/* Mirrors FluidSettings (src/fluid/mod.rs:55-78) field for field, then the
+ * DFSPH constants of src/fluid/dfsph.rs:8-16 made settable, then the device.
+ * sfl_settings_default gives the reference defaults (mod.rs:80-95,
+ * dfsph.rs:8-16). */
+typedef struct sfl_settings {
+  int32_t solver; /* enum sfl_solver */
+  float spacing;
+  float rest_density;
+  float support;
+  uint64_t max_particles;
+  float sound_speed;
+  float viscosity;
+  sfl_vec3 gravity;
+  float courant;
+  uint32_t max_substeps;
+  /* extensions, defaults = reference constants */
+  float divergence_tolerance; /* DIVERGENCE_TOLERANCE, dfsph.rs:11 */
+  float density_tolerance;    /* DENSITY_TOLERANCE, dfsph.rs:9 */
+  uint32_t min_iterations;    /* MIN_ITERATIONS, dfsph.rs:12 */
+  uint32_t max_iterations;    /* MAX_ITERATIONS, dfsph.rs:13 */
+  int32_t device;             /* HIP device ordinal */
+} sfl_settings;
+
+/* Mirrors FluidReport (src/fluid/step.rs:28-49); the last two fields split
+ * `iterations` by solve. Booleans are 0/1. */
+typedef struct sfl_report {
+  uint32_t substeps;
+  uint32_t capped;
+  float max_compression;
+  float max_speed;
+  uint32_t held_back;
+  uint32_t iterations;
+  uint32_t emitted;
+  uint32_t drained;
+  uint32_t lost;
+  float sound_speed;
+  uint32_t iterations_divergence;
+  uint32_t iterations_density;
+} sfl_report;
+
+/* FluidSettings::default (src/fluid/mod.rs:80-95). */
+void sfl_settings_default(sfl_settings* out);
+
+/* FluidWorld::new (src/fluid/mod.rs:344). SFL_INVALID_SETTINGS on settings
+ * out of range (mod.rs:98-117); SFL_DEVICE_ERROR if the device is absent. */
+typedef struct sfl_world sfl_world;
+int32_t sfl_create(const sfl_settings* settings, sfl_world** out);
+/* Drop for FluidWorld. Null is ignored. */
+void sfl_destroy(sfl_world* world);
+
+/* FluidWorld::settings / kernel radius / particle_mass / len
+ * (src/fluid/mod.rs:372-390). */
+int32_t sfl_get_settings(const sfl_world* world, sfl_settings* out);
+float sfl_particle_mass(const sfl_world* world);
+size_t sfl_len(const sfl_world* world);
+
+/* FluidWorld::add_particle (src/fluid/mod.rs:412-426), one call per
+ * particle in order; stops at the first failure, earlier ones stay added
+ * (as a loop of add_particle calls would). `ids_out` may be null. */
+int32_t sfl_add_particles(sfl_world* world, size_t count, const float* xyz,
+                          const float* vxyz, uint32_t* ids_out);
+
+/* FluidWorld::fill_box (src/fluid/mod.rs:493-524). */
+int32_t sfl_fill_box(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
+                     size_t* count_out);
+
+/* fill_box with every lattice site nudged by a seeded xorshift32 offset of up
+ * to +-jitter*spacing per axis, drawn x, y, z per particle in fill order.
+ * jitter = 0 is fill_box. The generator is the one the reference's tests use
+ * (src/fluid/mod.rs:755-761): u = state / 4294967295, offset =
+ * (u - 0.5) * (2 * jitter * spacing). A seed of 0 is replaced by 0x9E3779B9. */
+int32_t sfl_fill_box_jittered(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
+                              float jitter, uint32_t seed, size_t* count_out);
+
+/* FluidWorld::set_domain (src/fluid/mod.rs:433-444). A non-null box is
+ * validated, then refused with SFL_UNSUPPORTED: domain culling is not on the
+ * device path. Null clears it. */
+int32_t sfl_set_domain(sfl_world* world, const sfl_vec3* low,
+                       const sfl_vec3* high);
+
+/* add_box_container / add_wall (src/fluid/boundary.rs:41-91). */
+int32_t sfl_add_box_container(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
+                              uint32_t* boundary_out);
+int32_t sfl_add_wall(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
+                     uint32_t* boundary_out);
+/* add_wall_points (src/fluid/boundary.rs:139-151) for arbitrary points, such
+ * as spheres sampled as boundary particles. Points must be finite. */
+int32_t sfl_add_boundary_points(sfl_world* world, size_t count,
+                                const float* xyz, uint32_t* boundary_out);
+/* remove_boundary (src/fluid/boundary.rs:94-115); *removed_out = 0/1. */
+int32_t sfl_remove_boundary(sfl_world* world, uint32_t boundary,
+                            int32_t* removed_out);
+size_t sfl_wall_len(const sfl_world* world);
+/* wall_positions (boundary.rs:118-120), in their settled order. */
+int32_t sfl_read_wall_positions(const sfl_world* world, size_t cap,
+                                float* xyz);
+
+/* FluidWorld::step (src/fluid/step.rs:58-90). The dispatcher argument of the
+ * reference has no counterpart: the device path has one schedule. */
+int32_t sfl_step(sfl_world* world, float dt, sfl_report* report_out);
+
+/* update_neighbors / compute_densities (src/fluid/mod.rs:540-675): the
+ * neighbour build and the density pass alone, for parity checks. */
+int32_t sfl_update_neighbors(sfl_world* world);
+int32_t sfl_compute_densities(sfl_world* world);
+
+/* Readback of ids / positions / velocities / densities
+ * (src/fluid/mod.rs:393-408), `cap` elements at most. */
+int32_t sfl_read_ids(const sfl_world* world, size_t cap, uint32_t* out);
+int32_t sfl_read_positions(const sfl_world* world, size_t cap, float* xyz);
+int32_t sfl_read_velocities(const sfl_world* world, size_t cap, float* xyz);
+int32_t sfl_read_densities(const sfl_world* world, size_t cap, float* out);
+/* Per-particle acceleration of the last WCSPH substep (pressure, viscosity,
+ * walls and gravity: the vector step.rs:207 returns) or of the last DFSPH
+ * viscosity pass (dfsph.rs:94). Needs sfl_keep_forces(world, 1) before the
+ * step. */
+int32_t sfl_keep_forces(sfl_world* world, int32_t keep);
+int32_t sfl_read_forces(const sfl_world* world, size_t cap, float* xyz);
+
+/* Neighbour lists of the last neighbour build (src/fluid/mod.rs:279-308,
+ * 610-614): CSR `start` (len+1 entries) and `items`. kind 0 = fluid list,
+ * 1 = wall list. *items_len_out receives the item count even if `items` is
+ * too small (then nothing is copied and SFL_OUT_OF_RANGE is returned). */
+int32_t sfl_read_neighbors(sfl_world* world, int32_t kind, size_t start_cap,
+                           uint32_t* start, size_t items_cap, uint32_t* items,
+                           size_t* items_len_out);
+
+/* FluidWorld::state_hash (src/fluid/mod.rs:678-691): FNV-1a 64
+ * (src/snapshot/bytes.rs:6-13) over (id, x, v) in id order. */
+int32_t sfl_state_hash(const sfl_world* world, uint64_t* out);
+
+/* Detail of the last non-zero status on this world (NonFinite's `what`, or
+ * the HIP call that failed). Never null. Valid until the next call. */
+const char* sfl_error_detail(const sfl_world* world);
+/* Display text of a status code (src/error.rs:24-45). */
+const char* sfl_status_text(int32_t status);
+
+/* ---- measurement (no reference counterpart) ---------------------------- */
+
+/* Per-kernel timing with HIP events recorded on the world's stream around
+ * every launch. Off by default; enabling adds one event pair per launch. */
+enum sfl_phase {
+  SFL_PHASE_BOUNDS = 0,   /* cell bounds of the particles */
+  SFL_PHASE_SORT = 1,     /* cell count, scan, scatter, in-cell id sort */
+  SFL_PHASE_REORDER = 2,  /* gather of x, v, id into sorted order */
+  SFL_PHASE_DENSITY = 3,  /* density (+ DFSPH factors) */
+  SFL_PHASE_FORCE = 4,    /* WCSPH force + integrate */
+  SFL_PHASE_SOLVE = 5,    /* DFSPH persistent Jacobi solves */
+  SFL_PHASE_VISCOUS = 6,  /* DFSPH viscosity + gravity */
+  SFL_PHASE_ADVANCE = 7,  /* DFSPH advance */
+  SFL_PHASE_COUNT = 8
+};
+int32_t sfl_profile(sfl_world* world, int32_t enable);
+/* ms_out[SFL_PHASE_COUNT] summed milliseconds, launches_out[SFL_PHASE_COUNT]
+ * launch counts since the last reset; either may be null. */
+int32_t sfl_profile_read(sfl_world* world, double* ms_out,
+                         uint64_t* launches_out, int32_t reset);
+/* Fluid particle-substeps and Jacobi pushes since the last reset, for the
+ * algorithmic byte count. */
+int32_t sfl_work_read(sfl_world* world, uint64_t* particle_substeps,
+                      uint64_t* particle_pushes, uint64_t* particle_rates,
+                      int32_t reset);
+
+/* Blocks until every launch of the world has finished. */
+int32_t sfl_synchronize(sfl_world* world);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOLUNAR_FLUID_H */

@@ -1,0 +1,1184 @@
+// TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// CPU restatement of solunar's SPH particle-fluid step, the oracle that the
+// HIP path is checked against. Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it; the product library never links
+// or calls it.
+//
+// It follows, in operation order, the reference files
+//   src/fluid/mod.rs       (settings, kernel, grid, neighbour lists, density,
+//                           state hash, rest mass, chunked jobs)
+//   src/fluid/step.rs      (WCSPH step, viscosity, wall viscosity, advance)
+//   src/fluid/dfsph.rs     (DFSPH step, factors, density rate, Jacobi solve)
+//   src/fluid/boundary.rs  (walls, containers, settle order)
+//   src/snapshot/bytes.rs:6-13 (FNV-1a)
+//   src/dispatch.rs, src/error.rs
+// with each function citing the lines it restates. `R` is float for the
+// reference arithmetic and double for the FP64 yardstick; with R = float and
+// -ffp-contract=off every operation is the IEEE single operation the Rust
+// code performs, under these assumptions about the `algebrix` crate that the
+// reference cannot pin (its source is not available):
+//   * Vec3 / f32 divides each component; dot and length_squared are
+//     (x*x' + y*y') + z*z'; length is the sqrt of that;
+//   * normalize_or(v, f) is v * (1/|v|) when 1/|v| is finite and > 0, else f;
+//   * clamp is max then min per component (NaN-ignoring, as f32::max/min);
+//   * powi(n) is LLVM's square-and-multiply: x^3 = x*x^2, x^4 = (x^2)^2,
+//     x^7 = (x*x^2)*x^4;
+//   * f32 `sum` is a left fold.
+//
+// Pinning: the tests check this restatement against the reference's own
+// known answers and invariants (kernel integral and slope, dense table versus
+// search, brute-force neighbours, rest lattice density, thread-count
+// invariance, sort invariance of the hash, refused input, Tait hydrostatic
+// squeeze, ideal dam-break front, Martin-Moyce front, gate), plus FNV-1a
+// known answers. The DFSPH restatement has no physical known answer in the
+// reference (it is marked "Not ready" there): parity unpinned beyond the
+// invariants (thread invariance, f32 versus f64 agreement, divergence drop).
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <optional>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+namespace oracle {
+
+// src/error.rs:5-22, offset by one (0 is success)
+enum Status : int32_t {
+  kOk = 0,
+  kNonFinite = 4,
+  kInvalidSettings = 7,
+  kOutOfRange = 10,
+  kDispatchIncomplete = 11,
+  kFluidFull = 16,
+  kUnsupported = 101,
+};
+
+template <class R>
+struct Vec3 {
+  R x, y, z;
+  bool finite() const { return std::isfinite(x) && std::isfinite(y) && std::isfinite(z); }
+  bool operator==(const Vec3& o) const { return x == o.x && y == o.y && z == o.z; }
+  bool operator!=(const Vec3& o) const { return !(*this == o); }
+};
+template <class R> inline Vec3<R> operator+(Vec3<R> a, Vec3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline Vec3<R> operator-(Vec3<R> a, Vec3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline Vec3<R> operator-(Vec3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> inline Vec3<R> operator*(Vec3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline Vec3<R> operator/(Vec3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> inline R dot(Vec3<R> a, Vec3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline R length_squared(Vec3<R> a) { return dot(a, a); }
+template <class R> inline R length(Vec3<R> a) { return std::sqrt(length_squared(a)); }
+
+// f32::max / f32::min: a NaN operand yields the other one.
+template <class R> inline R rmax(R a, R b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a > b ? a : b)); }
+template <class R> inline R rmin(R a, R b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a < b ? a : b)); }
+template <class R> inline Vec3<R> vmax(Vec3<R> a, Vec3<R> b) { return {rmax(a.x, b.x), rmax(a.y, b.y), rmax(a.z, b.z)}; }
+template <class R> inline Vec3<R> vmin(Vec3<R> a, Vec3<R> b) { return {rmin(a.x, b.x), rmin(a.y, b.y), rmin(a.z, b.z)}; }
+template <class R> inline Vec3<R> clamp(Vec3<R> v, Vec3<R> lo, Vec3<R> hi) { return vmin(vmax(v, lo), hi); }
+template <class R> inline Vec3<R> normalize_or(Vec3<R> v, Vec3<R> fallback) {
+  R rcp = R(1) / length(v);
+  if (std::isfinite(rcp) && rcp > R(0)) return v * rcp;
+  return fallback;
+}
+
+// `x as i32` in Rust: saturating, NaN -> 0.
+inline int32_t sat_i32(double v) {
+  if (std::isnan(v)) return 0;
+  if (v >= 2147483647.0) return std::numeric_limits<int32_t>::max();
+  if (v <= -2147483648.0) return std::numeric_limits<int32_t>::min();
+  return static_cast<int32_t>(v);
+}
+
+enum Solver : int32_t { kWcsph = 0, kDfsph = 1 };
+
+// FluidSettings (src/fluid/mod.rs:55-95) and the DFSPH constants of
+// src/fluid/dfsph.rs:8-16 made settable. Always single precision, as in the
+// reference; the FP64 world converts them exactly.
+struct Settings {
+  int32_t solver = kWcsph;
+  float spacing = 0.05f;
+  float rest_density = 1000.0f;
+  float support = 2.0f;
+  uint64_t max_particles = uint64_t(1) << 22;
+  float sound_speed = 20.0f;
+  float viscosity = 0.15f;
+  Vec3<float> gravity{0.0f, -9.81f, 0.0f};
+  float courant = 0.4f;
+  uint32_t max_substeps = 2000;
+  float divergence_tolerance = 1e-3f;
+  float density_tolerance = 5e-4f;
+  uint32_t min_iterations = 2;
+  uint32_t max_iterations = 100;
+
+  // FluidSettings::validate (src/fluid/mod.rs:98-117)
+  bool valid() const {
+    auto positive = [](float v) { return std::isfinite(v) && v > 0.0f; };
+    bool support_ok = support >= 1.5f && support <= 4.0f;  // SUPPORT_RANGE, mod.rs:37
+    return positive(spacing) && positive(rest_density) && support_ok && max_particles > 0 &&
+           positive(sound_speed) && std::isfinite(viscosity) && viscosity >= 0.0f &&
+           gravity.finite() && courant > 0.0f && courant <= 1.0f && max_substeps > 0 &&
+           (solver == kWcsph || solver == kDfsph) && std::isfinite(divergence_tolerance) &&
+           std::isfinite(density_tolerance) && max_iterations >= min_iterations;
+  }
+  float radius() const { return spacing * support; }  // mod.rs:120-122
+};
+
+// Wendland C2 in 3D (src/fluid/mod.rs:126-165).
+template <class R>
+struct Kernel {
+  R radius, scale;
+  explicit Kernel(R r) : radius(r) {
+    R r3 = r * (r * r);  // powi(3)
+    R two_pi = R(2) * R(kPi);
+    scale = R(21) / (two_pi * r3);
+  }
+  static constexpr double kPi = std::is_same<R, float>::value ? double(3.14159265358979323846f) : 3.14159265358979323846;
+  R value(R distance) const {
+    R q = distance / radius;
+    if (q >= R(1)) return R(0);
+    R t = R(1) - q;
+    R t2 = t * t;
+    R t4 = t2 * t2;  // powi(4)
+    return scale * t4 * (R(1) + R(4) * q);
+  }
+  R slope(R distance) const {
+    R q = distance / radius;
+    if (q >= R(1)) return R(0);
+    R t = R(1) - q;
+    R t3 = t * (t * t);  // powi(3)
+    return scale * R(-20) * q * t3 / radius;
+  }
+  Vec3<R> gradient(Vec3<R> offset) const {
+    R distance = length(offset);
+    if (distance <= R(0)) return {R(0), R(0), R(0)};
+    return offset * (slope(distance) / distance);
+  }
+};
+
+struct CellKey {
+  int32_t x, y, z;
+  bool operator==(const CellKey& o) const { return x == o.x && y == o.y && z == o.z; }
+  bool operator!=(const CellKey& o) const { return !(*this == o); }
+  bool operator<(const CellKey& o) const { return std::tie(x, y, z) < std::tie(o.x, o.y, o.z); }
+};
+
+struct Range {
+  uint32_t begin, end;
+  bool operator==(const Range& o) const { return begin == o.begin && end == o.end; }
+};
+
+// Grid (src/fluid/mod.rs:169-262): occupied cells in key order with their
+// ranges, plus a dense slot table when the box is small enough.
+struct Grid {
+  std::vector<std::pair<CellKey, Range>> cells;
+  std::vector<uint32_t> dense;
+  CellKey low{0, 0, 0};
+  int64_t span[3] = {0, 0, 0};
+
+  // Grid::build (mod.rs:182-197); keys must be sorted.
+  template <class It>
+  static Grid build(It first, It last) {
+    Grid g;
+    uint32_t index = 0;
+    for (It it = first; it != last; ++it, ++index) {
+      CellKey cell = *it;
+      if (!g.cells.empty() && g.cells.back().first == cell) {
+        g.cells.back().second.end = index + 1;
+      } else {
+        g.cells.push_back({cell, {index, index + 1}});
+      }
+    }
+    g.index();
+    return g;
+  }
+
+  // Grid::index (mod.rs:199-227)
+  void index() {
+    if (cells.empty()) return;
+    CellKey lo = cells.front().first, hi = lo;
+    for (auto& c : cells) {
+      lo = {std::min(lo.x, c.first.x), std::min(lo.y, c.first.y), std::min(lo.z, c.first.z)};
+      hi = {std::max(hi.x, c.first.x), std::max(hi.y, c.first.y), std::max(hi.z, c.first.z)};
+    }
+    auto side = [](int32_t a, int32_t b) { return int64_t(b) - int64_t(a) + 1; };
+    int64_t s[3] = {side(lo.x, hi.x), side(lo.y, hi.y), side(lo.z, hi.z)};
+    // saturating products
+    auto sat_mul = [](int64_t a, int64_t b) -> int64_t {
+      __int128 p = (__int128)a * b;
+      if (p > std::numeric_limits<int64_t>::max()) return std::numeric_limits<int64_t>::max();
+      return (int64_t)p;
+    };
+    int64_t volume = sat_mul(sat_mul(s[0], s[1]), s[2]);
+    if (volume > 64 * (int64_t)cells.size() + 4096) return;  // DENSE_PER_CELL, DENSE_SLACK
+    dense.assign((size_t)volume, UINT32_MAX);
+    for (size_t slot = 0; slot < cells.size(); ++slot) {
+      const CellKey& c = cells[slot].first;
+      int64_t at = (int64_t(c.x - lo.x) * s[1] + int64_t(c.y - lo.y)) * s[2] + int64_t(c.z - lo.z);
+      dense[(size_t)at] = (uint32_t)slot;
+    }
+    low = lo;
+    span[0] = s[0];
+    span[1] = s[1];
+    span[2] = s[2];
+  }
+
+  // Grid::range (mod.rs:229-247)
+  std::optional<Range> range(CellKey key) const {
+    size_t slot;
+    if (dense.empty()) {
+      auto it = std::lower_bound(cells.begin(), cells.end(), key,
+                                 [](const std::pair<CellKey, Range>& c, const CellKey& k) { return c.first < k; });
+      if (it == cells.end() || it->first != key) return std::nullopt;
+      slot = (size_t)(it - cells.begin());
+    } else {
+      auto offset = [](int32_t v, int32_t lo, int64_t sp, int64_t& out) {
+        int64_t at = int64_t(v) - int64_t(lo);
+        if (at < 0 || at >= sp) return false;
+        out = at;
+        return true;
+      };
+      int64_t x, y, z;
+      if (!offset(key.x, low.x, span[0], x) || !offset(key.y, low.y, span[1], y) || !offset(key.z, low.z, span[2], z))
+        return std::nullopt;
+      uint32_t s = dense[(size_t)((x * span[1] + y) * span[2] + z)];
+      if (s == UINT32_MAX) return std::nullopt;
+      slot = s;
+    }
+    return cells[slot].second;
+  }
+
+  // Grid::around (mod.rs:250-261): dz outer, dy, dx inner.
+  void around(CellKey c, std::vector<Range>& out) const {
+    out.clear();
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          auto r = range({c.x + dx, c.y + dy, c.z + dz});
+          if (r) out.push_back(*r);
+        }
+  }
+};
+
+// Lists (src/fluid/mod.rs:279-308)
+struct Lists {
+  std::vector<uint32_t> start;
+  std::vector<uint32_t> items;
+  size_t count(size_t index) const {
+    if (index + 1 >= start.size()) return 0;
+    return start[index + 1] - start[index];
+  }
+};
+
+// in_chunks / in_chunks_of (src/fluid/mod.rs:711-740) on `threads` workers
+// with the CHUNK of mod.rs:36. Work writes per-index results, so the merge in
+// chunk order is the identity. `skip` drops jobs, like dispatch::testing::
+// SkipsJobs (src/dispatch.rs:50-65), to exercise DispatchIncomplete.
+struct Dispatch {
+  static constexpr size_t kChunk = 512;
+  int threads = 1;
+  std::function<bool(size_t)> skip;
+
+  // Returns false when a job was skipped (DispatchIncomplete).
+  template <class F>
+  bool run(size_t count, F&& work) const {
+    size_t jobs = (count + kChunk - 1) / kChunk;
+    std::vector<uint8_t> done(jobs, 0);
+    auto job = [&](size_t j) {
+      size_t b = j * kChunk, e = std::min(count, b + kChunk);
+      work(b, e);
+      done[j] = 1;
+    };
+    if (threads <= 1 || jobs <= 1) {
+      for (size_t j = 0; j < jobs; ++j)
+        if (!(skip && skip(j))) job(j);
+    } else {
+      // run backwards like dispatch::testing::Threads (src/dispatch.rs:28-48)
+      std::atomic<size_t> next{0};
+      std::vector<std::thread> pool;
+      int t = std::min<int>(threads, (int)jobs);
+      for (int w = 0; w < t; ++w)
+        pool.emplace_back([&]() {
+          for (;;) {
+            size_t k = next.fetch_add(1);
+            if (k >= jobs) break;
+            size_t j = jobs - 1 - k;
+            if (!(skip && skip(j))) job(j);
+          }
+        });
+      for (auto& th : pool) th.join();
+    }
+    for (uint8_t d : done)
+      if (!d) return false;
+    return true;
+  }
+};
+
+// FluidReport (src/fluid/step.rs:28-49), iterations split by solve.
+struct Report {
+  uint32_t substeps = 0;
+  uint32_t capped = 0;
+  float max_compression = 0;
+  float max_speed = 0;
+  uint32_t held_back = 0;
+  uint32_t iterations = 0;
+  uint32_t emitted = 0, drained = 0, lost = 0;
+  float sound_speed = 0;
+  uint32_t iterations_divergence = 0;
+  uint32_t iterations_density = 0;
+};
+
+// How the DFSPH solve averages its per-particle errors.
+//  kReferenceF32: the reference's f32 left fold in particle order
+//                 (src/fluid/dfsph.rs:263-267).
+//  kFixedTreeF64: the device path's FP64 sum of fixed shape: chunks of 512
+//                 particles; lane l of 64 sums e[512c + l + 64k], k = 0..7,
+//                 in order; lanes combine by halving (s[l] += s[l + h],
+//                 h = 32..1); chunk sums go to 256 slots, slot t summing chunks
+//                 t, t + 256, ... in order; slots combine by halving
+//                 (h = 128..1). Count of errors > 0 is exact.
+enum ResidualMode : int32_t { kReferenceF32 = 0, kFixedTreeF64 = 1 };
+
+template <class E>
+inline double fixed_tree_sum(const std::vector<E>& e) {
+  size_t n = e.size();
+  size_t chunks = (n + 511) / 512;
+  std::vector<double> partial(chunks);
+  for (size_t c = 0; c < chunks; ++c) {
+    double s[64];
+    for (int l = 0; l < 64; ++l) {
+      double acc = 0.0;
+      for (int k = 0; k < 8; ++k) {
+        size_t i = c * 512 + (size_t)l + 64 * (size_t)k;
+        if (i < n) acc += (double)e[i];
+      }
+      s[l] = acc;
+    }
+    for (int h = 32; h >= 1; h >>= 1)
+      for (int l = 0; l < h; ++l) s[l] = s[l] + s[l + h];
+    partial[c] = s[0];
+  }
+  double slot[256];
+  for (int t = 0; t < 256; ++t) {
+    double acc = 0.0;
+    for (size_t c = (size_t)t; c < chunks; c += 256) acc += partial[c];
+    slot[t] = acc;
+  }
+  for (int h = 128; h >= 1; h >>= 1)
+    for (int t = 0; t < h; ++t) slot[t] = slot[t] + slot[t + h];
+  return slot[0];
+}
+
+// One Jacobi iteration's stop inputs, for logging the two averages side by
+// side (the f32 fold the reference uses and the FP64 sum the device uses).
+struct SolveLog {
+  int32_t solve;  // 0 divergence, 1 density
+  uint32_t iteration;
+  uint32_t loaded;
+  float average_f32;
+  double average_f64;
+};
+
+template <class R>
+class World {
+ public:
+  using V = Vec3<R>;
+  static V cvt(Vec3<float> v) { return {R(v.x), R(v.y), R(v.z)}; }
+
+  Settings settings;
+  Kernel<R> kernel;
+  R mass;
+  std::vector<uint32_t> ids;
+  std::vector<V> positions, velocities;
+  std::vector<R> densities;
+  std::vector<V> forces;  // last pushes, for parity readback
+  uint32_t next_id = 0;
+  Grid grid;
+  Lists neighbors, wall_neighbors;
+  bool neighbors_built = false;
+
+  // Walls (src/fluid/boundary.rs:23-33)
+  struct Walls {
+    std::vector<V> positions;
+    std::vector<uint32_t> owners;
+    Grid grid;
+    uint32_t next = 0;
+    std::optional<std::pair<V, V>> bounds;
+    std::vector<std::tuple<uint32_t, V, V>> containers;
+  } walls;
+
+  Dispatch dispatch;
+  ResidualMode residual_mode = kReferenceF32;
+  std::vector<SolveLog> solve_log;
+  bool log_solves = false;
+  std::string detail;
+
+  explicit World(const Settings& s) : settings(s), kernel(R(s.spacing) * R(s.support)) {
+    // radius() is computed in f32 (mod.rs:120); the f64 world takes the f32
+    // product exactly
+    kernel = Kernel<R>(R(s.radius()));
+    mass = rest_mass();
+  }
+
+  R radius() const { return R(settings.radius()); }
+  size_t len() const { return ids.size(); }
+
+  // rest_mass (src/fluid/mod.rs:694-709)
+  R rest_mass() const {
+    Kernel<R> k(radius());
+    int reach = (int)std::ceil(settings.support);
+    R sum = R(0);
+    R spacing = R(settings.spacing);
+    for (int z = -reach; z <= reach; ++z)
+      for (int y = -reach; y <= reach; ++y)
+        for (int x = -reach; x <= reach; ++x) {
+          V offset = V{R(x), R(y), R(z)} * spacing;
+          sum += k.value(length(offset));
+        }
+    return R(settings.rest_density) / sum;
+  }
+
+  // add_particle (src/fluid/mod.rs:412-426)
+  int32_t add_particle(V position, V velocity, uint32_t* id_out) {
+    if (!position.finite() || !velocity.finite()) {
+      detail = "fluid particle";
+      return kNonFinite;
+    }
+    if (len() >= settings.max_particles || next_id == UINT32_MAX) return kFluidFull;
+    uint32_t id = next_id++;
+    ids.push_back(id);
+    positions.push_back(position);
+    velocities.push_back(velocity);
+    densities.push_back(R(0));
+    neighbors_built = false;
+    if (id_out) *id_out = id;
+    return kOk;
+  }
+
+  // fill_box (src/fluid/mod.rs:493-524); jitter > 0 adds the seeded
+  // xorshift32 offsets described in include/solunar_fluid.h
+  // (sfl_fill_box_jittered). Arithmetic stays in f32: the lattice is input.
+  int32_t fill_box(Vec3<float> low, Vec3<float> high, float jitter, uint32_t seed, size_t* count_out) {
+    if (!low.finite() || !high.finite()) {
+      detail = "fluid box";
+      return kNonFinite;
+    }
+    if (high.x < low.x || high.y < low.y || high.z < low.z) return kOutOfRange;
+    float spacing = settings.spacing;
+    auto count = [&](float extent) -> uint64_t {
+      float c = std::floor(extent / spacing);
+      c = std::min(c, 4294967295.0f);
+      return (uint64_t)sat_u64(c);
+    };
+    Vec3<float> size = {high.x - low.x, high.y - low.y, high.z - low.z};
+    uint64_t nx = count(size.x), ny = count(size.y), nz = count(size.z);
+    unsigned __int128 t = (unsigned __int128)nx * ny * nz;
+    uint64_t total = t > UINT64_MAX ? UINT64_MAX : (uint64_t)t;
+    uint64_t room = settings.max_particles > len() ? settings.max_particles - len() : 0;
+    uint64_t ids_left = (uint64_t)(UINT32_MAX - next_id);
+    if (total > room || total > ids_left) return kFluidFull;
+    float half = spacing * 0.5f;
+    Vec3<float> start = {low.x + half, low.y + half, low.z + half};
+    uint32_t state = seed ? seed : 0x9E3779B9u;
+    float amp = 2.0f * jitter * spacing;
+    auto draw = [&]() {
+      state ^= state << 13;
+      state ^= state >> 17;
+      state ^= state << 5;
+      return ((float)state / 4294967295.0f - 0.5f) * amp;
+    };
+    for (uint64_t y = 0; y < ny; ++y)
+      for (uint64_t z = 0; z < nz; ++z)
+        for (uint64_t x = 0; x < nx; ++x) {
+          Vec3<float> step = {(float)x * spacing, (float)y * spacing, (float)z * spacing};
+          Vec3<float> at = {start.x + step.x, start.y + step.y, start.z + step.z};
+          if (jitter != 0.0f) {
+            float jx = draw(), jy = draw(), jz = draw();
+            at = {at.x + jx, at.y + jy, at.z + jz};
+          }
+          int32_t st = add_particle(cvt(at), V{R(0), R(0), R(0)}, nullptr);
+          if (st != kOk) return st;
+        }
+    if (count_out) *count_out = (size_t)total;
+    return kOk;
+  }
+
+  static double sat_u64(float c) { return c < 0 ? 0.0 : (double)c; }
+
+  // cell_of (src/fluid/mod.rs:526-533): per-component division, floor,
+  // saturating cast.
+  CellKey cell_of(V p) const {
+    R r = radius();
+    R x = p.x / r, y = p.y / r, z = p.z / r;
+    return {sat_i32((double)std::floor(x)), sat_i32((double)std::floor(y)), sat_i32((double)std::floor(z))};
+  }
+
+  // `near` (src/fluid/mod.rs:264-277): candidates kept when
+  // |x_j - x_i|^2 < reach, appended in range order.
+  static void near(const std::vector<V>& pos, const std::vector<Range>& ranges, V at, R reach,
+                   std::vector<uint32_t>& out) {
+    for (const Range& r : ranges)
+      for (uint32_t other = r.begin; other < r.end; ++other)
+        if (length_squared(pos[other] - at) < reach) out.push_back(other);
+  }
+
+  // update_neighbors (src/fluid/mod.rs:540-608)
+  int32_t update_neighbors() {
+    size_t n = len();
+    struct Key {
+      CellKey cell;
+      uint32_t id, index;
+    };
+    std::vector<Key> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = {cell_of(positions[i]), ids[i], (uint32_t)i};
+    std::sort(order.begin(), order.end(), [](const Key& a, const Key& b) {
+      return std::tie(a.cell.x, a.cell.y, a.cell.z, a.id, a.index) <
+             std::tie(b.cell.x, b.cell.y, b.cell.z, b.id, b.index);
+    });
+    std::vector<uint32_t> nid(n);
+    std::vector<V> np(n), nv(n);
+    std::vector<R> nd(n);
+    for (size_t k = 0; k < n; ++k) {
+      uint32_t j = order[k].index;
+      nid[k] = ids[j];
+      np[k] = positions[j];
+      nv[k] = velocities[j];
+      nd[k] = densities[j];
+    }
+    ids.swap(nid);
+    positions.swap(np);
+    velocities.swap(nv);
+    densities.swap(nd);
+    std::vector<CellKey> keys(n);
+    for (size_t k = 0; k < n; ++k) keys[k] = order[k].cell;
+    grid = Grid::build(keys.begin(), keys.end());
+
+    R r = radius();
+    R reach = r * r;  // powi(2)
+    std::vector<uint32_t> fcount(n), wcount(n);
+    std::vector<std::vector<uint32_t>> fparts((n + Dispatch::kChunk - 1) / Dispatch::kChunk),
+        wparts(fparts.size());
+    bool ok = dispatch.run(n, [&](size_t b, size_t e) {
+      std::vector<Range> af, aw;
+      std::optional<CellKey> last;
+      auto& fl = fparts[b / Dispatch::kChunk];
+      auto& wl = wparts[b / Dispatch::kChunk];
+      for (size_t i = b; i < e; ++i) {
+        V at = positions[i];
+        CellKey cell = cell_of(at);
+        if (!last || *last != cell) {
+          last = cell;
+          grid.around(cell, af);
+          walls.grid.around(cell, aw);
+        }
+        size_t f0 = fl.size(), w0 = wl.size();
+        near(positions, af, at, reach, fl);
+        near(walls.positions, aw, at, reach, wl);
+        fcount[i] = (uint32_t)(fl.size() - f0);
+        wcount[i] = (uint32_t)(wl.size() - w0);
+      }
+    });
+    if (!ok) return kDispatchIncomplete;
+    auto join = [&](Lists& lists, const std::vector<uint32_t>& counts, std::vector<std::vector<uint32_t>>& parts) {
+      lists.start.assign(1, 0);
+      lists.items.clear();
+      for (size_t i = 0; i < n; ++i) lists.start.push_back(lists.start.back() + counts[i]);
+      for (auto& p : parts) lists.items.insert(lists.items.end(), p.begin(), p.end());
+    };
+    join(neighbors, fcount, fparts);
+    join(wall_neighbors, wcount, wparts);
+    neighbors_built = true;
+    return kOk;
+  }
+
+  // compute_densities (src/fluid/mod.rs:621-675); movers and ground sums are
+  // empty here, and adding an empty f32 sum to a positive one is exact.
+  int32_t compute_densities() {
+    if (!neighbors_built || neighbors.start.size() != len() + 1) return kOutOfRange;
+    std::vector<R> out(len());
+    bool ok = dispatch.run(len(), [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) {
+        V at = positions[i];
+        R sum = R(0);
+        for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k)
+          sum += kernel.value(length(positions[neighbors.items[k]] - at));
+        R wsum = R(0);
+        for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
+          wsum += kernel.value(length(walls.positions[wall_neighbors.items[k]] - at));
+        out[i] = mass * (sum + wsum);
+      }
+    });
+    if (!ok) return kDispatchIncomplete;
+    densities.swap(out);
+    return kOk;
+  }
+
+  // state_hash (src/fluid/mod.rs:678-691) over the f32 values.
+  uint64_t state_hash() const {
+    std::vector<size_t> order(len());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return ids[a] < ids[b]; });
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto feed = [&](uint32_t w) {
+      for (int b = 0; b < 4; ++b) {
+        h ^= (uint64_t)((w >> (8 * b)) & 0xffu);
+        h *= 0x00000100000001b3ull;
+      }
+    };
+    for (size_t i : order) {
+      feed(ids[i]);
+      float vals[6] = {(float)positions[i].x, (float)positions[i].y, (float)positions[i].z,
+                       (float)velocities[i].x, (float)velocities[i].y, (float)velocities[i].z};
+      for (float f : vals) {
+        uint32_t bits;
+        std::memcpy(&bits, &f, 4);
+        feed(bits);
+      }
+    }
+    return h;
+  }
+
+  // fastest (src/fluid/step.rs:92-96)
+  R fastest() const {
+    R most = R(0);
+    for (const V& v : velocities) most = rmax(most, length(v));
+    return most;
+  }
+
+  // step (src/fluid/step.rs:58-90). Flows and the domain are not carried.
+  int32_t step(float dt, Report& report) {
+    report = Report{};
+    if (!std::isfinite(dt) || dt < 0.0f) {
+      detail = "fluid step";
+      return kNonFinite;
+    }
+    if (dt == 0.0f) return kOk;
+    if (len() > 0) {
+      int32_t st = settings.solver == kWcsph ? wcsph(dt, report) : dfsph(dt, report);
+      if (st != kOk) return st;
+    }
+    report.max_speed = (float)fastest();
+    R rest = R(settings.rest_density);
+    R most = R(0);
+    for (R d : densities) most = rmax(most, d / rest - R(1));
+    report.max_compression = (float)most;
+    return kOk;
+  }
+
+  // wcsph / wcsph_steps (src/fluid/step.rs:102-142); no coupled bodies, so
+  // the sound speed is the configured one.
+  int32_t wcsph(float dt_in, Report& report) {
+    report.sound_speed = settings.sound_speed;
+    R dt = R(dt_in);
+    R fast = fastest();
+    R largest = R(settings.courant) * radius() / (R(settings.sound_speed) + fast);
+    R wanted = rmax(std::ceil(dt / largest), R(1));
+    R cap = R(settings.max_substeps);
+    uint32_t substeps = (uint32_t)rmin(wanted, cap);
+    report.capped = wanted > cap;
+    R small = dt / R(substeps);
+    for (uint32_t s = 0; s < substeps; ++s) {
+      uint32_t held = 0;
+      int32_t st = wcsph_substep(small, held);
+      if (st != kOk) return st;
+      report.held_back += held;
+    }
+    report.substeps = substeps;
+    return kOk;
+  }
+
+  // viscosity (src/fluid/step.rs:254-259)
+  R viscosity(R dt) const {
+    R r = radius();
+    return rmin(R(settings.viscosity) * R(settings.sound_speed) * r, R(0.05f) * r * r / dt);
+  }
+
+  // wall_viscous (src/fluid/step.rs:262-301), no terrains.
+  V wall_viscous(size_t index, R dt) const {
+    R r = radius();
+    R soften = R(0.01f) * r * r;
+    R stable = r * r / dt;
+    V at = positions[index], velocity = velocities[index];
+    R density = densities[index];
+    V push{R(0), R(0), R(0)};
+    V away{R(0), R(0), R(0)};
+    for (uint32_t k = wall_neighbors.start[index]; k < wall_neighbors.start[index + 1]; ++k)
+      away = away + (-kernel.gradient(at - walls.positions[wall_neighbors.items[k]]));
+    V normal = normalize_or(away, V{R(0), R(0), R(0)});
+    V into = normal * rmin(dot(velocity, normal), R(0));
+    R wall_viscosity = rmin(R(1) * R(settings.sound_speed) * r, R(0.25f) * stable);
+    for (uint32_t k = wall_neighbors.start[index]; k < wall_neighbors.start[index + 1]; ++k) {
+      V offset = at - walls.positions[wall_neighbors.items[k]];
+      R toward = dot(into, offset);
+      if (toward < R(0)) {
+        R squeeze = toward / (length_squared(offset) + soften);
+        R damp = -wall_viscosity * squeeze / density;
+        push = push - kernel.gradient(offset) * (mass * damp);
+      }
+    }
+    return push;
+  }
+
+  // viscous (src/fluid/step.rs:229-251)
+  V viscous(size_t index, R dt) const {
+    R r = radius();
+    R soften = R(0.01f) * r * r;
+    R nu = viscosity(dt);
+    V at = positions[index], velocity = velocities[index];
+    R density = densities[index];
+    V push{R(0), R(0), R(0)};
+    for (uint32_t k = neighbors.start[index]; k < neighbors.start[index + 1]; ++k) {
+      uint32_t other = neighbors.items[k];
+      if (other == index) continue;
+      V offset = at - positions[other];
+      R toward = dot(velocity - velocities[other], offset);
+      if (toward < R(0)) {
+        R squeeze = toward / (length_squared(offset) + soften);
+        R damp = -nu * squeeze * R(2) / (density + densities[other]);
+        push = push - kernel.gradient(offset) * (mass * damp);
+      }
+    }
+    return push + wall_viscous(index, dt);
+  }
+
+  // wcsph_substep (src/fluid/step.rs:144-224)
+  int32_t wcsph_substep(R dt, uint32_t& held) {
+    int32_t st = update_neighbors();
+    if (st != kOk) return st;
+    st = compute_densities();
+    if (st != kOk) return st;
+    size_t n = len();
+    R rest = R(settings.rest_density);
+    R c = R(settings.sound_speed);
+    R stiffness = rest * (c * c) / R(7);  // TAIT, step.rs:9
+    std::vector<R> pressures(n), scaled(n);
+    for (size_t i = 0; i < n; ++i) {
+      R ratio = rmin(densities[i] / rest, R(1.1f));  // PRESSURE_SQUEEZE_CAP
+      R r2 = ratio * ratio;
+      R r4 = r2 * r2;
+      R r7 = ratio * r2 * r4;  // powi(7)
+      pressures[i] = rmax(stiffness * (r7 - R(1)), R(0));
+    }
+    R r = radius();
+    R soften = R(0.01f) * r * r;
+    R nu = viscosity(dt);
+    for (size_t i = 0; i < n; ++i) scaled[i] = pressures[i] / (densities[i] * densities[i]);
+    V gravity = cvt(settings.gravity);
+    std::vector<V> pushes(n);
+    bool ok = dispatch.run(n, [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) {
+        V at = positions[i], velocity = velocities[i];
+        R density = densities[i];
+        R own = scaled[i];
+        V push{R(0), R(0), R(0)};
+        for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k) {
+          uint32_t other = neighbors.items[k];
+          V offset = at - positions[other];
+          R squared = length_squared(offset);
+          if (squared <= R(0)) continue;
+          R distance = std::sqrt(squared);
+          V gradient = offset * (kernel.slope(distance) / distance);
+          R toward = rmin(dot(velocity - velocities[other], offset), R(0));
+          R damp = -nu * toward / (squared + soften) * R(2) / (density + densities[other]);
+          push = push - gradient * (mass * (own + scaled[other] + damp));
+        }
+        R mirrored = own + pressures[i] / (rest * rest);
+        for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k) {
+          V offset = at - walls.positions[wall_neighbors.items[k]];
+          push = push - kernel.gradient(offset) * (mass * mirrored);
+        }
+        pushes[i] = push + wall_viscous(i, dt) + gravity;
+      }
+    });
+    if (!ok) return kDispatchIncomplete;
+    for (size_t i = 0; i < n; ++i) velocities[i] = velocities[i] + pushes[i] * dt;
+    forces = pushes;
+    return advance(dt, held);
+  }
+
+  // advance (src/fluid/step.rs:305-339), no terrains.
+  int32_t advance(R dt, uint32_t& held) {
+    held = 0;
+    R half = R(settings.spacing / 2.0f);
+    V rest{half, half, half};
+    for (size_t i = 0; i < len(); ++i) {
+      V& at = positions[i];
+      V& velocity = velocities[i];
+      V next = at + velocity * dt;
+      if (walls.bounds) {
+        V low = walls.bounds->first, high = walls.bounds->second;
+        V inside = clamp(next, low + rest, high - rest);
+        if (inside != next) {
+          if (clamp(next, low, high) != next) held += 1;
+          if (inside.x != next.x) velocity.x = R(0);
+          if (inside.y != next.y) velocity.y = R(0);
+          if (inside.z != next.z) velocity.z = R(0);
+          next = inside;
+        }
+      }
+      if (!next.finite() || !velocity.finite()) {
+        detail = "fluid particle";
+        return kNonFinite;
+      }
+      at = next;
+    }
+    return kOk;
+  }
+
+  // dfsph (src/fluid/dfsph.rs:46-77)
+  int32_t dfsph(float dt_in, Report& report) {
+    R spacing = R(settings.spacing);
+    R courant = R(settings.courant);
+    R pull = length(cvt(settings.gravity));
+    R left = R(dt_in);
+    while (left > R(0)) {
+      R small = courant * spacing / rmax(fastest(), R(1e-6f));
+      if (pull > R(0)) small = rmin(small, std::sqrt(R(2) * courant * spacing / pull));
+      if (report.substeps + 1 >= settings.max_substeps && small < left) {
+        report.capped = 1;
+        small = left;
+      }
+      small = rmin(small, left);
+      uint32_t idiv = 0, iden = 0, held = 0;
+      int32_t st = dfsph_substep(small, idiv, iden, held);
+      report.held_back += held;
+      report.iterations += idiv + iden;
+      report.iterations_divergence += idiv;
+      report.iterations_density += iden;
+      if (st != kOk) return st;
+      report.substeps += 1;
+      left -= small;
+    }
+    return kOk;
+  }
+
+  // gradients (src/fluid/dfsph.rs:108-136): every listed pair's kernel
+  // gradient, fluid then walls, in list order.
+  std::vector<V> gfluid, gwalls;
+  void gradients() {
+    gfluid.resize(neighbors.items.size());
+    gwalls.resize(wall_neighbors.items.size());
+    dispatch.run(len(), [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) {
+        V at = positions[i];
+        for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k)
+          gfluid[k] = kernel.gradient(at - positions[neighbors.items[k]]);
+        for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
+          gwalls[k] = kernel.gradient(at - walls.positions[wall_neighbors.items[k]]);
+      }
+    });
+  }
+
+  // factors (src/fluid/dfsph.rs:138-163)
+  std::vector<R> factors() {
+    std::vector<R> out(len());
+    dispatch.run(len(), [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) {
+        V sum{R(0), R(0), R(0)};
+        R squares = R(0);
+        for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k) {
+          V part = gfluid[k] * mass;
+          sum = sum + part;
+          squares += length_squared(part);
+        }
+        for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
+          sum = sum + gwalls[k] * mass;
+        R total = length_squared(sum) + squares;
+        out[i] = total > R(0) ? R(1) / total : R(0);
+      }
+    });
+    return out;
+  }
+
+  // density_rate (src/fluid/dfsph.rs:166-181)
+  R density_rate(size_t i) const {
+    V velocity = velocities[i];
+    R rate = R(0);
+    for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k)
+      rate += dot(velocity - velocities[neighbors.items[k]], gfluid[k]);
+    for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
+      rate += dot(velocity, gwalls[k]);
+    return rate * mass;
+  }
+
+  // push_apart (src/fluid/dfsph.rs:184-216)
+  void push_apart(const std::vector<R>& stiffness, R dt) {
+    std::vector<V> changes(len());
+    dispatch.run(len(), [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) {
+        R own = stiffness[i];
+        V change{R(0), R(0), R(0)};
+        for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k)
+          change = change - gfluid[k] * (mass * (own + stiffness[neighbors.items[k]]));
+        for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
+          change = change - gwalls[k] * (mass * own);
+        changes[i] = change * dt;
+      }
+    });
+    for (size_t i = 0; i < len(); ++i) velocities[i] = velocities[i] + changes[i];
+  }
+
+  // solve (src/fluid/dfsph.rs:222-275); `which` 0 = divergence, 1 = density.
+  uint32_t solve(int which, R dt, const std::vector<R>& alpha) {
+    R rest = R(settings.rest_density);
+    float tolerance = which == 0 ? settings.divergence_tolerance : settings.density_tolerance;
+    uint32_t iterations = 0;
+    size_t n = len();
+    std::vector<R> stiffness(n);
+    std::vector<R> errors(n);
+    for (;;) {
+      dispatch.run(n, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; ++i) {
+          R rate = density_rate(i);
+          R target, error;
+          if (which == 1) {
+            R over = rmax(densities[i] + dt * rate - rest, R(0));
+            target = over / (dt * dt);
+            error = over / rest;
+          } else {
+            size_t crowd = neighbors.count(i) + wall_neighbors.count(i);
+            R r = crowd < 20 ? R(0) : rmax(rate, R(0));  // SURFACE_NEIGHBORS
+            target = r / dt;
+            error = dt * r / rest;
+          }
+          stiffness[i] = target * alpha[i];
+          errors[i] = error;
+        }
+      });
+      uint32_t loaded = 0;
+      for (R e : errors) loaded += e > R(0);
+      uint32_t denom = std::max<uint32_t>(loaded, 1);
+      R sum32 = R(0);
+      bool need32 = residual_mode == kReferenceF32 || log_solves;
+      if (need32)
+        for (R e : errors) sum32 += e;
+      R avg32 = sum32 / R(denom);
+      double avg64 = 0.0;
+      bool need64 = residual_mode == kFixedTreeF64 || log_solves;
+      if (need64) avg64 = fixed_tree_sum(errors) / (double)denom;
+      if (log_solves) solve_log.push_back({which, iterations, loaded, (float)avg32, avg64});
+      bool below = residual_mode == kReferenceF32 ? (avg32 <= R(tolerance)) : (avg64 <= (double)tolerance);
+      bool settled = iterations >= settings.min_iterations && below;
+      if (settled || iterations >= settings.max_iterations) return iterations;
+      push_apart(stiffness, dt);
+      iterations += 1;
+    }
+  }
+
+  // dfsph_substep (src/fluid/dfsph.rs:79-103)
+  int32_t dfsph_substep(R dt, uint32_t& idiv, uint32_t& iden, uint32_t& held) {
+    int32_t st = update_neighbors();
+    if (st != kOk) return st;
+    st = compute_densities();
+    if (st != kOk) return st;
+    gradients();
+    std::vector<R> alpha = factors();
+    idiv = solve(0, dt, alpha);
+    V gravity = cvt(settings.gravity);
+    std::vector<V> pushes(len());
+    dispatch.run(len(), [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) pushes[i] = viscous(i, dt) + gravity;
+    });
+    for (size_t i = 0; i < len(); ++i) velocities[i] = velocities[i] + pushes[i] * dt;
+    forces = pushes;
+    iden = solve(1, dt, alpha);
+    return advance(dt, held);
+  }
+
+  // ---- walls (src/fluid/boundary.rs) ----
+
+  static int32_t check_box(Vec3<float> low, Vec3<float> high, std::string& detail) {
+    if (!low.finite() || !high.finite()) {
+      detail = "fluid box";
+      return kNonFinite;
+    }
+    if (high.x <= low.x || high.y <= low.y || high.z <= low.z) return kOutOfRange;
+    return kOk;
+  }
+
+  // cells_in (boundary.rs:122-126)
+  void cells_in(Vec3<float> extent, int64_t out[3]) const {
+    auto count = [&](float length) -> int64_t {
+      float c = std::round(length / settings.spacing);
+      c = std::min(c, 1e9f);
+      return (int64_t)c;
+    };
+    out[0] = count(extent.x);
+    out[1] = count(extent.y);
+    out[2] = count(extent.z);
+  }
+
+  // check_room (boundary.rs:128-137)
+  int32_t check_room(int64_t count) const {
+    uint64_t have = walls.positions.size();
+    uint64_t room = settings.max_particles > have ? settings.max_particles - have : 0;
+    if (count < 0 || (uint64_t)count > room || walls.next == UINT32_MAX) return kFluidFull;
+    return kOk;
+  }
+
+  // add_box_container (boundary.rs:41-68)
+  int32_t add_box_container(Vec3<float> low, Vec3<float> high, uint32_t* out) {
+    int32_t st = check_box(low, high, detail);
+    if (st != kOk) return st;
+    int64_t c[3];
+    cells_in({high.x - low.x, high.y - low.y, high.z - low.z}, c);
+    int64_t layers = (int64_t)std::ceil(settings.support);
+    int64_t shell = (c[0] + 2 * layers) * (c[1] + 2 * layers) * (c[2] + 2 * layers) - c[0] * c[1] * c[2];
+    st = check_room(shell);
+    if (st != kOk) return st;
+    float spacing = settings.spacing;
+    std::vector<Vec3<float>> points;
+    points.reserve((size_t)shell);
+    for (int64_t y = -layers; y < c[1] + layers; ++y)
+      for (int64_t z = -layers; z < c[2] + layers; ++z)
+        for (int64_t x = -layers; x < c[0] + layers; ++x) {
+          bool inside = x >= 0 && x < c[0] && y >= 0 && y < c[1] && z >= 0 && z < c[2];
+          if (!inside) {
+            Vec3<float> at = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+            points.push_back({low.x + at.x * spacing, low.y + at.y * spacing, low.z + at.z * spacing});
+          }
+        }
+    uint32_t id = add_wall_points(points);
+    walls.containers.push_back({id, cvt(low), cvt(high)});
+    walls.bounds = union_of(walls.containers);
+    if (out) *out = id;
+    return kOk;
+  }
+
+  // add_wall (boundary.rs:74-91)
+  int32_t add_wall(Vec3<float> low, Vec3<float> high, uint32_t* out) {
+    int32_t st = check_box(low, high, detail);
+    if (st != kOk) return st;
+    int64_t c[3];
+    cells_in({high.x - low.x, high.y - low.y, high.z - low.z}, c);
+    for (auto& v : c) v = std::max<int64_t>(v, 1);
+    int64_t count = c[0] * c[1] * c[2];
+    st = check_room(count);
+    if (st != kOk) return st;
+    float spacing = settings.spacing;
+    std::vector<Vec3<float>> points;
+    points.reserve((size_t)count);
+    for (int64_t y = 0; y < c[1]; ++y)
+      for (int64_t z = 0; z < c[2]; ++z)
+        for (int64_t x = 0; x < c[0]; ++x) {
+          Vec3<float> at = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+          points.push_back({low.x + at.x * spacing, low.y + at.y * spacing, low.z + at.z * spacing});
+        }
+    uint32_t id = add_wall_points(points);
+    if (out) *out = id;
+    return kOk;
+  }
+
+  // add_wall_points for caller points (the device path's boundary points).
+  int32_t add_boundary_points(const std::vector<Vec3<float>>& points, uint32_t* out) {
+    for (auto& p : points)
+      if (!p.finite()) {
+        detail = "fluid wall";
+        return kNonFinite;
+      }
+    int32_t st = check_room((int64_t)points.size());
+    if (st != kOk) return st;
+    uint32_t id = add_wall_points(points);
+    if (out) *out = id;
+    return kOk;
+  }
+
+  // add_wall_points (boundary.rs:139-151)
+  uint32_t add_wall_points(const std::vector<Vec3<float>>& points) {
+    uint32_t id = walls.next++;
+    for (auto& p : points) {
+      walls.owners.push_back(id);
+      walls.positions.push_back(cvt(p));
+    }
+    settle_walls();
+    return id;
+  }
+
+  // remove_boundary (boundary.rs:94-115), no terrains.
+  bool remove_boundary(uint32_t id) {
+    size_t before = walls.positions.size();
+    std::vector<V> kp;
+    std::vector<uint32_t> ko;
+    for (size_t i = 0; i < walls.positions.size(); ++i)
+      if (walls.owners[i] != id) {
+        kp.push_back(walls.positions[i]);
+        ko.push_back(walls.owners[i]);
+      }
+    walls.positions.swap(kp);
+    walls.owners.swap(ko);
+    bool had = false;
+    std::vector<std::tuple<uint32_t, V, V>> kc;
+    for (auto& c : walls.containers) {
+      if (std::get<0>(c) == id)
+        had = true;
+      else
+        kc.push_back(c);
+    }
+    walls.containers.swap(kc);
+    walls.bounds = union_of(walls.containers);
+    bool gone = walls.positions.size() != before || had;
+    if (gone) settle_walls();
+    return gone;
+  }
+
+  // settle_walls (boundary.rs:155-178): sort by (cell, owner, x/y/z bits,
+  // index).
+  void settle_walls() {
+    size_t n = walls.positions.size();
+    struct Key {
+      CellKey cell;
+      uint32_t owner, bx, by, bz;
+      size_t index;
+    };
+    auto bits = [](R v) {
+      float f = (float)v;
+      uint32_t b;
+      std::memcpy(&b, &f, 4);
+      return b;
+    };
+    std::vector<Key> order(n);
+    for (size_t i = 0; i < n; ++i) {
+      V p = walls.positions[i];
+      order[i] = {cell_of(p), walls.owners[i], bits(p.x), bits(p.y), bits(p.z), i};
+    }
+    std::sort(order.begin(), order.end(), [](const Key& a, const Key& b) {
+      return std::tie(a.cell.x, a.cell.y, a.cell.z, a.owner, a.bx, a.by, a.bz, a.index) <
+             std::tie(b.cell.x, b.cell.y, b.cell.z, b.owner, b.bx, b.by, b.bz, b.index);
+    });
+    std::vector<V> np(n);
+    std::vector<uint32_t> no(n);
+    std::vector<CellKey> keys(n);
+    for (size_t k = 0; k < n; ++k) {
+      np[k] = walls.positions[order[k].index];
+      no[k] = order[k].owner;
+      keys[k] = order[k].cell;
+    }
+    walls.positions.swap(np);
+    walls.owners.swap(no);
+    walls.grid = Grid::build(keys.begin(), keys.end());
+  }
+
+  // union (boundary.rs:240-247)
+  static std::optional<std::pair<V, V>> union_of(const std::vector<std::tuple<uint32_t, V, V>>& cs) {
+    std::optional<std::pair<V, V>> sum;
+    for (auto& c : cs) {
+      V lo = std::get<1>(c), hi = std::get<2>(c);
+      if (!sum)
+        sum = std::make_pair(lo, hi);
+      else
+        sum = std::make_pair(vmin(sum->first, lo), vmax(sum->second, hi));
+    }
+    return sum;
+  }
+};
+
+}  // namespace oracle

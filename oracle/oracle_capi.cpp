@@ -1,0 +1,377 @@
+// TEST INFRASTRUCTURE, NOT PRODUCT CODE: C entry points over the CPU oracle
+// (fluid_oracle.hpp) for the Python test harness, the smoke check and the
+// CPU baseline leg of bench.py. The calls take the same shapes as the
+// product's include/solunar_fluid.h with an `orc_` prefix, plus controls the
+// reference's dispatcher gives its tests (thread count, a skipped job), the
+// precision (32 or 64) and the DFSPH residual mode.
+#include <cstdio>
+#include <memory>
+
+#include "fluid_oracle.hpp"
+
+using namespace oracle;
+
+extern "C" {
+
+// Same layout as sfl_settings in include/solunar_fluid.h.
+struct orc_settings {
+  int32_t solver;
+  float spacing, rest_density, support;
+  uint64_t max_particles;
+  float sound_speed, viscosity;
+  float gx, gy, gz;
+  float courant;
+  uint32_t max_substeps;
+  float divergence_tolerance, density_tolerance;
+  uint32_t min_iterations, max_iterations;
+  int32_t device;
+};
+
+struct orc_report {
+  uint32_t substeps, capped;
+  float max_compression, max_speed;
+  uint32_t held_back, iterations, emitted, drained, lost;
+  float sound_speed;
+  uint32_t iterations_divergence, iterations_density;
+};
+}
+
+namespace {
+
+struct Handle {
+  int precision;
+  std::unique_ptr<World<float>> f;
+  std::unique_ptr<World<double>> d;
+};
+
+Settings convert(const orc_settings* s) {
+  Settings o;
+  o.solver = s->solver;
+  o.spacing = s->spacing;
+  o.rest_density = s->rest_density;
+  o.support = s->support;
+  o.max_particles = s->max_particles;
+  o.sound_speed = s->sound_speed;
+  o.viscosity = s->viscosity;
+  o.gravity = {s->gx, s->gy, s->gz};
+  o.courant = s->courant;
+  o.max_substeps = s->max_substeps;
+  o.divergence_tolerance = s->divergence_tolerance;
+  o.density_tolerance = s->density_tolerance;
+  o.min_iterations = s->min_iterations;
+  o.max_iterations = s->max_iterations;
+  return o;
+}
+
+template <class F>
+auto with(Handle* h, F&& fn) {
+  if (h->precision == 64) return fn(*h->d);
+  return fn(*h->f);
+}
+
+void fill_report(const Report& r, orc_report* out) {
+  if (!out) return;
+  out->substeps = r.substeps;
+  out->capped = r.capped;
+  out->max_compression = r.max_compression;
+  out->max_speed = r.max_speed;
+  out->held_back = r.held_back;
+  out->iterations = r.iterations;
+  out->emitted = r.emitted;
+  out->drained = r.drained;
+  out->lost = r.lost;
+  out->sound_speed = r.sound_speed;
+  out->iterations_divergence = r.iterations_divergence;
+  out->iterations_density = r.iterations_density;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_settings_default(orc_settings* out) {
+  Settings d;
+  out->solver = d.solver;
+  out->spacing = d.spacing;
+  out->rest_density = d.rest_density;
+  out->support = d.support;
+  out->max_particles = d.max_particles;
+  out->sound_speed = d.sound_speed;
+  out->viscosity = d.viscosity;
+  out->gx = d.gravity.x;
+  out->gy = d.gravity.y;
+  out->gz = d.gravity.z;
+  out->courant = d.courant;
+  out->max_substeps = d.max_substeps;
+  out->divergence_tolerance = d.divergence_tolerance;
+  out->density_tolerance = d.density_tolerance;
+  out->min_iterations = d.min_iterations;
+  out->max_iterations = d.max_iterations;
+  out->device = 0;
+}
+
+size_t orc_settings_size() { return sizeof(orc_settings); }
+size_t orc_report_size() { return sizeof(orc_report); }
+
+int32_t orc_create(const orc_settings* s, int32_t precision, void** out) {
+  if (!s || !out) return kOutOfRange;
+  Settings st = convert(s);
+  if (!st.valid()) return kInvalidSettings;
+  auto* h = new Handle();
+  h->precision = precision == 64 ? 64 : 32;
+  if (h->precision == 64)
+    h->d.reset(new World<double>(st));
+  else
+    h->f.reset(new World<float>(st));
+  *out = h;
+  return kOk;
+}
+
+void orc_destroy(void* w) { delete static_cast<Handle*>(w); }
+
+void orc_set_threads(void* w, int32_t threads) {
+  with(static_cast<Handle*>(w), [&](auto& x) {
+    x.dispatch.threads = threads < 1 ? 1 : threads;
+    return 0;
+  });
+}
+
+// Drops job `job` (-1: none) from every later dispatch, like
+// dispatch::testing::SkipsJobs (src/dispatch.rs:50-65).
+void orc_set_skip_job(void* w, int64_t job) {
+  with(static_cast<Handle*>(w), [&](auto& x) {
+    if (job < 0)
+      x.dispatch.skip = nullptr;
+    else
+      x.dispatch.skip = [job](size_t j) { return (int64_t)j == job; };
+    return 0;
+  });
+}
+
+void orc_set_residual_mode(void* w, int32_t mode) {
+  with(static_cast<Handle*>(w), [&](auto& x) {
+    x.residual_mode = mode == 1 ? kFixedTreeF64 : kReferenceF32;
+    return 0;
+  });
+}
+
+void orc_set_solve_log(void* w, int32_t enable) {
+  with(static_cast<Handle*>(w), [&](auto& x) {
+    x.log_solves = enable != 0;
+    x.solve_log.clear();
+    return 0;
+  });
+}
+
+// Rows of (solve, iteration, loaded, average_f32, average_f64).
+size_t orc_read_solve_log(void* w, size_t cap, int32_t* solve, uint32_t* iteration, uint32_t* loaded,
+                          float* avg32, double* avg64) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    size_t n = std::min(cap, x.solve_log.size());
+    for (size_t i = 0; i < n; ++i) {
+      solve[i] = x.solve_log[i].solve;
+      iteration[i] = x.solve_log[i].iteration;
+      loaded[i] = x.solve_log[i].loaded;
+      avg32[i] = x.solve_log[i].average_f32;
+      avg64[i] = x.solve_log[i].average_f64;
+    }
+    return x.solve_log.size();
+  });
+}
+
+size_t orc_len(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.len(); });
+}
+
+double orc_particle_mass(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return (double)x.mass; });
+}
+
+double orc_kernel_value(void* w, double distance) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = std::decay_t<decltype(x.mass)>;
+    return (double)x.kernel.value((R)distance);
+  });
+}
+
+double orc_kernel_slope(void* w, double distance) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = std::decay_t<decltype(x.mass)>;
+    return (double)x.kernel.slope((R)distance);
+  });
+}
+
+int32_t orc_add_particles(void* w, size_t n, const float* xyz, const float* vxyz, uint32_t* ids_out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = std::decay_t<decltype(x.mass)>;
+    for (size_t i = 0; i < n; ++i) {
+      Vec3<R> p{R(xyz[3 * i]), R(xyz[3 * i + 1]), R(xyz[3 * i + 2])};
+      Vec3<R> v{R(0), R(0), R(0)};
+      if (vxyz) v = {R(vxyz[3 * i]), R(vxyz[3 * i + 1]), R(vxyz[3 * i + 2])};
+      int32_t st = x.add_particle(p, v, ids_out ? ids_out + i : nullptr);
+      if (st != kOk) return st;
+    }
+    return (int32_t)kOk;
+  });
+}
+
+int32_t orc_fill_box_jittered(void* w, float lx, float ly, float lz, float hx, float hy, float hz, float jitter,
+                              uint32_t seed, size_t* count_out) {
+  return with(static_cast<Handle*>(w),
+              [&](auto& x) { return x.fill_box({lx, ly, lz}, {hx, hy, hz}, jitter, seed, count_out); });
+}
+
+int32_t orc_fill_box(void* w, float lx, float ly, float lz, float hx, float hy, float hz, size_t* count_out) {
+  return orc_fill_box_jittered(w, lx, ly, lz, hx, hy, hz, 0.0f, 0, count_out);
+}
+
+int32_t orc_add_box_container(void* w, float lx, float ly, float lz, float hx, float hy, float hz, uint32_t* out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.add_box_container({lx, ly, lz}, {hx, hy, hz}, out); });
+}
+
+int32_t orc_add_wall(void* w, float lx, float ly, float lz, float hx, float hy, float hz, uint32_t* out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.add_wall({lx, ly, lz}, {hx, hy, hz}, out); });
+}
+
+int32_t orc_add_boundary_points(void* w, size_t n, const float* xyz, uint32_t* out) {
+  std::vector<Vec3<float>> pts(n);
+  for (size_t i = 0; i < n; ++i) pts[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.add_boundary_points(pts, out); });
+}
+
+int32_t orc_remove_boundary(void* w, uint32_t id, int32_t* removed) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    bool r = x.remove_boundary(id);
+    if (removed) *removed = r ? 1 : 0;
+    return (int32_t)kOk;
+  });
+}
+
+size_t orc_wall_len(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.walls.positions.size(); });
+}
+
+int32_t orc_read_wall_positions(void* w, size_t cap, float* xyz) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    size_t n = std::min(cap, x.walls.positions.size());
+    for (size_t i = 0; i < n; ++i) {
+      xyz[3 * i] = (float)x.walls.positions[i].x;
+      xyz[3 * i + 1] = (float)x.walls.positions[i].y;
+      xyz[3 * i + 2] = (float)x.walls.positions[i].z;
+    }
+    return (int32_t)kOk;
+  });
+}
+
+int32_t orc_step(void* w, float dt, orc_report* out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    Report r;
+    int32_t st = x.step(dt, r);
+    if (st == kOk) fill_report(r, out);
+    return st;
+  });
+}
+
+int32_t orc_update_neighbors(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.update_neighbors(); });
+}
+
+int32_t orc_compute_densities(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.compute_densities(); });
+}
+
+int32_t orc_read_ids(void* w, size_t cap, uint32_t* out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    size_t n = std::min(cap, x.len());
+    for (size_t i = 0; i < n; ++i) out[i] = x.ids[i];
+    return (int32_t)kOk;
+  });
+}
+
+// Vector fields as float (precision 32) or double (precision 64) into a
+// caller buffer of the matching type: field 0 positions, 1 velocities,
+// 2 forces.
+int32_t orc_read_vectors(void* w, int32_t field, size_t cap, void* out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = std::decay_t<decltype(x.mass)>;
+    const auto& src = field == 0 ? x.positions : field == 1 ? x.velocities : x.forces;
+    R* o = static_cast<R*>(out);
+    size_t n = std::min(cap, src.size());
+    for (size_t i = 0; i < n; ++i) {
+      o[3 * i] = src[i].x;
+      o[3 * i + 1] = src[i].y;
+      o[3 * i + 2] = src[i].z;
+    }
+    return (int32_t)kOk;
+  });
+}
+
+int32_t orc_read_densities(void* w, size_t cap, void* out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = std::decay_t<decltype(x.mass)>;
+    R* o = static_cast<R*>(out);
+    size_t n = std::min(cap, x.densities.size());
+    for (size_t i = 0; i < n; ++i) o[i] = x.densities[i];
+    return (int32_t)kOk;
+  });
+}
+
+// Replaces the particles (ids, positions, velocities) wholesale; densities
+// reset to zero and lists are dropped. Used to feed both sides one state.
+int32_t orc_set_particles(void* w, size_t n, const uint32_t* ids, const float* xyz, const float* vxyz) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = std::decay_t<decltype(x.mass)>;
+    x.ids.assign(ids, ids + n);
+    x.positions.resize(n);
+    x.velocities.resize(n);
+    x.densities.assign(n, R(0));
+    uint32_t next = 0;
+    for (size_t i = 0; i < n; ++i) {
+      x.positions[i] = {R(xyz[3 * i]), R(xyz[3 * i + 1]), R(xyz[3 * i + 2])};
+      x.velocities[i] = {R(vxyz[3 * i]), R(vxyz[3 * i + 1]), R(vxyz[3 * i + 2])};
+      next = std::max(next, ids[i] + 1);
+    }
+    x.next_id = std::max(x.next_id, next);
+    x.neighbors_built = false;
+    return (int32_t)kOk;
+  });
+}
+
+int32_t orc_read_neighbors(void* w, int32_t kind, size_t start_cap, uint32_t* start, size_t items_cap,
+                           uint32_t* items, size_t* items_len) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    const Lists& l = kind == 0 ? x.neighbors : x.wall_neighbors;
+    if (items_len) *items_len = l.items.size();
+    if (!x.neighbors_built) return (int32_t)kOutOfRange;
+    if (start_cap < l.start.size() || items_cap < l.items.size()) return (int32_t)kOutOfRange;
+    std::copy(l.start.begin(), l.start.end(), start);
+    std::copy(l.items.begin(), l.items.end(), items);
+    return (int32_t)kOk;
+  });
+}
+
+uint64_t orc_state_hash(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.state_hash(); });
+}
+
+const char* orc_error_detail(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.detail.c_str(); });
+}
+
+// FNV-1a 64 (src/snapshot/bytes.rs:6-13) over raw bytes, for known answers.
+uint64_t orc_fnv1a(const uint8_t* data, size_t n) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= data[i];
+    h *= 0x00000100000001b3ull;
+  }
+  return h;
+}
+
+// The fixed-shape FP64 residual sum (ResidualMode::kFixedTreeF64) over
+// caller values, so tests can check it against exact arithmetic.
+double orc_fixed_tree_sum(const float* e, size_t n) {
+  std::vector<float> v(e, e + n);
+  return fixed_tree_sum(v);
+}
+}

@@ -1,0 +1,249 @@
+// The C ABI of include/solunar_fluid.h over sfl::World.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/solunar_fluid.h"
+#include "world.hpp"
+
+struct sfl_world {
+  sfl::World* w;
+};
+
+namespace {
+
+thread_local std::string g_create_detail;
+
+template <class F>
+int32_t guarded(sfl_world* h, F&& f) {
+  if (!h || !h->w) return SFL_INVALID_HANDLE;
+  try {
+    return f(*h->w);
+  } catch (const std::bad_alloc&) {
+    h->w->set_detail("host allocation failed");
+    return SFL_DEVICE_ERROR;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// FluidSettings::default (src/fluid/mod.rs:80-95) and the DFSPH constants of
+// src/fluid/dfsph.rs:8-16.
+void sfl_settings_default(sfl_settings* out) {
+  if (!out) return;
+  std::memset(out, 0, sizeof(*out));
+  out->solver = SFL_WCSPH;
+  out->spacing = 0.05f;
+  out->rest_density = 1000.0f;
+  out->support = 2.0f;
+  out->max_particles = uint64_t(1) << 22;
+  out->sound_speed = 20.0f;
+  out->viscosity = 0.15f;
+  out->gravity = {0.0f, -9.81f, 0.0f};
+  out->courant = 0.4f;
+  out->max_substeps = 2000;
+  out->divergence_tolerance = 1e-3f;
+  out->density_tolerance = 5e-4f;
+  out->min_iterations = 2;
+  out->max_iterations = 100;
+  out->device = 0;
+}
+
+int32_t sfl_create(const sfl_settings* settings, sfl_world** out) {
+  if (!settings || !out) return SFL_OUT_OF_RANGE;
+  *out = nullptr;
+  try {
+    sfl::World* w = nullptr;
+    std::string why;
+    int32_t st = sfl::World::create(*settings, &w, why);
+    g_create_detail = why;
+    if (st != SFL_OK) return st;
+    *out = new sfl_world{w};
+    return SFL_OK;
+  } catch (const std::bad_alloc&) {
+    g_create_detail = "host allocation failed";
+    return SFL_DEVICE_ERROR;
+  }
+}
+
+void sfl_destroy(sfl_world* world) {
+  if (!world) return;
+  delete world->w;
+  delete world;
+}
+
+int32_t sfl_get_settings(const sfl_world* world, sfl_settings* out) {
+  if (!world || !world->w || !out) return SFL_INVALID_HANDLE;
+  *out = world->w->settings();
+  return SFL_OK;
+}
+
+float sfl_particle_mass(const sfl_world* world) { return world && world->w ? world->w->mass() : 0.0f; }
+
+size_t sfl_len(const sfl_world* world) { return world && world->w ? world->w->len() : 0; }
+
+int32_t sfl_add_particles(sfl_world* world, size_t count, const float* xyz, const float* vxyz, uint32_t* ids_out) {
+  return guarded(world, [&](sfl::World& w) {
+    if (count && !xyz) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.add_particles(count, xyz, vxyz, ids_out);
+  });
+}
+
+int32_t sfl_fill_box(sfl_world* world, sfl_vec3 low, sfl_vec3 high, size_t* count_out) {
+  return guarded(world, [&](sfl::World& w) { return w.fill_box(low, high, 0.0f, 0, count_out); });
+}
+
+int32_t sfl_fill_box_jittered(sfl_world* world, sfl_vec3 low, sfl_vec3 high, float jitter, uint32_t seed,
+                              size_t* count_out) {
+  return guarded(world, [&](sfl::World& w) { return w.fill_box(low, high, jitter, seed, count_out); });
+}
+
+int32_t sfl_set_domain(sfl_world* world, const sfl_vec3* low, const sfl_vec3* high) {
+  return guarded(world, [&](sfl::World& w) { return w.set_domain(low, high); });
+}
+
+int32_t sfl_add_box_container(sfl_world* world, sfl_vec3 low, sfl_vec3 high, uint32_t* boundary_out) {
+  return guarded(world, [&](sfl::World& w) { return w.add_box_container(low, high, boundary_out); });
+}
+
+int32_t sfl_add_wall(sfl_world* world, sfl_vec3 low, sfl_vec3 high, uint32_t* boundary_out) {
+  return guarded(world, [&](sfl::World& w) { return w.add_wall(low, high, boundary_out); });
+}
+
+int32_t sfl_add_boundary_points(sfl_world* world, size_t count, const float* xyz, uint32_t* boundary_out) {
+  return guarded(world, [&](sfl::World& w) {
+    if (count && !xyz) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.add_boundary_points(count, xyz, boundary_out);
+  });
+}
+
+int32_t sfl_remove_boundary(sfl_world* world, uint32_t boundary, int32_t* removed_out) {
+  return guarded(world, [&](sfl::World& w) { return w.remove_boundary(boundary, removed_out); });
+}
+
+size_t sfl_wall_len(const sfl_world* world) { return world && world->w ? world->w->wall_len() : 0; }
+
+int32_t sfl_read_wall_positions(const sfl_world* world, size_t cap, float* xyz) {
+  if (!world || !world->w) return SFL_INVALID_HANDLE;
+  if (cap && !xyz) return SFL_OUT_OF_RANGE;
+  return world->w->read_walls(cap, xyz);
+}
+
+int32_t sfl_step(sfl_world* world, float dt, sfl_report* report_out) {
+  return guarded(world, [&](sfl::World& w) { return w.step(dt, report_out); });
+}
+
+int32_t sfl_update_neighbors(sfl_world* world) {
+  return guarded(world, [&](sfl::World& w) { return w.update_neighbors(); });
+}
+
+int32_t sfl_compute_densities(sfl_world* world) {
+  return guarded(world, [&](sfl::World& w) { return w.compute_densities(); });
+}
+
+int32_t sfl_read_ids(const sfl_world* world, size_t cap, uint32_t* out) {
+  return guarded(const_cast<sfl_world*>(world), [&](sfl::World& w) {
+    if (cap && !out) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.read_ids(cap, out);
+  });
+}
+
+int32_t sfl_read_positions(const sfl_world* world, size_t cap, float* xyz) {
+  return guarded(const_cast<sfl_world*>(world), [&](sfl::World& w) {
+    if (cap && !xyz) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.read_positions(cap, xyz);
+  });
+}
+
+int32_t sfl_read_velocities(const sfl_world* world, size_t cap, float* xyz) {
+  return guarded(const_cast<sfl_world*>(world), [&](sfl::World& w) {
+    if (cap && !xyz) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.read_velocities(cap, xyz);
+  });
+}
+
+int32_t sfl_read_densities(const sfl_world* world, size_t cap, float* out) {
+  return guarded(const_cast<sfl_world*>(world), [&](sfl::World& w) {
+    if (cap && !out) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.read_densities(cap, out);
+  });
+}
+
+int32_t sfl_keep_forces(sfl_world* world, int32_t keep) {
+  return guarded(world, [&](sfl::World& w) { return w.keep_forces(keep != 0); });
+}
+
+int32_t sfl_read_forces(const sfl_world* world, size_t cap, float* xyz) {
+  return guarded(const_cast<sfl_world*>(world), [&](sfl::World& w) {
+    if (cap && !xyz) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.read_forces(cap, xyz);
+  });
+}
+
+int32_t sfl_read_neighbors(sfl_world* world, int32_t kind, size_t start_cap, uint32_t* start, size_t items_cap,
+                           uint32_t* items, size_t* items_len_out) {
+  return guarded(world, [&](sfl::World& w) {
+    if (kind != 0 && kind != 1) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.read_neighbors(kind, start_cap, start, items_cap, items, items_len_out);
+  });
+}
+
+int32_t sfl_state_hash(const sfl_world* world, uint64_t* out) {
+  return guarded(const_cast<sfl_world*>(world), [&](sfl::World& w) {
+    if (!out) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.state_hash(out);
+  });
+}
+
+const char* sfl_error_detail(const sfl_world* world) {
+  if (!world || !world->w) return g_create_detail.c_str();
+  return world->w->detail();
+}
+
+// Display of PhysicsError (src/error.rs:24-45).
+const char* sfl_status_text(int32_t status) {
+  switch (status) {
+    case SFL_OK: return "ok";
+    case SFL_INVALID_HANDLE: return "invalid body handle";
+    case SFL_INVALID_FRAME: return "invalid frame";
+    case SFL_FRAME_TOO_DEEP: return "frame hierarchy too deep";
+    case SFL_NON_FINITE: return "non finite value";
+    case SFL_INVALID_SHAPE: return "invalid shape";
+    case SFL_INVALID_MATERIAL: return "invalid material";
+    case SFL_INVALID_SETTINGS: return "invalid settings";
+    case SFL_NOT_DYNAMIC: return "body is not dynamic";
+    case SFL_NOT_KINEMATIC: return "body is not kinematic";
+    case SFL_OUT_OF_RANGE: return "position out of range";
+    case SFL_DISPATCH_INCOMPLETE: return "dispatcher did not run every job";
+    case SFL_INVALID_JOINT: return "invalid joint";
+    case SFL_BODY_HAS_JOINTS: return "body has joints";
+    case SFL_SHAPE_IN_USE: return "a body still uses the shape";
+    case SFL_TERRAIN_BUSY: return "terrain edits are still waiting";
+    case SFL_FLUID_FULL: return "fluid has no room for more particles";
+    case SFL_DEVICE_ERROR: return "device error";
+    case SFL_UNSUPPORTED: return "unsupported on the device path";
+    default: return "unknown status";
+  }
+}
+
+int32_t sfl_profile(sfl_world* world, int32_t enable) {
+  return guarded(world, [&](sfl::World& w) { return w.profile(enable != 0); });
+}
+
+int32_t sfl_profile_read(sfl_world* world, double* ms_out, uint64_t* launches_out, int32_t reset) {
+  return guarded(world, [&](sfl::World& w) { return w.profile_read(ms_out, launches_out, reset != 0); });
+}
+
+int32_t sfl_work_read(sfl_world* world, uint64_t* particle_substeps, uint64_t* particle_pushes,
+                      uint64_t* particle_rates, int32_t reset) {
+  return guarded(world,
+                 [&](sfl::World& w) { return w.work_read(particle_substeps, particle_pushes, particle_rates, reset != 0); });
+}
+
+int32_t sfl_synchronize(sfl_world* world) {
+  return guarded(world, [&](sfl::World& w) { return w.synchronize(); });
+}
+
+}  // extern "C"

@@ -1,0 +1,278 @@
+// Device-side types and arithmetic shared by the gfx950 kernels of the SPH
+// fluid step.
+//
+// Every floating-point expression here is written in the operation order of
+// the reference's Rust code (src/fluid/mod.rs, step.rs, dfsph.rs) and the
+// library is compiled with -ffp-contract=off and correctly rounded f32
+// division and square root, so each pair term is the same IEEE single value
+// the CPU step computes. Sums over a particle's neighbours are left folds in
+// the reference's list order (cells dz, dy, dx; ids ascending within a
+// cell), one lane per particle, which keeps densities and forces bit-equal
+// to the CPU step rather than merely close.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfl {
+
+// Control flags (sticky within a step). Kernels never test them: they test
+// `Control::abort`, which only changes between launches (an overflow found by
+// the grid setup, a non-finite advance promoted at the next substep's start),
+// so every lane of a launch sees one value.
+constexpr uint32_t kFlagOverflow = 1u;   // the cell box outgrew the table
+constexpr uint32_t kFlagNonFinite = 2u;  // an advance produced a non-finite value
+constexpr uint32_t kFlagBarrier = 4u;    // a grid barrier gave up waiting
+
+constexpr int kResidualLogCap = 256;
+
+// Device-resident control block of one world: cell box, report reductions,
+// solve counters and the persistent kernels' barrier.
+struct Control {
+  int32_t bmin[3];
+  int32_t bmax[3];
+  int32_t lo[3];
+  uint32_t span[3];
+  uint32_t volume;
+  uint32_t flags;
+  uint32_t abort;        // non-zero: every later launch of the step is a no-op
+  uint32_t pad0;
+  uint64_t need_volume;  // cells the box needed when it overflowed
+  uint32_t bad_index;    // smallest sorted index whose advance went non-finite
+  uint32_t held;         // particles held back this step
+  uint32_t vmax_bits;    // bits of max |v| after the last advance
+  uint32_t comp_bits;    // bits of max(rho/rest - 1, 0), last density pass
+  uint32_t substeps_done;
+  uint32_t iterations[2];        // last solve's pushes: divergence, density
+  uint32_t iterations_step[2];   // pushes summed over this step
+  uint32_t rates_step;           // rate passes summed over this step
+  uint32_t bar_count;
+  uint32_t bar_gen;
+  uint32_t log_len;
+  uint32_t pad;
+  // average error of every rate pass (solve in the sign: + divergence, -
+  // density), for parity logs
+  double residual_log[kResidualLogCap];
+  uint32_t loaded_log[kResidualLogCap];
+};
+
+// A dense cell table over a box of cells: slot of cell (x, y, z) is
+// ((x - lo.x) * span.y + (y - lo.y)) * span.z + (z - lo.z); particles of slot
+// c are start[c] .. start[c + 1] in the sorted arrays.
+struct CellBox {
+  int32_t lo[3];
+  uint32_t span[3];
+};
+
+// Scalars of one substep, computed on the host in the reference's f32
+// arithmetic.
+struct StepParams {
+  uint32_t n;
+  float radius;
+  float reach;      // radius * radius
+  float scale;      // Wendland normalisation
+  float mass;
+  float rest;
+  float stiffness;  // rest * c^2 / 7
+  float dt;
+  float nu;         // viscosity(dt)
+  float wall_nu;    // wall viscosity
+  float soften;     // 0.01 r^2
+  float gx, gy, gz;
+  float half_spacing;
+  int32_t has_bounds;
+  float blo[3], bhi[3];
+  // walls (static, sorted by cell)
+  const float4* walls;
+  const uint32_t* wall_start;
+  CellBox wall_box;
+  uint32_t wall_cells;  // 0: no walls
+  // fluid cell table (box read from the control block)
+  const uint32_t* cell_start;
+  Control* ctl;
+};
+
+// ---- Wendland C2 (src/fluid/mod.rs:126-165) ------------------------------
+
+__device__ __forceinline__ float kernel_value(float distance, float radius, float scale) {
+  float q = distance / radius;
+  if (q >= 1.0f) return 0.0f;
+  float t = 1.0f - q;
+  float t2 = t * t;
+  float t4 = t2 * t2;
+  return scale * t4 * (1.0f + 4.0f * q);
+}
+
+__device__ __forceinline__ float kernel_slope(float distance, float radius, float scale) {
+  float q = distance / radius;
+  if (q >= 1.0f) return 0.0f;
+  float t = 1.0f - q;
+  float t3 = t * (t * t);
+  return scale * -20.0f * q * t3 / radius;
+}
+
+// Kernel::gradient (mod.rs:158-164) given the offset and its squared length.
+__device__ __forceinline__ void kernel_gradient(float ox, float oy, float oz, float squared, float radius, float scale,
+                                                float& gx, float& gy, float& gz) {
+  float distance = sqrtf(squared);
+  if (distance <= 0.0f) {
+    gx = 0.0f;
+    gy = 0.0f;
+    gz = 0.0f;
+    return;
+  }
+  float f = kernel_slope(distance, radius, scale) / distance;
+  gx = ox * f;
+  gy = oy * f;
+  gz = oz * f;
+}
+
+// (x*x' + y*y') + z*z', no contraction
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+  return ax * bx + ay * by + az * bz;
+}
+
+// `x as i32` (saturating, NaN -> 0) of floor(v / radius) (mod.rs:526-533)
+__device__ __forceinline__ int32_t cell_coord(float v, float radius) {
+  float c = floorf(v / radius);
+  if (!(c == c)) return 0;
+  if (c >= 2147483648.0f) return 2147483647;
+  if (c <= -2147483648.0f) return (int32_t)0x80000000u;
+  return (int32_t)c;
+}
+
+// Slot of cell (x, y, z) in `box`, or -1 outside it.
+__device__ __forceinline__ int64_t cell_slot(int64_t x, int64_t y, int64_t z, const int32_t lo[3],
+                                             const uint32_t span[3]) {
+  int64_t ax = x - lo[0], ay = y - lo[1], az = z - lo[2];
+  if (ax < 0 || ay < 0 || az < 0 || ax >= (int64_t)span[0] || ay >= (int64_t)span[1] || az >= (int64_t)span[2])
+    return -1;
+  return (ax * (int64_t)span[1] + ay) * (int64_t)span[2] + az;
+}
+
+__device__ __forceinline__ uint32_t id_of(float4 p) { return __float_as_uint(p.w); }
+
+// f32::max / f32::min with NaN operands yielding the other one
+__device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+
+__device__ __forceinline__ bool finite3(float x, float y, float z) {
+  return isfinite(x) && isfinite(y) && isfinite(z);
+}
+
+// Tait pressure (step.rs:151-158).
+__device__ __forceinline__ float tait_pressure(float density, float rest, float stiffness) {
+  float ratio = rmin(density / rest, 1.1f);
+  float r2 = ratio * ratio;
+  float r4 = r2 * r2;
+  float r7 = ratio * r2 * r4;
+  return rmax(stiffness * (r7 - 1.0f), 0.0f);
+}
+
+}  // namespace sfl
+
+namespace sfl {
+
+// Calls f(j, pj, ox, oy, oz, squared) for every particle j of `P` within
+// reach of `at`, o = at - P[j], in the reference's neighbour order: cells
+// dz, dy, dx (Grid::around, src/fluid/mod.rs:250-261), then the cell's range
+// in sorted order; membership |o|^2 < reach as in `near` (mod.rs:264-277),
+// whose |x_j - x_i|^2 has the same bits.
+template <class F>
+__device__ __forceinline__ void visit_near(const float4* __restrict__ P, const uint32_t* __restrict__ start,
+                                           const int32_t lo[3], const uint32_t span[3], float4 at, float radius,
+                                           float reach, F&& f) {
+  int32_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
+        if (c < 0) continue;
+        uint32_t s = start[c], e = start[c + 1];
+        for (uint32_t j = s; j < e; ++j) {
+          float4 pj = P[j];
+          float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+          float sq = dot3(ox, oy, oz, ox, oy, oz);
+          if (sq < reach) f(j, pj, ox, oy, oz, sq);
+        }
+      }
+}
+
+// Wave-wide max of a non-negative float, folded into `bits` by lane 0.
+__device__ __forceinline__ void wave_max_to(uint32_t* bits, float v) {
+  uint32_t b = __float_as_uint(v > 0.0f ? v : 0.0f);
+  for (int off = 32; off >= 1; off >>= 1) b = max(b, (uint32_t)__shfl_xor((int)b, off, 64));
+  if ((threadIdx.x & 63) == 0 && b) atomicMax(bits, b);
+}
+
+__device__ __forceinline__ void wave_sum_to(uint32_t* dst, uint32_t v) {
+  for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+// advance (src/fluid/step.rs:305-339) of one particle: next position, the
+// clamped velocity components zeroed, held-back count, finiteness.
+__device__ __forceinline__ bool advance_one(const StepParams& prm, float4 p, float& vx, float& vy, float& vz,
+                                            float& nx, float& ny, float& nz, uint32_t& held) {
+  nx = p.x + vx * prm.dt;
+  ny = p.y + vy * prm.dt;
+  nz = p.z + vz * prm.dt;
+  held = 0;
+  if (prm.has_bounds) {
+    float h = prm.half_spacing;
+    float ix = rmin(rmax(nx, prm.blo[0] + h), prm.bhi[0] - h);
+    float iy = rmin(rmax(ny, prm.blo[1] + h), prm.bhi[1] - h);
+    float iz = rmin(rmax(nz, prm.blo[2] + h), prm.bhi[2] - h);
+    if (ix != nx || iy != ny || iz != nz) {
+      float cx = rmin(rmax(nx, prm.blo[0]), prm.bhi[0]);
+      float cy = rmin(rmax(ny, prm.blo[1]), prm.bhi[1]);
+      float cz = rmin(rmax(nz, prm.blo[2]), prm.bhi[2]);
+      if (cx != nx || cy != ny || cz != nz) held = 1;
+      if (ix != nx) vx = 0.0f;
+      if (iy != ny) vy = 0.0f;
+      if (iz != nz) vz = 0.0f;
+      nx = ix;
+      ny = iy;
+      nz = iz;
+    }
+  }
+  return finite3(nx, ny, nz) && finite3(vx, vy, vz);
+}
+
+// wall_viscous (src/fluid/step.rs:262-301), no terrains; `away` is the
+// finished sum of -gradient over the particle's walls.
+template <class Visit>
+__device__ __forceinline__ void wall_viscous(const StepParams& prm, float4 at, float vx, float vy, float vz,
+                                             float density, float awx, float awy, float awz, Visit&& visit_walls,
+                                             float& px, float& py, float& pz) {
+  px = 0.0f;
+  py = 0.0f;
+  pz = 0.0f;
+  // normalize_or(away, ZERO)
+  float len = sqrtf(dot3(awx, awy, awz, awx, awy, awz));
+  float rcp = 1.0f / len;
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  if (isfinite(rcp) && rcp > 0.0f) {
+    nx = awx * rcp;
+    ny = awy * rcp;
+    nz = awz * rcp;
+  }
+  float t = rmin(dot3(vx, vy, vz, nx, ny, nz), 0.0f);
+  float ix = nx * t, iy = ny * t, iz = nz * t;
+  float wnu = prm.wall_nu;
+  visit_walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
+    float toward = dot3(ix, iy, iz, ox, oy, oz);
+    if (toward < 0.0f) {
+      float squeeze = toward / (sq + prm.soften);
+      float damp = -wnu * squeeze / density;
+      float gx, gy, gz;
+      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+      float k = prm.mass * damp;
+      px = px - gx * k;
+      py = py - gy * k;
+      pz = pz - gz * k;
+    }
+  });
+}
+
+}  // namespace sfl

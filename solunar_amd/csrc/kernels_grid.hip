@@ -1,0 +1,424 @@
+// Neighbour build on gfx950: cell bounds, counting sort by cell, cell-start
+// table, in-cell order by id, and the gather into sorted order. Replaces
+// update_neighbors' sort, gathers and Grid::build (src/fluid/mod.rs:540-566,
+// 182-247); the CSR list emission at the end replaces `near` + Lists::join
+// (mod.rs:264-308, 569-607) for the readback and list-density entry points.
+//
+// The permutation is the reference's: slots of the dense cell table are
+// lexicographic in (cx, cy, cz), so the counting sort orders cells exactly as
+// sort_unstable on (cell, id, index) does, and within a cell each particle's
+// slot is the number of cell-mates with a smaller id (ids are unique), which
+// does not depend on the order the counting atomics resolved in.
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace sfl {
+
+// ---- per-step / per-substep resets of the control block ----------------
+
+__global__ void k_step_begin(Control* ctl) {
+  if (threadIdx.x != 0) return;
+  ctl->flags = 0;
+  ctl->abort = 0;
+  ctl->held = 0;
+  ctl->bad_index = 0xffffffffu;
+  ctl->substeps_done = 0;
+  ctl->iterations_step[0] = 0;
+  ctl->iterations_step[1] = 0;
+  ctl->rates_step = 0;
+  ctl->log_len = 0;
+}
+
+__global__ void k_substep_begin(Control* ctl) {
+  if (threadIdx.x != 0) return;
+  if (ctl->flags & kFlagNonFinite) ctl->abort = 1;
+  if (!ctl->abort) ctl->substeps_done += 1;
+  ctl->bmin[0] = ctl->bmin[1] = ctl->bmin[2] = 0x7fffffff;
+  ctl->bmax[0] = ctl->bmax[1] = ctl->bmax[2] = (int32_t)0x80000000u;
+  ctl->comp_bits = 0;
+  ctl->vmax_bits = 0;
+}
+
+// fastest (src/fluid/step.rs:92-96): max |v| over the particles.
+__global__ __launch_bounds__(256) void k_fastest(const float4* __restrict__ V, uint32_t n, Control* ctl) {
+  float most = 0.0f;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 v = V[i];
+    most = rmax(most, sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z)));
+  }
+  wave_max_to(&ctl->vmax_bits, most);
+}
+
+// ---- cell bounds ---------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ P, uint32_t n, float radius,
+                                                Control* ctl) {
+  if (ctl->abort) return;
+  int32_t mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  int32_t mx[3] = {(int32_t)0x80000000u, (int32_t)0x80000000u, (int32_t)0x80000000u};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 p = P[i];
+    int32_t c[3] = {cell_coord(p.x, radius), cell_coord(p.y, radius), cell_coord(p.z, radius)};
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = min(mn[a], c[a]);
+      mx[a] = max(mx[a], c[a]);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      mn[a] = min(mn[a], __shfl_xor(mn[a], off, 64));
+      mx[a] = max(mx[a], __shfl_xor(mx[a], off, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&ctl->bmin[a], mn[a]);
+      atomicMax(&ctl->bmax[a], mx[a]);
+    }
+  }
+}
+
+// Box of the cell table from the bounds; flags an overflow when it needs more
+// slots than `capacity` (then every later launch of the step does nothing and
+// the host grows the table).
+__global__ void k_grid_setup(Control* ctl, uint32_t n, uint64_t capacity) {
+  if (threadIdx.x != 0) return;
+  if (ctl->abort) return;
+  if (n == 0) {
+    for (int a = 0; a < 3; ++a) {
+      ctl->lo[a] = 0;
+      ctl->span[a] = 1;
+    }
+    ctl->volume = 1;
+    return;
+  }
+  uint64_t vol = 1;
+  bool big = false;
+  uint32_t span[3];
+  for (int a = 0; a < 3; ++a) {
+    int64_t s = (int64_t)ctl->bmax[a] - (int64_t)ctl->bmin[a] + 1;
+    if (s > 0xffffffffll) {
+      big = true;
+      s = 0xffffffffll;
+    }
+    span[a] = (uint32_t)s;
+    uint64_t prod = vol * (uint64_t)s;
+    if (s != 0 && prod / (uint64_t)s != vol) big = true;
+    vol = big ? 0xffffffffffffffffull : prod;
+  }
+  if (big || vol > capacity || vol >= 0xffffffffull) {
+    ctl->flags |= kFlagOverflow;
+    ctl->abort = 1;
+    ctl->need_volume = vol;
+    return;
+  }
+  for (int a = 0; a < 3; ++a) {
+    ctl->lo[a] = ctl->bmin[a];
+    ctl->span[a] = span[a];
+  }
+  ctl->volume = (uint32_t)vol;
+}
+
+// Cell slot of every particle and its arrival rank in the slot.
+__global__ __launch_bounds__(256) void k_count(const float4* __restrict__ P, uint32_t n, float radius,
+                                               uint32_t* __restrict__ counts, uint2* __restrict__ slot_rank,
+                                               Control* ctl) {
+  if (ctl->abort) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+  uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+  float4 p = P[i];
+  int64_t c = cell_slot(cell_coord(p.x, radius), cell_coord(p.y, radius), cell_coord(p.z, radius), lo, span);
+  uint32_t slot = (uint32_t)c;
+  uint32_t rank = atomicAdd(&counts[slot], 1u);
+  slot_rank[i] = make_uint2(slot, rank);
+}
+
+// ---- exclusive scan: out[k] = sum(in[0..k)), out[len] = total -----------
+// Tiles of 256 threads x 8 items. `len` comes from `len_dev` when given (the
+// cell table's volume) else `len_host`. With `zero_input` the input is
+// cleared as it is read, which keeps the cell counts zeroed between substeps.
+
+constexpr uint32_t kScanItems = 8;
+constexpr uint32_t kScanTile = 256 * kScanItems;
+
+__device__ __forceinline__ uint32_t scan_len(const uint32_t* len_dev, uint32_t len_host) {
+  return len_dev ? *len_dev : len_host;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                    uint32_t* __restrict__ tile_sums, const uint32_t* len_dev,
+                                                    uint32_t len_host, int zero_input, const Control* ctl) {
+  if (ctl && ctl->abort) return;
+  uint32_t len = scan_len(len_dev, len_host);
+  uint32_t base = blockIdx.x * kScanTile;
+  if (base >= len) return;
+  __shared__ uint32_t warp_sums[4];
+  uint32_t v[kScanItems];
+  uint32_t local = 0;
+  uint32_t first = base + threadIdx.x * kScanItems;
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t idx = first + k;
+    uint32_t x = idx < len ? in[idx] : 0u;
+    if (zero_input && idx < len) in[idx] = 0u;
+    v[k] = local;
+    local += x;
+  }
+  // inclusive scan of `local` across the block
+  uint32_t lane = threadIdx.x & 63, warp = threadIdx.x >> 6;
+  uint32_t incl = local;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= (uint32_t)off) incl += y;
+  }
+  if (lane == 63) warp_sums[warp] = incl;
+  __syncthreads();
+  uint32_t warp_off = 0;
+  for (uint32_t w = 0; w < warp; ++w) warp_off += warp_sums[w];
+  uint32_t excl = warp_off + incl - local;
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    uint32_t idx = first + k;
+    if (idx < len) out[idx] = excl + v[k];
+  }
+  if (threadIdx.x == 255) tile_sums[blockIdx.x] = warp_off + incl;
+}
+
+// One block: exclusive scan of the tile sums in place, total to `total_out`.
+__global__ __launch_bounds__(1024) void k_scan_sums(uint32_t* __restrict__ tile_sums, const uint32_t* len_dev,
+                                                    uint32_t len_host, uint32_t* __restrict__ total_out,
+                                                    const Control* ctl) {
+  if (ctl && ctl->abort) return;
+  uint32_t len = scan_len(len_dev, len_host);
+  uint32_t tiles = (len + kScanTile - 1) / kScanTile;
+  __shared__ uint32_t warp_sums[16];
+  __shared__ uint32_t carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  uint32_t lane = threadIdx.x & 63, warp = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < tiles; base += 1024) {
+    uint32_t idx = base + threadIdx.x;
+    uint32_t x = idx < tiles ? tile_sums[idx] : 0u;
+    uint32_t incl = x;
+    for (int off = 1; off < 64; off <<= 1) {
+      uint32_t y = __shfl_up(incl, off, 64);
+      if (lane >= (uint32_t)off) incl += y;
+    }
+    if (lane == 63) warp_sums[warp] = incl;
+    __syncthreads();
+    uint32_t warp_off = 0;
+    for (uint32_t w = 0; w < warp; ++w) warp_off += warp_sums[w];
+    uint32_t carry = carry_s;
+    if (idx < tiles) tile_sums[idx] = carry + warp_off + incl - x;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry_s = carry + warp_off + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total_out = carry_s;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restrict__ tile_sums,
+                                                  const uint32_t* len_dev, uint32_t len_host,
+                                                  const uint32_t* __restrict__ total, const Control* ctl) {
+  if (ctl && ctl->abort) return;
+  uint32_t len = scan_len(len_dev, len_host);
+  uint32_t base = blockIdx.x * kScanTile;
+  if (base >= len) return;
+  uint32_t add = tile_sums[blockIdx.x];
+  for (uint32_t k = threadIdx.x; k < kScanTile; k += 256) {
+    uint32_t idx = base + k;
+    if (idx < len) out[idx] += add;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[len] = *total;
+}
+
+// Each particle to its cell's range, in arrival order, with its id.
+__global__ __launch_bounds__(256) void k_scatter(const float4* __restrict__ P, uint32_t n,
+                                                 const uint2* __restrict__ slot_rank,
+                                                 const uint32_t* __restrict__ start, uint2* __restrict__ pairs,
+                                                 const Control* ctl) {
+  if (ctl->abort) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint2 sr = slot_rank[i];
+  uint32_t id = id_of(P[i]);
+  pairs[start[sr.x] + sr.y] = make_uint2(i, id);
+}
+
+// Sorted slot = cell start + number of cell-mates with a smaller id; moves
+// x, id (P) and v, rho (V) there.
+__global__ __launch_bounds__(256) void k_rank_gather(const float4* __restrict__ PA, const float4* __restrict__ VA,
+                                                     uint32_t n, float radius, const uint2* __restrict__ pairs,
+                                                     const uint32_t* __restrict__ start, float4* __restrict__ PB,
+                                                     float4* __restrict__ VB, const Control* ctl) {
+  if (ctl->abort) return;
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+  uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+  uint2 me = pairs[q];
+  float4 p = PA[me.x];
+  float4 v = VA[me.x];
+  int64_t c = cell_slot(cell_coord(p.x, radius), cell_coord(p.y, radius), cell_coord(p.z, radius), lo, span);
+  uint32_t s = start[c], e = start[c + 1];
+  uint32_t rank = 0;
+  for (uint32_t k = s; k < e; ++k) rank += pairs[k].y < me.y ? 1u : 0u;
+  PB[s + rank] = p;
+  VB[s + rank] = v;
+}
+
+// ---- CSR neighbour lists (readback / list density only) ----------------
+
+// Visits, in the reference's order, every particle of `P` within reach of
+// `at` through the dense table, calling f(j) for each.
+template <class F>
+__device__ __forceinline__ void for_each_near(const float4* __restrict__ P, const uint32_t* __restrict__ start,
+                                              const int32_t lo[3], const uint32_t span[3], float4 at, float radius,
+                                              float reach, F&& f) {
+  int32_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
+        if (c < 0) continue;
+        uint32_t s = start[c], e = start[c + 1];
+        for (uint32_t j = s; j < e; ++j) {
+          float4 o = P[j];
+          float ox = o.x - at.x, oy = o.y - at.y, oz = o.z - at.z;
+          if (dot3(ox, oy, oz, ox, oy, oz) < reach) f(j);
+        }
+      }
+}
+
+// kind 0: fluid neighbours from `lp` (the positions the table was built on);
+// kind 1: walls. Counts per particle.
+__global__ __launch_bounds__(256) void k_list_count(const float4* __restrict__ lp, StepParams prm,
+                                                    uint32_t* __restrict__ counts, int kind) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= prm.n) return;
+  uint32_t cnt = 0;
+  float4 at = lp[i];
+  if (kind == 0) {
+    int32_t lo[3] = {prm.ctl->lo[0], prm.ctl->lo[1], prm.ctl->lo[2]};
+    uint32_t span[3] = {prm.ctl->span[0], prm.ctl->span[1], prm.ctl->span[2]};
+    for_each_near(lp, prm.cell_start, lo, span, at, prm.radius, prm.reach, [&](uint32_t) { ++cnt; });
+  } else if (prm.wall_cells) {
+    for_each_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
+                  [&](uint32_t) { ++cnt; });
+  }
+  counts[i] = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_list_emit(const float4* __restrict__ lp, StepParams prm,
+                                                   const uint32_t* __restrict__ list_start,
+                                                   uint32_t* __restrict__ items, int kind) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= prm.n) return;
+  uint32_t at_k = list_start[i];
+  float4 at = lp[i];
+  if (kind == 0) {
+    int32_t lo[3] = {prm.ctl->lo[0], prm.ctl->lo[1], prm.ctl->lo[2]};
+    uint32_t span[3] = {prm.ctl->span[0], prm.ctl->span[1], prm.ctl->span[2]};
+    for_each_near(lp, prm.cell_start, lo, span, at, prm.radius, prm.reach, [&](uint32_t j) { items[at_k++] = j; });
+  } else if (prm.wall_cells) {
+    for_each_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
+                  [&](uint32_t j) { items[at_k++] = j; });
+  }
+}
+
+// compute_densities (src/fluid/mod.rs:621-675) over the CSR lists, with the
+// current positions, as the reference does after a step.
+__global__ __launch_bounds__(256) void k_list_density(const float4* __restrict__ P, float4* __restrict__ V,
+                                                      StepParams prm, const uint32_t* __restrict__ fstart,
+                                                      const uint32_t* __restrict__ fitems,
+                                                      const uint32_t* __restrict__ wstart,
+                                                      const uint32_t* __restrict__ witems) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= prm.n) return;
+  float4 at = P[i];
+  float sum = 0.0f;
+  for (uint32_t k = fstart[i]; k < fstart[i + 1]; ++k) {
+    float4 o = P[fitems[k]];
+    float ox = o.x - at.x, oy = o.y - at.y, oz = o.z - at.z;
+    sum += kernel_value(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), prm.radius, prm.scale);
+  }
+  float wsum = 0.0f;
+  for (uint32_t k = wstart[i]; k < wstart[i + 1]; ++k) {
+    float4 o = prm.walls[witems[k]];
+    float ox = o.x - at.x, oy = o.y - at.y, oz = o.z - at.z;
+    wsum += kernel_value(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), prm.radius, prm.scale);
+  }
+  V[i].w = prm.mass * (sum + wsum);
+}
+
+// ---- launchers ------------------------------------------------------------
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+void launch_step_begin(hipStream_t s, Control* ctl) { hipLaunchKernelGGL(k_step_begin, 1, 64, 0, s, ctl); }
+void launch_substep_begin(hipStream_t s, Control* ctl) { hipLaunchKernelGGL(k_substep_begin, 1, 64, 0, s, ctl); }
+
+void launch_fastest(hipStream_t s, const float4* V, uint32_t n, Control* ctl) {
+  uint32_t blocks = blocks_for(n, 256);
+  if (blocks > 1024) blocks = 1024;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_fastest, blocks, 256, 0, s, V, n, ctl);
+}
+
+void launch_bounds(hipStream_t s, const float4* P, uint32_t n, float radius, Control* ctl) {
+  uint32_t blocks = blocks_for(n, 256);
+  if (blocks > 2048) blocks = 2048;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_bounds, blocks, 256, 0, s, P, n, radius, ctl);
+}
+
+void launch_grid_setup(hipStream_t s, Control* ctl, uint32_t n, uint64_t capacity) {
+  hipLaunchKernelGGL(k_grid_setup, 1, 64, 0, s, ctl, n, capacity);
+}
+
+void launch_count(hipStream_t s, const float4* P, uint32_t n, float radius, uint32_t* counts, uint2* slot_rank,
+                  Control* ctl) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_count, blocks_for(n, 256), 256, 0, s, P, n, radius, counts, slot_rank, ctl);
+}
+
+void launch_scan(hipStream_t s, uint32_t* in, uint32_t* out, uint32_t* tile_sums, uint32_t* total,
+                 const uint32_t* len_dev, uint64_t len_max, int zero_input, const Control* ctl) {
+  uint32_t tiles = blocks_for(len_max, kScanTile);
+  if (tiles == 0) tiles = 1;
+  uint32_t len_host = (uint32_t)len_max;
+  hipLaunchKernelGGL(k_scan_tiles, tiles, 256, 0, s, in, out, tile_sums, len_dev, len_host, zero_input, ctl);
+  hipLaunchKernelGGL(k_scan_sums, 1, 1024, 0, s, tile_sums, len_dev, len_host, total, ctl);
+  hipLaunchKernelGGL(k_scan_add, tiles, 256, 0, s, out, tile_sums, len_dev, len_host, total, ctl);
+}
+
+void launch_scatter(hipStream_t s, const float4* P, uint32_t n, const uint2* slot_rank, const uint32_t* start,
+                    uint2* pairs, const Control* ctl) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_scatter, blocks_for(n, 256), 256, 0, s, P, n, slot_rank, start, pairs, ctl);
+}
+
+void launch_rank_gather(hipStream_t s, const float4* PA, const float4* VA, uint32_t n, float radius,
+                        const uint2* pairs, const uint32_t* start, float4* PB, float4* VB, const Control* ctl) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_rank_gather, blocks_for(n, 256), 256, 0, s, PA, VA, n, radius, pairs, start, PB, VB, ctl);
+}
+
+void launch_list_count(hipStream_t s, const float4* lp, const StepParams& prm, uint32_t* counts, int kind) {
+  if (prm.n == 0) return;
+  hipLaunchKernelGGL(k_list_count, blocks_for(prm.n, 256), 256, 0, s, lp, prm, counts, kind);
+}
+
+void launch_list_emit(hipStream_t s, const float4* lp, const StepParams& prm, const uint32_t* list_start,
+                      uint32_t* items, int kind) {
+  if (prm.n == 0) return;
+  hipLaunchKernelGGL(k_list_emit, blocks_for(prm.n, 256), 256, 0, s, lp, prm, list_start, items, kind);
+}
+
+void launch_list_density(hipStream_t s, const float4* P, float4* V, const StepParams& prm, const uint32_t* fstart,
+                         const uint32_t* fitems, const uint32_t* wstart, const uint32_t* witems) {
+  if (prm.n == 0) return;
+  hipLaunchKernelGGL(k_list_density, blocks_for(prm.n, 256), 256, 0, s, P, V, prm, fstart, fitems, wstart, witems);
+}
+
+}  // namespace sfl

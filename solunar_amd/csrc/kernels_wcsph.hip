@@ -1,0 +1,212 @@
+// WCSPH substep on gfx950 (src/fluid/step.rs:144-224): the density pass with
+// the Tait pressure, then one fused pass of pressure force, artificial
+// viscosity, wall pressure, wall viscosity, gravity, the velocity update and
+// the symplectic advance. Both passes read the state gathered into sorted
+// order (B) and the force pass writes the next state back to A, so a
+// substep that aborted before the gather leaves A as it was.
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace sfl {
+
+// compute_densities (src/fluid/mod.rs:621-675) for every particle, then the
+// pressure over density squared of step.rs:151-172. rho goes to V.w, p/rho^2
+// to `scaled`. Also folds max(rho/rest - 1, 0) for the report (step.rs:84-88).
+__global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict__ P, float4* __restrict__ V,
+                                                       float* __restrict__ scaled, StepParams prm) {
+  Control* ctl = prm.ctl;
+  if (ctl->abort) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = i < prm.n;
+  float comp = 0.0f;
+  if (live) {
+    int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+    uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+    float4 at = P[i];
+    float sum = 0.0f;
+    visit_near(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+               [&](uint32_t, float4, float, float, float, float sq) {
+                 sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+               });
+    float wsum = 0.0f;
+    if (prm.wall_cells)
+      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
+                 [&](uint32_t, float4, float, float, float, float sq) {
+                   wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+                 });
+    float density = prm.mass * (sum + wsum);
+    float pressure = tait_pressure(density, prm.rest, prm.stiffness);
+    V[i].w = density;
+    scaled[i] = pressure / (density * density);
+    comp = density / prm.rest - 1.0f;
+  }
+  wave_max_to(&ctl->comp_bits, comp);
+}
+
+// The force loop of wcsph_substep (step.rs:173-211) with wall_viscous
+// (step.rs:262-301), v += push * dt (step.rs:212-222) and advance
+// (step.rs:305-339), for every particle. Reads the sorted state PB/VB, writes
+// PA/VA; `forces` (optional) keeps the push.
+__global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ PB, const float4* __restrict__ VB,
+                                                     const float* __restrict__ scaled, StepParams prm,
+                                                     float4* __restrict__ PA, float4* __restrict__ VA,
+                                                     float* __restrict__ forces) {
+  Control* ctl = prm.ctl;
+  if (ctl->abort) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = i < prm.n;
+  float speed = 0.0f;
+  uint32_t held = 0;
+  if (live) {
+    int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+    uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+    float4 at = PB[i];
+    float4 vi = VB[i];
+    float density = vi.w;
+    float own = scaled[i];
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    visit_near(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+               [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+                 if (sq <= 0.0f) return;
+                 float distance = sqrtf(sq);
+                 float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
+                 float gx = ox * f, gy = oy * f, gz = oz * f;
+                 float4 vj = VB[j];
+                 float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
+                 float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
+                 float k = prm.mass * (own + scaled[j] + damp);
+                 px = px - gx * k;
+                 py = py - gy * k;
+                 pz = pz - gz * k;
+               });
+    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    if (prm.wall_cells) {
+      float pressure = tait_pressure(density, prm.rest, prm.stiffness);
+      float mirrored = own + pressure / (prm.rest * prm.rest);
+      float awx = 0.0f, awy = 0.0f, awz = 0.0f;
+      auto walls = [&](auto&& f) {
+        visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
+      };
+      walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
+        float gx, gy, gz;
+        kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+        float k = prm.mass * mirrored;
+        px = px - gx * k;
+        py = py - gy * k;
+        pz = pz - gz * k;
+        awx = awx + -gx;
+        awy = awy + -gy;
+        awz = awz + -gz;
+      });
+      wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
+    }
+    // push + wall_viscous + gravity
+    float ax = px + wx + prm.gx;
+    float ay = py + wy + prm.gy;
+    float az = pz + wz + prm.gz;
+    if (forces) {
+      forces[3 * i] = ax;
+      forces[3 * i + 1] = ay;
+      forces[3 * i + 2] = az;
+    }
+    float vx = vi.x + ax * prm.dt;
+    float vy = vi.y + ay * prm.dt;
+    float vz = vi.z + az * prm.dt;
+    float nx, ny, nz;
+    bool ok = advance_one(prm, at, vx, vy, vz, nx, ny, nz, held);
+    if (!ok) {
+      atomicOr(&ctl->flags, kFlagNonFinite);
+      atomicMin(&ctl->bad_index, i);
+    }
+    PA[i] = make_float4(nx, ny, nz, at.w);
+    VA[i] = make_float4(vx, vy, vz, density);
+    speed = sqrtf(dot3(vx, vy, vz, vx, vy, vz));
+  }
+  wave_max_to(&ctl->vmax_bits, speed);
+  wave_sum_to(&ctl->held, held);
+}
+
+// Advances `PB` by v*dt into `PA` for particles after the first non-finite
+// one, undoing what the fused pass wrote past it: the reference stops its
+// advance there (step.rs:333-335), leaving later particles at their old
+// position with the pushed, unclamped velocity. Recomputes that velocity.
+__global__ __launch_bounds__(256) void k_wcsph_repair(const float4* __restrict__ PB, const float4* __restrict__ VB,
+                                                      const float* __restrict__ scaled, StepParams prm,
+                                                      float4* __restrict__ PA, float4* __restrict__ VA) {
+  Control* ctl = prm.ctl;
+  uint32_t bad = ctl->bad_index;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= prm.n || i < bad) return;
+  float4 at = PB[i];
+  if (i == bad) {
+    PA[i] = at;  // velocity keeps its zeroed components, position stays
+    return;
+  }
+  int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+  uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+  float4 vi = VB[i];
+  float density = vi.w;
+  float own = scaled[i];
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+  visit_near(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+             [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+               if (sq <= 0.0f) return;
+               float distance = sqrtf(sq);
+               float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
+               float gx = ox * f, gy = oy * f, gz = oz * f;
+               float4 vj = VB[j];
+               float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
+               float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
+               float k = prm.mass * (own + scaled[j] + damp);
+               px = px - gx * k;
+               py = py - gy * k;
+               pz = pz - gz * k;
+             });
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  if (prm.wall_cells) {
+    float pressure = tait_pressure(density, prm.rest, prm.stiffness);
+    float mirrored = own + pressure / (prm.rest * prm.rest);
+    float awx = 0.0f, awy = 0.0f, awz = 0.0f;
+    auto walls = [&](auto&& f) {
+      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
+    };
+    walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
+      float gx, gy, gz;
+      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+      float k = prm.mass * mirrored;
+      px = px - gx * k;
+      py = py - gy * k;
+      pz = pz - gz * k;
+      awx = awx + -gx;
+      awy = awy + -gy;
+      awz = awz + -gz;
+    });
+    wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
+  }
+  float ax = px + wx + prm.gx;
+  float ay = py + wy + prm.gy;
+  float az = pz + wz + prm.gz;
+  PA[i] = at;
+  VA[i] = make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density);
+}
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+void launch_wcsph_density(hipStream_t s, const float4* P, float4* V, float* scaled, const StepParams& prm) {
+  if (prm.n == 0) return;
+  hipLaunchKernelGGL(k_wcsph_density, blocks_for(prm.n, 256), 256, 0, s, P, V, scaled, prm);
+}
+
+void launch_wcsph_force(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
+                        const StepParams& prm, float4* PA, float4* VA, float* forces) {
+  if (prm.n == 0) return;
+  hipLaunchKernelGGL(k_wcsph_force, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA, forces);
+}
+
+void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
+                         const StepParams& prm, float4* PA, float4* VA) {
+  if (prm.n == 0) return;
+  hipLaunchKernelGGL(k_wcsph_repair, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA);
+}
+
+}  // namespace sfl

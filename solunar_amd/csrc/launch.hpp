@@ -1,0 +1,51 @@
+// Host-side launchers of the gfx950 kernels (defined next to the kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace sfl {
+
+// neighbour build (kernels_grid.hip)
+void launch_step_begin(hipStream_t s, Control* ctl);
+void launch_substep_begin(hipStream_t s, Control* ctl);
+void launch_fastest(hipStream_t s, const float4* V, uint32_t n, Control* ctl);
+void launch_bounds(hipStream_t s, const float4* P, uint32_t n, float radius, Control* ctl);
+void launch_grid_setup(hipStream_t s, Control* ctl, uint32_t n, uint64_t capacity);
+void launch_count(hipStream_t s, const float4* P, uint32_t n, float radius, uint32_t* counts, uint2* slot_rank,
+                  Control* ctl);
+void launch_scan(hipStream_t s, uint32_t* in, uint32_t* out, uint32_t* tile_sums, uint32_t* total,
+                 const uint32_t* len_dev, uint64_t len_max, int zero_input, const Control* ctl);
+void launch_scatter(hipStream_t s, const float4* P, uint32_t n, const uint2* slot_rank, const uint32_t* start,
+                    uint2* pairs, const Control* ctl);
+void launch_rank_gather(hipStream_t s, const float4* PA, const float4* VA, uint32_t n, float radius,
+                        const uint2* pairs, const uint32_t* start, float4* PB, float4* VB, const Control* ctl);
+void launch_list_count(hipStream_t s, const float4* lp, const StepParams& prm, uint32_t* counts, int kind);
+void launch_list_emit(hipStream_t s, const float4* lp, const StepParams& prm, const uint32_t* list_start,
+                      uint32_t* items, int kind);
+void launch_list_density(hipStream_t s, const float4* P, float4* V, const StepParams& prm, const uint32_t* fstart,
+                         const uint32_t* fitems, const uint32_t* wstart, const uint32_t* witems);
+
+// WCSPH (kernels_wcsph.hip)
+void launch_wcsph_density(hipStream_t s, const float4* P, float4* V, float* scaled, const StepParams& prm);
+void launch_wcsph_force(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
+                        const StepParams& prm, float4* PA, float4* VA, float* forces);
+void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
+                         const StepParams& prm, float4* PA, float4* VA);
+
+// DFSPH (kernels_dfsph.hip)
+void launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
+int solve_grid_limit();
+hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
+                              float* kappa, double* partial, uint32_t* pcount, int which, float tolerance,
+                              uint32_t min_it, uint32_t max_it, int grid);
+void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,
+                          float* forces);
+void launch_dfsph_advance(hipStream_t s, const float4* PB, const StepParams& prm, float4* PA, float4* VA,
+                          float4* VKEEP);
+void launch_dfsph_repair(hipStream_t s, const float4* PB, const float4* VKEEP, const StepParams& prm, float4* PA,
+                         float4* VA);
+
+}  // namespace sfl

@@ -1,0 +1,1149 @@
+// Host side of the device fluid world (see world.hpp). All scalar arithmetic
+// that feeds the kernels (kernel scale, particle mass, substep counts and
+// lengths, viscosities) is the reference's f32 arithmetic in its order; this
+// file is compiled with -ffp-contract=off like the kernels.
+#include "world.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <tuple>
+
+#include "launch.hpp"
+
+namespace sfl {
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;  // std::f32::consts::PI
+constexpr uint64_t kMaxCells = uint64_t(1) << 31;
+constexpr uint64_t kMaxWallCells = uint64_t(1) << 30;
+
+// Kernel::value (src/fluid/mod.rs:140-146) on the host.
+float kernel_value_host(float distance, float radius, float scale) {
+  float q = distance / radius;
+  if (q >= 1.0f) return 0.0f;
+  float t = 1.0f - q;
+  float t2 = t * t;
+  float t4 = t2 * t2;
+  return scale * t4 * (1.0f + 4.0f * q);
+}
+
+float length3(float x, float y, float z) { return std::sqrt(x * x + y * y + z * z); }
+
+float fmax_rust(float a, float b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a > b ? a : b)); }
+float fmin_rust(float a, float b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a < b ? a : b)); }
+
+int32_t sat_i32(float v) {
+  if (std::isnan(v)) return 0;
+  if (v >= 2147483648.0f) return std::numeric_limits<int32_t>::max();
+  if (v <= -2147483648.0f) return std::numeric_limits<int32_t>::min();
+  return (int32_t)v;
+}
+
+int32_t cell_coord_host(float v, float radius) { return sat_i32(std::floor(v / radius)); }
+
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+bool finite_vec(sfl_vec3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+
+// FluidSettings::validate (src/fluid/mod.rs:98-117) plus the extensions.
+bool settings_valid(const sfl_settings& s) {
+  auto positive = [](float v) { return std::isfinite(v) && v > 0.0f; };
+  bool support = s.support >= 1.5f && s.support <= 4.0f;
+  return (s.solver == SFL_WCSPH || s.solver == SFL_DFSPH) && positive(s.spacing) && positive(s.rest_density) &&
+         support && s.max_particles > 0 && positive(s.sound_speed) && std::isfinite(s.viscosity) &&
+         s.viscosity >= 0.0f && finite_vec(s.gravity) && s.courant > 0.0f && s.courant <= 1.0f &&
+         s.max_substeps > 0 && std::isfinite(s.divergence_tolerance) && std::isfinite(s.density_tolerance) &&
+         s.max_iterations >= s.min_iterations;
+}
+
+// check_box (src/fluid/boundary.rs:230-238)
+int32_t check_box(sfl_vec3 low, sfl_vec3 high, std::string& detail) {
+  if (!finite_vec(low) || !finite_vec(high)) {
+    detail = "fluid box";
+    return SFL_NON_FINITE;
+  }
+  if (high.x <= low.x || high.y <= low.y || high.z <= low.z) return SFL_OUT_OF_RANGE;
+  return SFL_OK;
+}
+
+struct DeviceGuard {
+  explicit DeviceGuard(int d) { (void)hipSetDevice(d); }
+};
+
+}  // namespace
+
+template <class T>
+hipError_t DeviceBuffer<T>::reserve(size_t n, size_t keep, hipStream_t s) {
+  if (n <= cap) return hipSuccess;
+  size_t want = std::max(n, cap + cap / 2);
+  T* fresh = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&fresh), want * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (keep && ptr) {
+    e = hipMemcpyAsync(fresh, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
+      (void)hipFree(fresh);
+      return e;
+    }
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(fresh);
+    return e;
+  }
+  release();
+  ptr = fresh;
+  cap = want;
+  return hipSuccess;
+}
+
+int32_t World::device_error(const char* what, hipError_t e) {
+  detail_ = std::string(what) + ": " + hipGetErrorString(e);
+  return SFL_DEVICE_ERROR;
+}
+
+#define SFL_HIP(call, what)                          \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return device_error(what, e_); \
+  } while (0)
+
+int32_t World::create(const sfl_settings& s, World** out, std::string& why) {
+  if (!settings_valid(s)) return SFL_INVALID_SETTINGS;
+  World* w = new World();
+  w->settings_ = s;
+  // radius (mod.rs:120-122), Kernel::new (mod.rs:133-138)
+  w->radius_ = s.spacing * s.support;
+  w->reach_ = w->radius_ * w->radius_;
+  float r = w->radius_;
+  float r3 = r * (r * r);
+  w->scale_ = 21.0f / (2.0f * kPi * r3);
+  // rest_mass (mod.rs:696-709)
+  int reach = (int)std::ceil(s.support);
+  float sum = 0.0f;
+  for (int z = -reach; z <= reach; ++z)
+    for (int y = -reach; y <= reach; ++y)
+      for (int x = -reach; x <= reach; ++x) {
+        float ox = (float)x * s.spacing, oy = (float)y * s.spacing, oz = (float)z * s.spacing;
+        sum += kernel_value_host(length3(ox, oy, oz), r, w->scale_);
+      }
+  w->mass_ = s.rest_density / sum;
+  int32_t st = w->init_device();
+  if (st != SFL_OK) {
+    why = w->detail_;
+    delete w;
+    return st;
+  }
+  *out = w;
+  return SFL_OK;
+}
+
+int32_t World::init_device() {
+  int count = 0;
+  SFL_HIP(hipGetDeviceCount(&count), "hipGetDeviceCount");
+  if (settings_.device < 0 || settings_.device >= count) {
+    detail_ = "no such HIP device";
+    return SFL_DEVICE_ERROR;
+  }
+  device_ = settings_.device;
+  SFL_HIP(hipSetDevice(device_), "hipSetDevice");
+  SFL_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  SFL_HIP(hipMalloc(reinterpret_cast<void**>(&ctl_), sizeof(Control)), "hipMalloc(control)");
+  SFL_HIP(hipMemset(ctl_, 0, sizeof(Control)), "hipMemset(control)");
+  SFL_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctl_host_), sizeof(Control), hipHostMallocDefault),
+          "hipHostMalloc(control)");
+  std::memset(ctl_host_, 0, sizeof(Control));
+  SFL_HIP(scan_total_.reserve(1, 0, stream_), "hipMalloc(scan total)");
+  solve_grid_ = solve_grid_limit();
+  if (solve_grid_ <= 0) {
+    detail_ = "occupancy query for the persistent solve failed";
+    return SFL_DEVICE_ERROR;
+  }
+  return SFL_OK;
+}
+
+World::~World() {
+  DeviceGuard g(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : free_events_) (void)hipEventDestroy(e);
+  if (ctl_) (void)hipFree(ctl_);
+  if (ctl_host_) (void)hipHostFree(ctl_host_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// ---- particles ---------------------------------------------------------------
+
+int32_t World::ensure_particles(size_t n) {
+  SFL_HIP(pa_.reserve(n, n_, stream_), "hipMalloc(positions)");
+  SFL_HIP(va_.reserve(n, n_, stream_), "hipMalloc(velocities)");
+  SFL_HIP(pb_.reserve(n, 0, stream_), "hipMalloc(sorted positions)");
+  SFL_HIP(vb_.reserve(n, 0, stream_), "hipMalloc(sorted velocities)");
+  SFL_HIP(aux_.reserve(n, 0, stream_), "hipMalloc(aux)");
+  SFL_HIP(kappa_.reserve(n, 0, stream_), "hipMalloc(kappa)");
+  SFL_HIP(slot_rank_.reserve(n, 0, stream_), "hipMalloc(slot ranks)");
+  SFL_HIP(pairs_.reserve(n, 0, stream_), "hipMalloc(pairs)");
+  size_t chunks = (n + 511) / 512;
+  SFL_HIP(partial_.reserve(std::max<size_t>(chunks, 1), 0, stream_), "hipMalloc(partials)");
+  SFL_HIP(pcount_.reserve(std::max<size_t>(chunks, 1), 0, stream_), "hipMalloc(partial counts)");
+  size_t tiles = (n + 1 + 2047) / 2048;
+  SFL_HIP(tile_sums_.reserve(std::max<size_t>(tiles, (cell_cap_ + 2047) / 2048 + 1), tile_sums_.cap ? 0 : 0, stream_),
+          "hipMalloc(tile sums)");
+  if (keep_forces_) SFL_HIP(forces_.reserve(3 * n, 0, stream_), "hipMalloc(forces)");
+  return SFL_OK;
+}
+
+// add_particle (src/fluid/mod.rs:412-426), in order, stopping at the first
+// refusal.
+int32_t World::add_particles(size_t count, const float* xyz, const float* vxyz, uint32_t* ids) {
+  DeviceGuard g(device_);
+  size_t ok = 0;
+  int32_t status = SFL_OK;
+  for (; ok < count; ++ok) {
+    const float* p = xyz + 3 * ok;
+    static const float zero[3] = {0.0f, 0.0f, 0.0f};
+    const float* v = vxyz ? vxyz + 3 * ok : zero;
+    if (!finite3(p) || !finite3(v)) {
+      detail_ = "fluid particle";
+      status = SFL_NON_FINITE;
+      break;
+    }
+    if (n_ + ok >= settings_.max_particles || next_id_ + ok >= 0xffffffffull) {
+      status = SFL_FLUID_FULL;
+      break;
+    }
+  }
+  if (ok == 0) return status;
+  int32_t st = ensure_particles(n_ + ok);
+  if (st != SFL_OK) return st;
+  std::vector<float4> hp(ok), hv(ok);
+  for (size_t i = 0; i < ok; ++i) {
+    uint32_t id = next_id_ + (uint32_t)i;
+    float idf;
+    std::memcpy(&idf, &id, 4);
+    hp[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idf);
+    float vx = vxyz ? vxyz[3 * i] : 0.0f, vy = vxyz ? vxyz[3 * i + 1] : 0.0f, vz = vxyz ? vxyz[3 * i + 2] : 0.0f;
+    hv[i] = make_float4(vx, vy, vz, 0.0f);
+    if (ids) ids[i] = id;
+    if (vmax_valid_) vmax_ = fmax_rust(vmax_, length3(vx, vy, vz));
+  }
+  SFL_HIP(hipMemcpyAsync(pa_.ptr + n_, hp.data(), ok * sizeof(float4), hipMemcpyHostToDevice, stream_),
+          "upload positions");
+  SFL_HIP(hipMemcpyAsync(va_.ptr + n_, hv.data(), ok * sizeof(float4), hipMemcpyHostToDevice, stream_),
+          "upload velocities");
+  SFL_HIP(hipStreamSynchronize(stream_), "upload");
+  n_ += ok;
+  next_id_ += (uint32_t)ok;
+  invalidate_lists();
+  forces_valid_ = false;
+  return status;
+}
+
+// fill_box (src/fluid/mod.rs:493-524), optionally jittered (see
+// sfl_fill_box_jittered).
+int32_t World::fill_box(sfl_vec3 low, sfl_vec3 high, float jitter, uint32_t seed, size_t* count_out) {
+  if (!finite_vec(low) || !finite_vec(high) || !std::isfinite(jitter)) {
+    detail_ = "fluid box";
+    return SFL_NON_FINITE;
+  }
+  if (high.x < low.x || high.y < low.y || high.z < low.z) return SFL_OUT_OF_RANGE;
+  float spacing = settings_.spacing;
+  auto count = [&](float extent) -> uint64_t {
+    float c = std::min(std::floor(extent / spacing), 4294967295.0f);
+    return c <= 0.0f ? 0 : (uint64_t)c;
+  };
+  uint64_t nx = count(high.x - low.x), ny = count(high.y - low.y), nz = count(high.z - low.z);
+  unsigned __int128 t = (unsigned __int128)nx * ny * nz;
+  uint64_t total = t > UINT64_MAX ? UINT64_MAX : (uint64_t)t;
+  uint64_t room = settings_.max_particles > n_ ? settings_.max_particles - n_ : 0;
+  uint64_t ids_left = 0xffffffffull - next_id_;
+  if (total > room || total > ids_left) return SFL_FLUID_FULL;
+  float half = spacing * 0.5f;
+  float sx = low.x + half, sy = low.y + half, sz = low.z + half;
+  uint32_t state = seed ? seed : 0x9E3779B9u;
+  float amp = 2.0f * jitter * spacing;
+  auto draw = [&]() {
+    state ^= state << 13;
+    state ^= state >> 17;
+    state ^= state << 5;
+    return ((float)state / 4294967295.0f - 0.5f) * amp;
+  };
+  std::vector<float> pts;
+  pts.reserve((size_t)total * 3);
+  for (uint64_t y = 0; y < ny; ++y)
+    for (uint64_t z = 0; z < nz; ++z)
+      for (uint64_t x = 0; x < nx; ++x) {
+        float px = sx + (float)x * spacing, py = sy + (float)y * spacing, pz = sz + (float)z * spacing;
+        if (jitter != 0.0f) {
+          float jx = draw(), jy = draw(), jz = draw();
+          px = px + jx;
+          py = py + jy;
+          pz = pz + jz;
+        }
+        pts.push_back(px);
+        pts.push_back(py);
+        pts.push_back(pz);
+      }
+  int32_t st = total ? add_particles((size_t)total, pts.data(), nullptr, nullptr) : SFL_OK;
+  if (st == SFL_OK && count_out) *count_out = (size_t)total;
+  return st;
+}
+
+// set_domain (src/fluid/mod.rs:433-444): validated, but a box is refused as
+// unsupported on the device path.
+int32_t World::set_domain(const sfl_vec3* low, const sfl_vec3* high) {
+  if (!low && !high) return SFL_OK;
+  if (!low || !high) return SFL_OUT_OF_RANGE;
+  if (!finite_vec(*low) || !finite_vec(*high)) {
+    detail_ = "fluid domain";
+    return SFL_NON_FINITE;
+  }
+  if (high->x <= low->x || high->y <= low->y || high->z <= low->z) return SFL_OUT_OF_RANGE;
+  detail_ = "domain culling is not on the device path";
+  return SFL_UNSUPPORTED;
+}
+
+// ---- walls (src/fluid/boundary.rs) ------------------------------------------
+
+int32_t World::add_box_container(sfl_vec3 low, sfl_vec3 high, uint32_t* id_out) {
+  DeviceGuard g(device_);
+  int32_t st = check_box(low, high, detail_);
+  if (st != SFL_OK) return st;
+  // cells_in (boundary.rs:122-126)
+  auto cells = [&](float length) -> int64_t {
+    float c = std::min(std::round(length / settings_.spacing), 1e9f);
+    return (int64_t)c;
+  };
+  int64_t c[3] = {cells(high.x - low.x), cells(high.y - low.y), cells(high.z - low.z)};
+  int64_t layers = (int64_t)std::ceil(settings_.support);
+  int64_t shell = (c[0] + 2 * layers) * (c[1] + 2 * layers) * (c[2] + 2 * layers) - c[0] * c[1] * c[2];
+  uint64_t have = wall_owner_.size();
+  uint64_t room = settings_.max_particles > have ? settings_.max_particles - have : 0;
+  if (shell < 0 || (uint64_t)shell > room || wall_next_ == 0xffffffffu) return SFL_FLUID_FULL;
+  float spacing = settings_.spacing;
+  std::vector<float> pts;
+  pts.reserve((size_t)shell * 3);
+  for (int64_t y = -layers; y < c[1] + layers; ++y)
+    for (int64_t z = -layers; z < c[2] + layers; ++z)
+      for (int64_t x = -layers; x < c[0] + layers; ++x) {
+        bool inside = x >= 0 && x < c[0] && y >= 0 && y < c[1] && z >= 0 && z < c[2];
+        if (inside) continue;
+        float ax = (float)x + 0.5f, ay = (float)y + 0.5f, az = (float)z + 0.5f;
+        pts.push_back(low.x + ax * spacing);
+        pts.push_back(low.y + ay * spacing);
+        pts.push_back(low.z + az * spacing);
+      }
+  uint32_t id = 0;
+  Container box{wall_next_, {low.x, low.y, low.z}, {high.x, high.y, high.z}};
+  containers_.push_back(box);
+  st = add_wall_points(pts, &id);
+  if (st != SFL_OK) {
+    containers_.pop_back();
+    return st;
+  }
+  if (id_out) *id_out = id;
+  return SFL_OK;
+}
+
+int32_t World::add_wall(sfl_vec3 low, sfl_vec3 high, uint32_t* id_out) {
+  DeviceGuard g(device_);
+  int32_t st = check_box(low, high, detail_);
+  if (st != SFL_OK) return st;
+  auto cells = [&](float length) -> int64_t {
+    float c = std::min(std::round(length / settings_.spacing), 1e9f);
+    return std::max<int64_t>((int64_t)c, 1);
+  };
+  int64_t c[3] = {cells(high.x - low.x), cells(high.y - low.y), cells(high.z - low.z)};
+  int64_t count = c[0] * c[1] * c[2];
+  uint64_t have = wall_owner_.size();
+  uint64_t room = settings_.max_particles > have ? settings_.max_particles - have : 0;
+  if (count < 0 || (uint64_t)count > room || wall_next_ == 0xffffffffu) return SFL_FLUID_FULL;
+  float spacing = settings_.spacing;
+  std::vector<float> pts;
+  pts.reserve((size_t)count * 3);
+  for (int64_t y = 0; y < c[1]; ++y)
+    for (int64_t z = 0; z < c[2]; ++z)
+      for (int64_t x = 0; x < c[0]; ++x) {
+        float ax = (float)x + 0.5f, ay = (float)y + 0.5f, az = (float)z + 0.5f;
+        pts.push_back(low.x + ax * spacing);
+        pts.push_back(low.y + ay * spacing);
+        pts.push_back(low.z + az * spacing);
+      }
+  return add_wall_points(pts, id_out);
+}
+
+int32_t World::add_boundary_points(size_t count, const float* xyz, uint32_t* id_out) {
+  DeviceGuard g(device_);
+  for (size_t i = 0; i < count; ++i)
+    if (!finite3(xyz + 3 * i)) {
+      detail_ = "fluid wall";
+      return SFL_NON_FINITE;
+    }
+  uint64_t have = wall_owner_.size();
+  uint64_t room = settings_.max_particles > have ? settings_.max_particles - have : 0;
+  if (count > room || wall_next_ == 0xffffffffu) return SFL_FLUID_FULL;
+  std::vector<float> pts(xyz, xyz + 3 * count);
+  return add_wall_points(pts, id_out);
+}
+
+// add_wall_points (boundary.rs:139-151); the bounds follow the containers
+// (boundary.rs:240-247).
+int32_t World::add_wall_points(const std::vector<float>& pts, uint32_t* id_out) {
+  uint32_t id = wall_next_;
+  size_t before = wall_owner_.size();
+  wall_pos_.insert(wall_pos_.end(), pts.begin(), pts.end());
+  wall_owner_.insert(wall_owner_.end(), pts.size() / 3, id);
+  int32_t st = settle_walls();
+  if (st != SFL_OK) {
+    // undo: the settle failed before touching the device
+    std::vector<float> keep_pos;
+    std::vector<uint32_t> keep_owner;
+    for (size_t i = 0; i < wall_owner_.size(); ++i)
+      if (wall_owner_[i] != id) {
+        keep_pos.insert(keep_pos.end(), wall_pos_.begin() + 3 * i, wall_pos_.begin() + 3 * i + 3);
+        keep_owner.push_back(wall_owner_[i]);
+      }
+    wall_pos_.swap(keep_pos);
+    wall_owner_.swap(keep_owner);
+    (void)before;
+    (void)settle_walls();
+    return st;
+  }
+  wall_next_ += 1;
+  if (id_out) *id_out = id;
+  return SFL_OK;
+}
+
+// remove_boundary (boundary.rs:94-115), no terrains.
+int32_t World::remove_boundary(uint32_t id, int32_t* removed) {
+  DeviceGuard g(device_);
+  size_t before = wall_owner_.size();
+  std::vector<float> keep_pos;
+  std::vector<uint32_t> keep_owner;
+  for (size_t i = 0; i < wall_owner_.size(); ++i)
+    if (wall_owner_[i] != id) {
+      keep_pos.insert(keep_pos.end(), wall_pos_.begin() + 3 * i, wall_pos_.begin() + 3 * i + 3);
+      keep_owner.push_back(wall_owner_[i]);
+    }
+  bool had = false;
+  std::vector<Container> keep_c;
+  for (auto& c : containers_) {
+    if (c.id == id)
+      had = true;
+    else
+      keep_c.push_back(c);
+  }
+  bool gone = keep_owner.size() != before || had;
+  if (gone) {
+    wall_pos_.swap(keep_pos);
+    wall_owner_.swap(keep_owner);
+    containers_.swap(keep_c);
+    int32_t st = settle_walls();
+    if (st != SFL_OK) return st;
+  }
+  if (removed) *removed = gone ? 1 : 0;
+  return SFL_OK;
+}
+
+// settle_walls (boundary.rs:155-178): sort by (cell, owner, x/y/z bits,
+// index), then the dense cell table the kernels look walls up in, and the
+// container bounds.
+int32_t World::settle_walls() {
+  size_t n = wall_owner_.size();
+  struct Key {
+    int32_t c[3];
+    uint32_t owner, bx, by, bz;
+    size_t index;
+  };
+  std::vector<Key> order(n);
+  for (size_t i = 0; i < n; ++i) {
+    const float* p = wall_pos_.data() + 3 * i;
+    Key k;
+    for (int a = 0; a < 3; ++a) k.c[a] = cell_coord_host(p[a], radius_);
+    k.owner = wall_owner_[i];
+    std::memcpy(&k.bx, p, 4);
+    std::memcpy(&k.by, p + 1, 4);
+    std::memcpy(&k.bz, p + 2, 4);
+    k.index = i;
+    order[i] = k;
+  }
+  std::sort(order.begin(), order.end(), [](const Key& a, const Key& b) {
+    return std::tie(a.c[0], a.c[1], a.c[2], a.owner, a.bx, a.by, a.bz, a.index) <
+           std::tie(b.c[0], b.c[1], b.c[2], b.owner, b.bx, b.by, b.bz, b.index);
+  });
+  // table box
+  CellBox box{};
+  uint64_t volume = 0;
+  if (n) {
+    int32_t lo[3] = {order[0].c[0], order[0].c[1], order[0].c[2]};
+    int32_t hi[3] = {lo[0], lo[1], lo[2]};
+    for (auto& k : order)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], k.c[a]);
+        hi[a] = std::max(hi[a], k.c[a]);
+      }
+    volume = 1;
+    for (int a = 0; a < 3; ++a) {
+      uint64_t s = (uint64_t)((int64_t)hi[a] - (int64_t)lo[a] + 1);
+      box.lo[a] = lo[a];
+      box.span[a] = (uint32_t)std::min<uint64_t>(s, 0xffffffffu);
+      volume = (s > kMaxWallCells || volume * s > kMaxWallCells) ? kMaxWallCells + 1 : volume * s;
+    }
+    if (volume > kMaxWallCells) {
+      detail_ = "walls span more cells than the device wall table holds";
+      return SFL_UNSUPPORTED;
+    }
+  }
+  std::vector<float> pos(3 * n);
+  std::vector<uint32_t> own(n);
+  std::vector<float4> dev(n);
+  std::vector<uint32_t> start(volume + 1, 0);
+  for (size_t k = 0; k < n; ++k) {
+    const float* p = wall_pos_.data() + 3 * order[k].index;
+    pos[3 * k] = p[0];
+    pos[3 * k + 1] = p[1];
+    pos[3 * k + 2] = p[2];
+    own[k] = order[k].owner;
+    dev[k] = make_float4(p[0], p[1], p[2], 0.0f);
+    uint64_t slot = ((uint64_t)(order[k].c[0] - box.lo[0]) * box.span[1] + (uint64_t)(order[k].c[1] - box.lo[1])) *
+                        box.span[2] +
+                    (uint64_t)(order[k].c[2] - box.lo[2]);
+    start[slot + 1] += 1;
+  }
+  for (uint64_t c = 0; c < volume; ++c) start[c + 1] += start[c];
+  if (n) {
+    SFL_HIP(walls_d_.reserve(n, 0, stream_), "hipMalloc(walls)");
+    SFL_HIP(wall_start_d_.reserve(volume + 1, 0, stream_), "hipMalloc(wall table)");
+    SFL_HIP(hipMemcpyAsync(walls_d_.ptr, dev.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream_),
+            "upload walls");
+    SFL_HIP(hipMemcpyAsync(wall_start_d_.ptr, start.data(), (volume + 1) * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, stream_),
+            "upload wall table");
+    SFL_HIP(hipStreamSynchronize(stream_), "upload walls");
+  }
+  wall_pos_.swap(pos);
+  wall_owner_.swap(own);
+  wall_box_ = box;
+  wall_cells_ = (uint32_t)volume;
+  // union of the containers (boundary.rs:240-247)
+  has_bounds_ = !containers_.empty();
+  for (size_t i = 0; i < containers_.size(); ++i)
+    for (int a = 0; a < 3; ++a) {
+      blo_[a] = i == 0 ? containers_[i].lo[a] : fmin_rust(blo_[a], containers_[i].lo[a]);
+      bhi_[a] = i == 0 ? containers_[i].hi[a] : fmax_rust(bhi_[a], containers_[i].hi[a]);
+    }
+  invalidate_lists();
+  return SFL_OK;
+}
+
+int32_t World::read_walls(size_t cap, float* out) const {
+  size_t n = std::min(cap, wall_owner_.size());
+  std::copy(wall_pos_.begin(), wall_pos_.begin() + 3 * n, out);
+  return SFL_OK;
+}
+
+// ---- stepping ------------------------------------------------------------------
+
+StepParams World::params(float dt) const {
+  const sfl_settings& s = settings_;
+  StepParams p{};
+  p.n = (uint32_t)n_;
+  p.radius = radius_;
+  p.reach = reach_;
+  p.scale = scale_;
+  p.mass = mass_;
+  p.rest = s.rest_density;
+  float c = s.sound_speed;
+  p.stiffness = s.rest_density * (c * c) / 7.0f;  // step.rs:149
+  p.dt = dt;
+  float r = radius_;
+  p.nu = fmin_rust(s.viscosity * c * r, 0.05f * r * r / dt);         // step.rs:254-259
+  p.wall_nu = fmin_rust(1.0f * c * r, 0.25f * (r * r / dt));         // step.rs:266, 282-283
+  p.soften = 0.01f * r * r;                                          // step.rs:165
+  p.gx = s.gravity.x;
+  p.gy = s.gravity.y;
+  p.gz = s.gravity.z;
+  p.half_spacing = s.spacing / 2.0f;  // step.rs:308
+  p.has_bounds = has_bounds_ ? 1 : 0;
+  for (int a = 0; a < 3; ++a) {
+    p.blo[a] = blo_[a];
+    p.bhi[a] = bhi_[a];
+  }
+  p.walls = walls_d_.ptr;
+  p.wall_start = wall_start_d_.ptr;
+  p.wall_box = wall_box_;
+  p.wall_cells = wall_owner_.empty() ? 0u : wall_cells_;
+  p.cell_start = cell_start_.ptr;
+  p.ctl = ctl_;
+  return p;
+}
+
+int32_t World::ensure_cells(uint64_t volume) {
+  if (volume <= cell_cap_ && counts_.ptr) return SFL_OK;
+  if (volume > kMaxCells) {
+    detail_ = "the fluid spread over more cells than the device cell table holds";
+    return SFL_OUT_OF_RANGE;
+  }
+  uint64_t cap = std::min<uint64_t>(volume + volume / 4 + 4096, kMaxCells);
+  counts_.release();
+  cell_start_.release();
+  SFL_HIP(hipStreamSynchronize(stream_), "sync");
+  SFL_HIP(counts_.reserve(cap, 0, stream_), "hipMalloc(cell counts)");
+  SFL_HIP(hipMemsetAsync(counts_.ptr, 0, cap * sizeof(uint32_t), stream_), "clear cell counts");
+  SFL_HIP(cell_start_.reserve(cap + 1, 0, stream_), "hipMalloc(cell table)");
+  size_t tiles = (cap + 2047) / 2048 + 1;
+  SFL_HIP(tile_sums_.reserve(std::max<size_t>(tiles, (n_ + 1 + 2047) / 2048), 0, stream_), "hipMalloc(tile sums)");
+  SFL_HIP(hipStreamSynchronize(stream_), "sync");
+  cell_cap_ = cap;
+  return SFL_OK;
+}
+
+// A first table size: the containers' cells with a margin, or a small one.
+int32_t World::ensure_cells_for_bounds() {
+  uint64_t want = std::max<uint64_t>(4096, 2 * (uint64_t)n_);
+  if (has_bounds_) {
+    uint64_t v = 1;
+    for (int a = 0; a < 3; ++a) {
+      int64_t lo = (int64_t)cell_coord_host(blo_[a], radius_) - 1;
+      int64_t hi = (int64_t)cell_coord_host(bhi_[a], radius_) + 1;
+      uint64_t s = (uint64_t)std::max<int64_t>(hi - lo + 1, 1);
+      v = std::min<uint64_t>(v * s, kMaxCells);
+    }
+    want = std::max(want, v);
+  }
+  return ensure_cells(std::min<uint64_t>(want, kMaxCells));
+}
+
+int32_t World::read_control() {
+  SFL_HIP(hipMemcpyAsync(ctl_host_, ctl_, offsetof(Control, residual_log), hipMemcpyDeviceToHost, stream_),
+          "read control");
+  SFL_HIP(hipStreamSynchronize(stream_), "step");
+  collect_profile();
+  return SFL_OK;
+}
+
+int32_t World::ensure_fastest() {
+  if (vmax_valid_) return SFL_OK;
+  launch_step_begin(stream_, ctl_);
+  SFL_HIP(hipMemsetAsync(&ctl_->vmax_bits, 0, sizeof(uint32_t), stream_), "clear vmax");
+  launch_fastest(stream_, va_.ptr, (uint32_t)n_, ctl_);
+  int32_t st = read_control();
+  if (st != SFL_OK) return st;
+  std::memcpy(&vmax_, &ctl_host_->vmax_bits, 4);
+  vmax_valid_ = true;
+  return SFL_OK;
+}
+
+void World::phase_begin(int phase) {
+  if (!profiling_) return;
+  hipEvent_t e;
+  if (free_events_.empty()) {
+    if (hipEventCreate(&e) != hipSuccess) return;
+  } else {
+    e = free_events_.back();
+    free_events_.pop_back();
+  }
+  (void)hipEventRecord(e, stream_);
+  open_ = e;
+  (void)phase;
+}
+
+void World::phase_end(int phase) {
+  if (!profiling_ || !open_) return;
+  hipEvent_t e;
+  if (free_events_.empty()) {
+    if (hipEventCreate(&e) != hipSuccess) return;
+  } else {
+    e = free_events_.back();
+    free_events_.pop_back();
+  }
+  (void)hipEventRecord(e, stream_);
+  pending_.push_back({phase, open_, e});
+  open_ = nullptr;
+}
+
+void World::collect_profile() {
+  for (auto& p : pending_) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      phase_ms_[p.phase] += ms;
+      phase_launches_[p.phase] += 1;
+    }
+    free_events_.push_back(p.a);
+    free_events_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
+// update_neighbors' sort, gathers and grid build (src/fluid/mod.rs:540-566)
+// from A into B.
+void World::enqueue_sort() {
+  uint32_t n = (uint32_t)n_;
+  launch_substep_begin(stream_, ctl_);
+  phase_begin(SFL_PHASE_BOUNDS);
+  launch_bounds(stream_, pa_.ptr, n, radius_, ctl_);
+  launch_grid_setup(stream_, ctl_, n, cell_cap_);
+  phase_end(SFL_PHASE_BOUNDS);
+  phase_begin(SFL_PHASE_SORT);
+  launch_count(stream_, pa_.ptr, n, radius_, counts_.ptr, slot_rank_.ptr, ctl_);
+  launch_scan(stream_, counts_.ptr, cell_start_.ptr, tile_sums_.ptr, scan_total_.ptr, &ctl_->volume, cell_cap_, 1,
+              ctl_);
+  launch_scatter(stream_, pa_.ptr, n, slot_rank_.ptr, cell_start_.ptr, pairs_.ptr, ctl_);
+  phase_end(SFL_PHASE_SORT);
+  phase_begin(SFL_PHASE_REORDER);
+  launch_rank_gather(stream_, pa_.ptr, va_.ptr, n, radius_, pairs_.ptr, cell_start_.ptr, pb_.ptr, vb_.ptr, ctl_);
+  phase_end(SFL_PHASE_REORDER);
+}
+
+// wcsph_substep (src/fluid/step.rs:144-224)
+void World::enqueue_wcsph_substep(const StepParams& prm) {
+  enqueue_sort();
+  phase_begin(SFL_PHASE_DENSITY);
+  launch_wcsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm);
+  phase_end(SFL_PHASE_DENSITY);
+  phase_begin(SFL_PHASE_FORCE);
+  launch_wcsph_force(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm, pa_.ptr, va_.ptr,
+                     keep_forces_ ? forces_.ptr : nullptr);
+  phase_end(SFL_PHASE_FORCE);
+}
+
+// dfsph_substep (src/fluid/dfsph.rs:79-103)
+int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
+  const sfl_settings& s = settings_;
+  enqueue_sort();
+  phase_begin(SFL_PHASE_DENSITY);
+  launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm);
+  phase_end(SFL_PHASE_DENSITY);
+  phase_begin(SFL_PHASE_SOLVE);
+  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, partial_.ptr, pcount_.ptr, 0,
+                             s.divergence_tolerance, s.min_iterations, s.max_iterations, solve_grid_),
+          "cooperative launch (divergence solve)");
+  phase_end(SFL_PHASE_SOLVE);
+  phase_begin(SFL_PHASE_VISCOUS);
+  launch_dfsph_viscous(stream_, pb_.ptr, vb_.ptr, prm, va_.ptr, keep_forces_ ? forces_.ptr : nullptr);
+  phase_end(SFL_PHASE_VISCOUS);
+  phase_begin(SFL_PHASE_SOLVE);
+  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, va_.ptr, aux_.ptr, kappa_.ptr, partial_.ptr, pcount_.ptr, 1,
+                             s.density_tolerance, s.min_iterations, s.max_iterations, solve_grid_),
+          "cooperative launch (density solve)");
+  phase_end(SFL_PHASE_SOLVE);
+  phase_begin(SFL_PHASE_ADVANCE);
+  launch_dfsph_advance(stream_, pb_.ptr, prm, pa_.ptr, va_.ptr, vb_.ptr);
+  phase_end(SFL_PHASE_ADVANCE);
+  return SFL_OK;
+}
+
+int32_t World::step(float dt, sfl_report* out) {
+  DeviceGuard g(device_);
+  sfl_report r{};
+  if (!std::isfinite(dt) || dt < 0.0f) {
+    detail_ = "fluid step";
+    return SFL_NON_FINITE;
+  }
+  if (dt == 0.0f) {
+    if (out) *out = r;
+    return SFL_OK;
+  }
+  if (n_ > 0) {
+    if (keep_forces_) {
+      int32_t st = ensure_particles(n_);
+      if (st != SFL_OK) return st;
+    }
+    int32_t st = settings_.solver == SFL_WCSPH ? wcsph(dt, r) : dfsph(dt, r);
+    if (st != SFL_OK) return st;
+    r.max_speed = vmax_;
+    float comp;
+    std::memcpy(&comp, &ctl_host_->comp_bits, 4);
+    r.max_compression = comp;
+  }
+  if (out) *out = r;
+  return SFL_OK;
+}
+
+// wcsph / wcsph_steps (src/fluid/step.rs:102-142); no coupled bodies, so the
+// sound speed is the configured one.
+int32_t World::wcsph(float dt, sfl_report& r) {
+  int32_t st = ensure_fastest();
+  if (st != SFL_OK) return st;
+  const sfl_settings& s = settings_;
+  r.sound_speed = s.sound_speed;
+  float largest = s.courant * radius_ / (s.sound_speed + vmax_);
+  float wanted = fmax_rust(std::ceil(dt / largest), 1.0f);
+  float cap = (float)s.max_substeps;
+  float take = fmin_rust(wanted, cap);
+  uint32_t substeps = take >= 4294967295.0f ? 0xffffffffu : (uint32_t)take;
+  r.capped = wanted > cap ? 1u : 0u;
+  float small = dt / (float)substeps;
+  st = ensure_cells_for_bounds();
+  if (st != SFL_OK) return st;
+  StepParams prm = params(small);
+  uint32_t done = 0, held = 0;
+  lists_valid_ = false;
+  while (done < substeps) {
+    launch_step_begin(stream_, ctl_);
+    for (uint32_t k = done; k < substeps; ++k) enqueue_wcsph_substep(prm);
+    st = read_control();
+    if (st != SFL_OK) return st;
+    const Control& c = *ctl_host_;
+    if (c.flags & kFlagOverflow) {
+      uint32_t completed = c.substeps_done ? c.substeps_done - 1 : 0;
+      held += c.held;
+      done += completed;
+      work_substeps_ += (uint64_t)completed * n_;
+      if (completed > 0) {
+        lists_valid_ = true;
+        lists_on_b_ = true;
+        lists_emitted_ = false;
+      }
+      st = ensure_cells(c.need_volume);
+      if (st != SFL_OK) return st;
+      prm = params(small);
+      continue;
+    }
+    lists_valid_ = true;
+    lists_on_b_ = true;
+    lists_emitted_ = false;
+    if (c.flags & kFlagNonFinite) {
+      launch_wcsph_repair(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm, pa_.ptr, va_.ptr);
+      SFL_HIP(hipStreamSynchronize(stream_), "repair");
+      vmax_valid_ = false;
+      detail_ = "fluid particle";
+      return SFL_NON_FINITE;
+    }
+    held += c.held;
+    work_substeps_ += (uint64_t)(substeps - done) * n_;
+    done = substeps;
+  }
+  std::memcpy(&vmax_, &ctl_host_->vmax_bits, 4);
+  vmax_valid_ = true;
+  forces_valid_ = keep_forces_;
+  r.substeps = substeps;
+  r.held_back = held;
+  return SFL_OK;
+}
+
+// dfsph (src/fluid/dfsph.rs:46-77)
+int32_t World::dfsph(float dt, sfl_report& r) {
+  int32_t st = ensure_fastest();
+  if (st != SFL_OK) return st;
+  const sfl_settings& s = settings_;
+  float spacing = s.spacing, courant = s.courant;
+  float pull = length3(s.gravity.x, s.gravity.y, s.gravity.z);
+  st = ensure_cells_for_bounds();
+  if (st != SFL_OK) return st;
+  float left = dt;
+  lists_valid_ = false;
+  while (left > 0.0f) {
+    float small = courant * spacing / fmax_rust(vmax_, 1e-6f);
+    if (pull > 0.0f) small = fmin_rust(small, std::sqrt(2.0f * courant * spacing / pull));
+    if (r.substeps + 1 >= s.max_substeps && small < left) {
+      r.capped = 1;
+      small = left;
+    }
+    small = fmin_rust(small, left);
+    StepParams prm = params(small);
+    for (;;) {
+      launch_step_begin(stream_, ctl_);
+      st = enqueue_dfsph_substep(prm);
+      if (st != SFL_OK) return st;
+      st = read_control();
+      if (st != SFL_OK) return st;
+      if (ctl_host_->flags & kFlagOverflow) {
+        st = ensure_cells(ctl_host_->need_volume);
+        if (st != SFL_OK) return st;
+        prm = params(small);
+        continue;
+      }
+      break;
+    }
+    const Control& c = *ctl_host_;
+    lists_valid_ = true;
+    lists_on_b_ = true;
+    lists_emitted_ = false;
+    if (c.flags & kFlagBarrier) {
+      detail_ = "a grid barrier of the persistent solve timed out";
+      return SFL_DEVICE_ERROR;
+    }
+    if (c.flags & kFlagNonFinite) {
+      launch_dfsph_repair(stream_, pb_.ptr, vb_.ptr, prm, pa_.ptr, va_.ptr);
+      SFL_HIP(hipStreamSynchronize(stream_), "repair");
+      vmax_valid_ = false;
+      detail_ = "fluid particle";
+      return SFL_NON_FINITE;
+    }
+    r.held_back += c.held;
+    r.iterations += c.iterations[0] + c.iterations[1];
+    r.iterations_divergence += c.iterations[0];
+    r.iterations_density += c.iterations[1];
+    r.substeps += 1;
+    work_substeps_ += n_;
+    work_pushes_ += (uint64_t)(c.iterations[0] + c.iterations[1]) * n_;
+    work_rates_ += (uint64_t)c.rates_step * n_;
+    left -= small;
+    std::memcpy(&vmax_, &c.vmax_bits, 4);
+  }
+  vmax_valid_ = true;
+  forces_valid_ = keep_forces_;
+  return SFL_OK;
+}
+
+// ---- neighbour build and densities on their own ---------------------------
+
+int32_t World::update_neighbors() {
+  DeviceGuard g(device_);
+  if (n_ == 0) {
+    lists_valid_ = true;
+    lists_on_b_ = false;
+    lists_emitted_ = false;
+    return SFL_OK;
+  }
+  int32_t st = ensure_cells_for_bounds();
+  if (st != SFL_OK) return st;
+  for (;;) {
+    launch_step_begin(stream_, ctl_);
+    enqueue_sort();
+    st = read_control();
+    if (st != SFL_OK) return st;
+    if (ctl_host_->flags & kFlagOverflow) {
+      st = ensure_cells(ctl_host_->need_volume);
+      if (st != SFL_OK) return st;
+      continue;
+    }
+    break;
+  }
+  pa_.swap(pb_);
+  va_.swap(vb_);
+  lists_valid_ = true;
+  lists_on_b_ = false;
+  lists_emitted_ = false;
+  forces_valid_ = false;
+  return SFL_OK;
+}
+
+int32_t World::emit_lists() {
+  if (lists_emitted_) return SFL_OK;
+  size_t n = n_;
+  SFL_HIP(fstart_.reserve(n + 1, 0, stream_), "hipMalloc(list starts)");
+  SFL_HIP(wstart_.reserve(n + 1, 0, stream_), "hipMalloc(wall list starts)");
+  SFL_HIP(list_counts_.reserve(std::max<size_t>(n, 1), 0, stream_), "hipMalloc(list counts)");
+  SFL_HIP(tile_sums_.reserve((n + 1 + 2047) / 2048 + 1, 0, stream_), "hipMalloc(tile sums)");
+  if (n == 0) {
+    uint32_t zero = 0;
+    SFL_HIP(hipMemcpy(fstart_.ptr, &zero, 4, hipMemcpyHostToDevice), "list starts");
+    SFL_HIP(hipMemcpy(wstart_.ptr, &zero, 4, hipMemcpyHostToDevice), "list starts");
+    fitems_len_ = witems_len_ = 0;
+    lists_emitted_ = true;
+    return SFL_OK;
+  }
+  const float4* lp = lists_on_b_ ? pb_.ptr : pa_.ptr;
+  StepParams prm = params(1.0f);
+  for (int kind = 0; kind < 2; ++kind) {
+    DeviceBuffer<uint32_t>& start = kind == 0 ? fstart_ : wstart_;
+    DeviceBuffer<uint32_t>& items = kind == 0 ? fitems_ : witems_;
+    launch_list_count(stream_, lp, prm, list_counts_.ptr, kind);
+    launch_scan(stream_, list_counts_.ptr, start.ptr, tile_sums_.ptr, scan_total_.ptr, nullptr, n, 0, nullptr);
+    uint32_t total = 0;
+    SFL_HIP(hipMemcpyAsync(&total, scan_total_.ptr, 4, hipMemcpyDeviceToHost, stream_), "list total");
+    SFL_HIP(hipStreamSynchronize(stream_), "list count");
+    SFL_HIP(items.reserve(std::max<uint32_t>(total, 1), 0, stream_), "hipMalloc(list items)");
+    launch_list_emit(stream_, lp, prm, start.ptr, items.ptr, kind);
+    (kind == 0 ? fitems_len_ : witems_len_) = total;
+  }
+  SFL_HIP(hipStreamSynchronize(stream_), "list emit");
+  lists_emitted_ = true;
+  return SFL_OK;
+}
+
+// compute_densities (src/fluid/mod.rs:621-675) over the lists of the last
+// build with the current positions.
+int32_t World::compute_densities() {
+  DeviceGuard g(device_);
+  if (!lists_valid_) return SFL_OUT_OF_RANGE;
+  int32_t st = emit_lists();
+  if (st != SFL_OK) return st;
+  StepParams prm = params(1.0f);
+  launch_list_density(stream_, pa_.ptr, va_.ptr, prm, fstart_.ptr, fitems_.ptr, wstart_.ptr, witems_.ptr);
+  SFL_HIP(hipStreamSynchronize(stream_), "densities");
+  return SFL_OK;
+}
+
+// ---- readback -----------------------------------------------------------------
+
+int32_t World::read_ids(size_t cap, uint32_t* out) {
+  DeviceGuard g(device_);
+  size_t n = std::min(cap, n_);
+  if (!n) return SFL_OK;
+  std::vector<float4> h(n);
+  SFL_HIP(hipMemcpyAsync(h.data(), pa_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read ids");
+  SFL_HIP(hipStreamSynchronize(stream_), "read ids");
+  for (size_t i = 0; i < n; ++i) std::memcpy(out + i, &h[i].w, 4);
+  return SFL_OK;
+}
+
+int32_t World::read_positions(size_t cap, float* out) {
+  DeviceGuard g(device_);
+  size_t n = std::min(cap, n_);
+  if (!n) return SFL_OK;
+  std::vector<float4> h(n);
+  SFL_HIP(hipMemcpyAsync(h.data(), pa_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read positions");
+  SFL_HIP(hipStreamSynchronize(stream_), "read positions");
+  for (size_t i = 0; i < n; ++i) {
+    out[3 * i] = h[i].x;
+    out[3 * i + 1] = h[i].y;
+    out[3 * i + 2] = h[i].z;
+  }
+  return SFL_OK;
+}
+
+int32_t World::read_velocities(size_t cap, float* out) {
+  DeviceGuard g(device_);
+  size_t n = std::min(cap, n_);
+  if (!n) return SFL_OK;
+  std::vector<float4> h(n);
+  SFL_HIP(hipMemcpyAsync(h.data(), va_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read velocities");
+  SFL_HIP(hipStreamSynchronize(stream_), "read velocities");
+  for (size_t i = 0; i < n; ++i) {
+    out[3 * i] = h[i].x;
+    out[3 * i + 1] = h[i].y;
+    out[3 * i + 2] = h[i].z;
+  }
+  return SFL_OK;
+}
+
+int32_t World::read_densities(size_t cap, float* out) {
+  DeviceGuard g(device_);
+  size_t n = std::min(cap, n_);
+  if (!n) return SFL_OK;
+  std::vector<float4> h(n);
+  SFL_HIP(hipMemcpyAsync(h.data(), va_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read densities");
+  SFL_HIP(hipStreamSynchronize(stream_), "read densities");
+  for (size_t i = 0; i < n; ++i) out[i] = h[i].w;
+  return SFL_OK;
+}
+
+int32_t World::keep_forces(bool keep) {
+  DeviceGuard g(device_);
+  keep_forces_ = keep;
+  forces_valid_ = false;
+  if (keep) {
+    SFL_HIP(forces_.reserve(std::max<size_t>(3 * n_, 3), 0, stream_), "hipMalloc(forces)");
+  }
+  return SFL_OK;
+}
+
+int32_t World::read_forces(size_t cap, float* out) {
+  DeviceGuard g(device_);
+  if (!forces_valid_) return SFL_OUT_OF_RANGE;
+  size_t n = std::min(cap, n_);
+  if (!n) return SFL_OK;
+  SFL_HIP(hipMemcpyAsync(out, forces_.ptr, 3 * n * sizeof(float), hipMemcpyDeviceToHost, stream_), "read forces");
+  SFL_HIP(hipStreamSynchronize(stream_), "read forces");
+  return SFL_OK;
+}
+
+int32_t World::read_neighbors(int kind, size_t start_cap, uint32_t* start, size_t items_cap, uint32_t* items,
+                              size_t* items_len) {
+  DeviceGuard g(device_);
+  if (!lists_valid_) return SFL_OUT_OF_RANGE;
+  int32_t st = emit_lists();
+  if (st != SFL_OK) return st;
+  size_t len = kind == 0 ? fitems_len_ : witems_len_;
+  if (items_len) *items_len = len;
+  if (start_cap < n_ + 1 || items_cap < len) return SFL_OUT_OF_RANGE;
+  const DeviceBuffer<uint32_t>& s = kind == 0 ? fstart_ : wstart_;
+  const DeviceBuffer<uint32_t>& it = kind == 0 ? fitems_ : witems_;
+  SFL_HIP(hipMemcpyAsync(start, s.ptr, (n_ + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "read lists");
+  if (len) SFL_HIP(hipMemcpyAsync(items, it.ptr, len * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "read lists");
+  SFL_HIP(hipStreamSynchronize(stream_), "read lists");
+  return SFL_OK;
+}
+
+// state_hash (src/fluid/mod.rs:678-691) with FNV-1a (src/snapshot/bytes.rs:6-13)
+int32_t World::state_hash(uint64_t* out) {
+  DeviceGuard g(device_);
+  size_t n = n_;
+  std::vector<float4> p(n), v(n);
+  if (n) {
+    SFL_HIP(hipMemcpyAsync(p.data(), pa_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read state");
+    SFL_HIP(hipMemcpyAsync(v.data(), va_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read state");
+    SFL_HIP(hipStreamSynchronize(stream_), "read state");
+  }
+  std::vector<std::pair<uint32_t, size_t>> order(n);
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t id;
+    std::memcpy(&id, &p[i].w, 4);
+    order[i] = {id, i};
+  }
+  std::sort(order.begin(), order.end());
+  uint64_t h = 0xcbf29ce484222325ull;
+  auto feed = [&](uint32_t w) {
+    for (int b = 0; b < 4; ++b) {
+      h ^= (uint64_t)((w >> (8 * b)) & 0xffu);
+      h *= 0x00000100000001b3ull;
+    }
+  };
+  for (auto& o : order) {
+    size_t i = o.second;
+    feed(o.first);
+    float vals[6] = {p[i].x, p[i].y, p[i].z, v[i].x, v[i].y, v[i].z};
+    for (float f : vals) {
+      uint32_t bits;
+      std::memcpy(&bits, &f, 4);
+      feed(bits);
+    }
+  }
+  *out = h;
+  return SFL_OK;
+}
+
+int32_t World::synchronize() {
+  DeviceGuard g(device_);
+  SFL_HIP(hipStreamSynchronize(stream_), "synchronize");
+  collect_profile();
+  return SFL_OK;
+}
+
+int32_t World::profile(bool enable) {
+  DeviceGuard g(device_);
+  SFL_HIP(hipStreamSynchronize(stream_), "synchronize");
+  collect_profile();
+  profiling_ = enable;
+  return SFL_OK;
+}
+
+int32_t World::profile_read(double* ms, uint64_t* launches, bool reset) {
+  DeviceGuard g(device_);
+  SFL_HIP(hipStreamSynchronize(stream_), "synchronize");
+  collect_profile();
+  for (int p = 0; p < SFL_PHASE_COUNT; ++p) {
+    if (ms) ms[p] = phase_ms_[p];
+    if (launches) launches[p] = phase_launches_[p];
+    if (reset) {
+      phase_ms_[p] = 0;
+      phase_launches_[p] = 0;
+    }
+  }
+  return SFL_OK;
+}
+
+int32_t World::work_read(uint64_t* substeps, uint64_t* pushes, uint64_t* rates, bool reset) {
+  if (substeps) *substeps = work_substeps_;
+  if (pushes) *pushes = work_pushes_;
+  if (rates) *rates = work_rates_;
+  if (reset) work_substeps_ = work_pushes_ = work_rates_ = 0;
+  return SFL_OK;
+}
+
+template struct DeviceBuffer<float4>;
+template struct DeviceBuffer<float>;
+template struct DeviceBuffer<uint2>;
+template struct DeviceBuffer<uint32_t>;
+template struct DeviceBuffer<double>;
+
+}  // namespace sfl

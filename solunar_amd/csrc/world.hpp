@@ -1,0 +1,166 @@
+// Host side of the device fluid world: the C++ counterpart of the
+// reference's FluidWorld (src/fluid/mod.rs:310-339) whose particle state
+// lives in HBM. It validates input, keeps the walls, sizes device buffers,
+// orders the launches of each step on the world's stream, and turns the
+// control block into FluidReport fields and PhysicsError codes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/solunar_fluid.h"
+#include "device.hpp"
+
+namespace sfl {
+
+template <class T>
+struct DeviceBuffer {
+  T* ptr = nullptr;
+  size_t cap = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+  }
+  // Grows to at least `n` elements; keeps the first `keep` elements.
+  hipError_t reserve(size_t n, size_t keep, hipStream_t s);
+  void swap(DeviceBuffer& o) {
+    std::swap(ptr, o.ptr);
+    std::swap(cap, o.cap);
+  }
+};
+
+class World {
+ public:
+  static int32_t create(const sfl_settings& s, World** out, std::string& why);
+  ~World();
+
+  int32_t add_particles(size_t count, const float* xyz, const float* vxyz, uint32_t* ids);
+  int32_t fill_box(sfl_vec3 low, sfl_vec3 high, float jitter, uint32_t seed, size_t* count);
+  int32_t set_domain(const sfl_vec3* low, const sfl_vec3* high);
+  int32_t add_box_container(sfl_vec3 low, sfl_vec3 high, uint32_t* id);
+  int32_t add_wall(sfl_vec3 low, sfl_vec3 high, uint32_t* id);
+  int32_t add_boundary_points(size_t count, const float* xyz, uint32_t* id);
+  int32_t remove_boundary(uint32_t id, int32_t* removed);
+
+  int32_t step(float dt, sfl_report* report);
+  int32_t update_neighbors();
+  int32_t compute_densities();
+
+  int32_t read_ids(size_t cap, uint32_t* out);
+  int32_t read_positions(size_t cap, float* out);
+  int32_t read_velocities(size_t cap, float* out);
+  int32_t read_densities(size_t cap, float* out);
+  int32_t read_forces(size_t cap, float* out);
+  int32_t read_neighbors(int kind, size_t start_cap, uint32_t* start, size_t items_cap, uint32_t* items,
+                         size_t* items_len);
+  int32_t read_walls(size_t cap, float* out) const;
+  int32_t state_hash(uint64_t* out);
+  int32_t keep_forces(bool keep);
+  int32_t synchronize();
+
+  int32_t profile(bool enable);
+  int32_t profile_read(double* ms, uint64_t* launches, bool reset);
+  int32_t work_read(uint64_t* substeps, uint64_t* pushes, uint64_t* rates, bool reset);
+
+  const sfl_settings& settings() const { return settings_; }
+  float mass() const { return mass_; }
+  size_t len() const { return n_; }
+  size_t wall_len() const { return wall_owner_.size(); }
+  const char* detail() const { return detail_.c_str(); }
+  void set_detail(const std::string& d) { detail_ = d; }
+
+ private:
+  World() = default;
+  int32_t device_error(const char* what, hipError_t e);
+  int32_t init_device();
+  StepParams params(float dt) const;
+  int32_t ensure_particles(size_t n);
+  int32_t ensure_cells(uint64_t volume);
+  int32_t ensure_cells_for_bounds();
+  int32_t read_control();
+  int32_t settle_walls();
+  int32_t wcsph(float dt, sfl_report& r);
+  int32_t dfsph(float dt, sfl_report& r);
+  void enqueue_sort();
+  void enqueue_wcsph_substep(const StepParams& prm);
+  int32_t enqueue_dfsph_substep(const StepParams& prm);
+  int32_t emit_lists();
+  int32_t ensure_fastest();
+  void invalidate_lists() {
+    lists_valid_ = false;
+    lists_emitted_ = false;
+  }
+  int32_t add_wall_points(const std::vector<float>& pts, uint32_t* id);
+  void phase_begin(int phase);
+  void phase_end(int phase);
+  void collect_profile();
+
+  sfl_settings settings_{};
+  float radius_ = 0, reach_ = 0, scale_ = 0, mass_ = 0;
+  size_t n_ = 0;
+  uint32_t next_id_ = 0;
+  float vmax_ = 0;
+  bool vmax_valid_ = true;
+  bool keep_forces_ = false;
+  bool forces_valid_ = false;
+  std::string detail_;
+
+  // walls, host copy in settled order (src/fluid/boundary.rs:23-33)
+  std::vector<float> wall_pos_;
+  std::vector<uint32_t> wall_owner_;
+  uint32_t wall_next_ = 0;
+  struct Container {
+    uint32_t id;
+    float lo[3], hi[3];
+  };
+  std::vector<Container> containers_;
+  bool has_bounds_ = false;
+  float blo_[3] = {0, 0, 0}, bhi_[3] = {0, 0, 0};
+
+  // device
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  Control* ctl_ = nullptr;       // device
+  Control* ctl_host_ = nullptr;  // pinned host mirror
+  int solve_grid_ = 0;
+  DeviceBuffer<float4> pa_, va_, pb_, vb_;
+  DeviceBuffer<float> aux_, kappa_, forces_;
+  DeviceBuffer<uint2> slot_rank_, pairs_;
+  DeviceBuffer<uint32_t> counts_, cell_start_, tile_sums_, scan_total_;
+  uint64_t cell_cap_ = 0;
+  DeviceBuffer<double> partial_;
+  DeviceBuffer<uint32_t> pcount_;
+  DeviceBuffer<float4> walls_d_;
+  DeviceBuffer<uint32_t> wall_start_d_;
+  CellBox wall_box_{};
+  uint32_t wall_cells_ = 0;
+
+  // neighbour lists of the last build (reference `neighbors`, `wall_neighbors`)
+  bool lists_valid_ = false;
+  bool lists_emitted_ = false;
+  bool lists_on_b_ = false;  // true: the build's positions are PB (after a step)
+  DeviceBuffer<uint32_t> fstart_, fitems_, wstart_, witems_, list_counts_;
+  size_t fitems_len_ = 0, witems_len_ = 0;
+
+  // measurement
+  bool profiling_ = false;
+  struct EventPair {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<EventPair> pending_;
+  std::vector<hipEvent_t> free_events_;
+  hipEvent_t open_ = nullptr;
+  double phase_ms_[SFL_PHASE_COUNT] = {};
+  uint64_t phase_launches_[SFL_PHASE_COUNT] = {};
+  uint64_t work_substeps_ = 0, work_pushes_ = 0, work_rates_ = 0;
+};
+
+}  // namespace sfl
