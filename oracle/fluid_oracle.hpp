@@ -52,6 +52,15 @@
 
 namespace oracle {
 
+// Planted mistakes, for tests that show the oracle's checks are not blind.
+// 0 = none; 1 = kernel value without its (1 + 4q) factor; 2 = pressure force
+// with the sign of the pair gradient flipped; 3 = neighbour cells visited dx
+// outer instead of dz outer.
+inline int& planted_mistake() {
+  static int m = 0;
+  return m;
+}
+
 // src/error.rs:5-22, offset by one (0 is success)
 enum Status : int32_t {
   kOk = 0,
@@ -149,6 +158,7 @@ struct Kernel {
     R t = R(1) - q;
     R t2 = t * t;
     R t4 = t2 * t2;  // powi(4)
+    if (planted_mistake() == 1) return scale * t4;
     return scale * t4 * (R(1) + R(4) * q);
   }
   R slope(R distance) const {
@@ -260,6 +270,15 @@ struct Grid {
   // Grid::around (mod.rs:250-261): dz outer, dy, dx inner.
   void around(CellKey c, std::vector<Range>& out) const {
     out.clear();
+    if (planted_mistake() == 3) {
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dz = -1; dz <= 1; ++dz) {
+            auto r = range({c.x + dx, c.y + dy, c.z + dz});
+            if (r) out.push_back(*r);
+          }
+      return;
+    }
     for (int dz = -1; dz <= 1; ++dz)
       for (int dy = -1; dy <= 1; ++dy)
         for (int dx = -1; dx <= 1; ++dx) {
@@ -787,6 +806,7 @@ class World {
           if (squared <= R(0)) continue;
           R distance = std::sqrt(squared);
           V gradient = offset * (kernel.slope(distance) / distance);
+          if (planted_mistake() == 2) gradient = -gradient;
           R toward = rmin(dot(velocity - velocities[other], offset), R(0));
           R damp = -nu * toward / (squared + soften) * R(2) / (density + densities[other]);
           push = push - gradient * (mass * (own + scaled[other] + damp));

@@ -1,0 +1,303 @@
+"""TEST INFRASTRUCTURE: ctypes loader of the CPU oracle (liboracle_fluid.so).
+
+`OracleWorld` has the same methods as `solunar_amd.fluid.FluidWorld`, so a
+test can run one scenario on both and compare. Only tests/, the smoke check
+and the CPU baseline leg of bench.py use it.
+
+Extras the reference's test dispatchers give: `threads` (jobs of 512
+particles on that many workers, run backwards like dispatch::testing::Threads,
+src/dispatch.rs:28-48), `skip_job` (dispatch::testing::SkipsJobs,
+src/dispatch.rs:50-65), `precision` (32 = reference arithmetic, 64 = FP64
+yardstick) and `residual_mode` (0 = the reference's f32 fold of DFSPH errors,
+1 = the device's fixed-shape FP64 sum).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+from solunar_amd.types import (CReport, CSettings, FluidReport, FluidSettings,
+                               PhysicsError)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_fluid.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def library() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    f = ctypes.c_float
+    sz = ctypes.c_size_t
+    i32 = ctypes.c_int32
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    box = [P, f, f, f, f, f, f]
+    sigs = {
+        "orc_settings_size": (sz, []),
+        "orc_report_size": (sz, []),
+        "orc_create": (i32, [ctypes.POINTER(CSettings), i32, ctypes.POINTER(P)]),
+        "orc_destroy": (None, [P]),
+        "orc_set_threads": (None, [P, i32]),
+        "orc_set_skip_job": (None, [P, ctypes.c_int64]),
+        "orc_set_residual_mode": (None, [P, i32]),
+        "orc_set_solve_log": (None, [P, i32]),
+        "orc_read_solve_log": (sz, [P, sz, ctypes.POINTER(i32), u32p, u32p, fp, ctypes.POINTER(ctypes.c_double)]),
+        "orc_len": (sz, [P]),
+        "orc_particle_mass": (ctypes.c_double, [P]),
+        "orc_kernel_value": (ctypes.c_double, [P, ctypes.c_double]),
+        "orc_kernel_slope": (ctypes.c_double, [P, ctypes.c_double]),
+        "orc_add_particles": (i32, [P, sz, fp, fp, u32p]),
+        "orc_fill_box": (i32, box + [ctypes.POINTER(sz)]),
+        "orc_fill_box_jittered": (i32, box + [f, ctypes.c_uint32, ctypes.POINTER(sz)]),
+        "orc_add_box_container": (i32, box + [u32p]),
+        "orc_add_wall": (i32, box + [u32p]),
+        "orc_add_boundary_points": (i32, [P, sz, fp, u32p]),
+        "orc_remove_boundary": (i32, [P, ctypes.c_uint32, ctypes.POINTER(i32)]),
+        "orc_wall_len": (sz, [P]),
+        "orc_read_wall_positions": (i32, [P, sz, fp]),
+        "orc_step": (i32, [P, f, ctypes.POINTER(CReport)]),
+        "orc_update_neighbors": (i32, [P]),
+        "orc_compute_densities": (i32, [P]),
+        "orc_read_ids": (i32, [P, sz, u32p]),
+        "orc_read_vectors": (i32, [P, i32, sz, ctypes.c_void_p]),
+        "orc_read_densities": (i32, [P, sz, ctypes.c_void_p]),
+        "orc_set_particles": (i32, [P, sz, u32p, fp, fp]),
+        "orc_read_neighbors": (i32, [P, i32, sz, u32p, sz, u32p, ctypes.POINTER(sz)]),
+        "orc_state_hash": (ctypes.c_uint64, [P]),
+        "orc_error_detail": (ctypes.c_char_p, [P]),
+        "orc_fnv1a": (ctypes.c_uint64, [ctypes.c_char_p, sz]),
+        "orc_fixed_tree_sum": (ctypes.c_double, [fp, sz]),
+        "orc_grid_selftest": (i32, []),
+        "orc_plant_mistake": (None, [i32]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    assert lib.orc_settings_size() == ctypes.sizeof(CSettings), "settings layout differs"
+    assert lib.orc_report_size() == ctypes.sizeof(CReport), "report layout differs"
+    _lib = lib
+    return lib
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1, 3))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class OracleWorld:
+    def __init__(self, settings: Optional[FluidSettings] = None, precision: int = 32, threads: int = 1,
+                 residual_mode: int = 0):
+        self._lib = library()
+        self._settings = settings or FluidSettings()
+        self.precision = 64 if precision == 64 else 32
+        h = ctypes.c_void_p()
+        st = self._lib.orc_create(ctypes.byref(CSettings.of(self._settings)), self.precision, ctypes.byref(h))
+        if st:
+            raise PhysicsError(st)
+        self._h = h
+        self.set_threads(threads)
+        self.set_residual_mode(residual_mode)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.orc_destroy(self._h)
+            self._h = None
+
+    def _check(self, st):
+        if st:
+            raise PhysicsError(st, self._lib.orc_error_detail(self._h).decode())
+
+    # controls
+    def set_threads(self, n: int) -> None:
+        self._lib.orc_set_threads(self._h, n)
+
+    def skip_job(self, job: int) -> None:
+        self._lib.orc_set_skip_job(self._h, job)
+
+    def set_residual_mode(self, mode: int) -> None:
+        self._lib.orc_set_residual_mode(self._h, mode)
+
+    def log_solves(self, enable: bool = True) -> None:
+        self._lib.orc_set_solve_log(self._h, 1 if enable else 0)
+
+    def solve_log(self):
+        n = self._lib.orc_read_solve_log(self._h, 0, None, None, None, None, None)
+        s = np.zeros(n, np.int32)
+        it = np.zeros(n, np.uint32)
+        ld = np.zeros(n, np.uint32)
+        a32 = np.zeros(n, np.float32)
+        a64 = np.zeros(n, np.float64)
+        self._lib.orc_read_solve_log(self._h, n, _p(s, ctypes.c_int32), _p(it, ctypes.c_uint32),
+                                     _p(ld, ctypes.c_uint32), _p(a32, ctypes.c_float), _p(a64, ctypes.c_double))
+        return {"solve": s, "iteration": it, "loaded": ld, "average_f32": a32, "average_f64": a64}
+
+    def kernel_value(self, d: float) -> float:
+        return self._lib.orc_kernel_value(self._h, d)
+
+    def kernel_slope(self, d: float) -> float:
+        return self._lib.orc_kernel_slope(self._h, d)
+
+    # API mirror
+    @property
+    def settings(self) -> FluidSettings:
+        return self._settings
+
+    def particle_mass(self) -> float:
+        return float(self._lib.orc_particle_mass(self._h))
+
+    def __len__(self) -> int:
+        return int(self._lib.orc_len(self._h))
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    def add_particle(self, position, velocity=(0.0, 0.0, 0.0)) -> int:
+        return int(self.add_particles([position], [velocity])[0])
+
+    def add_particles(self, positions, velocities=None) -> np.ndarray:
+        p = _f32(positions)
+        v = None if velocities is None else _f32(velocities)
+        ids = np.zeros(p.shape[0], np.uint32)
+        self._check(self._lib.orc_add_particles(self._h, p.shape[0], _p(p, ctypes.c_float),
+                                                None if v is None else _p(v, ctypes.c_float),
+                                                _p(ids, ctypes.c_uint32)))
+        return ids
+
+    def fill_box(self, low, high) -> int:
+        c = ctypes.c_size_t()
+        self._check(self._lib.orc_fill_box(self._h, *map(float, low), *map(float, high), ctypes.byref(c)))
+        return c.value
+
+    def fill_box_jittered(self, low, high, jitter: float, seed: int) -> int:
+        c = ctypes.c_size_t()
+        self._check(self._lib.orc_fill_box_jittered(self._h, *map(float, low), *map(float, high), jitter, seed,
+                                                    ctypes.byref(c)))
+        return c.value
+
+    def add_box_container(self, low, high) -> int:
+        o = ctypes.c_uint32()
+        self._check(self._lib.orc_add_box_container(self._h, *map(float, low), *map(float, high), ctypes.byref(o)))
+        return o.value
+
+    def add_wall(self, low, high) -> int:
+        o = ctypes.c_uint32()
+        self._check(self._lib.orc_add_wall(self._h, *map(float, low), *map(float, high), ctypes.byref(o)))
+        return o.value
+
+    def add_boundary_points(self, points) -> int:
+        p = _f32(points)
+        o = ctypes.c_uint32()
+        self._check(self._lib.orc_add_boundary_points(self._h, p.shape[0], _p(p, ctypes.c_float), ctypes.byref(o)))
+        return o.value
+
+    def remove_boundary(self, boundary: int) -> bool:
+        o = ctypes.c_int32()
+        self._check(self._lib.orc_remove_boundary(self._h, boundary, ctypes.byref(o)))
+        return bool(o.value)
+
+    def wall_positions(self) -> np.ndarray:
+        n = int(self._lib.orc_wall_len(self._h))
+        out = np.zeros((n, 3), np.float32)
+        self._check(self._lib.orc_read_wall_positions(self._h, n, _p(out, ctypes.c_float)))
+        return out
+
+    def step(self, dt: float) -> FluidReport:
+        r = CReport()
+        self._check(self._lib.orc_step(self._h, dt, ctypes.byref(r)))
+        return r.to_report()
+
+    def update_neighbors(self) -> None:
+        self._check(self._lib.orc_update_neighbors(self._h))
+
+    def compute_densities(self) -> None:
+        self._check(self._lib.orc_compute_densities(self._h))
+
+    def ids(self) -> np.ndarray:
+        out = np.zeros(len(self), np.uint32)
+        self._check(self._lib.orc_read_ids(self._h, out.shape[0], _p(out, ctypes.c_uint32)))
+        return out
+
+    def _vec(self, field: int) -> np.ndarray:
+        dt = np.float64 if self.precision == 64 else np.float32
+        out = np.zeros((len(self), 3), dt)
+        self._check(self._lib.orc_read_vectors(self._h, field, out.shape[0], out.ctypes.data))
+        return out
+
+    def positions(self) -> np.ndarray:
+        return self._vec(0)
+
+    def velocities(self) -> np.ndarray:
+        return self._vec(1)
+
+    def forces(self) -> np.ndarray:
+        return self._vec(2)
+
+    def keep_forces(self, keep: bool = True) -> None:
+        pass  # the oracle always keeps them
+
+    def densities(self) -> np.ndarray:
+        dt = np.float64 if self.precision == 64 else np.float32
+        out = np.zeros(len(self), dt)
+        self._check(self._lib.orc_read_densities(self._h, out.shape[0], out.ctypes.data))
+        return out
+
+    def set_particles(self, ids, positions, velocities) -> None:
+        ids = np.ascontiguousarray(ids, np.uint32)
+        p, v = _f32(positions), _f32(velocities)
+        self._check(self._lib.orc_set_particles(self._h, ids.shape[0], _p(ids, ctypes.c_uint32),
+                                                _p(p, ctypes.c_float), _p(v, ctypes.c_float)))
+
+    def _lists(self, kind: int):
+        n = len(self)
+        start = np.zeros(n + 1, np.uint32)
+        cnt = ctypes.c_size_t()
+        self._lib.orc_read_neighbors(self._h, kind, 0, None, 0, None, ctypes.byref(cnt))
+        items = np.zeros(max(cnt.value, 1), np.uint32)
+        self._check(self._lib.orc_read_neighbors(self._h, kind, n + 1, _p(start, ctypes.c_uint32), items.shape[0],
+                                                 _p(items, ctypes.c_uint32), ctypes.byref(cnt)))
+        return start, items[: cnt.value]
+
+    def neighbors(self):
+        return self._lists(0)
+
+    def wall_neighbors(self):
+        return self._lists(1)
+
+    def neighbors_of(self, index: int) -> np.ndarray:
+        s, it = self.neighbors()
+        return it[s[index]:s[index + 1]]
+
+    def state_hash(self) -> int:
+        return int(self._lib.orc_state_hash(self._h))
+
+
+def plant_mistake(which: int) -> None:
+    """Switches a deliberate mistake into the oracle (0 = none); CPU only."""
+    library().orc_plant_mistake(which)
+
+
+def fnv1a(data: bytes) -> int:
+    return int(library().orc_fnv1a(data, len(data)))
+
+
+def fixed_tree_sum(values) -> float:
+    v = np.ascontiguousarray(values, np.float32)
+    return float(library().orc_fixed_tree_sum(_p(v, ctypes.c_float), v.shape[0]))

@@ -358,6 +358,8 @@ const char* orc_error_detail(void* w) {
   return with(static_cast<Handle*>(w), [&](auto& x) { return x.detail.c_str(); });
 }
 
+void orc_plant_mistake(int32_t which) { planted_mistake() = which; }
+
 // FNV-1a 64 (src/snapshot/bytes.rs:6-13) over raw bytes, for known answers.
 uint64_t orc_fnv1a(const uint8_t* data, size_t n) {
   uint64_t h = 0xcbf29ce484222325ull;
@@ -366,6 +368,32 @@ uint64_t orc_fnv1a(const uint8_t* data, size_t n) {
     h *= 0x00000100000001b3ull;
   }
   return h;
+}
+
+// The reference's grid test (src/fluid/mod.rs:803-828) on this Grid: the
+// dense table and the binary search find the same ranges, a far-spread grid
+// keeps no table. Returns 1 when every check holds.
+int32_t orc_grid_selftest() {
+  std::vector<CellKey> keys = {{-3, 0, 2}, {-3, 0, 2}, {-1, 4, 0}, {0, 0, 0}, {5, -2, 1}};
+  std::sort(keys.begin(), keys.end());
+  Grid dense = Grid::build(keys.begin(), keys.end());
+  if (dense.dense.empty()) return 0;
+  Grid search = dense;
+  search.dense.clear();
+  for (int x = -5; x < 7; ++x)
+    for (int y = -4; y < 6; ++y)
+      for (int z = -2; z < 4; ++z) {
+        auto a = dense.range({x, y, z}), b = search.range({x, y, z});
+        if (a.has_value() != b.has_value()) return 0;
+        if (a && !(*a == *b)) return 0;
+      }
+  std::vector<CellKey> far_keys = {{-100000, 0, 0}, {100000, 0, 0}};
+  Grid far = Grid::build(far_keys.begin(), far_keys.end());
+  if (!far.dense.empty()) return 0;
+  auto r = far.range({100000, 0, 0});
+  if (!r || r->begin != 1 || r->end != 2) return 0;
+  if (far.range({INT32_MAX, INT32_MIN, 0})) return 0;
+  return 1;
 }
 
 // The fixed-shape FP64 residual sum (ResidualMode::kFixedTreeF64) over

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
 
 // ---- persistent Jacobi solve ------------------------------------------------
 
-constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr uint32_t kSpinLimit = 1u << 26;  // about 4 s of s_sleep(2)
 
 // Grid-wide barrier of a cooperative launch (every block resident). Returns
 // false if it waited past the spin limit, having flagged the world.

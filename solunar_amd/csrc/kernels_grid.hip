@@ -27,6 +27,7 @@ __global__ void k_step_begin(Control* ctl) {
   ctl->iterations_step[1] = 0;
   ctl->rates_step = 0;
   ctl->log_len = 0;
+  ctl->bar_count = 0;  // no persistent launch is in flight on this stream
 }
 
 __global__ void k_substep_begin(Control* ctl) {

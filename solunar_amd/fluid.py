@@ -1,0 +1,292 @@
+"""`FluidWorld` over the gfx950 library (libsolunar_fluid.so) through its C ABI.
+
+The methods mirror the reference's `FluidWorld` (src/fluid/mod.rs:341-692,
+src/fluid/step.rs:51-90, src/fluid/boundary.rs:35-120): same names, same
+argument meaning, errors raised as `PhysicsError` carrying the reference
+variant name. The reference's `dispatch` argument has no counterpart: the
+device has one schedule.
+
+There is no CPU fallback: if the library is missing this module raises on
+first use.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from .types import (CReport, CSettings, CVec3, FluidReport, FluidSettings,
+                    PhysicsError)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsolunar_fluid.so")
+
+_lib = None
+
+PHASES = ("bounds", "sort", "reorder", "density", "force", "solve", "viscous", "advance")
+
+
+def library() -> ctypes.CDLL:
+    """The loaded C library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    sz = ctypes.c_size_t
+    i32 = ctypes.c_int32
+    sigs = {
+        "sfl_settings_default": (None, [ctypes.POINTER(CSettings)]),
+        "sfl_create": (i32, [ctypes.POINTER(CSettings), ctypes.POINTER(P)]),
+        "sfl_destroy": (None, [P]),
+        "sfl_get_settings": (i32, [P, ctypes.POINTER(CSettings)]),
+        "sfl_particle_mass": (ctypes.c_float, [P]),
+        "sfl_len": (sz, [P]),
+        "sfl_add_particles": (i32, [P, sz, fp, fp, u32p]),
+        "sfl_fill_box": (i32, [P, CVec3, CVec3, ctypes.POINTER(sz)]),
+        "sfl_fill_box_jittered": (i32, [P, CVec3, CVec3, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(sz)]),
+        "sfl_set_domain": (i32, [P, ctypes.POINTER(CVec3), ctypes.POINTER(CVec3)]),
+        "sfl_add_box_container": (i32, [P, CVec3, CVec3, u32p]),
+        "sfl_add_wall": (i32, [P, CVec3, CVec3, u32p]),
+        "sfl_add_boundary_points": (i32, [P, sz, fp, u32p]),
+        "sfl_remove_boundary": (i32, [P, ctypes.c_uint32, ctypes.POINTER(i32)]),
+        "sfl_wall_len": (sz, [P]),
+        "sfl_read_wall_positions": (i32, [P, sz, fp]),
+        "sfl_step": (i32, [P, ctypes.c_float, ctypes.POINTER(CReport)]),
+        "sfl_update_neighbors": (i32, [P]),
+        "sfl_compute_densities": (i32, [P]),
+        "sfl_read_ids": (i32, [P, sz, u32p]),
+        "sfl_read_positions": (i32, [P, sz, fp]),
+        "sfl_read_velocities": (i32, [P, sz, fp]),
+        "sfl_read_densities": (i32, [P, sz, fp]),
+        "sfl_keep_forces": (i32, [P, i32]),
+        "sfl_read_forces": (i32, [P, sz, fp]),
+        "sfl_read_neighbors": (i32, [P, i32, sz, u32p, sz, u32p, ctypes.POINTER(sz)]),
+        "sfl_state_hash": (i32, [P, ctypes.POINTER(ctypes.c_uint64)]),
+        "sfl_error_detail": (ctypes.c_char_p, [P]),
+        "sfl_status_text": (ctypes.c_char_p, [i32]),
+        "sfl_profile": (i32, [P, i32]),
+        "sfl_profile_read": (i32, [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64), i32]),
+        "sfl_work_read": (i32, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                ctypes.POINTER(ctypes.c_uint64), i32]),
+        "sfl_synchronize": (i32, [P]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _f32(a, cols=3) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1, cols))
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class FluidWorld:
+    """The reference's FluidWorld with its particle state in HBM."""
+
+    def __init__(self, settings: Optional[FluidSettings] = None):
+        self._lib = library()
+        self._settings = settings or FluidSettings()
+        handle = ctypes.c_void_p()
+        st = self._lib.sfl_create(ctypes.byref(CSettings.of(self._settings)), ctypes.byref(handle))
+        if st:
+            raise PhysicsError(st, self._lib.sfl_error_detail(None).decode())
+        self._h = handle
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sfl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int) -> None:
+        if st:
+            raise PhysicsError(st, self._lib.sfl_error_detail(self._h).decode())
+
+    # ---- accessors (mod.rs:372-408) ----
+    @property
+    def settings(self) -> FluidSettings:
+        return self._settings
+
+    def particle_mass(self) -> float:
+        return float(self._lib.sfl_particle_mass(self._h))
+
+    def __len__(self) -> int:
+        return int(self._lib.sfl_len(self._h))
+
+    def len(self) -> int:
+        return len(self)
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    # ---- particles ----
+    def add_particle(self, position: Sequence[float], velocity: Sequence[float] = (0.0, 0.0, 0.0)) -> int:
+        ids = self.add_particles([position], [velocity])
+        return int(ids[0])
+
+    def add_particles(self, positions, velocities=None) -> np.ndarray:
+        p = _f32(positions)
+        v = None if velocities is None else _f32(velocities)
+        n = p.shape[0]
+        ids = np.zeros(n, dtype=np.uint32)
+        st = self._lib.sfl_add_particles(self._h, n, _ptr(p, ctypes.c_float),
+                                         None if v is None else _ptr(v, ctypes.c_float),
+                                         _ptr(ids, ctypes.c_uint32))
+        self._check(st)
+        return ids
+
+    def fill_box(self, low, high) -> int:
+        count = ctypes.c_size_t()
+        self._check(self._lib.sfl_fill_box(self._h, CVec3.of(low), CVec3.of(high), ctypes.byref(count)))
+        return count.value
+
+    def fill_box_jittered(self, low, high, jitter: float, seed: int) -> int:
+        count = ctypes.c_size_t()
+        self._check(self._lib.sfl_fill_box_jittered(self._h, CVec3.of(low), CVec3.of(high), jitter, seed,
+                                                    ctypes.byref(count)))
+        return count.value
+
+    def set_domain(self, domain: Optional[Tuple[Sequence[float], Sequence[float]]]) -> None:
+        if domain is None:
+            self._check(self._lib.sfl_set_domain(self._h, None, None))
+        else:
+            lo, hi = CVec3.of(domain[0]), CVec3.of(domain[1])
+            self._check(self._lib.sfl_set_domain(self._h, ctypes.byref(lo), ctypes.byref(hi)))
+
+    # ---- walls (boundary.rs) ----
+    def add_box_container(self, low, high) -> int:
+        out = ctypes.c_uint32()
+        self._check(self._lib.sfl_add_box_container(self._h, CVec3.of(low), CVec3.of(high), ctypes.byref(out)))
+        return out.value
+
+    def add_wall(self, low, high) -> int:
+        out = ctypes.c_uint32()
+        self._check(self._lib.sfl_add_wall(self._h, CVec3.of(low), CVec3.of(high), ctypes.byref(out)))
+        return out.value
+
+    def add_boundary_points(self, points) -> int:
+        p = _f32(points)
+        out = ctypes.c_uint32()
+        self._check(self._lib.sfl_add_boundary_points(self._h, p.shape[0], _ptr(p, ctypes.c_float),
+                                                      ctypes.byref(out)))
+        return out.value
+
+    def remove_boundary(self, boundary: int) -> bool:
+        out = ctypes.c_int32()
+        self._check(self._lib.sfl_remove_boundary(self._h, boundary, ctypes.byref(out)))
+        return bool(out.value)
+
+    def wall_positions(self) -> np.ndarray:
+        n = int(self._lib.sfl_wall_len(self._h))
+        out = np.zeros((n, 3), dtype=np.float32)
+        self._check(self._lib.sfl_read_wall_positions(self._h, n, _ptr(out, ctypes.c_float)))
+        return out
+
+    # ---- stepping ----
+    def step(self, dt: float) -> FluidReport:
+        rep = CReport()
+        self._check(self._lib.sfl_step(self._h, dt, ctypes.byref(rep)))
+        return rep.to_report()
+
+    def update_neighbors(self) -> None:
+        self._check(self._lib.sfl_update_neighbors(self._h))
+
+    def compute_densities(self) -> None:
+        self._check(self._lib.sfl_compute_densities(self._h))
+
+    # ---- readback ----
+    def ids(self) -> np.ndarray:
+        n = len(self)
+        out = np.zeros(n, dtype=np.uint32)
+        self._check(self._lib.sfl_read_ids(self._h, n, _ptr(out, ctypes.c_uint32)))
+        return out
+
+    def _vectors(self, fn) -> np.ndarray:
+        n = len(self)
+        out = np.zeros((n, 3), dtype=np.float32)
+        self._check(fn(self._h, n, _ptr(out, ctypes.c_float)))
+        return out
+
+    def positions(self) -> np.ndarray:
+        return self._vectors(self._lib.sfl_read_positions)
+
+    def velocities(self) -> np.ndarray:
+        return self._vectors(self._lib.sfl_read_velocities)
+
+    def densities(self) -> np.ndarray:
+        n = len(self)
+        out = np.zeros(n, dtype=np.float32)
+        self._check(self._lib.sfl_read_densities(self._h, n, _ptr(out, ctypes.c_float)))
+        return out
+
+    def keep_forces(self, keep: bool = True) -> None:
+        self._check(self._lib.sfl_keep_forces(self._h, 1 if keep else 0))
+
+    def forces(self) -> np.ndarray:
+        return self._vectors(self._lib.sfl_read_forces)
+
+    def _lists(self, kind: int) -> Tuple[np.ndarray, np.ndarray]:
+        n = len(self)
+        start = np.zeros(n + 1, dtype=np.uint32)
+        count = ctypes.c_size_t()
+        st = self._lib.sfl_read_neighbors(self._h, kind, n + 1, _ptr(start, ctypes.c_uint32), 0, None,
+                                          ctypes.byref(count))
+        if st not in (0, 10):
+            self._check(st)
+        items = np.zeros(max(count.value, 1), dtype=np.uint32)
+        self._check(self._lib.sfl_read_neighbors(self._h, kind, n + 1, _ptr(start, ctypes.c_uint32),
+                                                 items.shape[0], _ptr(items, ctypes.c_uint32),
+                                                 ctypes.byref(count)))
+        return start, items[: count.value]
+
+    def neighbors(self) -> Tuple[np.ndarray, np.ndarray]:
+        """CSR (start, items) of the fluid lists of the last build."""
+        return self._lists(0)
+
+    def wall_neighbors(self) -> Tuple[np.ndarray, np.ndarray]:
+        return self._lists(1)
+
+    def neighbors_of(self, index: int) -> np.ndarray:
+        start, items = self.neighbors()
+        return items[start[index]:start[index + 1]]
+
+    def state_hash(self) -> int:
+        out = ctypes.c_uint64()
+        self._check(self._lib.sfl_state_hash(self._h, ctypes.byref(out)))
+        return out.value
+
+    # ---- measurement ----
+    def profile(self, enable: bool) -> None:
+        self._check(self._lib.sfl_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self, reset: bool = False):
+        ms = (ctypes.c_double * len(PHASES))()
+        launches = (ctypes.c_uint64 * len(PHASES))()
+        self._check(self._lib.sfl_profile_read(self._h, ms, launches, 1 if reset else 0))
+        return {p: (ms[i], launches[i]) for i, p in enumerate(PHASES)}
+
+    def work_read(self, reset: bool = False):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self._lib.sfl_work_read(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                            1 if reset else 0))
+        return {"particle_substeps": a.value, "particle_pushes": b.value, "particle_rates": c.value}
+
+    def synchronize(self) -> None:
+        self._check(self._lib.sfl_synchronize(self._h))

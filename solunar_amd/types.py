@@ -1,0 +1,170 @@
+"""Plain data types of the fluid API: settings, report, errors.
+
+Mirrors `FluidSettings` (src/fluid/mod.rs:55-95), `FluidReport`
+(src/fluid/step.rs:28-49) and `PhysicsError` (src/error.rs:5-22) of the
+reference, plus the DFSPH constants of src/fluid/dfsph.rs:8-16 made settable.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field, fields
+from typing import Tuple
+
+WCSPH = 0
+DFSPH = 1
+
+# status codes of include/solunar_fluid.h (PhysicsError order, offset by one)
+STATUS_NAMES = {
+    0: "Ok",
+    1: "InvalidHandle",
+    2: "InvalidFrame",
+    3: "FrameTooDeep",
+    4: "NonFinite",
+    5: "InvalidShape",
+    6: "InvalidMaterial",
+    7: "InvalidSettings",
+    8: "NotDynamic",
+    9: "NotKinematic",
+    10: "OutOfRange",
+    11: "DispatchIncomplete",
+    12: "InvalidJoint",
+    13: "BodyHasJoints",
+    14: "ShapeInUse",
+    15: "TerrainBusy",
+    16: "FluidFull",
+    100: "DeviceError",
+    101: "Unsupported",
+}
+NON_FINITE = 4
+INVALID_SETTINGS = 7
+OUT_OF_RANGE = 10
+DISPATCH_INCOMPLETE = 11
+FLUID_FULL = 16
+DEVICE_ERROR = 100
+UNSUPPORTED = 101
+
+
+class PhysicsError(Exception):
+    """A non-zero status; `name` is the PhysicsError variant."""
+
+    def __init__(self, code: int, detail: str = ""):
+        self.code = code
+        self.name = STATUS_NAMES.get(code, f"Status{code}")
+        self.detail = detail
+        super().__init__(f"{self.name}" + (f"({detail})" if detail else ""))
+
+
+@dataclass
+class FluidSettings:
+    solver: int = WCSPH
+    spacing: float = 0.05
+    rest_density: float = 1000.0
+    support: float = 2.0
+    max_particles: int = 1 << 22
+    sound_speed: float = 20.0
+    viscosity: float = 0.15
+    gravity: Tuple[float, float, float] = (0.0, -9.81, 0.0)
+    courant: float = 0.4
+    max_substeps: int = 2000
+    divergence_tolerance: float = 1e-3
+    density_tolerance: float = 5e-4
+    min_iterations: int = 2
+    max_iterations: int = 100
+    device: int = 0
+
+    def radius(self) -> float:
+        import numpy as np
+
+        return float(np.float32(self.spacing) * np.float32(self.support))
+
+
+@dataclass
+class FluidReport:
+    substeps: int = 0
+    capped: bool = False
+    max_compression: float = 0.0
+    max_speed: float = 0.0
+    held_back: int = 0
+    iterations: int = 0
+    emitted: int = 0
+    drained: int = 0
+    lost: int = 0
+    sound_speed: float = 0.0
+    iterations_divergence: int = 0
+    iterations_density: int = 0
+
+
+class CSettings(ctypes.Structure):
+    """`sfl_settings` of include/solunar_fluid.h."""
+
+    _fields_ = [
+        ("solver", ctypes.c_int32),
+        ("spacing", ctypes.c_float),
+        ("rest_density", ctypes.c_float),
+        ("support", ctypes.c_float),
+        ("max_particles", ctypes.c_uint64),
+        ("sound_speed", ctypes.c_float),
+        ("viscosity", ctypes.c_float),
+        ("gravity", ctypes.c_float * 3),
+        ("courant", ctypes.c_float),
+        ("max_substeps", ctypes.c_uint32),
+        ("divergence_tolerance", ctypes.c_float),
+        ("density_tolerance", ctypes.c_float),
+        ("min_iterations", ctypes.c_uint32),
+        ("max_iterations", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+    ]
+
+    @classmethod
+    def of(cls, s: FluidSettings) -> "CSettings":
+        c = cls()
+        for f in fields(s):
+            v = getattr(s, f.name)
+            if f.name == "gravity":
+                c.gravity = (ctypes.c_float * 3)(*v)
+            else:
+                setattr(c, f.name, v)
+        return c
+
+
+class CReport(ctypes.Structure):
+    """`sfl_report` of include/solunar_fluid.h."""
+
+    _fields_ = [
+        ("substeps", ctypes.c_uint32),
+        ("capped", ctypes.c_uint32),
+        ("max_compression", ctypes.c_float),
+        ("max_speed", ctypes.c_float),
+        ("held_back", ctypes.c_uint32),
+        ("iterations", ctypes.c_uint32),
+        ("emitted", ctypes.c_uint32),
+        ("drained", ctypes.c_uint32),
+        ("lost", ctypes.c_uint32),
+        ("sound_speed", ctypes.c_float),
+        ("iterations_divergence", ctypes.c_uint32),
+        ("iterations_density", ctypes.c_uint32),
+    ]
+
+    def to_report(self) -> FluidReport:
+        return FluidReport(
+            substeps=self.substeps,
+            capped=bool(self.capped),
+            max_compression=self.max_compression,
+            max_speed=self.max_speed,
+            held_back=self.held_back,
+            iterations=self.iterations,
+            emitted=self.emitted,
+            drained=self.drained,
+            lost=self.lost,
+            sound_speed=self.sound_speed,
+            iterations_divergence=self.iterations_divergence,
+            iterations_density=self.iterations_density,
+        )
+
+
+class CVec3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("z", ctypes.c_float)]
+
+    @classmethod
+    def of(cls, v) -> "CVec3":
+        return cls(float(v[0]), float(v[1]), float(v[2]))

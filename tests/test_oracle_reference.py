@@ -1,0 +1,374 @@
+"""The reference's own fluid tests, run on the CPU oracle: they pin the
+restatement to the reference's known answers and invariants.
+
+Each test names the reference test it restates (src/fluid/mod.rs:776-972,
+src/fluid/step.rs:377-583). Thread-count invariance uses the oracle's
+512-particle jobs on 1, 4 and 7 workers run backwards, like the reference's
+dispatch::testing::Threads. A planted-mistake check at the end shows the
+density test is not blind: it fails for a kernel with a dropped term.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleWorld, fixed_tree_sum, fnv1a, library
+from solunar_amd import scenes
+from solunar_amd.types import DFSPH, WCSPH, FluidSettings, PhysicsError
+
+TICK = np.float32(1.0 / 60.0)
+
+
+def world(**kw):
+    return OracleWorld(FluidSettings(**kw))
+
+
+def shaken(count, threads=1):
+    w = OracleWorld(FluidSettings(), threads=threads)
+    scenes.shaken(w, count)
+    return w
+
+
+# mod.rs:776-801
+def test_the_kernel_holds_unit_mass_and_its_slope_matches():
+    w = OracleWorld(FluidSettings(spacing=0.05, support=2.0))  # radius 0.1
+    steps = 4000
+    dr = np.float32(0.1) / np.float32(steps)
+    total = np.float32(0.0)
+    for k in range(steps):
+        r = (np.float32(k) + np.float32(0.5)) * dr
+        total += np.float32(4.0) * np.float32(math.pi) * r * r * np.float32(w.kernel_value(float(r))) * dr
+    assert abs(total - 1.0) < 1e-3, total
+    assert w.kernel_value(0.1) == 0.0 and w.kernel_value(0.5) == 0.0
+    for r in [0.01, 0.03, 0.05, 0.08, 0.099]:
+        h = 1e-4
+        numeric = (np.float32(w.kernel_value(r + h)) - np.float32(w.kernel_value(r - h))) / np.float32(2 * h)
+        slope = w.kernel_slope(r)
+        assert abs(numeric - slope) < 1e-2 * max(abs(slope), 1.0), r
+        assert slope <= 0.0
+
+
+def test_the_kernel_matches_its_closed_form_in_fp64():
+    w = OracleWorld(FluidSettings(spacing=0.05), precision=64)
+    r = float(np.float32(0.05) * np.float32(2.0))  # the f32 radius, taken exactly
+    scale = 21.0 / (2.0 * math.pi * r ** 3)
+    for d in np.linspace(0.0, 0.12, 37):
+        q = d / r
+        v = scale * (1 - q) ** 4 * (1 + 4 * q) if q < 1 else 0.0
+        s = scale * -20 * q * (1 - q) ** 3 / r if q < 1 else 0.0
+        assert w.kernel_value(d) == pytest.approx(v, rel=1e-12, abs=1e-12)
+        assert w.kernel_slope(d) == pytest.approx(s, rel=1e-12, abs=1e-12)
+
+
+# mod.rs:803-828
+def test_the_dense_table_and_the_search_find_the_same_cells():
+    assert library().orc_grid_selftest() == 1
+
+
+# mod.rs:830-845
+def test_neighbors_are_exactly_the_brute_force_ones():
+    w = shaken(900)
+    w.update_neighbors()
+    p = w.positions()
+    reach = np.float32(0.1) * np.float32(0.1)
+    start, items = w.neighbors()
+    for i in range(len(w)):
+        d = p - p[i]
+        sq = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        brute = np.nonzero(sq < reach)[0]
+        assert np.array_equal(np.sort(items[start[i]:start[i + 1]]), brute), i
+
+
+def test_neighbour_lists_come_in_cell_order_then_id_order():
+    w = shaken(900)
+    w.update_neighbors()
+    p, ids = w.positions(), w.ids()
+    cell = np.floor(p / np.float32(0.1)).astype(np.int64)
+    key = list(zip(cell[:, 0], cell[:, 1], cell[:, 2], ids))
+    assert key == sorted(key), "not sorted by (cell, id)"
+    start, items = w.neighbors()
+    order = [(dz, dy, dx) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    for i in range(0, len(w), 37):
+        seq = []
+        for j in items[start[i]:start[i + 1]]:
+            d = tuple(int(v) for v in (cell[j] - cell[i]))
+            seq.append((order.index((d[2], d[1], d[0])), int(ids[j])))
+        assert seq == sorted(seq), i
+
+
+# mod.rs:847-864
+def test_any_thread_count_gives_the_same_lists_densities_and_hash():
+    def run(threads):
+        w = shaken(3000, threads)
+        w.update_neighbors()
+        w.compute_densities()
+        return w.neighbors()[1].tobytes(), w.densities().tobytes(), w.ids().tobytes(), w.state_hash()
+
+    one = run(1)
+    assert run(4) == one
+    assert run(7) == one
+
+
+# mod.rs:866-895
+def test_a_lattice_inside_is_at_rest_density():
+    w = world()
+    assert w.fill_box((0, 0, 0), (0.6, 0.6, 0.6)) == 12 * 12 * 12
+    w.update_neighbors()
+    w.compute_densities()
+    rest = 1000.0
+    p, d = w.positions(), w.densities()
+    margin = 0.1
+    inside = np.all((p > margin) & (p < 0.6 - margin), axis=1)
+    assert inside.sum() > 100
+    assert np.all(np.abs(d[inside] - rest) / rest < 1e-4)
+    assert np.all(d[~inside] <= rest * (1 + 1e-4))
+    cell = rest * 0.05 ** 3
+    assert abs(w.particle_mass() / cell - 1.0) < 0.1
+
+
+# mod.rs:897-913
+def test_ids_follow_particles_through_the_sort():
+    w = shaken(700)
+    before = dict(zip(w.ids().tolist(), map(tuple, w.positions())))
+    h = w.state_hash()
+    w.update_neighbors()
+    for i, pos in zip(w.ids().tolist(), map(tuple, w.positions())):
+        assert before[i] == pos
+    assert w.state_hash() == h
+
+
+# mod.rs:915-972
+def test_bad_input_is_refused_and_changes_nothing():
+    for bad in (dict(spacing=0.0), dict(rest_density=math.nan), dict(support=9.0), dict(max_particles=0)):
+        with pytest.raises(PhysicsError) as e:
+            OracleWorld(FluidSettings(**bad))
+        assert e.value.name == "InvalidSettings"
+    w = world(max_particles=100)
+    with pytest.raises(PhysicsError):
+        w.add_particle((math.nan, 0, 0))
+    with pytest.raises(PhysicsError):
+        w.add_particle((0, 0, 0), (math.nan, 0, 0))
+    with pytest.raises(PhysicsError):
+        w.fill_box((1, 1, 1), (0, 0, 0))
+    with pytest.raises(PhysicsError):
+        w.fill_box((0, 0, 0), (math.nan, 0, 0))
+    with pytest.raises(PhysicsError) as e:
+        w.fill_box((0, 0, 0), (1, 1, 1))
+    assert e.value.name == "FluidFull"
+    with pytest.raises(PhysicsError) as e:
+        w.fill_box((0, 0, 0), (1e30, 1e30, 1e30))
+    assert e.value.name == "FluidFull"
+    assert w.is_empty()
+    assert w.fill_box((0, 0, 0), (0.2, 0.2, 0.2)) == 64
+    with pytest.raises(PhysicsError) as e:
+        w.compute_densities()
+    assert e.value.name == "OutOfRange"
+    w.skip_job(0)
+    with pytest.raises(PhysicsError) as e:
+        w.update_neighbors()
+    assert e.value.name == "DispatchIncomplete"
+
+
+def tank(water, size, solver=WCSPH, **kw):
+    w = OracleWorld(FluidSettings(solver=solver, **kw), threads=8)
+    w.add_box_container((0, 0, 0), size)
+    w.fill_box((0, 0, 0), water)
+    return w
+
+
+def inside(w, size):
+    p = w.positions()
+    return bool(np.all(p >= 0) and np.all(p <= np.asarray(size, np.float32)))
+
+
+# step.rs:377-411
+@pytest.mark.slow
+def test_still_water_stays_still_and_presses_by_its_depth():
+    size = (0.5, 0.6, 0.5)
+    depth = 0.4
+    w = tank((0.5, depth, 0.5), size)
+    count = len(w)
+    held = 0
+    last = None
+    for _ in range(180):
+        last = w.step(TICK)
+        held += last.held_back
+        assert not last.capped
+    assert len(w) == count and inside(w, size) and held == 0
+    mean = float(np.mean(np.linalg.norm(w.velocities(), axis=1)))
+    assert last.max_speed < 0.2 and mean < 0.03, (last, mean)
+    stiffness = 1000.0 * 20.0 ** 2 / 7.0
+    expected = (1.0 + 1000.0 * 9.81 * depth / stiffness) ** (1.0 / 7.0) - 1.0
+    assert expected * 0.5 < last.max_compression < expected * 2.0
+    top = float(w.positions()[:, 1].max())
+    assert abs(top - (depth - 0.025)) < 0.05, top
+
+
+# step.rs:413-445
+def test_a_column_collapses_but_never_outruns_the_ideal_front():
+    size = (1.6, 0.6, 0.2)
+    height = 0.4
+    w = tank((0.4, height, 0.2), size)
+    start = float(w.positions()[:, 0].max())
+    ideal = 2.0 * math.sqrt(9.81 * height)
+    t = 0.0
+    last = start
+    held = 0
+    for _ in range(24):
+        held += w.step(TICK).held_back
+        t += float(TICK)
+        now = float(w.positions()[:, 0].max())
+        assert now - start <= ideal * t + 0.05, (t, now)
+        assert now >= last - 0.01
+        last = now
+    assert last - start > 0.4 and inside(w, size) and held == 0
+
+
+MARTIN_MOYCE = [(0.832, 1.217), (1.219, 1.474), (1.997, 2.292), (2.547, 2.995), (3.345, 4.134),
+                (4.034, 4.944), (4.418, 5.881), (5.091, 6.980), (5.685, 7.945), (6.306, 8.966)]
+
+
+# step.rs:447-507
+@pytest.mark.slow
+def test_the_dam_break_front_follows_martin_and_moyce():
+    width = 0.3
+    spacing = float(np.float32(0.3) / np.float32(18.0))
+    w = OracleWorld(FluidSettings(spacing=spacing), threads=8)
+    size = (3.2, 0.8, 0.1)
+    w.add_box_container((0, 0, 0), size)
+    w.fill_box((0, 0, 0), (width, 2 * width, 0.1))
+    count = len(w)
+    scale = math.sqrt(2 * 9.81 / width)
+    tick = np.float32(1.0 / 240.0)
+    t = np.float32(0.0)
+    errors = []
+    held = 0
+    for when, measured in MARTIN_MOYCE:
+        while float(t) * scale < when:
+            held += w.step(tick).held_back
+            t = np.float32(t + tick)
+        edge = float(w.positions()[:, 0].max())
+        front = (edge + spacing / 2) / width
+        err = (front - measured) / measured
+        assert -0.03 < err < 0.16, (when, front, measured)
+        errors.append(err)
+    assert sum(errors) / len(errors) < 0.10, errors
+    assert len(w) == count and held == 0 and inside(w, size)
+
+
+# step.rs:509-532
+def test_a_gate_holds_until_it_is_taken_away():
+    size = (1.2, 0.6, 0.2)
+    w = tank((0.4, 0.4, 0.2), size)
+    gate = w.add_wall((0.4, 0.0, 0.0), (0.5, 0.6, 0.2))
+    for _ in range(30):
+        w.step(TICK)
+    assert float(w.positions()[:, 0].max()) < 0.45
+    assert w.remove_boundary(gate)
+    assert not w.remove_boundary(gate)
+    for _ in range(20):
+        w.step(TICK)
+    assert float(w.positions()[:, 0].max()) > 0.7
+
+
+# step.rs:534-550
+@pytest.mark.parametrize("solver", [WCSPH, DFSPH])
+def test_any_thread_count_ends_on_the_same_state(solver):
+    def run(threads):
+        w = OracleWorld(FluidSettings(solver=solver), threads=threads)
+        w.add_box_container((0, 0, 0), (0.9, 0.5, 0.2))
+        w.fill_box((0, 0, 0), (0.3, 0.3, 0.2))
+        for _ in range(12):
+            w.step(TICK)
+        return w.state_hash()
+
+    one = run(1)
+    assert run(4) == one and run(3) == one and run(1) == one
+
+
+# step.rs:552-583
+def test_bad_steps_are_refused_and_a_cap_is_reported():
+    w = tank((0.2, 0.2, 0.2), (0.3, 0.3, 0.3))
+    for dt in (-1.0, math.nan):
+        with pytest.raises(PhysicsError):
+            w.step(dt)
+    assert w.step(0.0).substeps == 0
+    tight = world(max_substeps=2)
+    tight.fill_box((0, 0, 0), (0.1, 0.1, 0.1))
+    r = tight.step(TICK)
+    assert r.capped and r.substeps == 2
+    for bad in (dict(sound_speed=0.0), dict(courant=1.5), dict(viscosity=-1.0)):
+        with pytest.raises(PhysicsError):
+            world(**bad)
+
+
+def test_fnv1a_known_answers():
+    # published FNV-1a 64 test vectors
+    assert fnv1a(b"") == 0xCBF29CE484222325
+    assert fnv1a(b"a") == 0xAF63DC4C8601EC8C
+    assert fnv1a(b"foobar") == 0x85944171F73967E8
+
+
+def test_the_state_hash_is_fnv_of_id_position_velocity_in_id_order():
+    w = shaken(50)
+    w.update_neighbors()  # shuffles storage order; the hash must not care
+    ids, p, v = w.ids(), w.positions(), w.velocities()
+    order = np.argsort(ids)
+    blob = b"".join(np.uint32(ids[i]).tobytes() + p[i].astype("<f4").tobytes() + v[i].astype("<f4").tobytes()
+                    for i in order)
+    assert w.state_hash() == fnv1a(blob)
+
+
+def test_the_fixed_shape_residual_sum_is_the_documented_tree():
+    rng = np.random.default_rng(3)
+    for n in (1, 63, 512, 513, 5000, 200_000):
+        e = (rng.random(n, dtype=np.float32) * rng.choice([1e-6, 1e-3, 1.0], n)).astype(np.float32)
+        # the same shape written independently
+        chunks = (n + 511) // 512
+        part = []
+        for c in range(chunks):
+            s = [0.0] * 64
+            for lane in range(64):
+                acc = 0.0
+                for k in range(8):
+                    i = c * 512 + lane + 64 * k
+                    if i < n:
+                        acc += float(e[i])
+                s[lane] = acc
+            h = 32
+            while h >= 1:
+                for lane in range(h):
+                    s[lane] = s[lane] + s[lane + h]
+                h //= 2
+            part.append(s[0])
+        slot = [0.0] * 256
+        for t in range(256):
+            acc = 0.0
+            for c in range(t, chunks, 256):
+                acc += part[c]
+            slot[t] = acc
+        h = 128
+        while h >= 1:
+            for t in range(h):
+                slot[t] = slot[t] + slot[t + h]
+            h //= 2
+        got = fixed_tree_sum(e)
+        assert got == slot[0]
+        assert got == pytest.approx(math.fsum(map(float, e)), rel=1e-13)
+
+
+@pytest.mark.parametrize("mistake, check", [
+    (1, test_the_kernel_holds_unit_mass_and_its_slope_matches),
+    (2, test_a_column_collapses_but_never_outruns_the_ideal_front),
+    (3, test_neighbour_lists_come_in_cell_order_then_id_order),
+])
+def test_a_planted_mistake_in_the_oracle_fails_its_check(mistake, check):
+    from oracle.oracle import plant_mistake
+
+    plant_mistake(mistake)
+    try:
+        with pytest.raises(AssertionError):
+            check()
+    finally:
+        plant_mistake(0)
+    check()  # and passes again without it
