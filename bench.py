@@ -42,15 +42,19 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(steps: int):
-    """The CPU oracle (a C++ restatement of the reference step) on the first
-    `steps` steps of the same workload, all host threads (at most 16)."""
+def cpu_baseline(steps: int, warmup: int):
+    """The CPU oracle (a C++ restatement of the reference step, with the
+    reference's f32 stop sum) on the same workload: `warmup` untimed steps
+    from the initial state, as the device run, then `steps` timed steps, on
+    all host threads (at most 16)."""
     from oracle.oracle import OracleWorld
     from solunar_amd import scenes
 
     threads = min(16, os.cpu_count() or 1)
     w = OracleWorld(scenes.settings_dam_1m(), threads=threads)
     n = scenes.dam_break(w)
+    for _ in range(warmup):
+        w.step(DT)
     t0 = time.perf_counter()
     substeps = 0
     for _ in range(steps):
@@ -61,7 +65,7 @@ def cpu_baseline(steps: int):
         "unit": "particle-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {steps} step(s) of dam_1m ({n} particles, {substeps} substeps) from the initial state, "
+        "sample": f"steps {warmup}..{warmup + steps - 1} of dam_1m ({n} particles, {substeps} substeps), "
                   f"{dt:.1f} s wall on {threads} host threads",
     }
 
@@ -71,7 +75,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -87,8 +91,9 @@ def main() -> int:
     n = scenes.dam_break(world)
     walls = len(world.wall_positions())
     log(f"dam_1m: {n} fluid, {walls} wall particles; warmup {args.warmup} steps")
-    for _ in range(args.warmup):
-        world.step(DT)
+    for k in range(args.warmup):
+        r = world.step(DT)
+        log(f"warmup step {k}: {r.substeps} substeps, {r.iterations} iterations")
     world.synchronize()
     world.profile(True)
     world.profile_read(reset=True)
@@ -101,11 +106,15 @@ def main() -> int:
         r = world.step(DT)
         substeps += r.substeps
         iterations += r.iterations
+        if k % 5 == 4:
+            log(f"step {k + 1}/{args.steps}")
     world.synchronize()
     elapsed = time.perf_counter() - t0
     phases = world.profile_read(reset=True)
     work = world.work_read(reset=True)
     world.profile(False)
+    solve_blocks = world.solve_grid()
+    world.close()  # free device memory before the runtime (or a profiler) shuts down
     log(f"timed: {elapsed:.3f} s, {substeps} substeps, {iterations} iterations")
 
     value = n * args.steps / elapsed
@@ -142,7 +151,7 @@ def main() -> int:
         "config": {"workload": "dam_1m: DFSPH dam break, 1,024,000 fluid + 832,064 wall particles, spacing "
                                "0.005 m, step 5e-4 s, divergence tol 1e-3, density tol 1e-4",
                    "fluid_particles": n, "wall_particles": walls, "substeps": substeps,
-                   "jacobi_iterations": iterations},
+                   "jacobi_iterations": iterations, "solve_blocks": solve_blocks},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "launches": dom_launches, "avg_launch_ms": dom_ms / max(dom_launches, 1),
@@ -153,7 +162,7 @@ def main() -> int:
     }
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.warmup)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

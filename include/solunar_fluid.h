@@ -221,6 +221,10 @@ int32_t sfl_work_read(sfl_world* world, uint64_t* particle_substeps,
                       uint64_t* particle_pushes, uint64_t* particle_rates,
                       int32_t reset);
 
+/* Blocks of 512 threads the persistent DFSPH solve runs on (every one
+ * resident for its grid barrier). */
+int32_t sfl_solve_grid(const sfl_world* world);
+
 /* Blocks until every launch of the world has finished. */
 int32_t sfl_synchronize(sfl_world* world);
 

@@ -24,13 +24,26 @@ constexpr uint32_t kFlagOverflow = 1u;   // the cell box outgrew the table
 constexpr uint32_t kFlagNonFinite = 2u;  // an advance produced a non-finite value
 constexpr uint32_t kFlagBarrier = 4u;    // a grid barrier gave up waiting
 
+constexpr uint32_t kFlagPairs = 8u;     // a particle had more neighbours than the list holds
+
 constexpr int kResidualLogCap = 256;
+constexpr int kStripes = 64;
+
+// One of 64 copies of the report reductions; a block folds into stripe
+// blockIdx % 64, so no single address takes every wave's atomic.
+struct alignas(64) Stripe {
+  int32_t bmin[3];
+  int32_t bmax[3];
+  uint32_t vmax_bits;  // bits of max |v| after the last advance
+  uint32_t comp_bits;  // bits of max(rho/rest - 1, 0), last density pass
+  uint32_t held;       // particles held back this step
+  uint32_t pairs_max;  // most list entries any particle needed, this substep
+  uint32_t pad[6];
+};
 
 // Device-resident control block of one world: cell box, report reductions,
 // solve counters and the persistent kernels' barrier.
 struct Control {
-  int32_t bmin[3];
-  int32_t bmax[3];
   int32_t lo[3];
   uint32_t span[3];
   uint32_t volume;
@@ -39,22 +52,21 @@ struct Control {
   uint32_t pad0;
   uint64_t need_volume;  // cells the box needed when it overflowed
   uint32_t bad_index;    // smallest sorted index whose advance went non-finite
-  uint32_t held;         // particles held back this step
-  uint32_t vmax_bits;    // bits of max |v| after the last advance
-  uint32_t comp_bits;    // bits of max(rho/rest - 1, 0), last density pass
   uint32_t substeps_done;
   uint32_t iterations[2];        // last solve's pushes: divergence, density
   uint32_t iterations_step[2];   // pushes summed over this step
   uint32_t rates_step;           // rate passes summed over this step
-  uint32_t bar_count;
-  uint32_t bar_gen;
   uint32_t log_len;
-  uint32_t pad;
+  alignas(128) uint32_t bar_count;
+  uint32_t bar_gen;
+  alignas(128) Stripe stripes[kStripes];
   // average error of every rate pass (solve in the sign: + divergence, -
   // density), for parity logs
   double residual_log[kResidualLogCap];
   uint32_t loaded_log[kResidualLogCap];
 };
+
+__device__ __forceinline__ Stripe& stripe_of(Control* ctl) { return ctl->stripes[blockIdx.x & (kStripes - 1)]; }
 
 // A dense cell table over a box of cells: slot of cell (x, y, z) is
 // ((x - lo.x) * span.y + (y - lo.y)) * span.z + (z - lo.z); particles of slot
@@ -90,6 +102,13 @@ struct StepParams {
   // fluid cell table (box read from the control block)
   const uint32_t* cell_start;
   Control* ctl;
+  // per-substep neighbour lists (DFSPH): slot k of particle i at
+  // pairs[k * pair_stride + i] = (neighbour index, bits of slope(d)/d);
+  // fluid entries first, then walls; counts[i] = fluid | walls << 16
+  uint2* pairs;
+  uint32_t* pair_counts;
+  uint32_t pair_stride;
+  uint32_t pair_cap;
 };
 
 // ---- Wendland C2 (src/fluid/mod.rs:126-165) ------------------------------

@@ -1,30 +1,59 @@
 // DFSPH substep on gfx950 (src/fluid/dfsph.rs:79-103): density and factors,
-// the divergence and density Jacobi solves as one persistent cooperative
-// launch each, viscosity with gravity, and the advance.
+// the divergence and density Jacobi solves as one persistent launch each, viscosity with gravity, and the advance.
 //
-// The reference caches every pair's kernel gradient for the substep
-// (dfsph.rs:24-41, 108-136). Here each pass recomputes it from the fixed
-// positions with the same operations, which gives the same bits and trades
-// 12 bytes of HBM per pair per pass for arithmetic.
+// Neighbour lists: the density pass scans the 27 cells once per substep and
+// writes, for every particle, its neighbours in the reference's order with
+// the factor f = slope(d)/d of each pair (0 at d = 0). Positions do not move
+// until the advance, so every later pass of the substep rebuilds the pair's
+// kernel gradient as (x_i - x_j) * f, the same IEEE values the reference
+// caches per pair (dfsph.rs:24-41, 108-136), for 8 list bytes instead of 12
+// gradient bytes and without a second cell scan.
 //
 // Stop test: the reference averages the per-particle errors with an f32 left
 // fold in particle order (dfsph.rs:263-267), which no parallel sum can
 // reproduce. The solve instead sums them in FP64 with a fixed shape that the
-// CPU oracle replays exactly: lane l of the wave that owns chunk c (512
-// particles) adds errors 512c + l + 64k, k = 0..7, in order; the wave folds
-// lanes by halving (s_l += s_{l+h}, h = 32..1); chunk sums land in 256 slots,
-// slot t adding chunks t, t+256, ... in order; slots fold by halving
-// (h = 128..1). Every block computes that total itself after the barrier, so
-// all blocks take the same decision without a second barrier.
+// CPU oracle replays exactly: for chunk c of 512 particles, lane l of 64 adds
+// errors 512c + l + 64k, k = 0..7, in order; lanes fold by halving
+// (s_l += s_{l+h}, h = 32..1); chunk sums land in 256 slots, slot t adding
+// chunks t, t+256, ... in order; slots fold by halving (h = 128..1). Every
+// block computes that total itself after the barrier, so all blocks take the
+// same decision without a second barrier.
 #include "device.hpp"
 #include "launch.hpp"
 
 namespace sfl {
 
+constexpr uint32_t kSolveThreads = 512;  // one chunk of the stop sum per block round
+
+__device__ __forceinline__ bool list_aborted(const Control* ctl) {
+  return ctl->abort != 0 || (ctl->flags & kFlagPairs) != 0;
+}
+
+// Visits particle i's list: fluid(j, f) for its fluid neighbours, then
+// wall(w, f) for its walls, in the reference's order.
+template <class Fluid, class Wall>
+__device__ __forceinline__ void visit_list(const StepParams& prm, uint32_t i, Fluid&& fluid, Wall&& wall) {
+  uint32_t c = prm.pair_counts[i];
+  uint32_t nf = c & 0xffffu, nw = c >> 16;
+  const uint2* p = prm.pairs + i;
+  const uint32_t stride = prm.pair_stride;
+  uint32_t k = 0;
+  for (; k < nf; ++k) {
+    uint2 e = p[(size_t)k * stride];
+    fluid(e.x, __uint_as_float(e.y));
+  }
+  for (; k < nf + nw; ++k) {
+    uint2 e = p[(size_t)k * stride];
+    wall(e.x, __uint_as_float(e.y));
+  }
+}
+
 // compute_densities (src/fluid/mod.rs:621-675) fused with gradients + factors
-// (dfsph.rs:108-163) and the neighbour count the divergence solve uses
-// (dfsph.rs:249-255). rho goes to V.w; alpha to `alpha` with its sign bit set
-// for particles with fewer than 20 fluid + wall neighbours.
+// (dfsph.rs:108-163), the neighbour count the divergence solve uses
+// (dfsph.rs:249-255) and the substep's lists. rho goes to V.w; alpha to
+// `alpha` with its sign bit set for particles with fewer than 20 fluid + wall
+// neighbours. A particle with more neighbours than the lists hold flags the
+// world; later launches of the substep then do nothing and the host regrows.
 __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict__ P, float4* __restrict__ V,
                                                        float* __restrict__ alpha, StepParams prm) {
   Control* ctl = prm.ctl;
@@ -32,79 +61,97 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = i < prm.n;
   float comp = 0.0f;
+  uint32_t need = 0;
   if (live) {
     int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
     uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
     float4 at = P[i];
     float sum = 0.0f, wsum = 0.0f, squares = 0.0f;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    uint32_t crowd = 0;
+    uint32_t nf = 0, nw = 0;
+    const uint32_t cap = prm.pair_cap;
+    uint2* out = prm.pairs + i;
+    const size_t stride = prm.pair_stride;
     visit_near(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
-               [&](uint32_t, float4, float ox, float oy, float oz, float sq) {
+               [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
                  float distance = sqrtf(sq);
                  sum += kernel_value(distance, prm.radius, prm.scale);
-                 float gx, gy, gz;
-                 kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+                 float f = distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.scale) / distance;
+                 float gx = ox * f, gy = oy * f, gz = oz * f;
                  float qx = gx * prm.mass, qy = gy * prm.mass, qz = gz * prm.mass;
                  sx = sx + qx;
                  sy = sy + qy;
                  sz = sz + qz;
                  squares += dot3(qx, qy, qz, qx, qy, qz);
-                 ++crowd;
+                 if (nf < cap) out[nf * stride] = make_uint2(j, __float_as_uint(f));
+                 ++nf;
                });
     if (prm.wall_cells)
       visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
-                 [&](uint32_t, float4, float ox, float oy, float oz, float sq) {
-                   wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
-                   float gx, gy, gz;
-                   kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+                 [&](uint32_t w, float4, float ox, float oy, float oz, float sq) {
+                   float distance = sqrtf(sq);
+                   wsum += kernel_value(distance, prm.radius, prm.scale);
+                   float f = distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.scale) / distance;
+                   float gx = ox * f, gy = oy * f, gz = oz * f;
                    sx = sx + gx * prm.mass;
                    sy = sy + gy * prm.mass;
                    sz = sz + gz * prm.mass;
-                   ++crowd;
+                   uint32_t k = nf + nw;
+                   if (k < cap) out[k * stride] = make_uint2(w, __float_as_uint(f));
+                   ++nw;
                  });
+    need = nf + nw;
+    if (need > cap) atomicOr(&ctl->flags, kFlagPairs);
+    prm.pair_counts[i] = nf | (nw << 16);
     float density = prm.mass * (sum + wsum);
     float total = dot3(sx, sy, sz, sx, sy, sz) + squares;
     float a = total > 0.0f ? 1.0f / total : 0.0f;
-    alpha[i] = crowd < 20u ? -a : a;
+    alpha[i] = need < 20u ? -a : a;
     V[i].w = density;
     comp = density / prm.rest - 1.0f;
   }
-  wave_max_to(&ctl->comp_bits, comp);
+  wave_max_to(&stripe_of(ctl).comp_bits, comp);
+  for (int off = 32; off >= 1; off >>= 1) need = max(need, (uint32_t)__shfl_xor((int)need, off, 64));
+  if ((threadIdx.x & 63) == 0 && need) atomicMax(&stripe_of(ctl).pairs_max, need);
 }
 
 // ---- persistent Jacobi solve ------------------------------------------------
 
-constexpr uint32_t kSpinLimit = 1u << 26;  // about 4 s of s_sleep(2)
+// Wall-clock limit of one barrier wait: wall_clock64() counts at the
+// device's constant 100 MHz reference clock.
+constexpr uint64_t kBarrierTicks = 200000000ull;  // 2 s
 
-// Grid-wide barrier of a cooperative launch (every block resident). Returns
-// false if it waited past the spin limit, having flagged the world.
+// Grid-wide barrier of the persistent solve (every block resident). Every
+// wave releases its writes; thread 0 arrives, then polls the generation with
+// relaxed loads (an acquire per poll would invalidate the L2 that the blocks
+// still working are gathering from) and acquires once when it moves. Returns
+// false if it waited past the limit, having flagged the world.
 __device__ bool grid_barrier(Control* ctl) {
   __threadfence();
   __syncthreads();
   __shared__ int ok_s;
   if (threadIdx.x == 0) {
     int ok = 1;
-    uint32_t gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t arrived = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == gridDim.x - 1) {
       __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      uint32_t spins = 0;
-      while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > kSpinLimit) {
+      uint64_t start = wall_clock64();
+      while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(8);
+        if (wall_clock64() - start > kBarrierTicks) {
           ok = 0;
           atomicOr(&ctl->flags, kFlagBarrier);
           break;
         }
       }
+      __atomic_thread_fence(__ATOMIC_ACQUIRE);
     }
     ok_s = ok;
   }
   __syncthreads();
-  __threadfence();
   return ok_s != 0;
 }
 
@@ -124,50 +171,75 @@ struct SolveArgs {
 };
 
 // solve (src/fluid/dfsph.rs:222-275) with density_rate (dfsph.rs:166-181) and
-// push_apart (dfsph.rs:184-216).
-__global__ __launch_bounds__(256) void k_dfsph_solve(SolveArgs a) {
+// push_apart (dfsph.rs:184-216). Block b takes chunks b, b + grid, ...; its
+// thread t owns particle 512c + t, which is lane t % 64, term k = t / 64 of
+// the stop sum's chunk.
+__global__ __launch_bounds__(512) void k_dfsph_solve(SolveArgs a) {
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
-  if (ctl->abort) return;
-  const uint32_t waves = gridDim.x * 4u;
-  const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
-  const uint32_t lane = threadIdx.x & 63u;
+  if (list_aborted(ctl)) return;
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = t & 63u, term = t >> 6;
+  __shared__ double err_s[8][64];
+  __shared__ uint32_t cnt_s[8][64];
   __shared__ double slot[256];
   __shared__ uint32_t slot_count[256];
-  int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
-  uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
   const float dt = prm.dt;
-  auto walls = [&](float4 at, auto&& f) {
-    if (prm.wall_cells)
-      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
-  };
+  const float mass = prm.mass;
+  const float4* __restrict__ P = a.P;
+  float4* __restrict__ V = a.V;
+  const float4* __restrict__ W = prm.walls;
   uint32_t it = 0;
   for (;;) {
     // rate pass: stiffness and error of every particle
-    for (uint32_t c = wave; c < a.chunks; c += waves) {
-      double acc = 0.0;
-      uint32_t cnt = 0;
-      for (uint32_t k = 0; k < 8; ++k) {
-        uint32_t i = c * 512u + k * 64u + lane;
-        if (i >= prm.n) continue;
-        float4 at = a.P[i];
-        float4 vi = a.V[i];
+    for (uint32_t c = blockIdx.x; c < a.chunks; c += gridDim.x) {
+      uint32_t i = c * kSolveThreads + t;
+      float error = 0.0f;
+      if (i < prm.n) {
+        float4 at = P[i];
+        float4 vi = V[i];
         float rate = 0.0f;
-        visit_near(a.P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
-                   [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-                     float gx, gy, gz;
-                     kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
-                     float4 vj = a.V[j];
-                     rate += dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, gx, gy, gz);
-                   });
-        walls(at, [&](uint32_t, float4, float ox, float oy, float oz, float sq) {
-          float gx, gy, gz;
-          kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+        uint32_t counts = prm.pair_counts[i];
+        uint32_t nf = counts & 0xffffu, nw = counts >> 16;
+        const uint2* lp = prm.pairs + i;
+        const size_t stride = prm.pair_stride;
+        uint32_t k = 0;
+        // four entries' gathers in flight, then the terms in list order
+        for (; k + 4 <= nf; k += 4) {
+          uint2 e[4];
+          float4 pj[4], vj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) e[u] = lp[(k + u) * stride];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            pj[u] = P[e[u].x];
+            vj[u] = V[e[u].x];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            float f = __uint_as_float(e[u].y);
+            float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
+            rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
+          }
+        }
+        for (; k < nf; ++k) {
+          uint2 e = lp[k * stride];
+          float4 pj = P[e.x];
+          float4 vj = V[e.x];
+          float f = __uint_as_float(e.y);
+          float gx = (at.x - pj.x) * f, gy = (at.y - pj.y) * f, gz = (at.z - pj.z) * f;
+          rate += dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, gx, gy, gz);
+        }
+        for (; k < nf + nw; ++k) {
+          uint2 e = lp[k * stride];
+          float4 pw = W[e.x];
+          float f = __uint_as_float(e.y);
+          float gx = (at.x - pw.x) * f, gy = (at.y - pw.y) * f, gz = (at.z - pw.z) * f;
           rate += dot3(vi.x, vi.y, vi.z, gx, gy, gz);
-        });
-        rate = rate * prm.mass;
+        }
+        rate = rate * mass;
         float fa = a.alpha[i];
-        float target, error;
+        float target;
         if (a.which == 1) {
           float over = rmax(vi.w + dt * rate - prm.rest, 0.0f);
           target = over / (dt * dt);
@@ -178,43 +250,53 @@ __global__ __launch_bounds__(256) void k_dfsph_solve(SolveArgs a) {
           error = dt * r / prm.rest;
         }
         a.kappa[i] = target * fabsf(fa);
-        acc += (double)error;
-        cnt += error > 0.0f ? 1u : 0u;
       }
-      for (int h = 32; h >= 1; h >>= 1) {
-        acc = acc + __shfl_down(acc, h, 64);
-        cnt += (uint32_t)__shfl_down((int)cnt, h, 64);
+      err_s[term][lane] = (double)error;
+      cnt_s[term][lane] = error > 0.0f ? 1u : 0u;
+      __syncthreads();
+      if (term == 0) {
+        double acc = 0.0;
+        uint32_t cnt = 0;
+        for (uint32_t k = 0; k < 8; ++k) {
+          acc += err_s[k][lane];
+          cnt += cnt_s[k][lane];
+        }
+        for (int h = 32; h >= 1; h >>= 1) {
+          acc = acc + __shfl_down(acc, h, 64);
+          cnt += (uint32_t)__shfl_down((int)cnt, h, 64);
+        }
+        if (lane == 0) {
+          a.partial[c] = acc;
+          a.pcount[c] = cnt;
+        }
       }
-      if (lane == 0) {
-        a.partial[c] = acc;
-        a.pcount[c] = cnt;
-      }
+      __syncthreads();
     }
     if (!grid_barrier(ctl)) return;
     // every block folds the chunk sums the same way
-    {
+    if (t < 256) {
       double s = 0.0;
       uint32_t cn = 0;
-      for (uint32_t c = threadIdx.x; c < a.chunks; c += 256u) {
+      for (uint32_t c = t; c < a.chunks; c += 256u) {
         s += a.partial[c];
         cn += a.pcount[c];
       }
-      slot[threadIdx.x] = s;
-      slot_count[threadIdx.x] = cn;
-      __syncthreads();
-      for (uint32_t h = 128; h >= 1; h >>= 1) {
-        if (threadIdx.x < h) {
-          slot[threadIdx.x] = slot[threadIdx.x] + slot[threadIdx.x + h];
-          slot_count[threadIdx.x] += slot_count[threadIdx.x + h];
-        }
-        __syncthreads();
+      slot[t] = s;
+      slot_count[t] = cn;
+    }
+    __syncthreads();
+    for (uint32_t h = 128; h >= 1; h >>= 1) {
+      if (t < h) {
+        slot[t] = slot[t] + slot[t + h];
+        slot_count[t] += slot_count[t + h];
       }
+      __syncthreads();
     }
     double total = slot[0];
     uint32_t loaded = slot_count[0];
     __syncthreads();
     double average = total / (double)(loaded > 1u ? loaded : 1u);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && t == 0) {
       uint32_t at = ctl->log_len;
       if (at < (uint32_t)kResidualLogCap) {
         ctl->residual_log[at] = a.which == 0 ? average : -average;
@@ -225,41 +307,68 @@ __global__ __launch_bounds__(256) void k_dfsph_solve(SolveArgs a) {
     bool settled = it >= a.min_iterations && average <= (double)a.tolerance;
     if (settled || it >= a.max_iterations) break;
     // push pass
-    for (uint32_t c = wave; c < a.chunks; c += waves) {
-      for (uint32_t k = 0; k < 8; ++k) {
-        uint32_t i = c * 512u + k * 64u + lane;
-        if (i >= prm.n) continue;
-        float4 at = a.P[i];
-        float own = a.kappa[i];
-        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        visit_near(a.P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
-                   [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-                     float gx, gy, gz;
-                     kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
-                     float k2 = prm.mass * (own + a.kappa[j]);
-                     cx = cx - gx * k2;
-                     cy = cy - gy * k2;
-                     cz = cz - gz * k2;
-                   });
-        walls(at, [&](uint32_t, float4, float ox, float oy, float oz, float sq) {
-          float gx, gy, gz;
-          kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
-          float k2 = prm.mass * own;
+    for (uint32_t c = blockIdx.x; c < a.chunks; c += gridDim.x) {
+      uint32_t i = c * kSolveThreads + t;
+      if (i >= prm.n) continue;
+      float4 at = P[i];
+      float own = a.kappa[i];
+      float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+      uint32_t counts = prm.pair_counts[i];
+      uint32_t nf = counts & 0xffffu, nw = counts >> 16;
+      const uint2* lp = prm.pairs + i;
+      const size_t stride = prm.pair_stride;
+      uint32_t k = 0;
+      for (; k + 4 <= nf; k += 4) {
+        uint2 e[4];
+        float4 pj[4];
+        float kj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = lp[(k + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          pj[u] = P[e[u].x];
+          kj[u] = a.kappa[e[u].x];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          float f = __uint_as_float(e[u].y);
+          float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
+          float k2 = mass * (own + kj[u]);
           cx = cx - gx * k2;
           cy = cy - gy * k2;
           cz = cz - gz * k2;
-        });
-        float4 vi = a.V[i];
-        vi.x = vi.x + cx * dt;
-        vi.y = vi.y + cy * dt;
-        vi.z = vi.z + cz * dt;
-        a.V[i] = vi;
+        }
       }
+      for (; k < nf; ++k) {
+        uint2 e = lp[k * stride];
+        float4 pj = P[e.x];
+        float f = __uint_as_float(e.y);
+        float gx = (at.x - pj.x) * f, gy = (at.y - pj.y) * f, gz = (at.z - pj.z) * f;
+        float k2 = mass * (own + a.kappa[e.x]);
+        cx = cx - gx * k2;
+        cy = cy - gy * k2;
+        cz = cz - gz * k2;
+      }
+      for (; k < nf + nw; ++k) {
+        uint2 e = lp[k * stride];
+        float4 pw = W[e.x];
+        float f = __uint_as_float(e.y);
+        float gx = (at.x - pw.x) * f, gy = (at.y - pw.y) * f, gz = (at.z - pw.z) * f;
+        float k2 = mass * own;
+        cx = cx - gx * k2;
+        cy = cy - gy * k2;
+        cz = cz - gz * k2;
+      }
+      float4 vi = V[i];
+      vi.x = vi.x + cx * dt;
+      vi.y = vi.y + cy * dt;
+      vi.z = vi.z + cz * dt;
+      V[i] = vi;
     }
     if (!grid_barrier(ctl)) return;
     ++it;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0 && t == 0) {
     ctl->iterations[a.which] = it;
     ctl->iterations_step[a.which] += it;
     ctl->rates_step += it + 1;
@@ -267,51 +376,76 @@ __global__ __launch_bounds__(256) void k_dfsph_solve(SolveArgs a) {
 }
 
 // The viscosity pass of dfsph_substep (dfsph.rs:90-99): viscous
-// (src/fluid/step.rs:229-251) + gravity, v += push * dt. Reads VB, writes VA
-// (density carried along).
+// (src/fluid/step.rs:229-251, with wall_viscous step.rs:262-301) + gravity,
+// v += push * dt, over the substep's lists. Reads VB, writes VA (density
+// carried along).
 __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                        StepParams prm, float4* __restrict__ VA,
                                                        float* __restrict__ forces) {
   Control* ctl = prm.ctl;
-  if (ctl->abort) return;
+  if (list_aborted(ctl)) return;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= prm.n) return;
-  int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
-  uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+  const float4* __restrict__ W = prm.walls;
   float4 at = PB[i];
   float4 vi = VB[i];
   float density = vi.w;
   float px = 0.0f, py = 0.0f, pz = 0.0f;
-  visit_near(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
-             [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-               if (j == i) return;
-               float4 vj = VB[j];
-               float toward = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz);
-               if (toward < 0.0f) {
-                 float squeeze = toward / (sq + prm.soften);
-                 float damp = -prm.nu * squeeze * 2.0f / (density + vj.w);
-                 float gx, gy, gz;
-                 kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
-                 float k = prm.mass * damp;
-                 px = px - gx * k;
-                 py = py - gy * k;
-                 pz = pz - gz * k;
-               }
-             });
+  float awx = 0.0f, awy = 0.0f, awz = 0.0f;
+  visit_list(
+      prm, i,
+      [&](uint32_t j, float f) {
+        if (j == i) return;
+        float4 pj = PB[j];
+        float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+        float4 vj = VB[j];
+        float toward = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz);
+        if (toward < 0.0f) {
+          float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
+          float damp = -prm.nu * squeeze * 2.0f / (density + vj.w);
+          float k = prm.mass * damp;
+          px = px - (ox * f) * k;
+          py = py - (oy * f) * k;
+          pz = pz - (oz * f) * k;
+        }
+      },
+      [&](uint32_t w, float f) {
+        float4 pw = W[w];
+        awx = awx + -((at.x - pw.x) * f);
+        awy = awy + -((at.y - pw.y) * f);
+        awz = awz + -((at.z - pw.z) * f);
+      });
+  // wall_viscous: normalize_or(away, ZERO), motion into the walls only
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
-  if (prm.wall_cells) {
-    auto walls = [&](auto&& f) {
-      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
-    };
-    float awx = 0.0f, awy = 0.0f, awz = 0.0f;
-    walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
-      float gx, gy, gz;
-      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
-      awx = awx + -gx;
-      awy = awy + -gy;
-      awz = awz + -gz;
-    });
-    wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
+  uint32_t counts = prm.pair_counts[i];
+  uint32_t nf = counts & 0xffffu, nw = counts >> 16;
+  if (nw) {
+    float len = sqrtf(dot3(awx, awy, awz, awx, awy, awz));
+    float rcp = 1.0f / len;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (isfinite(rcp) && rcp > 0.0f) {
+      nx = awx * rcp;
+      ny = awy * rcp;
+      nz = awz * rcp;
+    }
+    float tn = rmin(dot3(vi.x, vi.y, vi.z, nx, ny, nz), 0.0f);
+    float ix = nx * tn, iy = ny * tn, iz = nz * tn;
+    const uint2* p = prm.pairs + i;
+    for (uint32_t k = nf; k < nf + nw; ++k) {
+      uint2 e = p[(size_t)k * prm.pair_stride];
+      float f = __uint_as_float(e.y);
+      float4 pw = W[e.x];
+      float ox = at.x - pw.x, oy = at.y - pw.y, oz = at.z - pw.z;
+      float toward = dot3(ix, iy, iz, ox, oy, oz);
+      if (toward < 0.0f) {
+        float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
+        float damp = -prm.wall_nu * squeeze / density;
+        float k2 = prm.mass * damp;
+        wx = wx - (ox * f) * k2;
+        wy = wy - (oy * f) * k2;
+        wz = wz - (oz * f) * k2;
+      }
+    }
   }
   float ax = px + wx + prm.gx;
   float ay = py + wy + prm.gy;
@@ -324,13 +458,13 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
   VA[i] = make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density);
 }
 
-// advance (src/fluid/step.rs:305-339): PB + VA*dt -> PA, VA. Particles past a
-// non-finite one keep their position and velocity, as in the reference.
+// advance (src/fluid/step.rs:305-339): PB + VA*dt -> PA, VA. The velocity
+// before the advance is kept in VKEEP for the repair after a non-finite one.
 __global__ __launch_bounds__(256) void k_dfsph_advance(const float4* __restrict__ PB, StepParams prm,
                                                        float4* __restrict__ PA, float4* __restrict__ VA,
                                                        float4* __restrict__ VKEEP) {
   Control* ctl = prm.ctl;
-  if (ctl->abort) return;
+  if (list_aborted(ctl)) return;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = i < prm.n;
   float speed = 0.0f;
@@ -349,13 +483,14 @@ __global__ __launch_bounds__(256) void k_dfsph_advance(const float4* __restrict_
     VA[i] = make_float4(vx, vy, vz, v.w);
     speed = sqrtf(dot3(vx, vy, vz, vx, vy, vz));
   }
-  wave_max_to(&ctl->vmax_bits, speed);
-  wave_sum_to(&ctl->held, held);
+  wave_max_to(&stripe_of(ctl).vmax_bits, speed);
+  wave_sum_to(&stripe_of(ctl).held, held);
 }
 
 // After a non-finite advance: particles past the first bad one go back to
 // their old position and pre-advance velocity (kept in VKEEP); the bad one
-// keeps its old position and its zeroed velocity components.
+// keeps its old position and its zeroed velocity components, as the
+// reference leaves them (step.rs:318-335).
 __global__ __launch_bounds__(256) void k_dfsph_repair(const float4* __restrict__ PB, const float4* __restrict__ VKEEP,
                                                       StepParams prm, float4* __restrict__ PA,
                                                       float4* __restrict__ VA) {
@@ -374,15 +509,22 @@ void launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alph
   hipLaunchKernelGGL(k_dfsph_density, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
 }
 
+// Blocks of the persistent solve. Every block must be resident at once for
+// the grid barrier; the occupancy query alone over-promised (a grid of its
+// size never met at the first barrier), so the grid also stays within the
+// device's resident threads per CU and at most two blocks per CU.
 int solve_grid_limit() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
-  int cus = 0;
+  int cus = 0, threads = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&threads, hipDeviceAttributeMaxThreadsPerMultiProcessor, dev) != hipSuccess) return 0;
   int per = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_dfsph_solve), 256, 0) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_dfsph_solve),
+                                                   kSolveThreads, 0) != hipSuccess)
     return 0;
+  per = per < threads / (int)kSolveThreads ? per : threads / (int)kSolveThreads;
+  per = per < 2 ? per : 2;
   return per * cus;
 }
 
@@ -398,17 +540,18 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   a.kappa = kappa;
   a.partial = partial;
   a.pcount = pcount;
-  a.chunks = (prm.n + 511u) / 512u;
+  a.chunks = (prm.n + kSolveThreads - 1) / kSolveThreads;
   a.which = which;
   a.tolerance = tolerance;
   a.min_iterations = min_it;
   a.max_iterations = max_it;
-  uint32_t want = (a.chunks + 3u) / 4u;  // one wave per chunk at most
-  uint32_t blocks = (uint32_t)grid < want ? (uint32_t)grid : want;
+  uint32_t blocks = (uint32_t)grid < a.chunks ? (uint32_t)grid : a.chunks;
   if (blocks == 0) blocks = 1;
-  void* args[] = {&a};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_dfsph_solve), dim3(blocks), dim3(256), args, 0,
-                                    s);
+  // A plain launch: the grid is sized to fit the device at once (two blocks
+  // per CU at most), so every block is resident for the barrier; it also
+  // keeps the launch capturable in a graph.
+  hipLaunchKernelGGL(k_dfsph_solve, dim3(blocks), dim3(kSolveThreads), 0, s, a);
+  return hipGetLastError();
 }
 
 void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,

@@ -18,9 +18,9 @@ namespace sfl {
 
 __global__ void k_step_begin(Control* ctl) {
   if (threadIdx.x != 0) return;
+  for (int k = 0; k < kStripes; ++k) ctl->stripes[k].held = 0;
   ctl->flags = 0;
   ctl->abort = 0;
-  ctl->held = 0;
   ctl->bad_index = 0xffffffffu;
   ctl->substeps_done = 0;
   ctl->iterations_step[0] = 0;
@@ -34,10 +34,14 @@ __global__ void k_substep_begin(Control* ctl) {
   if (threadIdx.x != 0) return;
   if (ctl->flags & kFlagNonFinite) ctl->abort = 1;
   if (!ctl->abort) ctl->substeps_done += 1;
-  ctl->bmin[0] = ctl->bmin[1] = ctl->bmin[2] = 0x7fffffff;
-  ctl->bmax[0] = ctl->bmax[1] = ctl->bmax[2] = (int32_t)0x80000000u;
-  ctl->comp_bits = 0;
-  ctl->vmax_bits = 0;
+  for (int k = 0; k < kStripes; ++k) {
+    Stripe& s = ctl->stripes[k];
+    s.bmin[0] = s.bmin[1] = s.bmin[2] = 0x7fffffff;
+    s.bmax[0] = s.bmax[1] = s.bmax[2] = (int32_t)0x80000000u;
+    s.comp_bits = 0;
+    s.vmax_bits = 0;
+    s.pairs_max = 0;
+  }
 }
 
 // fastest (src/fluid/step.rs:92-96): max |v| over the particles.
@@ -47,7 +51,7 @@ __global__ __launch_bounds__(256) void k_fastest(const float4* __restrict__ V, u
     float4 v = V[i];
     most = rmax(most, sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z)));
   }
-  wave_max_to(&ctl->vmax_bits, most);
+  wave_max_to(&stripe_of(ctl).vmax_bits, most);
 }
 
 // ---- cell bounds ---------------------------------------------------------
@@ -72,9 +76,10 @@ __global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ P, ui
     }
   }
   if ((threadIdx.x & 63) == 0) {
+    Stripe& s = stripe_of(ctl);
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&ctl->bmin[a], mn[a]);
-      atomicMax(&ctl->bmax[a], mx[a]);
+      atomicMin(&s.bmin[a], mn[a]);
+      atomicMax(&s.bmax[a], mx[a]);
     }
   }
 }
@@ -93,11 +98,18 @@ __global__ void k_grid_setup(Control* ctl, uint32_t n, uint64_t capacity) {
     ctl->volume = 1;
     return;
   }
+  int32_t bmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  int32_t bmax[3] = {(int32_t)0x80000000u, (int32_t)0x80000000u, (int32_t)0x80000000u};
+  for (int k = 0; k < kStripes; ++k)
+    for (int a = 0; a < 3; ++a) {
+      bmin[a] = min(bmin[a], ctl->stripes[k].bmin[a]);
+      bmax[a] = max(bmax[a], ctl->stripes[k].bmax[a]);
+    }
   uint64_t vol = 1;
   bool big = false;
   uint32_t span[3];
   for (int a = 0; a < 3; ++a) {
-    int64_t s = (int64_t)ctl->bmax[a] - (int64_t)ctl->bmin[a] + 1;
+    int64_t s = (int64_t)bmax[a] - (int64_t)bmin[a] + 1;
     if (s > 0xffffffffll) {
       big = true;
       s = 0xffffffffll;
@@ -114,7 +126,7 @@ __global__ void k_grid_setup(Control* ctl, uint32_t n, uint64_t capacity) {
     return;
   }
   for (int a = 0; a < 3; ++a) {
-    ctl->lo[a] = ctl->bmin[a];
+    ctl->lo[a] = bmin[a];
     ctl->span[a] = span[a];
   }
   ctl->volume = (uint32_t)vol;

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
     scaled[i] = pressure / (density * density);
     comp = density / prm.rest - 1.0f;
   }
-  wave_max_to(&ctl->comp_bits, comp);
+  wave_max_to(&stripe_of(ctl).comp_bits, comp);
 }
 
 // The force loop of wcsph_substep (step.rs:173-211) with wall_viscous
@@ -122,8 +122,8 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
     VA[i] = make_float4(vx, vy, vz, density);
     speed = sqrtf(dot3(vx, vy, vz, vx, vy, vz));
   }
-  wave_max_to(&ctl->vmax_bits, speed);
-  wave_sum_to(&ctl->held, held);
+  wave_max_to(&stripe_of(ctl).vmax_bits, speed);
+  wave_sum_to(&stripe_of(ctl).held, held);
 }
 
 // Advances `PB` by v*dt into `PA` for particles after the first non-finite

@@ -188,7 +188,7 @@ int32_t World::ensure_particles(size_t n) {
   SFL_HIP(aux_.reserve(n, 0, stream_), "hipMalloc(aux)");
   SFL_HIP(kappa_.reserve(n, 0, stream_), "hipMalloc(kappa)");
   SFL_HIP(slot_rank_.reserve(n, 0, stream_), "hipMalloc(slot ranks)");
-  SFL_HIP(pairs_.reserve(n, 0, stream_), "hipMalloc(pairs)");
+  SFL_HIP(scatter_.reserve(n, 0, stream_), "hipMalloc(scatter)");
   size_t chunks = (n + 511) / 512;
   SFL_HIP(partial_.reserve(std::max<size_t>(chunks, 1), 0, stream_), "hipMalloc(partials)");
   SFL_HIP(pcount_.reserve(std::max<size_t>(chunks, 1), 0, stream_), "hipMalloc(partial counts)");
@@ -581,7 +581,40 @@ StepParams World::params(float dt) const {
   p.wall_cells = wall_owner_.empty() ? 0u : wall_cells_;
   p.cell_start = cell_start_.ptr;
   p.ctl = ctl_;
+  p.pairs = pairs_.ptr;
+  p.pair_counts = pair_counts_.ptr;
+  p.pair_stride = pair_stride_;
+  p.pair_cap = pair_cap_;
   return p;
+}
+
+World::Totals World::totals() const {
+  Totals t{0.0f, 0.0f, 0, 0};
+  uint32_t vmax = 0, comp = 0;
+  for (int k = 0; k < kStripes; ++k) {
+    const Stripe& s = ctl_host_->stripes[k];
+    vmax = std::max(vmax, s.vmax_bits);
+    comp = std::max(comp, s.comp_bits);
+    t.held += s.held;
+    t.pairs_max = std::max(t.pairs_max, s.pairs_max);
+  }
+  std::memcpy(&t.vmax, &vmax, 4);
+  std::memcpy(&t.comp, &comp, 4);
+  return t;
+}
+
+// Lists of `cap` entries for each of n_ particles (column-major, stride n_).
+int32_t World::ensure_pairs(uint32_t cap) {
+  uint32_t stride = (uint32_t)std::max<size_t>(n_, 1);
+  if (cap <= pair_cap_ && stride <= pair_stride_ && pairs_.ptr) return SFL_OK;
+  cap = std::max(cap, pair_cap_);
+  stride = std::max(stride, pair_stride_);
+  pairs_.release();
+  SFL_HIP(pairs_.reserve((size_t)cap * stride, 0, stream_), "hipMalloc(neighbour lists)");
+  SFL_HIP(pair_counts_.reserve(stride, 0, stream_), "hipMalloc(neighbour counts)");
+  pair_cap_ = cap;
+  pair_stride_ = stride;
+  return SFL_OK;
 }
 
 int32_t World::ensure_cells(uint64_t volume) {
@@ -631,11 +664,11 @@ int32_t World::read_control() {
 int32_t World::ensure_fastest() {
   if (vmax_valid_) return SFL_OK;
   launch_step_begin(stream_, ctl_);
-  SFL_HIP(hipMemsetAsync(&ctl_->vmax_bits, 0, sizeof(uint32_t), stream_), "clear vmax");
+  launch_substep_begin(stream_, ctl_);
   launch_fastest(stream_, va_.ptr, (uint32_t)n_, ctl_);
   int32_t st = read_control();
   if (st != SFL_OK) return st;
-  std::memcpy(&vmax_, &ctl_host_->vmax_bits, 4);
+  vmax_ = totals().vmax;
   vmax_valid_ = true;
   return SFL_OK;
 }
@@ -694,10 +727,10 @@ void World::enqueue_sort() {
   launch_count(stream_, pa_.ptr, n, radius_, counts_.ptr, slot_rank_.ptr, ctl_);
   launch_scan(stream_, counts_.ptr, cell_start_.ptr, tile_sums_.ptr, scan_total_.ptr, &ctl_->volume, cell_cap_, 1,
               ctl_);
-  launch_scatter(stream_, pa_.ptr, n, slot_rank_.ptr, cell_start_.ptr, pairs_.ptr, ctl_);
+  launch_scatter(stream_, pa_.ptr, n, slot_rank_.ptr, cell_start_.ptr, scatter_.ptr, ctl_);
   phase_end(SFL_PHASE_SORT);
   phase_begin(SFL_PHASE_REORDER);
-  launch_rank_gather(stream_, pa_.ptr, va_.ptr, n, radius_, pairs_.ptr, cell_start_.ptr, pb_.ptr, vb_.ptr, ctl_);
+  launch_rank_gather(stream_, pa_.ptr, va_.ptr, n, radius_, scatter_.ptr, cell_start_.ptr, pb_.ptr, vb_.ptr, ctl_);
   phase_end(SFL_PHASE_REORDER);
 }
 
@@ -758,9 +791,7 @@ int32_t World::step(float dt, sfl_report* out) {
     int32_t st = settings_.solver == SFL_WCSPH ? wcsph(dt, r) : dfsph(dt, r);
     if (st != SFL_OK) return st;
     r.max_speed = vmax_;
-    float comp;
-    std::memcpy(&comp, &ctl_host_->comp_bits, 4);
-    r.max_compression = comp;
+    r.max_compression = totals().comp;
   }
   if (out) *out = r;
   return SFL_OK;
@@ -793,7 +824,7 @@ int32_t World::wcsph(float dt, sfl_report& r) {
     const Control& c = *ctl_host_;
     if (c.flags & kFlagOverflow) {
       uint32_t completed = c.substeps_done ? c.substeps_done - 1 : 0;
-      held += c.held;
+      held += totals().held;
       done += completed;
       work_substeps_ += (uint64_t)completed * n_;
       if (completed > 0) {
@@ -816,11 +847,11 @@ int32_t World::wcsph(float dt, sfl_report& r) {
       detail_ = "fluid particle";
       return SFL_NON_FINITE;
     }
-    held += c.held;
+    held += totals().held;
     work_substeps_ += (uint64_t)(substeps - done) * n_;
     done = substeps;
   }
-  std::memcpy(&vmax_, &ctl_host_->vmax_bits, 4);
+  vmax_ = totals().vmax;
   vmax_valid_ = true;
   forces_valid_ = keep_forces_;
   r.substeps = substeps;
@@ -836,6 +867,8 @@ int32_t World::dfsph(float dt, sfl_report& r) {
   float spacing = s.spacing, courant = s.courant;
   float pull = length3(s.gravity.x, s.gravity.y, s.gravity.z);
   st = ensure_cells_for_bounds();
+  if (st != SFL_OK) return st;
+  st = ensure_pairs(std::max<uint32_t>(pair_cap_, 64));
   if (st != SFL_OK) return st;
   float left = dt;
   lists_valid_ = false;
@@ -860,6 +893,12 @@ int32_t World::dfsph(float dt, sfl_report& r) {
         prm = params(small);
         continue;
       }
+      if (ctl_host_->flags & kFlagPairs) {
+        st = ensure_pairs(totals().pairs_max + 16);
+        if (st != SFL_OK) return st;
+        prm = params(small);
+        continue;
+      }
       break;
     }
     const Control& c = *ctl_host_;
@@ -877,7 +916,8 @@ int32_t World::dfsph(float dt, sfl_report& r) {
       detail_ = "fluid particle";
       return SFL_NON_FINITE;
     }
-    r.held_back += c.held;
+    Totals tot = totals();
+    r.held_back += tot.held;
     r.iterations += c.iterations[0] + c.iterations[1];
     r.iterations_divergence += c.iterations[0];
     r.iterations_density += c.iterations[1];
@@ -886,7 +926,7 @@ int32_t World::dfsph(float dt, sfl_report& r) {
     work_pushes_ += (uint64_t)(c.iterations[0] + c.iterations[1]) * n_;
     work_rates_ += (uint64_t)c.rates_step * n_;
     left -= small;
-    std::memcpy(&vmax_, &c.vmax_bits, 4);
+    vmax_ = tot.vmax;
   }
   vmax_valid_ = true;
   forces_valid_ = keep_forces_;

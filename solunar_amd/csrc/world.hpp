@@ -73,6 +73,7 @@ class World {
   float mass() const { return mass_; }
   size_t len() const { return n_; }
   size_t wall_len() const { return wall_owner_.size(); }
+  int solve_grid() const { return solve_grid_; }
   const char* detail() const { return detail_.c_str(); }
   void set_detail(const std::string& d) { detail_ = d; }
 
@@ -85,6 +86,12 @@ class World {
   int32_t ensure_cells(uint64_t volume);
   int32_t ensure_cells_for_bounds();
   int32_t read_control();
+  struct Totals {
+    float vmax, comp;
+    uint32_t held, pairs_max;
+  };
+  Totals totals() const;
+  int32_t ensure_pairs(uint32_t cap);
   int32_t settle_walls();
   int32_t wcsph(float dt, sfl_report& r);
   int32_t dfsph(float dt, sfl_report& r);
@@ -132,9 +139,12 @@ class World {
   int solve_grid_ = 0;
   DeviceBuffer<float4> pa_, va_, pb_, vb_;
   DeviceBuffer<float> aux_, kappa_, forces_;
-  DeviceBuffer<uint2> slot_rank_, pairs_;
+  DeviceBuffer<uint2> slot_rank_, scatter_;
   DeviceBuffer<uint32_t> counts_, cell_start_, tile_sums_, scan_total_;
   uint64_t cell_cap_ = 0;
+  DeviceBuffer<uint2> pairs_;
+  DeviceBuffer<uint32_t> pair_counts_;
+  uint32_t pair_cap_ = 0, pair_stride_ = 0;
   DeviceBuffer<double> partial_;
   DeviceBuffer<uint32_t> pcount_;
   DeviceBuffer<float4> walls_d_;

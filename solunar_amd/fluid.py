@@ -75,6 +75,7 @@ def library() -> ctypes.CDLL:
         "sfl_work_read": (i32, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                 ctypes.POINTER(ctypes.c_uint64), i32]),
         "sfl_synchronize": (i32, [P]),
+        "sfl_solve_grid": (i32, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -287,6 +288,9 @@ class FluidWorld:
         self._check(self._lib.sfl_work_read(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                             1 if reset else 0))
         return {"particle_substeps": a.value, "particle_pushes": b.value, "particle_rates": c.value}
+
+    def solve_grid(self) -> int:
+        return int(self._lib.sfl_solve_grid(self._h))
 
     def synchronize(self) -> None:
         self._check(self._lib.sfl_synchronize(self._h))

@@ -245,3 +245,21 @@ def test_a_capped_step_reports_it():
     rg, rc = gpu.step(1 / 60), cpu.step(1 / 60)
     assert rg.capped and rg.substeps == 2 and rg == rc
     assert gpu.state_hash() == cpu.state_hash()
+
+
+@pytest.mark.parametrize("spacing, steps", [(0.01, 3), (0.005, 1)])
+def test_dfsph_dam_break_at_scale_matches(spacing, steps):
+    # 128,000 and 1,024,000 particles: hundreds of blocks meet at the solve's
+    # grid barriers
+    s = replace_settings(scenes.settings_dam_1m(), spacing=spacing)
+    gpu, cpu = pair(s, residual_mode=1, threads=8)
+    for w in (gpu, cpu):
+        scenes.dam_break(w)
+    for k in range(steps):
+        rg, rc = gpu.step(5e-4), cpu.step(5e-4)
+        print(f"step {k}: {rg}", flush=True)
+        assert rg == rc, (k, rg, rc)
+    assert np.array_equal(gpu.ids(), cpu.ids())
+    assert np.array_equal(gpu.positions(), cpu.positions())
+    assert np.array_equal(gpu.velocities(), cpu.velocities())
+    assert np.array_equal(gpu.densities(), cpu.densities())
