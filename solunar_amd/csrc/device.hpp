@@ -68,6 +68,20 @@ struct Control {
 
 __device__ __forceinline__ Stripe& stripe_of(Control* ctl) { return ctl->stripes[blockIdx.x & (kStripes - 1)]; }
 
+// MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own
+// L2. Consecutive sorted particles are spatial neighbours, so a launch over
+// particles maps hardware block b to logical block (b % 8) * (n / 8) + b / 8:
+// the blocks one XCD runs then cover one contiguous range (an x-slab of
+// cells) instead of every eighth stripe of the domain. A bijection on
+// [0, gridDim.x); blocks past the last multiple of 8 keep their index.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+  uint32_t b = blockIdx.x, n = gridDim.x;
+  uint32_t q = n / kXcds;
+  if (b >= q * kXcds) return b;
+  return (b % kXcds) * q + b / kXcds;
+}
+
 // A dense cell table over a box of cells: slot of cell (x, y, z) is
 // ((x - lo.x) * span.y + (y - lo.y)) * span.z + (z - lo.z); particles of slot
 // c are start[c] .. start[c + 1] in the sorted arrays.

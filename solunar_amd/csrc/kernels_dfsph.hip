@@ -23,7 +23,17 @@
 
 namespace sfl {
 
-constexpr uint32_t kSolveThreads = 512;  // one chunk of the stop sum per block round
+constexpr uint32_t kChunk = 512;          // particles of one chunk of the stop sum
+constexpr uint32_t kSolveThreads = 1024;  // threads of a solve block: two chunks at a time
+constexpr uint32_t kSolveWaves = kSolveThreads / 64;
+
+// List entries stream through once per pass; loading them non-temporally
+// keeps them from evicting the neighbour positions and velocities the
+// gathers reuse from L2.
+__device__ __forceinline__ uint2 list_entry(const uint2* lp, size_t at) {
+  unsigned long long e = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(lp + at));
+  return make_uint2((uint32_t)e, (uint32_t)(e >> 32));
+}
 
 __device__ __forceinline__ bool list_aborted(const Control* ctl) {
   return ctl->abort != 0 || (ctl->flags & kFlagPairs) != 0;
@@ -58,7 +68,7 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
                                                        float* __restrict__ alpha, StepParams prm) {
   Control* ctl = prm.ctl;
   if (ctl->abort) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   bool live = i < prm.n;
   float comp = 0.0f;
   uint32_t need = 0;
@@ -122,16 +132,21 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
 constexpr uint64_t kBarrierTicks = 200000000ull;  // 2 s
 
 // Grid-wide barrier of the persistent solve (every block resident). Every
-// wave releases its writes; thread 0 arrives, then polls the generation with
-// relaxed loads (an acquire per poll would invalidate the L2 that the blocks
-// still working are gathering from) and acquires once when it moves. Returns
-// false if it waited past the limit, having flagged the world.
+// wave waits until its own stores have reached the L2 (a block barrier alone
+// does not), then thread 0 alone releases the block's writes at agent scope,
+// arrives, polls the generation with relaxed loads (an acquire per poll
+// would invalidate the L2 the blocks still working gather from) and acquires
+// once when it moves; one fence pair per block instead of one per wave cut
+// the barrier from 88 to about 32 microseconds at 512 blocks. Returns false
+// if it waited past the limit, having flagged the world.
 __device__ bool grid_barrier(Control* ctl) {
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   __shared__ int ok_s;
   if (threadIdx.x == 0) {
     int ok = 1;
+    __threadfence();
     uint32_t gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t arrived = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == gridDim.x - 1) {
@@ -147,8 +162,8 @@ __device__ bool grid_barrier(Control* ctl) {
           break;
         }
       }
-      __atomic_thread_fence(__ATOMIC_ACQUIRE);
     }
+    __threadfence();
     ok_s = ok;
   }
   __syncthreads();
@@ -161,6 +176,7 @@ struct SolveArgs {
   float4* V;            // velocities (updated in place); w = density
   const float* alpha;   // factors, sign bit = surface particle
   float* kappa;         // stiffness of the last rate pass
+  float* errors;        // error of the last rate pass
   double* partial;      // per chunk error sums
   uint32_t* pcount;     // per chunk count of errors > 0
   uint32_t chunks;
@@ -172,16 +188,21 @@ struct SolveArgs {
 
 // solve (src/fluid/dfsph.rs:222-275) with density_rate (dfsph.rs:166-181) and
 // push_apart (dfsph.rs:184-216). Block b takes chunks b, b + grid, ...; its
-// thread t owns particle 512c + t, which is lane t % 64, term k = t / 64 of
+// each half of the block walks every other chunk of the block's list; in a
+// chunk, thread t owns particle 512c + t, which is lane t % 64, term k = t / 64 of
 // the stop sum's chunk.
-__global__ __launch_bounds__(512) void k_dfsph_solve(SolveArgs a) {
+__global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
   if (list_aborted(ctl)) return;
+  // the chunks of XCD x are the x-th eighth of the sorted particles (the
+  // grid is a multiple of 8 blocks); its blocks share them round-robin
+  const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, rank = blockIdx.x / kXcds;
+  const uint32_t c_begin = (uint32_t)((uint64_t)a.chunks * xcd / kXcds);
+  const uint32_t c_end = (uint32_t)((uint64_t)a.chunks * (xcd + 1) / kXcds);
   const uint32_t t = threadIdx.x;
-  const uint32_t lane = t & 63u, term = t >> 6;
-  __shared__ double err_s[8][64];
-  __shared__ uint32_t cnt_s[8][64];
+  const uint32_t lane = t & 63u, wave = t >> 6;
+  const uint32_t half = t / kChunk, tc = t % kChunk;
   __shared__ double slot[256];
   __shared__ uint32_t slot_count[256];
   const float dt = prm.dt;
@@ -192,8 +213,8 @@ __global__ __launch_bounds__(512) void k_dfsph_solve(SolveArgs a) {
   uint32_t it = 0;
   for (;;) {
     // rate pass: stiffness and error of every particle
-    for (uint32_t c = blockIdx.x; c < a.chunks; c += gridDim.x) {
-      uint32_t i = c * kSolveThreads + t;
+    for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
+      uint32_t i = c * kChunk + tc;
       float error = 0.0f;
       if (i < prm.n) {
         float4 at = P[i];
@@ -203,39 +224,58 @@ __global__ __launch_bounds__(512) void k_dfsph_solve(SolveArgs a) {
         uint32_t nf = counts & 0xffffu, nw = counts >> 16;
         const uint2* lp = prm.pairs + i;
         const size_t stride = prm.pair_stride;
+        // Entries go four at a time: the next four list loads are issued
+        // before the current four neighbours are gathered and summed in list
+        // order. Slots past the list point at the particle itself, which
+        // keeps every load in bounds; their terms are skipped.
+        uint2 nxt[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
         uint32_t k = 0;
-        // four entries' gathers in flight, then the terms in list order
-        for (; k + 4 <= nf; k += 4) {
-          uint2 e[4];
+        for (; k < nf; k += 4) {
+          uint2 cur[4];
           float4 pj[4], vj[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) e[u] = lp[(k + u) * stride];
-#pragma unroll
           for (int u = 0; u < 4; ++u) {
-            pj[u] = P[e[u].x];
-            vj[u] = V[e[u].x];
+            cur[u] = nxt[u];
+            nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(i, 0u);
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            float f = __uint_as_float(e[u].y);
-            float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
-            rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
+            pj[u] = P[cur[u].x];
+            vj[u] = V[cur[u].x];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (k + u < nf) {
+              float f = __uint_as_float(cur[u].y);
+              float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
+              rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
+            }
           }
         }
-        for (; k < nf; ++k) {
-          uint2 e = lp[k * stride];
-          float4 pj = P[e.x];
-          float4 vj = V[e.x];
-          float f = __uint_as_float(e.y);
-          float gx = (at.x - pj.x) * f, gy = (at.y - pj.y) * f, gz = (at.z - pj.z) * f;
-          rate += dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, gx, gy, gz);
-        }
-        for (; k < nf + nw; ++k) {
-          uint2 e = lp[k * stride];
-          float4 pw = W[e.x];
-          float f = __uint_as_float(e.y);
-          float gx = (at.x - pw.x) * f, gy = (at.y - pw.y) * f, gz = (at.z - pw.z) * f;
-          rate += dot3(vi.x, vi.y, vi.z, gx, gy, gz);
+        // walls the same way (a particle at a corner lists ~40 of them)
+        const uint32_t nt = nf + nw;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) nxt[u] = nf + u < nt ? list_entry(lp, (nf + u) * stride) : make_uint2(0u, 0u);
+        for (k = nf; k < nt; k += 4) {
+          uint2 cur[4];
+          float4 pw[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            cur[u] = nxt[u];
+            nxt[u] = k + 4 + u < nt ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) pw[u] = W[cur[u].x];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (k + u < nt) {
+              float f = __uint_as_float(cur[u].y);
+              float gx = (at.x - pw[u].x) * f, gy = (at.y - pw[u].y) * f, gz = (at.z - pw[u].z) * f;
+              rate += dot3(vi.x, vi.y, vi.z, gx, gy, gz);
+            }
+          }
         }
         rate = rate * mass;
         float fa = a.alpha[i];
@@ -250,16 +290,23 @@ __global__ __launch_bounds__(512) void k_dfsph_solve(SolveArgs a) {
           error = dt * r / prm.rest;
         }
         a.kappa[i] = target * fabsf(fa);
+        a.errors[i] = error;
       }
-      err_s[term][lane] = (double)error;
-      cnt_s[term][lane] = error > 0.0f ? 1u : 0u;
-      __syncthreads();
-      if (term == 0) {
+    }
+    // the stop sum of this block's chunks, wave w taking chunks w, w + 8, ...
+    // of its list; lane l adds errors 512c + l + 64k in k order, as defined
+    __syncthreads();
+    {
+      uint32_t mine = 0;
+      for (uint32_t c = c_begin + rank; c < c_end; c += per_xcd, ++mine) {
+        if (mine % kSolveWaves != wave) continue;
         double acc = 0.0;
         uint32_t cnt = 0;
         for (uint32_t k = 0; k < 8; ++k) {
-          acc += err_s[k][lane];
-          cnt += cnt_s[k][lane];
+          uint32_t i = c * kChunk + k * 64u + lane;
+          float e = i < prm.n ? a.errors[i] : 0.0f;
+          acc += (double)e;
+          cnt += e > 0.0f ? 1u : 0u;
         }
         for (int h = 32; h >= 1; h >>= 1) {
           acc = acc + __shfl_down(acc, h, 64);
@@ -270,7 +317,6 @@ __global__ __launch_bounds__(512) void k_dfsph_solve(SolveArgs a) {
           a.pcount[c] = cnt;
         }
       }
-      __syncthreads();
     }
     if (!grid_barrier(ctl)) return;
     // every block folds the chunk sums the same way
@@ -307,8 +353,8 @@ __global__ __launch_bounds__(512) void k_dfsph_solve(SolveArgs a) {
     bool settled = it >= a.min_iterations && average <= (double)a.tolerance;
     if (settled || it >= a.max_iterations) break;
     // push pass
-    for (uint32_t c = blockIdx.x; c < a.chunks; c += gridDim.x) {
-      uint32_t i = c * kSolveThreads + t;
+    for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
+      uint32_t i = c * kChunk + tc;
       if (i >= prm.n) continue;
       float4 at = P[i];
       float own = a.kappa[i];
@@ -317,47 +363,60 @@ __global__ __launch_bounds__(512) void k_dfsph_solve(SolveArgs a) {
       uint32_t nf = counts & 0xffffu, nw = counts >> 16;
       const uint2* lp = prm.pairs + i;
       const size_t stride = prm.pair_stride;
+      uint2 nxt[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
       uint32_t k = 0;
-      for (; k + 4 <= nf; k += 4) {
-        uint2 e[4];
+      for (; k < nf; k += 4) {
+        uint2 cur[4];
         float4 pj[4];
         float kj[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) e[u] = lp[(k + u) * stride];
-#pragma unroll
         for (int u = 0; u < 4; ++u) {
-          pj[u] = P[e[u].x];
-          kj[u] = a.kappa[e[u].x];
+          cur[u] = nxt[u];
+          nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(i, 0u);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          float f = __uint_as_float(e[u].y);
-          float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
-          float k2 = mass * (own + kj[u]);
-          cx = cx - gx * k2;
-          cy = cy - gy * k2;
-          cz = cz - gz * k2;
+          pj[u] = P[cur[u].x];
+          kj[u] = a.kappa[cur[u].x];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (k + u < nf) {
+            float f = __uint_as_float(cur[u].y);
+            float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
+            float k2 = mass * (own + kj[u]);
+            cx = cx - gx * k2;
+            cy = cy - gy * k2;
+            cz = cz - gz * k2;
+          }
         }
       }
-      for (; k < nf; ++k) {
-        uint2 e = lp[k * stride];
-        float4 pj = P[e.x];
-        float f = __uint_as_float(e.y);
-        float gx = (at.x - pj.x) * f, gy = (at.y - pj.y) * f, gz = (at.z - pj.z) * f;
-        float k2 = mass * (own + a.kappa[e.x]);
-        cx = cx - gx * k2;
-        cy = cy - gy * k2;
-        cz = cz - gz * k2;
-      }
-      for (; k < nf + nw; ++k) {
-        uint2 e = lp[k * stride];
-        float4 pw = W[e.x];
-        float f = __uint_as_float(e.y);
-        float gx = (at.x - pw.x) * f, gy = (at.y - pw.y) * f, gz = (at.z - pw.z) * f;
-        float k2 = mass * own;
-        cx = cx - gx * k2;
-        cy = cy - gy * k2;
-        cz = cz - gz * k2;
+      const uint32_t nt = nf + nw;
+      const float kw = mass * own;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) nxt[u] = nf + u < nt ? list_entry(lp, (nf + u) * stride) : make_uint2(0u, 0u);
+      for (k = nf; k < nt; k += 4) {
+        uint2 cur[4];
+        float4 pw[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          cur[u] = nxt[u];
+          nxt[u] = k + 4 + u < nt ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pw[u] = W[cur[u].x];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (k + u < nt) {
+            float f = __uint_as_float(cur[u].y);
+            float gx = (at.x - pw[u].x) * f, gy = (at.y - pw[u].y) * f, gz = (at.z - pw[u].z) * f;
+            cx = cx - gx * kw;
+            cy = cy - gy * kw;
+            cz = cz - gz * kw;
+          }
+        }
       }
       float4 vi = V[i];
       vi.x = vi.x + cx * dt;
@@ -384,7 +443,7 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
                                                        float* __restrict__ forces) {
   Control* ctl = prm.ctl;
   if (list_aborted(ctl)) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= prm.n) return;
   const float4* __restrict__ W = prm.walls;
   float4 at = PB[i];
@@ -465,7 +524,7 @@ __global__ __launch_bounds__(256) void k_dfsph_advance(const float4* __restrict_
                                                        float4* __restrict__ VKEEP) {
   Control* ctl = prm.ctl;
   if (list_aborted(ctl)) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   bool live = i < prm.n;
   float speed = 0.0f;
   uint32_t held = 0;
@@ -496,7 +555,7 @@ __global__ __launch_bounds__(256) void k_dfsph_repair(const float4* __restrict__
                                                       float4* __restrict__ VA) {
   Control* ctl = prm.ctl;
   uint32_t bad = ctl->bad_index;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= prm.n || i < bad) return;
   PA[i] = PB[i];
   if (i > bad) VA[i] = VKEEP[i];
@@ -510,10 +569,11 @@ void launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alph
 }
 
 // Blocks of the persistent solve. Every block must be resident at once for
-// the grid barrier; the occupancy query alone over-promised (a grid of its
-// size never met at the first barrier), so the grid also stays within the
-// device's resident threads per CU and at most two blocks per CU.
-int solve_grid_limit() {
+// the grid barrier: at most what the occupancy query and the resident
+// threads per CU allow, and one 1024-thread block per CU by default: the
+// barrier's cost grows with the blocks that meet at it, and one block per
+// CU ran the 1M-particle solve fastest.
+int solve_grid_limit(int blocks_per_cu) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   int cus = 0, threads = 0;
@@ -524,13 +584,14 @@ int solve_grid_limit() {
                                                    kSolveThreads, 0) != hipSuccess)
     return 0;
   per = per < threads / (int)kSolveThreads ? per : threads / (int)kSolveThreads;
-  per = per < 2 ? per : 2;
+  int cap = blocks_per_cu > 0 && blocks_per_cu <= 2 ? blocks_per_cu : 1;
+  per = per < cap ? per : cap;
   return per * cus;
 }
 
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
-                              float* kappa, double* partial, uint32_t* pcount, int which, float tolerance,
-                              uint32_t min_it, uint32_t max_it, int grid) {
+                              float* kappa, float* errors, double* partial, uint32_t* pcount, int which,
+                              float tolerance, uint32_t min_it, uint32_t max_it, int grid) {
   if (prm.n == 0) return hipSuccess;
   SolveArgs a;
   a.prm = prm;
@@ -538,15 +599,21 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   a.V = V;
   a.alpha = alpha;
   a.kappa = kappa;
+  a.errors = errors;
   a.partial = partial;
   a.pcount = pcount;
-  a.chunks = (prm.n + kSolveThreads - 1) / kSolveThreads;
+  a.chunks = (prm.n + kChunk - 1) / kChunk;
   a.which = which;
   a.tolerance = tolerance;
   a.min_iterations = min_it;
   a.max_iterations = max_it;
-  uint32_t blocks = (uint32_t)grid < a.chunks ? (uint32_t)grid : a.chunks;
-  if (blocks == 0) blocks = 1;
+  // a multiple of 8 blocks so every XCD gets the same share; at least one
+  // chunk per XCD group or the small grid of a small world
+  uint32_t blocks = (uint32_t)grid;
+  uint32_t want = (a.chunks + kXcds - 1) / kXcds * kXcds;
+  if (want < blocks) blocks = want;
+  blocks = blocks / kXcds * kXcds;
+  if (blocks == 0) blocks = kXcds;
   // A plain launch: the grid is sized to fit the device at once (two blocks
   // per CU at most), so every block is resident for the barrier; it also
   // keeps the launch capturable in a graph.

@@ -47,7 +47,7 @@ __global__ void k_substep_begin(Control* ctl) {
 // fastest (src/fluid/step.rs:92-96): max |v| over the particles.
 __global__ __launch_bounds__(256) void k_fastest(const float4* __restrict__ V, uint32_t n, Control* ctl) {
   float most = 0.0f;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float4 v = V[i];
     most = rmax(most, sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z)));
   }
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ P, ui
   if (ctl->abort) return;
   int32_t mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
   int32_t mx[3] = {(int32_t)0x80000000u, (int32_t)0x80000000u, (int32_t)0x80000000u};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float4 p = P[i];
     int32_t c[3] = {cell_coord(p.x, radius), cell_coord(p.y, radius), cell_coord(p.z, radius)};
     for (int a = 0; a < 3; ++a) {
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_count(const float4* __restrict__ P, uin
                                                uint32_t* __restrict__ counts, uint2* __restrict__ slot_rank,
                                                Control* ctl) {
   if (ctl->abort) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
   uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_scatter(const float4* __restrict__ P, u
                                                  const uint32_t* __restrict__ start, uint2* __restrict__ pairs,
                                                  const Control* ctl) {
   if (ctl->abort) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint2 sr = slot_rank[i];
   uint32_t id = id_of(P[i]);
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_rank_gather(const float4* __restrict__ 
                                                      const uint32_t* __restrict__ start, float4* __restrict__ PB,
                                                      float4* __restrict__ VB, const Control* ctl) {
   if (ctl->abort) return;
-  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
   if (q >= n) return;
   int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
   uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
@@ -307,7 +307,7 @@ __device__ __forceinline__ void for_each_near(const float4* __restrict__ P, cons
 // kind 1: walls. Counts per particle.
 __global__ __launch_bounds__(256) void k_list_count(const float4* __restrict__ lp, StepParams prm,
                                                     uint32_t* __restrict__ counts, int kind) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= prm.n) return;
   uint32_t cnt = 0;
   float4 at = lp[i];
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_list_count(const float4* __restrict__ l
 __global__ __launch_bounds__(256) void k_list_emit(const float4* __restrict__ lp, StepParams prm,
                                                    const uint32_t* __restrict__ list_start,
                                                    uint32_t* __restrict__ items, int kind) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= prm.n) return;
   uint32_t at_k = list_start[i];
   float4 at = lp[i];
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_list_density(const float4* __restrict__
                                                       const uint32_t* __restrict__ fitems,
                                                       const uint32_t* __restrict__ wstart,
                                                       const uint32_t* __restrict__ witems) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= prm.n) return;
   float4 at = P[i];
   float sum = 0.0f;

@@ -16,7 +16,7 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
                                                        float* __restrict__ scaled, StepParams prm) {
   Control* ctl = prm.ctl;
   if (ctl->abort) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   bool live = i < prm.n;
   float comp = 0.0f;
   if (live) {
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
                                                      float* __restrict__ forces) {
   Control* ctl = prm.ctl;
   if (ctl->abort) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   bool live = i < prm.n;
   float speed = 0.0f;
   uint32_t held = 0;
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_wcsph_repair(const float4* __restrict__
                                                       float4* __restrict__ PA, float4* __restrict__ VA) {
   Control* ctl = prm.ctl;
   uint32_t bad = ctl->bad_index;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= prm.n || i < bad) return;
   float4 at = PB[i];
   if (i == bad) {

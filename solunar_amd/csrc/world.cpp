@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -157,7 +158,10 @@ int32_t World::init_device() {
           "hipHostMalloc(control)");
   std::memset(ctl_host_, 0, sizeof(Control));
   SFL_HIP(scan_total_.reserve(1, 0, stream_), "hipMalloc(scan total)");
-  solve_grid_ = solve_grid_limit();
+  // SOLUNAR_SOLVE_BLOCKS_PER_CU (1..3) narrows the persistent solve's grid,
+  // a tuning knob; the default is the widest grid that stays resident
+  const char* knob = std::getenv("SOLUNAR_SOLVE_BLOCKS_PER_CU");
+  solve_grid_ = solve_grid_limit(knob ? std::atoi(knob) : 0);
   if (solve_grid_ <= 0) {
     detail_ = "occupancy query for the persistent solve failed";
     return SFL_DEVICE_ERROR;
@@ -187,6 +191,7 @@ int32_t World::ensure_particles(size_t n) {
   SFL_HIP(vb_.reserve(n, 0, stream_), "hipMalloc(sorted velocities)");
   SFL_HIP(aux_.reserve(n, 0, stream_), "hipMalloc(aux)");
   SFL_HIP(kappa_.reserve(n, 0, stream_), "hipMalloc(kappa)");
+  SFL_HIP(errors_.reserve(n, 0, stream_), "hipMalloc(errors)");
   SFL_HIP(slot_rank_.reserve(n, 0, stream_), "hipMalloc(slot ranks)");
   SFL_HIP(scatter_.reserve(n, 0, stream_), "hipMalloc(scatter)");
   size_t chunks = (n + 511) / 512;
@@ -754,7 +759,7 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm);
   phase_end(SFL_PHASE_DENSITY);
   phase_begin(SFL_PHASE_SOLVE);
-  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, partial_.ptr, pcount_.ptr, 0,
+  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, partial_.ptr, pcount_.ptr, 0,
                              s.divergence_tolerance, s.min_iterations, s.max_iterations, solve_grid_),
           "cooperative launch (divergence solve)");
   phase_end(SFL_PHASE_SOLVE);
@@ -762,7 +767,7 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   launch_dfsph_viscous(stream_, pb_.ptr, vb_.ptr, prm, va_.ptr, keep_forces_ ? forces_.ptr : nullptr);
   phase_end(SFL_PHASE_VISCOUS);
   phase_begin(SFL_PHASE_SOLVE);
-  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, va_.ptr, aux_.ptr, kappa_.ptr, partial_.ptr, pcount_.ptr, 1,
+  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, va_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, partial_.ptr, pcount_.ptr, 1,
                              s.density_tolerance, s.min_iterations, s.max_iterations, solve_grid_),
           "cooperative launch (density solve)");
   phase_end(SFL_PHASE_SOLVE);

@@ -138,7 +138,7 @@ class World {
   Control* ctl_host_ = nullptr;  // pinned host mirror
   int solve_grid_ = 0;
   DeviceBuffer<float4> pa_, va_, pb_, vb_;
-  DeviceBuffer<float> aux_, kappa_, forces_;
+  DeviceBuffer<float> aux_, kappa_, errors_, forces_;
   DeviceBuffer<uint2> slot_rank_, scatter_;
   DeviceBuffer<uint32_t> counts_, cell_start_, tile_sums_, scan_total_;
   uint64_t cell_cap_ = 0;
