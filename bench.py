@@ -77,7 +77,19 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 256 MiB device copy, the known-size kernel that scales the rocprofv3 byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4}))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: the fluid step here is built and timed on a single MI355X "
               "(multi-GPU sharding is a design note, not a code path)", file=sys.stderr)
@@ -129,12 +141,15 @@ def main() -> int:
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     step_bytes = (SUBSTEP_FIXED_BYTES * work["particle_substeps"] + RATE_BYTES * work["particle_rates"]
                   + PUSH_BYTES * work["particle_pushes"])
+    # HBM bytes per launch from the committed rocprofv3 counter summary
+    # (FETCH_SIZE/WRITE_SIZE per Jacobi iteration, scaled by a plain copy),
+    # times this run's iterations per launch
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_dam_1m.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and dom == "solve" and dom_launches:
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
-
+            per_iteration = json.load(f)["bytes_per_iteration"]
+        traffic = per_iteration * (work["particle_pushes"] / n) / dom_launches
     out = {
         "metric": "particle-steps/s at 1M DFSPH",
         "value": value,
