@@ -77,6 +77,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["dam_1m", "dam_16m"], default="dam_1m",
+                    help="dam_16m: 16,000,000 fluid particles at spacing 0.002 with 200 sphere obstacles")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device copy, the known-size kernel that scales the rocprofv3 byte counters")
     args = ap.parse_args()
@@ -98,11 +100,12 @@ def main() -> int:
     from solunar_amd import scenes
     from solunar_amd.fluid import FluidWorld
 
-    settings = scenes.settings_dam_1m()
+    large = args.workload == "dam_16m"
+    settings = scenes.settings_dam_16m() if large else scenes.settings_dam_1m()
     world = FluidWorld(settings)
-    n = scenes.dam_break(world)
+    n = scenes.dam_with_obstacles(world) if large else scenes.dam_break(world)
     walls = len(world.wall_positions())
-    log(f"dam_1m: {n} fluid, {walls} wall particles; warmup {args.warmup} steps")
+    log(f"{args.workload}: {n} fluid, {walls} wall particles; warmup {args.warmup} steps")
     for k in range(args.warmup):
         r = world.step(DT)
         log(f"warmup step {k}: {r.substeps} substeps, {r.iterations} iterations")
@@ -126,6 +129,7 @@ def main() -> int:
     work = world.work_read(reset=True)
     world.profile(False)
     solve_blocks = world.solve_grid()
+    memory = world.memory_bytes()
     world.close()  # free device memory before the runtime (or a profiler) shuts down
     log(f"timed: {elapsed:.3f} s, {substeps} substeps, {iterations} iterations")
 
@@ -146,12 +150,14 @@ def main() -> int:
     # times this run's iterations per launch
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_dam_1m.json")
+    if large:
+        tpath = ""
     if os.path.exists(tpath) and dom == "solve" and dom_launches:
         with open(tpath) as f:
             per_iteration = json.load(f)["bytes_per_iteration"]
         traffic = per_iteration * (work["particle_pushes"] / n) / dom_launches
     out = {
-        "metric": "particle-steps/s at 1M DFSPH",
+        "metric": "particle-steps/s at 1M DFSPH" if not large else "particle-steps/s at 16M DFSPH with obstacles",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -163,8 +169,11 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "dam_1m: DFSPH dam break, 1,024,000 fluid + 832,064 wall particles, spacing "
-                               "0.005 m, step 5e-4 s, divergence tol 1e-3, density tol 1e-4",
+        "config": {"workload": ("dam_1m: DFSPH dam break, 1,024,000 fluid + 832,064 wall particles, spacing "
+                                "0.005 m, step 5e-4 s, divergence tol 1e-3, density tol 1e-4") if not large else
+                               ("dam_16m: DFSPH dam break, 16,000,000 fluid particles, spacing 0.002 m, container "
+                                "walls + 200 sphere obstacles (radius 0.02-0.08 m) as boundary particles, step 5e-4 s"),
+                   "device_memory_bytes": memory, "device_memory_fraction_of_288GB": memory / 288e9,
                    "fluid_particles": n, "wall_particles": walls, "substeps": substeps,
                    "jacobi_iterations": iterations, "solve_blocks": solve_blocks},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -175,7 +184,7 @@ def main() -> int:
                           "unit": "GB/s", "bytes": step_bytes},
         "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
     }
-    if not args.no_cpu:
+    if not args.no_cpu and not large:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.warmup)
     else:

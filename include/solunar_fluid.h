@@ -221,7 +221,11 @@ int32_t sfl_work_read(sfl_world* world, uint64_t* particle_substeps,
                       uint64_t* particle_pushes, uint64_t* particle_rates,
                       int32_t reset);
 
-/* Blocks of 512 threads the persistent DFSPH solve runs on (every one
+/* Device memory the world holds, in bytes (particle state, sort and cell
+ * tables, neighbour lists, walls). */
+size_t sfl_memory_bytes(const sfl_world* world);
+
+/* Blocks of 1024 threads the persistent DFSPH solve runs on (every one
  * resident for its grid barrier). */
 int32_t sfl_solve_grid(const sfl_world* world);
 

@@ -1147,6 +1147,14 @@ int32_t World::state_hash(uint64_t* out) {
   return SFL_OK;
 }
 
+size_t World::device_bytes() const {
+  auto b = [](const auto& buf) { return buf.cap * sizeof(*buf.ptr); };
+  return sizeof(Control) + b(pa_) + b(va_) + b(pb_) + b(vb_) + b(aux_) + b(kappa_) + b(errors_) + b(forces_) +
+         b(slot_rank_) + b(scatter_) + b(counts_) + b(cell_start_) + b(tile_sums_) + b(scan_total_) + b(pairs_) +
+         b(pair_counts_) + b(partial_) + b(pcount_) + b(walls_d_) + b(wall_start_d_) + b(fstart_) + b(fitems_) +
+         b(wstart_) + b(witems_) + b(list_counts_);
+}
+
 int32_t World::synchronize() {
   DeviceGuard g(device_);
   SFL_HIP(hipStreamSynchronize(stream_), "synchronize");

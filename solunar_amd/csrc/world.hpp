@@ -74,6 +74,7 @@ class World {
   size_t len() const { return n_; }
   size_t wall_len() const { return wall_owner_.size(); }
   int solve_grid() const { return solve_grid_; }
+  size_t device_bytes() const;
   const char* detail() const { return detail_.c_str(); }
   void set_detail(const std::string& d) { detail_ = d; }
 

@@ -76,6 +76,7 @@ def library() -> ctypes.CDLL:
                                 ctypes.POINTER(ctypes.c_uint64), i32]),
         "sfl_synchronize": (i32, [P]),
         "sfl_solve_grid": (i32, [P]),
+        "sfl_memory_bytes": (sz, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -288,6 +289,9 @@ class FluidWorld:
         self._check(self._lib.sfl_work_read(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                             1 if reset else 0))
         return {"particle_substeps": a.value, "particle_pushes": b.value, "particle_rates": c.value}
+
+    def memory_bytes(self) -> int:
+        return int(self._lib.sfl_memory_bytes(self._h))
 
     def solve_grid(self) -> int:
         return int(self._lib.sfl_solve_grid(self._h))

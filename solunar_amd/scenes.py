@@ -77,3 +77,47 @@ def shaken_positions(count: int) -> np.ndarray:
 def shaken(world, count: int):
     world.add_particles(shaken_positions(count))
     return world
+
+
+def sphere_points(center, radius: float, spacing: float, reach: float) -> np.ndarray:
+    """Boundary points of a sphere the way the reference samples a body
+    (shape_points, src/fluid/couple.rs:452-507): lattice points `spacing`
+    apart, centred on the body, kept when 0 <= radius - |p| <= reach; then
+    moved to `center`. f32 throughout."""
+    f = np.float32
+    sp, r, reach = f(spacing), f(radius), f(reach)
+    cells = int(min(max(int(np.floor(f(2.0) * r / sp)), 0), 1 << 12))
+    start = -(f(cells) - f(1.0)) * sp / f(2.0)
+    idx = np.arange(cells, dtype=np.float32)
+    lat = start + idx * sp
+    # loop order y, z, x as in the reference
+    y, z, x = np.meshgrid(lat, lat, lat, indexing="ij")
+    at = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.float32)
+    length = np.sqrt((at[:, 0] * at[:, 0] + at[:, 1] * at[:, 1]) + at[:, 2] * at[:, 2]).astype(np.float32)
+    depth = (r - length).astype(np.float32)
+    keep = (depth >= 0) & (depth <= reach)
+    return (at[keep] + np.asarray(center, dtype=np.float32)).astype(np.float32)
+
+
+def obstacle_field(count: int, spacing: float, seed: int = 7, region=((0.5, 0.05, 0.05), (1.5, 0.8, 0.75)),
+                   radii=(0.02, 0.08)) -> np.ndarray:
+    """`count` spheres with radii uniform in `radii`, centres uniform in
+    `region` (downstream of the water block), sampled with shell depth of one
+    kernel radius (2 spacings); all their points in one array."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.asarray(region[0]), np.asarray(region[1])
+    parts = []
+    for _ in range(count):
+        r = rng.uniform(*radii)
+        c = rng.uniform(lo + r, hi - r)
+        parts.append(sphere_points(c, r, spacing, 2.0 * spacing))
+    return np.concatenate(parts) if parts else np.zeros((0, 3), np.float32)
+
+
+def dam_with_obstacles(world, obstacles: int = 200, seed: int = 7) -> int:
+    """The dam break plus `obstacles` spheres added as one boundary."""
+    n = dam_break(world)
+    pts = obstacle_field(obstacles, world.settings.spacing, seed)
+    if len(pts):
+        world.add_boundary_points(pts)
+    return n

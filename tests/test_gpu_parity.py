@@ -263,3 +263,20 @@ def test_dfsph_dam_break_at_scale_matches(spacing, steps):
     assert np.array_equal(gpu.positions(), cpu.positions())
     assert np.array_equal(gpu.velocities(), cpu.velocities())
     assert np.array_equal(gpu.densities(), cpu.densities())
+
+
+@pytest.mark.parametrize("solver", [WCSPH, DFSPH])
+def test_sphere_obstacles_as_boundary_points_match(solver):
+    # the 16M configuration's obstacles at a coarse spacing: 200 spheres added
+    # through sfl_add_boundary_points, both solvers
+    base = scenes.settings_dam_small() if solver == WCSPH else scenes.settings_dam_1m()
+    s = replace_settings(base, spacing=0.02)
+    gpu, cpu = pair(s, residual_mode=1)
+    for w in (gpu, cpu):
+        scenes.dam_with_obstacles(w, obstacles=200, seed=3)
+    assert np.array_equal(gpu.wall_positions(), cpu.wall_positions())
+    dt = 1e-4 if solver == WCSPH else 5e-4
+    for k in range(10):
+        rg, rc = gpu.step(dt), cpu.step(dt)
+        assert rg == rc, (k, rg, rc)
+    assert gpu.state_hash() == cpu.state_hash()
