@@ -11,7 +11,13 @@ Printed: one JSON line with particle-steps/s, the HBM roofline of the
 dominant kernel (algorithmic bytes from HIP events on the world's stream over
 the timed region) and the CPU oracle timed on this host on a bounded sample.
 
+With --workload batch_256: 256 WCSPH dam breaks (spacing 0.02 m, block sides
+uniform in [0.2, 0.8] m, 2,000-64,000 particles each, 3.86M in all) stepped
+as one batch, `steps` steps of 1e-4 s in one graph launch, next to the same
+256 worlds stepped one by one on the single-world path.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
+                       [--workload dam_1m|dam_16m|batch_256]
 """
 from __future__ import annotations
 
@@ -70,6 +76,92 @@ def cpu_baseline(steps: int, warmup: int):
     }
 
 
+# Batched WCSPH: algorithmic bytes per fluid particle and substep of each
+# kernel group (FP32 state in float4 pairs (x, id) and (v, rho))
+#   sort    : bounds 16, count 16 + 8, scatter 8 + 16 + 8, gather 8 + 32 + 32 = 144
+#   density : x 16 read, rho 4 + p/rho^2 4 written                            = 24
+#   force   : x 16 + v 16 + p/rho^2 4 read, x 16 + v 16 written                = 68
+BATCH_BYTES = {"sort": 144, "density": 24, "force": 68}
+BATCH_DT = 1e-4
+
+
+def run_batch(args) -> dict:
+    from solunar_amd import scenes
+    from solunar_amd.fluid import FluidBatch, FluidWorld
+
+    worlds = scenes.batch_worlds(FluidWorld, 256)
+    sizes = [len(w) for w in worlds]
+    n = sum(sizes)
+    log(f"batch_256: {len(worlds)} scenes, {n} fluid particles ({min(sizes)}..{max(sizes)} per scene)")
+    batch = FluidBatch(worlds)
+    tuning = batch.autotune(BATCH_DT, repeats=3)
+    log(f"tuned: sort {tuning['sort_block']}, density {tuning['density_block']}, force {tuning['force_block']}")
+    if args.warmup:
+        batch.step(BATCH_DT, args.warmup)
+    batch.prepare(args.steps)
+    batch.synchronize()
+    log(f"timing {args.steps} batched steps ({batch.graph_nodes()} graph nodes)")
+    t0 = time.perf_counter()
+    reports = batch.step(BATCH_DT, args.steps)
+    elapsed = time.perf_counter() - t0
+    substeps = sum(r.substeps for r in reports)
+    nodes = batch.graph_nodes()
+    memory = batch.memory_bytes()
+    tuning_after = batch.autotune(BATCH_DT, repeats=3)
+    batch.close()
+    value = n * args.steps / elapsed
+    log(f"batched: {elapsed:.3f} s, {value:.3e} particle-steps/s")
+    # the same scenes on the single-world path, one world after another
+    per_steps = max(1, min(args.steps, args.per_scene_steps))
+    for w in worlds:
+        for _ in range(args.warmup):
+            w.step(BATCH_DT)
+        w.synchronize()
+    log(f"timing {per_steps} steps of each of the {len(worlds)} worlds alone")
+    t0 = time.perf_counter()
+    for k, w in enumerate(worlds):
+        for _ in range(per_steps):
+            w.step(BATCH_DT)
+        if k % 64 == 63:
+            log(f"world {k + 1}/{len(worlds)}")
+    for w in worlds:
+        w.synchronize()
+    alone = time.perf_counter() - t0
+    alone_value = n * per_steps / alone
+    for w in worlds:
+        w.close()
+    group_ms = {g: tuning_after[f"{g}_ms"][tuning_after["blocks"].index(tuning_after[f"{g}_block"])]
+                for g in BATCH_BYTES}
+    dom = max(group_ms, key=group_ms.get)
+    dom_bytes = BATCH_BYTES[dom] * n
+    achieved = dom_bytes / (group_ms[dom] * 1e-3) / 1e9
+    return {
+        "metric": "particle-steps/s, 256 batched WCSPH scenes",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batch_256: 256 WCSPH dam breaks, spacing 0.02 m, blocks U[0.2, 0.8] m per axis "
+                               "with 2,000-64,000 particles, 1.6 x 1.6 x 0.8 m containers, step 1e-4 s",
+                   "fluid_particles": n, "scenes": len(worlds), "substeps": substeps, "graph_nodes": nodes,
+                   "device_memory_bytes": memory, "tuning": tuning},
+        "per_scene_launches": {"value": alone_value, "unit": "particle-steps/s", "steps": per_steps,
+                               "seconds": alone, "speedup": value / alone_value},
+        "roofline": {"bound": "hbm", "kernel": f"batch {dom}", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": group_ms[dom], "algorithmic_bytes_per_launch": dom_bytes,
+                     "group_ms": group_ms},
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,8 +169,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dam_1m", "dam_16m"], default="dam_1m",
-                    help="dam_16m: 16,000,000 fluid particles at spacing 0.002 with 200 sphere obstacles")
+    ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256"], default="dam_1m",
+                    help="dam_16m: 16,000,000 fluid particles at spacing 0.002 with 200 sphere obstacles; "
+                         "batch_256: 256 WCSPH scenes in one batch")
+    ap.add_argument("--per-scene-steps", type=int, default=10,
+                    help="batch_256: steps of each world timed on the single-world path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="one 256 MiB device copy, the known-size kernel that scales the rocprofv3 byte counters")
     args = ap.parse_args()
@@ -96,6 +191,10 @@ def main() -> int:
         print("bench.py measures one GPU: the fluid step here is built and timed on a single MI355X "
               "(multi-GPU sharding is a design note, not a code path)", file=sys.stderr)
         return 2
+
+    if args.workload == "batch_256":
+        print(json.dumps(run_batch(args)), flush=True)
+        return 0
 
     from solunar_amd import scenes
     from solunar_amd.fluid import FluidWorld

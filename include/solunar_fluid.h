@@ -232,6 +232,78 @@ int32_t sfl_solve_grid(const sfl_world* world);
 /* Blocks until every launch of the world has finished. */
 int32_t sfl_synchronize(sfl_world* world);
 
+/* ---- Batched WCSPH ---------------------------------------------------------
+ *
+ * No reference counterpart as one object: the reference steps many scenes as
+ * many `FluidWorld`s, one `step` (src/fluid/step.rs:58) per world. A batch
+ * takes a copy of the state of `count` WCSPH worlds with equal settings and
+ * steps all of them together; scene s's results are bit-equal to world s
+ * stepping alone. The run of `steps` steps is one HIP graph launch (captured
+ * on first use and kept while the step count, substep slots, block shapes
+ * and cell table stay the same); block shapes come from sfl_batch_autotune.
+ */
+typedef struct sfl_batch sfl_batch;
+
+enum { SFL_BATCH_CANDIDATES = 5 };
+
+typedef struct sfl_batch_tuning {
+  uint32_t sort_block;     /* threads per block: bounds, count, scatter, gather */
+  uint32_t density_block;  /* density pass */
+  uint32_t force_block;    /* force and advance pass */
+  uint32_t candidates;     /* entries used below (sfl_batch_autotune) */
+  uint32_t blocks[SFL_BATCH_CANDIDATES];   /* block sizes tried */
+  float sort_ms[SFL_BATCH_CANDIDATES];     /* best time of each, milliseconds */
+  float density_ms[SFL_BATCH_CANDIDATES];
+  float force_ms[SFL_BATCH_CANDIDATES];
+} sfl_batch_tuning;
+
+/* Copies the worlds' particles, walls and containers. Every world must be
+ * WCSPH, on one device, with equal settings apart from max_particles. The
+ * worlds are not changed. SFL_INVALID_SETTINGS otherwise. */
+int32_t sfl_batch_create(sfl_world* const* worlds, size_t count,
+                         sfl_batch** out);
+void sfl_batch_destroy(sfl_batch* batch);
+size_t sfl_batch_scenes(const sfl_batch* batch);
+size_t sfl_batch_len(const sfl_batch* batch, size_t scene);
+
+/* `steps` steps of `dt` for every scene; reports[scene] (count entries, may
+ * be null) gets the last step's FluidReport of each scene. A non-finite
+ * advance finishes that step for every scene and returns SFL_NON_FINITE
+ * ("fluid particle"); sfl_batch_scene_failed names the scenes. */
+int32_t sfl_batch_step(sfl_batch* batch, float dt, uint32_t steps,
+                       sfl_report* reports);
+int32_t sfl_batch_scene_failed(const sfl_batch* batch, size_t scene,
+                               int32_t* failed);
+/* Captures the graph a call of `steps` steps launches, ahead of that call
+ * (graphs are kept per step count until the shapes or the cell table
+ * change). */
+int32_t sfl_batch_prepare(sfl_batch* batch, uint32_t steps);
+
+/* Times every candidate block size of the three kernel groups on the
+ * batch's current state (one substep of `dt`, best of `repeats` runs) and
+ * keeps the fastest. The state is not changed. */
+int32_t sfl_batch_autotune(sfl_batch* batch, float dt, uint32_t repeats,
+                           sfl_batch_tuning* out);
+int32_t sfl_batch_set_tuning(sfl_batch* batch, const sfl_batch_tuning* tuning);
+int32_t sfl_batch_get_tuning(const sfl_batch* batch, sfl_batch_tuning* out);
+/* 1 (default): run steps as one graph launch; 0: plain stream launches. */
+int32_t sfl_batch_use_graph(sfl_batch* batch, int32_t on);
+
+/* Scene state in the scene's current order (any pointer may be null). */
+int32_t sfl_batch_read_state(sfl_batch* batch, size_t scene, size_t cap,
+                             uint32_t* ids, float* xyz, float* vxyz,
+                             float* densities);
+int32_t sfl_batch_state_hash(sfl_batch* batch, size_t scene, uint64_t* out);
+/* Writes the scenes back into `count` worlds (the ones the batch was made
+ * from, or any with the same particle counts and settings). */
+int32_t sfl_batch_store(sfl_batch* batch, sfl_world* const* worlds,
+                        size_t count);
+const char* sfl_batch_error_detail(const sfl_batch* batch);
+size_t sfl_batch_memory_bytes(const sfl_batch* batch);
+/* Kernel nodes of the captured graph (0 before the first graph launch). */
+uint64_t sfl_batch_graph_nodes(const sfl_batch* batch);
+int32_t sfl_batch_synchronize(sfl_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,3 +11,9 @@ def FluidWorld(*args, **kwargs):  # noqa: N802 - mirrors the reference type name
     from .fluid import FluidWorld as _World
 
     return _World(*args, **kwargs)
+
+
+def FluidBatch(*args, **kwargs):  # noqa: N802
+    from .fluid import FluidBatch as _Batch
+
+    return _Batch(*args, **kwargs)

@@ -2,12 +2,18 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/solunar_fluid.h"
+#include "batch.hpp"
 #include "world.hpp"
 
 struct sfl_world {
   sfl::World* w;
+};
+
+struct sfl_batch {
+  sfl::Batch* b;
 };
 
 namespace {
@@ -21,6 +27,17 @@ int32_t guarded(sfl_world* h, F&& f) {
     return f(*h->w);
   } catch (const std::bad_alloc&) {
     h->w->set_detail("host allocation failed");
+    return SFL_DEVICE_ERROR;
+  }
+}
+
+template <class F>
+int32_t guarded_batch(sfl_batch* h, F&& f) {
+  if (!h || !h->b) return SFL_INVALID_HANDLE;
+  try {
+    return f(*h->b);
+  } catch (const std::bad_alloc&) {
+    h->b->set_detail("host allocation failed");
     return SFL_DEVICE_ERROR;
   }
 }
@@ -248,6 +265,115 @@ int32_t sfl_solve_grid(const sfl_world* world) { return world && world->w ? worl
 
 int32_t sfl_synchronize(sfl_world* world) {
   return guarded(world, [&](sfl::World& w) { return w.synchronize(); });
+}
+
+// ---- batched WCSPH ------------------------------------------------------------
+
+int32_t sfl_batch_create(sfl_world* const* worlds, size_t count, sfl_batch** out) {
+  if (!out) return SFL_OUT_OF_RANGE;
+  *out = nullptr;
+  if (!worlds) return SFL_OUT_OF_RANGE;
+  try {
+    std::vector<sfl::World*> ws(count);
+    for (size_t s = 0; s < count; ++s) {
+      if (!worlds[s] || !worlds[s]->w) return SFL_INVALID_HANDLE;
+      ws[s] = worlds[s]->w;
+    }
+    sfl::Batch* b = nullptr;
+    std::string why;
+    int32_t st = sfl::Batch::create(ws.data(), count, &b, why);
+    g_create_detail = why;
+    if (st != SFL_OK) return st;
+    *out = new sfl_batch{b};
+    return SFL_OK;
+  } catch (const std::bad_alloc&) {
+    g_create_detail = "host allocation failed";
+    return SFL_DEVICE_ERROR;
+  }
+}
+
+void sfl_batch_destroy(sfl_batch* batch) {
+  if (!batch) return;
+  delete batch->b;
+  delete batch;
+}
+
+size_t sfl_batch_scenes(const sfl_batch* batch) { return batch && batch->b ? batch->b->scenes() : 0; }
+
+size_t sfl_batch_len(const sfl_batch* batch, size_t scene) {
+  return batch && batch->b ? batch->b->len(scene) : 0;
+}
+
+int32_t sfl_batch_step(sfl_batch* batch, float dt, uint32_t steps, sfl_report* reports) {
+  return guarded_batch(batch, [&](sfl::Batch& b) { return b.step(dt, steps, reports); });
+}
+
+int32_t sfl_batch_prepare(sfl_batch* batch, uint32_t steps) {
+  return guarded_batch(batch, [&](sfl::Batch& b) { return b.prepare(steps); });
+}
+
+int32_t sfl_batch_scene_failed(const sfl_batch* batch, size_t scene, int32_t* failed) {
+  if (!batch || !batch->b) return SFL_INVALID_HANDLE;
+  if (!failed) return SFL_OUT_OF_RANGE;
+  return batch->b->scene_failed(scene, failed);
+}
+
+int32_t sfl_batch_autotune(sfl_batch* batch, float dt, uint32_t repeats, sfl_batch_tuning* out) {
+  return guarded_batch(batch, [&](sfl::Batch& b) { return b.autotune(dt, repeats, out); });
+}
+
+int32_t sfl_batch_set_tuning(sfl_batch* batch, const sfl_batch_tuning* tuning) {
+  return guarded_batch(batch, [&](sfl::Batch& b) {
+    if (!tuning) return (int32_t)SFL_OUT_OF_RANGE;
+    return b.set_tuning(*tuning);
+  });
+}
+
+int32_t sfl_batch_get_tuning(const sfl_batch* batch, sfl_batch_tuning* out) {
+  if (!batch || !batch->b) return SFL_INVALID_HANDLE;
+  if (!out) return SFL_OUT_OF_RANGE;
+  return batch->b->get_tuning(out);
+}
+
+int32_t sfl_batch_use_graph(sfl_batch* batch, int32_t on) {
+  return guarded_batch(batch, [&](sfl::Batch& b) { return b.use_graph(on != 0); });
+}
+
+int32_t sfl_batch_read_state(sfl_batch* batch, size_t scene, size_t cap, uint32_t* ids, float* xyz, float* vxyz,
+                             float* densities) {
+  return guarded_batch(batch, [&](sfl::Batch& b) { return b.read_state(scene, cap, ids, xyz, vxyz, densities); });
+}
+
+int32_t sfl_batch_state_hash(sfl_batch* batch, size_t scene, uint64_t* out) {
+  return guarded_batch(batch, [&](sfl::Batch& b) {
+    if (!out) return (int32_t)SFL_OUT_OF_RANGE;
+    return b.state_hash(scene, out);
+  });
+}
+
+int32_t sfl_batch_store(sfl_batch* batch, sfl_world* const* worlds, size_t count) {
+  return guarded_batch(batch, [&](sfl::Batch& b) {
+    if (!worlds) return (int32_t)SFL_OUT_OF_RANGE;
+    std::vector<sfl::World*> ws(count);
+    for (size_t s = 0; s < count; ++s) {
+      if (!worlds[s] || !worlds[s]->w) return (int32_t)SFL_INVALID_HANDLE;
+      ws[s] = worlds[s]->w;
+    }
+    return b.store(ws.data(), count);
+  });
+}
+
+const char* sfl_batch_error_detail(const sfl_batch* batch) {
+  if (!batch || !batch->b) return g_create_detail.c_str();
+  return batch->b->detail();
+}
+
+size_t sfl_batch_memory_bytes(const sfl_batch* batch) { return batch && batch->b ? batch->b->device_bytes() : 0; }
+
+uint64_t sfl_batch_graph_nodes(const sfl_batch* batch) { return batch && batch->b ? batch->b->graph_nodes() : 0; }
+
+int32_t sfl_batch_synchronize(sfl_batch* batch) {
+  return guarded_batch(batch, [&](sfl::Batch& b) { return b.synchronize(); });
 }
 
 }  // extern "C"

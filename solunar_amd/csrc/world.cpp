@@ -76,31 +76,6 @@ struct DeviceGuard {
 
 }  // namespace
 
-template <class T>
-hipError_t DeviceBuffer<T>::reserve(size_t n, size_t keep, hipStream_t s) {
-  if (n <= cap) return hipSuccess;
-  size_t want = std::max(n, cap + cap / 2);
-  T* fresh = nullptr;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(&fresh), want * sizeof(T));
-  if (e != hipSuccess) return e;
-  if (keep && ptr) {
-    e = hipMemcpyAsync(fresh, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) {
-      (void)hipFree(fresh);
-      return e;
-    }
-  }
-  e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
-    (void)hipFree(fresh);
-    return e;
-  }
-  release();
-  ptr = fresh;
-  cap = want;
-  return hipSuccess;
-}
-
 int32_t World::device_error(const char* what, hipError_t e) {
   detail_ = std::string(what) + ": " + hipGetErrorString(e);
   return SFL_DEVICE_ERROR;
@@ -1109,16 +1084,7 @@ int32_t World::read_neighbors(int kind, size_t start_cap, uint32_t* start, size_
   return SFL_OK;
 }
 
-// state_hash (src/fluid/mod.rs:678-691) with FNV-1a (src/snapshot/bytes.rs:6-13)
-int32_t World::state_hash(uint64_t* out) {
-  DeviceGuard g(device_);
-  size_t n = n_;
-  std::vector<float4> p(n), v(n);
-  if (n) {
-    SFL_HIP(hipMemcpyAsync(p.data(), pa_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read state");
-    SFL_HIP(hipMemcpyAsync(v.data(), va_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read state");
-    SFL_HIP(hipStreamSynchronize(stream_), "read state");
-  }
+uint64_t hash_particles(const float4* p, const float4* v, size_t n) {
   std::vector<std::pair<uint32_t, size_t>> order(n);
   for (size_t i = 0; i < n; ++i) {
     uint32_t id;
@@ -1143,7 +1109,20 @@ int32_t World::state_hash(uint64_t* out) {
       feed(bits);
     }
   }
-  *out = h;
+  return h;
+}
+
+// state_hash (src/fluid/mod.rs:678-691) with FNV-1a (src/snapshot/bytes.rs:6-13)
+int32_t World::state_hash(uint64_t* out) {
+  DeviceGuard g(device_);
+  size_t n = n_;
+  std::vector<float4> p(n), v(n);
+  if (n) {
+    SFL_HIP(hipMemcpyAsync(p.data(), pa_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read state");
+    SFL_HIP(hipMemcpyAsync(v.data(), va_.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, stream_), "read state");
+    SFL_HIP(hipStreamSynchronize(stream_), "read state");
+  }
+  *out = hash_particles(p.data(), v.data(), n);
   return SFL_OK;
 }
 

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -36,7 +37,40 @@ struct DeviceBuffer {
   }
 };
 
+template <class T>
+hipError_t DeviceBuffer<T>::reserve(size_t n, size_t keep, hipStream_t s) {
+  if (n <= cap) return hipSuccess;
+  size_t want = std::max(n, cap + cap / 2);
+  T* fresh = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&fresh), want * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (keep && ptr) {
+    e = hipMemcpyAsync(fresh, ptr, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
+      (void)hipFree(fresh);
+      return e;
+    }
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(fresh);
+    return e;
+  }
+  release();
+  ptr = fresh;
+  cap = want;
+  return hipSuccess;
+}
+
+// FNV-1a 64 of (id, x, v) in id order (src/fluid/mod.rs:678-691,
+// src/snapshot/bytes.rs:6-13) over float4 pairs (x, id bits) and (v, rho).
+uint64_t hash_particles(const float4* p, const float4* v, size_t n);
+
+class Batch;
+
 class World {
+  friend class Batch;
+
  public:
   static int32_t create(const sfl_settings& s, World** out, std::string& why);
   ~World();

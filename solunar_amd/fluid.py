@@ -17,8 +17,8 @@ from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
-from .types import (CReport, CSettings, CVec3, FluidReport, FluidSettings,
-                    PhysicsError)
+from .types import (CBatchTuning, CReport, CSettings, CVec3, FluidReport,
+                    FluidSettings, PhysicsError)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsolunar_fluid.so")
 
@@ -77,6 +77,24 @@ def library() -> ctypes.CDLL:
         "sfl_synchronize": (i32, [P]),
         "sfl_solve_grid": (i32, [P]),
         "sfl_memory_bytes": (sz, [P]),
+        "sfl_batch_create": (i32, [ctypes.POINTER(P), sz, ctypes.POINTER(P)]),
+        "sfl_batch_destroy": (None, [P]),
+        "sfl_batch_scenes": (sz, [P]),
+        "sfl_batch_len": (sz, [P, sz]),
+        "sfl_batch_step": (i32, [P, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(CReport)]),
+        "sfl_batch_scene_failed": (i32, [P, sz, ctypes.POINTER(i32)]),
+        "sfl_batch_prepare": (i32, [P, ctypes.c_uint32]),
+        "sfl_batch_autotune": (i32, [P, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(CBatchTuning)]),
+        "sfl_batch_set_tuning": (i32, [P, ctypes.POINTER(CBatchTuning)]),
+        "sfl_batch_get_tuning": (i32, [P, ctypes.POINTER(CBatchTuning)]),
+        "sfl_batch_use_graph": (i32, [P, i32]),
+        "sfl_batch_read_state": (i32, [P, sz, sz, u32p, fp, fp, fp]),
+        "sfl_batch_state_hash": (i32, [P, sz, ctypes.POINTER(ctypes.c_uint64)]),
+        "sfl_batch_store": (i32, [P, ctypes.POINTER(P), sz]),
+        "sfl_batch_error_detail": (ctypes.c_char_p, [P]),
+        "sfl_batch_memory_bytes": (sz, [P]),
+        "sfl_batch_graph_nodes": (ctypes.c_uint64, [P]),
+        "sfl_batch_synchronize": (i32, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -298,3 +316,103 @@ class FluidWorld:
 
     def synchronize(self) -> None:
         self._check(self._lib.sfl_synchronize(self._h))
+
+
+class FluidBatch:
+    """Many WCSPH worlds stepped together (include/solunar_fluid.h,
+    sfl_batch_*). Scene s starts as a copy of worlds[s] and steps bit-equal
+    to it; `store` hands the scenes back to worlds."""
+
+    def __init__(self, worlds: Sequence[FluidWorld]):
+        self._lib = library()
+        handles = (ctypes.c_void_p * len(worlds))(*[w._h for w in worlds])
+        h = ctypes.c_void_p()
+        st = self._lib.sfl_batch_create(handles, len(worlds), ctypes.byref(h))
+        if st:
+            raise PhysicsError(st, self._lib.sfl_batch_error_detail(None).decode())
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sfl_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st: int) -> None:
+        if st:
+            raise PhysicsError(st, self._lib.sfl_batch_error_detail(self._h).decode())
+
+    def scenes(self) -> int:
+        return int(self._lib.sfl_batch_scenes(self._h))
+
+    def len(self, scene: int) -> int:
+        return int(self._lib.sfl_batch_len(self._h, scene))
+
+    def step(self, dt: float, steps: int = 1):
+        """`steps` steps of every scene; the last step's report of each."""
+        n = self.scenes()
+        reports = (CReport * n)()
+        self._check(self._lib.sfl_batch_step(self._h, dt, steps, reports))
+        return [r.to_report() for r in reports]
+
+    def prepare(self, steps: int) -> None:
+        """Capture the graph of a `steps`-step call ahead of it."""
+        self._check(self._lib.sfl_batch_prepare(self._h, steps))
+
+    def scene_failed(self, scene: int) -> bool:
+        out = ctypes.c_int32()
+        self._check(self._lib.sfl_batch_scene_failed(self._h, scene, ctypes.byref(out)))
+        return bool(out.value)
+
+    def autotune(self, dt: float, repeats: int = 3) -> dict:
+        t = CBatchTuning()
+        self._check(self._lib.sfl_batch_autotune(self._h, dt, repeats, ctypes.byref(t)))
+        return t.to_dict()
+
+    def tuning(self) -> dict:
+        t = CBatchTuning()
+        self._check(self._lib.sfl_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.to_dict()
+
+    def set_tuning(self, sort_block: int, density_block: int, force_block: int) -> None:
+        t = CBatchTuning()
+        t.sort_block, t.density_block, t.force_block = sort_block, density_block, force_block
+        self._check(self._lib.sfl_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def use_graph(self, on: bool) -> None:
+        self._check(self._lib.sfl_batch_use_graph(self._h, 1 if on else 0))
+
+    def state(self, scene: int):
+        """(ids, positions, velocities, densities) of a scene, current order."""
+        n = self.len(scene)
+        ids = np.zeros(n, np.uint32)
+        x = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        rho = np.zeros(n, np.float32)
+        self._check(self._lib.sfl_batch_read_state(self._h, scene, n, _ptr(ids, ctypes.c_uint32),
+                                                   _ptr(x, ctypes.c_float), _ptr(v, ctypes.c_float),
+                                                   _ptr(rho, ctypes.c_float)))
+        return ids, x, v, rho
+
+    def state_hash(self, scene: int) -> int:
+        out = ctypes.c_uint64()
+        self._check(self._lib.sfl_batch_state_hash(self._h, scene, ctypes.byref(out)))
+        return int(out.value)
+
+    def store(self, worlds: Sequence[FluidWorld]) -> None:
+        handles = (ctypes.c_void_p * len(worlds))(*[w._h for w in worlds])
+        self._check(self._lib.sfl_batch_store(self._h, handles, len(worlds)))
+
+    def memory_bytes(self) -> int:
+        return int(self._lib.sfl_batch_memory_bytes(self._h))
+
+    def graph_nodes(self) -> int:
+        return int(self._lib.sfl_batch_graph_nodes(self._h))
+
+    def synchronize(self) -> None:
+        self._check(self._lib.sfl_batch_synchronize(self._h))

@@ -8,6 +8,9 @@ low corner, lattice sites nudged by up to +-0.05 spacing per axis):
   dam_small   WCSPH, spacing 0.02, 0.4 x 0.8 x 0.4 block: 16,000 particles
   dam_1m      DFSPH, spacing 0.005, same block: 1,024,000 particles
   dam_16m     DFSPH, spacing 0.002: 16,000,000 particles
+  batch_256   256 WCSPH dam breaks at spacing 0.02, block sides uniform in
+              [0.2, 0.8] m per axis, redrawn until the block holds 2,000 to
+              64,000 particles
 The Wendland C2 kernel with support 2 stands for "h = 2 dx"; containers have
 ceil(support) = 2 wall layers (src/fluid/boundary.rs:41-68).
 """
@@ -121,3 +124,37 @@ def dam_with_obstacles(world, obstacles: int = 200, seed: int = 7) -> int:
     if len(pts):
         world.add_boundary_points(pts)
     return n
+
+
+def batch_blocks(count: int = 256, seed: int = SEED, spacing: float = 0.02, low: int = 2000,
+                 high: int = 64000):
+    """Block sides of `count` dam-break scenes: uniform in [0.2, 0.8] m per
+    axis, drawn again until fill_box's count (floor(side / spacing) per axis,
+    in f32) lies in [low, high]. Returns [(sides, particles)]."""
+    rng = np.random.default_rng(seed)
+    sp = np.float32(spacing)
+    out = []
+    while len(out) < count:
+        sides = rng.uniform(0.2, 0.8, size=3).astype(np.float32)
+        n = int(np.prod(np.floor(sides / sp).astype(np.int64)))
+        if low <= n <= high:
+            out.append((tuple(float(x) for x in sides), n))
+    return out
+
+
+def batch_scene(world, sides, index: int, seed: int = SEED) -> int:
+    """Scene `index` of a batch: the container, then its jittered block."""
+    world.add_box_container((0.0, 0.0, 0.0), BOX)
+    return world.fill_box_jittered((0.0, 0.0, 0.0), sides, JITTER, (seed + 7919 * (index + 1)) & 0xFFFFFFFF)
+
+
+def batch_worlds(make_world, count: int = 256, seed: int = SEED, settings: FluidSettings = None):
+    """`count` worlds made by `make_world(settings)`, each one batch scene."""
+    settings = settings or settings_dam_small()
+    worlds = []
+    for k, (sides, n) in enumerate(batch_blocks(count, seed, settings.spacing)):
+        w = make_world(settings)
+        got = batch_scene(w, sides, k, seed)
+        assert got == n, (got, n)
+        worlds.append(w)
+    return worlds

@@ -168,3 +168,33 @@ class CVec3(ctypes.Structure):
     @classmethod
     def of(cls, v) -> "CVec3":
         return cls(float(v[0]), float(v[1]), float(v[2]))
+
+
+BATCH_CANDIDATES = 5
+
+
+class CBatchTuning(ctypes.Structure):
+    """`sfl_batch_tuning` of include/solunar_fluid.h."""
+
+    _fields_ = [
+        ("sort_block", ctypes.c_uint32),
+        ("density_block", ctypes.c_uint32),
+        ("force_block", ctypes.c_uint32),
+        ("candidates", ctypes.c_uint32),
+        ("blocks", ctypes.c_uint32 * BATCH_CANDIDATES),
+        ("sort_ms", ctypes.c_float * BATCH_CANDIDATES),
+        ("density_ms", ctypes.c_float * BATCH_CANDIDATES),
+        ("force_ms", ctypes.c_float * BATCH_CANDIDATES),
+    ]
+
+    def to_dict(self) -> dict:
+        n = self.candidates
+        return {
+            "sort_block": self.sort_block,
+            "density_block": self.density_block,
+            "force_block": self.force_block,
+            "blocks": list(self.blocks[:n]),
+            "sort_ms": [float(x) for x in self.sort_ms[:n]],
+            "density_ms": [float(x) for x in self.density_ms[:n]],
+            "force_ms": [float(x) for x in self.force_ms[:n]],
+        }

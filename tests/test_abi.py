@@ -11,7 +11,7 @@ import re
 import pytest
 
 from solunar_amd import fluid
-from solunar_amd.types import CReport, CSettings, FluidSettings
+from solunar_amd.types import CBatchTuning, CReport, CSettings, FluidSettings
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -38,6 +38,9 @@ def test_struct_layouts_match_the_header():
     assert ctypes.sizeof(CSettings) == 72
     assert CSettings.max_particles.offset == 16
     assert ctypes.sizeof(CReport) == 48
+    # sfl_batch_tuning: 4 u32, u32[5], 3 x float[5]
+    assert ctypes.sizeof(CBatchTuning) == 96
+    assert CBatchTuning.sort_ms.offset == 36
 
 
 def test_defaults_are_the_reference_defaults():
