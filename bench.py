@@ -162,6 +162,93 @@ def run_batch(args) -> dict:
     }
 
 
+def run_adjoint(args) -> dict:
+    """Differentiable WCSPH: 64,000 particles, 200 steps of 1e-4 s, loss =
+    mean |x_T - x_T*|^2 against the viscosity-0.08 rollout. Times the plain
+    forward rollout, the recording forward (checkpoints every 20 steps) and
+    the reverse pass (segment replay + adjoint kernels); the metric is
+    reverse / forward."""
+    import numpy as np
+
+    from solunar_amd import scenes
+    from solunar_amd.fluid import FluidWorld, Tape
+
+    steps, dt, every = 200, 1e-4, 20
+    x0 = scenes.adjoint_positions()
+    v0 = scenes.adjoint_velocities(len(x0))
+    n = len(x0)
+
+    def world(alpha):
+        w = FluidWorld(scenes.settings_adjoint_64k(alpha))
+        scenes.container(w)
+        w.add_particles(x0, v0)
+        return w
+
+    log(f"adjoint_64k: {n} particles, {steps} steps; forward rollout (viscosity 0.08)")
+    target_world = world(0.08)
+    target_world.step(dt)  # first-call set-up outside the timing
+    target_world.close()
+    target_world = world(0.08)
+    target_world.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        target_world.step(dt)
+    target_world.synchronize()
+    forward = time.perf_counter() - t0
+    target = np.empty_like(x0)
+    target[target_world.ids()] = target_world.positions()
+    target_world.close()
+    log(f"forward {forward * 1e3:.1f} ms; recording rollout (viscosity 0.05)")
+    w = world(0.05)
+    w.synchronize()
+    t0 = time.perf_counter()
+    tape = Tape(w, dt, steps, every)
+    w.synchronize()
+    record = time.perf_counter() - t0
+    xT = np.empty_like(x0)
+    xT[w.ids()] = w.positions()
+    seed = (2.0 * (xT.astype(np.float64) - target) / n).astype(np.float32)
+    loss = float(np.sum((xT.astype(np.float64) - target) ** 2) / n)
+    log(f"record {record * 1e3:.1f} ms; reverse pass (warm-up, then {max(args.steps, 1)} timed)")
+    tape.adjoint(seed)
+    runs = []
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        gx, gv, ga = tape.adjoint(seed)
+        runs.append(time.perf_counter() - t0)
+    reverse = min(runs)
+    memory = tape.memory_bytes()
+    tape.close()
+    w.close()
+    log(f"reverse {reverse * 1e3:.1f} ms (best of {len(runs)})")
+    return {
+        "metric": "adjoint:forward time ratio, 64k WCSPH rollout",
+        "value": reverse / forward,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": reverse * 1e3 / steps,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "adjoint_64k: WCSPH, spacing 0.01 m, 0.4 m cube of 64,000 particles in a 1.6 x 1.6 x "
+                               "0.8 m container, v0 ~ N(0, 0.1 m/s), 200 steps of 1e-4 s, checkpoints every 20 "
+                               "steps, loss = mean |x_T - x_T*|^2 against viscosity 0.08, gradients in v0 and alpha",
+                   "forward_ms": forward * 1e3, "record_ms": record * 1e3, "reverse_ms": reverse * 1e3,
+                   "reverse_runs_ms": [r * 1e3 for r in runs],
+                   "record_plus_reverse_over_forward": (record + reverse) / forward,
+                   "forward_particle_steps_per_s": n * steps / forward,
+                   "reverse_particle_steps_per_s": n * steps / reverse,
+                   "loss": loss, "dL_dalpha": ga, "dL_dv0_norm": float(np.linalg.norm(gv)),
+                   "tape_device_bytes": memory},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,7 +256,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256"], default="dam_1m",
+    ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256", "adjoint_64k"], default="dam_1m",
                     help="dam_16m: 16,000,000 fluid particles at spacing 0.002 with 200 sphere obstacles; "
                          "batch_256: 256 WCSPH scenes in one batch")
     ap.add_argument("--per-scene-steps", type=int, default=10,
@@ -194,6 +281,9 @@ def main() -> int:
 
     if args.workload == "batch_256":
         print(json.dumps(run_batch(args)), flush=True)
+        return 0
+    if args.workload == "adjoint_64k":
+        print(json.dumps(run_adjoint(args)), flush=True)
         return 0
 
     from solunar_amd import scenes

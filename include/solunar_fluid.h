@@ -304,6 +304,34 @@ size_t sfl_batch_memory_bytes(const sfl_batch* batch);
 uint64_t sfl_batch_graph_nodes(const sfl_batch* batch);
 int32_t sfl_batch_synchronize(sfl_batch* batch);
 
+/* ---- WCSPH adjoint ---------------------------------------------------------
+ *
+ * No reference counterpart: the reference has no differentiation. A tape
+ * records `steps` calls of sfl_step(world, dt) (the world advances as it
+ * would), keeping the particle state every `every` steps. sfl_wcsph_adjoint
+ * then runs the hand-written reverse pass of those steps (density, Tait
+ * pressure, pressure and viscosity forces, walls, symplectic Euler with the
+ * container clamp), replaying each segment from its checkpoint. Neighbour
+ * sets, substep counts and clamp decisions are those of the recorded run.
+ * The world must outlive the tape and must not change between recording
+ * and the reverse pass (SFL_OUT_OF_RANGE otherwise); its ids must be
+ * 0..len-1 (a world that only ever gained particles).
+ */
+typedef struct sfl_tape sfl_tape;
+
+int32_t sfl_wcsph_record(sfl_world* world, float dt, uint32_t steps,
+                         uint32_t every, sfl_tape** out);
+/* xbar, vbar: dL/dx, dL/dv of the final state, 3 floats per particle id.
+ * xbar0, vbar0: dL/dx, dL/dv of the state at recording start, per id.
+ * viscosity_bar: dL/d(settings.viscosity), the artificial-viscosity alpha
+ * (may be null). */
+int32_t sfl_wcsph_adjoint(sfl_tape* tape, const float* xbar, const float* vbar,
+                          float* xbar0, float* vbar0, float* viscosity_bar);
+void sfl_tape_destroy(sfl_tape* tape);
+uint64_t sfl_tape_substeps(const sfl_tape* tape);
+size_t sfl_tape_memory_bytes(const sfl_tape* tape);
+const char* sfl_tape_error_detail(const sfl_tape* tape);
+
 #ifdef __cplusplus
 }
 #endif

@@ -436,6 +436,9 @@ class World {
   } walls;
 
   Dispatch dispatch;
+  // called before (false) and after (true) every WCSPH substep with its
+  // length; the adjoint tape (adjoint_oracle.hpp) records through it
+  std::function<void(bool, R)> substep_hook;
   ResidualMode residual_mode = kReferenceF32;
   std::vector<SolveLog> solve_log;
   bool log_solves = false;
@@ -707,8 +710,10 @@ class World {
     R small = dt / R(substeps);
     for (uint32_t s = 0; s < substeps; ++s) {
       uint32_t held = 0;
+      if (substep_hook) substep_hook(false, small);
       int32_t st = wcsph_substep(small, held);
       if (st != kOk) return st;
+      if (substep_hook) substep_hook(true, small);
       report.held_back += held;
     }
     report.substeps = substeps;

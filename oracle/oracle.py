@@ -74,6 +74,9 @@ def library() -> ctypes.CDLL:
         "orc_compute_densities": (i32, [P]),
         "orc_read_ids": (i32, [P, sz, u32p]),
         "orc_read_vectors": (i32, [P, i32, sz, ctypes.c_void_p]),
+        "orc_add_particles_f64": (i32, [P, sz, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+        "orc_adjoint_record": (i32, [P, ctypes.c_float, ctypes.c_uint32, i32]),
+        "orc_adjoint_reverse": (i32, [P] + [ctypes.POINTER(ctypes.c_double)] * 5),
         "orc_read_densities": (i32, [P, sz, ctypes.c_void_p]),
         "orc_set_particles": (i32, [P, sz, u32p, fp, fp]),
         "orc_read_neighbors": (i32, [P, i32, sz, u32p, sz, u32p, ctypes.POINTER(sz)]),
@@ -287,6 +290,41 @@ class OracleWorld:
 
     def state_hash(self) -> int:
         return int(self._lib.orc_state_hash(self._h))
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, 3))
+
+
+def _add_particles_f64(self, positions, velocities) -> None:
+    """Particles from float64 inputs (exact in the FP64 world)."""
+    x, v = _f64(positions), _f64(velocities)
+    self._check(self._lib.orc_add_particles_f64(self._h, len(x), _p(x, ctypes.c_double), _p(v, ctypes.c_double)))
+
+
+def _record(self, dt: float, steps: int, reverse_precision: Optional[int] = None) -> None:
+    """`steps` WCSPH steps of dt, recorded for `adjoint`; the reverse pass
+    runs in the world's precision, or in FP64 with reverse_precision=64."""
+    rp = self.precision if reverse_precision is None else reverse_precision
+    self._check(self._lib.orc_adjoint_record(self._h, dt, steps, rp))
+
+
+def _adjoint(self, xbar, vbar=None):
+    """Per-id adjoints of the final x, v -> per-id adjoints of the recorded
+    initial x, v and dL/d(viscosity), in the world's precision."""
+    xb = _f64(xbar)
+    vb = _f64(np.zeros_like(xb) if vbar is None else vbar)
+    x0 = np.zeros_like(xb)
+    v0 = np.zeros_like(xb)
+    a = ctypes.c_double()
+    self._check(self._lib.orc_adjoint_reverse(self._h, _p(xb, ctypes.c_double), _p(vb, ctypes.c_double),
+                                              _p(x0, ctypes.c_double), _p(v0, ctypes.c_double), ctypes.byref(a)))
+    return x0, v0, a.value
+
+
+OracleWorld.add_particles_f64 = _add_particles_f64
+OracleWorld.record = _record
+OracleWorld.adjoint = _adjoint
 
 
 def plant_mistake(which: int) -> None:

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <memory>
 
+#include "adjoint_oracle.hpp"
 #include "fluid_oracle.hpp"
 
 using namespace oracle;
@@ -42,6 +43,9 @@ struct Handle {
   int precision;
   std::unique_ptr<World<float>> f;
   std::unique_ptr<World<double>> d;
+  std::unique_ptr<AdjointTape<float>> tf;
+  std::unique_ptr<AdjointTape<float, double>> tfd;
+  std::unique_ptr<AdjointTape<double>> td;
 };
 
 Settings convert(const orc_settings* s) {
@@ -402,4 +406,70 @@ double orc_fixed_tree_sum(const float* e, size_t n) {
   std::vector<float> v(e, e + n);
   return fixed_tree_sum(v);
 }
+
+// Particles from double inputs (exact in the FP64 world), for finite
+// differences that perturb below f32 resolution.
+int32_t orc_add_particles_f64(void* w, size_t n, const double* xyz, const double* vxyz) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = std::decay_t<decltype(x.mass)>;
+    for (size_t i = 0; i < n; ++i) {
+      Vec3<R> p{R(xyz[3 * i]), R(xyz[3 * i + 1]), R(xyz[3 * i + 2])};
+      Vec3<R> v{R(vxyz[3 * i]), R(vxyz[3 * i + 1]), R(vxyz[3 * i + 2])};
+      int32_t st = x.add_particle(p, v, nullptr);
+      if (st != kOk) return st;
+    }
+    return (int32_t)kOk;
+  });
 }
+
+// Records `steps` WCSPH steps of dt for the adjoint (adjoint_oracle.hpp).
+// `reverse_precision` 64 runs the reverse pass of an f32 world in FP64.
+int32_t orc_adjoint_record(void* w, float dt, uint32_t steps, int32_t reverse_precision) {
+  auto* h = static_cast<Handle*>(w);
+  h->tf.reset();
+  h->tfd.reset();
+  h->td.reset();
+  if (h->precision == 64) {
+    h->td.reset(new AdjointTape<double>(*h->d));
+    return h->td->record(dt, steps);
+  }
+  if (reverse_precision == 64) {
+    h->tfd.reset(new AdjointTape<float, double>(*h->f));
+    return h->tfd->record(dt, steps);
+  }
+  h->tf.reset(new AdjointTape<float>(*h->f));
+  return h->tf->record(dt, steps);
+}
+
+// Reverse pass of the recorded steps: per-id adjoints of the final x, v in,
+// of the initial x, v out, and dL/d(viscosity).
+int32_t orc_adjoint_reverse(void* w, const double* xbar, const double* vbar, double* xbar0, double* vbar0,
+                            double* alpha_bar) {
+  auto* h = static_cast<Handle*>(w);
+  auto run = [&](auto& tape) {
+    using R = std::decay_t<decltype(tape.substeps[0].dt)>;
+    size_t n = tape.world.len();
+    std::vector<Vec3<R>> xb(n), vb(n), x0, v0;
+    for (size_t i = 0; i < n; ++i) {
+      xb[i] = {R(xbar[3 * i]), R(xbar[3 * i + 1]), R(xbar[3 * i + 2])};
+      vb[i] = {R(vbar[3 * i]), R(vbar[3 * i + 1]), R(vbar[3 * i + 2])};
+    }
+    R a = tape.reverse(xb, vb, x0, v0);
+    for (size_t i = 0; i < n; ++i) {
+      xbar0[3 * i] = x0[i].x;
+      xbar0[3 * i + 1] = x0[i].y;
+      xbar0[3 * i + 2] = x0[i].z;
+      vbar0[3 * i] = v0[i].x;
+      vbar0[3 * i + 1] = v0[i].y;
+      vbar0[3 * i + 2] = v0[i].z;
+    }
+    if (alpha_bar) *alpha_bar = (double)a;
+    return (int32_t)kOk;
+  };
+  if (h->td) return run(*h->td);
+  if (h->tfd) return run(*h->tfd);
+  if (h->tf) return run(*h->tf);
+  return (int32_t)kOutOfRange;
+}
+
+}  // extern "C"

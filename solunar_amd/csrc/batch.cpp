@@ -727,6 +727,7 @@ int32_t Batch::store(World* const* worlds, size_t count) {
     std::memcpy(&w.vmax_, &state_host_[s].vmax_bits, 4);
     w.vmax_valid_ = true;
     w.forces_valid_ = false;
+    ++w.version_;
     w.invalidate_lists();
   }
   SFL_HIP(hipStreamSynchronize(stream_), "store");

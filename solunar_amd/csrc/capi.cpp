@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "../../include/solunar_fluid.h"
+#include "adjoint.hpp"
 #include "batch.hpp"
 #include "world.hpp"
 
@@ -14,6 +15,10 @@ struct sfl_world {
 
 struct sfl_batch {
   sfl::Batch* b;
+};
+
+struct sfl_tape {
+  sfl::Tape* t;
 };
 
 namespace {
@@ -374,6 +379,52 @@ uint64_t sfl_batch_graph_nodes(const sfl_batch* batch) { return batch && batch->
 
 int32_t sfl_batch_synchronize(sfl_batch* batch) {
   return guarded_batch(batch, [&](sfl::Batch& b) { return b.synchronize(); });
+}
+
+// ---- WCSPH adjoint ---------------------------------------------------------------
+
+int32_t sfl_wcsph_record(sfl_world* world, float dt, uint32_t steps, uint32_t every, sfl_tape** out) {
+  if (!out) return SFL_OUT_OF_RANGE;
+  *out = nullptr;
+  return guarded(world, [&](sfl::World& w) {
+    sfl::Tape* t = nullptr;
+    std::string why;
+    int32_t st = sfl::Tape::record(&w, dt, steps, every, &t, why);
+    if (st != SFL_OK) {
+      w.set_detail(why);
+      g_create_detail = why;
+      return st;
+    }
+    *out = new sfl_tape{t};
+    return (int32_t)SFL_OK;
+  });
+}
+
+int32_t sfl_wcsph_adjoint(sfl_tape* tape, const float* xbar, const float* vbar, float* xbar0, float* vbar0,
+                          float* viscosity_bar) {
+  if (!tape || !tape->t) return SFL_INVALID_HANDLE;
+  if (!xbar || !vbar || !xbar0 || !vbar0) return SFL_OUT_OF_RANGE;
+  try {
+    return tape->t->adjoint(xbar, vbar, xbar0, vbar0, viscosity_bar);
+  } catch (const std::bad_alloc&) {
+    tape->t->set_detail("host allocation failed");
+    return SFL_DEVICE_ERROR;
+  }
+}
+
+void sfl_tape_destroy(sfl_tape* tape) {
+  if (!tape) return;
+  delete tape->t;
+  delete tape;
+}
+
+uint64_t sfl_tape_substeps(const sfl_tape* tape) { return tape && tape->t ? tape->t->substeps() : 0; }
+
+size_t sfl_tape_memory_bytes(const sfl_tape* tape) { return tape && tape->t ? tape->t->device_bytes() : 0; }
+
+const char* sfl_tape_error_detail(const sfl_tape* tape) {
+  if (!tape || !tape->t) return g_create_detail.c_str();
+  return tape->t->detail();
 }
 
 }  // extern "C"

@@ -244,13 +244,15 @@ __device__ __forceinline__ void wave_sum_to(uint32_t* dst, uint32_t v) {
 }
 
 // advance (src/fluid/step.rs:305-339) of one particle: next position, the
-// clamped velocity components zeroed, held-back count, finiteness.
+// clamped velocity components zeroed, held-back count, finiteness. `clamped`
+// gets bit c set when component c was clamped inside the container.
 __device__ __forceinline__ bool advance_one(const StepParams& prm, float4 p, float& vx, float& vy, float& vz,
-                                            float& nx, float& ny, float& nz, uint32_t& held) {
+                                            float& nx, float& ny, float& nz, uint32_t& held, uint32_t& clamped) {
   nx = p.x + vx * prm.dt;
   ny = p.y + vy * prm.dt;
   nz = p.z + vz * prm.dt;
   held = 0;
+  clamped = 0;
   if (prm.has_bounds) {
     float h = prm.half_spacing;
     float ix = rmin(rmax(nx, prm.blo[0] + h), prm.bhi[0] - h);
@@ -261,15 +263,30 @@ __device__ __forceinline__ bool advance_one(const StepParams& prm, float4 p, flo
       float cy = rmin(rmax(ny, prm.blo[1]), prm.bhi[1]);
       float cz = rmin(rmax(nz, prm.blo[2]), prm.bhi[2]);
       if (cx != nx || cy != ny || cz != nz) held = 1;
-      if (ix != nx) vx = 0.0f;
-      if (iy != ny) vy = 0.0f;
-      if (iz != nz) vz = 0.0f;
+      if (ix != nx) {
+        vx = 0.0f;
+        clamped |= 1u;
+      }
+      if (iy != ny) {
+        vy = 0.0f;
+        clamped |= 2u;
+      }
+      if (iz != nz) {
+        vz = 0.0f;
+        clamped |= 4u;
+      }
       nx = ix;
       ny = iy;
       nz = iz;
     }
   }
   return finite3(nx, ny, nz) && finite3(vx, vy, vz);
+}
+
+__device__ __forceinline__ bool advance_one(const StepParams& prm, float4 p, float& vx, float& vy, float& vz,
+                                            float& nx, float& ny, float& nz, uint32_t& held) {
+  uint32_t clamped;
+  return advance_one(prm, p, vx, vy, vz, nx, ny, nz, held, clamped);
 }
 
 // wall_viscous (src/fluid/step.rs:262-301), no terrains; `away` is the

@@ -259,11 +259,13 @@ __global__ __launch_bounds__(256) void k_scatter(const float4* __restrict__ P, u
 }
 
 // Sorted slot = cell start + number of cell-mates with a smaller id; moves
-// x, id (P) and v, rho (V) there.
+// x, id (P) and v, rho (V) there. `src` (optional) keeps each sorted slot's
+// index before the sort.
 __global__ __launch_bounds__(256) void k_rank_gather(const float4* __restrict__ PA, const float4* __restrict__ VA,
                                                      uint32_t n, float radius, const uint2* __restrict__ pairs,
                                                      const uint32_t* __restrict__ start, float4* __restrict__ PB,
-                                                     float4* __restrict__ VB, const Control* ctl) {
+                                                     float4* __restrict__ VB, const Control* ctl,
+                                                     uint32_t* __restrict__ src) {
   if (ctl->abort) return;
   uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
   if (q >= n) return;
@@ -278,6 +280,7 @@ __global__ __launch_bounds__(256) void k_rank_gather(const float4* __restrict__ 
   for (uint32_t k = s; k < e; ++k) rank += pairs[k].y < me.y ? 1u : 0u;
   PB[s + rank] = p;
   VB[s + rank] = v;
+  if (src) src[s + rank] = me.x;
 }
 
 // ---- CSR neighbour lists (readback / list density only) ----------------
@@ -412,9 +415,11 @@ void launch_scatter(hipStream_t s, const float4* P, uint32_t n, const uint2* slo
 }
 
 void launch_rank_gather(hipStream_t s, const float4* PA, const float4* VA, uint32_t n, float radius,
-                        const uint2* pairs, const uint32_t* start, float4* PB, float4* VB, const Control* ctl) {
+                        const uint2* pairs, const uint32_t* start, float4* PB, float4* VB, const Control* ctl,
+                        uint32_t* src) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_rank_gather, blocks_for(n, 256), 256, 0, s, PA, VA, n, radius, pairs, start, PB, VB, ctl);
+  hipLaunchKernelGGL(k_rank_gather, blocks_for(n, 256), 256, 0, s, PA, VA, n, radius, pairs, start, PB, VB, ctl,
+                     src);
 }
 
 void launch_list_count(hipStream_t s, const float4* lp, const StepParams& prm, uint32_t* counts, int kind) {

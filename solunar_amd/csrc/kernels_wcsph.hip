@@ -46,11 +46,12 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
 // The force loop of wcsph_substep (step.rs:173-211) with wall_viscous
 // (step.rs:262-301), v += push * dt (step.rs:212-222) and advance
 // (step.rs:305-339), for every particle. Reads the sorted state PB/VB, writes
-// PA/VA; `forces` (optional) keeps the push.
+// PA/VA; `forces` (optional) keeps the push, `clamps` (optional) the
+// clamped components of the advance (bits x, y, z).
 __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                      const float* __restrict__ scaled, StepParams prm,
                                                      float4* __restrict__ PA, float4* __restrict__ VA,
-                                                     float* __restrict__ forces) {
+                                                     float* __restrict__ forces, uint8_t* __restrict__ clamps) {
   Control* ctl = prm.ctl;
   if (ctl->abort) return;
   uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
@@ -113,7 +114,9 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
     float vy = vi.y + ay * prm.dt;
     float vz = vi.z + az * prm.dt;
     float nx, ny, nz;
-    bool ok = advance_one(prm, at, vx, vy, vz, nx, ny, nz, held);
+    uint32_t clamped;
+    bool ok = advance_one(prm, at, vx, vy, vz, nx, ny, nz, held, clamped);
+    if (clamps) clamps[i] = (uint8_t)clamped;
     if (!ok) {
       atomicOr(&ctl->flags, kFlagNonFinite);
       atomicMin(&ctl->bad_index, i);
@@ -198,9 +201,10 @@ void launch_wcsph_density(hipStream_t s, const float4* P, float4* V, float* scal
 }
 
 void launch_wcsph_force(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
-                        const StepParams& prm, float4* PA, float4* VA, float* forces) {
+                        const StepParams& prm, float4* PA, float4* VA, float* forces, uint8_t* clamps) {
   if (prm.n == 0) return;
-  hipLaunchKernelGGL(k_wcsph_force, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA, forces);
+  hipLaunchKernelGGL(k_wcsph_force, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
+                     clamps);
 }
 
 void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,

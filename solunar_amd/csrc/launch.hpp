@@ -21,7 +21,8 @@ void launch_scan(hipStream_t s, uint32_t* in, uint32_t* out, uint32_t* tile_sums
 void launch_scatter(hipStream_t s, const float4* P, uint32_t n, const uint2* slot_rank, const uint32_t* start,
                     uint2* pairs, const Control* ctl);
 void launch_rank_gather(hipStream_t s, const float4* PA, const float4* VA, uint32_t n, float radius,
-                        const uint2* pairs, const uint32_t* start, float4* PB, float4* VB, const Control* ctl);
+                        const uint2* pairs, const uint32_t* start, float4* PB, float4* VB, const Control* ctl,
+                        uint32_t* src = nullptr);
 void launch_list_count(hipStream_t s, const float4* lp, const StepParams& prm, uint32_t* counts, int kind);
 void launch_list_emit(hipStream_t s, const float4* lp, const StepParams& prm, const uint32_t* list_start,
                       uint32_t* items, int kind);
@@ -31,7 +32,7 @@ void launch_list_density(hipStream_t s, const float4* P, float4* V, const StepPa
 // WCSPH (kernels_wcsph.hip)
 void launch_wcsph_density(hipStream_t s, const float4* P, float4* V, float* scaled, const StepParams& prm);
 void launch_wcsph_force(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
-                        const StepParams& prm, float4* PA, float4* VA, float* forces);
+                        const StepParams& prm, float4* PA, float4* VA, float* forces, uint8_t* clamps = nullptr);
 void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
                          const StepParams& prm, float4* PA, float4* VA);
 

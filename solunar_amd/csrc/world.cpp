@@ -182,6 +182,7 @@ int32_t World::ensure_particles(size_t n) {
 // add_particle (src/fluid/mod.rs:412-426), in order, stopping at the first
 // refusal.
 int32_t World::add_particles(size_t count, const float* xyz, const float* vxyz, uint32_t* ids) {
+  ++version_;
   DeviceGuard g(device_);
   size_t ok = 0;
   int32_t status = SFL_OK;
@@ -228,6 +229,7 @@ int32_t World::add_particles(size_t count, const float* xyz, const float* vxyz, 
 // fill_box (src/fluid/mod.rs:493-524), optionally jittered (see
 // sfl_fill_box_jittered).
 int32_t World::fill_box(sfl_vec3 low, sfl_vec3 high, float jitter, uint32_t seed, size_t* count_out) {
+  ++version_;
   if (!finite_vec(low) || !finite_vec(high) || !std::isfinite(jitter)) {
     detail_ = "fluid box";
     return SFL_NON_FINITE;
@@ -278,6 +280,7 @@ int32_t World::fill_box(sfl_vec3 low, sfl_vec3 high, float jitter, uint32_t seed
 // set_domain (src/fluid/mod.rs:433-444): validated, but a box is refused as
 // unsupported on the device path.
 int32_t World::set_domain(const sfl_vec3* low, const sfl_vec3* high) {
+  ++version_;
   if (!low && !high) return SFL_OK;
   if (!low || !high) return SFL_OUT_OF_RANGE;
   if (!finite_vec(*low) || !finite_vec(*high)) {
@@ -292,6 +295,7 @@ int32_t World::set_domain(const sfl_vec3* low, const sfl_vec3* high) {
 // ---- walls (src/fluid/boundary.rs) ------------------------------------------
 
 int32_t World::add_box_container(sfl_vec3 low, sfl_vec3 high, uint32_t* id_out) {
+  ++version_;
   DeviceGuard g(device_);
   int32_t st = check_box(low, high, detail_);
   if (st != SFL_OK) return st;
@@ -332,6 +336,7 @@ int32_t World::add_box_container(sfl_vec3 low, sfl_vec3 high, uint32_t* id_out) 
 }
 
 int32_t World::add_wall(sfl_vec3 low, sfl_vec3 high, uint32_t* id_out) {
+  ++version_;
   DeviceGuard g(device_);
   int32_t st = check_box(low, high, detail_);
   if (st != SFL_OK) return st;
@@ -359,6 +364,7 @@ int32_t World::add_wall(sfl_vec3 low, sfl_vec3 high, uint32_t* id_out) {
 }
 
 int32_t World::add_boundary_points(size_t count, const float* xyz, uint32_t* id_out) {
+  ++version_;
   DeviceGuard g(device_);
   for (size_t i = 0; i < count; ++i)
     if (!finite3(xyz + 3 * i)) {
@@ -402,6 +408,7 @@ int32_t World::add_wall_points(const std::vector<float>& pts, uint32_t* id_out) 
 
 // remove_boundary (boundary.rs:94-115), no terrains.
 int32_t World::remove_boundary(uint32_t id, int32_t* removed) {
+  ++version_;
   DeviceGuard g(device_);
   size_t before = wall_owner_.size();
   std::vector<float> keep_pos;
@@ -753,6 +760,7 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
 }
 
 int32_t World::step(float dt, sfl_report* out) {
+  ++version_;
   DeviceGuard g(device_);
   sfl_report r{};
   if (!std::isfinite(dt) || dt < 0.0f) {
@@ -916,6 +924,7 @@ int32_t World::dfsph(float dt, sfl_report& r) {
 // ---- neighbour build and densities on their own ---------------------------
 
 int32_t World::update_neighbors() {
+  ++version_;
   DeviceGuard g(device_);
   if (n_ == 0) {
     lists_valid_ = true;
@@ -983,6 +992,7 @@ int32_t World::emit_lists() {
 // compute_densities (src/fluid/mod.rs:621-675) over the lists of the last
 // build with the current positions.
 int32_t World::compute_densities() {
+  ++version_;
   DeviceGuard g(device_);
   if (!lists_valid_) return SFL_OUT_OF_RANGE;
   int32_t st = emit_lists();

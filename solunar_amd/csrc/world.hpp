@@ -67,9 +67,11 @@ hipError_t DeviceBuffer<T>::reserve(size_t n, size_t keep, hipStream_t s) {
 uint64_t hash_particles(const float4* p, const float4* v, size_t n);
 
 class Batch;
+class Tape;
 
 class World {
   friend class Batch;
+  friend class Tape;
 
  public:
   static int32_t create(const sfl_settings& s, World** out, std::string& why);
@@ -148,6 +150,7 @@ class World {
   float radius_ = 0, reach_ = 0, scale_ = 0, mass_ = 0;
   size_t n_ = 0;
   uint32_t next_id_ = 0;
+  uint64_t version_ = 0;  // bumped by every call that may change the particles
   float vmax_ = 0;
   bool vmax_valid_ = true;
   bool keep_forces_ = false;

@@ -95,6 +95,12 @@ def library() -> ctypes.CDLL:
         "sfl_batch_memory_bytes": (sz, [P]),
         "sfl_batch_graph_nodes": (ctypes.c_uint64, [P]),
         "sfl_batch_synchronize": (i32, [P]),
+        "sfl_wcsph_record": (i32, [P, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(P)]),
+        "sfl_wcsph_adjoint": (i32, [P, fp, fp, fp, fp, fp]),
+        "sfl_tape_destroy": (None, [P]),
+        "sfl_tape_substeps": (ctypes.c_uint64, [P]),
+        "sfl_tape_memory_bytes": (sz, [P]),
+        "sfl_tape_error_detail": (ctypes.c_char_p, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -316,6 +322,54 @@ class FluidWorld:
 
     def synchronize(self) -> None:
         self._check(self._lib.sfl_synchronize(self._h))
+
+
+class Tape:
+    """A recorded WCSPH rollout of a world and its reverse pass
+    (include/solunar_fluid.h, sfl_wcsph_record / sfl_wcsph_adjoint)."""
+
+    def __init__(self, world: "FluidWorld", dt: float, steps: int, every: int = 20):
+        self._lib = library()
+        self.world = world  # the tape reads the world's arrays; keep it alive
+        h = ctypes.c_void_p()
+        st = self._lib.sfl_wcsph_record(world._h, dt, steps, every, ctypes.byref(h))
+        if st:
+            raise PhysicsError(st, self._lib.sfl_error_detail(world._h).decode())
+        self._h = h
+        self.n = len(world)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sfl_tape_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def adjoint(self, xbar, vbar=None):
+        """dL/dx, dL/dv of the final state (per id, (n, 3)) -> dL/dx0,
+        dL/dv0 per id and dL/d(viscosity)."""
+        xb = _f32(xbar)
+        vb = _f32(np.zeros_like(xb) if vbar is None else vbar)
+        if len(xb) != self.n or len(vb) != self.n:
+            raise ValueError("adjoints need one row per particle")
+        x0 = np.zeros((self.n, 3), np.float32)
+        v0 = np.zeros((self.n, 3), np.float32)
+        alpha = ctypes.c_float()
+        st = self._lib.sfl_wcsph_adjoint(self._h, _ptr(xb, ctypes.c_float), _ptr(vb, ctypes.c_float),
+                                         _ptr(x0, ctypes.c_float), _ptr(v0, ctypes.c_float), ctypes.byref(alpha))
+        if st:
+            raise PhysicsError(st, self._lib.sfl_tape_error_detail(self._h).decode())
+        return x0, v0, float(alpha.value)
+
+    def substeps(self) -> int:
+        return int(self._lib.sfl_tape_substeps(self._h))
+
+    def memory_bytes(self) -> int:
+        return int(self._lib.sfl_tape_memory_bytes(self._h))
 
 
 class FluidBatch:

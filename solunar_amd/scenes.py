@@ -8,6 +8,9 @@ low corner, lattice sites nudged by up to +-0.05 spacing per axis):
   dam_small   WCSPH, spacing 0.02, 0.4 x 0.8 x 0.4 block: 16,000 particles
   dam_1m      DFSPH, spacing 0.005, same block: 1,024,000 particles
   dam_16m     DFSPH, spacing 0.002: 16,000,000 particles
+  adjoint_64k WCSPH, spacing 0.01, 0.4 x 0.4 x 0.4 block: 64,000 particles with
+              seeded N(0, 0.1 m/s) initial velocities, 200 steps of 1e-4 s,
+              loss = mean |x_T - x_T*|^2 against the viscosity-0.08 rollout
   batch_256   256 WCSPH dam breaks at spacing 0.02, block sides uniform in
               [0.2, 0.8] m per axis, redrawn until the block holds 2,000 to
               64,000 particles
@@ -158,3 +161,53 @@ def batch_worlds(make_world, count: int = 256, seed: int = SEED, settings: Fluid
         assert got == n, (got, n)
         worlds.append(w)
     return worlds
+
+
+ADJOINT_BLOCK = (0.4, 0.4, 0.4)
+ADJOINT_SEED = 2024
+
+
+def settings_adjoint_64k(viscosity: float = 0.05) -> FluidSettings:
+    """WCSPH at spacing 0.01 for the differentiable rollout."""
+    return replace(settings_dam_small(), spacing=0.01, viscosity=viscosity)
+
+
+def adjoint_positions(spacing: float = 0.01) -> np.ndarray:
+    """The jittered 0.4 m cube of the differentiable rollout, in fill order
+    (ids 0..63,999 at spacing 0.01), as fill_box_jittered lays it out."""
+    f = np.float32
+    sp = f(spacing)
+    n = [int(np.floor(f(side) / sp)) for side in ADJOINT_BLOCK]
+    half = sp * f(0.5)
+    state = np.uint32(SEED)
+    amp = f(2.0) * f(JITTER) * sp
+    umax = f(4294967295.0)
+    out = np.zeros((n[0] * n[1] * n[2], 3), np.float32)
+    k = 0
+
+    def draw():
+        nonlocal state
+        s = int(state)
+        s ^= (s << 13) & 0xFFFFFFFF
+        s ^= s >> 17
+        s ^= (s << 5) & 0xFFFFFFFF
+        state = np.uint32(s)
+        return (f(s) / umax - f(0.5)) * amp
+
+    for y in range(n[1]):
+        for z in range(n[2]):
+            for x in range(n[0]):
+                at = [half + f(x) * sp, half + f(y) * sp, half + f(z) * sp]
+                j = [draw(), draw(), draw()]
+                out[k] = [at[0] + j[0], at[1] + j[1], at[2] + j[2]]
+                k += 1
+    return out
+
+
+def adjoint_velocities(n: int, seed: int = ADJOINT_SEED) -> np.ndarray:
+    """Initial velocities N(0, 0.1 m/s) per component, f32."""
+    return np.random.default_rng(seed).normal(0.0, 0.1, size=(n, 3)).astype(np.float32)
+
+
+def container(world) -> None:
+    world.add_box_container((0.0, 0.0, 0.0), BOX)

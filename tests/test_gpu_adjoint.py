@@ -1,0 +1,192 @@
+"""The device WCSPH adjoint (sfl_wcsph_record / sfl_wcsph_adjoint) against
+the oracle's FP64 reference gradients and FP64 central differences.
+
+The device runs the reference's f32 rollout bit for bit and its reverse pass
+in FP32. Its FP64 reference is the oracle's reverse pass in FP64 along the
+same f32 rollout (AdjointTape<float, double>), seeded with the same dL/dx_T:
+the per-particle gradients must match within 1e-4 (relative L2 norm),
+measured 3e-6 on the oracle's own f32 reverse pass. The gradient of the FP64
+rollout differs more, because the f32 and FP64 trajectories part (6.5e-6 m
+after 200 steps on the 64k scene) and some pairs then sit on the other side
+of a kink (the viscosity's min(u, 0), the Tait max): 2e-2 per particle on
+the 64k scene, 6e-5 for the viscosity gradient; those are stated and held to
+5e-2 and 1e-3. Central differences of the FP64 rollout pin the derivation:
+the device's viscosity gradient of its own loss must match the FP64
+difference quotient (eps 1e-3) within 1e-3 (measured 5.9e-4), and the FP64
+adjoint's velocity gradient along a seeded direction the quotient at eps
+1e-5 (at eps 1e-3 the piecewise-smooth loss of the full rollout has not
+converged; test_oracle_adjoint.py); the device's velocity gradient along
+that direction differs from it by the trajectory split (1.5e-2). Every
+reverse pass also checks that each replayed segment ends bit-equal to the
+recorded run.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle.oracle import OracleWorld
+from solunar_amd import scenes
+from solunar_amd.fluid import FluidWorld, Tape
+from solunar_amd.rollout import rollout
+from solunar_amd.types import DFSPH, FluidSettings, PhysicsError
+
+pytestmark = pytest.mark.gpu
+
+SMALL_BOX = (0.4, 0.4, 0.4)
+
+
+def small_settings(alpha):
+    return FluidSettings(spacing=0.02, viscosity=alpha)
+
+
+def small_scene():
+    w = OracleWorld(small_settings(0.05))
+    w.add_box_container((0, 0, 0), SMALL_BOX)
+    w.fill_box_jittered((0, 0, 0), (0.2, 0.3, 0.2), 0.05, 12345)
+    x0 = w.positions()
+    v0 = np.random.default_rng(3).normal(0.0, 0.1, size=x0.shape).astype(np.float32)
+    return x0, v0
+
+
+def oracle_run(settings, box, x0, v0, steps, dt, record=False, precision=64):
+    w = OracleWorld(settings, precision=precision, threads=16)
+    w.add_box_container((0, 0, 0), box)
+    w.add_particles_f64(x0.astype(np.float64), v0.astype(np.float64))
+    if record:
+        w.record(dt, steps, 64)
+    else:
+        for _ in range(steps):
+            w.step(dt)
+    return w, w.positions()[np.argsort(w.ids())]
+
+
+def device_run(settings, box, x0, v0, steps, dt, every):
+    w = FluidWorld(settings)
+    w.add_box_container((0, 0, 0), box)
+    w.add_particles(x0, v0)
+    tape = Tape(w, dt, steps, every)
+    xT = np.empty_like(x0)
+    xT[w.ids()] = w.positions()
+    return w, tape, xT
+
+
+def rel_norm(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_device_adjoint_matches_the_fp64_reference_on_a_small_scene():
+    x0, v0 = small_scene()
+    steps, dt = 30, 1e-4
+    print("case: small adjoint", flush=True)
+    _, target64 = oracle_run(small_settings(0.08), SMALL_BOX, x0, v0, steps, dt)
+    _, x64 = oracle_run(small_settings(0.05), SMALL_BOX, x0, v0, steps, dt)
+    seed = (2.0 * (x64 - target64) / len(x0)).astype(np.float32)
+    ow, x32 = oracle_run(small_settings(0.05), SMALL_BOX, x0, v0, steps, dt, record=True, precision=32)
+    gx64, gv64, ga64 = ow.adjoint(seed)
+    w, tape, xT = device_run(small_settings(0.05), SMALL_BOX, x0, v0, steps, dt, every=7)
+    assert np.array_equal(xT, x32), "the device rollout is the reference's f32 rollout"
+    assert tape.substeps() == steps
+    gx, gv, ga = tape.adjoint(seed)
+    assert rel_norm(gv, gv64) < 1e-4
+    assert rel_norm(gx, gx64) < 1e-4
+    assert abs(ga - ga64) / abs(ga64) < 1e-4
+    # the tape can run its reverse pass again with another seed
+    gx2, gv2, ga2 = tape.adjoint(2.0 * seed.astype(np.float32))
+    assert np.allclose(gv2, 2.0 * gv, rtol=1e-5, atol=0)
+
+
+def test_the_rollout_harness_backpropagates_through_the_tape():
+    x0, v0 = small_scene()
+    steps, dt = 30, 1e-4
+    print("case: torch harness", flush=True)
+    build = lambda w: w.add_box_container((0, 0, 0), SMALL_BOX)  # noqa: E731
+    with torch.no_grad():
+        target = rollout(torch.from_numpy(v0), torch.tensor(0.08), x0, small_settings(0.08), dt, steps, 10, build)
+    v = torch.from_numpy(v0.copy()).requires_grad_(True)
+    alpha = torch.tensor(0.05, requires_grad=True)
+    xT = rollout(v, alpha, x0, small_settings(0.05), dt, steps, 10, build)
+    loss = ((xT.double() - target.double()) ** 2).sum() / len(x0)
+    loss.backward()
+    # the same gradient straight from the tape
+    w, tape, xT2 = device_run(small_settings(0.05), SMALL_BOX, x0, v0, steps, dt, every=10)
+    assert np.array_equal(xT.detach().numpy(), xT2)
+    seed = (2.0 * (xT2.astype(np.float64) - target.numpy()) / len(x0)).astype(np.float32)
+    _, gv, ga = tape.adjoint(seed)
+    assert np.allclose(v.grad.numpy(), gv, rtol=1e-6, atol=0)
+    assert abs(alpha.grad.item() - ga) <= 1e-6 * abs(ga)
+    # the viscosity gradient against the FP64 central difference
+    _, target64 = oracle_run(small_settings(0.08), SMALL_BOX, x0, v0, steps, dt)
+    ap, am = float(np.float32(0.051)), float(np.float32(0.049))
+    lp = np.sum((oracle_run(small_settings(ap), SMALL_BOX, x0, v0, steps, dt)[1] - target64) ** 2) / len(x0)
+    lm = np.sum((oracle_run(small_settings(am), SMALL_BOX, x0, v0, steps, dt)[1] - target64) ** 2) / len(x0)
+    fd = (lp - lm) / (ap - am)
+    assert abs(alpha.grad.item() - fd) / abs(fd) < 1e-3
+
+
+def test_the_64k_rollout_gradients():
+    # 64,000 particles, 200 steps of 1e-4 s, checkpoints every 20 steps
+    s = scenes.settings_adjoint_64k
+    x0 = scenes.adjoint_positions()
+    v0 = scenes.adjoint_velocities(len(x0))
+    steps, dt = 200, 1e-4
+    print("case: 64k adjoint, FP64 oracle runs", flush=True)
+    _, target64 = oracle_run(s(0.08), scenes.BOX, x0, v0, steps, dt)
+    ow, x64 = oracle_run(s(0.05), scenes.BOX, x0, v0, steps, dt, record=True)
+    seed = (2.0 * (x64 - target64) / len(x0)).astype(np.float32)
+    _, gv64, ga64 = ow.adjoint(seed)
+    del ow
+    print("case: 64k adjoint, f32 rollout with the FP64 reverse pass", flush=True)
+    ow, x32 = oracle_run(s(0.05), scenes.BOX, x0, v0, steps, dt, record=True, precision=32)
+    _, gv32, ga32 = ow.adjoint(seed)
+    del ow
+    print("case: 64k adjoint, device", flush=True)
+    _, t_tape, target = device_run(s(0.08), scenes.BOX, x0, v0, steps, dt, every=20)
+    del t_tape
+    w, tape, xT = device_run(s(0.05), scenes.BOX, x0, v0, steps, dt, every=20)
+    assert np.array_equal(xT, x32), "the device rollout is the reference's f32 rollout"
+    _, gv, ga = tape.adjoint(seed)
+    err_v, err_a = rel_norm(gv, gv32), abs(ga - ga32) / abs(ga32)
+    print(f"device vs FP64 reverse pass of the f32 rollout: v0 {err_v:.2e}, alpha {err_a:.2e}", flush=True)
+    assert err_v < 1e-4 and err_a < 1e-4
+    err_v, err_a = rel_norm(gv, gv64), abs(ga - ga64) / abs(ga64)
+    print(f"device vs gradient of the FP64 rollout: v0 {err_v:.2e}, alpha {err_a:.2e}", flush=True)
+    assert err_v < 5e-2 and err_a < 1e-3
+    # end to end with the device's own loss
+    _, gv_own, ga_own = tape.adjoint(2.0 * (xT - target) / len(x0))
+    print("case: 64k adjoint, FP64 central differences", flush=True)
+    L = lambda a, v: np.sum((oracle_run(s(a), scenes.BOX, x0, v, steps, dt)[1] - target64) ** 2) / len(x0)  # noqa
+    ap, am = float(np.float32(0.051)), float(np.float32(0.049))
+    fd_a = (L(ap, v0) - L(am, v0)) / (ap - am)
+    d = np.random.default_rng(7).normal(size=v0.shape)
+    d /= np.linalg.norm(d)
+    eps = 1e-5
+    fd_v = (L(0.05, v0 + eps * d) - L(0.05, v0 - eps * d)) / (2 * eps)
+    err_fa = abs(ga_own - fd_a) / abs(fd_a)
+    err_fv64 = abs(float(np.sum(gv64 * d)) - fd_v) / abs(fd_v)
+    err_fv = abs(float(np.sum(gv_own * d)) - fd_v) / abs(fd_v)
+    print(f"FP64 central differences: device alpha {err_fa:.2e}; FP64 adjoint v0 along d {err_fv64:.2e}; "
+          f"device v0 along d {err_fv:.2e}", flush=True)
+    assert err_fa < 1e-3 and err_fv64 < 1e-3
+    # the device's velocity gradient is that of the f32 rollout; against the
+    # FP64 rollout's difference quotient it carries the trajectory split
+    assert err_fv < 5e-2
+
+
+def test_a_tape_refuses_a_changed_world_and_a_dfsph_world():
+    x0, v0 = small_scene()
+    w = FluidWorld(small_settings(0.05))
+    w.add_box_container((0, 0, 0), SMALL_BOX)
+    w.add_particles(x0, v0)
+    tape = Tape(w, 1e-4, 3, 2)
+    w.step(1e-4)
+    with pytest.raises(PhysicsError) as e:
+        tape.adjoint(np.zeros_like(x0))
+    assert e.value.name == "OutOfRange"
+    d = FluidWorld(FluidSettings(solver=DFSPH, spacing=0.02))
+    d.add_particles(x0[:10])
+    with pytest.raises(PhysicsError) as e:
+        Tape(d, 1e-4, 3, 2)
+    assert e.value.name == "InvalidSettings"
+    with pytest.raises(PhysicsError) as e:
+        Tape(w, 1e-4, 0, 2)
+    assert e.value.name == "OutOfRange"
