@@ -25,6 +25,7 @@ constexpr uint32_t kFlagNonFinite = 2u;  // an advance produced a non-finite val
 constexpr uint32_t kFlagBarrier = 4u;    // a grid barrier gave up waiting
 
 constexpr uint32_t kFlagPairs = 8u;     // a particle had more neighbours than the list holds
+constexpr uint32_t kFlagWide = 32u;     // a neighbour sat past the width of its cell's mask
 
 constexpr int kResidualLogCap = 256;
 constexpr int kStripes = 64;
@@ -118,12 +119,23 @@ struct StepParams {
   Control* ctl;
   // per-substep neighbour lists (DFSPH): slot k of particle i at
   // pairs[k * pair_stride + i] = (neighbour index, bits of slope(d)/d);
-  // fluid entries first, then walls; counts[i] = fluid | walls << 16
+  // counts[i] = fluid | walls << 16. In the mask formats the lists hold the
+  // walls only; fluid neighbours are masks[k * pair_stride + i], k the
+  // neighbour cell in the reference's order (dz, dy, dx), bit b set when
+  // particle start[cell k] + b is a neighbour, and own_slot[i] the
+  // particle's own cell. In the list format fluid entries come first.
   uint2* pairs;
   uint32_t* pair_counts;
   uint32_t pair_stride;
   uint32_t pair_cap;
+  void* masks;
+  uint32_t* own_slot;
+  int32_t format;  // kMasks16, kMasks32 or kLists
 };
+
+constexpr int kMasks16 = 0;
+constexpr int kMasks32 = 1;
+constexpr int kLists = 2;
 
 // ---- Wendland C2 (src/fluid/mod.rs:126-165) ------------------------------
 

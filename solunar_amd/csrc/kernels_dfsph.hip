@@ -1,13 +1,18 @@
 // DFSPH substep on gfx950 (src/fluid/dfsph.rs:79-103): density and factors,
 // the divergence and density Jacobi solves as one persistent launch each, viscosity with gravity, and the advance.
 //
-// Neighbour lists: the density pass scans the 27 cells once per substep and
-// writes, for every particle, its neighbours in the reference's order with
-// the factor f = slope(d)/d of each pair (0 at d = 0). Positions do not move
-// until the advance, so every later pass of the substep rebuilds the pair's
-// kernel gradient as (x_i - x_j) * f, the same IEEE values the reference
-// caches per pair (dfsph.rs:24-41, 108-136), for 8 list bytes instead of 12
-// gradient bytes and without a second cell scan.
+// Neighbours: the density pass scans the 27 cells once per substep and
+// records, for every particle, which particles of each of the 27 cells are
+// its neighbours, as one bit mask per cell (16 or 32 bits; bit b is the
+// cell's b-th particle in id order). Positions do not move until the
+// advance, so every later pass of the substep walks the masks in the
+// reference's order (cells dz, dy, dx; ids ascending) and rebuilds each
+// pair's kernel gradient as (x_i - x_j) * f with f = slope(d)/d recomputed
+// by the same expression: the same IEEE values the reference caches per pair
+// (dfsph.rs:24-41, 108-136), for 54 mask bytes per particle instead of 8 list
+// bytes per neighbour. Walls keep explicit lists of (wall, f). A cell with
+// more particles than its mask has bits moves the world to 32-bit masks,
+// and past that to explicit lists of every neighbour (the list format).
 //
 // Stop test: the reference averages the per-particle errors with an f32 left
 // fold in particle order (dfsph.rs:263-267), which no parallel sum can
@@ -36,7 +41,65 @@ __device__ __forceinline__ uint2 list_entry(const uint2* lp, size_t at) {
 }
 
 __device__ __forceinline__ bool list_aborted(const Control* ctl) {
-  return ctl->abort != 0 || (ctl->flags & kFlagPairs) != 0;
+  return ctl->abort != 0 || (ctl->flags & (kFlagPairs | kFlagWide)) != 0;
+}
+
+template <int FORMAT>
+struct MaskOf {
+  using T = uint16_t;
+  static constexpr uint32_t bits = 16;
+};
+template <>
+struct MaskOf<kMasks32> {
+  using T = uint32_t;
+  static constexpr uint32_t bits = 32;
+};
+
+// f = slope(d)/d of a pair with |o|^2 = sq, 0 at d = 0: the expression the
+// density pass evaluates, so the same bits.
+__device__ __forceinline__ float pair_factor(float sq, const StepParams& prm) {
+  float distance = sqrtf(sq);
+  return distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.scale) / distance;
+}
+
+// Walks particle i's fluid neighbours from its masks in the reference's
+// order, four at a time: body(j[4], ok[4]) gets the next up to four
+// neighbours (slots past the end point at i itself and are not ok). The
+// masks and cell starts of one dz plane (9 cells) are loaded together.
+template <int FORMAT, class Body>
+__device__ __forceinline__ void mask_walk(const StepParams& prm, uint32_t i, uint32_t own, int64_t sysz, int64_t sz,
+                                          Body&& body) {
+  using Mask = typename MaskOf<FORMAT>::T;
+  const Mask* __restrict__ masks = static_cast<const Mask*>(prm.masks) + i;
+  const size_t stride = prm.pair_stride;
+  const uint32_t* __restrict__ start = prm.cell_start;
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz) {
+    Mask mk[9];
+    uint32_t cs[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) mk[q] = masks[(size_t)((dz + 1) * 9 + q) * stride];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      int64_t dy = q / 3 - 1, dx = q % 3 - 1;
+      cs[q] = mk[q] ? start[(int64_t)own + dx * sysz + dy * sz + dz] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      uint32_t m = mk[q];
+      while (m) {
+        uint32_t j[4];
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          ok[u] = m != 0u;
+          j[u] = ok[u] ? cs[q] + (uint32_t)__builtin_ctz(m) : i;
+          m = ok[u] ? m & (m - 1u) : m;
+        }
+        body(j, ok);
+      }
+    }
+  }
 }
 
 // Visits particle i's list: fluid(j, f) for its fluid neighbours, then
@@ -60,10 +123,13 @@ __device__ __forceinline__ void visit_list(const StepParams& prm, uint32_t i, Fl
 
 // compute_densities (src/fluid/mod.rs:621-675) fused with gradients + factors
 // (dfsph.rs:108-163), the neighbour count the divergence solve uses
-// (dfsph.rs:249-255) and the substep's lists. rho goes to V.w; alpha to
-// `alpha` with its sign bit set for particles with fewer than 20 fluid + wall
-// neighbours. A particle with more neighbours than the lists hold flags the
-// world; later launches of the substep then do nothing and the host regrows.
+// (dfsph.rs:249-255) and the substep's neighbours (masks or lists). rho goes
+// to V.w; alpha to `alpha` with its sign bit set for particles with fewer
+// than 20 fluid + wall neighbours. A particle with more listed neighbours
+// than the lists hold, or a neighbour past its cell mask's width, flags the
+// world; later launches of the substep then do nothing and the host regrows
+// or widens.
+template <int FORMAT>
 __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict__ P, float4* __restrict__ V,
                                                        float* __restrict__ alpha, StepParams prm) {
   Control* ctl = prm.ctl;
@@ -82,20 +148,56 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
     const uint32_t cap = prm.pair_cap;
     uint2* out = prm.pairs + i;
     const size_t stride = prm.pair_stride;
-    visit_near(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
-               [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-                 float distance = sqrtf(sq);
-                 sum += kernel_value(distance, prm.radius, prm.scale);
-                 float f = distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.scale) / distance;
-                 float gx = ox * f, gy = oy * f, gz = oz * f;
-                 float qx = gx * prm.mass, qy = gy * prm.mass, qz = gz * prm.mass;
-                 sx = sx + qx;
-                 sy = sy + qy;
-                 sz = sz + qz;
-                 squares += dot3(qx, qy, qz, qx, qy, qz);
-                 if (nf < cap) out[nf * stride] = make_uint2(j, __float_as_uint(f));
-                 ++nf;
-               });
+    auto fluid = [&](uint32_t j, float ox, float oy, float oz, float sq) {
+      float distance = sqrtf(sq);
+      sum += kernel_value(distance, prm.radius, prm.scale);
+      float f = distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.scale) / distance;
+      float gx = ox * f, gy = oy * f, gz = oz * f;
+      float qx = gx * prm.mass, qy = gy * prm.mass, qz = gz * prm.mass;
+      sx = sx + qx;
+      sy = sy + qy;
+      sz = sz + qz;
+      squares += dot3(qx, qy, qz, qx, qy, qz);
+      if (FORMAT == kLists && nf < cap) out[nf * stride] = make_uint2(j, __float_as_uint(f));
+      ++nf;
+    };
+    if constexpr (FORMAT == kLists) {
+      visit_near(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+                 [&](uint32_t j, float4, float ox, float oy, float oz, float sq) { fluid(j, ox, oy, oz, sq); });
+    } else {
+      // visit_near's walk, recording each cell's neighbours as a mask
+      using Mask = typename MaskOf<FORMAT>::T;
+      Mask* masks = static_cast<Mask*>(prm.masks) + i;
+      bool wide = false;
+      int32_t cx = cell_coord(at.x, prm.radius), cy = cell_coord(at.y, prm.radius), cz = cell_coord(at.z, prm.radius);
+      prm.own_slot[i] = (uint32_t)cell_slot(cx, cy, cz, lo, span);
+      int k = 0;
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx, ++k) {
+            int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
+            uint32_t m = 0;
+            if (c >= 0) {
+              uint32_t s0 = prm.cell_start[c], e0 = prm.cell_start[c + 1];
+              for (uint32_t j = s0; j < e0; ++j) {
+                float4 pj = P[j];
+                float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+                float sq = dot3(ox, oy, oz, ox, oy, oz);
+                if (sq < prm.reach) {
+                  fluid(j, ox, oy, oz, sq);
+                  uint32_t b = j - s0;
+                  if (b < MaskOf<FORMAT>::bits)
+                    m |= 1u << b;
+                  else
+                    wide = true;
+                }
+              }
+            }
+            masks[(size_t)k * stride] = (Mask)m;
+          }
+      if (wide) atomicOr(&ctl->flags, kFlagWide);
+    }
+    const uint32_t wall0 = FORMAT == kLists ? nf : 0u;
     if (prm.wall_cells)
       visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
                  [&](uint32_t w, float4, float ox, float oy, float oz, float sq) {
@@ -106,17 +208,18 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
                    sx = sx + gx * prm.mass;
                    sy = sy + gy * prm.mass;
                    sz = sz + gz * prm.mass;
-                   uint32_t k = nf + nw;
+                   uint32_t k = wall0 + nw;
                    if (k < cap) out[k * stride] = make_uint2(w, __float_as_uint(f));
                    ++nw;
                  });
-    need = nf + nw;
-    if (need > cap) atomicOr(&ctl->flags, kFlagPairs);
+    uint32_t listed = wall0 + nw;
+    if (listed > cap) atomicOr(&ctl->flags, kFlagPairs);
+    need = listed;
     prm.pair_counts[i] = nf | (nw << 16);
     float density = prm.mass * (sum + wsum);
     float total = dot3(sx, sy, sz, sx, sy, sz) + squares;
     float a = total > 0.0f ? 1.0f / total : 0.0f;
-    alpha[i] = need < 20u ? -a : a;
+    alpha[i] = nf + nw < 20u ? -a : a;
     V[i].w = density;
     comp = density / prm.rest - 1.0f;
   }
@@ -175,7 +278,7 @@ struct SolveArgs {
   const float4* P;      // positions, fixed during the solve
   float4* V;            // velocities (updated in place); w = density
   const float* alpha;   // factors, sign bit = surface particle
-  float* kappa;         // stiffness of the last rate pass
+  float4* kappa;        // (x, stiffness) of the last rate pass: the push gathers both at once
   float* errors;        // error of the last rate pass
   double* partial;      // per chunk error sums
   uint32_t* pcount;     // per chunk count of errors > 0
@@ -191,10 +294,12 @@ struct SolveArgs {
 // each half of the block walks every other chunk of the block's list; in a
 // chunk, thread t owns particle 512c + t, which is lane t % 64, term k = t / 64 of
 // the stop sum's chunk.
+template <int FORMAT>
 __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
   if (list_aborted(ctl)) return;
+  const int64_t szz = ctl->span[2], sysz = (int64_t)ctl->span[1] * szz;
   // the chunks of XCD x are the x-th eighth of the sorted particles (the
   // grid is a multiple of 8 blocks); its blocks share them round-robin
   const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, rank = blockIdx.x / kXcds;
@@ -224,27 +329,54 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
         uint32_t nf = counts & 0xffffu, nw = counts >> 16;
         const uint2* lp = prm.pairs + i;
         const size_t stride = prm.pair_stride;
-        // Entries go four at a time: the next four list loads are issued
-        // before the current four neighbours are gathered and summed in list
-        // order. Slots past the list point at the particle itself, which
-        // keeps every load in bounds; their terms are skipped.
         uint2 nxt[4];
+        if constexpr (FORMAT != kLists) {
+          mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
+            float4 pj[4], vj[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
-        uint32_t k = 0;
-        for (; k < nf; k += 4) {
-          uint2 cur[4];
-          float4 pj[4], vj[4];
+            for (int u = 0; u < 4; ++u) {
+              pj[u] = P[j[u]];
+              vj[u] = V[j[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              if (ok[u]) {
+                float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
+                float f = pair_factor(dot3(ox, oy, oz, ox, oy, oz), prm);
+                float gx = ox * f, gy = oy * f, gz = oz * f;
+                rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
+              }
+            }
+          });
+        } else {
+        // Entries go four at a time, in a three-stage pipeline: while batch
+        // k is summed in list order, the neighbours of batch k + 1 are being
+        // gathered and the list entries of batch k + 2 loaded. Slots past
+        // the list point at the particle itself, which keeps every load in
+        // bounds; their terms are skipped.
+        uint2 cur[4];
+        float4 pj[4], vj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          cur[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
+          nxt[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(i, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          pj[u] = P[cur[u].x];
+          vj[u] = V[cur[u].x];
+        }
+        for (uint32_t k = 0; k < nf; k += 4) {
+          float4 pn[4], vn[4];
+          uint2 after[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            cur[u] = nxt[u];
-            nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(i, 0u);
+            pn[u] = P[nxt[u].x];
+            vn[u] = V[nxt[u].x];
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            pj[u] = P[cur[u].x];
-            vj[u] = V[cur[u].x];
-          }
+          for (int u = 0; u < 4; ++u)
+            after[u] = k + 8 + u < nf ? list_entry(lp, (k + 8 + u) * stride) : make_uint2(i, 0u);
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             if (k + u < nf) {
@@ -253,12 +385,21 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
               rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
             }
           }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            cur[u] = nxt[u];
+            nxt[u] = after[u];
+            pj[u] = pn[u];
+            vj[u] = vn[u];
+          }
+        }
         }
         // walls the same way (a particle at a corner lists ~40 of them)
-        const uint32_t nt = nf + nw;
+        const uint32_t w0 = FORMAT == kLists ? nf : 0u;
+        const uint32_t nt = w0 + nw;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) nxt[u] = nf + u < nt ? list_entry(lp, (nf + u) * stride) : make_uint2(0u, 0u);
-        for (k = nf; k < nt; k += 4) {
+        for (int u = 0; u < 4; ++u) nxt[u] = w0 + u < nt ? list_entry(lp, (w0 + u) * stride) : make_uint2(0u, 0u);
+        for (uint32_t k = w0; k < nt; k += 4) {
           uint2 cur[4];
           float4 pw[4];
 #pragma unroll
@@ -289,7 +430,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
           target = r / dt;
           error = dt * r / prm.rest;
         }
-        a.kappa[i] = target * fabsf(fa);
+        a.kappa[i] = make_float4(at.x, at.y, at.z, target * fabsf(fa));
         a.errors[i] = error;
       }
     }
@@ -356,48 +497,75 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
       uint32_t i = c * kChunk + tc;
       if (i >= prm.n) continue;
-      float4 at = P[i];
-      float own = a.kappa[i];
+      float4 at = a.kappa[i];  // x as P holds it, own stiffness in w
+      float own = at.w;
       float cx = 0.0f, cy = 0.0f, cz = 0.0f;
       uint32_t counts = prm.pair_counts[i];
       uint32_t nf = counts & 0xffffu, nw = counts >> 16;
       const uint2* lp = prm.pairs + i;
       const size_t stride = prm.pair_stride;
       uint2 nxt[4];
+      if constexpr (FORMAT != kLists) {
+        mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
+          float4 pj[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
-      uint32_t k = 0;
-      for (; k < nf; k += 4) {
-        uint2 cur[4];
-        float4 pj[4];
-        float kj[4];
+          for (int u = 0; u < 4; ++u) pj[u] = a.kappa[j[u]];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          cur[u] = nxt[u];
-          nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(i, 0u);
-        }
+          for (int u = 0; u < 4; ++u) {
+            if (ok[u]) {
+              float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
+              float f = pair_factor(dot3(ox, oy, oz, ox, oy, oz), prm);
+              float gx = ox * f, gy = oy * f, gz = oz * f;
+              float k2 = mass * (own + pj[u].w);
+              cx = cx - gx * k2;
+              cy = cy - gy * k2;
+              cz = cz - gz * k2;
+            }
+          }
+        });
+      } else {
+      uint2 cur[4];
+      float4 pj[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          pj[u] = P[cur[u].x];
-          kj[u] = a.kappa[cur[u].x];
-        }
+      for (int u = 0; u < 4; ++u) {
+        cur[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
+        nxt[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(i, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pj[u] = a.kappa[cur[u].x];
+      for (uint32_t k = 0; k < nf; k += 4) {
+        float4 pn[4];
+        uint2 after[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pn[u] = a.kappa[nxt[u].x];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          after[u] = k + 8 + u < nf ? list_entry(lp, (k + 8 + u) * stride) : make_uint2(i, 0u);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           if (k + u < nf) {
             float f = __uint_as_float(cur[u].y);
             float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
-            float k2 = mass * (own + kj[u]);
+            float k2 = mass * (own + pj[u].w);
             cx = cx - gx * k2;
             cy = cy - gy * k2;
             cz = cz - gz * k2;
           }
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          cur[u] = nxt[u];
+          nxt[u] = after[u];
+          pj[u] = pn[u];
+        }
       }
-      const uint32_t nt = nf + nw;
+      }
+      const uint32_t w0 = FORMAT == kLists ? nf : 0u;
+      const uint32_t nt = w0 + nw;
       const float kw = mass * own;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) nxt[u] = nf + u < nt ? list_entry(lp, (nf + u) * stride) : make_uint2(0u, 0u);
-      for (k = nf; k < nt; k += 4) {
+      for (int u = 0; u < 4; ++u) nxt[u] = w0 + u < nt ? list_entry(lp, (w0 + u) * stride) : make_uint2(0u, 0u);
+      for (uint32_t k = w0; k < nt; k += 4) {
         uint2 cur[4];
         float4 pw[4];
 #pragma unroll
@@ -438,6 +606,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
 // (src/fluid/step.rs:229-251, with wall_viscous step.rs:262-301) + gravity,
 // v += push * dt, over the substep's lists. Reads VB, writes VA (density
 // carried along).
+template <int FORMAT>
 __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                        StepParams prm, float4* __restrict__ VA,
                                                        float* __restrict__ forces) {
@@ -451,29 +620,51 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
   float density = vi.w;
   float px = 0.0f, py = 0.0f, pz = 0.0f;
   float awx = 0.0f, awy = 0.0f, awz = 0.0f;
-  visit_list(
-      prm, i,
-      [&](uint32_t j, float f) {
-        if (j == i) return;
-        float4 pj = PB[j];
-        float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
-        float4 vj = VB[j];
-        float toward = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz);
-        if (toward < 0.0f) {
-          float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
-          float damp = -prm.nu * squeeze * 2.0f / (density + vj.w);
-          float k = prm.mass * damp;
-          px = px - (ox * f) * k;
-          py = py - (oy * f) * k;
-          pz = pz - (oz * f) * k;
+  auto fluid = [&](uint32_t j, float f) {
+    float4 pj = PB[j];
+    float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+    float4 vj = VB[j];
+    float toward = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz);
+    if (toward < 0.0f) {
+      float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
+      float damp = -prm.nu * squeeze * 2.0f / (density + vj.w);
+      float k = prm.mass * damp;
+      px = px - (ox * f) * k;
+      py = py - (oy * f) * k;
+      pz = pz - (oz * f) * k;
+    }
+  };
+  auto wall = [&](uint32_t w, float f) {
+    float4 pw = W[w];
+    awx = awx + -((at.x - pw.x) * f);
+    awy = awy + -((at.y - pw.y) * f);
+    awz = awz + -((at.z - pw.z) * f);
+  };
+  if constexpr (FORMAT == kLists) {
+    visit_list(
+        prm, i,
+        [&](uint32_t j, float f) {
+          if (j != i) fluid(j, f);
+        },
+        wall);
+  } else {
+    const int64_t szz = ctl->span[2], sysz = (int64_t)ctl->span[1] * szz;
+    mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (ok[u] && j[u] != i) {
+          float4 pj = PB[j[u]];
+          float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+          fluid(j[u], pair_factor(dot3(ox, oy, oz, ox, oy, oz), prm));
         }
-      },
-      [&](uint32_t w, float f) {
-        float4 pw = W[w];
-        awx = awx + -((at.x - pw.x) * f);
-        awy = awy + -((at.y - pw.y) * f);
-        awz = awz + -((at.z - pw.z) * f);
-      });
+      }
+    });
+    uint32_t nw = prm.pair_counts[i] >> 16;
+    for (uint32_t k = 0; k < nw; ++k) {
+      uint2 e = prm.pairs[(size_t)k * prm.pair_stride + i];
+      wall(e.x, __uint_as_float(e.y));
+    }
+  }
   // wall_viscous: normalize_or(away, ZERO), motion into the walls only
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
   uint32_t counts = prm.pair_counts[i];
@@ -490,7 +681,8 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
     float tn = rmin(dot3(vi.x, vi.y, vi.z, nx, ny, nz), 0.0f);
     float ix = nx * tn, iy = ny * tn, iz = nz * tn;
     const uint2* p = prm.pairs + i;
-    for (uint32_t k = nf; k < nf + nw; ++k) {
+    const uint32_t w0 = FORMAT == kLists ? nf : 0u;
+    for (uint32_t k = w0; k < w0 + nw; ++k) {
       uint2 e = p[(size_t)k * prm.pair_stride];
       float f = __uint_as_float(e.y);
       float4 pw = W[e.x];
@@ -563,9 +755,19 @@ __global__ __launch_bounds__(256) void k_dfsph_repair(const float4* __restrict__
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
+#define SFL_FORMAT_LAUNCH(KERNEL, format, ...)                                    \
+  do {                                                                            \
+    if ((format) == kMasks16)                                                     \
+      hipLaunchKernelGGL(KERNEL<kMasks16>, __VA_ARGS__);                          \
+    else if ((format) == kMasks32)                                                \
+      hipLaunchKernelGGL(KERNEL<kMasks32>, __VA_ARGS__);                          \
+    else                                                                          \
+      hipLaunchKernelGGL(KERNEL<kLists>, __VA_ARGS__);                            \
+  } while (0)
+
 void launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm) {
   if (prm.n == 0) return;
-  hipLaunchKernelGGL(k_dfsph_density, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
+  SFL_FORMAT_LAUNCH(k_dfsph_density, prm.format, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
 }
 
 // Blocks of the persistent solve. Every block must be resident at once for
@@ -580,9 +782,16 @@ int solve_grid_limit(int blocks_per_cu) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&threads, hipDeviceAttributeMaxThreadsPerMultiProcessor, dev) != hipSuccess) return 0;
   int per = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_dfsph_solve),
-                                                   kSolveThreads, 0) != hipSuccess)
-    return 0;
+  // the smallest occupancy of the three formats' solve kernels
+  const void* kernels[3] = {reinterpret_cast<const void*>(k_dfsph_solve<kMasks16>),
+                            reinterpret_cast<const void*>(k_dfsph_solve<kMasks32>),
+                            reinterpret_cast<const void*>(k_dfsph_solve<kLists>)};
+  per = 1 << 30;
+  for (const void* k : kernels) {
+    int p = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k, kSolveThreads, 0) != hipSuccess) return 0;
+    per = p < per ? p : per;
+  }
   per = per < threads / (int)kSolveThreads ? per : threads / (int)kSolveThreads;
   int cap = blocks_per_cu > 0 && blocks_per_cu <= 2 ? blocks_per_cu : 1;
   per = per < cap ? per : cap;
@@ -590,7 +799,7 @@ int solve_grid_limit(int blocks_per_cu) {
 }
 
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
-                              float* kappa, float* errors, double* partial, uint32_t* pcount, int which,
+                              float4* kappa, float* errors, double* partial, uint32_t* pcount, int which,
                               float tolerance, uint32_t min_it, uint32_t max_it, int grid) {
   if (prm.n == 0) return hipSuccess;
   SolveArgs a;
@@ -617,14 +826,14 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   // A plain launch: the grid is sized to fit the device at once (two blocks
   // per CU at most), so every block is resident for the barrier; it also
   // keeps the launch capturable in a graph.
-  hipLaunchKernelGGL(k_dfsph_solve, dim3(blocks), dim3(kSolveThreads), 0, s, a);
+  SFL_FORMAT_LAUNCH(k_dfsph_solve, prm.format, dim3(blocks), dim3(kSolveThreads), 0, s, a);
   return hipGetLastError();
 }
 
 void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,
                           float* forces) {
   if (prm.n == 0) return;
-  hipLaunchKernelGGL(k_dfsph_viscous, blocks_for(prm.n, 256), 256, 0, s, PB, VB, prm, VA, forces);
+  SFL_FORMAT_LAUNCH(k_dfsph_viscous, prm.format, blocks_for(prm.n, 256), 256, 0, s, PB, VB, prm, VA, forces);
 }
 
 void launch_dfsph_advance(hipStream_t s, const float4* PB, const StepParams& prm, float4* PA, float4* VA,

@@ -40,7 +40,7 @@ void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, cons
 void launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
 int solve_grid_limit(int blocks_per_cu);
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
-                              float* kappa, float* errors, double* partial, uint32_t* pcount, int which,
+                              float4* kappa, float* errors, double* partial, uint32_t* pcount, int which,
                               float tolerance, uint32_t min_it, uint32_t max_it, int grid);
 void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,
                           float* forces);

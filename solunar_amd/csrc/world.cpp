@@ -137,6 +137,11 @@ int32_t World::init_device() {
   // a tuning knob; the default is the widest grid that stays resident
   const char* knob = std::getenv("SOLUNAR_SOLVE_BLOCKS_PER_CU");
   solve_grid_ = solve_grid_limit(knob ? std::atoi(knob) : 0);
+  // SOLUNAR_NEIGHBOUR_FORMAT (0: 16-bit masks, 1: 32-bit masks, 2: lists,
+  // the default) picks the DFSPH neighbour format a world starts with; masks
+  // widen on demand from there
+  const char* fmt = std::getenv("SOLUNAR_NEIGHBOUR_FORMAT");
+  if (fmt) format_ = std::min(std::max(std::atoi(fmt), (int)kMasks16), (int)kLists);
   if (solve_grid_ <= 0) {
     detail_ = "occupancy query for the persistent solve failed";
     return SFL_DEVICE_ERROR;
@@ -572,6 +577,9 @@ StepParams World::params(float dt) const {
   p.pair_counts = pair_counts_.ptr;
   p.pair_stride = pair_stride_;
   p.pair_cap = pair_cap_;
+  p.masks = masks_.ptr;
+  p.own_slot = own_slot_.ptr;
+  p.format = format_;
   return p;
 }
 
@@ -599,6 +607,11 @@ int32_t World::ensure_pairs(uint32_t cap) {
   pairs_.release();
   SFL_HIP(pairs_.reserve((size_t)cap * stride, 0, stream_), "hipMalloc(neighbour lists)");
   SFL_HIP(pair_counts_.reserve(stride, 0, stream_), "hipMalloc(neighbour counts)");
+  if (format_ != kLists) {
+    // 27 masks per particle at the widest mask (32 bits)
+    SFL_HIP(masks_.reserve((size_t)27 * stride, 0, stream_), "hipMalloc(neighbour masks)");
+    SFL_HIP(own_slot_.reserve(stride, 0, stream_), "hipMalloc(neighbour masks)");
+  }
   pair_cap_ = cap;
   pair_stride_ = stride;
   return SFL_OK;
@@ -856,7 +869,8 @@ int32_t World::dfsph(float dt, sfl_report& r) {
   float pull = length3(s.gravity.x, s.gravity.y, s.gravity.z);
   st = ensure_cells_for_bounds();
   if (st != SFL_OK) return st;
-  st = ensure_pairs(std::max<uint32_t>(pair_cap_, 64));
+  // lists of walls only in the mask formats, of every neighbour otherwise
+  st = ensure_pairs(std::max<uint32_t>(pair_cap_, format_ == kLists ? 64 : 16));
   if (st != SFL_OK) return st;
   float left = dt;
   lists_valid_ = false;
@@ -877,6 +891,17 @@ int32_t World::dfsph(float dt, sfl_report& r) {
       if (st != SFL_OK) return st;
       if (ctl_host_->flags & kFlagOverflow) {
         st = ensure_cells(ctl_host_->need_volume);
+        if (st != SFL_OK) return st;
+        prm = params(small);
+        continue;
+      }
+      if (ctl_host_->flags & kFlagWide) {
+        // a cell held more particles than a mask has bits: widen the masks,
+        // or list every neighbour
+        format_ = format_ == kMasks16 ? kMasks32 : kLists;
+        uint32_t cap = format_ == kLists ? std::max<uint32_t>(totals().pairs_max + 64, 64) : pair_cap_;
+        pair_cap_ = 0;  // reallocate with the new format
+        st = ensure_pairs(cap);
         if (st != SFL_OK) return st;
         prm = params(small);
         continue;
@@ -1140,6 +1165,7 @@ size_t World::device_bytes() const {
   auto b = [](const auto& buf) { return buf.cap * sizeof(*buf.ptr); };
   return sizeof(Control) + b(pa_) + b(va_) + b(pb_) + b(vb_) + b(aux_) + b(kappa_) + b(errors_) + b(forces_) +
          b(slot_rank_) + b(scatter_) + b(counts_) + b(cell_start_) + b(tile_sums_) + b(scan_total_) + b(pairs_) +
+         b(masks_) + b(own_slot_) +
          b(pair_counts_) + b(partial_) + b(pcount_) + b(walls_d_) + b(wall_start_d_) + b(fstart_) + b(fitems_) +
          b(wstart_) + b(witems_) + b(list_counts_);
 }

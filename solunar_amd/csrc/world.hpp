@@ -176,13 +176,19 @@ class World {
   Control* ctl_host_ = nullptr;  // pinned host mirror
   int solve_grid_ = 0;
   DeviceBuffer<float4> pa_, va_, pb_, vb_;
-  DeviceBuffer<float> aux_, kappa_, errors_, forces_;
+  DeviceBuffer<float> aux_, errors_, forces_;
+  DeviceBuffer<float4> kappa_;  // (x, stiffness) per particle during a solve
   DeviceBuffer<uint2> slot_rank_, scatter_;
   DeviceBuffer<uint32_t> counts_, cell_start_, tile_sums_, scan_total_;
   uint64_t cell_cap_ = 0;
   DeviceBuffer<uint2> pairs_;
   DeviceBuffer<uint32_t> pair_counts_;
   uint32_t pair_cap_ = 0, pair_stride_ = 0;
+  // DFSPH neighbour format: explicit lists (the default, fastest on
+  // MI355X), or the compact masks (27 per particle, 16 or 32 bits, with each
+  // particle's own cell), which widen on demand
+  DeviceBuffer<uint32_t> masks_, own_slot_;
+  int format_ = kLists;
   DeviceBuffer<double> partial_;
   DeviceBuffer<uint32_t> pcount_;
   DeviceBuffer<float4> walls_d_;

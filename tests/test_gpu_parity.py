@@ -10,6 +10,7 @@ device's fixed-shape FP64 stop sum); mode 0 (the reference's f32 fold) is
 compared on iteration counts in test_oracle_dfsph.py.
 """
 import math
+from dataclasses import replace
 
 import numpy as np
 import pytest
@@ -278,5 +279,35 @@ def test_sphere_obstacles_as_boundary_points_match(solver):
     dt = 1e-4 if solver == WCSPH else 5e-4
     for k in range(10):
         rg, rc = gpu.step(dt), cpu.step(dt)
+        assert rg == rc, (k, rg, rc)
+    assert gpu.state_hash() == cpu.state_hash()
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_dfsph_matches_in_every_neighbour_format(fmt, monkeypatch):
+    # 16-bit masks, 32-bit masks, explicit lists: one arithmetic, one result
+    monkeypatch.setenv("SOLUNAR_NEIGHBOUR_FORMAT", str(fmt))
+    s = replace(scenes.settings_dam_1m(), spacing=0.02)
+    gpu, cpu = pair(s, residual_mode=1)
+    for w in (gpu, cpu):
+        scenes.dam_break(w)
+    for k in range(6):
+        rg, rc = gpu.step(5e-4), cpu.step(5e-4)
+        assert rg == rc, (fmt, k, rg, rc)
+    assert gpu.state_hash() == cpu.state_hash()
+
+
+def test_crowded_cells_widen_the_neighbour_format():
+    # particles at half the spacing put 64 in one cell: past 16- and 32-bit
+    # masks, so the world falls back to explicit lists and still matches
+    s = FluidSettings(solver=DFSPH, spacing=0.02)
+    gpu, cpu = pair(s, residual_mode=1)
+    g = np.arange(0.0, 0.04, 0.01, dtype=np.float32) + np.float32(0.321)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    for w in (gpu, cpu):
+        w.add_box_container((0, 0, 0), (0.8, 0.8, 0.8))
+        w.add_particles(pts)
+    for k in range(2):
+        rg, rc = gpu.step(1e-4), cpu.step(1e-4)
         assert rg == rc, (k, rg, rc)
     assert gpu.state_hash() == cpu.state_hash()
