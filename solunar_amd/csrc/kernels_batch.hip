@@ -199,11 +199,26 @@ __global__ __launch_bounds__(BS) void k_b_bounds(BatchArgs b, const float4* __re
       mn[a] = min(mn[a], __shfl_xor(mn[a], off, 64));
       mx[a] = max(mx[a], __shfl_xor(mx[a], off, 64));
     }
+  // the block's waves fold in LDS, then one lane per block updates the
+  // scene (every lane of a block belongs to one scene)
+  constexpr uint32_t kWaves = BS / 64;
+  __shared__ int32_t part[kWaves][6];
+  const uint32_t w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&st.bmin[a], mn[a]);
-      atomicMax(&st.bmax[a], mx[a]);
+      part[w][a] = mn[a];
+      part[w][3 + a] = mx[a];
     }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int a = threadIdx.x;
+    int32_t v = part[0][a];
+    for (uint32_t k = 1; k < kWaves; ++k) v = a < 3 ? min(v, part[k][a]) : max(v, part[k][a]);
+    if (a < 3)
+      atomicMin(&st.bmin[a], v);
+    else
+      atomicMax(&st.bmax[a - 3], v);
+  }
 }
 
 // One block: every scene's cell box and the first slot of its range (scenes
