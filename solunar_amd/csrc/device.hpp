@@ -133,6 +133,18 @@ struct StepParams {
   int32_t format;  // kMasks16, kMasks32 or kLists
 };
 
+// LDS per particle a tile of the list solve stages: float4 + float2 (rate
+// pass), float4 (push pass)
+constexpr uint32_t kTileBytes = 24;
+
+// A fluid entry of the list format carries the neighbour's index in its low
+// 28 bits and, in the top 4, which of the 3x3 neighbour columns (dx, dy) it
+// sits in: (dx + 1) * 3 + (dy + 1). Wall entries carry their index alone.
+constexpr uint32_t kListIndexBits = 28;
+constexpr uint32_t kListIndexMask = (1u << kListIndexBits) - 1u;
+__device__ __forceinline__ uint32_t list_index(uint32_t x) { return x & kListIndexMask; }
+__device__ __forceinline__ uint32_t list_column(uint32_t x) { return x >> kListIndexBits; }
+
 constexpr int kMasks16 = 0;
 constexpr int kMasks32 = 1;
 constexpr int kLists = 2;

@@ -113,7 +113,7 @@ __device__ __forceinline__ void visit_list(const StepParams& prm, uint32_t i, Fl
   uint32_t k = 0;
   for (; k < nf; ++k) {
     uint2 e = p[(size_t)k * stride];
-    fluid(e.x, __uint_as_float(e.y));
+    fluid(list_index(e.x), __uint_as_float(e.y));
   }
   for (; k < nf + nw; ++k) {
     uint2 e = p[(size_t)k * stride];
@@ -162,8 +162,22 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
       ++nf;
     };
     if constexpr (FORMAT == kLists) {
-      visit_near(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
-                 [&](uint32_t j, float4, float ox, float oy, float oz, float sq) { fluid(j, ox, oy, oz, sq); });
+      // visit_near's walk, tagging each entry with its neighbour column
+      int32_t cx = cell_coord(at.x, prm.radius), cy = cell_coord(at.y, prm.radius), cz = cell_coord(at.z, prm.radius);
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
+            if (c < 0) continue;
+            const uint32_t column = (uint32_t)((dx + 1) * 3 + (dy + 1)) << kListIndexBits;
+            uint32_t s0 = prm.cell_start[c], e0 = prm.cell_start[c + 1];
+            for (uint32_t j = s0; j < e0; ++j) {
+              float4 pj = P[j];
+              float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+              float sq = dot3(ox, oy, oz, ox, oy, oz);
+              if (sq < prm.reach) fluid(j | column, ox, oy, oz, sq);
+            }
+          }
     } else {
       // visit_near's walk, recording each cell's neighbours as a mask
       using Mask = typename MaskOf<FORMAT>::T;
@@ -287,177 +301,573 @@ struct SolveArgs {
   float tolerance;
   uint32_t min_iterations;
   uint32_t max_iterations;
+  uint32_t lds_cap;     // particles one half-block stages in LDS; 0: no tiles
 };
 
+// ---- one particle of a pass ------------------------------------------------
+//
+// density_rate (dfsph.rs:166-181) and the stiffness and error of solve
+// (dfsph.rs:222-262) for particle i, then push_apart (dfsph.rs:184-216).
+// Neighbour data comes from global memory (`Glob`) or from the half-block's
+// LDS tile (`Lds`); both give the same values, so the same bits.
+
+struct Glob {
+  const float4* __restrict__ P;
+  const float4* __restrict__ V;
+  const float4* __restrict__ K;
+};
+
+// A tile's staged neighbourhood in LDS: x and vx in A, vy and vz in B for
+// the rate pass; (x, stiffness) in Q for the push pass. Staged slot of list
+// entry e of a particle in processed column p: index(e) + delta[p][column(e)].
+struct Lds {
+  const float4* A;
+  const float2* B;
+  const float4* Q;
+  const int32_t* delta;  // 9 entries of the particle's column
+};
+
+template <int FORMAT>
+__device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, float4 at, float4 vi, uint32_t nf,
+                                            const Glob& g, int64_t sysz, int64_t szz) {
+  const StepParams& prm = a.prm;
+  float rate = 0.0f;
+  const uint2* lp = prm.pairs + i;
+  const size_t stride = prm.pair_stride;
+  if constexpr (FORMAT != kLists) {
+    mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
+      float4 pj[4], vj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        pj[u] = g.P[j[u]];
+        vj[u] = g.V[j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (ok[u]) {
+          float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
+          float f = pair_factor(dot3(ox, oy, oz, ox, oy, oz), prm);
+          float gx = ox * f, gy = oy * f, gz = oz * f;
+          rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
+        }
+      }
+    });
+  } else {
+    // Entries go four at a time, in a three-stage pipeline: while batch k
+    // is summed in list order, the neighbours of batch k + 1 are being
+    // gathered and the list entries of batch k + 2 loaded. Slots past the
+    // list point at the particle itself, which keeps every load in bounds;
+    // their terms are skipped.
+    uint2 cur[4], nxt[4];
+    float4 pj[4], vj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cur[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
+      nxt[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(i, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      pj[u] = g.P[list_index(cur[u].x)];
+      vj[u] = g.V[list_index(cur[u].x)];
+    }
+    for (uint32_t k = 0; k < nf; k += 4) {
+      float4 pn[4], vn[4];
+      uint2 after[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        pn[u] = g.P[list_index(nxt[u].x)];
+        vn[u] = g.V[list_index(nxt[u].x)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) after[u] = k + 8 + u < nf ? list_entry(lp, (k + 8 + u) * stride) : make_uint2(i, 0u);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k + u < nf) {
+          float f = __uint_as_float(cur[u].y);
+          float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
+          rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cur[u] = nxt[u];
+        nxt[u] = after[u];
+        pj[u] = pn[u];
+        vj[u] = vn[u];
+      }
+    }
+  }
+  return rate;
+}
+
+__device__ __forceinline__ float fluid_rate_lds(const SolveArgs& a, uint32_t i, float4 at, float4 vi, uint32_t nf,
+                                                const Lds& l) {
+  const StepParams& prm = a.prm;
+  float rate = 0.0f;
+  const uint2* lp = prm.pairs + i;
+  const size_t stride = prm.pair_stride;
+  uint2 nxt[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(0u, 0u);
+  for (uint32_t k = 0; k < nf; k += 4) {
+    uint2 cur[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cur[u] = nxt[u];
+      nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k + u < nf) {
+        int32_t s = (int32_t)list_index(cur[u].x) + l.delta[list_column(cur[u].x)];
+        float4 pv = l.A[s];
+        float2 w = l.B[s];
+        float f = __uint_as_float(cur[u].y);
+        float gx = (at.x - pv.x) * f, gy = (at.y - pv.y) * f, gz = (at.z - pv.z) * f;
+        rate += dot3(vi.x - pv.w, vi.y - w.x, vi.z - w.y, gx, gy, gz);
+      }
+    }
+  }
+  return rate;
+}
+
+// Walls of the rate pass, then stiffness and error; writes kappa and errors.
+template <int FORMAT>
+__device__ __forceinline__ void finish_rate(const SolveArgs& a, uint32_t i, float4 at, float4 vi, float rate,
+                                            uint32_t nf, uint32_t nw) {
+  const StepParams& prm = a.prm;
+  const float4* __restrict__ W = prm.walls;
+  const uint2* lp = prm.pairs + i;
+  const size_t stride = prm.pair_stride;
+  // walls four at a time (a particle at a corner lists ~40 of them)
+  const uint32_t w0 = FORMAT == kLists ? nf : 0u;
+  const uint32_t nt = w0 + nw;
+  uint2 nxt[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) nxt[u] = w0 + u < nt ? list_entry(lp, (w0 + u) * stride) : make_uint2(0u, 0u);
+  for (uint32_t k = w0; k < nt; k += 4) {
+    uint2 cur[4];
+    float4 pw[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cur[u] = nxt[u];
+      nxt[u] = k + 4 + u < nt ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pw[u] = W[cur[u].x];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k + u < nt) {
+        float f = __uint_as_float(cur[u].y);
+        float gx = (at.x - pw[u].x) * f, gy = (at.y - pw[u].y) * f, gz = (at.z - pw[u].z) * f;
+        rate += dot3(vi.x, vi.y, vi.z, gx, gy, gz);
+      }
+    }
+  }
+  const float dt = prm.dt;
+  rate = rate * prm.mass;
+  float fa = a.alpha[i];
+  float target, error;
+  if (a.which == 1) {
+    float over = rmax(vi.w + dt * rate - prm.rest, 0.0f);
+    target = over / (dt * dt);
+    error = over / prm.rest;
+  } else {
+    float r = signbit(fa) ? 0.0f : rmax(rate, 0.0f);
+    target = r / dt;
+    error = dt * r / prm.rest;
+  }
+  a.kappa[i] = make_float4(at.x, at.y, at.z, target * fabsf(fa));
+  a.errors[i] = error;
+}
+
+template <int FORMAT>
+__device__ __forceinline__ void rate_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, int64_t sysz,
+                                         int64_t szz) {
+  float4 at = a.P[i];
+  float4 vi = a.V[i];
+  uint32_t counts = a.prm.pair_counts[i];
+  uint32_t nf = counts & 0xffffu, nw = counts >> 16;
+  float rate;
+  if constexpr (FORMAT == kLists) {
+    rate = l ? fluid_rate_lds(a, i, at, vi, nf, *l) : fluid_rate<FORMAT>(a, i, at, vi, nf, g, sysz, szz);
+  } else {
+    rate = fluid_rate<FORMAT>(a, i, at, vi, nf, g, sysz, szz);
+  }
+  finish_rate<FORMAT>(a, i, at, vi, rate, nf, nw);
+}
+
+template <int FORMAT>
+__device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, int64_t sysz,
+                                         int64_t szz) {
+  const StepParams& prm = a.prm;
+  const float mass = prm.mass;
+  float4 at = a.kappa[i];  // x as P holds it, own stiffness in w
+  float own = at.w;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  uint32_t counts = prm.pair_counts[i];
+  uint32_t nf = counts & 0xffffu, nw = counts >> 16;
+  const uint2* lp = prm.pairs + i;
+  const size_t stride = prm.pair_stride;
+  if constexpr (FORMAT != kLists) {
+    mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
+      float4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pj[u] = g.K[j[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (ok[u]) {
+          float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
+          float f = pair_factor(dot3(ox, oy, oz, ox, oy, oz), prm);
+          float gx = ox * f, gy = oy * f, gz = oz * f;
+          float k2 = mass * (own + pj[u].w);
+          cx = cx - gx * k2;
+          cy = cy - gy * k2;
+          cz = cz - gz * k2;
+        }
+      }
+    });
+  } else if (l) {
+    uint2 nxt[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(0u, 0u);
+    for (uint32_t k = 0; k < nf; k += 4) {
+      uint2 cur[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cur[u] = nxt[u];
+        nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k + u < nf) {
+          int32_t s = (int32_t)list_index(cur[u].x) + l->delta[list_column(cur[u].x)];
+          float4 q = l->Q[s];
+          float f = __uint_as_float(cur[u].y);
+          float gx = (at.x - q.x) * f, gy = (at.y - q.y) * f, gz = (at.z - q.z) * f;
+          float k2 = mass * (own + q.w);
+          cx = cx - gx * k2;
+          cy = cy - gy * k2;
+          cz = cz - gz * k2;
+        }
+      }
+    }
+  } else {
+    uint2 cur[4], nxt[4];
+    float4 pj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cur[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
+      nxt[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(i, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pj[u] = g.K[list_index(cur[u].x)];
+    for (uint32_t k = 0; k < nf; k += 4) {
+      float4 pn[4];
+      uint2 after[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pn[u] = g.K[list_index(nxt[u].x)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) after[u] = k + 8 + u < nf ? list_entry(lp, (k + 8 + u) * stride) : make_uint2(i, 0u);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k + u < nf) {
+          float f = __uint_as_float(cur[u].y);
+          float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
+          float k2 = mass * (own + pj[u].w);
+          cx = cx - gx * k2;
+          cy = cy - gy * k2;
+          cz = cz - gz * k2;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cur[u] = nxt[u];
+        nxt[u] = after[u];
+        pj[u] = pn[u];
+      }
+    }
+  }
+  const float4* __restrict__ W = prm.walls;
+  const uint32_t w0 = FORMAT == kLists ? nf : 0u;
+  const uint32_t nt = w0 + nw;
+  const float kw = mass * own;
+  uint2 nxt[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) nxt[u] = w0 + u < nt ? list_entry(lp, (w0 + u) * stride) : make_uint2(0u, 0u);
+  for (uint32_t k = w0; k < nt; k += 4) {
+    uint2 cur[4];
+    float4 pw[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cur[u] = nxt[u];
+      nxt[u] = k + 4 + u < nt ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pw[u] = W[cur[u].x];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k + u < nt) {
+        float f = __uint_as_float(cur[u].y);
+        float gx = (at.x - pw[u].x) * f, gy = (at.y - pw[u].y) * f, gz = (at.z - pw[u].z) * f;
+        cx = cx - gx * kw;
+        cy = cy - gy * kw;
+        cz = cz - gz * kw;
+      }
+    }
+  }
+  const float dt = prm.dt;
+  float4 vi = a.V[i];
+  vi.x = vi.x + cx * dt;
+  vi.y = vi.y + cy * dt;
+  vi.z = vi.z + cz * dt;
+  a.V[i] = vi;
+}
+
+// ---- LDS tiles --------------------------------------------------------------
+//
+// The cell box is cut into tiles of 2 x 2 cell columns by `tz` cells in z
+// (about 16, the z span split evenly), numbered x-major like the cells. A
+// tile's particles are four contiguous ranges (one per column); the cells
+// that can hold their neighbours are the 4 x 4 columns around them over the
+// tile's z range widened by one cell: sixteen contiguous ranges, which the
+// half-block stages into its LDS with coalesced loads. A list entry's
+// neighbour column (dx, dy) then names the staged range directly.
+
+struct TileShape {
+  uint32_t sx, sy, sz, ntx, nty, ntz, tz, count;
+};
+
+__device__ __forceinline__ TileShape tile_shape(const Control* ctl) {
+  TileShape t;
+  t.sx = ctl->span[0];
+  t.sy = ctl->span[1];
+  t.sz = ctl->span[2];
+  t.ntx = (t.sx + 1) / 2;
+  t.nty = (t.sy + 1) / 2;
+  t.ntz = (t.sz + 15) / 16;
+  t.tz = (t.sz + t.ntz - 1) / t.ntz;
+  t.count = t.ntx * t.nty * t.ntz;
+  return t;
+}
+
+// Per half-block: the staged ranges' starts and LDS prefix, the processed
+// ranges' starts and prefix, and the slot offsets per (column, entry column).
+struct TileState {
+  uint32_t ss[16];
+  uint32_t pre[17];
+  uint32_t ps[4];
+  uint32_t ppre[5];
+  int32_t delta[4][9];
+};
+
+// Threads 0..19 of the half fill `ts` for tile `tile` (all 20 before the
+// caller's barrier); the prefixes are folded after it by `tile_prefix`.
+__device__ __forceinline__ void tile_ranges(const TileShape& sh, uint32_t tile, const uint32_t* __restrict__ start,
+                                            TileState& ts, uint32_t tc, uint32_t* len_s, uint32_t* plen_s) {
+  uint32_t tx = tile / (sh.nty * sh.ntz), ty = (tile / sh.ntz) % sh.nty, tzi = tile % sh.ntz;
+  uint32_t z0 = tzi * sh.tz, z1 = min(z0 + sh.tz, sh.sz);
+  if (tc < 16) {
+    int64_t x = (int64_t)(2 * tx) - 1 + (tc >> 2), y = (int64_t)(2 * ty) - 1 + (tc & 3);
+    uint32_t s0 = 0, n = 0;
+    if (x >= 0 && y >= 0 && x < sh.sx && y < sh.sy) {
+      uint64_t base = ((uint64_t)x * sh.sy + (uint64_t)y) * sh.sz;
+      uint32_t za = z0 > 0 ? z0 - 1 : 0, zb = min(z1 + 1, sh.sz);
+      s0 = start[base + za];
+      n = start[base + zb] - s0;
+    }
+    ts.ss[tc] = s0;
+    len_s[tc] = n;
+  } else if (tc < 20) {
+    uint32_t p = tc - 16;
+    uint64_t x = 2 * tx + (p >> 1), y = 2 * ty + (p & 1);
+    uint32_t s0 = 0, n = 0;
+    if (x < sh.sx && y < sh.sy) {
+      uint64_t base = (x * sh.sy + y) * sh.sz;
+      s0 = start[base + z0];
+      n = start[base + z1] - s0;
+    }
+    ts.ps[p] = s0;
+    plen_s[p] = n;
+  }
+}
+
+__device__ __forceinline__ void tile_prefix(TileState& ts, uint32_t tc, const uint32_t* len_s,
+                                            const uint32_t* plen_s) {
+  if (tc == 0) {
+    uint32_t acc = 0;
+    for (int k = 0; k < 16; ++k) {
+      ts.pre[k] = acc;
+      acc += len_s[k];
+    }
+    ts.pre[16] = acc;
+    acc = 0;
+    for (int k = 0; k < 4; ++k) {
+      ts.ppre[k] = acc;
+      acc += plen_s[k];
+    }
+    ts.ppre[4] = acc;
+  }
+}
+
+__device__ __forceinline__ void tile_delta(TileState& ts, uint32_t tc) {
+  if (tc < 36) {
+    uint32_t p = tc / 9, c = tc % 9;
+    int dx = (int)(c / 3) - 1, dy = (int)(c % 3) - 1;
+    uint32_t col = (uint32_t)((1 + (int)(p >> 1) + dx) * 4 + (1 + (int)(p & 1) + dy));
+    ts.delta[p][c] = (int32_t)ts.pre[col] - (int32_t)ts.ss[col];
+  }
+}
+
+// staged slot s -> global index (the 16 prefixes are ascending)
+__device__ __forceinline__ uint32_t staged_index(const TileState& ts, uint32_t s) {
+  uint32_t lo = 0;
+#pragma unroll
+  for (uint32_t step = 8; step >= 1; step >>= 1)
+    if (ts.pre[lo + step] <= s) lo += step;
+  return ts.ss[lo] + (s - ts.pre[lo]);
+}
+
+__device__ __forceinline__ uint32_t processed_index(const TileState& ts, uint32_t q, uint32_t& p) {
+  p = 0;
+  while (p < 3 && ts.ppre[p + 1] <= q) ++p;
+  return ts.ps[p] + (q - ts.ppre[p]);
+}
+
+// One pass over the tiles (rate = true: rate pass, else push pass). Tiles of
+// XCD x are the x-th eighth; block `rank` of the XCD takes every per_xcd-th,
+// its two halves alternating, in rounds so that both halves meet at every
+// block barrier.
+template <int FORMAT>
+__device__ void tiled_pass(const SolveArgs& a, bool rate, float4* lds, TileState (&tstate)[2],
+                           uint32_t (&len_s)[2][16], uint32_t (&plen_s)[2][4], const Glob& g) {
+  const StepParams& prm = a.prm;
+  const TileShape sh = tile_shape(prm.ctl);
+  const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, rank = blockIdx.x / kXcds;
+  const uint32_t t_begin = (uint32_t)((uint64_t)sh.count * xcd / kXcds);
+  const uint32_t t_end = (uint32_t)((uint64_t)sh.count * (xcd + 1) / kXcds);
+  const uint32_t mine = t_begin + rank < t_end ? (t_end - t_begin - rank + per_xcd - 1) / per_xcd : 0u;
+  const uint32_t rounds = (mine + 1) / 2;
+  const uint32_t t = threadIdx.x, half = t / kChunk, tc = t % kChunk;
+  const uint32_t cap = a.lds_cap;
+  // each half's LDS: A (float4) then B (float2) for the rate pass, Q (float4)
+  // for the push pass
+  float4* part = lds + (size_t)half * (cap * 3 / 2);
+  float4* A = part;
+  float2* B = reinterpret_cast<float2*>(part + cap);
+  float4* Q = part;
+  TileState& ts = tstate[half];
+  for (uint32_t r = 0; r < rounds; ++r) {
+    uint32_t k = 2 * r + half;
+    bool have = k < mine;
+    uint32_t tile = t_begin + rank + k * per_xcd;
+    if (have) tile_ranges(sh, tile, prm.cell_start, ts, tc, len_s[half], plen_s[half]);
+    __syncthreads();
+    if (have) tile_prefix(ts, tc, len_s[half], plen_s[half]);
+    __syncthreads();
+    const uint32_t staged = have ? ts.pre[16] : 0u;
+    const uint32_t count = have ? ts.ppre[4] : 0u;
+    const bool fits = staged <= cap;
+    if (have && fits && count) {
+      tile_delta(ts, tc);
+      for (uint32_t s = tc; s < staged; s += kChunk) {
+        uint32_t gi = staged_index(ts, s);
+        if (rate) {
+          float4 p = a.P[gi];
+          float4 v = a.V[gi];
+          A[s] = make_float4(p.x, p.y, p.z, v.x);
+          B[s] = make_float2(v.y, v.z);
+        } else {
+          Q[s] = a.kappa[gi];
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t q = tc; q < count; q += kChunk) {
+      uint32_t p;
+      uint32_t i = processed_index(ts, q, p);
+      if (fits) {
+        Lds l{A, B, Q, ts.delta[p]};
+        if (rate)
+          rate_one<FORMAT>(a, i, g, &l, 0, 0);
+        else
+          push_one<FORMAT>(a, i, g, &l, 0, 0);
+      } else if (rate) {
+        rate_one<FORMAT>(a, i, g, nullptr, 0, 0);
+      } else {
+        push_one<FORMAT>(a, i, g, nullptr, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Chunk sums of the stop test for chunks c, c + step, ...: wave w of the
+// block takes every 16th of them; lane l adds errors 512c + l + 64k in k
+// order, as defined above.
+__device__ __forceinline__ void chunk_sums(const SolveArgs& a, uint32_t first, uint32_t last, uint32_t step) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t mine = 0;
+  for (uint32_t c = first; c < last; c += step, ++mine) {
+    if (mine % kSolveWaves != wave) continue;
+    double acc = 0.0;
+    uint32_t cnt = 0;
+    for (uint32_t k = 0; k < 8; ++k) {
+      uint32_t i = c * kChunk + k * 64u + lane;
+      float e = i < a.prm.n ? a.errors[i] : 0.0f;
+      acc += (double)e;
+      cnt += e > 0.0f ? 1u : 0u;
+    }
+    for (int h = 32; h >= 1; h >>= 1) {
+      acc = acc + __shfl_down(acc, h, 64);
+      cnt += (uint32_t)__shfl_down((int)cnt, h, 64);
+    }
+    if (lane == 0) {
+      a.partial[c] = acc;
+      a.pcount[c] = cnt;
+    }
+  }
+}
+
 // solve (src/fluid/dfsph.rs:222-275) with density_rate (dfsph.rs:166-181) and
-// push_apart (dfsph.rs:184-216). Block b takes chunks b, b + grid, ...; its
-// each half of the block walks every other chunk of the block's list; in a
-// chunk, thread t owns particle 512c + t, which is lane t % 64, term k = t / 64 of
-// the stop sum's chunk.
+// push_apart (dfsph.rs:184-216). Without tiles, block b takes chunks of 512
+// particles (its XCD's eighth, every per_xcd-th), each half of the block
+// every other one, thread t owning particle 512c + t, and sums its own
+// chunks before the barrier. With tiles (the list format, LDS permitting),
+// the passes walk tiles and the chunk sums follow a barrier of their own.
 template <int FORMAT>
 __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
+  extern __shared__ float4 lds[];
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
   if (list_aborted(ctl)) return;
   const int64_t szz = ctl->span[2], sysz = (int64_t)ctl->span[1] * szz;
-  // the chunks of XCD x are the x-th eighth of the sorted particles (the
-  // grid is a multiple of 8 blocks); its blocks share them round-robin
   const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, rank = blockIdx.x / kXcds;
   const uint32_t c_begin = (uint32_t)((uint64_t)a.chunks * xcd / kXcds);
   const uint32_t c_end = (uint32_t)((uint64_t)a.chunks * (xcd + 1) / kXcds);
   const uint32_t t = threadIdx.x;
-  const uint32_t lane = t & 63u, wave = t >> 6;
   const uint32_t half = t / kChunk, tc = t % kChunk;
   __shared__ double slot[256];
   __shared__ uint32_t slot_count[256];
-  const float dt = prm.dt;
-  const float mass = prm.mass;
-  const float4* __restrict__ P = a.P;
-  float4* __restrict__ V = a.V;
-  const float4* __restrict__ W = prm.walls;
+  __shared__ TileState tstate[2];
+  __shared__ uint32_t len_s[2][16], plen_s[2][4];
+  bool tiled = false;
+  if constexpr (FORMAT == kLists) tiled = a.lds_cap > 0;
+  const Glob g{a.P, a.V, a.kappa};
   uint32_t it = 0;
   for (;;) {
     // rate pass: stiffness and error of every particle
-    for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
-      uint32_t i = c * kChunk + tc;
-      float error = 0.0f;
-      if (i < prm.n) {
-        float4 at = P[i];
-        float4 vi = V[i];
-        float rate = 0.0f;
-        uint32_t counts = prm.pair_counts[i];
-        uint32_t nf = counts & 0xffffu, nw = counts >> 16;
-        const uint2* lp = prm.pairs + i;
-        const size_t stride = prm.pair_stride;
-        uint2 nxt[4];
-        if constexpr (FORMAT != kLists) {
-          mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
-            float4 pj[4], vj[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              pj[u] = P[j[u]];
-              vj[u] = V[j[u]];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              if (ok[u]) {
-                float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
-                float f = pair_factor(dot3(ox, oy, oz, ox, oy, oz), prm);
-                float gx = ox * f, gy = oy * f, gz = oz * f;
-                rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
-              }
-            }
-          });
-        } else {
-        // Entries go four at a time, in a three-stage pipeline: while batch
-        // k is summed in list order, the neighbours of batch k + 1 are being
-        // gathered and the list entries of batch k + 2 loaded. Slots past
-        // the list point at the particle itself, which keeps every load in
-        // bounds; their terms are skipped.
-        uint2 cur[4];
-        float4 pj[4], vj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          cur[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
-          nxt[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(i, 0u);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          pj[u] = P[cur[u].x];
-          vj[u] = V[cur[u].x];
-        }
-        for (uint32_t k = 0; k < nf; k += 4) {
-          float4 pn[4], vn[4];
-          uint2 after[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            pn[u] = P[nxt[u].x];
-            vn[u] = V[nxt[u].x];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            after[u] = k + 8 + u < nf ? list_entry(lp, (k + 8 + u) * stride) : make_uint2(i, 0u);
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (k + u < nf) {
-              float f = __uint_as_float(cur[u].y);
-              float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
-              rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            cur[u] = nxt[u];
-            nxt[u] = after[u];
-            pj[u] = pn[u];
-            vj[u] = vn[u];
-          }
-        }
-        }
-        // walls the same way (a particle at a corner lists ~40 of them)
-        const uint32_t w0 = FORMAT == kLists ? nf : 0u;
-        const uint32_t nt = w0 + nw;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) nxt[u] = w0 + u < nt ? list_entry(lp, (w0 + u) * stride) : make_uint2(0u, 0u);
-        for (uint32_t k = w0; k < nt; k += 4) {
-          uint2 cur[4];
-          float4 pw[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            cur[u] = nxt[u];
-            nxt[u] = k + 4 + u < nt ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) pw[u] = W[cur[u].x];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (k + u < nt) {
-              float f = __uint_as_float(cur[u].y);
-              float gx = (at.x - pw[u].x) * f, gy = (at.y - pw[u].y) * f, gz = (at.z - pw[u].z) * f;
-              rate += dot3(vi.x, vi.y, vi.z, gx, gy, gz);
-            }
-          }
-        }
-        rate = rate * mass;
-        float fa = a.alpha[i];
-        float target;
-        if (a.which == 1) {
-          float over = rmax(vi.w + dt * rate - prm.rest, 0.0f);
-          target = over / (dt * dt);
-          error = over / prm.rest;
-        } else {
-          float r = signbit(fa) ? 0.0f : rmax(rate, 0.0f);
-          target = r / dt;
-          error = dt * r / prm.rest;
-        }
-        a.kappa[i] = make_float4(at.x, at.y, at.z, target * fabsf(fa));
-        a.errors[i] = error;
+    if (tiled) {
+      if constexpr (FORMAT == kLists) tiled_pass<FORMAT>(a, true, lds, tstate, len_s, plen_s, g);
+      if (!grid_barrier(ctl)) return;
+      chunk_sums(a, blockIdx.x, a.chunks, gridDim.x);
+    } else {
+      for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
+        uint32_t i = c * kChunk + tc;
+        if (i < prm.n) rate_one<FORMAT>(a, i, g, nullptr, sysz, szz);
       }
-    }
-    // the stop sum of this block's chunks, wave w taking chunks w, w + 8, ...
-    // of its list; lane l adds errors 512c + l + 64k in k order, as defined
-    __syncthreads();
-    {
-      uint32_t mine = 0;
-      for (uint32_t c = c_begin + rank; c < c_end; c += per_xcd, ++mine) {
-        if (mine % kSolveWaves != wave) continue;
-        double acc = 0.0;
-        uint32_t cnt = 0;
-        for (uint32_t k = 0; k < 8; ++k) {
-          uint32_t i = c * kChunk + k * 64u + lane;
-          float e = i < prm.n ? a.errors[i] : 0.0f;
-          acc += (double)e;
-          cnt += e > 0.0f ? 1u : 0u;
-        }
-        for (int h = 32; h >= 1; h >>= 1) {
-          acc = acc + __shfl_down(acc, h, 64);
-          cnt += (uint32_t)__shfl_down((int)cnt, h, 64);
-        }
-        if (lane == 0) {
-          a.partial[c] = acc;
-          a.pcount[c] = cnt;
-        }
-      }
+      __syncthreads();
+      chunk_sums(a, c_begin + rank, c_end, per_xcd);
     }
     if (!grid_barrier(ctl)) return;
     // every block folds the chunk sums the same way
@@ -494,103 +904,13 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     bool settled = it >= a.min_iterations && average <= (double)a.tolerance;
     if (settled || it >= a.max_iterations) break;
     // push pass
-    for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
-      uint32_t i = c * kChunk + tc;
-      if (i >= prm.n) continue;
-      float4 at = a.kappa[i];  // x as P holds it, own stiffness in w
-      float own = at.w;
-      float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-      uint32_t counts = prm.pair_counts[i];
-      uint32_t nf = counts & 0xffffu, nw = counts >> 16;
-      const uint2* lp = prm.pairs + i;
-      const size_t stride = prm.pair_stride;
-      uint2 nxt[4];
-      if constexpr (FORMAT != kLists) {
-        mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
-          float4 pj[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) pj[u] = a.kappa[j[u]];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (ok[u]) {
-              float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
-              float f = pair_factor(dot3(ox, oy, oz, ox, oy, oz), prm);
-              float gx = ox * f, gy = oy * f, gz = oz * f;
-              float k2 = mass * (own + pj[u].w);
-              cx = cx - gx * k2;
-              cy = cy - gy * k2;
-              cz = cz - gz * k2;
-            }
-          }
-        });
-      } else {
-      uint2 cur[4];
-      float4 pj[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        cur[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
-        nxt[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(i, 0u);
+    if (tiled) {
+      if constexpr (FORMAT == kLists) tiled_pass<FORMAT>(a, false, lds, tstate, len_s, plen_s, g);
+    } else {
+      for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
+        uint32_t i = c * kChunk + tc;
+        if (i < prm.n) push_one<FORMAT>(a, i, g, nullptr, sysz, szz);
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) pj[u] = a.kappa[cur[u].x];
-      for (uint32_t k = 0; k < nf; k += 4) {
-        float4 pn[4];
-        uint2 after[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) pn[u] = a.kappa[nxt[u].x];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          after[u] = k + 8 + u < nf ? list_entry(lp, (k + 8 + u) * stride) : make_uint2(i, 0u);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (k + u < nf) {
-            float f = __uint_as_float(cur[u].y);
-            float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
-            float k2 = mass * (own + pj[u].w);
-            cx = cx - gx * k2;
-            cy = cy - gy * k2;
-            cz = cz - gz * k2;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          cur[u] = nxt[u];
-          nxt[u] = after[u];
-          pj[u] = pn[u];
-        }
-      }
-      }
-      const uint32_t w0 = FORMAT == kLists ? nf : 0u;
-      const uint32_t nt = w0 + nw;
-      const float kw = mass * own;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) nxt[u] = w0 + u < nt ? list_entry(lp, (w0 + u) * stride) : make_uint2(0u, 0u);
-      for (uint32_t k = w0; k < nt; k += 4) {
-        uint2 cur[4];
-        float4 pw[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          cur[u] = nxt[u];
-          nxt[u] = k + 4 + u < nt ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) pw[u] = W[cur[u].x];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (k + u < nt) {
-            float f = __uint_as_float(cur[u].y);
-            float gx = (at.x - pw[u].x) * f, gy = (at.y - pw[u].y) * f, gz = (at.z - pw[u].z) * f;
-            cx = cx - gx * kw;
-            cy = cy - gy * kw;
-            cz = cz - gz * kw;
-          }
-        }
-      }
-      float4 vi = V[i];
-      vi.x = vi.x + cx * dt;
-      vi.y = vi.y + cy * dt;
-      vi.z = vi.z + cz * dt;
-      V[i] = vi;
     }
     if (!grid_barrier(ctl)) return;
     ++it;
@@ -774,13 +1094,17 @@ void launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alph
 // the grid barrier: at most what the occupancy query and the resident
 // threads per CU allow, and one 1024-thread block per CU by default: the
 // barrier's cost grows with the blocks that meet at it, and one block per
-// CU ran the 1M-particle solve fastest.
-int solve_grid_limit(int blocks_per_cu) {
+// CU ran the 1M-particle solve fastest. The list format's solve also takes
+// the LDS left beside its static arrays for its tiles (`tile_bytes`, 0 when
+// `tiles` is false or one block per CU is not the grid).
+int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
+  *tile_bytes = 0;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
-  int cus = 0, threads = 0;
+  int cus = 0, threads = 0, lds = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&threads, hipDeviceAttributeMaxThreadsPerMultiProcessor, dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
   int per = 0;
   // the smallest occupancy of the three formats' solve kernels
   const void* kernels[3] = {reinterpret_cast<const void*>(k_dfsph_solve<kMasks16>),
@@ -795,12 +1119,26 @@ int solve_grid_limit(int blocks_per_cu) {
   per = per < threads / (int)kSolveThreads ? per : threads / (int)kSolveThreads;
   int cap = blocks_per_cu > 0 && blocks_per_cu <= 2 ? blocks_per_cu : 1;
   per = per < cap ? per : cap;
+  if (tiles && per == 1) {
+    hipFuncAttributes attr;
+    const void* k = reinterpret_cast<const void*>(k_dfsph_solve<kLists>);
+    if (hipFuncGetAttributes(&attr, k) != hipSuccess) return 0;
+    int64_t room = (int64_t)lds - (int64_t)attr.sharedSizeBytes;
+    // whole staged particles of both halves, rounded down to 64 each
+    room = room / (2 * kTileBytes) / 64 * 64 * (2 * kTileBytes);
+    if (room >= 2 * kTileBytes * 512) {
+      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)room) != hipSuccess) return 0;
+      int p = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k, kSolveThreads, (size_t)room) != hipSuccess) return 0;
+      if (p >= 1) *tile_bytes = (uint32_t)room;
+    }
+  }
   return per * cus;
 }
 
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
                               float4* kappa, float* errors, double* partial, uint32_t* pcount, int which,
-                              float tolerance, uint32_t min_it, uint32_t max_it, int grid) {
+                              float tolerance, uint32_t min_it, uint32_t max_it, int grid, uint32_t tile_bytes) {
   if (prm.n == 0) return hipSuccess;
   SolveArgs a;
   a.prm = prm;
@@ -816,6 +1154,8 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   a.tolerance = tolerance;
   a.min_iterations = min_it;
   a.max_iterations = max_it;
+  const bool tiled = prm.format == kLists && tile_bytes > 0;
+  a.lds_cap = tiled ? tile_bytes / (2 * kTileBytes) : 0u;
   // a multiple of 8 blocks so every XCD gets the same share; at least one
   // chunk per XCD group or the small grid of a small world
   uint32_t blocks = (uint32_t)grid;
@@ -826,7 +1166,7 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   // A plain launch: the grid is sized to fit the device at once (two blocks
   // per CU at most), so every block is resident for the barrier; it also
   // keeps the launch capturable in a graph.
-  SFL_FORMAT_LAUNCH(k_dfsph_solve, prm.format, dim3(blocks), dim3(kSolveThreads), 0, s, a);
+  SFL_FORMAT_LAUNCH(k_dfsph_solve, prm.format, dim3(blocks), dim3(kSolveThreads), tiled ? tile_bytes : 0u, s, a);
   return hipGetLastError();
 }
 

@@ -136,7 +136,17 @@ int32_t World::init_device() {
   // SOLUNAR_SOLVE_BLOCKS_PER_CU (1..3) narrows the persistent solve's grid,
   // a tuning knob; the default is the widest grid that stays resident
   const char* knob = std::getenv("SOLUNAR_SOLVE_BLOCKS_PER_CU");
-  solve_grid_ = solve_grid_limit(knob ? std::atoi(knob) : 0);
+  // SOLUNAR_SOLVE_TILES=0 makes the list solve gather from global memory
+  // instead of staging each tile's neighbourhood in LDS
+  const char* tiles = std::getenv("SOLUNAR_SOLVE_TILES");
+  solve_grid_ = solve_grid_limit(knob ? std::atoi(knob) : 0, !(tiles && std::atoi(tiles) == 0), &solve_lds_);
+  // SOLUNAR_SOLVE_TILE_PARTICLES caps the particles one half-block stages
+  // (rounded down to 64; tiles with more gather from global memory)
+  const char* tile_cap = std::getenv("SOLUNAR_SOLVE_TILE_PARTICLES");
+  if (tile_cap) {
+    long c = std::max(std::atol(tile_cap), 0L) / 64 * 64;
+    solve_lds_ = (uint32_t)std::min<uint64_t>(solve_lds_, (uint64_t)c * 2 * kTileBytes);
+  }
   // SOLUNAR_NEIGHBOUR_FORMAT (0: 16-bit masks, 1: 32-bit masks, 2: lists,
   // the default) picks the DFSPH neighbour format a world starts with; masks
   // widen on demand from there
@@ -755,7 +765,7 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   phase_end(SFL_PHASE_DENSITY);
   phase_begin(SFL_PHASE_SOLVE);
   SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, partial_.ptr, pcount_.ptr, 0,
-                             s.divergence_tolerance, s.min_iterations, s.max_iterations, solve_grid_),
+                             s.divergence_tolerance, s.min_iterations, s.max_iterations, solve_grid_, solve_lds_),
           "cooperative launch (divergence solve)");
   phase_end(SFL_PHASE_SOLVE);
   phase_begin(SFL_PHASE_VISCOUS);
@@ -763,7 +773,7 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   phase_end(SFL_PHASE_VISCOUS);
   phase_begin(SFL_PHASE_SOLVE);
   SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, va_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, partial_.ptr, pcount_.ptr, 1,
-                             s.density_tolerance, s.min_iterations, s.max_iterations, solve_grid_),
+                             s.density_tolerance, s.min_iterations, s.max_iterations, solve_grid_, solve_lds_),
           "cooperative launch (density solve)");
   phase_end(SFL_PHASE_SOLVE);
   phase_begin(SFL_PHASE_ADVANCE);
@@ -869,6 +879,12 @@ int32_t World::dfsph(float dt, sfl_report& r) {
   float pull = length3(s.gravity.x, s.gravity.y, s.gravity.z);
   st = ensure_cells_for_bounds();
   if (st != SFL_OK) return st;
+  // list entries keep the neighbour's index in their low 28 bits: past that
+  // many particles the masks hold the fluid neighbours
+  if (format_ == kLists && n_ > (size_t)kListIndexMask + 1) {
+    format_ = kMasks32;
+    pair_cap_ = 0;
+  }
   // lists of walls only in the mask formats, of every neighbour otherwise
   st = ensure_pairs(std::max<uint32_t>(pair_cap_, format_ == kLists ? 64 : 16));
   if (st != SFL_OK) return st;
@@ -898,6 +914,10 @@ int32_t World::dfsph(float dt, sfl_report& r) {
       if (ctl_host_->flags & kFlagWide) {
         // a cell held more particles than a mask has bits: widen the masks,
         // or list every neighbour
+        if (format_ == kMasks32 && n_ > (size_t)kListIndexMask + 1) {
+          detail_ = "a cell holds more than 32 particles in a world too large for neighbour lists";
+          return SFL_OUT_OF_RANGE;
+        }
         format_ = format_ == kMasks16 ? kMasks32 : kLists;
         uint32_t cap = format_ == kLists ? std::max<uint32_t>(totals().pairs_max + 64, 64) : pair_cap_;
         pair_cap_ = 0;  // reallocate with the new format

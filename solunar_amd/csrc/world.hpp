@@ -175,6 +175,7 @@ class World {
   Control* ctl_ = nullptr;       // device
   Control* ctl_host_ = nullptr;  // pinned host mirror
   int solve_grid_ = 0;
+  uint32_t solve_lds_ = 0;  // dynamic LDS of the list solve's tiles
   DeviceBuffer<float4> pa_, va_, pb_, vb_;
   DeviceBuffer<float> aux_, errors_, forces_;
   DeviceBuffer<float4> kappa_;  // (x, stiffness) per particle during a solve

@@ -297,6 +297,25 @@ def test_dfsph_matches_in_every_neighbour_format(fmt, monkeypatch):
     assert gpu.state_hash() == cpu.state_hash()
 
 
+@pytest.mark.parametrize("knobs", [{"SOLUNAR_SOLVE_TILES": "0"}, {"SOLUNAR_SOLVE_TILE_PARTICLES": "640"}],
+                         ids=["global-gathers", "small-tiles"])
+def test_dfsph_solve_matches_with_and_without_lds_tiles(knobs, monkeypatch):
+    # the list solve stages each tile's neighbourhood in LDS; a tile that
+    # does not fit gathers from global memory; without tiles every particle
+    # does. A cap of 640 staged particles sends the dense tiles of this scene
+    # down the global path and keeps the sparse ones in LDS.
+    for k, v in knobs.items():
+        monkeypatch.setenv(k, v)
+    s = replace(scenes.settings_dam_1m(), spacing=0.02)
+    gpu, cpu = pair(s, residual_mode=1)
+    for w in (gpu, cpu):
+        scenes.dam_break(w)
+    for k in range(6):
+        rg, rc = gpu.step(5e-4), cpu.step(5e-4)
+        assert rg == rc, (knobs, k, rg, rc)
+    assert gpu.state_hash() == cpu.state_hash()
+
+
 def test_crowded_cells_widen_the_neighbour_format():
     # particles at half the spacing put 64 in one cell: past 16- and 32-bit
     # masks, so the world falls back to explicit lists and still matches
