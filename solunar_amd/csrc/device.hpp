@@ -58,8 +58,14 @@ struct Control {
   uint32_t iterations_step[2];   // pushes summed over this step
   uint32_t rates_step;           // rate passes summed over this step
   uint32_t log_len;
+  // the persistent solve's grid barrier: blocks arrived per XCD group
+  // (blockIdx % 8), XCD groups arrived, generation; one cache line each
   alignas(128) uint32_t bar_count;
-  uint32_t bar_gen;
+  alignas(128) uint32_t bar_gen;
+  alignas(128) uint32_t bar_xcd[8][32];
+  // the list solve's tile queues: next tile of each XCD group's share, per
+  // pass parity, one cache line each; zero between launches
+  alignas(128) uint32_t tile_next[2][8][32];
   alignas(128) Stripe stripes[kStripes];
   // average error of every rate pass (solve in the sign: + divergence, -
   // density), for parity logs

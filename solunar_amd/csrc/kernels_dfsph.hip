@@ -248,14 +248,17 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
 // device's constant 100 MHz reference clock.
 constexpr uint64_t kBarrierTicks = 200000000ull;  // 2 s
 
-// Grid-wide barrier of the persistent solve (every block resident). Every
-// wave waits until its own stores have reached the L2 (a block barrier alone
-// does not), then thread 0 alone releases the block's writes at agent scope,
-// arrives, polls the generation with relaxed loads (an acquire per poll
-// would invalidate the L2 the blocks still working gather from) and acquires
-// once when it moves; one fence pair per block instead of one per wave cut
-// the barrier from 88 to about 32 microseconds at 512 blocks. Returns false
-// if it waited past the limit, having flagged the world.
+// Grid-wide barrier of the persistent solve (every block resident, the grid
+// a multiple of 8). Every wave waits until its own stores have reached the
+// L2 (a block barrier alone does not); then thread 0 of the block arrives at
+// its XCD group's count (blockIdx % 8) with a release, and the last block of
+// a group arrives at the group count; the last group bumps the generation.
+// The other blocks poll the generation with relaxed loads (an acquire per
+// poll would invalidate the L2 the blocks still working gather from) and
+// acquire once when it moves. One release and one acquire per block, and
+// 32-deep instead of 256-deep atomic chains, each word on its own cache
+// line: 7.6 instead of 29 microseconds per barrier over 256 blocks. Returns
+// false if it waited past the limit, having flagged the world.
 __device__ bool grid_barrier(Control* ctl) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_s_waitcnt(0);
@@ -263,16 +266,23 @@ __device__ bool grid_barrier(Control* ctl) {
   __shared__ int ok_s;
   if (threadIdx.x == 0) {
     int ok = 1;
-    __threadfence();
-    uint32_t gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t arrived = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == gridDim.x - 1) {
+    const uint32_t group = blockIdx.x % kXcds, per_group = gridDim.x / kXcds;
+    uint32_t* mine = &ctl->bar_xcd[group][0];
+    uint32_t gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t arrived = __hip_atomic_fetch_add(mine, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    bool last = false;
+    if (arrived == per_group - 1) {
+      __hip_atomic_store(mine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t groups = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      last = groups == kXcds - 1;
+    }
+    if (last) {
       __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       uint64_t start = wall_clock64();
       while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        __builtin_amdgcn_s_sleep(8);
+        __builtin_amdgcn_s_sleep(1);
         if (wall_clock64() - start > kBarrierTicks) {
           ok = 0;
           atomicOr(&ctl->flags, kFlagBarrier);
@@ -280,7 +290,7 @@ __device__ bool grid_barrier(Control* ctl) {
         }
       }
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     ok_s = ok;
   }
   __syncthreads();
@@ -327,7 +337,9 @@ struct Lds {
   const int32_t* delta;  // 9 entries of the particle's column
 };
 
-template <int FORMAT>
+// DEEP: the three-stage pipeline below; otherwise one batch of entries
+// ahead, for the tiled kernel's fallback, whose registers go to its tiles.
+template <int FORMAT, bool DEEP = true>
 __device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, float4 at, float4 vi, uint32_t nf,
                                             const Glob& g, int64_t sysz, int64_t szz) {
   const StepParams& prm = a.prm;
@@ -352,6 +364,32 @@ __device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, floa
         }
       }
     });
+  } else if constexpr (!DEEP) {
+    uint2 nxt[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
+    for (uint32_t k = 0; k < nf; k += 4) {
+      uint2 cur[4];
+      float4 pj[4], vj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cur[u] = nxt[u];
+        nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(i, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        pj[u] = g.P[list_index(cur[u].x)];
+        vj[u] = g.V[list_index(cur[u].x)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k + u < nf) {
+          float f = __uint_as_float(cur[u].y);
+          float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
+          rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
+        }
+      }
+    }
   } else {
     // Entries go four at a time, in a three-stage pipeline: while batch k
     // is summed in list order, the neighbours of batch k + 1 are being
@@ -406,15 +444,22 @@ __device__ __forceinline__ float fluid_rate_lds(const SolveArgs& a, uint32_t i, 
   float rate = 0.0f;
   const uint2* lp = prm.pairs + i;
   const size_t stride = prm.pair_stride;
-  uint2 nxt[4];
+  // entries stream three batches ahead of the sum: nothing waits on LDS
+  uint2 n1[4], n2[4], n3[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(0u, 0u);
+  for (int u = 0; u < 4; ++u) {
+    n1[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(0u, 0u);
+    n2[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(0u, 0u);
+    n3[u] = 8u + u < nf ? list_entry(lp, (8u + u) * stride) : make_uint2(0u, 0u);
+  }
   for (uint32_t k = 0; k < nf; k += 4) {
     uint2 cur[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      cur[u] = nxt[u];
-      nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
+      cur[u] = n1[u];
+      n1[u] = n2[u];
+      n2[u] = n3[u];
+      n3[u] = k + 12 + u < nf ? list_entry(lp, (k + 12 + u) * stride) : make_uint2(0u, 0u);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -481,7 +526,7 @@ __device__ __forceinline__ void finish_rate(const SolveArgs& a, uint32_t i, floa
   a.errors[i] = error;
 }
 
-template <int FORMAT>
+template <int FORMAT, bool DEEP = true>
 __device__ __forceinline__ void rate_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, int64_t sysz,
                                          int64_t szz) {
   float4 at = a.P[i];
@@ -490,14 +535,14 @@ __device__ __forceinline__ void rate_one(const SolveArgs& a, uint32_t i, const G
   uint32_t nf = counts & 0xffffu, nw = counts >> 16;
   float rate;
   if constexpr (FORMAT == kLists) {
-    rate = l ? fluid_rate_lds(a, i, at, vi, nf, *l) : fluid_rate<FORMAT>(a, i, at, vi, nf, g, sysz, szz);
+    rate = l ? fluid_rate_lds(a, i, at, vi, nf, *l) : fluid_rate<FORMAT, DEEP>(a, i, at, vi, nf, g, sysz, szz);
   } else {
     rate = fluid_rate<FORMAT>(a, i, at, vi, nf, g, sysz, szz);
   }
   finish_rate<FORMAT>(a, i, at, vi, rate, nf, nw);
 }
 
-template <int FORMAT>
+template <int FORMAT, bool DEEP = true>
 __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, int64_t sysz,
                                          int64_t szz) {
   const StepParams& prm = a.prm;
@@ -528,15 +573,21 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
       }
     });
   } else if (l) {
-    uint2 nxt[4];
+    uint2 n1[4], n2[4], n3[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(0u, 0u);
+    for (int u = 0; u < 4; ++u) {
+      n1[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(0u, 0u);
+      n2[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(0u, 0u);
+      n3[u] = 8u + u < nf ? list_entry(lp, (8u + u) * stride) : make_uint2(0u, 0u);
+    }
     for (uint32_t k = 0; k < nf; k += 4) {
       uint2 cur[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        cur[u] = nxt[u];
-        nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
+        cur[u] = n1[u];
+        n1[u] = n2[u];
+        n2[u] = n3[u];
+        n3[u] = k + 12 + u < nf ? list_entry(lp, (k + 12 + u) * stride) : make_uint2(0u, 0u);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -546,6 +597,32 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
           float f = __uint_as_float(cur[u].y);
           float gx = (at.x - q.x) * f, gy = (at.y - q.y) * f, gz = (at.z - q.z) * f;
           float k2 = mass * (own + q.w);
+          cx = cx - gx * k2;
+          cy = cy - gy * k2;
+          cz = cz - gz * k2;
+        }
+      }
+    }
+  } else if constexpr (!DEEP) {
+    uint2 nxt[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
+    for (uint32_t k = 0; k < nf; k += 4) {
+      uint2 cur[4];
+      float4 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cur[u] = nxt[u];
+        nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(i, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pj[u] = g.K[list_index(cur[u].x)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k + u < nf) {
+          float f = __uint_as_float(cur[u].y);
+          float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
+          float k2 = mass * (own + pj[u].w);
           cx = cx - gx * k2;
           cy = cy - gy * k2;
           cz = cz - gz * k2;
@@ -661,71 +738,70 @@ struct TileState {
   int32_t delta[4][9];
 };
 
-// Threads 0..19 of the half fill `ts` for tile `tile` (all 20 before the
-// caller's barrier); the prefixes are folded after it by `tile_prefix`.
-__device__ __forceinline__ void tile_ranges(const TileShape& sh, uint32_t tile, const uint32_t* __restrict__ start,
-                                            TileState& ts, uint32_t tc, uint32_t* len_s, uint32_t* plen_s) {
-  uint32_t tx = tile / (sh.nty * sh.ntz), ty = (tile / sh.ntz) % sh.nty, tzi = tile % sh.ntz;
-  uint32_t z0 = tzi * sh.tz, z1 = min(z0 + sh.tz, sh.sz);
-  if (tc < 16) {
-    int64_t x = (int64_t)(2 * tx) - 1 + (tc >> 2), y = (int64_t)(2 * ty) - 1 + (tc & 3);
-    uint32_t s0 = 0, n = 0;
-    if (x >= 0 && y >= 0 && x < sh.sx && y < sh.sy) {
-      uint64_t base = ((uint64_t)x * sh.sy + (uint64_t)y) * sh.sz;
-      uint32_t za = z0 > 0 ? z0 - 1 : 0, zb = min(z1 + 1, sh.sz);
-      s0 = start[base + za];
-      n = start[base + zb] - s0;
+// A tile's column ranges, in two steps so the loads of the next tile's can
+// be in flight while this tile's particles are summed. Wave 0 of a half
+// (all 64 lanes, `have` uniform): lanes 0..15 load the staged columns'
+// ranges, 16..19 the processed columns'.
+struct TileLoad {
+  uint32_t s0, n;
+};
+
+__device__ __forceinline__ TileLoad tile_load(const TileShape& sh, uint32_t tile, bool have,
+                                              const uint32_t* __restrict__ start, uint32_t lane) {
+  TileLoad r{0u, 0u};
+  if (have) {
+    uint32_t tx = tile / (sh.nty * sh.ntz), ty = (tile / sh.ntz) % sh.nty, tzi = tile % sh.ntz;
+    uint32_t z0 = tzi * sh.tz, z1 = min(z0 + sh.tz, sh.sz);
+    if (lane < 16) {
+      int64_t x = (int64_t)(2 * tx) - 1 + (lane >> 2), y = (int64_t)(2 * ty) - 1 + (lane & 3);
+      if (x >= 0 && y >= 0 && x < sh.sx && y < sh.sy) {
+        uint64_t base = ((uint64_t)x * sh.sy + (uint64_t)y) * sh.sz;
+        uint32_t za = z0 > 0 ? z0 - 1 : 0, zb = min(z1 + 1, sh.sz);
+        r.s0 = start[base + za];
+        r.n = start[base + zb] - r.s0;
+      }
+    } else if (lane < 20) {
+      uint32_t p = lane - 16;
+      uint64_t x = 2 * tx + (p >> 1), y = 2 * ty + (p & 1);
+      if (x < sh.sx && y < sh.sy) {
+        uint64_t base = (x * sh.sy + y) * sh.sz;
+        r.s0 = start[base + z0];
+        r.n = start[base + z1] - r.s0;
+      }
     }
-    ts.ss[tc] = s0;
-    len_s[tc] = n;
-  } else if (tc < 20) {
-    uint32_t p = tc - 16;
-    uint64_t x = 2 * tx + (p >> 1), y = 2 * ty + (p & 1);
-    uint32_t s0 = 0, n = 0;
-    if (x < sh.sx && y < sh.sy) {
-      uint64_t base = (x * sh.sy + y) * sh.sz;
-      s0 = start[base + z0];
-      n = start[base + z1] - s0;
-    }
-    ts.ps[p] = s0;
-    plen_s[p] = n;
   }
+  return r;
 }
 
-__device__ __forceinline__ void tile_prefix(TileState& ts, uint32_t tc, const uint32_t* len_s,
-                                            const uint32_t* plen_s) {
-  if (tc == 0) {
-    uint32_t acc = 0;
-    for (int k = 0; k < 16; ++k) {
-      ts.pre[k] = acc;
-      acc += len_s[k];
-    }
-    ts.pre[16] = acc;
-    acc = 0;
-    for (int k = 0; k < 4; ++k) {
-      ts.ppre[k] = acc;
-      acc += plen_s[k];
-    }
-    ts.ppre[4] = acc;
+// Then the same wave scans both prefixes (segments lanes 0..15, 16..19) and
+// lanes 0..35 write the slot offsets into `ts`.
+__device__ __forceinline__ void tile_scan(TileLoad r, TileState& ts, uint32_t lane) {
+  uint32_t incl = r.n;
+  for (uint32_t d = 1; d < 16; d <<= 1) {
+    uint32_t up = (uint32_t)__shfl_up((int)incl, d, 64);
+    bool inside = lane < 16 ? lane >= d : (lane < 20 && lane >= 16 + d);
+    if (inside) incl += up;
   }
-}
-
-__device__ __forceinline__ void tile_delta(TileState& ts, uint32_t tc) {
-  if (tc < 36) {
-    uint32_t p = tc / 9, c = tc % 9;
+  const uint32_t excl = incl - r.n;
+  if (lane < 16) {
+    ts.ss[lane] = r.s0;
+    ts.pre[lane] = excl;
+    if (lane == 15) ts.pre[16] = incl;
+  } else if (lane < 20) {
+    ts.ps[lane - 16] = r.s0;
+    ts.ppre[lane - 16] = excl;
+    if (lane == 19) ts.ppre[4] = incl;
+  }
+  // staged slot of list entry e in processed column p: index(e) +
+  // delta[p][column(e)], the column's LDS start less its first index
+  const int32_t here = (int32_t)excl - (int32_t)r.s0;
+  uint32_t col = 0, p = lane / 9, c = lane % 9;
+  if (lane < 36) {
     int dx = (int)(c / 3) - 1, dy = (int)(c % 3) - 1;
-    uint32_t col = (uint32_t)((1 + (int)(p >> 1) + dx) * 4 + (1 + (int)(p & 1) + dy));
-    ts.delta[p][c] = (int32_t)ts.pre[col] - (int32_t)ts.ss[col];
+    col = (uint32_t)((1 + (int)(p >> 1) + dx) * 4 + (1 + (int)(p & 1) + dy));
   }
-}
-
-// staged slot s -> global index (the 16 prefixes are ascending)
-__device__ __forceinline__ uint32_t staged_index(const TileState& ts, uint32_t s) {
-  uint32_t lo = 0;
-#pragma unroll
-  for (uint32_t step = 8; step >= 1; step >>= 1)
-    if (ts.pre[lo + step] <= s) lo += step;
-  return ts.ss[lo] + (s - ts.pre[lo]);
+  int32_t dv = __shfl(here, (int)col, 64);
+  if (lane < 36) ts.delta[p][c] = dv;
 }
 
 __device__ __forceinline__ uint32_t processed_index(const TileState& ts, uint32_t q, uint32_t& p) {
@@ -734,21 +810,64 @@ __device__ __forceinline__ uint32_t processed_index(const TileState& ts, uint32_
   return ts.ps[p] + (q - ts.ppre[p]);
 }
 
-// One pass over the tiles (rate = true: rate pass, else push pass). Tiles of
-// XCD x are the x-th eighth; block `rank` of the XCD takes every per_xcd-th,
-// its two halves alternating, in rounds so that both halves meet at every
-// block barrier.
+constexpr uint32_t kNoTile = 0xffffffffu;
+
+// Tiles [begin, end) of XCD group v: the v-th eighth, numbered x-major, so
+// one group's tiles are one slab of the box.
+__device__ __forceinline__ uint32_t group_begin(const TileShape& sh, uint32_t v) {
+  return (uint32_t)((uint64_t)sh.count * v / kXcds);
+}
+
+// Wave 0 of the block takes the next two tiles (one per half) for this
+// pass: from its own group's queue while it lasts, then from the others in
+// turn. `done` holds the groups this block found empty; each failed take
+// adds one, so the loop ends. Result in grab_s[0..1] (kNoTile for none).
+__device__ __forceinline__ void take_tiles(Control* ctl, const TileShape& sh, uint32_t parity, uint32_t& done,
+                                           uint32_t* grab_s) {
+  const uint32_t lane = threadIdx.x & 63u, group = blockIdx.x % kXcds;
+  for (;;) {
+    uint32_t v = (group + lane) % kXcds;
+    bool left = false;
+    if (lane < kXcds && !((done >> v) & 1u)) {
+      uint32_t next = __hip_atomic_load(&ctl->tile_next[parity][v][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      left = next < group_begin(sh, v + 1) - group_begin(sh, v);
+    }
+    uint64_t m = __ballot(left);
+    if (m == 0) {
+      if (lane == 0) grab_s[0] = grab_s[1] = kNoTile;
+      return;
+    }
+    v = (group + (uint32_t)__builtin_ctzll(m)) % kXcds;
+    const uint32_t b = group_begin(sh, v), e = group_begin(sh, v + 1);
+    uint32_t k = 0;
+    if (lane == 0) k = __hip_atomic_fetch_add(&ctl->tile_next[parity][v][0], 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    k = (uint32_t)__shfl((int)k, 0, 64);
+    if (b + k < e) {
+      if (lane == 0) {
+        grab_s[0] = b + k;
+        grab_s[1] = b + k + 1 < e ? b + k + 1 : kNoTile;
+      }
+      return;
+    }
+    done |= 1u << v;
+  }
+}
+
+// One pass over the tiles (rate = true: rate pass, else push pass), queued
+// per XCD group with stealing (tile_next[parity]). Each round the block's
+// halves take a tile each: stage its neighbourhood in LDS, then every
+// particle of it sums from LDS (a tile past the LDS gathers from global
+// memory instead). The next round's tiles are taken while this round
+// stages, and their column ranges loaded while it sums (both double
+// buffered), so neither latency sits between rounds.
 template <int FORMAT>
-__device__ void tiled_pass(const SolveArgs& a, bool rate, float4* lds, TileState (&tstate)[2],
-                           uint32_t (&len_s)[2][16], uint32_t (&plen_s)[2][4], const Glob& g) {
+__device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float4* lds, TileState (&tstate)[2][2],
+                           uint32_t (&grab_s)[2][2], const Glob& g) {
   const StepParams& prm = a.prm;
-  const TileShape sh = tile_shape(prm.ctl);
-  const uint32_t xcd = blockIdx.x % kXcds, per_xcd = gridDim.x / kXcds, rank = blockIdx.x / kXcds;
-  const uint32_t t_begin = (uint32_t)((uint64_t)sh.count * xcd / kXcds);
-  const uint32_t t_end = (uint32_t)((uint64_t)sh.count * (xcd + 1) / kXcds);
-  const uint32_t mine = t_begin + rank < t_end ? (t_end - t_begin - rank + per_xcd - 1) / per_xcd : 0u;
-  const uint32_t rounds = (mine + 1) / 2;
+  Control* ctl = prm.ctl;
+  const TileShape sh = tile_shape(ctl);
   const uint32_t t = threadIdx.x, half = t / kChunk, tc = t % kChunk;
+  const uint32_t lane = t & 63u, wave = tc >> 6;
   const uint32_t cap = a.lds_cap;
   // each half's LDS: A (float4) then B (float2) for the rate pass, Q (float4)
   // for the push pass
@@ -756,33 +875,54 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, float4* lds, TileState
   float4* A = part;
   float2* B = reinterpret_cast<float2*>(part + cap);
   float4* Q = part;
-  TileState& ts = tstate[half];
-  for (uint32_t r = 0; r < rounds; ++r) {
-    uint32_t k = 2 * r + half;
-    bool have = k < mine;
-    uint32_t tile = t_begin + rank + k * per_xcd;
-    if (have) tile_ranges(sh, tile, prm.cell_start, ts, tc, len_s[half], plen_s[half]);
-    __syncthreads();
-    if (have) tile_prefix(ts, tc, len_s[half], plen_s[half]);
-    __syncthreads();
-    const uint32_t staged = have ? ts.pre[16] : 0u;
-    const uint32_t count = have ? ts.ppre[4] : 0u;
+  uint32_t done = 0;
+  if (t < 64) take_tiles(ctl, sh, parity, done, grab_s[0]);
+  __syncthreads();
+  if (wave == 0) {
+    const uint32_t tile = grab_s[0][half];
+    tile_scan(tile_load(sh, tile, tile != kNoTile, prm.cell_start, lane), tstate[half][0], lane);
+  }
+  __syncthreads();
+  for (uint32_t r = 0;; ++r) {
+    const uint32_t cur = r & 1u, nxt = cur ^ 1u;
+    if (grab_s[cur][0] == kNoTile) break;
+    if (t < 64) take_tiles(ctl, sh, parity, done, grab_s[nxt]);
+    const TileState& ts = tstate[half][cur];
+    const uint32_t staged = ts.pre[16];
+    const uint32_t count = ts.ppre[4];
     const bool fits = staged <= cap;
-    if (have && fits && count) {
-      tile_delta(ts, tc);
-      for (uint32_t s = tc; s < staged; s += kChunk) {
-        uint32_t gi = staged_index(ts, s);
-        if (rate) {
-          float4 p = a.P[gi];
-          float4 v = a.V[gi];
-          A[s] = make_float4(p.x, p.y, p.z, v.x);
-          B[s] = make_float2(v.y, v.z);
-        } else {
-          Q[s] = a.kappa[gi];
+    if (fits && count) {
+      // wave w stages columns w and w + 8, lanes on consecutive particles
+      for (uint32_t c = wave; c < 16; c += 8) {
+        const uint32_t from = ts.ss[c], to = ts.pre[c], len = ts.pre[c + 1] - to;
+        for (uint32_t s = lane; s < len; s += 128) {
+          const bool two = s + 64 < len;
+          if (rate) {
+            float4 p0 = a.P[from + s], v0 = a.V[from + s];
+            float4 p1, v1;
+            if (two) {
+              p1 = a.P[from + s + 64];
+              v1 = a.V[from + s + 64];
+            }
+            A[to + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
+            B[to + s] = make_float2(v0.y, v0.z);
+            if (two) {
+              A[to + s + 64] = make_float4(p1.x, p1.y, p1.z, v1.x);
+              B[to + s + 64] = make_float2(v1.y, v1.z);
+            }
+          } else {
+            float4 q0 = a.kappa[from + s], q1;
+            if (two) q1 = a.kappa[from + s + 64];
+            Q[to + s] = q0;
+            if (two) Q[to + s + 64] = q1;
+          }
         }
       }
     }
     __syncthreads();
+    const uint32_t tile_n = grab_s[nxt][half];
+    TileLoad next{0u, 0u};
+    if (wave == 0) next = tile_load(sh, tile_n, tile_n != kNoTile, prm.cell_start, lane);
     for (uint32_t q = tc; q < count; q += kChunk) {
       uint32_t p;
       uint32_t i = processed_index(ts, q, p);
@@ -793,11 +933,12 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, float4* lds, TileState
         else
           push_one<FORMAT>(a, i, g, &l, 0, 0);
       } else if (rate) {
-        rate_one<FORMAT>(a, i, g, nullptr, 0, 0);
+        rate_one<FORMAT, false>(a, i, g, nullptr, 0, 0);
       } else {
-        push_one<FORMAT>(a, i, g, nullptr, 0, 0);
+        push_one<FORMAT, false>(a, i, g, nullptr, 0, 0);
       }
     }
+    if (wave == 0) tile_scan(next, tstate[half][nxt], lane);
     __syncthreads();
   }
 }
@@ -835,7 +976,7 @@ __device__ __forceinline__ void chunk_sums(const SolveArgs& a, uint32_t first, u
 // every other one, thread t owning particle 512c + t, and sums its own
 // chunks before the barrier. With tiles (the list format, LDS permitting),
 // the passes walk tiles and the chunk sums follow a barrier of their own.
-template <int FORMAT>
+template <int FORMAT, bool TILED>
 __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   extern __shared__ float4 lds[];
   const StepParams& prm = a.prm;
@@ -849,22 +990,26 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   const uint32_t half = t / kChunk, tc = t % kChunk;
   __shared__ double slot[256];
   __shared__ uint32_t slot_count[256];
-  __shared__ TileState tstate[2];
-  __shared__ uint32_t len_s[2][16], plen_s[2][4];
+  __shared__ TileState tstate[2][2];
+  __shared__ uint32_t grab_s[2][2];
   bool tiled = false;
-  if constexpr (FORMAT == kLists) tiled = a.lds_cap > 0;
+  if constexpr (TILED) tiled = a.lds_cap > 0;
   const Glob g{a.P, a.V, a.kappa};
-  uint32_t it = 0;
+  uint32_t it = 0, pass = 0;
   for (;;) {
     // rate pass: stiffness and error of every particle
     if (tiled) {
-      if constexpr (FORMAT == kLists) tiled_pass<FORMAT>(a, true, lds, tstate, len_s, plen_s, g);
+      // the queues of the next pass start again from zero
+      if (blockIdx.x == 0 && t < kXcds)
+        __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if constexpr (TILED) tiled_pass<FORMAT>(a, true, pass & 1u, lds, tstate, grab_s, g);
+      ++pass;
       if (!grid_barrier(ctl)) return;
       chunk_sums(a, blockIdx.x, a.chunks, gridDim.x);
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
         uint32_t i = c * kChunk + tc;
-        if (i < prm.n) rate_one<FORMAT>(a, i, g, nullptr, sysz, szz);
+        if (i < prm.n) rate_one<FORMAT, !TILED>(a, i, g, nullptr, sysz, szz);
       }
       __syncthreads();
       chunk_sums(a, c_begin + rank, c_end, per_xcd);
@@ -905,16 +1050,23 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     if (settled || it >= a.max_iterations) break;
     // push pass
     if (tiled) {
-      if constexpr (FORMAT == kLists) tiled_pass<FORMAT>(a, false, lds, tstate, len_s, plen_s, g);
+      if (blockIdx.x == 0 && t < kXcds)
+        __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if constexpr (TILED) tiled_pass<FORMAT>(a, false, pass & 1u, lds, tstate, grab_s, g);
+      ++pass;
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
         uint32_t i = c * kChunk + tc;
-        if (i < prm.n) push_one<FORMAT>(a, i, g, nullptr, sysz, szz);
+        if (i < prm.n) push_one<FORMAT, !TILED>(a, i, g, nullptr, sysz, szz);
       }
     }
     if (!grid_barrier(ctl)) return;
     ++it;
   }
+  // every block is past the last pass: leave both queues at zero for the
+  // next launch
+  if (blockIdx.x == 0 && t < 2 * kXcds)
+    __hip_atomic_store(&ctl->tile_next[t / kXcds][t % kXcds][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (blockIdx.x == 0 && t == 0) {
     ctl->iterations[a.which] = it;
     ctl->iterations_step[a.which] += it;
@@ -1106,10 +1258,11 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
   if (hipDeviceGetAttribute(&threads, hipDeviceAttributeMaxThreadsPerMultiProcessor, dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
   int per = 0;
-  // the smallest occupancy of the three formats' solve kernels
-  const void* kernels[3] = {reinterpret_cast<const void*>(k_dfsph_solve<kMasks16>),
-                            reinterpret_cast<const void*>(k_dfsph_solve<kMasks32>),
-                            reinterpret_cast<const void*>(k_dfsph_solve<kLists>)};
+  // the smallest occupancy of the solve kernels
+  const void* kernels[4] = {reinterpret_cast<const void*>(k_dfsph_solve<kMasks16, false>),
+                            reinterpret_cast<const void*>(k_dfsph_solve<kMasks32, false>),
+                            reinterpret_cast<const void*>(k_dfsph_solve<kLists, false>),
+                            reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>)};
   per = 1 << 30;
   for (const void* k : kernels) {
     int p = 0;
@@ -1121,7 +1274,7 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
   per = per < cap ? per : cap;
   if (tiles && per == 1) {
     hipFuncAttributes attr;
-    const void* k = reinterpret_cast<const void*>(k_dfsph_solve<kLists>);
+    const void* k = reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>);
     if (hipFuncGetAttributes(&attr, k) != hipSuccess) return 0;
     int64_t room = (int64_t)lds - (int64_t)attr.sharedSizeBytes;
     // whole staged particles of both halves, rounded down to 64 each
@@ -1166,7 +1319,14 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   // A plain launch: the grid is sized to fit the device at once (two blocks
   // per CU at most), so every block is resident for the barrier; it also
   // keeps the launch capturable in a graph.
-  SFL_FORMAT_LAUNCH(k_dfsph_solve, prm.format, dim3(blocks), dim3(kSolveThreads), tiled ? tile_bytes : 0u, s, a);
+  if (tiled)
+    hipLaunchKernelGGL((k_dfsph_solve<kLists, true>), dim3(blocks), dim3(kSolveThreads), tile_bytes, s, a);
+  else if (prm.format == kMasks16)
+    hipLaunchKernelGGL((k_dfsph_solve<kMasks16, false>), dim3(blocks), dim3(kSolveThreads), 0, s, a);
+  else if (prm.format == kMasks32)
+    hipLaunchKernelGGL((k_dfsph_solve<kMasks32, false>), dim3(blocks), dim3(kSolveThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_dfsph_solve<kLists, false>), dim3(blocks), dim3(kSolveThreads), 0, s, a);
   return hipGetLastError();
 }
 

@@ -27,7 +27,13 @@ __global__ void k_step_begin(Control* ctl) {
   ctl->iterations_step[1] = 0;
   ctl->rates_step = 0;
   ctl->log_len = 0;
-  ctl->bar_count = 0;  // no persistent launch is in flight on this stream
+  // no persistent launch is in flight on this stream
+  ctl->bar_count = 0;
+  for (int k = 0; k < 8; ++k) {
+    ctl->bar_xcd[k][0] = 0;
+    ctl->tile_next[0][k][0] = 0;
+    ctl->tile_next[1][k][0] = 0;
+  }
 }
 
 __global__ void k_substep_begin(Control* ctl) {
