@@ -129,15 +129,23 @@ struct StepParams {
   // walls only; fluid neighbours are masks[k * pair_stride + i], k the
   // neighbour cell in the reference's order (dz, dy, dx), bit b set when
   // particle start[cell k] + b is a neighbour, and own_slot[i] the
-  // particle's own cell. In the list format fluid entries come first.
+  // particle's own cell. In the list format fluid entries come first. In
+  // the compact format the lists hold the walls only and fluid entry k is
+  // (near[k * pair_stride + i], factor[k * pair_stride + i]).
   uint2* pairs;
   uint32_t* pair_counts;
   uint32_t pair_stride;
   uint32_t pair_cap;
   void* masks;
   uint32_t* own_slot;
-  int32_t format;  // kMasks16, kMasks32 or kLists
+  uint16_t* near;
+  float* factor;
+  int32_t format;  // kMasks16, kMasks32, kLists or kCompact
 };
+
+// Rows the neighbour list arrays carry past their capacity: the solve's
+// loads run up to 15 slots ahead of a list's end.
+constexpr uint32_t kListSlack = 16;
 
 // LDS per particle a tile of the list solve stages: float4 + float2 (rate
 // pass), float4 (push pass)
@@ -151,9 +159,26 @@ constexpr uint32_t kListIndexMask = (1u << kListIndexBits) - 1u;
 __device__ __forceinline__ uint32_t list_index(uint32_t x) { return x & kListIndexMask; }
 __device__ __forceinline__ uint32_t list_column(uint32_t x) { return x >> kListIndexBits; }
 
+// A fluid entry of the compact format names its neighbour in 16 bits: the
+// neighbour column (dx, dy) in the top 4, as (dx + 1) * 4 + (dy + 1), and in
+// the low 12 its offset from the first particle of that column's cells from
+// z0 - 1 on, z0 the first cell in z of the particle's solve tile: the ranges
+// the tiled solve stages in LDS. A neighbour past offset 4095 widens to
+// lists.
+constexpr uint32_t kNearBits = 12;
+constexpr uint32_t kNearMask = (1u << kNearBits) - 1u;
+
 constexpr int kMasks16 = 0;
 constexpr int kMasks32 = 1;
 constexpr int kLists = 2;
+constexpr int kCompact = 3;
+
+// The solve's tiles are 2 x 2 cell columns by tile_depth(span z) cells: the
+// z span split into pieces of at most 16, as even as they go.
+__host__ __device__ __forceinline__ uint32_t tile_depth(uint32_t sz) {
+  uint32_t pieces = (sz + 15) / 16;
+  return pieces ? (sz + pieces - 1) / pieces : 1u;
+}
 
 // ---- Wendland C2 (src/fluid/mod.rs:126-165) ------------------------------
 

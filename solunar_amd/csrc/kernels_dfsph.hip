@@ -23,6 +23,8 @@
 // chunks t, t+256, ... in order; slots fold by halving (h = 128..1). Every
 // block computes that total itself after the barrier, so all blocks take the
 // same decision without a second barrier.
+#include <algorithm>
+
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -32,12 +34,35 @@ constexpr uint32_t kChunk = 512;          // particles of one chunk of the stop 
 constexpr uint32_t kSolveThreads = 1024;  // threads of a solve block: two chunks at a time
 constexpr uint32_t kSolveWaves = kSolveThreads / 64;
 
-// List entries stream through once per pass; loading them non-temporally
-// keeps them from evicting the neighbour positions and velocities the
-// gathers reuse from L2.
-__device__ __forceinline__ uint2 list_entry(const uint2* lp, size_t at) {
-  unsigned long long e = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(lp + at));
-  return make_uint2((uint32_t)e, (uint32_t)(e >> 32));
+// List slots are read with raw buffer loads: a buffer descriptor based at
+// row k of the column-major array sits in scalar registers, rows k + 1 ..
+// k + 3 are scalar offsets from it, and the lane's offset is 32-bit, so no
+// load needs 64-bit vector address arithmetic. The arrays carry
+// kListSlack rows past their capacity, so loads ahead of a list's end stay
+// unconditional (a load under a branch makes the compiler wait for every
+// load in flight at the join) and in bounds; their values are discarded.
+// The entries stream through once per pass and load non-temporally,
+// keeping them from evicting the data the gathers reuse. Word 3 of the
+// descriptor: a raw buffer of 32-bit format, as the gfx9 family reads it.
+constexpr int kRawBufferWord3 = 0x00020000;
+constexpr int kStreamLoad = 2;  // cache policy bits: non-temporal
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t list_rows(const void* base, uint32_t elem, const StepParams& prm,
+                                                            uint32_t k) {
+  const size_t row = (size_t)k * prm.pair_stride * elem;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(static_cast<const char*>(base)) + row, 0, -1,
+                                           kRawBufferWord3);
+}
+
+// list slots k .. k + 3 of particle i: (index, bits of the factor)
+__device__ __forceinline__ void list_entries(const StepParams& prm, uint32_t k, uint32_t i, uint2 (&e)[4]) {
+  const __amdgpu_buffer_rsrc_t rows = list_rows(prm.pairs, 8u, prm, k);
+  const uint32_t step = prm.pair_stride * 8u;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b64(rows, i * 8u, u * step, kStreamLoad);
+    e[u] = make_uint2(v[0], v[1]);
+  }
 }
 
 __device__ __forceinline__ bool list_aborted(const Control* ctl) {
@@ -158,31 +183,57 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
       sy = sy + qy;
       sz = sz + qz;
       squares += dot3(qx, qy, qz, qx, qy, qz);
-      if (FORMAT == kLists && nf < cap) out[nf * stride] = make_uint2(j, __float_as_uint(f));
+      if constexpr (FORMAT == kLists) {
+        if (nf < cap) out[nf * stride] = make_uint2(j, __float_as_uint(f));
+      } else if constexpr (FORMAT == kCompact) {
+        if (nf < cap) {
+          prm.near[nf * stride + i] = (uint16_t)j;
+          prm.factor[nf * stride + i] = f;
+        }
+      }
       ++nf;
     };
-    if constexpr (FORMAT == kLists) {
+    bool wide = false;
+    if constexpr (FORMAT == kLists || FORMAT == kCompact) {
       // visit_near's walk, tagging each entry with its neighbour column
       int32_t cx = cell_coord(at.x, prm.radius), cy = cell_coord(at.y, prm.radius), cz = cell_coord(at.z, prm.radius);
+      // compact: offsets count from z0 - 1 of the particle's solve tile
+      int64_t za = 0;
+      if constexpr (FORMAT == kCompact) {
+        const uint32_t depth = tile_depth(span[2]);
+        const uint32_t z0 = (uint32_t)((int64_t)cz - lo[2]) / depth * depth;
+        za = z0 > 0 ? z0 - 1 : 0;
+      }
       for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
           for (int dx = -1; dx <= 1; ++dx) {
             int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
             if (c < 0) continue;
-            const uint32_t column = (uint32_t)((dx + 1) * 3 + (dy + 1)) << kListIndexBits;
+            const uint32_t code = FORMAT == kCompact ? (uint32_t)((dx + 1) * 4 + (dy + 1))
+                                                     : (uint32_t)((dx + 1) * 3 + (dy + 1));
+            uint32_t first = 0;
+            if constexpr (FORMAT == kCompact) first = prm.cell_start[c - ((int64_t)cz + dz - lo[2]) + za];
             uint32_t s0 = prm.cell_start[c], e0 = prm.cell_start[c + 1];
             for (uint32_t j = s0; j < e0; ++j) {
               float4 pj = P[j];
               float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
               float sq = dot3(ox, oy, oz, ox, oy, oz);
-              if (sq < prm.reach) fluid(j | column, ox, oy, oz, sq);
+              if (sq < prm.reach) {
+                if constexpr (FORMAT == kLists) {
+                  fluid(j | code << kListIndexBits, ox, oy, oz, sq);
+                } else {
+                  const uint32_t off = j - first;
+                  wide = wide || off > kNearMask;
+                  fluid(code << kNearBits | (off & kNearMask), ox, oy, oz, sq);
+                }
+              }
             }
           }
+      if (wide) atomicOr(&ctl->flags, kFlagWide);
     } else {
       // visit_near's walk, recording each cell's neighbours as a mask
       using Mask = typename MaskOf<FORMAT>::T;
       Mask* masks = static_cast<Mask*>(prm.masks) + i;
-      bool wide = false;
       int32_t cx = cell_coord(at.x, prm.radius), cy = cell_coord(at.y, prm.radius), cz = cell_coord(at.z, prm.radius);
       prm.own_slot[i] = (uint32_t)cell_slot(cx, cy, cz, lo, span);
       int k = 0;
@@ -226,7 +277,7 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
                    if (k < cap) out[k * stride] = make_uint2(w, __float_as_uint(f));
                    ++nw;
                  });
-    uint32_t listed = wall0 + nw;
+    uint32_t listed = FORMAT == kCompact ? max(nf, nw) : wall0 + nw;
     if (listed > cap) atomicOr(&ctl->flags, kFlagPairs);
     need = listed;
     prm.pair_counts[i] = nf | (nw << 16);
@@ -330,23 +381,81 @@ struct Glob {
 // A tile's staged neighbourhood in LDS: x and vx in A, vy and vz in B for
 // the rate pass; (x, stiffness) in Q for the push pass. Staged slot of list
 // entry e of a particle in processed column p: index(e) + delta[p][column(e)].
+// Per half-block: the staged columns' first indexes and lengths and the
+// column stride of the tile's LDS (its longest staged column; column c
+// starts at slot c * stride), the processed ranges' starts and prefix, and
+// for the list format the slot offsets per (processed column, entry column).
+struct TileState {
+  uint32_t ss[16];
+  uint32_t len[16];
+  uint32_t stride;
+  uint32_t ps[4];
+  uint32_t ppre[5];
+  int32_t delta[4][9];  // list format: c * stride - ss[c], c the entry's staged column
+};
 struct Lds {
   const float4* A;
   const float2* B;
   const float4* Q;
-  const int32_t* delta;  // 9 entries of the particle's column
+  const TileState* ts;  // the tile's ranges and slot tables
+  uint32_t p;           // the particle's processed column
+  uint32_t col0;        // staged column of its (-1, -1) neighbour column
+  uint32_t stride;      // staged column stride
 };
+
+// A fluid entry of either list format: neighbour reference and factor.
+struct Ent {
+  uint32_t r;
+  float f;
+};
+
+// fluid entries k .. k + 3 of particle i, either list format
+template <int FORMAT>
+__device__ __forceinline__ void fluid_entries(const StepParams& prm, uint32_t i, uint32_t k, Ent (&e)[4]) {
+  if constexpr (FORMAT == kCompact) {
+    const __amdgpu_buffer_rsrc_t near = list_rows(prm.near, 2u, prm, k);
+    const __amdgpu_buffer_rsrc_t factor = list_rows(prm.factor, 4u, prm, k);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      e[u].r = __builtin_amdgcn_raw_buffer_load_b16(near, i * 2u, u * prm.pair_stride * 2u, kStreamLoad);
+      e[u].f = __uint_as_float(
+          __builtin_amdgcn_raw_buffer_load_b32(factor, i * 4u, u * prm.pair_stride * 4u, kStreamLoad));
+    }
+  } else {
+    uint2 v[4];
+    list_entries(prm, k, i, v);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) e[u] = Ent{v[u].x, __uint_as_float(v[u].y)};
+  }
+}
+
+// LDS slot of an entry of the tile's staged neighbourhood
+// The compact code (dx + 1) * 4 + (dy + 1) is the staged column less col0.
+template <int FORMAT>
+__device__ __forceinline__ int32_t entry_slot(const Lds& l, uint32_t r) {
+  if constexpr (FORMAT == kCompact)
+    return (int32_t)(__umul24(l.col0 + (r >> kNearBits), l.stride) + (r & kNearMask));
+  else
+    return (int32_t)list_index(r) + l.ts->delta[l.p][list_column(r)];
+}
+
+// particle index of an entry
+template <int FORMAT>
+__device__ __forceinline__ uint32_t entry_index(const Lds* l, uint32_t r) {
+  if constexpr (FORMAT == kCompact)
+    return l->ts->ss[l->col0 + (r >> kNearBits)] + (r & kNearMask);
+  else
+    return list_index(r);
+}
 
 // DEEP: the three-stage pipeline below; otherwise one batch of entries
 // ahead, for the tiled kernel's fallback, whose registers go to its tiles.
 template <int FORMAT, bool DEEP = true>
 __device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, float4 at, float4 vi, uint32_t nf,
-                                            const Glob& g, int64_t sysz, int64_t szz) {
+                                            const Glob& g, const Lds* l, int64_t sysz, int64_t szz) {
   const StepParams& prm = a.prm;
   float rate = 0.0f;
-  const uint2* lp = prm.pairs + i;
-  const size_t stride = prm.pair_stride;
-  if constexpr (FORMAT != kLists) {
+  if constexpr (FORMAT == kMasks16 || FORMAT == kMasks32) {
     mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
       float4 pj[4], vj[4];
 #pragma unroll
@@ -365,32 +474,33 @@ __device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, floa
       }
     });
   } else if constexpr (!DEEP) {
-    uint2 nxt[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
+    // one batch of entries ahead; slots past the list gather the particle
+    // itself and are skipped
+    Ent nxt[4];
+    fluid_entries<FORMAT>(prm, i, 0, nxt);
     for (uint32_t k = 0; k < nf; k += 4) {
-      uint2 cur[4];
+      Ent cur[4];
       float4 pj[4], vj[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        cur[u] = nxt[u];
-        nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(i, 0u);
-      }
+      for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
+      fluid_entries<FORMAT>(prm, i, k + 4, nxt);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        pj[u] = g.P[list_index(cur[u].x)];
-        vj[u] = g.V[list_index(cur[u].x)];
+        const uint32_t j = k + u < nf ? entry_index<FORMAT>(l, cur[u].r) : i;
+        pj[u] = g.P[j];
+        vj[u] = g.V[j];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (k + u < nf) {
-          float f = __uint_as_float(cur[u].y);
+          float f = cur[u].f;
           float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
           rate += dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, gx, gy, gz);
         }
       }
     }
   } else {
+    static_assert(FORMAT == kLists, "the deep pipeline reads the list format");
     // Entries go four at a time, in a three-stage pipeline: while batch k
     // is summed in list order, the neighbours of batch k + 1 are being
     // gathered and the list entries of batch k + 2 loaded. Slots past the
@@ -398,10 +508,12 @@ __device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, floa
     // their terms are skipped.
     uint2 cur[4], nxt[4];
     float4 pj[4], vj[4];
+    list_entries(prm, 0, i, cur);
+    list_entries(prm, 4, i, nxt);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      cur[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
-      nxt[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(i, 0u);
+      if ((uint32_t)u >= nf) cur[u].x = i;
+      if (4u + u >= nf) nxt[u].x = i;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -416,8 +528,10 @@ __device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, floa
         pn[u] = g.P[list_index(nxt[u].x)];
         vn[u] = g.V[list_index(nxt[u].x)];
       }
+      list_entries(prm, k + 8, i, after);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) after[u] = k + 8 + u < nf ? list_entry(lp, (k + 8 + u) * stride) : make_uint2(i, 0u);
+      for (int u = 0; u < 4; ++u)
+        if (k + 8 + u >= nf) after[u].x = i;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (k + u < nf) {
@@ -438,42 +552,91 @@ __device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, floa
   return rate;
 }
 
+// Four entries of particle i from list slot k on; the loops below skip
+// those past the list.
+template <int FORMAT>
+__device__ __forceinline__ void load_batch(const StepParams& prm, uint32_t i, uint32_t k, uint32_t nf, Ent (&e)[4]) {
+  fluid_entries<FORMAT>(prm, i, k, e);
+}
+
+// Staged slot of a batch entry; entries past the list read slot 0.
+template <int FORMAT>
+__device__ __forceinline__ int32_t batch_slot(const Lds& l, const Ent& e, bool live) {
+  return entry_slot<FORMAT>(l, e.r) & -(int32_t)live;
+}
+
+// The rate pass's fluid sum over the tile's LDS: batches of four entries,
+// three in flight from global memory, each batch's LDS reads issued
+// together before its terms are added in list order.
+template <int FORMAT>
 __device__ __forceinline__ float fluid_rate_lds(const SolveArgs& a, uint32_t i, float4 at, float4 vi, uint32_t nf,
                                                 const Lds& l) {
   const StepParams& prm = a.prm;
   float rate = 0.0f;
-  const uint2* lp = prm.pairs + i;
-  const size_t stride = prm.pair_stride;
-  // entries stream three batches ahead of the sum: nothing waits on LDS
-  uint2 n1[4], n2[4], n3[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    n1[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(0u, 0u);
-    n2[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(0u, 0u);
-    n3[u] = 8u + u < nf ? list_entry(lp, (8u + u) * stride) : make_uint2(0u, 0u);
-  }
-  for (uint32_t k = 0; k < nf; k += 4) {
-    uint2 cur[4];
+  Ent b0[4], b1[4], b2[4];
+  load_batch<FORMAT>(prm, i, 0, nf, b0);
+  load_batch<FORMAT>(prm, i, 4, nf, b1);
+  load_batch<FORMAT>(prm, i, 8, nf, b2);
+  auto sum = [&](const Ent(&e)[4], uint32_t k) {
+    float4 pv[4];
+    float2 w[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      cur[u] = n1[u];
-      n1[u] = n2[u];
-      n2[u] = n3[u];
-      n3[u] = k + 12 + u < nf ? list_entry(lp, (k + 12 + u) * stride) : make_uint2(0u, 0u);
+      const int32_t s = batch_slot<FORMAT>(l, e[u], k + u < nf);
+      pv[u] = l.A[s];
+      w[u] = l.B[s];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      if (k + u < nf) {
-        int32_t s = (int32_t)list_index(cur[u].x) + l.delta[list_column(cur[u].x)];
-        float4 pv = l.A[s];
-        float2 w = l.B[s];
-        float f = __uint_as_float(cur[u].y);
-        float gx = (at.x - pv.x) * f, gy = (at.y - pv.y) * f, gz = (at.z - pv.z) * f;
-        rate += dot3(vi.x - pv.w, vi.y - w.x, vi.z - w.y, gx, gy, gz);
-      }
+      const float f = e[u].f;
+      const float gx = (at.x - pv[u].x) * f, gy = (at.y - pv[u].y) * f, gz = (at.z - pv[u].z) * f;
+      const float term = dot3(vi.x - pv[u].w, vi.y - w[u].x, vi.z - w[u].y, gx, gy, gz);
+      // past the list: add -0, the identity of IEEE addition, branch-free
+      rate = rate + (k + u < nf ? term : -0.0f);
     }
+  };
+  for (uint32_t k = 0; k < nf; k += 12) {
+    sum(b0, k);
+    load_batch<FORMAT>(prm, i, k + 12, nf, b0);
+    if (k + 4 >= nf) break;
+    sum(b1, k + 4);
+    load_batch<FORMAT>(prm, i, k + 16, nf, b1);
+    if (k + 8 >= nf) break;
+    sum(b2, k + 8);
+    load_batch<FORMAT>(prm, i, k + 20, nf, b2);
   }
   return rate;
+}
+
+// Wall entries w0 .. nt - 1 of particle i, four at a time in list order:
+// entries load two batches ahead and wall positions are gathered one batch
+// ahead, so no batch waits on a load issued just before it. body(f, pw,
+// live) sees each entry; `live` is false past the list.
+template <class Body>
+__device__ __forceinline__ void walk_walls(const StepParams& prm, uint32_t i, uint32_t w0, uint32_t nt, Body&& body) {
+  if (w0 >= nt) return;
+  const float4* __restrict__ W = prm.walls;
+  uint2 e0[4], e1[4];
+  float4 p0[4];
+  list_entries(prm, w0, i, e0);
+  list_entries(prm, w0 + 4, i, e1);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) p0[u] = W[w0 + u < nt ? e0[u].x : 0u];
+  for (uint32_t k = w0; k < nt; k += 4) {
+    uint2 e2[4];
+    float4 p1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p1[u] = W[k + 4 + u < nt ? e1[u].x : 0u];
+    list_entries(prm, k + 8, i, e2);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) body(__uint_as_float(e0[u].y), p0[u], k + u < nt);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      e0[u] = e1[u];
+      e1[u] = e2[u];
+      p0[u] = p1[u];
+    }
+  }
 }
 
 // Walls of the rate pass, then stiffness and error; writes kappa and errors.
@@ -481,34 +644,13 @@ template <int FORMAT>
 __device__ __forceinline__ void finish_rate(const SolveArgs& a, uint32_t i, float4 at, float4 vi, float rate,
                                             uint32_t nf, uint32_t nw) {
   const StepParams& prm = a.prm;
-  const float4* __restrict__ W = prm.walls;
-  const uint2* lp = prm.pairs + i;
-  const size_t stride = prm.pair_stride;
-  // walls four at a time (a particle at a corner lists ~40 of them)
+  // walls in list order (a particle at a corner lists ~40 of them)
   const uint32_t w0 = FORMAT == kLists ? nf : 0u;
-  const uint32_t nt = w0 + nw;
-  uint2 nxt[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) nxt[u] = w0 + u < nt ? list_entry(lp, (w0 + u) * stride) : make_uint2(0u, 0u);
-  for (uint32_t k = w0; k < nt; k += 4) {
-    uint2 cur[4];
-    float4 pw[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      cur[u] = nxt[u];
-      nxt[u] = k + 4 + u < nt ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) pw[u] = W[cur[u].x];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (k + u < nt) {
-        float f = __uint_as_float(cur[u].y);
-        float gx = (at.x - pw[u].x) * f, gy = (at.y - pw[u].y) * f, gz = (at.z - pw[u].z) * f;
-        rate += dot3(vi.x, vi.y, vi.z, gx, gy, gz);
-      }
-    }
-  }
+  walk_walls(prm, i, w0, w0 + nw, [&](float f, float4 pw, bool live) {
+    float gx = (at.x - pw.x) * f, gy = (at.y - pw.y) * f, gz = (at.z - pw.z) * f;
+    float term = dot3(vi.x, vi.y, vi.z, gx, gy, gz);
+    rate = rate + (live ? term : -0.0f);
+  });
   const float dt = prm.dt;
   rate = rate * prm.mass;
   float fa = a.alpha[i];
@@ -527,24 +669,25 @@ __device__ __forceinline__ void finish_rate(const SolveArgs& a, uint32_t i, floa
 }
 
 template <int FORMAT, bool DEEP = true>
-__device__ __forceinline__ void rate_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, int64_t sysz,
-                                         int64_t szz) {
+__device__ __forceinline__ void rate_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, bool staged,
+                                         int64_t sysz, int64_t szz) {
   float4 at = a.P[i];
   float4 vi = a.V[i];
   uint32_t counts = a.prm.pair_counts[i];
   uint32_t nf = counts & 0xffffu, nw = counts >> 16;
   float rate;
-  if constexpr (FORMAT == kLists) {
-    rate = l ? fluid_rate_lds(a, i, at, vi, nf, *l) : fluid_rate<FORMAT, DEEP>(a, i, at, vi, nf, g, sysz, szz);
+  if constexpr (FORMAT == kLists || FORMAT == kCompact) {
+    rate = staged ? fluid_rate_lds<FORMAT>(a, i, at, vi, nf, *l)
+                  : fluid_rate<FORMAT, DEEP>(a, i, at, vi, nf, g, l, sysz, szz);
   } else {
-    rate = fluid_rate<FORMAT>(a, i, at, vi, nf, g, sysz, szz);
+    rate = fluid_rate<FORMAT>(a, i, at, vi, nf, g, l, sysz, szz);
   }
   finish_rate<FORMAT>(a, i, at, vi, rate, nf, nw);
 }
 
 template <int FORMAT, bool DEEP = true>
-__device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, int64_t sysz,
-                                         int64_t szz) {
+__device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, bool staged,
+                                         int64_t sysz, int64_t szz) {
   const StepParams& prm = a.prm;
   const float mass = prm.mass;
   float4 at = a.kappa[i];  // x as P holds it, own stiffness in w
@@ -552,9 +695,7 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
   uint32_t counts = prm.pair_counts[i];
   uint32_t nf = counts & 0xffffu, nw = counts >> 16;
-  const uint2* lp = prm.pairs + i;
-  const size_t stride = prm.pair_stride;
-  if constexpr (FORMAT != kLists) {
+  if constexpr (FORMAT == kMasks16 || FORMAT == kMasks32) {
     mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
       float4 pj[4];
 #pragma unroll
@@ -572,55 +713,52 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
         }
       }
     });
-  } else if (l) {
-    uint2 n1[4], n2[4], n3[4];
+  } else if (staged) {
+    Ent b0[4], b1[4], b2[4];
+    load_batch<FORMAT>(prm, i, 0, nf, b0);
+    load_batch<FORMAT>(prm, i, 4, nf, b1);
+    load_batch<FORMAT>(prm, i, 8, nf, b2);
+    auto sum = [&](const Ent(&e)[4], uint32_t k) {
+      float4 q[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      n1[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(0u, 0u);
-      n2[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(0u, 0u);
-      n3[u] = 8u + u < nf ? list_entry(lp, (8u + u) * stride) : make_uint2(0u, 0u);
-    }
-    for (uint32_t k = 0; k < nf; k += 4) {
-      uint2 cur[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        cur[u] = n1[u];
-        n1[u] = n2[u];
-        n2[u] = n3[u];
-        n3[u] = k + 12 + u < nf ? list_entry(lp, (k + 12 + u) * stride) : make_uint2(0u, 0u);
-      }
+      for (int u = 0; u < 4; ++u) q[u] = l->Q[batch_slot<FORMAT>(*l, e[u], k + u < nf)];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        if (k + u < nf) {
-          int32_t s = (int32_t)list_index(cur[u].x) + l->delta[list_column(cur[u].x)];
-          float4 q = l->Q[s];
-          float f = __uint_as_float(cur[u].y);
-          float gx = (at.x - q.x) * f, gy = (at.y - q.y) * f, gz = (at.z - q.z) * f;
-          float k2 = mass * (own + q.w);
-          cx = cx - gx * k2;
-          cy = cy - gy * k2;
-          cz = cz - gz * k2;
-        }
+        const float f = e[u].f;
+        const float gx = (at.x - q[u].x) * f, gy = (at.y - q[u].y) * f, gz = (at.z - q[u].z) * f;
+        const float k2 = mass * (own + q[u].w);
+        // past the list: subtract +0, the identity of IEEE subtraction
+        const bool live = k + u < nf;
+        cx = cx - (live ? gx * k2 : 0.0f);
+        cy = cy - (live ? gy * k2 : 0.0f);
+        cz = cz - (live ? gz * k2 : 0.0f);
       }
+    };
+    for (uint32_t k = 0; k < nf; k += 12) {
+      sum(b0, k);
+      load_batch<FORMAT>(prm, i, k + 12, nf, b0);
+      if (k + 4 >= nf) break;
+      sum(b1, k + 4);
+      load_batch<FORMAT>(prm, i, k + 16, nf, b1);
+      if (k + 8 >= nf) break;
+      sum(b2, k + 8);
+      load_batch<FORMAT>(prm, i, k + 20, nf, b2);
     }
   } else if constexpr (!DEEP) {
-    uint2 nxt[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) nxt[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
+    Ent nxt[4];
+    fluid_entries<FORMAT>(prm, i, 0, nxt);
     for (uint32_t k = 0; k < nf; k += 4) {
-      uint2 cur[4];
+      Ent cur[4];
       float4 pj[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        cur[u] = nxt[u];
-        nxt[u] = k + 4 + u < nf ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(i, 0u);
-      }
+      for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
+      fluid_entries<FORMAT>(prm, i, k + 4, nxt);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) pj[u] = g.K[list_index(cur[u].x)];
+      for (int u = 0; u < 4; ++u) pj[u] = g.K[k + u < nf ? entry_index<FORMAT>(l, cur[u].r) : i];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (k + u < nf) {
-          float f = __uint_as_float(cur[u].y);
+          float f = cur[u].f;
           float gx = (at.x - pj[u].x) * f, gy = (at.y - pj[u].y) * f, gz = (at.z - pj[u].z) * f;
           float k2 = mass * (own + pj[u].w);
           cx = cx - gx * k2;
@@ -630,12 +768,15 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
       }
     }
   } else {
+    static_assert(FORMAT == kLists, "the deep pipeline reads the list format");
     uint2 cur[4], nxt[4];
     float4 pj[4];
+    list_entries(prm, 0, i, cur);
+    list_entries(prm, 4, i, nxt);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      cur[u] = (uint32_t)u < nf ? list_entry(lp, u * stride) : make_uint2(i, 0u);
-      nxt[u] = 4u + u < nf ? list_entry(lp, (4u + u) * stride) : make_uint2(i, 0u);
+      if ((uint32_t)u >= nf) cur[u].x = i;
+      if (4u + u >= nf) nxt[u].x = i;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) pj[u] = g.K[list_index(cur[u].x)];
@@ -644,8 +785,10 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
       uint2 after[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) pn[u] = g.K[list_index(nxt[u].x)];
+      list_entries(prm, k + 8, i, after);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) after[u] = k + 8 + u < nf ? list_entry(lp, (k + 8 + u) * stride) : make_uint2(i, 0u);
+      for (int u = 0; u < 4; ++u)
+        if (k + 8 + u >= nf) after[u].x = i;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (k + u < nf) {
@@ -665,34 +808,14 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
       }
     }
   }
-  const float4* __restrict__ W = prm.walls;
   const uint32_t w0 = FORMAT == kLists ? nf : 0u;
-  const uint32_t nt = w0 + nw;
   const float kw = mass * own;
-  uint2 nxt[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) nxt[u] = w0 + u < nt ? list_entry(lp, (w0 + u) * stride) : make_uint2(0u, 0u);
-  for (uint32_t k = w0; k < nt; k += 4) {
-    uint2 cur[4];
-    float4 pw[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      cur[u] = nxt[u];
-      nxt[u] = k + 4 + u < nt ? list_entry(lp, (k + 4 + u) * stride) : make_uint2(0u, 0u);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) pw[u] = W[cur[u].x];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (k + u < nt) {
-        float f = __uint_as_float(cur[u].y);
-        float gx = (at.x - pw[u].x) * f, gy = (at.y - pw[u].y) * f, gz = (at.z - pw[u].z) * f;
-        cx = cx - gx * kw;
-        cy = cy - gy * kw;
-        cz = cz - gz * kw;
-      }
-    }
-  }
+  walk_walls(prm, i, w0, w0 + nw, [&](float f, float4 pw, bool live) {
+    float gx = (at.x - pw.x) * f, gy = (at.y - pw.y) * f, gz = (at.z - pw.z) * f;
+    cx = cx - (live ? gx * kw : 0.0f);
+    cy = cy - (live ? gy * kw : 0.0f);
+    cz = cz - (live ? gz * kw : 0.0f);
+  });
   const float dt = prm.dt;
   float4 vi = a.V[i];
   vi.x = vi.x + cx * dt;
@@ -728,15 +851,6 @@ __device__ __forceinline__ TileShape tile_shape(const Control* ctl) {
   return t;
 }
 
-// Per half-block: the staged ranges' starts and LDS prefix, the processed
-// ranges' starts and prefix, and the slot offsets per (column, entry column).
-struct TileState {
-  uint32_t ss[16];
-  uint32_t pre[17];
-  uint32_t ps[4];
-  uint32_t ppre[5];
-  int32_t delta[4][9];
-};
 
 // A tile's column ranges, in two steps so the loads of the next tile's can
 // be in flight while this tile's particles are summed. Wave 0 of a half
@@ -773,35 +887,36 @@ __device__ __forceinline__ TileLoad tile_load(const TileShape& sh, uint32_t tile
   return r;
 }
 
-// Then the same wave scans both prefixes (segments lanes 0..15, 16..19) and
-// lanes 0..35 write the slot offsets into `ts`.
+// Then the same wave takes the longest staged column (lanes 0..15), scans
+// the processed prefix (lanes 16..19), and lanes 0..35 write the list
+// format's slot offsets into `ts`.
 __device__ __forceinline__ void tile_scan(TileLoad r, TileState& ts, uint32_t lane) {
+  uint32_t longest = lane < 16 ? r.n : 0u;
+  for (int d = 8; d >= 1; d >>= 1) longest = max(longest, (uint32_t)__shfl_xor((int)longest, d, 64));
+  const uint32_t stride = (uint32_t)__shfl((int)longest, 0, 64);
   uint32_t incl = r.n;
-  for (uint32_t d = 1; d < 16; d <<= 1) {
+  for (uint32_t d = 1; d < 4; d <<= 1) {
     uint32_t up = (uint32_t)__shfl_up((int)incl, d, 64);
-    bool inside = lane < 16 ? lane >= d : (lane < 20 && lane >= 16 + d);
-    if (inside) incl += up;
+    if (lane < 20 && lane >= 16 + d) incl += up;
   }
-  const uint32_t excl = incl - r.n;
   if (lane < 16) {
     ts.ss[lane] = r.s0;
-    ts.pre[lane] = excl;
-    if (lane == 15) ts.pre[16] = incl;
+    ts.len[lane] = r.n;
+    if (lane == 0) ts.stride = stride;
   } else if (lane < 20) {
     ts.ps[lane - 16] = r.s0;
-    ts.ppre[lane - 16] = excl;
+    ts.ppre[lane - 16] = incl - r.n;
     if (lane == 19) ts.ppre[4] = incl;
   }
-  // staged slot of list entry e in processed column p: index(e) +
-  // delta[p][column(e)], the column's LDS start less its first index
-  const int32_t here = (int32_t)excl - (int32_t)r.s0;
+  // list format: staged slot of entry e of processed column p is
+  // index(e) + delta[p][column(e)]
   uint32_t col = 0, p = lane / 9, c = lane % 9;
   if (lane < 36) {
     int dx = (int)(c / 3) - 1, dy = (int)(c % 3) - 1;
     col = (uint32_t)((1 + (int)(p >> 1) + dx) * 4 + (1 + (int)(p & 1) + dy));
   }
-  int32_t dv = __shfl(here, (int)col, 64);
-  if (lane < 36) ts.delta[p][c] = dv;
+  const uint32_t first = (uint32_t)__shfl((int)r.s0, (int)col, 64);
+  if (lane < 36) ts.delta[p][c] = (int32_t)(col * stride) - (int32_t)first;
 }
 
 __device__ __forceinline__ uint32_t processed_index(const TileState& ts, uint32_t q, uint32_t& p) {
@@ -888,13 +1003,13 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
     if (grab_s[cur][0] == kNoTile) break;
     if (t < 64) take_tiles(ctl, sh, parity, done, grab_s[nxt]);
     const TileState& ts = tstate[half][cur];
-    const uint32_t staged = ts.pre[16];
+    const uint32_t stride = __builtin_amdgcn_readfirstlane(ts.stride);
     const uint32_t count = ts.ppre[4];
-    const bool fits = staged <= cap;
+    const bool fits = 16u * stride <= cap;
     if (fits && count) {
       // wave w stages columns w and w + 8, lanes on consecutive particles
       for (uint32_t c = wave; c < 16; c += 8) {
-        const uint32_t from = ts.ss[c], to = ts.pre[c], len = ts.pre[c + 1] - to;
+        const uint32_t from = ts.ss[c], to = c * stride, len = ts.len[c];
         for (uint32_t s = lane; s < len; s += 128) {
           const bool two = s + 64 < len;
           if (rate) {
@@ -926,17 +1041,11 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
     for (uint32_t q = tc; q < count; q += kChunk) {
       uint32_t p;
       uint32_t i = processed_index(ts, q, p);
-      if (fits) {
-        Lds l{A, B, Q, ts.delta[p]};
-        if (rate)
-          rate_one<FORMAT>(a, i, g, &l, 0, 0);
-        else
-          push_one<FORMAT>(a, i, g, &l, 0, 0);
-      } else if (rate) {
-        rate_one<FORMAT, false>(a, i, g, nullptr, 0, 0);
-      } else {
-        push_one<FORMAT, false>(a, i, g, nullptr, 0, 0);
-      }
+      const Lds l{A, B, Q, &ts, p, 4u * (p >> 1) + (p & 1u), stride};
+      if (rate)
+        rate_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
+      else
+        push_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
     }
     if (wave == 0) tile_scan(next, tstate[half][nxt], lane);
     __syncthreads();
@@ -1009,7 +1118,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
         uint32_t i = c * kChunk + tc;
-        if (i < prm.n) rate_one<FORMAT, !TILED>(a, i, g, nullptr, sysz, szz);
+        if (i < prm.n) rate_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz);
       }
       __syncthreads();
       chunk_sums(a, c_begin + rank, c_end, per_xcd);
@@ -1019,7 +1128,23 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     if (t < 256) {
       double s = 0.0;
       uint32_t cn = 0;
-      for (uint32_t c = t; c < a.chunks; c += 256u) {
+      uint32_t c = t;
+      // eight loads in flight, added in chunk order
+      for (; c + 7u * 256u < a.chunks; c += 8u * 256u) {
+        double v[8];
+        uint32_t w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          v[u] = a.partial[c + u * 256u];
+          w[u] = a.pcount[c + u * 256u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          s += v[u];
+          cn += w[u];
+        }
+      }
+      for (; c < a.chunks; c += 256u) {
         s += a.partial[c];
         cn += a.pcount[c];
       }
@@ -1027,13 +1152,28 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
       slot_count[t] = cn;
     }
     __syncthreads();
-    for (uint32_t h = 128; h >= 1; h >>= 1) {
+    // halving folds: slot t += slot t + h for h = 128, 64 in LDS, then
+    // h = 32 .. 1 across wave 0's lanes
+    for (uint32_t h = 128; h >= 64; h >>= 1) {
       if (t < h) {
         slot[t] = slot[t] + slot[t + h];
         slot_count[t] += slot_count[t + h];
       }
       __syncthreads();
     }
+    if (t < 64) {
+      double v = slot[t];
+      uint32_t cn = slot_count[t];
+      for (int h = 32; h >= 1; h >>= 1) {
+        v = v + __shfl_down(v, h, 64);
+        cn += (uint32_t)__shfl_down((int)cn, h, 64);
+      }
+      if (t == 0) {
+        slot[0] = v;
+        slot_count[0] = cn;
+      }
+    }
+    __syncthreads();
     double total = slot[0];
     uint32_t loaded = slot_count[0];
     __syncthreads();
@@ -1057,7 +1197,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
         uint32_t i = c * kChunk + tc;
-        if (i < prm.n) push_one<FORMAT, !TILED>(a, i, g, nullptr, sysz, szz);
+        if (i < prm.n) push_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz);
       }
     }
     if (!grid_barrier(ctl)) return;
@@ -1119,6 +1259,31 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
           if (j != i) fluid(j, f);
         },
         wall);
+  } else if constexpr (FORMAT == kCompact) {
+    // offsets count from the first particle of each neighbour column's
+    // cells from z0 - 1 on, z0 the first cell of the particle's tile
+    const int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+    const uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+    const int32_t cx = cell_coord(at.x, prm.radius), cy = cell_coord(at.y, prm.radius);
+    const int32_t cz = cell_coord(at.z, prm.radius);
+    const uint32_t depth = tile_depth(span[2]);
+    const uint32_t z0 = (uint32_t)((int64_t)cz - lo[2]) / depth * depth;
+    const int64_t za = (int64_t)lo[2] + (z0 > 0 ? z0 - 1 : 0);
+    const uint32_t c = prm.pair_counts[i];
+    const uint32_t nf = c & 0xffffu, nw = c >> 16;
+    for (uint32_t k = 0; k < nf; ++k) {
+      const size_t at_k = (size_t)k * prm.pair_stride + i;
+      const uint32_t r = prm.near[at_k];
+      const uint32_t code = r >> kNearBits;
+      const int64_t first = cell_slot((int64_t)cx + (int64_t)(code >> 2) - 1, (int64_t)cy + (int64_t)(code & 3u) - 1,
+                                      za, lo, span);
+      const uint32_t j = prm.cell_start[first] + (r & kNearMask);
+      if (j != i) fluid(j, prm.factor[at_k]);
+    }
+    for (uint32_t k = 0; k < nw; ++k) {
+      uint2 e = prm.pairs[(size_t)k * prm.pair_stride + i];
+      wall(e.x, __uint_as_float(e.y));
+    }
   } else {
     const int64_t szz = ctl->span[2], sysz = (int64_t)ctl->span[1] * szz;
     mask_walk<FORMAT>(prm, i, prm.own_slot[i], sysz, szz, [&](const uint32_t(&j)[4], const bool(&ok)[4]) {
@@ -1233,6 +1398,8 @@ static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)(
       hipLaunchKernelGGL(KERNEL<kMasks16>, __VA_ARGS__);                          \
     else if ((format) == kMasks32)                                                \
       hipLaunchKernelGGL(KERNEL<kMasks32>, __VA_ARGS__);                          \
+    else if ((format) == kCompact)                                                \
+      hipLaunchKernelGGL(KERNEL<kCompact>, __VA_ARGS__);                          \
     else                                                                          \
       hipLaunchKernelGGL(KERNEL<kLists>, __VA_ARGS__);                            \
   } while (0)
@@ -1259,10 +1426,11 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
   int per = 0;
   // the smallest occupancy of the solve kernels
-  const void* kernels[4] = {reinterpret_cast<const void*>(k_dfsph_solve<kMasks16, false>),
+  const void* kernels[5] = {reinterpret_cast<const void*>(k_dfsph_solve<kMasks16, false>),
                             reinterpret_cast<const void*>(k_dfsph_solve<kMasks32, false>),
                             reinterpret_cast<const void*>(k_dfsph_solve<kLists, false>),
-                            reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>)};
+                            reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>),
+                            reinterpret_cast<const void*>(k_dfsph_solve<kCompact, true>)};
   per = 1 << 30;
   for (const void* k : kernels) {
     int p = 0;
@@ -1273,17 +1441,26 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
   int cap = blocks_per_cu > 0 && blocks_per_cu <= 2 ? blocks_per_cu : 1;
   per = per < cap ? per : cap;
   if (tiles && per == 1) {
-    hipFuncAttributes attr;
-    const void* k = reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>);
-    if (hipFuncGetAttributes(&attr, k) != hipSuccess) return 0;
-    int64_t room = (int64_t)lds - (int64_t)attr.sharedSizeBytes;
+    // both tiled kernels (list and compact formats) get the same room
+    const void* tiled[2] = {reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>),
+                            reinterpret_cast<const void*>(k_dfsph_solve<kCompact, true>)};
+    int64_t room = (int64_t)lds;
+    for (const void* k : tiled) {
+      hipFuncAttributes attr;
+      if (hipFuncGetAttributes(&attr, k) != hipSuccess) return 0;
+      room = std::min<int64_t>(room, (int64_t)lds - (int64_t)attr.sharedSizeBytes);
+    }
     // whole staged particles of both halves, rounded down to 64 each
     room = room / (2 * kTileBytes) / 64 * 64 * (2 * kTileBytes);
     if (room >= 2 * kTileBytes * 512) {
-      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)room) != hipSuccess) return 0;
-      int p = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k, kSolveThreads, (size_t)room) != hipSuccess) return 0;
-      if (p >= 1) *tile_bytes = (uint32_t)room;
+      bool fit = true;
+      for (const void* k : tiled) {
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)room) != hipSuccess) return 0;
+        int p = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k, kSolveThreads, (size_t)room) != hipSuccess) return 0;
+        fit = fit && p >= 1;
+      }
+      if (fit) *tile_bytes = (uint32_t)room;
     }
   }
   return per * cus;
@@ -1307,7 +1484,9 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   a.tolerance = tolerance;
   a.min_iterations = min_it;
   a.max_iterations = max_it;
-  const bool tiled = prm.format == kLists && tile_bytes > 0;
+  // the compact format is read by the tiled kernel alone
+  if (prm.format == kCompact && tile_bytes == 0) return hipErrorInvalidValue;
+  const bool tiled = (prm.format == kLists || prm.format == kCompact) && tile_bytes > 0;
   a.lds_cap = tiled ? tile_bytes / (2 * kTileBytes) : 0u;
   // a multiple of 8 blocks so every XCD gets the same share; at least one
   // chunk per XCD group or the small grid of a small world
@@ -1319,7 +1498,9 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   // A plain launch: the grid is sized to fit the device at once (two blocks
   // per CU at most), so every block is resident for the barrier; it also
   // keeps the launch capturable in a graph.
-  if (tiled)
+  if (tiled && prm.format == kCompact)
+    hipLaunchKernelGGL((k_dfsph_solve<kCompact, true>), dim3(blocks), dim3(kSolveThreads), tile_bytes, s, a);
+  else if (tiled)
     hipLaunchKernelGGL((k_dfsph_solve<kLists, true>), dim3(blocks), dim3(kSolveThreads), tile_bytes, s, a);
   else if (prm.format == kMasks16)
     hipLaunchKernelGGL((k_dfsph_solve<kMasks16, false>), dim3(blocks), dim3(kSolveThreads), 0, s, a);

@@ -148,10 +148,12 @@ int32_t World::init_device() {
     solve_lds_ = (uint32_t)std::min<uint64_t>(solve_lds_, (uint64_t)c * 2 * kTileBytes);
   }
   // SOLUNAR_NEIGHBOUR_FORMAT (0: 16-bit masks, 1: 32-bit masks, 2: lists,
-  // the default) picks the DFSPH neighbour format a world starts with; masks
-  // widen on demand from there
+  // 3: compact lists, the default where the solve has its LDS tiles) picks
+  // the DFSPH neighbour format a world starts with; each widens on demand
   const char* fmt = std::getenv("SOLUNAR_NEIGHBOUR_FORMAT");
-  if (fmt) format_ = std::min(std::max(std::atoi(fmt), (int)kMasks16), (int)kLists);
+  format_ = solve_lds_ ? kCompact : kLists;
+  if (fmt) format_ = std::min(std::max(std::atoi(fmt), (int)kMasks16), (int)kCompact);
+  if (format_ == kCompact && !solve_lds_) format_ = kLists;  // compact lists need the tiled solve
   if (solve_grid_ <= 0) {
     detail_ = "occupancy query for the persistent solve failed";
     return SFL_DEVICE_ERROR;
@@ -589,6 +591,8 @@ StepParams World::params(float dt) const {
   p.pair_cap = pair_cap_;
   p.masks = masks_.ptr;
   p.own_slot = own_slot_.ptr;
+  p.near = near_.ptr;
+  p.factor = factor_.ptr;
   p.format = format_;
   return p;
 }
@@ -615,12 +619,19 @@ int32_t World::ensure_pairs(uint32_t cap) {
   cap = std::max(cap, pair_cap_);
   stride = std::max(stride, pair_stride_);
   pairs_.release();
-  SFL_HIP(pairs_.reserve((size_t)cap * stride, 0, stream_), "hipMalloc(neighbour lists)");
+  // kListSlack rows past the capacity: the solve's loads run ahead of a
+  // list's end
+  const size_t rows = (size_t)cap + kListSlack;
+  SFL_HIP(pairs_.reserve(rows * stride, 0, stream_), "hipMalloc(neighbour lists)");
   SFL_HIP(pair_counts_.reserve(stride, 0, stream_), "hipMalloc(neighbour counts)");
-  if (format_ != kLists) {
+  if (format_ == kMasks16 || format_ == kMasks32) {
     // 27 masks per particle at the widest mask (32 bits)
     SFL_HIP(masks_.reserve((size_t)27 * stride, 0, stream_), "hipMalloc(neighbour masks)");
     SFL_HIP(own_slot_.reserve(stride, 0, stream_), "hipMalloc(neighbour masks)");
+  }
+  if (format_ == kCompact) {
+    SFL_HIP(near_.reserve(rows * stride, 0, stream_), "hipMalloc(compact neighbours)");
+    SFL_HIP(factor_.reserve(rows * stride, 0, stream_), "hipMalloc(compact neighbours)");
   }
   pair_cap_ = cap;
   pair_stride_ = stride;
@@ -886,7 +897,7 @@ int32_t World::dfsph(float dt, sfl_report& r) {
     pair_cap_ = 0;
   }
   // lists of walls only in the mask formats, of every neighbour otherwise
-  st = ensure_pairs(std::max<uint32_t>(pair_cap_, format_ == kLists ? 64 : 16));
+  st = ensure_pairs(std::max<uint32_t>(pair_cap_, format_ == kLists || format_ == kCompact ? 64 : 16));
   if (st != SFL_OK) return st;
   float left = dt;
   lists_valid_ = false;
@@ -918,8 +929,16 @@ int32_t World::dfsph(float dt, sfl_report& r) {
           detail_ = "a cell holds more than 32 particles in a world too large for neighbour lists";
           return SFL_OUT_OF_RANGE;
         }
-        format_ = format_ == kMasks16 ? kMasks32 : kLists;
-        uint32_t cap = format_ == kLists ? std::max<uint32_t>(totals().pairs_max + 64, 64) : pair_cap_;
+        // compact lists widen to full ones (their counts bound fluid and
+        // walls apart), or to 32-bit masks past the lists' index range
+        const int from = format_;
+        if (from == kCompact)
+          format_ = n_ > (size_t)kListIndexMask + 1 ? kMasks32 : kLists;
+        else
+          format_ = from == kMasks16 ? kMasks32 : kLists;
+        uint32_t cap = pair_cap_;
+        if (format_ == kLists)
+          cap = from == kCompact ? 2 * totals().pairs_max + 16 : std::max<uint32_t>(totals().pairs_max + 64, 64);
         pair_cap_ = 0;  // reallocate with the new format
         st = ensure_pairs(cap);
         if (st != SFL_OK) return st;
@@ -1185,7 +1204,7 @@ size_t World::device_bytes() const {
   auto b = [](const auto& buf) { return buf.cap * sizeof(*buf.ptr); };
   return sizeof(Control) + b(pa_) + b(va_) + b(pb_) + b(vb_) + b(aux_) + b(kappa_) + b(errors_) + b(forces_) +
          b(slot_rank_) + b(scatter_) + b(counts_) + b(cell_start_) + b(tile_sums_) + b(scan_total_) + b(pairs_) +
-         b(masks_) + b(own_slot_) +
+         b(masks_) + b(own_slot_) + b(near_) + b(factor_) +
          b(pair_counts_) + b(partial_) + b(pcount_) + b(walls_d_) + b(wall_start_d_) + b(fstart_) + b(fitems_) +
          b(wstart_) + b(witems_) + b(list_counts_);
 }

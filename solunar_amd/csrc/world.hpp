@@ -185,10 +185,14 @@ class World {
   DeviceBuffer<uint2> pairs_;
   DeviceBuffer<uint32_t> pair_counts_;
   uint32_t pair_cap_ = 0, pair_stride_ = 0;
-  // DFSPH neighbour format: explicit lists (the default, fastest on
-  // MI355X), or the compact masks (27 per particle, 16 or 32 bits, with each
-  // particle's own cell), which widen on demand
+  // DFSPH neighbour format: compact lists (6 bytes a fluid entry, read by
+  // the tiled solve; the default when the solve has its LDS tiles), full
+  // lists (8 bytes), or masks (27 per particle, 16 or 32 bits, with each
+  // particle's own cell). Each widens on demand: masks 16 -> 32 -> lists,
+  // compact -> lists.
   DeviceBuffer<uint32_t> masks_, own_slot_;
+  DeviceBuffer<uint16_t> near_;  // compact format: fluid neighbours
+  DeviceBuffer<float> factor_;
   int format_ = kLists;
   DeviceBuffer<double> partial_;
   DeviceBuffer<uint32_t> pcount_;

@@ -283,9 +283,10 @@ def test_sphere_obstacles_as_boundary_points_match(solver):
     assert gpu.state_hash() == cpu.state_hash()
 
 
-@pytest.mark.parametrize("fmt", [0, 1, 2])
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
 def test_dfsph_matches_in_every_neighbour_format(fmt, monkeypatch):
-    # 16-bit masks, 32-bit masks, explicit lists: one arithmetic, one result
+    # 16-bit masks, 32-bit masks, lists, compact lists: one arithmetic, one
+    # result
     monkeypatch.setenv("SOLUNAR_NEIGHBOUR_FORMAT", str(fmt))
     s = replace(scenes.settings_dam_1m(), spacing=0.02)
     gpu, cpu = pair(s, residual_mode=1)
@@ -329,4 +330,22 @@ def test_crowded_cells_widen_the_neighbour_format():
     for k in range(2):
         rg, rc = gpu.step(1e-4), cpu.step(1e-4)
         assert rg == rc, (k, rg, rc)
+    assert gpu.state_hash() == cpu.state_hash()
+
+
+def test_compact_lists_widen_past_their_offsets():
+    # 343 particles a cell in a column 12 cells tall: the column's staged
+    # range holds 4,116 particles, past the compact format's 12-bit offsets,
+    # so the world widens to full lists and still matches
+    s = FluidSettings(solver=DFSPH, spacing=0.02)
+    gpu, cpu = pair(s, residual_mode=1)
+    step = np.float32(0.04) / np.float32(7)
+    xy = np.arange(7, dtype=np.float32) * step + np.float32(0.3205)
+    z = np.arange(84, dtype=np.float32) * step + np.float32(0.2005)
+    pts = np.stack(np.meshgrid(xy, xy, z, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    for w in (gpu, cpu):
+        w.add_box_container((0, 0, 0), (0.8, 0.8, 0.8))
+        w.add_particles(pts)
+    rg, rc = gpu.step(1e-4), cpu.step(1e-4)
+    assert rg == rc, (rg, rc)
     assert gpu.state_hash() == cpu.state_hash()
