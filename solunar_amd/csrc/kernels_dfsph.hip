@@ -933,6 +933,14 @@ __device__ __forceinline__ uint32_t group_begin(const TileShape& sh, uint32_t v)
   return (uint32_t)((uint64_t)sh.count * v / kXcds);
 }
 
+// Each pass starts with a fixed tile per half-block, the same in every pass
+// (its ranges are set up once per launch): tile b + 2 * rank + half of the
+// block's group, b the group's first, rank = blockIdx / 8. The rest of the
+// group's tiles are its queue.
+__device__ __forceinline__ uint32_t fixed_tiles(const TileShape& sh, uint32_t v) {
+  return min(2u * (gridDim.x / kXcds), group_begin(sh, v + 1) - group_begin(sh, v));
+}
+
 // Wave 0 of the block takes the next two tiles (one per half) for this
 // pass: from its own group's queue while it lasts, then from the others in
 // turn. `done` holds the groups this block found empty; each failed take
@@ -945,7 +953,7 @@ __device__ __forceinline__ void take_tiles(Control* ctl, const TileShape& sh, ui
     bool left = false;
     if (lane < kXcds && !((done >> v) & 1u)) {
       uint32_t next = __hip_atomic_load(&ctl->tile_next[parity][v][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      left = next < group_begin(sh, v + 1) - group_begin(sh, v);
+      left = fixed_tiles(sh, v) + next < group_begin(sh, v + 1) - group_begin(sh, v);
     }
     uint64_t m = __ballot(left);
     if (m == 0) {
@@ -953,7 +961,7 @@ __device__ __forceinline__ void take_tiles(Control* ctl, const TileShape& sh, ui
       return;
     }
     v = (group + (uint32_t)__builtin_ctzll(m)) % kXcds;
-    const uint32_t b = group_begin(sh, v), e = group_begin(sh, v + 1);
+    const uint32_t b = group_begin(sh, v) + fixed_tiles(sh, v), e = group_begin(sh, v + 1);
     uint32_t k = 0;
     if (lane == 0) k = __hip_atomic_fetch_add(&ctl->tile_next[parity][v][0], 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     k = (uint32_t)__shfl((int)k, 0, 64);
@@ -974,9 +982,10 @@ __device__ __forceinline__ void take_tiles(Control* ctl, const TileShape& sh, ui
 // particle of it sums from LDS (a tile past the LDS gathers from global
 // memory instead). The next round's tiles are taken while this round
 // stages, and their column ranges loaded while it sums (both double
-// buffered), so neither latency sits between rounds.
+// buffered), so neither latency sits between rounds. (Halves run apart,
+// each with its own LDS-counter barrier, measured 11% slower.)
 template <int FORMAT>
-__device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float4* lds, TileState (&tstate)[2][2],
+__device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float4* lds, TileState (&tstate)[2][3],
                            uint32_t (&grab_s)[2][2], const Glob& g) {
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
@@ -991,46 +1000,56 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
   float2* B = reinterpret_cast<float2*>(part + cap);
   float4* Q = part;
   uint32_t done = 0;
-  if (t < 64) take_tiles(ctl, sh, parity, done, grab_s[0]);
-  __syncthreads();
-  if (wave == 0) {
-    const uint32_t tile = grab_s[0][half];
-    tile_scan(tile_load(sh, tile, tile != kNoTile, prm.cell_start, lane), tstate[half][0], lane);
-  }
-  __syncthreads();
+  // round 0: the fixed tiles (state 2); round r > 0: the tiles taken during
+  // round r - 1 (state (r - 1) & 1)
+  const TileState* tsp = &tstate[half][2];
+  bool any = 2u * (blockIdx.x / kXcds) < fixed_tiles(sh, blockIdx.x % kXcds);
   for (uint32_t r = 0;; ++r) {
-    const uint32_t cur = r & 1u, nxt = cur ^ 1u;
-    if (grab_s[cur][0] == kNoTile) break;
+    const uint32_t nxt = r & 1u;
+    if (!any && r > 0) break;
     if (t < 64) take_tiles(ctl, sh, parity, done, grab_s[nxt]);
-    const TileState& ts = tstate[half][cur];
+    const TileState& ts = *tsp;
     const uint32_t stride = __builtin_amdgcn_readfirstlane(ts.stride);
     const uint32_t count = ts.ppre[4];
     const bool fits = 16u * stride <= cap;
     if (fits && count) {
-      // wave w stages columns w and w + 8, lanes on consecutive particles
-      for (uint32_t c = wave; c < 16; c += 8) {
-        const uint32_t from = ts.ss[c], to = c * stride, len = ts.len[c];
-        for (uint32_t s = lane; s < len; s += 128) {
-          const bool two = s + 64 < len;
-          if (rate) {
-            float4 p0 = a.P[from + s], v0 = a.V[from + s];
-            float4 p1, v1;
-            if (two) {
-              p1 = a.P[from + s + 64];
-              v1 = a.V[from + s + 64];
-            }
-            A[to + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
-            B[to + s] = make_float2(v0.y, v0.z);
-            if (two) {
-              A[to + s + 64] = make_float4(p1.x, p1.y, p1.z, v1.x);
-              B[to + s + 64] = make_float2(v1.y, v1.z);
-            }
-          } else {
-            float4 q0 = a.kappa[from + s], q1;
-            if (two) q1 = a.kappa[from + s + 64];
-            Q[to + s] = q0;
-            if (two) Q[to + s + 64] = q1;
+      // wave w stages columns w and w + 8, lanes on consecutive particles,
+      // two rows of 64 of each column per trip: eight loads in flight (past
+      // a column's end they read particle 0 and are not stored)
+      const uint32_t c0 = wave, c1 = wave + 8u;
+      const uint32_t f0 = ts.ss[c0], n0 = ts.len[c0], f1 = ts.ss[c1], n1 = ts.len[c1];
+      const uint32_t to0 = c0 * stride, to1 = c1 * stride;
+      const uint32_t rows = max(n0, n1);
+      for (uint32_t s = lane; s < rows; s += 128) {
+        const uint32_t sb = s + 64;
+        const bool h0 = s < n0, h1 = s < n1, h2 = sb < n0, h3 = sb < n1;
+        const uint32_t j0 = h0 ? f0 + s : 0u, j1 = h1 ? f1 + s : 0u;
+        const uint32_t j2 = h2 ? f0 + sb : 0u, j3 = h3 ? f1 + sb : 0u;
+        if (rate) {
+          const float4 p0 = a.P[j0], p1 = a.P[j1], p2 = a.P[j2], p3 = a.P[j3];
+          const float4 v0 = a.V[j0], v1 = a.V[j1], v2 = a.V[j2], v3 = a.V[j3];
+          if (h0) {
+            A[to0 + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
+            B[to0 + s] = make_float2(v0.y, v0.z);
           }
+          if (h1) {
+            A[to1 + s] = make_float4(p1.x, p1.y, p1.z, v1.x);
+            B[to1 + s] = make_float2(v1.y, v1.z);
+          }
+          if (h2) {
+            A[to0 + sb] = make_float4(p2.x, p2.y, p2.z, v2.x);
+            B[to0 + sb] = make_float2(v2.y, v2.z);
+          }
+          if (h3) {
+            A[to1 + sb] = make_float4(p3.x, p3.y, p3.z, v3.x);
+            B[to1 + sb] = make_float2(v3.y, v3.z);
+          }
+        } else {
+          const float4 q0 = a.kappa[j0], q1 = a.kappa[j1], q2 = a.kappa[j2], q3 = a.kappa[j3];
+          if (h0) Q[to0 + s] = q0;
+          if (h1) Q[to1 + s] = q1;
+          if (h2) Q[to0 + sb] = q2;
+          if (h3) Q[to1 + sb] = q3;
         }
       }
     }
@@ -1049,6 +1068,8 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
     }
     if (wave == 0) tile_scan(next, tstate[half][nxt], lane);
     __syncthreads();
+    tsp = &tstate[half][nxt];
+    any = grab_s[nxt][0] != kNoTile;
   }
 }
 
@@ -1099,11 +1120,23 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   const uint32_t half = t / kChunk, tc = t % kChunk;
   __shared__ double slot[256];
   __shared__ uint32_t slot_count[256];
-  __shared__ TileState tstate[2][2];
+  __shared__ TileState tstate[2][3];
   __shared__ uint32_t grab_s[2][2];
+
   bool tiled = false;
   if constexpr (TILED) tiled = a.lds_cap > 0;
   const Glob g{a.P, a.V, a.kappa};
+  if constexpr (TILED) {
+    // the ranges of each half's fixed tile, the same in every pass
+    if (tiled && (t % kChunk) < 64) {
+      const TileShape sh = tile_shape(ctl);
+      const uint32_t group = blockIdx.x % kXcds, k = 2u * rank + half;
+      const bool have = k < fixed_tiles(sh, group);
+      const uint32_t tile = group_begin(sh, group) + k;
+      tile_scan(tile_load(sh, tile, have, prm.cell_start, t & 63u), tstate[half][2], t & 63u);
+    }
+    __syncthreads();
+  }
   uint32_t it = 0, pass = 0;
   for (;;) {
     // rate pass: stiffness and error of every particle
