@@ -146,6 +146,16 @@ __device__ __forceinline__ void visit_list(const StepParams& prm, uint32_t i, Fl
   }
 }
 
+// Compact entries k = 4g .. 4g + 3 of particle i form group g: near[(g * N +
+// i) * 4 + k % 4] and factor[(g * N + i) * 4 + k % 4], so a lane moves a
+// group as one 8-byte and one 16-byte access.
+__device__ __forceinline__ void store_group(const StepParams& prm, uint32_t i, uint32_t g, unsigned long long near,
+                                            float4 factor) {
+  const size_t at = (size_t)g * prm.pair_stride + i;
+  reinterpret_cast<unsigned long long*>(prm.near)[at] = near;
+  reinterpret_cast<float4*>(prm.factor)[at] = factor;
+}
+
 // compute_densities (src/fluid/mod.rs:621-675) fused with gradients + factors
 // (dfsph.rs:108-163), the neighbour count the divergence solve uses
 // (dfsph.rs:249-255) and the substep's neighbours (masks or lists). rho goes
@@ -173,6 +183,8 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
     const uint32_t cap = prm.pair_cap;
     uint2* out = prm.pairs + i;
     const size_t stride = prm.pair_stride;
+    unsigned long long gnear = 0;  // compact: the open group of four entries
+    float4 gfac = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     auto fluid = [&](uint32_t j, float ox, float oy, float oz, float sq) {
       float distance = sqrtf(sq);
       sum += kernel_value(distance, prm.radius, prm.scale);
@@ -186,10 +198,11 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
       if constexpr (FORMAT == kLists) {
         if (nf < cap) out[nf * stride] = make_uint2(j, __float_as_uint(f));
       } else if constexpr (FORMAT == kCompact) {
-        if (nf < cap) {
-          prm.near[nf * stride + i] = (uint16_t)j;
-          prm.factor[nf * stride + i] = f;
-        }
+        // entries go out four at a time (8 + 16 bytes a lane), far fewer
+        // partial-line writes than one 2-byte and one 4-byte store each
+        gnear = gnear >> 16 | (unsigned long long)j << 48;
+        gfac = make_float4(gfac.y, gfac.z, gfac.w, f);
+        if ((nf & 3u) == 3u && nf < cap) store_group(prm, i, nf >> 2, gnear, gfac);
       }
       ++nf;
     };
@@ -230,6 +243,16 @@ __global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict_
             }
           }
       if (wide) atomicOr(&ctl->flags, kFlagWide);
+      if constexpr (FORMAT == kCompact) {
+        // the last, partial group, its entries shifted down to slot 0
+        if ((nf & 3u) != 0u && nf <= cap) {
+          for (uint32_t r = nf & 3u; r < 4u; ++r) {
+            gnear >>= 16;
+            gfac = make_float4(gfac.y, gfac.z, gfac.w, 0.0f);
+          }
+          store_group(prm, i, nf >> 2, gnear, gfac);
+        }
+      }
     } else {
       // visit_near's walk, recording each cell's neighbours as a mask
       using Mask = typename MaskOf<FORMAT>::T;
@@ -413,14 +436,14 @@ struct Ent {
 template <int FORMAT>
 __device__ __forceinline__ void fluid_entries(const StepParams& prm, uint32_t i, uint32_t k, Ent (&e)[4]) {
   if constexpr (FORMAT == kCompact) {
-    const __amdgpu_buffer_rsrc_t near = list_rows(prm.near, 2u, prm, k);
-    const __amdgpu_buffer_rsrc_t factor = list_rows(prm.factor, 4u, prm, k);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      e[u].r = __builtin_amdgcn_raw_buffer_load_b16(near, i * 2u, u * prm.pair_stride * 2u, kStreamLoad);
-      e[u].f = __uint_as_float(
-          __builtin_amdgcn_raw_buffer_load_b32(factor, i * 4u, u * prm.pair_stride * 4u, kStreamLoad));
-    }
+    // k is a multiple of 4: group k / 4, one 8-byte and one 16-byte load
+    const uint32_t g = k >> 2;
+    auto n = __builtin_amdgcn_raw_buffer_load_b64(list_rows(prm.near, 8u, prm, g), i * 8u, 0, kStreamLoad);
+    auto f = __builtin_amdgcn_raw_buffer_load_b128(list_rows(prm.factor, 16u, prm, g), i * 16u, 0, kStreamLoad);
+    e[0] = Ent{n[0] & 0xffffu, __uint_as_float(f[0])};
+    e[1] = Ent{n[0] >> 16, __uint_as_float(f[1])};
+    e[2] = Ent{n[1] & 0xffffu, __uint_as_float(f[2])};
+    e[3] = Ent{n[1] >> 16, __uint_as_float(f[3])};
   } else {
     uint2 v[4];
     list_entries(prm, k, i, v);
@@ -1305,7 +1328,7 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
     const uint32_t c = prm.pair_counts[i];
     const uint32_t nf = c & 0xffffu, nw = c >> 16;
     for (uint32_t k = 0; k < nf; ++k) {
-      const size_t at_k = (size_t)k * prm.pair_stride + i;
+      const size_t at_k = ((size_t)(k >> 2) * prm.pair_stride + i) * 4 + (k & 3u);
       const uint32_t r = prm.near[at_k];
       const uint32_t code = r >> kNearBits;
       const int64_t first = cell_slot((int64_t)cx + (int64_t)(code >> 2) - 1, (int64_t)cy + (int64_t)(code & 3u) - 1,

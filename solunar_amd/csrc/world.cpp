@@ -630,8 +630,10 @@ int32_t World::ensure_pairs(uint32_t cap) {
     SFL_HIP(own_slot_.reserve(stride, 0, stream_), "hipMalloc(neighbour masks)");
   }
   if (format_ == kCompact) {
-    SFL_HIP(near_.reserve(rows * stride, 0, stream_), "hipMalloc(compact neighbours)");
-    SFL_HIP(factor_.reserve(rows * stride, 0, stream_), "hipMalloc(compact neighbours)");
+    // groups of four entries a lane: (rows rounded up to 4) * stride
+    const size_t grouped = (rows + 3) / 4 * 4 * stride;
+    SFL_HIP(near_.reserve(grouped, 0, stream_), "hipMalloc(compact neighbours)");
+    SFL_HIP(factor_.reserve(grouped, 0, stream_), "hipMalloc(compact neighbours)");
   }
   pair_cap_ = cap;
   pair_stride_ = stride;
