@@ -1317,7 +1317,10 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
         wall);
   } else if constexpr (FORMAT == kCompact) {
     // offsets count from the first particle of each neighbour column's
-    // cells from z0 - 1 on, z0 the first cell of the particle's tile
+    // cells from z0 - 1 on, z0 the first cell of the particle's tile: those
+    // firsts go to LDS once (a row per lane), then entries go four at a
+    // time, each batch's gathers issued together
+    __shared__ uint32_t firsts[11][256];
     const int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
     const uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
     const int32_t cx = cell_coord(at.x, prm.radius), cy = cell_coord(at.y, prm.radius);
@@ -1325,16 +1328,44 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
     const uint32_t depth = tile_depth(span[2]);
     const uint32_t z0 = (uint32_t)((int64_t)cz - lo[2]) / depth * depth;
     const int64_t za = (int64_t)lo[2] + (z0 > 0 ? z0 - 1 : 0);
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (uint32_t code = 0; code < 11; ++code) {
+      if ((code & 3u) == 3u) continue;  // no column: dy would be 2
+      const int64_t first = cell_slot((int64_t)cx + (int64_t)(code >> 2) - 1,
+                                      (int64_t)cy + (int64_t)(code & 3u) - 1, za, lo, span);
+      firsts[code][t] = first >= 0 ? prm.cell_start[first] : 0u;
+    }
     const uint32_t c = prm.pair_counts[i];
     const uint32_t nf = c & 0xffffu, nw = c >> 16;
-    for (uint32_t k = 0; k < nf; ++k) {
-      const size_t at_k = ((size_t)(k >> 2) * prm.pair_stride + i) * 4 + (k & 3u);
-      const uint32_t r = prm.near[at_k];
-      const uint32_t code = r >> kNearBits;
-      const int64_t first = cell_slot((int64_t)cx + (int64_t)(code >> 2) - 1, (int64_t)cy + (int64_t)(code & 3u) - 1,
-                                      za, lo, span);
-      const uint32_t j = prm.cell_start[first] + (r & kNearMask);
-      if (j != i) fluid(j, prm.factor[at_k]);
+    for (uint32_t k = 0; k < nf; k += 4) {
+      Ent e[4];
+      fluid_entries<kCompact>(prm, i, k, e);
+      uint32_t j[4];
+      float4 pj[4], vj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) j[u] = k + u < nf ? firsts[e[u].r >> kNearBits][t] + (e[u].r & kNearMask) : i;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        pj[u] = PB[j[u]];
+        vj[u] = VB[j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (k + u < nf && j[u] != i) {
+          float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
+          float toward = dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, ox, oy, oz);
+          if (toward < 0.0f) {
+            float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
+            float damp = -prm.nu * squeeze * 2.0f / (density + vj[u].w);
+            float kk = prm.mass * damp;
+            const float f = e[u].f;
+            px = px - (ox * f) * kk;
+            py = py - (oy * f) * kk;
+            pz = pz - (oz * f) * kk;
+          }
+        }
+      }
     }
     for (uint32_t k = 0; k < nw; ++k) {
       uint2 e = prm.pairs[(size_t)k * prm.pair_stride + i];
@@ -1460,9 +1491,10 @@ static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)(
       hipLaunchKernelGGL(KERNEL<kLists>, __VA_ARGS__);                            \
   } while (0)
 
-void launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm) {
-  if (prm.n == 0) return;
+hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm) {
+  if (prm.n == 0) return hipSuccess;
   SFL_FORMAT_LAUNCH(k_dfsph_density, prm.format, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
+  return hipGetLastError();
 }
 
 // Blocks of the persistent solve. Every block must be resident at once for

@@ -37,7 +37,7 @@ void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, cons
                          const StepParams& prm, float4* PA, float4* VA);
 
 // DFSPH (kernels_dfsph.hip)
-void launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
+hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
 int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes);
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
                               float4* kappa, float* errors, double* partial, uint32_t* pcount, int which,

@@ -774,7 +774,7 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   const sfl_settings& s = settings_;
   enqueue_sort();
   phase_begin(SFL_PHASE_DENSITY);
-  launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm);
+  SFL_HIP(launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm), "density pass");
   phase_end(SFL_PHASE_DENSITY);
   phase_begin(SFL_PHASE_SOLVE);
   SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, partial_.ptr, pcount_.ptr, 0,
