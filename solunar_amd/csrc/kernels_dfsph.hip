@@ -385,7 +385,7 @@ struct SolveArgs {
   float tolerance;
   uint32_t min_iterations;
   uint32_t max_iterations;
-  uint32_t lds_cap;     // particles one half-block stages in LDS; 0: no tiles
+  uint32_t lds_cap;     // particles one block stages in LDS; 0: no tiles
 };
 
 // ---- one particle of a pass ------------------------------------------------
@@ -404,17 +404,19 @@ struct Glob {
 // A tile's staged neighbourhood in LDS: x and vx in A, vy and vz in B for
 // the rate pass; (x, stiffness) in Q for the push pass. Staged slot of list
 // entry e of a particle in processed column p: index(e) + delta[p][column(e)].
-// Per half-block: the staged columns' first indexes and lengths and the
-// column stride of the tile's LDS (its longest staged column; column c
-// starts at slot c * stride), the processed ranges' starts and prefix, and
-// for the list format the slot offsets per (processed column, entry column).
+// A tile's state: the staged columns' first indexes and lengths and the
+// column stride of its LDS (its longest staged column; column c starts at
+// slot c * stride), the processed ranges' starts and prefix, and for the
+// list format the slot offsets per (processed column, entry column).
+constexpr uint32_t kStagedColumns = 24;    // 6 (x) by 4 (y)
+constexpr uint32_t kProcessedColumns = 8;  // 4 (x) by 2 (y)
 struct TileState {
-  uint32_t ss[16];
-  uint32_t len[16];
+  uint32_t ss[kStagedColumns];
+  uint32_t len[kStagedColumns];
   uint32_t stride;
-  uint32_t ps[4];
-  uint32_t ppre[5];
-  int32_t delta[4][9];  // list format: c * stride - ss[c], c the entry's staged column
+  uint32_t ps[kProcessedColumns];
+  uint32_t ppre[kProcessedColumns + 1];
+  int32_t delta[kProcessedColumns][9];  // list format: c * stride - ss[c], c the entry's staged column
 };
 struct Lds {
   const float4* A;
@@ -849,13 +851,15 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
 
 // ---- LDS tiles --------------------------------------------------------------
 //
-// The cell box is cut into tiles of 2 x 2 cell columns by `tz` cells in z
-// (about 16, the z span split evenly), numbered x-major like the cells. A
-// tile's particles are four contiguous ranges (one per column); the cells
-// that can hold their neighbours are the 4 x 4 columns around them over the
-// tile's z range widened by one cell: sixteen contiguous ranges, which the
-// half-block stages into its LDS with coalesced loads. A list entry's
-// neighbour column (dx, dy) then names the staged range directly.
+// The cell box is cut into tiles of 4 (x) by 2 (y) cell columns by
+// tile_depth cells in z, numbered x-major like the cells. A tile's
+// particles are eight contiguous ranges (one per column); the cells that can
+// hold their neighbours are the 6 x 4 columns around them over the tile's z
+// range widened by one cell: 24 contiguous ranges, which the block stages
+// into its LDS with coalesced loads. Staged column c = 4 * (x offset) + (y
+// offset), so a compact entry's column code (dx + 1) * 4 + (dy + 1) plus
+// col0 = 4 * px + py of the particle's processed column (px, py) names its
+// staged column.
 
 struct TileShape {
   uint32_t sx, sy, sz, ntx, nty, ntz, tz, count;
@@ -866,7 +870,7 @@ __device__ __forceinline__ TileShape tile_shape(const Control* ctl) {
   t.sx = ctl->span[0];
   t.sy = ctl->span[1];
   t.sz = ctl->span[2];
-  t.ntx = (t.sx + 1) / 2;
+  t.ntx = (t.sx + 3) / 4;
   t.nty = (t.sy + 1) / 2;
   t.ntz = (t.sz + 15) / 16;
   t.tz = (t.sz + t.ntz - 1) / t.ntz;
@@ -874,11 +878,10 @@ __device__ __forceinline__ TileShape tile_shape(const Control* ctl) {
   return t;
 }
 
-
 // A tile's column ranges, in two steps so the loads of the next tile's can
-// be in flight while this tile's particles are summed. Wave 0 of a half
-// (all 64 lanes, `have` uniform): lanes 0..15 load the staged columns'
-// ranges, 16..19 the processed columns'.
+// be in flight while this tile's particles are summed. Wave 0 (all 64
+// lanes, `have` uniform): lanes 0..23 load the staged columns' ranges,
+// 24..31 the processed columns'.
 struct TileLoad {
   uint32_t s0, n;
 };
@@ -889,17 +892,17 @@ __device__ __forceinline__ TileLoad tile_load(const TileShape& sh, uint32_t tile
   if (have) {
     uint32_t tx = tile / (sh.nty * sh.ntz), ty = (tile / sh.ntz) % sh.nty, tzi = tile % sh.ntz;
     uint32_t z0 = tzi * sh.tz, z1 = min(z0 + sh.tz, sh.sz);
-    if (lane < 16) {
-      int64_t x = (int64_t)(2 * tx) - 1 + (lane >> 2), y = (int64_t)(2 * ty) - 1 + (lane & 3);
+    if (lane < kStagedColumns) {
+      int64_t x = (int64_t)(4 * tx) - 1 + (lane >> 2), y = (int64_t)(2 * ty) - 1 + (lane & 3);
       if (x >= 0 && y >= 0 && x < sh.sx && y < sh.sy) {
         uint64_t base = ((uint64_t)x * sh.sy + (uint64_t)y) * sh.sz;
         uint32_t za = z0 > 0 ? z0 - 1 : 0, zb = min(z1 + 1, sh.sz);
         r.s0 = start[base + za];
         r.n = start[base + zb] - r.s0;
       }
-    } else if (lane < 20) {
-      uint32_t p = lane - 16;
-      uint64_t x = 2 * tx + (p >> 1), y = 2 * ty + (p & 1);
+    } else if (lane < kStagedColumns + kProcessedColumns) {
+      uint32_t p = lane - kStagedColumns;
+      uint64_t x = 4 * tx + (p >> 1), y = 2 * ty + (p & 1);
       if (x < sh.sx && y < sh.sy) {
         uint64_t base = (x * sh.sy + y) * sh.sz;
         r.s0 = start[base + z0];
@@ -910,41 +913,39 @@ __device__ __forceinline__ TileLoad tile_load(const TileShape& sh, uint32_t tile
   return r;
 }
 
-// Then the same wave takes the longest staged column (lanes 0..15), scans
-// the processed prefix (lanes 16..19), and lanes 0..35 write the list
-// format's slot offsets into `ts`.
+// Then the same wave takes the longest staged column, scans the processed
+// prefix, and writes the list format's slot offsets into `ts`.
 __device__ __forceinline__ void tile_scan(TileLoad r, TileState& ts, uint32_t lane) {
-  uint32_t longest = lane < 16 ? r.n : 0u;
-  for (int d = 8; d >= 1; d >>= 1) longest = max(longest, (uint32_t)__shfl_xor((int)longest, d, 64));
+  uint32_t longest = lane < kStagedColumns ? r.n : 0u;
+  for (int d = 16; d >= 1; d >>= 1) longest = max(longest, (uint32_t)__shfl_xor((int)longest, d, 64));
   const uint32_t stride = (uint32_t)__shfl((int)longest, 0, 64);
   uint32_t incl = r.n;
-  for (uint32_t d = 1; d < 4; d <<= 1) {
+  for (uint32_t d = 1; d < kProcessedColumns; d <<= 1) {
     uint32_t up = (uint32_t)__shfl_up((int)incl, d, 64);
-    if (lane < 20 && lane >= 16 + d) incl += up;
+    if (lane < kStagedColumns + kProcessedColumns && lane >= kStagedColumns + d) incl += up;
   }
-  if (lane < 16) {
+  if (lane < kStagedColumns) {
     ts.ss[lane] = r.s0;
     ts.len[lane] = r.n;
     if (lane == 0) ts.stride = stride;
-  } else if (lane < 20) {
-    ts.ps[lane - 16] = r.s0;
-    ts.ppre[lane - 16] = incl - r.n;
-    if (lane == 19) ts.ppre[4] = incl;
+  } else if (lane < kStagedColumns + kProcessedColumns) {
+    ts.ps[lane - kStagedColumns] = r.s0;
+    ts.ppre[lane - kStagedColumns] = incl - r.n;
+    if (lane == kStagedColumns + kProcessedColumns - 1) ts.ppre[kProcessedColumns] = incl;
   }
   // list format: staged slot of entry e of processed column p is
-  // index(e) + delta[p][column(e)]
-  uint32_t col = 0, p = lane / 9, c = lane % 9;
-  if (lane < 36) {
-    int dx = (int)(c / 3) - 1, dy = (int)(c % 3) - 1;
-    col = (uint32_t)((1 + (int)(p >> 1) + dx) * 4 + (1 + (int)(p & 1) + dy));
+  // index(e) + delta[p][column(e)]; 72 entries, lanes take k and k + 64
+  for (uint32_t k = lane; k < kProcessedColumns * 9; k += 64) {
+    const uint32_t p = k / 9, c = k % 9;
+    const int dx = (int)(c / 3) - 1, dy = (int)(c % 3) - 1;
+    const uint32_t col = (uint32_t)((1 + (int)(p >> 1) + dx) * 4 + (1 + (int)(p & 1) + dy));
+    ts.delta[p][c] = (int32_t)(col * stride) - (int32_t)ts.ss[col];
   }
-  const uint32_t first = (uint32_t)__shfl((int)r.s0, (int)col, 64);
-  if (lane < 36) ts.delta[p][c] = (int32_t)(col * stride) - (int32_t)first;
 }
 
 __device__ __forceinline__ uint32_t processed_index(const TileState& ts, uint32_t q, uint32_t& p) {
   p = 0;
-  while (p < 3 && ts.ppre[p + 1] <= q) ++p;
+  while (p < kProcessedColumns - 1 && ts.ppre[p + 1] <= q) ++p;
   return ts.ps[p] + (q - ts.ppre[p]);
 }
 
@@ -956,20 +957,19 @@ __device__ __forceinline__ uint32_t group_begin(const TileShape& sh, uint32_t v)
   return (uint32_t)((uint64_t)sh.count * v / kXcds);
 }
 
-// Each pass starts with a fixed tile per half-block, the same in every pass
-// (its ranges are set up once per launch): tile b + 2 * rank + half of the
-// block's group, b the group's first, rank = blockIdx / 8. The rest of the
-// group's tiles are its queue.
+// Each pass starts with a fixed tile per block, the same in every pass (its
+// ranges are set up once per launch): tile b + rank of the block's group, b
+// the group's first, rank = blockIdx / 8. The rest of the group's tiles are
+// its queue.
 __device__ __forceinline__ uint32_t fixed_tiles(const TileShape& sh, uint32_t v) {
-  return min(2u * (gridDim.x / kXcds), group_begin(sh, v + 1) - group_begin(sh, v));
+  return min(gridDim.x / kXcds, group_begin(sh, v + 1) - group_begin(sh, v));
 }
 
-// Wave 0 of the block takes the next two tiles (one per half) for this
-// pass: from its own group's queue while it lasts, then from the others in
-// turn. `done` holds the groups this block found empty; each failed take
-// adds one, so the loop ends. Result in grab_s[0..1] (kNoTile for none).
-__device__ __forceinline__ void take_tiles(Control* ctl, const TileShape& sh, uint32_t parity, uint32_t& done,
-                                           uint32_t* grab_s) {
+// Wave 0 takes the block's next tile for this pass: from its own group's
+// queue while it lasts, then from the others in turn. `done` holds the
+// groups this block found empty; each failed take adds one, so the loop
+// ends. Returns kNoTile when every queue is empty.
+__device__ __forceinline__ uint32_t take_tile(Control* ctl, const TileShape& sh, uint32_t parity, uint32_t& done) {
   const uint32_t lane = threadIdx.x & 63u, group = blockIdx.x % kXcds;
   for (;;) {
     uint32_t v = (group + lane) % kXcds;
@@ -979,68 +979,58 @@ __device__ __forceinline__ void take_tiles(Control* ctl, const TileShape& sh, ui
       left = fixed_tiles(sh, v) + next < group_begin(sh, v + 1) - group_begin(sh, v);
     }
     uint64_t m = __ballot(left);
-    if (m == 0) {
-      if (lane == 0) grab_s[0] = grab_s[1] = kNoTile;
-      return;
-    }
+    if (m == 0) return kNoTile;
     v = (group + (uint32_t)__builtin_ctzll(m)) % kXcds;
     const uint32_t b = group_begin(sh, v) + fixed_tiles(sh, v), e = group_begin(sh, v + 1);
     uint32_t k = 0;
-    if (lane == 0) k = __hip_atomic_fetch_add(&ctl->tile_next[parity][v][0], 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) k = __hip_atomic_fetch_add(&ctl->tile_next[parity][v][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     k = (uint32_t)__shfl((int)k, 0, 64);
-    if (b + k < e) {
-      if (lane == 0) {
-        grab_s[0] = b + k;
-        grab_s[1] = b + k + 1 < e ? b + k + 1 : kNoTile;
-      }
-      return;
-    }
+    if (b + k < e) return b + k;
     done |= 1u << v;
   }
 }
 
 // One pass over the tiles (rate = true: rate pass, else push pass), queued
-// per XCD group with stealing (tile_next[parity]). Each round the block's
-// halves take a tile each: stage its neighbourhood in LDS, then every
-// particle of it sums from LDS (a tile past the LDS gathers from global
-// memory instead). The next round's tiles are taken while this round
-// stages, and their column ranges loaded while it sums (both double
-// buffered), so neither latency sits between rounds. (Halves run apart,
-// each with its own LDS-counter barrier, measured 11% slower.)
+// per XCD group with stealing (tile_next[parity]). Each round the block
+// takes a tile: stages its neighbourhood in LDS, then every particle of it
+// sums from LDS (a tile past the LDS gathers from global memory instead).
+// The next round's tile is taken and its column ranges loaded by the last
+// wave while the others sum (both double buffered), so neither latency
+// sits between rounds.
 template <int FORMAT>
-__device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float4* lds, TileState (&tstate)[2][3],
-                           uint32_t (&grab_s)[2][2], const Glob& g) {
+__device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float4* lds, TileState (&tstate)[3],
+                           uint32_t (&grab_s)[2], const Glob& g) {
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
   const TileShape sh = tile_shape(ctl);
-  const uint32_t t = threadIdx.x, half = t / kChunk, tc = t % kChunk;
-  const uint32_t lane = t & 63u, wave = tc >> 6;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   const uint32_t cap = a.lds_cap;
-  // each half's LDS: A (float4) then B (float2) for the rate pass, Q (float4)
-  // for the push pass
-  float4* part = lds + (size_t)half * (cap * 3 / 2);
-  float4* A = part;
-  float2* B = reinterpret_cast<float2*>(part + cap);
-  float4* Q = part;
+  // the LDS: A (float4) then B (float2) for the rate pass, Q (float4) for
+  // the push pass
+  float4* A = lds;
+  float2* B = reinterpret_cast<float2*>(lds + cap);
+  float4* Q = lds;
   uint32_t done = 0;
-  // round 0: the fixed tiles (state 2); round r > 0: the tiles taken during
+  // round 0: the fixed tile (state 2); round r > 0: the tile taken during
   // round r - 1 (state (r - 1) & 1)
-  const TileState* tsp = &tstate[half][2];
-  bool any = 2u * (blockIdx.x / kXcds) < fixed_tiles(sh, blockIdx.x % kXcds);
+  const TileState* tsp = &tstate[2];
+  bool any = blockIdx.x / kXcds < fixed_tiles(sh, blockIdx.x % kXcds);
   for (uint32_t r = 0;; ++r) {
     const uint32_t nxt = r & 1u;
     if (!any && r > 0) break;
-    if (t < 64) take_tiles(ctl, sh, parity, done, grab_s[nxt]);
     const TileState& ts = *tsp;
     const uint32_t stride = __builtin_amdgcn_readfirstlane(ts.stride);
-    const uint32_t count = ts.ppre[4];
-    const bool fits = 16u * stride <= cap;
+    const uint32_t count = ts.ppre[kProcessedColumns];
+    const bool fits = kStagedColumns * stride <= cap;
     if (fits && count) {
-      // wave w stages columns w and w + 8, lanes on consecutive particles,
-      // two rows of 64 of each column per trip: eight loads in flight (past
-      // a column's end they read particle 0 and are not stored)
-      const uint32_t c0 = wave, c1 = wave + 8u;
-      const uint32_t f0 = ts.ss[c0], n0 = ts.len[c0], f1 = ts.ss[c1], n1 = ts.len[c1];
+      // wave w stages columns w and w + 16 (when below 24), lanes on
+      // consecutive particles, two rows of 64 of each column per trip:
+      // eight loads in flight (past a column's end they read particle 0
+      // and are not stored)
+      const uint32_t c0 = wave, c1 = wave + 16u;
+      const bool second = c1 < kStagedColumns;
+      const uint32_t f0 = ts.ss[c0], n0 = ts.len[c0];
+      const uint32_t f1 = second ? ts.ss[c1] : 0u, n1 = second ? ts.len[c1] : 0u;
       const uint32_t to0 = c0 * stride, to1 = c1 * stride;
       const uint32_t rows = max(n0, n1);
       for (uint32_t s = lane; s < rows; s += 128) {
@@ -1077,10 +1067,15 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
       }
     }
     __syncthreads();
-    const uint32_t tile_n = grab_s[nxt][half];
+    // the last wave (whose particles, if any, come last) takes the next
+    // tile and loads its ranges while the others sum
     TileLoad next{0u, 0u};
-    if (wave == 0) next = tile_load(sh, tile_n, tile_n != kNoTile, prm.cell_start, lane);
-    for (uint32_t q = tc; q < count; q += kChunk) {
+    if (wave == kSolveWaves - 1) {
+      const uint32_t tile = take_tile(ctl, sh, parity, done);
+      if (lane == 0) grab_s[nxt] = tile;
+      next = tile_load(sh, tile, tile != kNoTile, prm.cell_start, lane);
+    }
+    for (uint32_t q = t; q < count; q += kSolveThreads) {
       uint32_t p;
       uint32_t i = processed_index(ts, q, p);
       const Lds l{A, B, Q, &ts, p, 4u * (p >> 1) + (p & 1u), stride};
@@ -1089,10 +1084,10 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
       else
         push_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
     }
-    if (wave == 0) tile_scan(next, tstate[half][nxt], lane);
+    if (wave == kSolveWaves - 1) tile_scan(next, tstate[nxt], lane);
     __syncthreads();
-    tsp = &tstate[half][nxt];
-    any = grab_s[nxt][0] != kNoTile;
+    tsp = &tstate[nxt];
+    any = grab_s[nxt] != kNoTile;
   }
 }
 
@@ -1143,20 +1138,20 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   const uint32_t half = t / kChunk, tc = t % kChunk;
   __shared__ double slot[256];
   __shared__ uint32_t slot_count[256];
-  __shared__ TileState tstate[2][3];
-  __shared__ uint32_t grab_s[2][2];
+  __shared__ TileState tstate[3];
+  __shared__ uint32_t grab_s[2];
 
   bool tiled = false;
   if constexpr (TILED) tiled = a.lds_cap > 0;
   const Glob g{a.P, a.V, a.kappa};
   if constexpr (TILED) {
-    // the ranges of each half's fixed tile, the same in every pass
-    if (tiled && (t % kChunk) < 64) {
+    // the ranges of the block's fixed tile, the same in every pass
+    if (tiled && t < 64) {
       const TileShape sh = tile_shape(ctl);
-      const uint32_t group = blockIdx.x % kXcds, k = 2u * rank + half;
-      const bool have = k < fixed_tiles(sh, group);
-      const uint32_t tile = group_begin(sh, group) + k;
-      tile_scan(tile_load(sh, tile, have, prm.cell_start, t & 63u), tstate[half][2], t & 63u);
+      const uint32_t group = blockIdx.x % kXcds;
+      const bool have = rank < fixed_tiles(sh, group);
+      const uint32_t tile = group_begin(sh, group) + rank;
+      tile_scan(tile_load(sh, tile, have, prm.cell_start, t), tstate[2], t);
     }
     __syncthreads();
   }
@@ -1538,9 +1533,9 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
       if (hipFuncGetAttributes(&attr, k) != hipSuccess) return 0;
       room = std::min<int64_t>(room, (int64_t)lds - (int64_t)attr.sharedSizeBytes);
     }
-    // whole staged particles of both halves, rounded down to 64 each
-    room = room / (2 * kTileBytes) / 64 * 64 * (2 * kTileBytes);
-    if (room >= 2 * kTileBytes * 512) {
+    // whole staged particles, rounded down to 64
+    room = room / kTileBytes / 64 * 64 * kTileBytes;
+    if (room >= kTileBytes * 1024) {
       bool fit = true;
       for (const void* k : tiled) {
         if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)room) != hipSuccess) return 0;
@@ -1575,7 +1570,7 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   // the compact format is read by the tiled kernel alone
   if (prm.format == kCompact && tile_bytes == 0) return hipErrorInvalidValue;
   const bool tiled = (prm.format == kLists || prm.format == kCompact) && tile_bytes > 0;
-  a.lds_cap = tiled ? tile_bytes / (2 * kTileBytes) : 0u;
+  a.lds_cap = tiled ? tile_bytes / kTileBytes : 0u;
   // a multiple of 8 blocks so every XCD gets the same share; at least one
   // chunk per XCD group or the small grid of a small world
   uint32_t blocks = (uint32_t)grid;

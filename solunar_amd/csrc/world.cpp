@@ -140,12 +140,12 @@ int32_t World::init_device() {
   // instead of staging each tile's neighbourhood in LDS
   const char* tiles = std::getenv("SOLUNAR_SOLVE_TILES");
   solve_grid_ = solve_grid_limit(knob ? std::atoi(knob) : 0, !(tiles && std::atoi(tiles) == 0), &solve_lds_);
-  // SOLUNAR_SOLVE_TILE_PARTICLES caps the particles one half-block stages
+  // SOLUNAR_SOLVE_TILE_PARTICLES caps the particles one block stages
   // (rounded down to 64; tiles with more gather from global memory)
   const char* tile_cap = std::getenv("SOLUNAR_SOLVE_TILE_PARTICLES");
   if (tile_cap) {
     long c = std::max(std::atol(tile_cap), 0L) / 64 * 64;
-    solve_lds_ = (uint32_t)std::min<uint64_t>(solve_lds_, (uint64_t)c * 2 * kTileBytes);
+    solve_lds_ = (uint32_t)std::min<uint64_t>(solve_lds_, (uint64_t)c * kTileBytes);
   }
   // SOLUNAR_NEIGHBOUR_FORMAT (0: 16-bit masks, 1: 32-bit masks, 2: lists,
   // 3: compact lists, the default where the solve has its LDS tiles) picks
