@@ -1118,6 +1118,57 @@ __device__ __forceinline__ void chunk_sums(const SolveArgs& a, uint32_t first, u
   }
 }
 
+// The tiled solve's stop sums: block b takes the slots s = b, b + grid, ...
+// below 256 and folds each itself: the waves sum its chunks s + 256m (as
+// chunk_sums does), then thread 0 adds them in m order from 0.0, which is
+// the slot the fold below would build from the chunk sums. The slot goes
+// to partial[s] and pcount[s]; `scratch` is the block's staging LDS, free
+// between passes.
+__device__ __forceinline__ void slot_sums(const SolveArgs& a, double* scratch, uint32_t room) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t* scratch_count = reinterpret_cast<uint32_t*>(scratch + room);
+  for (uint32_t s = blockIdx.x; s < 256u; s += gridDim.x) {
+    const uint32_t chunks = s < a.chunks ? (a.chunks - s + 255u) / 256u : 0u;
+    double total = 0.0;
+    uint32_t count = 0;
+    // in runs of `room` chunks (room covers 1M particles' 8 per slot many
+    // times over; larger worlds take several runs)
+    for (uint32_t m0 = 0; m0 < chunks; m0 += room) {
+      const uint32_t m1 = min(m0 + room, chunks);
+      for (uint32_t m = m0 + wave; m < m1; m += kSolveWaves) {
+        const uint32_t c = s + 256u * m;
+        double acc = 0.0;
+        uint32_t cnt = 0;
+        for (uint32_t k = 0; k < 8; ++k) {
+          uint32_t i = c * kChunk + k * 64u + lane;
+          float e = i < a.prm.n ? a.errors[i] : 0.0f;
+          acc += (double)e;
+          cnt += e > 0.0f ? 1u : 0u;
+        }
+        for (int h = 32; h >= 1; h >>= 1) {
+          acc = acc + __shfl_down(acc, h, 64);
+          cnt += (uint32_t)__shfl_down((int)cnt, h, 64);
+        }
+        if (lane == 0) {
+          scratch[m - m0] = acc;
+          scratch_count[m - m0] = cnt;
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (uint32_t m = m0; m < m1; ++m) {
+          total += scratch[m - m0];
+          count += scratch_count[m - m0];
+        }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      a.partial[s] = total;
+      a.pcount[s] = count;
+    }
+  }
+}
+
 // solve (src/fluid/dfsph.rs:222-275) with density_rate (dfsph.rs:166-181) and
 // push_apart (dfsph.rs:184-216). Without tiles, block b takes chunks of 512
 // particles (its XCD's eighth, every per_xcd-th), each half of the block
@@ -1165,7 +1216,8 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
       if constexpr (TILED) tiled_pass<FORMAT>(a, true, pass & 1u, lds, tstate, grab_s, g);
       ++pass;
       if (!grid_barrier(ctl)) return;
-      chunk_sums(a, blockIdx.x, a.chunks, gridDim.x);
+      // the staging LDS as (double, count) pairs: 16 bytes per particle slot
+      slot_sums(a, reinterpret_cast<double*>(lds), a.lds_cap);
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
         uint32_t i = c * kChunk + tc;
@@ -1175,8 +1227,14 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
       chunk_sums(a, c_begin + rank, c_end, per_xcd);
     }
     if (!grid_barrier(ctl)) return;
-    // every block folds the chunk sums the same way
-    if (t < 256) {
+    // every block folds the chunk sums the same way (tiled: the slots are
+    // folded already)
+    if (tiled) {
+      if (t < 256) {
+        slot[t] = a.partial[t];
+        slot_count[t] = a.pcount[t];
+      }
+    } else if (t < 256) {
       double s = 0.0;
       uint32_t cn = 0;
       uint32_t c = t;

@@ -187,8 +187,9 @@ int32_t World::ensure_particles(size_t n) {
   SFL_HIP(slot_rank_.reserve(n, 0, stream_), "hipMalloc(slot ranks)");
   SFL_HIP(scatter_.reserve(n, 0, stream_), "hipMalloc(scatter)");
   size_t chunks = (n + 511) / 512;
-  SFL_HIP(partial_.reserve(std::max<size_t>(chunks, 1), 0, stream_), "hipMalloc(partials)");
-  SFL_HIP(pcount_.reserve(std::max<size_t>(chunks, 1), 0, stream_), "hipMalloc(partial counts)");
+  // chunk sums, or the 256 slot sums of the tiled solve
+  SFL_HIP(partial_.reserve(std::max<size_t>(chunks, 256), 0, stream_), "hipMalloc(partials)");
+  SFL_HIP(pcount_.reserve(std::max<size_t>(chunks, 256), 0, stream_), "hipMalloc(partial counts)");
   size_t tiles = (n + 1 + 2047) / 2048;
   SFL_HIP(tile_sums_.reserve(std::max<size_t>(tiles, (cell_cap_ + 2047) / 2048 + 1), tile_sums_.cap ? 0 : 0, stream_),
           "hipMalloc(tile sums)");
