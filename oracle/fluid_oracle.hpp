@@ -359,42 +359,72 @@ struct Report {
 // How the DFSPH solve averages its per-particle errors.
 //  kReferenceF32: the reference's f32 left fold in particle order
 //                 (src/fluid/dfsph.rs:263-267).
-//  kFixedTreeF64: the device path's FP64 sum of fixed shape: chunks of 512
-//                 particles; lane l of 64 sums e[512c + l + 64k], k = 0..7,
-//                 in order; lanes combine by halving (s[l] += s[l + h],
-//                 h = 32..1); chunk sums go to 256 slots, slot t summing chunks
-//                 t, t + 256, ... in order; slots combine by halving
-//                 (h = 128..1). Count of errors > 0 is exact.
+//  kFixedTreeF64: the device path's FP64 sum of fixed shape (tile_tree_sum).
+//                 Count of errors > 0 is exact.
 enum ResidualMode : int32_t { kReferenceF32 = 0, kFixedTreeF64 = 1 };
 
+// The device's FP64 stop sum (kernels_dfsph.hip, fold_tiles and the tile
+// sums of its rate pass), over the particles' cells in particle order (sorted
+// by cell, then id). The cell box is the fluid's: span s = max - min + 1 per
+// axis. Solve tiles are 4 x 2 cell columns by tz cells, ntx = ceil(sx/4),
+// nty = ceil(sy/2), ntz = ceil(sz/16), tz = ceil(sz/ntz), numbered
+// (tx * nty + ty) * ntz + tz_index. A tile's particles in particle order are
+// q = 0, 1, ...: accumulator t of 1024 adds e[q] for q = t, t + 1024, ... in
+// order from 0.0; the 16 groups of 64 fold by halving (s[l] += s[l + h],
+// h = 32..1), their 16 results likewise (h = 8..1): the tile's sum. Lane l of
+// 64 adds tile sums l, l + 64, ... in order from 0.0; the lanes fold by
+// halving (h = 32..1).
 template <class E>
-inline double fixed_tree_sum(const std::vector<E>& e) {
-  size_t n = e.size();
-  size_t chunks = (n + 511) / 512;
-  std::vector<double> partial(chunks);
-  for (size_t c = 0; c < chunks; ++c) {
-    double s[64];
-    for (int l = 0; l < 64; ++l) {
-      double acc = 0.0;
-      for (int k = 0; k < 8; ++k) {
-        size_t i = c * 512 + (size_t)l + 64 * (size_t)k;
-        if (i < n) acc += (double)e[i];
-      }
-      s[l] = acc;
+inline double tile_tree_sum(const std::vector<E>& e, const std::vector<CellKey>& cells) {
+  const size_t n = e.size();
+  if (n == 0) return 0.0;
+  int64_t lo[3] = {cells[0].x, cells[0].y, cells[0].z}, hi[3] = {lo[0], lo[1], lo[2]};
+  for (const CellKey& c : cells) {
+    const int64_t v[3] = {c.x, c.y, c.z};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], v[a]);
+      hi[a] = std::max(hi[a], v[a]);
     }
-    for (int h = 32; h >= 1; h >>= 1)
-      for (int l = 0; l < h; ++l) s[l] = s[l] + s[l + h];
-    partial[c] = s[0];
   }
-  double slot[256];
-  for (int t = 0; t < 256; ++t) {
-    double acc = 0.0;
-    for (size_t c = (size_t)t; c < chunks; c += 256) acc += partial[c];
-    slot[t] = acc;
+  const uint64_t sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+  const uint64_t ntx = (sx + 3) / 4, nty = (sy + 1) / 2, ntz = (sz + 15) / 16, tz = (sz + ntz - 1) / ntz;
+  const uint64_t tiles = ntx * nty * ntz;
+  // each tile's particles, in particle order (a stable counting sort)
+  std::vector<uint64_t> of(n), start(tiles + 1, 0);
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t x = cells[i].x - lo[0], y = cells[i].y - lo[1], z = cells[i].z - lo[2];
+    of[i] = ((x / 4) * nty + y / 2) * ntz + z / tz;
+    ++start[of[i] + 1];
   }
-  for (int h = 128; h >= 1; h >>= 1)
-    for (int t = 0; t < h; ++t) slot[t] = slot[t] + slot[t + h];
-  return slot[0];
+  for (uint64_t t = 0; t < tiles; ++t) start[t + 1] += start[t];
+  std::vector<uint64_t> at(start.begin(), start.end() - 1);
+  std::vector<size_t> members(n);
+  for (size_t i = 0; i < n; ++i) members[at[of[i]]++] = i;
+  std::vector<double> tsum(tiles);
+  std::vector<double> acc(1024);
+  for (uint64_t t = 0; t < tiles; ++t) {
+    std::fill(acc.begin(), acc.end(), 0.0);
+    for (uint64_t q = 0; q < start[t + 1] - start[t]; ++q) acc[q % 1024] += (double)e[members[start[t] + q]];
+    double ws[16];
+    for (int w = 0; w < 16; ++w) {
+      double* s = &acc[64 * w];
+      for (int h = 32; h >= 1; h >>= 1)
+        for (int l = 0; l < h; ++l) s[l] = s[l] + s[l + h];
+      ws[w] = s[0];
+    }
+    for (int h = 8; h >= 1; h >>= 1)
+      for (int w = 0; w < h; ++w) ws[w] = ws[w] + ws[w + h];
+    tsum[t] = ws[0];
+  }
+  double lane[64];
+  for (int l = 0; l < 64; ++l) {
+    double s = 0.0;
+    for (uint64_t k = (uint64_t)l; k < tiles; k += 64) s += tsum[k];
+    lane[l] = s;
+  }
+  for (int h = 32; h >= 1; h >>= 1)
+    for (int l = 0; l < h; ++l) lane[l] = lane[l] + lane[l + h];
+  return lane[0];
 }
 
 // One Jacobi iteration's stop inputs, for logging the two averages side by
@@ -960,6 +990,12 @@ class World {
     size_t n = len();
     std::vector<R> stiffness(n);
     std::vector<R> errors(n);
+    // the cells of the device's solve tiles (positions hold still in a solve)
+    std::vector<CellKey> cells;
+    if (residual_mode == kFixedTreeF64 || log_solves) {
+      cells.resize(n);
+      for (size_t i = 0; i < n; ++i) cells[i] = cell_of(positions[i]);
+    }
     for (;;) {
       dispatch.run(n, [&](size_t b, size_t e) {
         for (size_t i = b; i < e; ++i) {
@@ -989,7 +1025,7 @@ class World {
       R avg32 = sum32 / R(denom);
       double avg64 = 0.0;
       bool need64 = residual_mode == kFixedTreeF64 || log_solves;
-      if (need64) avg64 = fixed_tree_sum(errors) / (double)denom;
+      if (need64) avg64 = tile_tree_sum(errors, cells) / (double)denom;
       if (log_solves) solve_log.push_back({which, iterations, loaded, (float)avg32, avg64});
       bool below = residual_mode == kReferenceF32 ? (avg32 <= R(tolerance)) : (avg64 <= (double)tolerance);
       bool settled = iterations >= settings.min_iterations && below;

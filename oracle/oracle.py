@@ -9,7 +9,7 @@ particles on that many workers, run backwards like dispatch::testing::Threads,
 src/dispatch.rs:28-48), `skip_job` (dispatch::testing::SkipsJobs,
 src/dispatch.rs:50-65), `precision` (32 = reference arithmetic, 64 = FP64
 yardstick) and `residual_mode` (0 = the reference's f32 fold of DFSPH errors,
-1 = the device's fixed-shape FP64 sum).
+1 = the device's fixed-shape FP64 sum over its solve tiles).
 """
 from __future__ import annotations
 
@@ -83,7 +83,7 @@ def library() -> ctypes.CDLL:
         "orc_state_hash": (ctypes.c_uint64, [P]),
         "orc_error_detail": (ctypes.c_char_p, [P]),
         "orc_fnv1a": (ctypes.c_uint64, [ctypes.c_char_p, sz]),
-        "orc_fixed_tree_sum": (ctypes.c_double, [fp, sz]),
+        "orc_tile_tree_sum": (ctypes.c_double, [fp, ctypes.POINTER(i32), sz]),
         "orc_grid_selftest": (i32, []),
         "orc_plant_mistake": (None, [i32]),
     }
@@ -336,6 +336,10 @@ def fnv1a(data: bytes) -> int:
     return int(library().orc_fnv1a(data, len(data)))
 
 
-def fixed_tree_sum(values) -> float:
+def tile_tree_sum(values, cells) -> float:
+    """The device's FP64 stop sum of `values` whose cells (n x 3 int32, in
+    sorted order: by cell x, y, z) are `cells`."""
     v = np.ascontiguousarray(values, np.float32)
-    return float(library().orc_fixed_tree_sum(_p(v, ctypes.c_float), v.shape[0]))
+    c = np.ascontiguousarray(cells, np.int32).reshape(-1, 3)
+    assert c.shape[0] == v.shape[0]
+    return float(library().orc_tile_tree_sum(_p(v, ctypes.c_float), _p(c, ctypes.c_int32), v.shape[0]))

@@ -401,10 +401,13 @@ int32_t orc_grid_selftest() {
 }
 
 // The fixed-shape FP64 residual sum (ResidualMode::kFixedTreeF64) over
-// caller values, so tests can check it against exact arithmetic.
-double orc_fixed_tree_sum(const float* e, size_t n) {
+// caller values and cells (x, y, z per value, in sorted order), so tests can
+// check it against exact arithmetic.
+double orc_tile_tree_sum(const float* e, const int32_t* xyz, size_t n) {
   std::vector<float> v(e, e + n);
-  return fixed_tree_sum(v);
+  std::vector<CellKey> cells(n);
+  for (size_t i = 0; i < n; ++i) cells[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+  return tile_tree_sum(v, cells);
 }
 
 // Particles from double inputs (exact in the FP64 world), for finite

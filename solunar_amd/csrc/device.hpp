@@ -26,6 +26,7 @@ constexpr uint32_t kFlagBarrier = 4u;    // a grid barrier gave up waiting
 
 constexpr uint32_t kFlagPairs = 8u;     // a particle had more neighbours than the list holds
 constexpr uint32_t kFlagWide = 32u;     // a neighbour sat past the width of its cell's mask
+constexpr uint32_t kFlagTiles = 64u;    // the solve tiles outnumbered the stop sum's room
 
 constexpr int kResidualLogCap = 256;
 constexpr int kStripes = 64;

@@ -17,12 +17,16 @@
 // Stop test: the reference averages the per-particle errors with an f32 left
 // fold in particle order (dfsph.rs:263-267), which no parallel sum can
 // reproduce. The solve instead sums them in FP64 with a fixed shape that the
-// CPU oracle replays exactly: for chunk c of 512 particles, lane l of 64 adds
-// errors 512c + l + 64k, k = 0..7, in order; lanes fold by halving
-// (s_l += s_{l+h}, h = 32..1); chunk sums land in 256 slots, slot t adding
-// chunks t, t+256, ... in order; slots fold by halving (h = 128..1). Every
-// block computes that total itself after the barrier, so all blocks take the
-// same decision without a second barrier.
+// CPU oracle replays exactly, over the solve tiles of the fluid cell box
+// (4 x 2 cell columns by tz cells in z, tz = ceil(sz / ceil(sz / 16)), numbered
+// x-major, then y, then z). A tile's particles in particle order are its
+// processed order q = 0, 1, ...: thread t of 1024 adds the errors of
+// q = t, t + 1024, ... in order from 0.0; each wave's 64 threads fold by
+// halving (s_l += s_{l+h}, h = 32..1), then the 16 wave sums (h = 8..1): the
+// tile sum. Lane l of 64 adds tile sums l, l + 64, ... in order; the lanes
+// fold by halving (h = 32..1). The tiled rate pass forms each tile's sum as
+// it goes, so after the pass's barrier every wave folds the tile sums itself
+// and all blocks take the same decision with no further barrier.
 #include <algorithm>
 
 #include "device.hpp"
@@ -377,10 +381,11 @@ struct SolveArgs {
   float4* V;            // velocities (updated in place); w = density
   const float* alpha;   // factors, sign bit = surface particle
   float4* kappa;        // (x, stiffness) of the last rate pass: the push gathers both at once
-  float* errors;        // error of the last rate pass
-  double* partial;      // per chunk error sums
-  uint32_t* pcount;     // per chunk count of errors > 0
-  uint32_t chunks;
+  float* errors;        // error of the last rate pass (untiled passes)
+  double* tile_sum;     // per solve tile error sums
+  uint32_t* tile_count; // per solve tile count of errors > 0
+  uint32_t tile_cap;    // room of both
+  uint32_t chunks;      // 512-particle chunks the untiled passes walk
   int32_t which;        // 0 divergence, 1 density
   float tolerance;
   uint32_t min_iterations;
@@ -408,6 +413,7 @@ struct Glob {
 // column stride of its LDS (its longest staged column; column c starts at
 // slot c * stride), the processed ranges' starts and prefix, and for the
 // list format the slot offsets per (processed column, entry column).
+constexpr uint32_t kNoTile = 0xffffffffu;
 constexpr uint32_t kStagedColumns = 24;    // 6 (x) by 4 (y)
 constexpr uint32_t kProcessedColumns = 8;  // 4 (x) by 2 (y)
 struct TileState {
@@ -417,6 +423,7 @@ struct TileState {
   uint32_t ps[kProcessedColumns];
   uint32_t ppre[kProcessedColumns + 1];
   int32_t delta[kProcessedColumns][9];  // list format: c * stride - ss[c], c the entry's staged column
+  uint32_t tile;                        // its number, kNoTile for none
 };
 struct Lds {
   const float4* A;
@@ -664,9 +671,10 @@ __device__ __forceinline__ void walk_walls(const StepParams& prm, uint32_t i, ui
   }
 }
 
-// Walls of the rate pass, then stiffness and error; writes kappa and errors.
+// Walls of the rate pass, then stiffness and error; writes kappa, returns the
+// error.
 template <int FORMAT>
-__device__ __forceinline__ void finish_rate(const SolveArgs& a, uint32_t i, float4 at, float4 vi, float rate,
+__device__ __forceinline__ float finish_rate(const SolveArgs& a, uint32_t i, float4 at, float4 vi, float rate,
                                             uint32_t nf, uint32_t nw) {
   const StepParams& prm = a.prm;
   // walls in list order (a particle at a corner lists ~40 of them)
@@ -690,11 +698,11 @@ __device__ __forceinline__ void finish_rate(const SolveArgs& a, uint32_t i, floa
     error = dt * r / prm.rest;
   }
   a.kappa[i] = make_float4(at.x, at.y, at.z, target * fabsf(fa));
-  a.errors[i] = error;
+  return error;
 }
 
 template <int FORMAT, bool DEEP = true>
-__device__ __forceinline__ void rate_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, bool staged,
+__device__ __forceinline__ float rate_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, bool staged,
                                          int64_t sysz, int64_t szz) {
   float4 at = a.P[i];
   float4 vi = a.V[i];
@@ -707,7 +715,7 @@ __device__ __forceinline__ void rate_one(const SolveArgs& a, uint32_t i, const G
   } else {
     rate = fluid_rate<FORMAT>(a, i, at, vi, nf, g, l, sysz, szz);
   }
-  finish_rate<FORMAT>(a, i, at, vi, rate, nf, nw);
+  return finish_rate<FORMAT>(a, i, at, vi, rate, nf, nw);
 }
 
 template <int FORMAT, bool DEEP = true>
@@ -883,12 +891,12 @@ __device__ __forceinline__ TileShape tile_shape(const Control* ctl) {
 // lanes, `have` uniform): lanes 0..23 load the staged columns' ranges,
 // 24..31 the processed columns'.
 struct TileLoad {
-  uint32_t s0, n;
+  uint32_t s0, n, tile;
 };
 
 __device__ __forceinline__ TileLoad tile_load(const TileShape& sh, uint32_t tile, bool have,
                                               const uint32_t* __restrict__ start, uint32_t lane) {
-  TileLoad r{0u, 0u};
+  TileLoad r{0u, 0u, have ? tile : kNoTile};
   if (have) {
     uint32_t tx = tile / (sh.nty * sh.ntz), ty = (tile / sh.ntz) % sh.nty, tzi = tile % sh.ntz;
     uint32_t z0 = tzi * sh.tz, z1 = min(z0 + sh.tz, sh.sz);
@@ -933,6 +941,7 @@ __device__ __forceinline__ void tile_scan(TileLoad r, TileState& ts, uint32_t la
     ts.ppre[lane - kStagedColumns] = incl - r.n;
     if (lane == kStagedColumns + kProcessedColumns - 1) ts.ppre[kProcessedColumns] = incl;
   }
+  if (lane == 0) ts.tile = r.tile;
   // list format: staged slot of entry e of processed column p is
   // index(e) + delta[p][column(e)]; 72 entries, lanes take k and k + 64
   for (uint32_t k = lane; k < kProcessedColumns * 9; k += 64) {
@@ -948,8 +957,6 @@ __device__ __forceinline__ uint32_t processed_index(const TileState& ts, uint32_
   while (p < kProcessedColumns - 1 && ts.ppre[p + 1] <= q) ++p;
   return ts.ps[p] + (q - ts.ppre[p]);
 }
-
-constexpr uint32_t kNoTile = 0xffffffffu;
 
 // Tiles [begin, end) of XCD group v: the v-th eighth, numbered x-major, so
 // one group's tiles are one slab of the box.
@@ -990,6 +997,121 @@ __device__ __forceinline__ uint32_t take_tile(Control* ctl, const TileShape& sh,
   }
 }
 
+// ---- the stop sum over solve tiles ------------------------------------------
+
+// Thread sums (value, count) of a wave folded by halving, h = 32..1; lane 0
+// holds the wave's.
+__device__ __forceinline__ void wave_fold(double& v, uint32_t& c) {
+  for (int h = 32; h >= 1; h >>= 1) {
+    v = v + __shfl_down(v, h, 64);
+    c += (uint32_t)__shfl_down((int)c, h, 64);
+  }
+}
+
+// One wave: the block's 16 wave sums folded by halving (h = 8..1) into the
+// tile's sum, stored at `tile`.
+__device__ __forceinline__ void store_tile_sum(const SolveArgs& a, uint32_t tile, const double* wsum,
+                                               const uint32_t* wcnt, uint32_t lane) {
+  double v = lane < kSolveWaves ? wsum[lane] : 0.0;
+  uint32_t c = lane < kSolveWaves ? wcnt[lane] : 0u;
+  for (int h = kSolveWaves / 2; h >= 1; h >>= 1) {
+    v = v + __shfl_down(v, h, 64);
+    c += (uint32_t)__shfl_down((int)c, h, 64);
+  }
+  if (lane == 0 && tile < a.tile_cap) {
+    a.tile_sum[tile] = v;
+    a.tile_count[tile] = c;
+  }
+}
+
+// Every wave: lane l adds tile sums l, l + 64, ... in order (eight loads in
+// flight), the lanes fold by halving; returns the total and its count.
+__device__ __forceinline__ double fold_tiles(const SolveArgs& a, uint32_t tiles, uint32_t& loaded) {
+  const uint32_t lane = threadIdx.x & 63u;
+  double s = 0.0;
+  uint32_t cn = 0;
+  uint32_t k = lane;
+  for (; k + 7u * 64u < tiles; k += 8u * 64u) {
+    double v[8];
+    uint32_t w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      v[u] = a.tile_sum[k + u * 64u];
+      w[u] = a.tile_count[k + u * 64u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s += v[u];
+      cn += w[u];
+    }
+  }
+  for (; k < tiles; k += 64u) {
+    s += a.tile_sum[k];
+    cn += a.tile_count[k];
+  }
+  wave_fold(s, cn);
+  loaded = (uint32_t)__shfl((int)cn, 0, 64);
+  return __shfl(s, 0, 64);
+}
+
+// The same fold, for the tiled solve: the block's threads copy the tile
+// sums into its staging LDS (free between passes) `room` at a time, room a
+// multiple of 64, so lane l's tiles keep their order; every wave then adds
+// them from LDS.
+__device__ __forceinline__ double fold_tiles_lds(const SolveArgs& a, uint32_t tiles, float4* lds, uint32_t room,
+                                                 uint32_t& loaded) {
+  double* L = reinterpret_cast<double*>(lds);
+  uint32_t* C = reinterpret_cast<uint32_t*>(L + room);
+  const uint32_t t = threadIdx.x, lane = t & 63u;
+  double s = 0.0;
+  uint32_t cn = 0;
+  for (uint32_t base = 0; base < tiles; base += room) {
+    const uint32_t m = min(room, tiles - base);
+    for (uint32_t k = t; k < m; k += kSolveThreads) {
+      L[k] = a.tile_sum[base + k];
+      C[k] = a.tile_count[base + k];
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < m; k += 64u) {
+      s += L[k];
+      cn += C[k];
+    }
+    // the LDS is overwritten next (next room, or the push pass's staging)
+    __syncthreads();
+  }
+  wave_fold(s, cn);
+  loaded = (uint32_t)__shfl((int)cn, 0, 64);
+  return __shfl(s, 0, 64);
+}
+
+// The untiled passes' tile sums, from the errors they stored: block b takes
+// tiles b, b + grid, ...; wave 0 loads a tile's ranges, then every thread
+// adds its errors in processed order.
+__device__ void tile_sums_from_errors(const SolveArgs& a, TileState& ts, double* wsum, uint32_t* wcnt) {
+  const TileShape sh = tile_shape(a.prm.ctl);
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  for (uint32_t tile = blockIdx.x; tile < sh.count; tile += gridDim.x) {
+    if (wave == 0) tile_scan(tile_load(sh, tile, true, a.prm.cell_start, lane), ts, lane);
+    __syncthreads();
+    const uint32_t count = ts.ppre[kProcessedColumns];
+    double acc = 0.0;
+    uint32_t cnt = 0;
+    for (uint32_t q = t; q < count; q += kSolveThreads) {
+      uint32_t p;
+      const float e = a.errors[processed_index(ts, q, p)];
+      acc += (double)e;
+      cnt += e > 0.0f ? 1u : 0u;
+    }
+    wave_fold(acc, cnt);
+    if (lane == 0) {
+      wsum[wave] = acc;
+      wcnt[wave] = cnt;
+    }
+    __syncthreads();
+    if (wave == 0) store_tile_sum(a, tile, wsum, wcnt, lane);
+  }
+}
+
 // One pass over the tiles (rate = true: rate pass, else push pass), queued
 // per XCD group with stealing (tile_next[parity]). Each round the block
 // takes a tile: stages its neighbourhood in LDS, then every particle of it
@@ -997,9 +1119,13 @@ __device__ __forceinline__ uint32_t take_tile(Control* ctl, const TileShape& sh,
 // The next round's tile is taken and its column ranges loaded by the last
 // wave while the others sum (both double buffered), so neither latency
 // sits between rounds.
+//
+// The rate pass also forms each tile's error sum: every thread adds the
+// errors of its particles, the waves fold theirs into `wsum`, and wave 0
+// folds those into the tile's sum at the start of the next round.
 template <int FORMAT>
 __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float4* lds, TileState (&tstate)[3],
-                           uint32_t (&grab_s)[2], const Glob& g) {
+                           uint32_t (&grab_s)[2], const Glob& g, double* wsum, uint32_t* wcnt) {
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
   const TileShape sh = tile_shape(ctl);
@@ -1015,10 +1141,13 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
   // round r - 1 (state (r - 1) & 1)
   const TileState* tsp = &tstate[2];
   bool any = blockIdx.x / kXcds < fixed_tiles(sh, blockIdx.x % kXcds);
+  uint32_t summed = kNoTile;  // the last round's tile (rate pass), its wave sums in wsum
   for (uint32_t r = 0;; ++r) {
+    if (rate && wave == 0 && summed != kNoTile) store_tile_sum(a, summed, wsum, wcnt, lane);
     const uint32_t nxt = r & 1u;
     if (!any && r > 0) break;
     const TileState& ts = *tsp;
+    summed = __builtin_amdgcn_readfirstlane(ts.tile);
     const uint32_t stride = __builtin_amdgcn_readfirstlane(ts.stride);
     const uint32_t count = ts.ppre[kProcessedColumns];
     const bool fits = kStagedColumns * stride <= cap;
@@ -1075,14 +1204,26 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
       if (lane == 0) grab_s[nxt] = tile;
       next = tile_load(sh, tile, tile != kNoTile, prm.cell_start, lane);
     }
+    double acc = 0.0;
+    uint32_t cnt = 0;
     for (uint32_t q = t; q < count; q += kSolveThreads) {
       uint32_t p;
       uint32_t i = processed_index(ts, q, p);
       const Lds l{A, B, Q, &ts, p, 4u * (p >> 1) + (p & 1u), stride};
-      if (rate)
-        rate_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
-      else
+      if (rate) {
+        const float e = rate_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
+        acc += (double)e;
+        cnt += e > 0.0f ? 1u : 0u;
+      } else {
         push_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
+      }
+    }
+    if (rate) {
+      wave_fold(acc, cnt);
+      if (lane == 0) {
+        wsum[wave] = acc;
+        wcnt[wave] = cnt;
+      }
     }
     if (wave == kSolveWaves - 1) tile_scan(next, tstate[nxt], lane);
     __syncthreads();
@@ -1091,90 +1232,6 @@ __device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float
   }
 }
 
-// Chunk sums of the stop test for chunks c, c + step, ...: wave w of the
-// block takes every 16th of them; lane l adds errors 512c + l + 64k in k
-// order, as defined above.
-__device__ __forceinline__ void chunk_sums(const SolveArgs& a, uint32_t first, uint32_t last, uint32_t step) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t mine = 0;
-  for (uint32_t c = first; c < last; c += step, ++mine) {
-    if (mine % kSolveWaves != wave) continue;
-    double acc = 0.0;
-    uint32_t cnt = 0;
-    for (uint32_t k = 0; k < 8; ++k) {
-      uint32_t i = c * kChunk + k * 64u + lane;
-      float e = i < a.prm.n ? a.errors[i] : 0.0f;
-      acc += (double)e;
-      cnt += e > 0.0f ? 1u : 0u;
-    }
-    for (int h = 32; h >= 1; h >>= 1) {
-      acc = acc + __shfl_down(acc, h, 64);
-      cnt += (uint32_t)__shfl_down((int)cnt, h, 64);
-    }
-    if (lane == 0) {
-      a.partial[c] = acc;
-      a.pcount[c] = cnt;
-    }
-  }
-}
-
-// The tiled solve's stop sums: block b takes the slots s = b, b + grid, ...
-// below 256 and folds each itself: the waves sum its chunks s + 256m (as
-// chunk_sums does), then thread 0 adds them in m order from 0.0, which is
-// the slot the fold below would build from the chunk sums. The slot goes
-// to partial[s] and pcount[s]; `scratch` is the block's staging LDS, free
-// between passes.
-__device__ __forceinline__ void slot_sums(const SolveArgs& a, double* scratch, uint32_t room) {
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t* scratch_count = reinterpret_cast<uint32_t*>(scratch + room);
-  for (uint32_t s = blockIdx.x; s < 256u; s += gridDim.x) {
-    const uint32_t chunks = s < a.chunks ? (a.chunks - s + 255u) / 256u : 0u;
-    double total = 0.0;
-    uint32_t count = 0;
-    // in runs of `room` chunks (room covers 1M particles' 8 per slot many
-    // times over; larger worlds take several runs)
-    for (uint32_t m0 = 0; m0 < chunks; m0 += room) {
-      const uint32_t m1 = min(m0 + room, chunks);
-      for (uint32_t m = m0 + wave; m < m1; m += kSolveWaves) {
-        const uint32_t c = s + 256u * m;
-        double acc = 0.0;
-        uint32_t cnt = 0;
-        for (uint32_t k = 0; k < 8; ++k) {
-          uint32_t i = c * kChunk + k * 64u + lane;
-          float e = i < a.prm.n ? a.errors[i] : 0.0f;
-          acc += (double)e;
-          cnt += e > 0.0f ? 1u : 0u;
-        }
-        for (int h = 32; h >= 1; h >>= 1) {
-          acc = acc + __shfl_down(acc, h, 64);
-          cnt += (uint32_t)__shfl_down((int)cnt, h, 64);
-        }
-        if (lane == 0) {
-          scratch[m - m0] = acc;
-          scratch_count[m - m0] = cnt;
-        }
-      }
-      __syncthreads();
-      if (threadIdx.x == 0)
-        for (uint32_t m = m0; m < m1; ++m) {
-          total += scratch[m - m0];
-          count += scratch_count[m - m0];
-        }
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      a.partial[s] = total;
-      a.pcount[s] = count;
-    }
-  }
-}
-
-// solve (src/fluid/dfsph.rs:222-275) with density_rate (dfsph.rs:166-181) and
-// push_apart (dfsph.rs:184-216). Without tiles, block b takes chunks of 512
-// particles (its XCD's eighth, every per_xcd-th), each half of the block
-// every other one, thread t owning particle 512c + t, and sums its own
-// chunks before the barrier. With tiles (the list format, LDS permitting),
-// the passes walk tiles and the chunk sums follow a barrier of their own.
 template <int FORMAT, bool TILED>
 __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   extern __shared__ float4 lds[];
@@ -1187,11 +1244,17 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   const uint32_t c_end = (uint32_t)((uint64_t)a.chunks * (xcd + 1) / kXcds);
   const uint32_t t = threadIdx.x;
   const uint32_t half = t / kChunk, tc = t % kChunk;
-  __shared__ double slot[256];
-  __shared__ uint32_t slot_count[256];
+  __shared__ double wsum[kSolveWaves];
+  __shared__ uint32_t wcnt[kSolveWaves];
   __shared__ TileState tstate[3];
   __shared__ uint32_t grab_s[2];
 
+  // the stop sum's room is sized for the largest cell box the table holds;
+  // every block sees the same count and leaves before any barrier
+  if (tile_shape(ctl).count > a.tile_cap) {
+    if (blockIdx.x == 0 && t == 0) atomicOr(&ctl->flags, kFlagTiles);
+    return;
+  }
   bool tiled = false;
   if constexpr (TILED) tiled = a.lds_cap > 0;
   const Glob g{a.P, a.V, a.kappa};
@@ -1208,84 +1271,30 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   }
   uint32_t it = 0, pass = 0;
   for (;;) {
-    // rate pass: stiffness and error of every particle
+    // rate pass: stiffness and error of every particle, and the tile sums
     if (tiled) {
       // the queues of the next pass start again from zero
       if (blockIdx.x == 0 && t < kXcds)
         __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if constexpr (TILED) tiled_pass<FORMAT>(a, true, pass & 1u, lds, tstate, grab_s, g);
+      if constexpr (TILED) tiled_pass<FORMAT>(a, true, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
       ++pass;
       if (!grid_barrier(ctl)) return;
-      // the staging LDS as (double, count) pairs: 16 bytes per particle slot
-      slot_sums(a, reinterpret_cast<double*>(lds), a.lds_cap);
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
         uint32_t i = c * kChunk + tc;
-        if (i < prm.n) rate_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz);
+        if (i < prm.n) a.errors[i] = rate_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz);
       }
-      __syncthreads();
-      chunk_sums(a, c_begin + rank, c_end, per_xcd);
+      if (!grid_barrier(ctl)) return;
+      tile_sums_from_errors(a, tstate[0], wsum, wcnt);
+      if (!grid_barrier(ctl)) return;
     }
-    if (!grid_barrier(ctl)) return;
-    // every block folds the chunk sums the same way (tiled: the slots are
-    // folded already)
-    if (tiled) {
-      if (t < 256) {
-        slot[t] = a.partial[t];
-        slot_count[t] = a.pcount[t];
-      }
-    } else if (t < 256) {
-      double s = 0.0;
-      uint32_t cn = 0;
-      uint32_t c = t;
-      // eight loads in flight, added in chunk order
-      for (; c + 7u * 256u < a.chunks; c += 8u * 256u) {
-        double v[8];
-        uint32_t w[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          v[u] = a.partial[c + u * 256u];
-          w[u] = a.pcount[c + u * 256u];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          s += v[u];
-          cn += w[u];
-        }
-      }
-      for (; c < a.chunks; c += 256u) {
-        s += a.partial[c];
-        cn += a.pcount[c];
-      }
-      slot[t] = s;
-      slot_count[t] = cn;
-    }
-    __syncthreads();
-    // halving folds: slot t += slot t + h for h = 128, 64 in LDS, then
-    // h = 32 .. 1 across wave 0's lanes
-    for (uint32_t h = 128; h >= 64; h >>= 1) {
-      if (t < h) {
-        slot[t] = slot[t] + slot[t + h];
-        slot_count[t] += slot_count[t + h];
-      }
-      __syncthreads();
-    }
-    if (t < 64) {
-      double v = slot[t];
-      uint32_t cn = slot_count[t];
-      for (int h = 32; h >= 1; h >>= 1) {
-        v = v + __shfl_down(v, h, 64);
-        cn += (uint32_t)__shfl_down((int)cn, h, 64);
-      }
-      if (t == 0) {
-        slot[0] = v;
-        slot_count[0] = cn;
-      }
-    }
-    __syncthreads();
-    double total = slot[0];
-    uint32_t loaded = slot_count[0];
-    __syncthreads();
+    // every wave folds the tile sums the same way
+    uint32_t loaded = 0;
+    double total;
+    if (tiled)
+      total = fold_tiles_lds(a, tile_shape(ctl).count, lds, a.lds_cap * kTileBytes / 12u / 64u * 64u, loaded);
+    else
+      total = fold_tiles(a, tile_shape(ctl).count, loaded);
     double average = total / (double)(loaded > 1u ? loaded : 1u);
     if (blockIdx.x == 0 && t == 0) {
       uint32_t at = ctl->log_len;
@@ -1301,7 +1310,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     if (tiled) {
       if (blockIdx.x == 0 && t < kXcds)
         __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if constexpr (TILED) tiled_pass<FORMAT>(a, false, pass & 1u, lds, tstate, grab_s, g);
+      if constexpr (TILED) tiled_pass<FORMAT>(a, false, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
       ++pass;
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
@@ -1608,8 +1617,9 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
 }
 
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
-                              float4* kappa, float* errors, double* partial, uint32_t* pcount, int which,
-                              float tolerance, uint32_t min_it, uint32_t max_it, int grid, uint32_t tile_bytes) {
+                              float4* kappa, float* errors, double* tile_sum, uint32_t* tile_count,
+                              uint32_t tile_cap, int which, float tolerance, uint32_t min_it, uint32_t max_it,
+                              int grid, uint32_t tile_bytes) {
   if (prm.n == 0) return hipSuccess;
   SolveArgs a;
   a.prm = prm;
@@ -1618,8 +1628,9 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   a.alpha = alpha;
   a.kappa = kappa;
   a.errors = errors;
-  a.partial = partial;
-  a.pcount = pcount;
+  a.tile_sum = tile_sum;
+  a.tile_count = tile_count;
+  a.tile_cap = tile_cap;
   a.chunks = (prm.n + kChunk - 1) / kChunk;
   a.which = which;
   a.tolerance = tolerance;

@@ -40,9 +40,9 @@ void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, cons
 hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
 int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes);
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
-                              float4* kappa, float* errors, double* partial, uint32_t* pcount, int which,
-                              float tolerance, uint32_t min_it, uint32_t max_it, int grid,
-                              uint32_t tile_bytes);
+                              float4* kappa, float* errors, double* tile_sum, uint32_t* tile_count,
+                              uint32_t tile_cap, int which, float tolerance, uint32_t min_it, uint32_t max_it,
+                              int grid, uint32_t tile_bytes);
 void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,
                           float* forces);
 void launch_dfsph_advance(hipStream_t s, const float4* PB, const StepParams& prm, float4* PA, float4* VA,

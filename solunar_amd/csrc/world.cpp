@@ -186,10 +186,6 @@ int32_t World::ensure_particles(size_t n) {
   SFL_HIP(errors_.reserve(n, 0, stream_), "hipMalloc(errors)");
   SFL_HIP(slot_rank_.reserve(n, 0, stream_), "hipMalloc(slot ranks)");
   SFL_HIP(scatter_.reserve(n, 0, stream_), "hipMalloc(scatter)");
-  size_t chunks = (n + 511) / 512;
-  // chunk sums, or the 256 slot sums of the tiled solve
-  SFL_HIP(partial_.reserve(std::max<size_t>(chunks, 256), 0, stream_), "hipMalloc(partials)");
-  SFL_HIP(pcount_.reserve(std::max<size_t>(chunks, 256), 0, stream_), "hipMalloc(partial counts)");
   size_t tiles = (n + 1 + 2047) / 2048;
   SFL_HIP(tile_sums_.reserve(std::max<size_t>(tiles, (cell_cap_ + 2047) / 2048 + 1), tile_sums_.cap ? 0 : 0, stream_),
           "hipMalloc(tile sums)");
@@ -656,6 +652,12 @@ int32_t World::ensure_cells(uint64_t volume) {
   SFL_HIP(cell_start_.reserve(cap + 1, 0, stream_), "hipMalloc(cell table)");
   size_t tiles = (cap + 2047) / 2048 + 1;
   SFL_HIP(tile_sums_.reserve(std::max<size_t>(tiles, (n_ + 1 + 2047) / 2048), 0, stream_), "hipMalloc(tile sums)");
+  // solve tiles of a box of V cells: ceil(sx/4) ceil(sy/2) ceil(sz/16), at
+  // most max(1, 2V/3) (sy = 1 and sx or sz >= 2 halves a factor; sy >= 2
+  // gives ceil(sy/2) <= 2sy/3)
+  const size_t stop = 2 * cap / 3 + 2;
+  SFL_HIP(stop_sum_.reserve(stop, 0, stream_), "hipMalloc(stop sums)");
+  SFL_HIP(stop_count_.reserve(stop, 0, stream_), "hipMalloc(stop counts)");
   SFL_HIP(hipStreamSynchronize(stream_), "sync");
   cell_cap_ = cap;
   return SFL_OK;
@@ -778,16 +780,18 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   SFL_HIP(launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm), "density pass");
   phase_end(SFL_PHASE_DENSITY);
   phase_begin(SFL_PHASE_SOLVE);
-  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, partial_.ptr, pcount_.ptr, 0,
-                             s.divergence_tolerance, s.min_iterations, s.max_iterations, solve_grid_, solve_lds_),
+  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, stop_sum_.ptr,
+                             stop_count_.ptr, stop_cap(), 0, s.divergence_tolerance, s.min_iterations, s.max_iterations,
+                             solve_grid_, solve_lds_),
           "cooperative launch (divergence solve)");
   phase_end(SFL_PHASE_SOLVE);
   phase_begin(SFL_PHASE_VISCOUS);
   launch_dfsph_viscous(stream_, pb_.ptr, vb_.ptr, prm, va_.ptr, keep_forces_ ? forces_.ptr : nullptr);
   phase_end(SFL_PHASE_VISCOUS);
   phase_begin(SFL_PHASE_SOLVE);
-  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, va_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, partial_.ptr, pcount_.ptr, 1,
-                             s.density_tolerance, s.min_iterations, s.max_iterations, solve_grid_, solve_lds_),
+  SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, va_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, stop_sum_.ptr,
+                             stop_count_.ptr, stop_cap(), 1, s.density_tolerance, s.min_iterations, s.max_iterations,
+                             solve_grid_, solve_lds_),
           "cooperative launch (density solve)");
   phase_end(SFL_PHASE_SOLVE);
   phase_begin(SFL_PHASE_ADVANCE);
@@ -962,6 +966,10 @@ int32_t World::dfsph(float dt, sfl_report& r) {
     lists_emitted_ = false;
     if (c.flags & kFlagBarrier) {
       detail_ = "a grid barrier of the persistent solve timed out";
+      return SFL_DEVICE_ERROR;
+    }
+    if (c.flags & kFlagTiles) {
+      detail_ = "the solve tiles outnumbered the room of the stop sum";
       return SFL_DEVICE_ERROR;
     }
     if (c.flags & kFlagNonFinite) {
@@ -1208,7 +1216,7 @@ size_t World::device_bytes() const {
   return sizeof(Control) + b(pa_) + b(va_) + b(pb_) + b(vb_) + b(aux_) + b(kappa_) + b(errors_) + b(forces_) +
          b(slot_rank_) + b(scatter_) + b(counts_) + b(cell_start_) + b(tile_sums_) + b(scan_total_) + b(pairs_) +
          b(masks_) + b(own_slot_) + b(near_) + b(factor_) +
-         b(pair_counts_) + b(partial_) + b(pcount_) + b(walls_d_) + b(wall_start_d_) + b(fstart_) + b(fitems_) +
+         b(pair_counts_) + b(stop_sum_) + b(stop_count_) + b(walls_d_) + b(wall_start_d_) + b(fstart_) + b(fitems_) +
          b(wstart_) + b(witems_) + b(list_counts_);
 }
 

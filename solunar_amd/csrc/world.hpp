@@ -194,8 +194,13 @@ class World {
   DeviceBuffer<uint16_t> near_;  // compact format: fluid neighbours
   DeviceBuffer<float> factor_;
   int format_ = kLists;
-  DeviceBuffer<double> partial_;
-  DeviceBuffer<uint32_t> pcount_;
+  // the DFSPH stop sum's per solve tile sums and counts, sized with the
+  // cell table
+  DeviceBuffer<double> stop_sum_;
+  DeviceBuffer<uint32_t> stop_count_;
+  uint32_t stop_cap() const {
+    return (uint32_t)std::min<size_t>(std::min(stop_sum_.cap, stop_count_.cap), 0xffffffffu);
+  }
   DeviceBuffer<float4> walls_d_;
   DeviceBuffer<uint32_t> wall_start_d_;
   CellBox wall_box_{};

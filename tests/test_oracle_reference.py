@@ -12,7 +12,7 @@ import math
 import numpy as np
 import pytest
 
-from oracle.oracle import OracleWorld, fixed_tree_sum, fnv1a, library
+from oracle.oracle import OracleWorld, fnv1a, library, tile_tree_sum
 from solunar_amd import scenes
 from solunar_amd.types import DFSPH, WCSPH, FluidSettings, PhysicsError
 
@@ -319,42 +319,53 @@ def test_the_state_hash_is_fnv_of_id_position_velocity_in_id_order():
     assert w.state_hash() == fnv1a(blob)
 
 
-def test_the_fixed_shape_residual_sum_is_the_documented_tree():
+def sorted_cells(rng, n, span):
+    """n random cells inside `span`, in the sorted (x, y, z) order."""
+    c = np.stack([rng.integers(0, s, n) for s in span], axis=1).astype(np.int32)
+    return c[np.lexsort((c[:, 2], c[:, 1], c[:, 0]))]
+
+
+def test_the_stop_sum_counts_every_error_once():
+    # integer errors add exactly in any order, so any dropped or doubled term
+    # shows; the layouts cross tile edges in x (4), y (2) and z (16, cut even)
     rng = np.random.default_rng(3)
-    for n in (1, 63, 512, 513, 5000, 200_000):
+    for n, span in ((1, (1, 1, 1)), (700, (9, 5, 40)), (5000, (30, 3, 17)), (3000, (1, 1, 2000)),
+                    (20_000, (3, 40, 7)), (2500, (1, 1, 1))):
+        cells = sorted_cells(rng, n, span)
+        e = rng.integers(0, 1000, n).astype(np.float32)
+        assert tile_tree_sum(e, cells) == float(e.astype(np.int64).sum())
+    # and each position alone
+    cells = sorted_cells(rng, 300, (9, 5, 40))
+    for i in range(300):
+        e = np.zeros(300, np.float32)
+        e[i] = 0.75
+        assert tile_tree_sum(e, cells) == 0.75
+
+
+def test_the_stop_sum_rounds_in_the_documented_order():
+    u = 2.0 ** -53  # half an ulp of 1.0: 1 + u rounds to 1, 1 + 2u does not
+    # one cell of 1025 particles: thread 0 adds q = 0 then q = 1024 before the
+    # wave fold, so (1 + u) + u -> 1, where a tree over all three gives 1 + 2u
+    e = np.zeros(1025, np.float32)
+    e[0], e[1], e[1024] = 1.0, u, u
+    assert tile_tree_sum(e, np.zeros((1025, 3), np.int32)) == 1.0
+    # 65 tiles along x (4 columns each): lane 0 adds tiles 0 and 64 before
+    # the lane fold, lane 1 holds tile 1
+    cells = np.zeros((3, 3), np.int32)
+    cells[:, 0] = (0, 4, 259)
+    assert tile_tree_sum(np.array([1.0, u, u], np.float32), cells) == 1.0
+    # the same values in one tile: threads 0, 1, 2 fold as (e0 + e2) + e1
+    cells[:, 0] = (0, 1, 2)
+    assert tile_tree_sum(np.array([u, 1.0, u], np.float32), cells) == 1.0 + 2 * u
+    assert tile_tree_sum(np.array([1.0, u, u], np.float32), cells) == 1.0
+
+
+def test_the_stop_sum_agrees_with_an_exact_sum():
+    rng = np.random.default_rng(5)
+    for n, span in ((63, (2, 2, 2)), (5000, (20, 20, 20)), (200_000, (60, 60, 30))):
+        cells = sorted_cells(rng, n, span)
         e = (rng.random(n, dtype=np.float32) * rng.choice([1e-6, 1e-3, 1.0], n)).astype(np.float32)
-        # the same shape written independently
-        chunks = (n + 511) // 512
-        part = []
-        for c in range(chunks):
-            s = [0.0] * 64
-            for lane in range(64):
-                acc = 0.0
-                for k in range(8):
-                    i = c * 512 + lane + 64 * k
-                    if i < n:
-                        acc += float(e[i])
-                s[lane] = acc
-            h = 32
-            while h >= 1:
-                for lane in range(h):
-                    s[lane] = s[lane] + s[lane + h]
-                h //= 2
-            part.append(s[0])
-        slot = [0.0] * 256
-        for t in range(256):
-            acc = 0.0
-            for c in range(t, chunks, 256):
-                acc += part[c]
-            slot[t] = acc
-        h = 128
-        while h >= 1:
-            for t in range(h):
-                slot[t] = slot[t] + slot[t + h]
-            h //= 2
-        got = fixed_tree_sum(e)
-        assert got == slot[0]
-        assert got == pytest.approx(math.fsum(map(float, e)), rel=1e-13)
+        assert tile_tree_sum(e, cells) == pytest.approx(math.fsum(map(float, e)), rel=1e-13)
 
 
 @pytest.mark.parametrize("mistake, check", [
