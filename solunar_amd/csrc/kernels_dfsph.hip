@@ -1087,8 +1087,8 @@ __device__ __forceinline__ double fold_tiles_lds(const SolveArgs& a, uint32_t ti
 // The untiled passes' tile sums, from the errors they stored: block b takes
 // tiles b, b + grid, ...; wave 0 loads a tile's ranges, then every thread
 // adds its errors in processed order.
-__device__ void tile_sums_from_errors(const SolveArgs& a, TileState& ts, double* wsum, uint32_t* wcnt) {
-  const TileShape sh = tile_shape(a.prm.ctl);
+__device__ void tile_sums_from_errors(const SolveArgs& a, const TileShape& sh, TileState& ts, double* wsum,
+                                      uint32_t* wcnt) {
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   for (uint32_t tile = blockIdx.x; tile < sh.count; tile += gridDim.x) {
     if (wave == 0) tile_scan(tile_load(sh, tile, true, a.prm.cell_start, lane), ts, lane);
@@ -1124,11 +1124,11 @@ __device__ void tile_sums_from_errors(const SolveArgs& a, TileState& ts, double*
 // errors of its particles, the waves fold theirs into `wsum`, and wave 0
 // folds those into the tile's sum at the start of the next round.
 template <int FORMAT>
-__device__ void tiled_pass(const SolveArgs& a, bool rate, uint32_t parity, float4* lds, TileState (&tstate)[3],
-                           uint32_t (&grab_s)[2], const Glob& g, double* wsum, uint32_t* wcnt) {
+__device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, bool rate, uint32_t parity, float4* lds,
+                           TileState (&tstate)[3], uint32_t (&grab_s)[2], const Glob& g, double* wsum,
+                           uint32_t* wcnt) {
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
-  const TileShape sh = tile_shape(ctl);
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   const uint32_t cap = a.lds_cap;
   // the LDS: A (float4) then B (float2) for the rate pass, Q (float4) for
@@ -1248,10 +1248,14 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   __shared__ uint32_t wcnt[kSolveWaves];
   __shared__ TileState tstate[3];
   __shared__ uint32_t grab_s[2];
+  __shared__ TileShape shape_s;  // read once: the passes start without a global load
 
   // the stop sum's room is sized for the largest cell box the table holds;
   // every block sees the same count and leaves before any barrier
-  if (tile_shape(ctl).count > a.tile_cap) {
+  if (t == 0) shape_s = tile_shape(ctl);
+  __syncthreads();
+  const TileShape& sh = shape_s;
+  if (sh.count > a.tile_cap) {
     if (blockIdx.x == 0 && t == 0) atomicOr(&ctl->flags, kFlagTiles);
     return;
   }
@@ -1261,7 +1265,6 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   if constexpr (TILED) {
     // the ranges of the block's fixed tile, the same in every pass
     if (tiled && t < 64) {
-      const TileShape sh = tile_shape(ctl);
       const uint32_t group = blockIdx.x % kXcds;
       const bool have = rank < fixed_tiles(sh, group);
       const uint32_t tile = group_begin(sh, group) + rank;
@@ -1276,7 +1279,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
       // the queues of the next pass start again from zero
       if (blockIdx.x == 0 && t < kXcds)
         __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if constexpr (TILED) tiled_pass<FORMAT>(a, true, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
+      if constexpr (TILED) tiled_pass<FORMAT>(a, sh, true, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
       ++pass;
       if (!grid_barrier(ctl)) return;
     } else {
@@ -1285,16 +1288,16 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
         if (i < prm.n) a.errors[i] = rate_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz);
       }
       if (!grid_barrier(ctl)) return;
-      tile_sums_from_errors(a, tstate[0], wsum, wcnt);
+      tile_sums_from_errors(a, sh, tstate[0], wsum, wcnt);
       if (!grid_barrier(ctl)) return;
     }
     // every wave folds the tile sums the same way
     uint32_t loaded = 0;
     double total;
     if (tiled)
-      total = fold_tiles_lds(a, tile_shape(ctl).count, lds, a.lds_cap * kTileBytes / 12u / 64u * 64u, loaded);
+      total = fold_tiles_lds(a, sh.count, lds, a.lds_cap * kTileBytes / 12u / 64u * 64u, loaded);
     else
-      total = fold_tiles(a, tile_shape(ctl).count, loaded);
+      total = fold_tiles(a, sh.count, loaded);
     double average = total / (double)(loaded > 1u ? loaded : 1u);
     if (blockIdx.x == 0 && t == 0) {
       uint32_t at = ctl->log_len;
@@ -1310,7 +1313,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     if (tiled) {
       if (blockIdx.x == 0 && t < kXcds)
         __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if constexpr (TILED) tiled_pass<FORMAT>(a, false, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
+      if constexpr (TILED) tiled_pass<FORMAT>(a, sh, false, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
       ++pass;
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
