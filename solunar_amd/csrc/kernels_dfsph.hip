@@ -34,7 +34,7 @@
 
 namespace sfl {
 
-constexpr uint32_t kChunk = 512;          // particles of one chunk of the stop sum
+constexpr uint32_t kChunk = 512;          // particles of one chunk of an untiled pass
 constexpr uint32_t kSolveThreads = 1024;  // threads of a solve block: two chunks at a time
 constexpr uint32_t kSolveWaves = kSolveThreads / 64;
 
