@@ -391,6 +391,8 @@ struct SolveArgs {
   uint32_t min_iterations;
   uint32_t max_iterations;
   uint32_t lds_cap;     // particles one block stages in LDS; 0: no tiles
+  const float4* Vin;    // tiled density solve: velocities before the viscosity pass, which it runs first
+  float* forces;        // that pass's accelerations, or null
 };
 
 // ---- one particle of a pass ------------------------------------------------
@@ -997,6 +999,132 @@ __device__ __forceinline__ uint32_t take_tile(Control* ctl, const TileShape& sh,
   }
 }
 
+// ---- the viscosity pass (dfsph.rs:90-99) -------------------------------------
+//
+// viscous (src/fluid/step.rs:229-251) for one fluid pair: only when the
+// particles approach, in the reference's operation order.
+__device__ __forceinline__ void viscous_pair(const StepParams& prm, float4 at, float4 vi, float3 pj, float4 vj,
+                                             float f, float& px, float& py, float& pz) {
+  float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+  float toward = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz);
+  if (toward < 0.0f) {
+    float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
+    float damp = -prm.nu * squeeze * 2.0f / (vi.w + vj.w);
+    float k = prm.mass * damp;
+    px = px - (ox * f) * k;
+    py = py - (oy * f) * k;
+    pz = pz - (oz * f) * k;
+  }
+}
+
+// The walls' part and the result: the walls' pull in list order, then
+// wall_viscous (step.rs:262-301: normalize_or(away, ZERO), motion into the
+// walls only), then v + (fluid + walls + gravity) * dt into VA (density
+// carried along) and `forces`.
+template <int FORMAT>
+__device__ __forceinline__ void viscous_finish(const StepParams& prm, uint32_t i, float4 at, float4 vi, float px,
+                                               float py, float pz, uint32_t nf, uint32_t nw, float4* VA,
+                                               float* forces) {
+  const float density = vi.w;
+  const uint32_t w0 = FORMAT == kLists ? nf : 0u;
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  if (nw) {
+    // both walks in list order, their loads run ahead (walk_walls)
+    float awx = 0.0f, awy = 0.0f, awz = 0.0f;
+    walk_walls(prm, i, w0, w0 + nw, [&](float f, float4 pw, bool live) {
+      if (live) {
+        awx = awx + -((at.x - pw.x) * f);
+        awy = awy + -((at.y - pw.y) * f);
+        awz = awz + -((at.z - pw.z) * f);
+      }
+    });
+    float len = sqrtf(dot3(awx, awy, awz, awx, awy, awz));
+    float rcp = 1.0f / len;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (isfinite(rcp) && rcp > 0.0f) {
+      nx = awx * rcp;
+      ny = awy * rcp;
+      nz = awz * rcp;
+    }
+    float tn = rmin(dot3(vi.x, vi.y, vi.z, nx, ny, nz), 0.0f);
+    float ix = nx * tn, iy = ny * tn, iz = nz * tn;
+    walk_walls(prm, i, w0, w0 + nw, [&](float f, float4 pw, bool live) {
+      float ox = at.x - pw.x, oy = at.y - pw.y, oz = at.z - pw.z;
+      float toward = dot3(ix, iy, iz, ox, oy, oz);
+      if (live && toward < 0.0f) {
+        float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
+        float damp = -prm.wall_nu * squeeze / density;
+        float k2 = prm.mass * damp;
+        wx = wx - (ox * f) * k2;
+        wy = wy - (oy * f) * k2;
+        wz = wz - (oz * f) * k2;
+      }
+    });
+  }
+  float ax = px + wx + prm.gx;
+  float ay = py + wy + prm.gy;
+  float az = pz + wz + prm.gz;
+  if (forces) {
+    forces[3 * i] = ax;
+    forces[3 * i + 1] = ay;
+    forces[3 * i + 2] = az;
+  }
+  VA[i] = make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density);
+}
+
+// The viscosity pass for particle i of a tile, inside the density solve's
+// launch: neighbours from the tile's LDS (A = (x, y, z, vx), C = (vy, vz,
+// density)) or from global memory; the same values as k_dfsph_viscous.
+template <int FORMAT>
+__device__ __forceinline__ void viscous_one(const SolveArgs& a, uint32_t i, const Lds* l, const float4* C,
+                                            bool staged) {
+  const StepParams& prm = a.prm;
+  const float4 at = a.P[i];
+  const float4 vi = a.Vin[i];
+  const uint32_t counts = prm.pair_counts[i];
+  const uint32_t nf = counts & 0xffffu, nw = counts >> 16;
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+  // batches of four entries, three in flight (as the rate pass)
+  Ent b0[4], b1[4], b2[4];
+  load_batch<FORMAT>(prm, i, 0, nf, b0);
+  load_batch<FORMAT>(prm, i, 4, nf, b1);
+  load_batch<FORMAT>(prm, i, 8, nf, b2);
+  auto sum = [&](const Ent(&e)[4], uint32_t k) {
+    float3 pj[4];
+    float4 vj[4];
+    uint32_t j[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool live = k + u < nf;
+      j[u] = live ? entry_index<FORMAT>(l, e[u].r) : i;
+      if (staged) {
+        const int32_t s = batch_slot<FORMAT>(*l, e[u], live);
+        const float4 q = l->A[s], c = C[s];
+        pj[u] = make_float3(q.x, q.y, q.z);
+        vj[u] = make_float4(q.w, c.x, c.y, c.z);
+      } else {
+        const float4 q = a.P[j[u]];
+        pj[u] = make_float3(q.x, q.y, q.z);
+        vj[u] = a.Vin[j[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k + u < nf && j[u] != i) viscous_pair(prm, at, vi, pj[u], vj[u], e[u].f, px, py, pz);
+  };
+  for (uint32_t k = 0; k < nf; k += 12) {
+    sum(b0, k);
+    load_batch<FORMAT>(prm, i, k + 12, nf, b0);
+    if (k + 4 >= nf) break;
+    sum(b1, k + 4);
+    load_batch<FORMAT>(prm, i, k + 16, nf, b1);
+    if (k + 8 >= nf) break;
+    sum(b2, k + 8);
+    load_batch<FORMAT>(prm, i, k + 20, nf, b2);
+  }
+  viscous_finish<FORMAT>(prm, i, at, vi, px, py, pz, nf, nw, a.V, a.forces);
+}
+
 // ---- the stop sum over solve tiles ------------------------------------------
 
 // Thread sums (value, count) of a wave folded by halving, h = 32..1; lane 0
@@ -1112,7 +1240,9 @@ __device__ void tile_sums_from_errors(const SolveArgs& a, const TileShape& sh, T
   }
 }
 
-// One pass over the tiles (rate = true: rate pass, else push pass), queued
+constexpr int kRatePass = 0, kPushPass = 1, kViscousPass = 2;
+
+// One pass over the tiles (rate, push or viscosity pass), queued
 // per XCD group with stealing (tile_next[parity]). Each round the block
 // takes a tile: stages its neighbourhood in LDS, then every particle of it
 // sums from LDS (a tile past the LDS gathers from global memory instead).
@@ -1124,18 +1254,21 @@ __device__ void tile_sums_from_errors(const SolveArgs& a, const TileShape& sh, T
 // errors of its particles, the waves fold theirs into `wsum`, and wave 0
 // folds those into the tile's sum at the start of the next round.
 template <int FORMAT>
-__device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, bool rate, uint32_t parity, float4* lds,
+__device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, uint32_t parity, float4* lds,
                            TileState (&tstate)[3], uint32_t (&grab_s)[2], const Glob& g, double* wsum,
                            uint32_t* wcnt) {
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-  const uint32_t cap = a.lds_cap;
+  const bool rate = mode == kRatePass, viscous = mode == kViscousPass;
   // the LDS: A (float4) then B (float2) for the rate pass, Q (float4) for
-  // the push pass
+  // the push pass, A then C (float4: vy, vz, density) for the viscosity
+  // pass, which stages 32 bytes a particle instead of 24
+  const uint32_t cap = viscous ? a.lds_cap * kTileBytes / 32u : a.lds_cap;
   float4* A = lds;
   float2* B = reinterpret_cast<float2*>(lds + cap);
   float4* Q = lds;
+  float4* C = lds + cap;
   uint32_t done = 0;
   // round 0: the fixed tile (state 2); round r > 0: the tile taken during
   // round r - 1 (state (r - 1) & 1)
@@ -1167,7 +1300,26 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, bool rate, u
         const bool h0 = s < n0, h1 = s < n1, h2 = sb < n0, h3 = sb < n1;
         const uint32_t j0 = h0 ? f0 + s : 0u, j1 = h1 ? f1 + s : 0u;
         const uint32_t j2 = h2 ? f0 + sb : 0u, j3 = h3 ? f1 + sb : 0u;
-        if (rate) {
+        if (viscous) {
+          const float4 p0 = a.P[j0], p1 = a.P[j1], p2 = a.P[j2], p3 = a.P[j3];
+          const float4 v0 = a.Vin[j0], v1 = a.Vin[j1], v2 = a.Vin[j2], v3 = a.Vin[j3];
+          if (h0) {
+            A[to0 + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
+            C[to0 + s] = make_float4(v0.y, v0.z, v0.w, 0.0f);
+          }
+          if (h1) {
+            A[to1 + s] = make_float4(p1.x, p1.y, p1.z, v1.x);
+            C[to1 + s] = make_float4(v1.y, v1.z, v1.w, 0.0f);
+          }
+          if (h2) {
+            A[to0 + sb] = make_float4(p2.x, p2.y, p2.z, v2.x);
+            C[to0 + sb] = make_float4(v2.y, v2.z, v2.w, 0.0f);
+          }
+          if (h3) {
+            A[to1 + sb] = make_float4(p3.x, p3.y, p3.z, v3.x);
+            C[to1 + sb] = make_float4(v3.y, v3.z, v3.w, 0.0f);
+          }
+        } else if (rate) {
           const float4 p0 = a.P[j0], p1 = a.P[j1], p2 = a.P[j2], p3 = a.P[j3];
           const float4 v0 = a.V[j0], v1 = a.V[j1], v2 = a.V[j2], v3 = a.V[j3];
           if (h0) {
@@ -1214,6 +1366,8 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, bool rate, u
         const float e = rate_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
         acc += (double)e;
         cnt += e > 0.0f ? 1u : 0u;
+      } else if (viscous) {
+        viscous_one<FORMAT>(a, i, &l, C, fits);
       } else {
         push_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
       }
@@ -1273,13 +1427,23 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     __syncthreads();
   }
   uint32_t it = 0, pass = 0;
+  if constexpr (TILED) {
+    if (tiled && a.Vin) {
+      // the density solve's launch runs the viscosity pass first
+      if (blockIdx.x == 0 && t < kXcds)
+        __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      tiled_pass<FORMAT>(a, sh, kViscousPass, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
+      ++pass;
+      if (!grid_barrier(ctl)) return;
+    }
+  }
   for (;;) {
     // rate pass: stiffness and error of every particle, and the tile sums
     if (tiled) {
       // the queues of the next pass start again from zero
       if (blockIdx.x == 0 && t < kXcds)
         __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if constexpr (TILED) tiled_pass<FORMAT>(a, sh, true, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
+      if constexpr (TILED) tiled_pass<FORMAT>(a, sh, kRatePass, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
       ++pass;
       if (!grid_barrier(ctl)) return;
     } else {
@@ -1313,7 +1477,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     if (tiled) {
       if (blockIdx.x == 0 && t < kXcds)
         __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if constexpr (TILED) tiled_pass<FORMAT>(a, sh, false, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
+      if constexpr (TILED) tiled_pass<FORMAT>(a, sh, kPushPass, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
       ++pass;
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
@@ -1335,10 +1499,11 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
   }
 }
 
-// The viscosity pass of dfsph_substep (dfsph.rs:90-99): viscous
-// (src/fluid/step.rs:229-251, with wall_viscous step.rs:262-301) + gravity,
-// v += push * dt, over the substep's lists. Reads VB, writes VA (density
-// carried along).
+// The viscosity pass of dfsph_substep (dfsph.rs:90-99) as a kernel of its
+// own, for the solves without tiles: viscous (src/fluid/step.rs:229-251, with
+// wall_viscous step.rs:262-301) + gravity, v += push * dt, over the
+// substep's lists. Reads VB, writes VA (density carried along). The tiled
+// density solve runs the same arithmetic first in its own launch.
 template <int FORMAT>
 __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                        StepParams prm, float4* __restrict__ VA,
@@ -1347,39 +1512,22 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
   if (list_aborted(ctl)) return;
   uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= prm.n) return;
-  const float4* __restrict__ W = prm.walls;
-  float4 at = PB[i];
-  float4 vi = VB[i];
-  float density = vi.w;
+  const float4 at = PB[i];
+  const float4 vi = VB[i];
+  const uint32_t counts = prm.pair_counts[i];
+  const uint32_t nf = counts & 0xffffu, nw = counts >> 16;
   float px = 0.0f, py = 0.0f, pz = 0.0f;
-  float awx = 0.0f, awy = 0.0f, awz = 0.0f;
   auto fluid = [&](uint32_t j, float f) {
-    float4 pj = PB[j];
-    float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
-    float4 vj = VB[j];
-    float toward = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz);
-    if (toward < 0.0f) {
-      float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
-      float damp = -prm.nu * squeeze * 2.0f / (density + vj.w);
-      float k = prm.mass * damp;
-      px = px - (ox * f) * k;
-      py = py - (oy * f) * k;
-      pz = pz - (oz * f) * k;
-    }
-  };
-  auto wall = [&](uint32_t w, float f) {
-    float4 pw = W[w];
-    awx = awx + -((at.x - pw.x) * f);
-    awy = awy + -((at.y - pw.y) * f);
-    awz = awz + -((at.z - pw.z) * f);
+    const float4 pj = PB[j];
+    viscous_pair(prm, at, vi, make_float3(pj.x, pj.y, pj.z), VB[j], f, px, py, pz);
   };
   if constexpr (FORMAT == kLists) {
-    visit_list(
-        prm, i,
-        [&](uint32_t j, float f) {
-          if (j != i) fluid(j, f);
-        },
-        wall);
+    const uint2* p = prm.pairs + i;
+    for (uint32_t k = 0; k < nf; ++k) {
+      uint2 e = p[(size_t)k * prm.pair_stride];
+      const uint32_t j = list_index(e.x);
+      if (j != i) fluid(j, __uint_as_float(e.y));
+    }
   } else if constexpr (FORMAT == kCompact) {
     // offsets count from the first particle of each neighbour column's
     // cells from z0 - 1 on, z0 the first cell of the particle's tile: those
@@ -1401,8 +1549,6 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
                                       (int64_t)cy + (int64_t)(code & 3u) - 1, za, lo, span);
       firsts[code][t] = first >= 0 ? prm.cell_start[first] : 0u;
     }
-    const uint32_t c = prm.pair_counts[i];
-    const uint32_t nf = c & 0xffffu, nw = c >> 16;
     for (uint32_t k = 0; k < nf; k += 4) {
       Ent e[4];
       fluid_entries<kCompact>(prm, i, k, e);
@@ -1416,25 +1562,9 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
         vj[u] = VB[j[u]];
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (k + u < nf && j[u] != i) {
-          float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
-          float toward = dot3(vi.x - vj[u].x, vi.y - vj[u].y, vi.z - vj[u].z, ox, oy, oz);
-          if (toward < 0.0f) {
-            float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
-            float damp = -prm.nu * squeeze * 2.0f / (density + vj[u].w);
-            float kk = prm.mass * damp;
-            const float f = e[u].f;
-            px = px - (ox * f) * kk;
-            py = py - (oy * f) * kk;
-            pz = pz - (oz * f) * kk;
-          }
-        }
-      }
-    }
-    for (uint32_t k = 0; k < nw; ++k) {
-      uint2 e = prm.pairs[(size_t)k * prm.pair_stride + i];
-      wall(e.x, __uint_as_float(e.y));
+      for (int u = 0; u < 4; ++u)
+        if (k + u < nf && j[u] != i)
+          viscous_pair(prm, at, vi, make_float3(pj[u].x, pj[u].y, pj[u].z), vj[u], e[u].f, px, py, pz);
     }
   } else {
     const int64_t szz = ctl->span[2], sysz = (int64_t)ctl->span[1] * szz;
@@ -1448,54 +1578,8 @@ __global__ __launch_bounds__(256) void k_dfsph_viscous(const float4* __restrict_
         }
       }
     });
-    uint32_t nw = prm.pair_counts[i] >> 16;
-    for (uint32_t k = 0; k < nw; ++k) {
-      uint2 e = prm.pairs[(size_t)k * prm.pair_stride + i];
-      wall(e.x, __uint_as_float(e.y));
-    }
   }
-  // wall_viscous: normalize_or(away, ZERO), motion into the walls only
-  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
-  uint32_t counts = prm.pair_counts[i];
-  uint32_t nf = counts & 0xffffu, nw = counts >> 16;
-  if (nw) {
-    float len = sqrtf(dot3(awx, awy, awz, awx, awy, awz));
-    float rcp = 1.0f / len;
-    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    if (isfinite(rcp) && rcp > 0.0f) {
-      nx = awx * rcp;
-      ny = awy * rcp;
-      nz = awz * rcp;
-    }
-    float tn = rmin(dot3(vi.x, vi.y, vi.z, nx, ny, nz), 0.0f);
-    float ix = nx * tn, iy = ny * tn, iz = nz * tn;
-    const uint2* p = prm.pairs + i;
-    const uint32_t w0 = FORMAT == kLists ? nf : 0u;
-    for (uint32_t k = w0; k < w0 + nw; ++k) {
-      uint2 e = p[(size_t)k * prm.pair_stride];
-      float f = __uint_as_float(e.y);
-      float4 pw = W[e.x];
-      float ox = at.x - pw.x, oy = at.y - pw.y, oz = at.z - pw.z;
-      float toward = dot3(ix, iy, iz, ox, oy, oz);
-      if (toward < 0.0f) {
-        float squeeze = toward / (dot3(ox, oy, oz, ox, oy, oz) + prm.soften);
-        float damp = -prm.wall_nu * squeeze / density;
-        float k2 = prm.mass * damp;
-        wx = wx - (ox * f) * k2;
-        wy = wy - (oy * f) * k2;
-        wz = wz - (oz * f) * k2;
-      }
-    }
-  }
-  float ax = px + wx + prm.gx;
-  float ay = py + wy + prm.gy;
-  float az = pz + wz + prm.gz;
-  if (forces) {
-    forces[3 * i] = ax;
-    forces[3 * i + 1] = ay;
-    forces[3 * i + 2] = az;
-  }
-  VA[i] = make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density);
+  viscous_finish<FORMAT>(prm, i, at, vi, px, py, pz, nf, nw, VA, forces);
 }
 
 // advance (src/fluid/step.rs:305-339): PB + VA*dt -> PA, VA. The velocity
@@ -1619,10 +1703,14 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
   return per * cus;
 }
 
+bool dfsph_solve_tiled(const StepParams& prm, uint32_t tile_bytes) {
+  return (prm.format == kLists || prm.format == kCompact) && tile_bytes > 0;
+}
+
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
                               float4* kappa, float* errors, double* tile_sum, uint32_t* tile_count,
                               uint32_t tile_cap, int which, float tolerance, uint32_t min_it, uint32_t max_it,
-                              int grid, uint32_t tile_bytes) {
+                              int grid, uint32_t tile_bytes, const float4* vin, float* forces) {
   if (prm.n == 0) return hipSuccess;
   SolveArgs a;
   a.prm = prm;
@@ -1641,7 +1729,11 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   a.max_iterations = max_it;
   // the compact format is read by the tiled kernel alone
   if (prm.format == kCompact && tile_bytes == 0) return hipErrorInvalidValue;
-  const bool tiled = (prm.format == kLists || prm.format == kCompact) && tile_bytes > 0;
+  const bool tiled = dfsph_solve_tiled(prm, tile_bytes);
+  // the viscosity pass runs inside the tiled density solve only
+  if (vin && !(tiled && which == 1)) return hipErrorInvalidValue;
+  a.Vin = vin;
+  a.forces = forces;
   a.lds_cap = tiled ? tile_bytes / kTileBytes : 0u;
   // a multiple of 8 blocks so every XCD gets the same share; at least one
   // chunk per XCD group or the small grid of a small world

@@ -42,7 +42,10 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes);
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
                               float4* kappa, float* errors, double* tile_sum, uint32_t* tile_count,
                               uint32_t tile_cap, int which, float tolerance, uint32_t min_it, uint32_t max_it,
-                              int grid, uint32_t tile_bytes);
+                              int grid, uint32_t tile_bytes, const float4* vin = nullptr, float* forces = nullptr);
+// Whether the solve runs on LDS tiles (then its density solve also runs the
+// viscosity pass, given the velocities before it).
+bool dfsph_solve_tiled(const StepParams& prm, uint32_t tile_bytes);
 void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,
                           float* forces);
 void launch_dfsph_advance(hipStream_t s, const float4* PB, const StepParams& prm, float4* PA, float4* VA,

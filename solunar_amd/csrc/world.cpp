@@ -785,13 +785,18 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
                              solve_grid_, solve_lds_),
           "cooperative launch (divergence solve)");
   phase_end(SFL_PHASE_SOLVE);
-  phase_begin(SFL_PHASE_VISCOUS);
-  launch_dfsph_viscous(stream_, pb_.ptr, vb_.ptr, prm, va_.ptr, keep_forces_ ? forces_.ptr : nullptr);
-  phase_end(SFL_PHASE_VISCOUS);
+  // the tiled density solve runs the viscosity pass (VB -> VA) itself
+  const bool fused = dfsph_solve_tiled(prm, solve_lds_);
+  if (!fused) {
+    phase_begin(SFL_PHASE_VISCOUS);
+    launch_dfsph_viscous(stream_, pb_.ptr, vb_.ptr, prm, va_.ptr, keep_forces_ ? forces_.ptr : nullptr);
+    phase_end(SFL_PHASE_VISCOUS);
+  }
   phase_begin(SFL_PHASE_SOLVE);
   SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, va_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, stop_sum_.ptr,
                              stop_count_.ptr, stop_cap(), 1, s.density_tolerance, s.min_iterations, s.max_iterations,
-                             solve_grid_, solve_lds_),
+                             solve_grid_, solve_lds_, fused ? vb_.ptr : nullptr,
+                             fused && keep_forces_ ? forces_.ptr : nullptr),
           "cooperative launch (density solve)");
   phase_end(SFL_PHASE_SOLVE);
   phase_begin(SFL_PHASE_ADVANCE);
