@@ -42,6 +42,7 @@ DT = 5e-4
 RATE_BYTES = 36
 PUSH_BYTES = 40
 SUBSTEP_FIXED_BYTES = 292
+VISCOUS_BYTES = 40  # viscosity and gravity: x, v, rho 28 read, v 12 written
 
 
 def log(msg: str) -> None:
@@ -328,6 +329,10 @@ def main() -> int:
     dom_ms, dom_launches = phases[dom]
     if dom == "solve":
         dom_bytes = RATE_BYTES * work["particle_rates"] + PUSH_BYTES * work["particle_pushes"]
+        # the tiled density solve runs the viscosity pass in its own launch
+        # (no viscous launches of their own then)
+        if phases.get("viscous", (0.0, 0))[1] == 0:
+            dom_bytes += VISCOUS_BYTES * work["particle_substeps"]
     else:
         per = {"bounds": 12, "sort": 40, "reorder": 64, "density": 24, "viscous": 52, "advance": 48, "force": 52}
         dom_bytes = per.get(dom, 0) * work["particle_substeps"]
