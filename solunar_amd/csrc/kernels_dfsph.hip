@@ -169,7 +169,7 @@ __device__ __forceinline__ void store_group(const StepParams& prm, uint32_t i, u
 // world; later launches of the substep then do nothing and the host regrows
 // or widens.
 template <int FORMAT>
-__global__ __launch_bounds__(256) void k_dfsph_density(const float4* __restrict__ P, float4* __restrict__ V,
+__global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restrict__ P, float4* __restrict__ V,
                                                        float* __restrict__ alpha, StepParams prm) {
   Control* ctl = prm.ctl;
   if (ctl->abort) return;
