@@ -391,7 +391,7 @@ __global__ __launch_bounds__(BS) void k_b_density(BatchArgs b, const float4* __r
 // The fused force, velocity and advance pass of the single world
 // (k_wcsph_force) for the scenes that step in this slot.
 template <uint32_t BS>
-__global__ __launch_bounds__(BS) void k_b_force(BatchArgs b, const float4* __restrict__ PB,
+__global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __restrict__ PB,
                                                 const float4* __restrict__ VB, const float* __restrict__ scaled,
                                                 float4* __restrict__ PA, float4* __restrict__ VA,
                                                 const uint2* __restrict__ map) {
