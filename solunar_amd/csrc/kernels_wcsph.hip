@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
 // (step.rs:305-339), for every particle. Reads the sorted state PB/VB, writes
 // PA/VA; `forces` (optional) keeps the push, `clamps` (optional) the
 // clamped components of the advance (bits x, y, z).
-__global__ __launch_bounds__(256, 8) void k_wcsph_force(const float4* __restrict__ PB, const float4* __restrict__ VB,
+__global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                      const float* __restrict__ scaled, StepParams prm,
                                                      float4* __restrict__ PA, float4* __restrict__ VA,
                                                      float* __restrict__ forces, uint8_t* __restrict__ clamps) {
