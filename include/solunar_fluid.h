@@ -132,11 +132,30 @@ int32_t sfl_fill_box(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
 int32_t sfl_fill_box_jittered(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
                               float jitter, uint32_t seed, size_t* count_out);
 
-/* FluidWorld::set_domain (src/fluid/mod.rs:433-444). A non-null box is
- * validated, then refused with SFL_UNSUPPORTED: domain culling is not on the
- * device path. Null clears it. */
+/* FluidWorld::set_domain (src/fluid/mod.rs:433-444): water outside the box
+ * (inclusive bounds) is taken out after each step, after the flows, and
+ * counted in sfl_report.lost (keep_in_domain, mod.rs:451-465). Null clears
+ * it. */
 int32_t sfl_set_domain(sfl_world* world, const sfl_vec3* low,
                        const sfl_vec3* high);
+
+/* FluidWorld::add_emitter / add_drain (src/fluid/flow.rs:92-111). After each
+ * step an emitter fills every lattice slot of its box (spacing apart, half a
+ * spacing in from its low corner) that no water is nearer to than 0.9
+ * spacing, the new water moving at `velocity`; a drain takes out the water
+ * in its box. Drains run first, then emitters, each in the order added
+ * (run_flows, flow.rs:144-223), and the report counts them in emitted and
+ * drained. A full world stops an emitter with SFL_FLUID_FULL after the
+ * particles that fit. An emitter box of more than 2^28 slots is refused at
+ * the step with SFL_OUT_OF_RANGE. */
+int32_t sfl_add_emitter(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
+                        sfl_vec3 velocity, uint32_t* flow_out);
+int32_t sfl_add_drain(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
+                      uint32_t* flow_out);
+
+/* FluidWorld::remove_flow (src/fluid/flow.rs:113-118): *removed_out is 0
+ * for an id that is not there. */
+int32_t sfl_remove_flow(sfl_world* world, uint32_t flow, int32_t* removed_out);
 
 /* add_box_container / add_wall (src/fluid/boundary.rs:41-91). */
 int32_t sfl_add_box_container(sfl_world* world, sfl_vec3 low, sfl_vec3 high,

@@ -363,6 +363,10 @@ struct Report {
 //                 Count of errors > 0 is exact.
 enum ResidualMode : int32_t { kReferenceF32 = 0, kFixedTreeF64 = 1 };
 
+// An emitter's lattice slots, at most (the reference sizes its slot flags
+// by the box, up to 4096 per axis; past this a box is refused).
+constexpr int64_t kEmitterSlotsMax = int64_t(1) << 28;
+
 // The device's FP64 stop sum (kernels_dfsph.hip, fold_tiles and the tile
 // sums of its rate pass), over the particles' cells in particle order (sorted
 // by cell, then id). The cell box is the fluid's: span s = max - min + 1 per
@@ -514,6 +518,169 @@ class World {
     neighbors_built = false;
     if (id_out) *id_out = id;
     return kOk;
+  }
+
+  // ---- flows and the domain (src/fluid/flow.rs, src/fluid/mod.rs:428-486) ----
+
+  struct Flow {
+    uint32_t id;
+    Vec3<float> low, high;
+    bool drain;
+    Vec3<float> velocity;  // an emitter's
+  };
+  std::vector<Flow> flow_list;
+  uint32_t flow_next = 0;
+  bool has_domain = false;
+  Vec3<float> domain_low{0, 0, 0}, domain_high{0, 0, 0};
+
+  // set_domain (src/fluid/mod.rs:433-444); null bounds clear it
+  int32_t set_domain(const Vec3<float>* low, const Vec3<float>* high) {
+    if (low && high) {
+      if (!low->finite() || !high->finite()) {
+        detail = "fluid domain";
+        return kNonFinite;
+      }
+      if (high->x <= low->x || high->y <= low->y || high->z <= low->z) return kOutOfRange;
+      has_domain = true;
+      domain_low = *low;
+      domain_high = *high;
+    } else {
+      has_domain = false;
+    }
+    return kOk;
+  }
+
+  // add_emitter / add_drain / add_flow (src/fluid/flow.rs:92-139)
+  int32_t add_flow(Vec3<float> low, Vec3<float> high, bool drain, Vec3<float> velocity, uint32_t* id_out) {
+    if (!drain && !velocity.finite()) {
+      detail = "emitter velocity";
+      return kNonFinite;
+    }
+    if (!low.finite() || !high.finite()) {
+      detail = "flow box";
+      return kNonFinite;
+    }
+    if (high.x <= low.x || high.y <= low.y || high.z <= low.z) return kOutOfRange;
+    if (flow_next == UINT32_MAX) return kFluidFull;
+    uint32_t id = flow_next++;
+    flow_list.push_back({id, low, high, drain, velocity});
+    if (id_out) *id_out = id;
+    return kOk;
+  }
+
+  // remove_flow (src/fluid/flow.rs:113-118)
+  bool remove_flow(uint32_t id) {
+    size_t before = flow_list.size();
+    flow_list.erase(std::remove_if(flow_list.begin(), flow_list.end(), [&](const Flow& f) { return f.id == id; }),
+                    flow_list.end());
+    return flow_list.size() != before;
+  }
+
+  // keep_only (src/fluid/mod.rs:468-489): the kept particles stay in order
+  void keep_only(const std::vector<char>& keep) {
+    size_t w = 0;
+    for (size_t i = 0; i < len(); ++i) {
+      if (!keep[i]) continue;
+      ids[w] = ids[i];
+      positions[w] = positions[i];
+      velocities[w] = velocities[i];
+      densities[w] = densities[i];
+      ++w;
+    }
+    ids.resize(w);
+    positions.resize(w);
+    velocities.resize(w);
+    densities.resize(w);
+    neighbors_built = false;
+  }
+
+  // run_flows (src/fluid/flow.rs:144-223): drains, then emitters, each in
+  // the order they were added. A full world stops an emitter with an error,
+  // the particles it added before staying.
+  int32_t run_flows(uint32_t& emitted, uint32_t& drained) {
+    emitted = drained = 0;
+    if (flow_list.empty()) return kOk;
+    const std::vector<Flow> list = flow_list;
+    for (const Flow& f : list) {
+      if (!f.drain) continue;
+      std::vector<char> keep(len());
+      size_t gone = 0;
+      for (size_t i = 0; i < len(); ++i) {
+        const V& at = positions[i];
+        bool inside = at.x >= R(f.low.x) && at.y >= R(f.low.y) && at.z >= R(f.low.z) && at.x <= R(f.high.x) &&
+                      at.y <= R(f.high.y) && at.z <= R(f.high.z);
+        keep[i] = !inside;
+        gone += inside;
+      }
+      if (gone > 0) {
+        drained += (uint32_t)gone;
+        keep_only(keep);
+      }
+    }
+    const R spacing = R(settings.spacing);
+    for (const Flow& f : list) {
+      if (f.drain) continue;
+      const V low = cvt(f.low), size = cvt(f.high) - cvt(f.low);
+      auto count = [&](R extent) -> int64_t {
+        R c = std::floor(extent / spacing);
+        int64_t v = std::isnan(c) ? 0 : (c >= R(9.2e18) ? INT64_MAX : (c <= R(-9.2e18) ? INT64_MIN : (int64_t)c));
+        return std::min<int64_t>(std::max<int64_t>(v, 0), 1 << 12);
+      };
+      const int64_t cx = count(size.x), cy = count(size.y), cz = count(size.z);
+      const int64_t total = cx * cy * cz;
+      if (total == 0) continue;
+      if (total > (int64_t)kEmitterSlotsMax) {
+        detail = "emitter box";
+        return kOutOfRange;
+      }
+      // CLEAR_SHARE = 0.9 (flow.rs:8); powi(2) as one product
+      const R near = R(0.9f) * spacing;
+      const R clear = near * near;
+      std::vector<char> taken((size_t)total, 0);
+      for (const V& at : positions) {
+        const V local = (at - low) / spacing - V{R(0.5), R(0.5), R(0.5)};
+        auto nearest = [](R value) -> int64_t {
+          R r = std::round(value);
+          return std::isnan(r) ? 0 : (r >= R(9.2e18) ? INT64_MAX : (r <= R(-9.2e18) ? INT64_MIN : (int64_t)r));
+        };
+        const int64_t px = nearest(local.x), py = nearest(local.y), pz = nearest(local.z);
+        // a slot index this far out has no slot within one of it (and the
+        // reference's +-1 around a saturated value marks nothing either)
+        if (px < -1 || px > cx || py < -1 || py > cy || pz < -1 || pz > cz) continue;
+        for (int64_t y = py - 1; y <= py + 1; ++y)
+          for (int64_t z = pz - 1; z <= pz + 1; ++z)
+            for (int64_t x = px - 1; x <= px + 1; ++x) {
+              if (!(x >= 0 && x < cx && y >= 0 && y < cy && z >= 0 && z < cz)) continue;
+              const V center = low + V{R(x) + R(0.5), R(y) + R(0.5), R(z) + R(0.5)} * spacing;
+              if (length_squared(at - center) < clear) taken[(size_t)((y * cz + z) * cx + x)] = 1;
+            }
+      }
+      for (int64_t y = 0; y < cy; ++y)
+        for (int64_t z = 0; z < cz; ++z)
+          for (int64_t x = 0; x < cx; ++x) {
+            if (taken[(size_t)((y * cz + z) * cx + x)]) continue;
+            const V at = low + V{R(x) + R(0.5), R(y) + R(0.5), R(z) + R(0.5)} * spacing;
+            int32_t st = add_particle(at, cvt(f.velocity), nullptr);
+            if (st != kOk) return st;
+            ++emitted;
+          }
+    }
+    return kOk;
+  }
+
+  // keep_in_domain (src/fluid/mod.rs:451-465): water whose clamp to the
+  // domain moves it is taken out
+  uint32_t keep_in_domain() {
+    if (!has_domain) return 0;
+    const V lo = cvt(domain_low), hi = cvt(domain_high);
+    std::vector<char> keep(len());
+    uint32_t lost = 0;
+    for (size_t i = 0; i < len(); ++i) {
+      keep[i] = clamp(positions[i], lo, hi) == positions[i];
+      lost += !keep[i];
+    }
+    if (lost > 0) keep_only(keep);
+    return lost;
   }
 
   // fill_box (src/fluid/mod.rs:493-524); jitter > 0 adds the seeded
@@ -706,7 +873,7 @@ class World {
     return most;
   }
 
-  // step (src/fluid/step.rs:58-90). Flows and the domain are not carried.
+  // step (src/fluid/step.rs:58-90)
   int32_t step(float dt, Report& report) {
     report = Report{};
     if (!std::isfinite(dt) || dt < 0.0f) {
@@ -718,6 +885,12 @@ class World {
       int32_t st = settings.solver == kWcsph ? wcsph(dt, report) : dfsph(dt, report);
       if (st != kOk) return st;
     }
+    uint32_t emitted = 0, drained = 0;
+    int32_t flowed = run_flows(emitted, drained);
+    if (flowed != kOk) return flowed;
+    report.emitted = emitted;
+    report.drained = drained;
+    report.lost = keep_in_domain();
     report.max_speed = (float)fastest();
     R rest = R(settings.rest_density);
     R most = R(0);

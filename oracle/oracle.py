@@ -67,6 +67,11 @@ def library() -> ctypes.CDLL:
         "orc_add_wall": (i32, box + [u32p]),
         "orc_add_boundary_points": (i32, [P, sz, fp, u32p]),
         "orc_remove_boundary": (i32, [P, ctypes.c_uint32, ctypes.POINTER(i32)]),
+        "orc_set_domain": (i32, [P, fp, fp]),
+        "orc_add_emitter": (i32, [P, fp, fp, fp, u32p]),
+        "orc_add_drain": (i32, [P, fp, fp, u32p]),
+        "orc_remove_flow": (i32, [P, ctypes.c_uint32]),
+        "orc_run_flows": (i32, [P, u32p, u32p]),
         "orc_wall_len": (sz, [P]),
         "orc_read_wall_positions": (i32, [P, sz, fp]),
         "orc_step": (i32, [P, f, ctypes.POINTER(CReport)]),
@@ -99,6 +104,10 @@ def library() -> ctypes.CDLL:
 
 def _f32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1, 3))
+
+
+def _v3(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(3))
 
 
 def _p(a, t):
@@ -204,6 +213,36 @@ class OracleWorld:
         o = ctypes.c_uint32()
         self._check(self._lib.orc_add_wall(self._h, *map(float, low), *map(float, high), ctypes.byref(o)))
         return o.value
+
+    def set_domain(self, domain) -> None:
+        """Water outside domain = (low, high) is taken out after each step; None clears it."""
+        if domain is None:
+            self._check(self._lib.orc_set_domain(self._h, None, None))
+            return
+        lo, hi = _v3(domain[0]), _v3(domain[1])
+        self._check(self._lib.orc_set_domain(self._h, _p(lo, ctypes.c_float), _p(hi, ctypes.c_float)))
+
+    def add_emitter(self, low, high, velocity) -> int:
+        lo, hi, v = _v3(low), _v3(high), _v3(velocity)
+        o = ctypes.c_uint32()
+        self._check(self._lib.orc_add_emitter(self._h, _p(lo, ctypes.c_float), _p(hi, ctypes.c_float),
+                                              _p(v, ctypes.c_float), ctypes.byref(o)))
+        return o.value
+
+    def add_drain(self, low, high) -> int:
+        lo, hi = _v3(low), _v3(high)
+        o = ctypes.c_uint32()
+        self._check(self._lib.orc_add_drain(self._h, _p(lo, ctypes.c_float), _p(hi, ctypes.c_float), ctypes.byref(o)))
+        return o.value
+
+    def run_flows(self):
+        """Drains then emitters once, without a step: (emitted, drained)."""
+        e, d = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self._lib.orc_run_flows(self._h, ctypes.byref(e), ctypes.byref(d)))
+        return e.value, d.value
+
+    def remove_flow(self, flow_id: int) -> bool:
+        return bool(self._lib.orc_remove_flow(self._h, flow_id))
 
     def add_boundary_points(self, points) -> int:
         p = _f32(points)

@@ -237,6 +237,38 @@ int32_t orc_add_wall(void* w, float lx, float ly, float lz, float hx, float hy, 
   return with(static_cast<Handle*>(w), [&](auto& x) { return x.add_wall({lx, ly, lz}, {hx, hy, hz}, out); });
 }
 
+// FluidWorld::set_domain (src/fluid/mod.rs:433-444); a null corner clears it
+int32_t orc_set_domain(void* w, const float* low, const float* high) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    if (!low || !high) return x.set_domain(nullptr, nullptr);
+    Vec3<float> lo{low[0], low[1], low[2]}, hi{high[0], high[1], high[2]};
+    return x.set_domain(&lo, &hi);
+  });
+}
+
+// add_emitter / add_drain / remove_flow (src/fluid/flow.rs:92-118)
+int32_t orc_add_emitter(void* w, const float* low, const float* high, const float* velocity, uint32_t* out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    return x.add_flow({low[0], low[1], low[2]}, {high[0], high[1], high[2]}, false,
+                      {velocity[0], velocity[1], velocity[2]}, out);
+  });
+}
+
+int32_t orc_add_drain(void* w, const float* low, const float* high, uint32_t* out) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    return x.add_flow({low[0], low[1], low[2]}, {high[0], high[1], high[2]}, true, {0.0f, 0.0f, 0.0f}, out);
+  });
+}
+
+// run_flows alone (the reference's tests call it without a step)
+int32_t orc_run_flows(void* w, uint32_t* emitted, uint32_t* drained) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.run_flows(*emitted, *drained); });
+}
+
+int32_t orc_remove_flow(void* w, uint32_t id) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return (int32_t)x.remove_flow(id); });
+}
+
 int32_t orc_add_boundary_points(void* w, size_t n, const float* xyz, uint32_t* out) {
   std::vector<Vec3<float>> pts(n);
   for (size_t i = 0; i < n; ++i) pts[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};

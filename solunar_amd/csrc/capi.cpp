@@ -126,6 +126,18 @@ int32_t sfl_set_domain(sfl_world* world, const sfl_vec3* low, const sfl_vec3* hi
   return guarded(world, [&](sfl::World& w) { return w.set_domain(low, high); });
 }
 
+int32_t sfl_add_emitter(sfl_world* world, sfl_vec3 low, sfl_vec3 high, sfl_vec3 velocity, uint32_t* flow_out) {
+  return guarded(world, [&](sfl::World& w) { return w.add_emitter(low, high, velocity, flow_out); });
+}
+
+int32_t sfl_add_drain(sfl_world* world, sfl_vec3 low, sfl_vec3 high, uint32_t* flow_out) {
+  return guarded(world, [&](sfl::World& w) { return w.add_drain(low, high, flow_out); });
+}
+
+int32_t sfl_remove_flow(sfl_world* world, uint32_t flow, int32_t* removed_out) {
+  return guarded(world, [&](sfl::World& w) { return w.remove_flow(flow, removed_out); });
+}
+
 int32_t sfl_add_box_container(sfl_world* world, sfl_vec3 low, sfl_vec3 high, uint32_t* boundary_out) {
   return guarded(world, [&](sfl::World& w) { return w.add_box_container(low, high, boundary_out); });
 }

@@ -28,6 +28,22 @@ constexpr uint32_t kFlagPairs = 8u;     // a particle had more neighbours than t
 constexpr uint32_t kFlagWide = 32u;     // a neighbour sat past the width of its cell's mask
 constexpr uint32_t kFlagTiles = 64u;    // the solve tiles outnumbered the stop sum's room
 
+// A drain's or the domain's box, and an emitter's lattice (flows after a
+// step, kernels_flow.hip).
+struct FlowBox {
+  float lo[3], hi[3];
+};
+struct EmitBox {
+  float lo[3];
+  float spacing;
+  int64_t cells[3];  // lattice slots per axis (x, y, z)
+  float clear;       // (0.9 spacing)^2
+  float vel[3];
+};
+// An emitter's lattice slots, at most (the reference sizes its slot flags by
+// the box, up to 4096 per axis; past this a box is refused).
+constexpr int64_t kEmitterSlotsMax = int64_t(1) << 28;
+
 constexpr int kResidualLogCap = 256;
 constexpr int kStripes = 64;
 

@@ -36,6 +36,15 @@ void launch_wcsph_force(hipStream_t s, const float4* PB, const float4* VB, const
 void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
                          const StepParams& prm, float4* PA, float4* VA);
 
+// Flows and the domain after a step (kernels_flow.hip)
+void launch_flow_keep(hipStream_t s, const float4* P, uint32_t n, const FlowBox& box, int domain, uint32_t* keep);
+void launch_flow_gather(hipStream_t s, const float4* PA, const float4* VA, uint32_t n, const uint32_t* keep,
+                        const uint32_t* start, float4* PB, float4* VB);
+void launch_emit_mark(hipStream_t s, const float4* P, uint32_t n, const EmitBox& e, uint32_t* free, uint64_t slots);
+void launch_emit_append(hipStream_t s, const uint32_t* free, const uint32_t* start, uint64_t slots, const EmitBox& e,
+                        uint32_t n, uint32_t limit, uint32_t first_id, float4* P, float4* V);
+void launch_flow_totals(hipStream_t s, const float4* V, uint32_t n, float rest, Control* ctl);
+
 // DFSPH (kernels_dfsph.hip)
 hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
 int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes);

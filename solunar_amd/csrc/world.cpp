@@ -295,15 +295,167 @@ int32_t World::fill_box(sfl_vec3 low, sfl_vec3 high, float jitter, uint32_t seed
 // unsupported on the device path.
 int32_t World::set_domain(const sfl_vec3* low, const sfl_vec3* high) {
   ++version_;
-  if (!low && !high) return SFL_OK;
+  if (!low && !high) {
+    has_domain_ = false;
+    return SFL_OK;
+  }
   if (!low || !high) return SFL_OUT_OF_RANGE;
   if (!finite_vec(*low) || !finite_vec(*high)) {
     detail_ = "fluid domain";
     return SFL_NON_FINITE;
   }
   if (high->x <= low->x || high->y <= low->y || high->z <= low->z) return SFL_OUT_OF_RANGE;
-  detail_ = "domain culling is not on the device path";
-  return SFL_UNSUPPORTED;
+  has_domain_ = true;
+  domain_ = FlowBox{{low->x, low->y, low->z}, {high->x, high->y, high->z}};
+  return SFL_OK;
+}
+
+// ---- flows (src/fluid/flow.rs) ----------------------------------------------
+
+// add_emitter / add_drain / add_flow (flow.rs:92-139)
+int32_t World::add_flow(sfl_vec3 low, sfl_vec3 high, bool drain, sfl_vec3 velocity, uint32_t* id_out) {
+  if (!drain && !finite_vec(velocity)) {
+    detail_ = "emitter velocity";
+    return SFL_NON_FINITE;
+  }
+  if (!finite_vec(low) || !finite_vec(high)) {
+    detail_ = "flow box";
+    return SFL_NON_FINITE;
+  }
+  if (high.x <= low.x || high.y <= low.y || high.z <= low.z) return SFL_OUT_OF_RANGE;
+  if (flow_next_ == 0xffffffffu) return SFL_FLUID_FULL;
+  const uint32_t id = flow_next_++;
+  flows_.push_back(Flow{id, low, high, drain, velocity});
+  if (id_out) *id_out = id;
+  return SFL_OK;
+}
+
+int32_t World::add_emitter(sfl_vec3 low, sfl_vec3 high, sfl_vec3 velocity, uint32_t* id) {
+  return add_flow(low, high, false, velocity, id);
+}
+
+int32_t World::add_drain(sfl_vec3 low, sfl_vec3 high, uint32_t* id) {
+  return add_flow(low, high, true, sfl_vec3{0.0f, 0.0f, 0.0f}, id);
+}
+
+// remove_flow (flow.rs:113-118)
+int32_t World::remove_flow(uint32_t id, int32_t* removed) {
+  const size_t before = flows_.size();
+  flows_.erase(std::remove_if(flows_.begin(), flows_.end(), [&](const Flow& f) { return f.id == id; }),
+               flows_.end());
+  if (removed) *removed = flows_.size() != before ? 1 : 0;
+  return SFL_OK;
+}
+
+int32_t World::read_count(const uint32_t* dev, uint32_t* out) {
+  SFL_HIP(hipMemcpyAsync(out, dev, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "read count");
+  SFL_HIP(hipStreamSynchronize(stream_), "flows");
+  return SFL_OK;
+}
+
+// keep_only (src/fluid/mod.rs:468-489) of the particles outside a drain box
+// or inside the domain: flags, their exclusive scan, the kept particles
+// gathered in order to B and copied back to A.
+int32_t World::keep_only(const FlowBox& box, bool domain, uint32_t& gone) {
+  gone = 0;
+  if (n_ == 0) return SFL_OK;
+  const uint32_t n = (uint32_t)n_;
+  SFL_HIP(flow_flags_.reserve(n_ + 1, 0, stream_), "hipMalloc(flow flags)");
+  SFL_HIP(flow_start_.reserve(n_ + 1, 0, stream_), "hipMalloc(flow starts)");
+  SFL_HIP(flow_tiles_.reserve((n_ + 2047) / 2048 + 1, 0, stream_), "hipMalloc(flow tiles)");
+  SFL_HIP(flow_total_.reserve(1, 0, stream_), "hipMalloc(flow total)");
+  launch_flow_keep(stream_, pa_.ptr, n, box, domain ? 1 : 0, flow_flags_.ptr);
+  launch_scan(stream_, flow_flags_.ptr, flow_start_.ptr, flow_tiles_.ptr, flow_total_.ptr, nullptr, n, 0, nullptr);
+  uint32_t kept = 0;
+  int32_t st = read_count(flow_total_.ptr, &kept);
+  if (st != SFL_OK) return st;
+  gone = n - kept;
+  if (gone == 0) return SFL_OK;
+  launch_flow_gather(stream_, pa_.ptr, va_.ptr, n, flow_flags_.ptr, flow_start_.ptr, pb_.ptr, vb_.ptr);
+  SFL_HIP(hipMemcpyAsync(pa_.ptr, pb_.ptr, (size_t)kept * sizeof(float4), hipMemcpyDeviceToDevice, stream_),
+          "keep positions");
+  SFL_HIP(hipMemcpyAsync(va_.ptr, vb_.ptr, (size_t)kept * sizeof(float4), hipMemcpyDeviceToDevice, stream_),
+          "keep velocities");
+  SFL_HIP(hipGetLastError(), "flow kernels");
+  n_ = kept;
+  invalidate_lists();
+  forces_valid_ = false;
+  return SFL_OK;
+}
+
+// One emitter (flow.rs:170-220): the lattice slots of its box with no water
+// nearer than 0.9 spacing, in y, z, x order, become particles moving at its
+// velocity. A full world stops it after the particles that fit.
+int32_t World::emit(const sfl_vec3& low, const sfl_vec3& high, const sfl_vec3& velocity, uint32_t& added) {
+  added = 0;
+  const float spacing = settings_.spacing;
+  auto count = [&](float extent) -> int64_t {
+    const float c = std::floor(extent / spacing);
+    int64_t v = std::isnan(c) ? 0 : (c >= 9.2e18f ? INT64_MAX : (c <= -9.2e18f ? INT64_MIN : (int64_t)c));
+    return std::min<int64_t>(std::max<int64_t>(v, 0), 1 << 12);
+  };
+  const int64_t cx = count(high.x - low.x), cy = count(high.y - low.y), cz = count(high.z - low.z);
+  const int64_t total = cx * cy * cz;
+  if (total == 0) return SFL_OK;
+  if (total > kEmitterSlotsMax) {
+    detail_ = "emitter box";
+    return SFL_OUT_OF_RANGE;
+  }
+  const float near = 0.9f * spacing;  // CLEAR_SHARE (flow.rs:8), squared as one product
+  EmitBox e{{low.x, low.y, low.z}, spacing, {cx, cy, cz}, near * near, {velocity.x, velocity.y, velocity.z}};
+  const size_t slots = (size_t)total;
+  SFL_HIP(flow_flags_.reserve(slots + 1, 0, stream_), "hipMalloc(emitter slots)");
+  SFL_HIP(flow_start_.reserve(slots + 1, 0, stream_), "hipMalloc(emitter ranks)");
+  SFL_HIP(flow_tiles_.reserve((slots + 2047) / 2048 + 1, 0, stream_), "hipMalloc(flow tiles)");
+  SFL_HIP(flow_total_.reserve(1, 0, stream_), "hipMalloc(flow total)");
+  launch_emit_mark(stream_, pa_.ptr, (uint32_t)n_, e, flow_flags_.ptr, slots);
+  launch_scan(stream_, flow_flags_.ptr, flow_start_.ptr, flow_tiles_.ptr, flow_total_.ptr, nullptr, slots, 0,
+              nullptr);
+  uint32_t open = 0;
+  int32_t st = read_count(flow_total_.ptr, &open);
+  if (st != SFL_OK) return st;
+  // add_particle's room (src/fluid/mod.rs:416): below max_particles, ids below u32::MAX
+  const uint64_t room = n_ < settings_.max_particles ? settings_.max_particles - n_ : 0;
+  const uint64_t ids = 0xffffffffull - next_id_;
+  const uint32_t limit = (uint32_t)std::min<uint64_t>({room, ids, (uint64_t)open});
+  if (limit > 0) {
+    st = ensure_particles(n_ + limit);
+    if (st != SFL_OK) return st;
+    launch_emit_append(stream_, flow_flags_.ptr, flow_start_.ptr, slots, e, (uint32_t)n_, limit, next_id_, pa_.ptr,
+                       va_.ptr);
+    SFL_HIP(hipGetLastError(), "emitter kernels");
+    n_ += limit;
+    next_id_ += limit;
+    added = limit;
+    invalidate_lists();
+    forces_valid_ = false;
+  }
+  return open > limit ? SFL_FLUID_FULL : SFL_OK;
+}
+
+// run_flows (flow.rs:144-223): drains, then emitters, each in the order
+// they were added.
+int32_t World::run_flows(uint32_t& emitted, uint32_t& drained, bool& changed) {
+  emitted = drained = 0;
+  if (flows_.empty()) return SFL_OK;
+  const std::vector<Flow> list = flows_;
+  for (const Flow& f : list) {
+    if (!f.drain) continue;
+    uint32_t gone = 0;
+    int32_t st = keep_only(FlowBox{{f.low.x, f.low.y, f.low.z}, {f.high.x, f.high.y, f.high.z}}, false, gone);
+    if (st != SFL_OK) return st;
+    drained += gone;
+    changed = changed || gone > 0;
+  }
+  for (const Flow& f : list) {
+    if (f.drain) continue;
+    uint32_t added = 0;
+    int32_t st = emit(f.low, f.high, f.velocity, added);
+    emitted += added;
+    changed = changed || added > 0;
+    if (st != SFL_OK) return st;
+  }
+  return SFL_OK;
 }
 
 // ---- walls (src/fluid/boundary.rs) ------------------------------------------
@@ -826,6 +978,34 @@ int32_t World::step(float dt, sfl_report* out) {
     if (st != SFL_OK) return st;
     r.max_speed = vmax_;
     r.max_compression = totals().comp;
+  }
+  // flows, then the domain (src/fluid/step.rs:76-82)
+  bool changed = false;
+  uint32_t emitted = 0, drained = 0, lost = 0;
+  int32_t st = run_flows(emitted, drained, changed);
+  if (st != SFL_OK) {
+    vmax_valid_ = false;
+    return st;
+  }
+  if (has_domain_) {
+    st = keep_only(domain_, true, lost);
+    if (st != SFL_OK) return st;
+    changed = changed || lost > 0;
+  }
+  r.emitted = emitted;
+  r.drained = drained;
+  r.lost = lost;
+  if (changed) {
+    // the report's fastest and most squeezed over the particles now there
+    // (step.rs:83-88)
+    launch_flow_totals(stream_, va_.ptr, (uint32_t)n_, settings_.rest_density, ctl_);
+    st = read_control();
+    if (st != SFL_OK) return st;
+    const Totals t = totals();
+    r.max_speed = t.vmax;
+    r.max_compression = t.comp;
+    vmax_ = t.vmax;
+    vmax_valid_ = true;
   }
   if (out) *out = r;
   return SFL_OK;

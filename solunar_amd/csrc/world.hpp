@@ -80,6 +80,9 @@ class World {
   int32_t add_particles(size_t count, const float* xyz, const float* vxyz, uint32_t* ids);
   int32_t fill_box(sfl_vec3 low, sfl_vec3 high, float jitter, uint32_t seed, size_t* count);
   int32_t set_domain(const sfl_vec3* low, const sfl_vec3* high);
+  int32_t add_emitter(sfl_vec3 low, sfl_vec3 high, sfl_vec3 velocity, uint32_t* id);
+  int32_t add_drain(sfl_vec3 low, sfl_vec3 high, uint32_t* id);
+  int32_t remove_flow(uint32_t id, int32_t* removed);
   int32_t add_box_container(sfl_vec3 low, sfl_vec3 high, uint32_t* id);
   int32_t add_wall(sfl_vec3 low, sfl_vec3 high, uint32_t* id);
   int32_t add_boundary_points(size_t count, const float* xyz, uint32_t* id);
@@ -142,6 +145,11 @@ class World {
     lists_emitted_ = false;
   }
   int32_t add_wall_points(const std::vector<float>& pts, uint32_t* id);
+  int32_t add_flow(sfl_vec3 low, sfl_vec3 high, bool drain, sfl_vec3 velocity, uint32_t* id);
+  int32_t run_flows(uint32_t& emitted, uint32_t& drained, bool& changed);
+  int32_t keep_only(const FlowBox& box, bool domain, uint32_t& gone);
+  int32_t emit(const sfl_vec3& low, const sfl_vec3& high, const sfl_vec3& velocity, uint32_t& added);
+  int32_t read_count(const uint32_t* dev, uint32_t* out);
   void phase_begin(int phase);
   void phase_end(int phase);
   void collect_profile();
@@ -166,6 +174,17 @@ class World {
     float lo[3], hi[3];
   };
   std::vector<Container> containers_;
+  // flows (src/fluid/flow.rs) and the domain (src/fluid/mod.rs:428-465)
+  struct Flow {
+    uint32_t id;
+    sfl_vec3 low, high;
+    bool drain;
+    sfl_vec3 velocity;
+  };
+  std::vector<Flow> flows_;
+  uint32_t flow_next_ = 0;
+  bool has_domain_ = false;
+  FlowBox domain_{};
   bool has_bounds_ = false;
   float blo_[3] = {0, 0, 0}, bhi_[3] = {0, 0, 0};
 
@@ -196,6 +215,9 @@ class World {
   int format_ = kLists;
   // the DFSPH stop sum's per solve tile sums and counts, sized with the
   // cell table
+  // keep flags and their scan (drains and the domain), free-slot flags and
+  // their scan (emitters)
+  DeviceBuffer<uint32_t> flow_flags_, flow_start_, flow_tiles_, flow_total_;
   DeviceBuffer<double> stop_sum_;
   DeviceBuffer<uint32_t> stop_count_;
   uint32_t stop_cap() const {

@@ -51,6 +51,9 @@ def library() -> ctypes.CDLL:
         "sfl_fill_box": (i32, [P, CVec3, CVec3, ctypes.POINTER(sz)]),
         "sfl_fill_box_jittered": (i32, [P, CVec3, CVec3, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(sz)]),
         "sfl_set_domain": (i32, [P, ctypes.POINTER(CVec3), ctypes.POINTER(CVec3)]),
+        "sfl_add_emitter": (i32, [P, CVec3, CVec3, CVec3, u32p]),
+        "sfl_add_drain": (i32, [P, CVec3, CVec3, u32p]),
+        "sfl_remove_flow": (i32, [P, ctypes.c_uint32, ctypes.POINTER(i32)]),
         "sfl_add_box_container": (i32, [P, CVec3, CVec3, u32p]),
         "sfl_add_wall": (i32, [P, CVec3, CVec3, u32p]),
         "sfl_add_boundary_points": (i32, [P, sz, fp, u32p]),
@@ -195,6 +198,23 @@ class FluidWorld:
         else:
             lo, hi = CVec3.of(domain[0]), CVec3.of(domain[1])
             self._check(self._lib.sfl_set_domain(self._h, ctypes.byref(lo), ctypes.byref(hi)))
+
+    # ---- flows (flow.rs) ----
+    def add_emitter(self, low, high, velocity) -> int:
+        out = ctypes.c_uint32()
+        self._check(self._lib.sfl_add_emitter(self._h, CVec3.of(low), CVec3.of(high), CVec3.of(velocity),
+                                              ctypes.byref(out)))
+        return out.value
+
+    def add_drain(self, low, high) -> int:
+        out = ctypes.c_uint32()
+        self._check(self._lib.sfl_add_drain(self._h, CVec3.of(low), CVec3.of(high), ctypes.byref(out)))
+        return out.value
+
+    def remove_flow(self, flow_id: int) -> bool:
+        out = ctypes.c_int32()
+        self._check(self._lib.sfl_remove_flow(self._h, flow_id, ctypes.byref(out)))
+        return out.value != 0
 
     # ---- walls (boundary.rs) ----
     def add_box_container(self, low, high) -> int:

@@ -220,8 +220,11 @@ def test_bad_input_is_refused_and_changes_nothing():
         gpu.step(-1.0)
     assert e.value.name == "NonFinite"
     with pytest.raises(PhysicsError) as e:
-        gpu.set_domain(((0, 0, 0), (1, 1, 1)))
-    assert e.value.name == "Unsupported"
+        gpu.set_domain(((1, 1, 1), (0, 0, 0)))
+    assert e.value.name == "OutOfRange"
+    with pytest.raises(PhysicsError) as e:
+        gpu.add_emitter((0, 0, 0), (1, 1, 1), (float("nan"), 0, 0))
+    assert e.value.name == "NonFinite"
     for bad in (dict(spacing=0.0), dict(support=9.0), dict(max_particles=0), dict(courant=1.5)):
         with pytest.raises(PhysicsError) as e:
             FluidWorld(FluidSettings(**bad))
