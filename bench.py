@@ -17,7 +17,7 @@ as one batch, `steps` steps of 1e-4 s in one graph launch, next to the same
 256 worlds stepped one by one on the single-world path.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
-                       [--workload dam_1m|dam_16m|batch_256]
+                       [--workload dam_1m|dam_16m|batch_256|adjoint_64k|flows_1m]
 """
 from __future__ import annotations
 
@@ -250,6 +250,80 @@ def run_adjoint(args) -> dict:
     }
 
 
+# flows_1m: the dam_1m scene with an emitter above the water (40 x 10 x 40
+# lattice slots, new water moving down at 1 m/s), a drain on the far floor
+# and a domain around the container. After each step a drain or the domain
+# reads every particle's position (16 B) to flag it, an emitter reads them
+# once more to mark its taken slots; the flows' algorithmic bytes per step
+# are those reads.
+FLOW_EMITTER = ((0.1, 0.85, 0.1), (0.3, 0.9, 0.3), (0.0, -1.0, 0.0))
+FLOW_DRAIN = ((1.2, 0.0, 0.0), (1.6, 0.1, 0.8))
+FLOW_DOMAIN = ((-0.05, -0.05, -0.05), (1.65, 1.65, 0.85))
+
+
+def run_flows(args) -> dict:
+    from solunar_amd import scenes
+    from solunar_amd.fluid import FluidWorld
+
+    world = FluidWorld(scenes.settings_dam_1m())
+    n0 = scenes.dam_break(world)
+    world.add_emitter(*FLOW_EMITTER)
+    world.add_drain(*FLOW_DRAIN)
+    world.set_domain(FLOW_DOMAIN)
+    log(f"flows_1m: {n0} fluid particles, emitter, drain, domain; warmup {args.warmup} steps")
+    for k in range(args.warmup):
+        r = world.step(DT)
+        log(f"warmup step {k}: {r.substeps} substeps, emitted {r.emitted}, drained {r.drained}, lost {r.lost}")
+    world.synchronize()
+    world.profile(True)
+    world.profile_read(reset=True)
+    world.synchronize()
+    particle_steps = 0
+    emitted = drained = lost = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        particle_steps += len(world)
+        r = world.step(DT)
+        emitted += r.emitted
+        drained += r.drained
+        lost += r.lost
+    world.synchronize()
+    elapsed = time.perf_counter() - t0
+    phases = world.profile_read(reset=True)
+    world.profile(False)
+    n_end = len(world)
+    world.close()
+    flows_ms, flows_launches = phases.get("flows", (0.0, 0))
+    flow_bytes = 16 * 3 * particle_steps  # drain, domain and emitter each read every position
+    flows_s = flows_ms * 1e-3
+    return {
+        "metric": "particle-steps/s at 1M DFSPH with flows",
+        "value": particle_steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "flows_1m: dam_1m (DFSPH, 1,024,000 fluid particles, spacing 0.005 m, step 5e-4 s) "
+                               "with an emitter of 40 x 10 x 40 slots above the water (new water at 1 m/s "
+                               "downwards), a drain on the far floor and a domain around the container",
+                   "fluid_particles_start": n0, "fluid_particles_end": n_end,
+                   "emitted": emitted, "drained": drained, "lost": lost},
+        "flows": {"ms_per_step": flows_ms / max(args.steps, 1), "share_of_step": flows_s / elapsed,
+                  "launches": flows_launches,
+                  "achieved_GBps": flow_bytes / flows_s / 1e9 if flows_s > 0 else None,
+                  "algorithmic_bytes": flow_bytes},
+        "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,9 +331,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256", "adjoint_64k"], default="dam_1m",
+    ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256", "adjoint_64k", "flows_1m"],
+                    default="dam_1m",
                     help="dam_16m: 16,000,000 fluid particles at spacing 0.002 with 200 sphere obstacles; "
-                         "batch_256: 256 WCSPH scenes in one batch")
+                         "batch_256: 256 WCSPH scenes in one batch; flows_1m: dam_1m with an emitter, a "
+                         "drain and a domain")
     ap.add_argument("--per-scene-steps", type=int, default=10,
                     help="batch_256: steps of each world timed on the single-world path")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -285,6 +361,9 @@ def main() -> int:
         return 0
     if args.workload == "adjoint_64k":
         print(json.dumps(run_adjoint(args)), flush=True)
+        return 0
+    if args.workload == "flows_1m":
+        print(json.dumps(run_flows(args)), flush=True)
         return 0
 
     from solunar_amd import scenes

@@ -227,7 +227,8 @@ enum sfl_phase {
   SFL_PHASE_SOLVE = 5,    /* DFSPH persistent Jacobi solves */
   SFL_PHASE_VISCOUS = 6,  /* DFSPH viscosity + gravity */
   SFL_PHASE_ADVANCE = 7,  /* DFSPH advance */
-  SFL_PHASE_COUNT = 8
+  SFL_PHASE_FLOWS = 8,    /* drains, emitters and the domain after a step */
+  SFL_PHASE_COUNT = 9
 };
 int32_t sfl_profile(sfl_world* world, int32_t enable);
 /* ms_out[SFL_PHASE_COUNT] summed milliseconds, launches_out[SFL_PHASE_COUNT]

@@ -982,6 +982,8 @@ int32_t World::step(float dt, sfl_report* out) {
   // flows, then the domain (src/fluid/step.rs:76-82)
   bool changed = false;
   uint32_t emitted = 0, drained = 0, lost = 0;
+  const bool flowing = !flows_.empty() || has_domain_;
+  if (flowing) phase_begin(SFL_PHASE_FLOWS);
   int32_t st = run_flows(emitted, drained, changed);
   if (st != SFL_OK) {
     vmax_valid_ = false;
@@ -1007,6 +1009,7 @@ int32_t World::step(float dt, sfl_report* out) {
     vmax_ = t.vmax;
     vmax_valid_ = true;
   }
+  if (flowing) phase_end(SFL_PHASE_FLOWS);
   if (out) *out = r;
   return SFL_OK;
 }

@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsolunar_
 
 _lib = None
 
-PHASES = ("bounds", "sort", "reorder", "density", "force", "solve", "viscous", "advance")
+PHASES = ("bounds", "sort", "reorder", "density", "force", "solve", "viscous", "advance", "flows")
 
 
 def library() -> ctypes.CDLL:
