@@ -49,7 +49,9 @@ enum sfl_status {
   /* No reference counterpart: a HIP call failed (detail names it). */
   SFL_DEVICE_ERROR = 100,
   /* No reference counterpart: a feature the device path does not carry. */
-  SFL_UNSUPPORTED = 101
+  SFL_UNSUPPORTED = 101,
+  /* A snapshot was refused; sfl_snapshot_error says why (SnapshotError). */
+  SFL_SNAPSHOT_REFUSED = 102
 };
 
 enum sfl_solver { SFL_WCSPH = 0, SFL_DFSPH = 1 };
@@ -156,6 +158,45 @@ int32_t sfl_add_drain(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
 /* FluidWorld::remove_flow (src/fluid/flow.rs:113-118): *removed_out is 0
  * for an id that is not there. */
 int32_t sfl_remove_flow(sfl_world* world, uint32_t flow, int32_t* removed_out);
+
+/* ---- snapshots (src/fluid/save.rs) ---- */
+
+/* SnapshotError (src/snapshot/mod.rs:56-63): `version` is set for
+ * UNSUPPORTED_VERSION, `what` (a static string) for INVALID. */
+enum sfl_snapshot_kind {
+  SFL_SNAPSHOT_TOO_LARGE = 1,
+  SFL_SNAPSHOT_TRUNCATED = 2,
+  SFL_SNAPSHOT_BAD_MAGIC = 3,
+  SFL_SNAPSHOT_UNSUPPORTED_VERSION = 4,
+  SFL_SNAPSHOT_BAD_CHECKSUM = 5,
+  SFL_SNAPSHOT_INVALID = 6
+};
+typedef struct sfl_snapshot_error {
+  int32_t kind; /* enum sfl_snapshot_kind, 0 when not refused */
+  uint32_t version;
+  const char* what;
+} sfl_snapshot_error;
+
+/* FluidWorld::save_snapshot (src/fluid/save.rs:17-66): the SLFL version 3
+ * bytes of the world (settings, particles in order, next id, walls, flows,
+ * domain, FNV-1a checksum), the same bytes the reference writes. With `out`
+ * null, only *len_out is set; with `cap` too small, *len_out is set and
+ * SFL_OUT_OF_RANGE returned. Worlds here hold no coupled bodies or terrain,
+ * so those counts are 0. */
+int32_t sfl_save_snapshot(sfl_world* world, uint8_t* out, size_t cap,
+                          size_t* len_out);
+
+/* FluidWorld::load_snapshot (src/fluid/save.rs:68-195): a new world from
+ * untrusted bytes, checked as the reference checks them (size, magic,
+ * checksum, version 1..3, settings, counts against the bytes left, ids,
+ * walls, flows, domain, no trailing bytes). A refused snapshot returns
+ * SFL_SNAPSHOT_REFUSED with *error_out filled; one that holds coupled bodies
+ * or terrain returns SFL_UNSUPPORTED. Settings the snapshot does not hold
+ * (the extensions, device included) come from `base` (defaults when null).
+ * Densities start at the rest density, as the reference's do. */
+int32_t sfl_load_snapshot(const uint8_t* data, size_t len,
+                          const sfl_settings* base, sfl_world** world_out,
+                          sfl_snapshot_error* error_out);
 
 /* add_box_container / add_wall (src/fluid/boundary.rs:41-91). */
 int32_t sfl_add_box_container(sfl_world* world, sfl_vec3 low, sfl_vec3 high,

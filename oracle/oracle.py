@@ -21,7 +21,7 @@ from typing import Optional
 import numpy as np
 
 from solunar_amd.types import (CReport, CSettings, FluidReport, FluidSettings,
-                               PhysicsError)
+                               PhysicsError, SnapshotError)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_fluid.so")
@@ -72,6 +72,9 @@ def library() -> ctypes.CDLL:
         "orc_add_drain": (i32, [P, fp, fp, u32p]),
         "orc_remove_flow": (i32, [P, ctypes.c_uint32]),
         "orc_run_flows": (i32, [P, u32p, u32p]),
+        "orc_save_snapshot": (i32, [P, ctypes.POINTER(ctypes.c_uint8), sz, ctypes.POINTER(sz)]),
+        "orc_load_snapshot": (i32, [ctypes.POINTER(ctypes.c_uint8), sz, ctypes.POINTER(CSettings), i32,
+                                    ctypes.POINTER(P), ctypes.POINTER(i32), u32p, ctypes.POINTER(ctypes.c_char_p)]),
         "orc_wall_len": (sz, [P]),
         "orc_read_wall_positions": (i32, [P, sz, fp]),
         "orc_step": (i32, [P, f, ctypes.POINTER(CReport)]),
@@ -104,6 +107,21 @@ def library() -> ctypes.CDLL:
 
 def _f32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1, 3))
+
+
+def settings_from_snapshot(data: bytes, base: Optional[FluidSettings]) -> FluidSettings:
+    """The settings fields an accepted snapshot holds (save.rs:20-40) over `base`."""
+    import struct
+    from dataclasses import replace
+
+    solver = data[8]
+    spacing, rest, support, sound, viscosity, courant = struct.unpack_from("<6f", data, 9)
+    (max_particles,) = struct.unpack_from("<Q", data, 33)
+    gravity = struct.unpack_from("<3f", data, 41)
+    (max_substeps,) = struct.unpack_from("<I", data, 53)
+    return replace(base or FluidSettings(), solver=solver, spacing=spacing, rest_density=rest, support=support,
+                   sound_speed=sound, viscosity=viscosity, courant=courant, max_particles=max_particles,
+                   gravity=tuple(gravity), max_substeps=max_substeps)
 
 
 def _v3(a) -> np.ndarray:
@@ -234,6 +252,42 @@ class OracleWorld:
         o = ctypes.c_uint32()
         self._check(self._lib.orc_add_drain(self._h, _p(lo, ctypes.c_float), _p(hi, ctypes.c_float), ctypes.byref(o)))
         return o.value
+
+    def save_snapshot(self) -> bytes:
+        """The SLFL version 3 bytes (src/fluid/save.rs:17-66)."""
+        n = ctypes.c_size_t()
+        self._check(self._lib.orc_save_snapshot(self._h, None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_uint8 * n.value)()
+        self._check(self._lib.orc_save_snapshot(self._h, buf, n.value, ctypes.byref(n)))
+        return bytes(buf)
+
+    @classmethod
+    def load_snapshot(cls, data: bytes, settings: Optional[FluidSettings] = None, precision: int = 32,
+                      threads: int = 1, residual_mode: int = 0) -> "OracleWorld":
+        """A world from snapshot bytes (src/fluid/save.rs:68-195); raises
+        SnapshotError for refused bytes, PhysicsError(101) for coupled bodies
+        or terrain."""
+        lib = library()
+        data = bytes(data)
+        buf = (ctypes.c_uint8 * max(len(data), 1)).from_buffer_copy(data + b"\0")
+        base = CSettings.of(settings) if settings is not None else None
+        h = ctypes.c_void_p()
+        kind, version, what = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_char_p()
+        st = lib.orc_load_snapshot(buf, len(data), ctypes.byref(base) if base is not None else None,
+                                   64 if precision == 64 else 32, ctypes.byref(h), ctypes.byref(kind),
+                                   ctypes.byref(version), ctypes.byref(what))
+        if st == 101:
+            raise PhysicsError(101, (what.value or b"").decode())
+        if st:
+            raise SnapshotError(kind.value, version.value, (what.value or b"").decode())
+        w = cls.__new__(cls)
+        w._lib = lib
+        w._h = h
+        w.precision = 64 if precision == 64 else 32
+        w._settings = settings_from_snapshot(data, settings)
+        w.set_threads(threads)
+        w.set_residual_mode(residual_mode)
+        return w
 
     def run_flows(self):
         """Drains then emitters once, without a step: (emitted, drained)."""

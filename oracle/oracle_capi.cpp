@@ -8,6 +8,7 @@
 #include <memory>
 
 #include "adjoint_oracle.hpp"
+#include "snapshot_oracle.hpp"
 #include "fluid_oracle.hpp"
 
 using namespace oracle;
@@ -258,6 +259,45 @@ int32_t orc_add_drain(void* w, const float* low, const float* high, uint32_t* ou
   return with(static_cast<Handle*>(w), [&](auto& x) {
     return x.add_flow({low[0], low[1], low[2]}, {high[0], high[1], high[2]}, true, {0.0f, 0.0f, 0.0f}, out);
   });
+}
+
+// save_snapshot (src/fluid/save.rs:17-66): out null sets *len only; a short
+// buffer returns kOutOfRange
+int32_t orc_save_snapshot(void* w, uint8_t* out, size_t cap, size_t* len) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    std::vector<uint8_t> bytes = save_snapshot(x);
+    if (len) *len = bytes.size();
+    if (!out) return (int32_t)kOk;
+    if (cap < bytes.size()) return (int32_t)kOutOfRange;
+    std::memcpy(out, bytes.data(), bytes.size());
+    return (int32_t)kOk;
+  });
+}
+
+// load_snapshot (src/fluid/save.rs:68-195): *kind 0 and a world, or the
+// refusal (1..6 as SnapshotError, 101 for coupled bodies or terrain)
+int32_t orc_load_snapshot(const uint8_t* data, size_t len, const orc_settings* base, int32_t precision, void** out,
+                          int32_t* kind, uint32_t* version, const char** what) {
+  *out = nullptr;
+  Settings b = base ? convert(base) : Settings{};
+  SnapshotState st;
+  static const uint8_t none = 0;
+  SnapshotResult r = parse_snapshot(data ? data : &none, len, b, st);
+  *kind = r.kind;
+  *version = r.version;
+  *what = r.what;
+  if (r.kind != kSnapOk) return r.kind;
+  auto* h = new Handle();
+  h->precision = precision == 64 ? 64 : 32;
+  if (h->precision == 64) {
+    h->d.reset(new World<double>(st.settings));
+    apply_snapshot(*h->d, st);
+  } else {
+    h->f.reset(new World<float>(st.settings));
+    apply_snapshot(*h->f, st);
+  }
+  *out = h;
+  return kOk;
 }
 
 // run_flows alone (the reference's tests call it without a step)

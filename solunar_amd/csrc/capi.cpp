@@ -90,6 +90,46 @@ int32_t sfl_create(const sfl_settings* settings, sfl_world** out) {
   }
 }
 
+int32_t sfl_save_snapshot(sfl_world* world, uint8_t* out, size_t cap, size_t* len_out) {
+  return guarded(world, [&](sfl::World& w) {
+    std::vector<uint8_t> bytes;
+    int32_t st = w.save_snapshot(bytes);
+    if (st != SFL_OK) return st;
+    if (len_out) *len_out = bytes.size();
+    if (!out) return (int32_t)SFL_OK;
+    if (cap < bytes.size()) return (int32_t)SFL_OUT_OF_RANGE;
+    std::memcpy(out, bytes.data(), bytes.size());
+    return (int32_t)SFL_OK;
+  });
+}
+
+int32_t sfl_load_snapshot(const uint8_t* data, size_t len, const sfl_settings* base, sfl_world** world_out,
+                          sfl_snapshot_error* error_out) {
+  if (!world_out || (!data && len)) return SFL_OUT_OF_RANGE;
+  *world_out = nullptr;
+  if (error_out) *error_out = sfl_snapshot_error{0, 0, nullptr};
+  try {
+    sfl_settings b;
+    if (base)
+      b = *base;
+    else
+      sfl_settings_default(&b);
+    sfl::World* w = nullptr;
+    sfl::SnapshotFailure fail{0, 0, nullptr};
+    std::string why;
+    static const uint8_t none = 0;
+    int32_t st = sfl::World::load_snapshot(data ? data : &none, len, b, &w, fail, why);
+    g_create_detail = why;
+    if (error_out) *error_out = sfl_snapshot_error{fail.kind, fail.version, fail.what};
+    if (st != SFL_OK) return st;
+    *world_out = new sfl_world{w};
+    return SFL_OK;
+  } catch (const std::bad_alloc&) {
+    g_create_detail = "host allocation failed";
+    return SFL_DEVICE_ERROR;
+  }
+}
+
 void sfl_destroy(sfl_world* world) {
   if (!world) return;
   delete world->w;

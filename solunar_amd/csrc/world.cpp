@@ -87,6 +87,41 @@ int32_t World::device_error(const char* what, hipError_t e) {
     if (e_ != hipSuccess) return device_error(what, e_); \
   } while (0)
 
+bool World::valid_settings(const sfl_settings& s) { return settings_valid(s); }
+
+// The particles of a snapshot, in its order, with its ids; densities at the
+// rest density (save.rs:143).
+int32_t World::restore(size_t count, const uint32_t* ids, const float* xyz, const float* vxyz, uint32_t next_id) {
+  DeviceGuard g(device_);
+  int32_t st = ensure_particles(count);
+  if (st != SFL_OK) return st;
+  std::vector<float4> hp(count), hv(count);
+  float vmax = 0.0f;
+  for (size_t i = 0; i < count; ++i) {
+    float idf;
+    std::memcpy(&idf, &ids[i], 4);
+    hp[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], idf);
+    hv[i] = make_float4(vxyz[3 * i], vxyz[3 * i + 1], vxyz[3 * i + 2], settings_.rest_density);
+    vmax = fmax_rust(vmax, length3(vxyz[3 * i], vxyz[3 * i + 1], vxyz[3 * i + 2]));
+  }
+  if (count) {
+    SFL_HIP(hipMemcpyAsync(pa_.ptr, hp.data(), count * sizeof(float4), hipMemcpyHostToDevice, stream_),
+            "upload positions");
+    SFL_HIP(hipMemcpyAsync(va_.ptr, hv.data(), count * sizeof(float4), hipMemcpyHostToDevice, stream_),
+            "upload velocities");
+    SFL_HIP(hipStreamSynchronize(stream_), "upload");
+  }
+  n_ = count;
+  next_id_ = next_id;
+  vmax_ = vmax;
+  vmax_valid_ = true;
+  invalidate_lists();
+  forces_valid_ = false;
+  ++version_;
+  return SFL_OK;
+}
+
+
 int32_t World::create(const sfl_settings& s, World** out, std::string& why) {
   if (!settings_valid(s)) return SFL_INVALID_SETTINGS;
   World* w = new World();

@@ -69,12 +69,24 @@ uint64_t hash_particles(const float4* p, const float4* v, size_t n);
 class Batch;
 class Tape;
 
+// Why a snapshot was refused (sfl_snapshot_error).
+struct SnapshotFailure {
+  int32_t kind;
+  uint32_t version;
+  const char* what;
+};
+
 class World {
   friend class Batch;
   friend class Tape;
 
  public:
   static int32_t create(const sfl_settings& s, World** out, std::string& why);
+  static bool valid_settings(const sfl_settings& s);
+  // snapshots (src/fluid/save.rs), snapshot.cpp
+  int32_t save_snapshot(std::vector<uint8_t>& bytes);
+  static int32_t load_snapshot(const uint8_t* data, size_t size, const sfl_settings& base, World** out,
+                               SnapshotFailure& fail, std::string& why);
   ~World();
 
   int32_t add_particles(size_t count, const float* xyz, const float* vxyz, uint32_t* ids);
@@ -146,6 +158,7 @@ class World {
   }
   int32_t add_wall_points(const std::vector<float>& pts, uint32_t* id);
   int32_t add_flow(sfl_vec3 low, sfl_vec3 high, bool drain, sfl_vec3 velocity, uint32_t* id);
+  int32_t restore(size_t count, const uint32_t* ids, const float* xyz, const float* vxyz, uint32_t next_id);
   int32_t run_flows(uint32_t& emitted, uint32_t& drained, bool& changed);
   int32_t keep_only(const FlowBox& box, bool domain, uint32_t& gone);
   int32_t emit(const sfl_vec3& low, const sfl_vec3& high, const sfl_vec3& velocity, uint32_t& added);

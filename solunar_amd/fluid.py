@@ -17,14 +17,26 @@ from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
-from .types import (CBatchTuning, CReport, CSettings, CVec3, FluidReport,
-                    FluidSettings, PhysicsError)
+from .types import (CBatchTuning, CReport, CSettings, CSnapshotError, CVec3,
+                    FluidReport, FluidSettings, PhysicsError)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsolunar_fluid.so")
 
 _lib = None
 
 PHASES = ("bounds", "sort", "reorder", "density", "force", "solve", "viscous", "advance", "flows")
+
+
+SNAPSHOT_REFUSED = 102  # SFL_SNAPSHOT_REFUSED
+
+
+def settings_of(c: CSettings) -> FluidSettings:
+    """FluidSettings from an `sfl_settings`."""
+    kw = {}
+    for f in FluidSettings.__dataclass_fields__:
+        v = getattr(c, f)
+        kw[f] = tuple(float(x) for x in v) if f == "gravity" else v
+    return FluidSettings(**kw)
 
 
 def library() -> ctypes.CDLL:
@@ -54,6 +66,9 @@ def library() -> ctypes.CDLL:
         "sfl_add_emitter": (i32, [P, CVec3, CVec3, CVec3, u32p]),
         "sfl_add_drain": (i32, [P, CVec3, CVec3, u32p]),
         "sfl_remove_flow": (i32, [P, ctypes.c_uint32, ctypes.POINTER(i32)]),
+        "sfl_save_snapshot": (i32, [P, ctypes.POINTER(ctypes.c_uint8), sz, ctypes.POINTER(sz)]),
+        "sfl_load_snapshot": (i32, [ctypes.POINTER(ctypes.c_uint8), sz, ctypes.POINTER(CSettings), ctypes.POINTER(P),
+                                    ctypes.POINTER(CSnapshotError)]),
         "sfl_add_box_container": (i32, [P, CVec3, CVec3, u32p]),
         "sfl_add_wall": (i32, [P, CVec3, CVec3, u32p]),
         "sfl_add_boundary_points": (i32, [P, sz, fp, u32p]),
@@ -215,6 +230,40 @@ class FluidWorld:
         out = ctypes.c_int32()
         self._check(self._lib.sfl_remove_flow(self._h, flow_id, ctypes.byref(out)))
         return out.value != 0
+
+    # ---- snapshots (save.rs) ----
+    def save_snapshot(self) -> bytes:
+        """The SLFL version 3 bytes of the world (src/fluid/save.rs:17-66)."""
+        n = ctypes.c_size_t()
+        self._check(self._lib.sfl_save_snapshot(self._h, None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_uint8 * n.value)()
+        self._check(self._lib.sfl_save_snapshot(self._h, buf, n.value, ctypes.byref(n)))
+        return bytes(buf)
+
+    @classmethod
+    def load_snapshot(cls, data: bytes, settings: Optional[FluidSettings] = None) -> "FluidWorld":
+        """A new world from snapshot bytes (src/fluid/save.rs:68-195); raises
+        SnapshotError for refused bytes. `settings` gives what the snapshot
+        does not hold (the extensions and the device)."""
+        lib = library()
+        data = bytes(data)
+        buf = (ctypes.c_uint8 * max(len(data), 1)).from_buffer_copy(data + b"\0")
+        base = CSettings.of(settings) if settings is not None else None
+        handle = ctypes.c_void_p()
+        err = CSnapshotError()
+        st = lib.sfl_load_snapshot(buf, len(data), ctypes.byref(base) if base is not None else None,
+                                   ctypes.byref(handle), ctypes.byref(err))
+        if st == SNAPSHOT_REFUSED:
+            raise err.error()
+        if st:
+            raise PhysicsError(st, lib.sfl_error_detail(None).decode())
+        w = cls.__new__(cls)
+        w._lib = lib
+        w._h = handle
+        c = CSettings()
+        lib.sfl_get_settings(handle, ctypes.byref(c))
+        w._settings = settings_of(c)
+        return w
 
     # ---- walls (boundary.rs) ----
     def add_box_container(self, low, high) -> int:

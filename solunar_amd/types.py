@@ -44,6 +44,38 @@ DEVICE_ERROR = 100
 UNSUPPORTED = 101
 
 
+SNAPSHOT_KINDS = {1: "TooLarge", 2: "Truncated", 3: "BadMagic", 4: "UnsupportedVersion", 5: "BadChecksum",
+                  6: "Invalid"}
+
+
+class SnapshotError(Exception):
+    """SnapshotError (src/snapshot/mod.rs:56-63): `name` is the variant,
+    `version` set for UnsupportedVersion, `what` for Invalid."""
+
+    def __init__(self, kind: int, version: int = 0, what: str = ""):
+        self.kind = kind
+        self.name = SNAPSHOT_KINDS.get(kind, f"Kind{kind}")
+        self.version = version
+        self.what = what
+        super().__init__(f"{self.name}" + (f" {version}" if kind == 4 else "") + (f": {what}" if what else ""))
+
+    def __eq__(self, other):
+        return (isinstance(other, SnapshotError) and (self.kind, self.version, self.what) ==
+                (other.kind, other.version, other.what))
+
+    def __hash__(self):
+        return hash((self.kind, self.version, self.what))
+
+
+class CSnapshotError(ctypes.Structure):
+    """`sfl_snapshot_error` of include/solunar_fluid.h."""
+
+    _fields_ = [("kind", ctypes.c_int32), ("version", ctypes.c_uint32), ("what", ctypes.c_char_p)]
+
+    def error(self) -> SnapshotError:
+        return SnapshotError(self.kind, self.version, self.what.decode() if self.what else "")
+
+
 class PhysicsError(Exception):
     """A non-zero status; `name` is the PhysicsError variant."""
 
