@@ -17,7 +17,7 @@ as one batch, `steps` steps of 1e-4 s in one graph launch, next to the same
 256 worlds stepped one by one on the single-world path.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
-                       [--workload dam_1m|dam_16m|batch_256|adjoint_64k|flows_1m]
+                       [--workload dam_1m|dam_16m|batch_256|adjoint_64k|flows_1m|snapshot_1m]
 """
 from __future__ import annotations
 
@@ -324,6 +324,57 @@ def run_flows(args) -> dict:
     }
 
 
+def run_snapshot(args) -> dict:
+    """snapshot_1m: save and load of the dam_1m world after `warmup` steps
+    (1,024,000 particles at 28 B and 832,064 walls at 16 B in the SLFL
+    bytes). Both cross PCIe: save reads x and v back, load uploads them and
+    settles the walls."""
+    from solunar_amd import scenes
+    from solunar_amd.fluid import FluidWorld
+
+    world = FluidWorld(scenes.settings_dam_1m())
+    n = scenes.dam_break(world)
+    for _ in range(args.warmup):
+        world.step(DT)
+    world.synchronize()
+    reps = max(1, min(args.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        data = world.save_snapshot()
+    save_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loaded = FluidWorld.load_snapshot(data)
+        loaded.synchronize()
+        same = loaded.state_hash() == world.state_hash()
+        loaded.close()
+    load_s = (time.perf_counter() - t0) / reps
+    world.close()
+    return {
+        "metric": "snapshot bytes/s, 1M DFSPH world (save)",
+        "value": len(data) / save_s,
+        "unit": "B/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": args.warmup,
+        "ms_per_step": save_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "snapshot_1m: SLFL v3 save and load of the dam_1m world (1,024,000 fluid, "
+                               "832,064 wall particles) after the warmup steps",
+                   "bytes": len(data), "fluid_particles": n},
+        "save_ms": save_s * 1e3,
+        "load_ms": load_s * 1e3,
+        "load_bytes_per_s": len(data) / load_s,
+        "loaded_state_hash_equal": same,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,7 +382,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256", "adjoint_64k", "flows_1m"],
+    ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256", "adjoint_64k", "flows_1m",
+                                           "snapshot_1m"],
                     default="dam_1m",
                     help="dam_16m: 16,000,000 fluid particles at spacing 0.002 with 200 sphere obstacles; "
                          "batch_256: 256 WCSPH scenes in one batch; flows_1m: dam_1m with an emitter, a "
@@ -364,6 +416,9 @@ def main() -> int:
         return 0
     if args.workload == "flows_1m":
         print(json.dumps(run_flows(args)), flush=True)
+        return 0
+    if args.workload == "snapshot_1m":
+        print(json.dumps(run_snapshot(args)), flush=True)
         return 0
 
     from solunar_amd import scenes

@@ -92,14 +92,11 @@ int32_t sfl_create(const sfl_settings* settings, sfl_world** out) {
 
 int32_t sfl_save_snapshot(sfl_world* world, uint8_t* out, size_t cap, size_t* len_out) {
   return guarded(world, [&](sfl::World& w) {
-    std::vector<uint8_t> bytes;
-    int32_t st = w.save_snapshot(bytes);
-    if (st != SFL_OK) return st;
-    if (len_out) *len_out = bytes.size();
+    const size_t size = w.snapshot_size();
+    if (len_out) *len_out = size;
     if (!out) return (int32_t)SFL_OK;
-    if (cap < bytes.size()) return (int32_t)SFL_OUT_OF_RANGE;
-    std::memcpy(out, bytes.data(), bytes.size());
-    return (int32_t)SFL_OK;
+    if (cap < size) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.save_snapshot(out);
   });
 }
 

@@ -32,17 +32,27 @@ uint64_t fnv1a(const uint8_t* data, size_t n) {
   return h;
 }
 
-// Writer (src/snapshot/bytes.rs:15-70)
+// Writer (src/snapshot/bytes.rs:15-70), into a buffer of the exact size
 struct Out {
-  std::vector<uint8_t>& data;
-  void bytes(const void* p, size_t n) {
-    const uint8_t* b = static_cast<const uint8_t*>(p);
-    data.insert(data.end(), b, b + n);
+  uint8_t* at;
+  uint8_t* end;  // never written past: a short buffer leaves `over` set
+  bool over = false;
+  bool room(size_t n) {
+    if ((size_t)(end - at) < n) over = true;
+    return !over;
   }
-  void u8(uint8_t v) { data.push_back(v); }
+  void bytes(const void* p, size_t n) {
+    if (!room(n)) return;
+    std::memcpy(at, p, n);
+    at += n;
+  }
+  void u8(uint8_t v) {
+    if (room(1)) *at++ = v;
+  }
   void u32(uint32_t v) {
-    uint8_t b[4] = {(uint8_t)v, (uint8_t)(v >> 8), (uint8_t)(v >> 16), (uint8_t)(v >> 24)};
-    bytes(b, 4);
+    if (!room(4)) return;
+    at[0] = (uint8_t)v, at[1] = (uint8_t)(v >> 8), at[2] = (uint8_t)(v >> 16), at[3] = (uint8_t)(v >> 24);
+    at += 4;
   }
   void u64(uint64_t v) {
     u32((uint32_t)v);
@@ -129,8 +139,16 @@ bool inside_out(const sfl_vec3& lo, const sfl_vec3& hi) { return hi.x <= lo.x ||
     if (e_ != hipSuccess) return device_error(what, e_); \
   } while (0)
 
-// save_snapshot (save.rs:17-66)
-int32_t World::save_snapshot(std::vector<uint8_t>& bytes) {
+// The size save_snapshot writes.
+size_t World::snapshot_size() const {
+  size_t flows = 0;
+  for (const Flow& f : flows_) flows += f.drain ? 29 : 41;
+  return 4 + 4 + 1 + 24 + 8 + 12 + 4 + 4 + n_ * 28 + 4 + 4 + wall_owner_.size() * 16 + 4 + 4 +
+         containers_.size() * 28 + 4 + flows + 4 + 4 + 4 + (has_domain_ ? 25 : 1) + kChecksumBytes;
+}
+
+// save_snapshot (save.rs:17-66) into `bytes`, snapshot_size() of them
+int32_t World::save_snapshot(uint8_t* bytes) {
   Device g(device_);
   std::vector<float4> p(n_), v(n_);
   if (n_) {
@@ -139,9 +157,8 @@ int32_t World::save_snapshot(std::vector<uint8_t>& bytes) {
             "read velocities");
     SFL_HIP(hipStreamSynchronize(stream_), "snapshot");
   }
-  bytes.clear();
-  bytes.reserve(64 + n_ * 28 + wall_owner_.size() * 16);
-  Out out{bytes};
+  const size_t size = snapshot_size();
+  Out out{bytes, bytes + size};
   const sfl_settings& s = settings_;
   out.bytes(kMagic, 4);
   out.u32(kVersion);
@@ -195,7 +212,11 @@ int32_t World::save_snapshot(std::vector<uint8_t>& bytes) {
   } else {
     out.u8(0);
   }
-  out.u64(fnv1a(bytes.data(), bytes.size()));
+  if (!out.over) out.u64(fnv1a(bytes, (size_t)(out.at - bytes)));
+  if (out.over || (size_t)(out.at - bytes) != size) {
+    detail_ = "snapshot size";
+    return SFL_DEVICE_ERROR;
+  }
   return SFL_OK;
 }
 

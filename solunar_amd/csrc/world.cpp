@@ -661,10 +661,12 @@ int32_t World::settle_walls() {
     k.index = i;
     order[i] = k;
   }
-  std::sort(order.begin(), order.end(), [](const Key& a, const Key& b) {
+  auto before = [](const Key& a, const Key& b) {
     return std::tie(a.c[0], a.c[1], a.c[2], a.owner, a.bx, a.by, a.bz, a.index) <
            std::tie(b.c[0], b.c[1], b.c[2], b.owner, b.bx, b.by, b.bz, b.index);
-  });
+  };
+  // walls from a snapshot come settled already
+  if (!std::is_sorted(order.begin(), order.end(), before)) std::sort(order.begin(), order.end(), before);
   // table box
   CellBox box{};
   uint64_t volume = 0;

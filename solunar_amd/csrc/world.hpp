@@ -84,7 +84,8 @@ class World {
   static int32_t create(const sfl_settings& s, World** out, std::string& why);
   static bool valid_settings(const sfl_settings& s);
   // snapshots (src/fluid/save.rs), snapshot.cpp
-  int32_t save_snapshot(std::vector<uint8_t>& bytes);
+  size_t snapshot_size() const;
+  int32_t save_snapshot(uint8_t* bytes);
   static int32_t load_snapshot(const uint8_t* data, size_t size, const sfl_settings& base, World** out,
                                SnapshotFailure& fail, std::string& why);
   ~World();

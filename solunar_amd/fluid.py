@@ -236,9 +236,11 @@ class FluidWorld:
         """The SLFL version 3 bytes of the world (src/fluid/save.rs:17-66)."""
         n = ctypes.c_size_t()
         self._check(self._lib.sfl_save_snapshot(self._h, None, 0, ctypes.byref(n)))
-        buf = (ctypes.c_uint8 * n.value)()
+        out = bytearray(n.value)
+        buf = (ctypes.c_uint8 * n.value).from_buffer(out) if n.value else None
         self._check(self._lib.sfl_save_snapshot(self._h, buf, n.value, ctypes.byref(n)))
-        return bytes(buf)
+        del buf
+        return bytes(out)
 
     @classmethod
     def load_snapshot(cls, data: bytes, settings: Optional[FluidSettings] = None) -> "FluidWorld":
