@@ -57,6 +57,10 @@ int32_t Tape::record(World* w, float dt, uint32_t steps, uint32_t every, Tape** 
     why = "the adjoint needs particles with ids 0..len-1";
     return SFL_OUT_OF_RANGE;
   }
+  if (!w->flows_.empty() || w->has_domain_) {
+    why = "the adjoint differentiates steps that keep the particles: no flows or domain";
+    return SFL_UNSUPPORTED;
+  }
   std::unique_ptr<Tape> t(new Tape());
   t->w_ = w;
   t->dt_ = dt;

@@ -78,6 +78,10 @@ int32_t Batch::create(World* const* worlds, size_t count, Batch** out, std::stri
       why = "the worlds of a batch need equal settings";
       return SFL_INVALID_SETTINGS;
     }
+    if (!w.flows_.empty() || w.has_domain_) {
+      why = "a batch steps worlds without flows or a domain";
+      return SFL_UNSUPPORTED;
+    }
     total += w.n_;
   }
   if (total >= (uint64_t(1) << 31)) {

@@ -190,3 +190,9 @@ def test_a_tape_refuses_a_changed_world_and_a_dfsph_world():
     with pytest.raises(PhysicsError) as e:
         Tape(w, 1e-4, 0, 2)
     assert e.value.name == "OutOfRange"
+    f = FluidWorld(small_settings(0.05))
+    f.add_particles(x0, v0)
+    f.set_domain(((-1, -1, -1), (2, 2, 2)))
+    with pytest.raises(PhysicsError) as e:
+        Tape(f, 1e-4, 3, 2)
+    assert e.value.name == "Unsupported"

@@ -211,3 +211,9 @@ def test_batch_refuses_mixed_settings():
     with pytest.raises(PhysicsError) as e:
         FluidBatch([a, b])
     assert e.value.name == "InvalidSettings"
+    # flows and a domain run in World::step, which a batch does not call
+    c = FluidWorld(scenes.settings_dam_small())
+    c.add_drain((0, 0, 0), (0.1, 0.1, 0.1))
+    with pytest.raises(PhysicsError) as e:
+        FluidBatch([a, c])
+    assert e.value.name == "Unsupported"
