@@ -190,7 +190,7 @@ constexpr int kMasks32 = 1;
 constexpr int kLists = 2;
 constexpr int kCompact = 3;
 
-// The solve's tiles are 2 x 2 cell columns by tile_depth(span z) cells: the
+// The solve's tiles are 4 x 2 cell columns by tile_depth(span z) cells: the
 // z span split into pieces of at most 16, as even as they go.
 __host__ __device__ __forceinline__ uint32_t tile_depth(uint32_t sz) {
   uint32_t pieces = (sz + 15) / 16;

@@ -399,8 +399,8 @@ struct SolveArgs {
 //
 // density_rate (dfsph.rs:166-181) and the stiffness and error of solve
 // (dfsph.rs:222-262) for particle i, then push_apart (dfsph.rs:184-216).
-// Neighbour data comes from global memory (`Glob`) or from the half-block's
-// LDS tile (`Lds`); both give the same values, so the same bits.
+// Neighbour data comes from global memory (`Glob`) or from the block's LDS
+// tile (`Lds`); both give the same values, so the same bits.
 
 struct Glob {
   const float4* __restrict__ P;
