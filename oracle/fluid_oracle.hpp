@@ -64,7 +64,9 @@ inline int& planted_mistake() {
 // src/error.rs:5-22, offset by one (0 is success)
 enum Status : int32_t {
   kOk = 0,
+  kInvalidHandle = 1,
   kNonFinite = 4,
+  kInvalidShape = 5,
   kInvalidSettings = 7,
   kOutOfRange = 10,
   kDispatchIncomplete = 11,
@@ -342,6 +344,128 @@ struct Dispatch {
   }
 };
 
+// ---- coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) ----
+
+// Shapes a body couples with (src/shape.rs:8-49): a cuboid (a, b, c its half
+// extents), a sphere (a its radius), a capsule along local y (a its radius,
+// b the half height of its cylinder).
+enum ShapeKind : int32_t { kCuboid = 0, kSphere = 1, kCapsule = 2 };
+struct Shape {
+  int32_t kind;
+  float a, b, c;
+};
+
+// Where the rigid world has a coupled body this step (SubstepPose,
+// src/substep.rs) and whether it takes pushes (a dynamic body;
+// PhysicsWorld::apply_force fails on the others).
+struct Pose {
+  uint32_t index, generation;
+  double position[3];
+  float rotation[4];  // x, y, z, w
+  float linear[3], angular[3];
+  int32_t dynamic;
+};
+
+// What the water gave one coupled body over a step: the lift of lift.rs
+// (when the body is dynamic and so took it) and the damping of mover_push,
+// both as impulses (src/fluid/couple.rs:236-251).
+struct BodyPush {
+  uint32_t index, generation;
+  int32_t lifted;
+  float lift_impulse[3], lift_angular[3];
+  float impulse[3], angular[3];
+};
+
+// Most probes one body takes (src/fluid/lift.rs:8).
+constexpr size_t kMaxProbes = 4096;
+
+// Vec3::cross as glam writes it: (y z' - y' z, z x' - z' x, x y' - x' y).
+template <class R>
+inline Vec3<R> cross(Vec3<R> a, Vec3<R> b) {
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+
+// Quat::rotate_vec3 of `algebrix`, whose source is not available: taken to
+// be glam's v (w^2 - |u|^2) + u (2 (v.u)) + (u x v)(2 w), u = (x, y, z).
+template <class R>
+inline Vec3<R> rotate(const float q[4], Vec3<R> v) {
+  Vec3<R> u{R(q[0]), R(q[1]), R(q[2])};
+  R w = R(q[3]);
+  R b2 = dot(u, u);
+  return v * (w * w - b2) + u * (dot(v, u) * R(2)) + cross(u, v) * (w * R(2));
+}
+
+// f32::total_cmp / f64::total_cmp as an integer key.
+inline int32_t total_key(float v) {
+  int32_t b;
+  std::memcpy(&b, &v, 4);
+  return b ^ (int32_t)((uint32_t)(b >> 31) >> 1);
+}
+inline int64_t total_key(double v) {
+  int64_t b;
+  std::memcpy(&b, &v, 8);
+  return b ^ (int64_t)((uint64_t)(b >> 63) >> 1);
+}
+
+// f32::clamp: NaN passes through.
+template <class R>
+inline R clamp_rust(R v, R lo, R hi) {
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
+  return v;
+}
+
+// shape_points (src/fluid/couple.rs:443-502): lattice points `spacing`
+// apart inside the shape, no deeper than `reach`, centred on it, loops y, z,
+// x. A capsule with a negative or non-finite half height is refused as
+// Shape::validate refuses it (src/shape.rs:59-62).
+inline int32_t shape_points(const Shape& s, float spacing, float reach, size_t limit, std::vector<Vec3<float>>& out) {
+  using F = Vec3<float>;
+  out.clear();
+  F half;
+  switch (s.kind) {
+    case kCuboid:
+      half = {s.a, s.b, s.c};
+      break;
+    case kSphere:
+      half = {s.a, s.a, s.a};
+      break;
+    case kCapsule:
+      if (!std::isfinite(s.b) || s.b < 0.0f) return kInvalidShape;
+      half = {s.a, s.b + s.a, s.a};
+      break;
+    default:
+      return kInvalidShape;
+  }
+  if (!half.finite() || rmin(half.x, rmin(half.y, half.z)) <= 0.0f) return kInvalidShape;
+  auto count = [&](float extent) -> int64_t {
+    float f = std::floor(2.0f * extent / spacing);
+    int64_t c = std::isnan(f) ? 0 : (f >= 9.2e18f ? INT64_MAX : (f <= -9.2e18f ? INT64_MIN : (int64_t)f));
+    return std::min<int64_t>(std::max<int64_t>(c, 0), 1 << 12);
+  };
+  int64_t cx = count(half.x), cy = count(half.y), cz = count(half.z);
+  if (cx * cy * cz > (int64_t)limit) return kFluidFull;
+  auto start = [&](int64_t cells) { return -((float)cells - 1.0f) * spacing / 2.0f; };
+  F origin{start(cx), start(cy), start(cz)};
+  for (int64_t y = 0; y < cy; ++y)
+    for (int64_t z = 0; z < cz; ++z)
+      for (int64_t x = 0; x < cx; ++x) {
+        F at = origin + F{(float)x, (float)y, (float)z} * spacing;
+        float inside;
+        if (s.kind == kCuboid) {
+          F room{half.x - std::fabs(at.x), half.y - std::fabs(at.y), half.z - std::fabs(at.z)};
+          inside = rmin(rmin(room.x, room.y), room.z);
+        } else if (s.kind == kSphere) {
+          inside = s.a - length(at);
+        } else {
+          float yy = clamp_rust(at.y, -s.b, s.b);
+          inside = s.a - length(at - F{0.0f, yy, 0.0f});
+        }
+        if (inside >= 0.0f && inside <= reach) out.push_back(at);
+      }
+  return kOk;
+}
+
 // FluidReport (src/fluid/step.rs:28-49), iterations split by solve.
 struct Report {
   uint32_t substeps = 0;
@@ -468,6 +592,25 @@ class World {
     std::optional<std::pair<V, V>> bounds;
     std::vector<std::tuple<uint32_t, V, V>> containers;
   } walls;
+
+  // Coupled (src/fluid/couple.rs:15-32): wall points and probes in body
+  // axes, where the body was and how fast it went this step, what the
+  // damping did to it, how far its centre has gone.
+  struct Coupled {
+    uint32_t index, generation;
+    std::vector<Vec3<float>> points, probes;
+    float probe_spacing;
+    V center{}, velocity{}, impulse{}, angular{}, moved{};
+  };
+  std::vector<Coupled> coupled;
+  // Movers (src/fluid/couple.rs:34-47): every coupled body's points this
+  // step, sorted by (cell, owner, place in the previous order).
+  struct Movers {
+    std::vector<V> start, positions, velocities;
+    std::vector<uint32_t> owners;
+    Grid grid;
+  } movers;
+  Lists mover_neighbors;
 
   Dispatch dispatch;
   // called before (false) and after (true) every WCSPH substep with its
@@ -783,14 +926,15 @@ class World {
 
     R r = radius();
     R reach = r * r;  // powi(2)
-    std::vector<uint32_t> fcount(n), wcount(n);
+    std::vector<uint32_t> fcount(n), wcount(n), mcount(n);
     std::vector<std::vector<uint32_t>> fparts((n + Dispatch::kChunk - 1) / Dispatch::kChunk),
-        wparts(fparts.size());
+        wparts(fparts.size()), mparts(fparts.size());
     bool ok = dispatch.run(n, [&](size_t b, size_t e) {
-      std::vector<Range> af, aw;
+      std::vector<Range> af, aw, am;
       std::optional<CellKey> last;
       auto& fl = fparts[b / Dispatch::kChunk];
       auto& wl = wparts[b / Dispatch::kChunk];
+      auto& ml = mparts[b / Dispatch::kChunk];
       for (size_t i = b; i < e; ++i) {
         V at = positions[i];
         CellKey cell = cell_of(at);
@@ -798,12 +942,15 @@ class World {
           last = cell;
           grid.around(cell, af);
           walls.grid.around(cell, aw);
+          movers.grid.around(cell, am);
         }
-        size_t f0 = fl.size(), w0 = wl.size();
+        size_t f0 = fl.size(), w0 = wl.size(), m0 = ml.size();
         near(positions, af, at, reach, fl);
         near(walls.positions, aw, at, reach, wl);
+        near(movers.positions, am, at, reach, ml);
         fcount[i] = (uint32_t)(fl.size() - f0);
         wcount[i] = (uint32_t)(wl.size() - w0);
+        mcount[i] = (uint32_t)(ml.size() - m0);
       }
     });
     if (!ok) return kDispatchIncomplete;
@@ -815,12 +962,13 @@ class World {
     };
     join(neighbors, fcount, fparts);
     join(wall_neighbors, wcount, wparts);
+    join(mover_neighbors, mcount, mparts);
     neighbors_built = true;
     return kOk;
   }
 
-  // compute_densities (src/fluid/mod.rs:621-675); movers and ground sums are
-  // empty here, and adding an empty f32 sum to a positive one is exact.
+  // compute_densities (src/fluid/mod.rs:621-675), no terrains: the ground
+  // sum is empty, and adding it (+0) to a non-negative sum is exact.
   int32_t compute_densities() {
     if (!neighbors_built || neighbors.start.size() != len() + 1) return kOutOfRange;
     std::vector<R> out(len());
@@ -833,7 +981,10 @@ class World {
         R wsum = R(0);
         for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
           wsum += kernel.value(length(walls.positions[wall_neighbors.items[k]] - at));
-        out[i] = mass * (sum + wsum);
+        R msum = R(0);
+        for (uint32_t k = mover_neighbors.start[i]; k < mover_neighbors.start[i + 1]; ++k)
+          msum += kernel.value(length(movers.positions[mover_neighbors.items[k]] - at));
+        out[i] = mass * (sum + wsum + msum);
       }
     });
     if (!ok) return kDispatchIncomplete;
@@ -873,6 +1024,371 @@ class World {
     return most;
   }
 
+  // ---- coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) ----
+
+  size_t coupled_points() const {
+    size_t n = 0;
+    for (const Coupled& k : coupled) n += k.points.size();
+    return n;
+  }
+
+  // couple (src/fluid/couple.rs:165-198): wall points as deep as the kernel
+  // reaches, probes through the whole volume, coarser (x1.25 a time) until
+  // at most kMaxProbes; the bodies kept in handle order.
+  int32_t couple(uint32_t index, uint32_t generation, const Shape& shape) {
+    std::vector<Vec3<float>> points;
+    int32_t st = shape_points(shape, settings.spacing, settings.radius(), (size_t)settings.max_particles, points);
+    if (st != kOk) return st;
+    if (points.empty()) return kInvalidShape;
+    size_t used = coupled_points();
+    size_t room = settings.max_particles > used ? (size_t)settings.max_particles - used : 0;
+    if (points.size() > room) return kFluidFull;
+    float probe_spacing = settings.spacing;
+    std::vector<Vec3<float>> probes;
+    for (;;) {
+      st = shape_points(shape, probe_spacing, std::numeric_limits<float>::infinity(), kMaxProbes, probes);
+      if (st == kFluidFull) {
+        probe_spacing *= 1.25f;
+        continue;
+      }
+      if (st != kOk) return st;
+      if (probes.empty()) {
+        probe_spacing = settings.spacing;
+        probes = points;
+      }
+      break;
+    }
+    uncouple(index, generation);
+    Coupled k{index, generation, std::move(points), std::move(probes), probe_spacing};
+    coupled.push_back(std::move(k));
+    std::stable_sort(coupled.begin(), coupled.end(), [](const Coupled& a, const Coupled& b) {
+      return std::tie(a.index, a.generation) < std::tie(b.index, b.generation);
+    });
+    return kOk;
+  }
+
+  // uncouple (src/fluid/couple.rs:200-205)
+  bool uncouple(uint32_t index, uint32_t generation) {
+    size_t before = coupled.size();
+    coupled.erase(std::remove_if(coupled.begin(), coupled.end(),
+                                 [&](const Coupled& k) { return k.index == index && k.generation == generation; }),
+                  coupled.end());
+    return coupled.size() != before;
+  }
+
+  static const Pose* pose_of(const std::vector<Pose>& poses, uint32_t index, uint32_t generation) {
+    for (const Pose& p : poses)
+      if (p.index == index && p.generation == generation) return &p;
+    return nullptr;
+  }
+  static V center_of(const Pose& p) {  // position.to_dvec3().as_vec3()
+    return V{R((float)p.position[0]), R((float)p.position[1]), R((float)p.position[2])};
+  }
+  static V v3(const float f[3]) { return V{R(f[0]), R(f[1]), R(f[2])}; }
+
+  // place_movers_at (src/fluid/couple.rs:258-291): bodies without a pose
+  // are uncoupled; every point placed where its body is, moving with it.
+  void place_movers(const std::vector<Pose>& poses) {
+    coupled.erase(std::remove_if(coupled.begin(), coupled.end(),
+                                 [&](const Coupled& k) { return !pose_of(poses, k.index, k.generation); }),
+                  coupled.end());
+    movers = Movers{};
+    for (size_t owner = 0; owner < coupled.size(); ++owner) {
+      Coupled& k = coupled[owner];
+      const Pose& p = *pose_of(poses, k.index, k.generation);
+      V center = center_of(p);
+      k.center = center;
+      k.velocity = v3(p.linear);
+      k.impulse = V{R(0), R(0), R(0)};
+      k.angular = V{R(0), R(0), R(0)};
+      for (const Vec3<float>& local : k.points) {
+        V arm = rotate(p.rotation, cvt(local));
+        V at = center + arm;
+        V velocity = v3(p.linear) + cross(v3(p.angular), arm);
+        movers.start.push_back(at);
+        movers.positions.push_back(at);
+        movers.velocities.push_back(velocity);
+        movers.owners.push_back((uint32_t)owner);
+      }
+    }
+    carry_movers(R(0));
+  }
+
+  // carry_movers (src/fluid/couple.rs:293-319)
+  void carry_movers(R elapsed) {
+    size_t m = movers.start.size();
+    struct Key {
+      CellKey cell;
+      uint32_t owner, index;
+    };
+    std::vector<Key> order(m);
+    for (size_t i = 0; i < m; ++i) {
+      V at = movers.start[i] + movers.velocities[i] * elapsed;
+      movers.positions[i] = at;
+      order[i] = {cell_of(at), movers.owners[i], (uint32_t)i};
+    }
+    std::sort(order.begin(), order.end(), [](const Key& a, const Key& b) {
+      return std::tie(a.cell.x, a.cell.y, a.cell.z, a.owner, a.index) <
+             std::tie(b.cell.x, b.cell.y, b.cell.z, b.owner, b.index);
+    });
+    Movers next;
+    std::vector<CellKey> keys(m);
+    for (size_t k = 0; k < m; ++k) {
+      uint32_t i = order[k].index;
+      next.start.push_back(movers.start[i]);
+      next.positions.push_back(movers.positions[i]);
+      next.velocities.push_back(movers.velocities[i]);
+      next.owners.push_back(order[k].owner);
+      keys[k] = order[k].cell;
+    }
+    next.grid = Grid::build(keys.begin(), keys.end());
+    movers = std::move(next);
+    for (Coupled& k : coupled) k.moved = k.velocity * elapsed;
+  }
+
+  // fastest_near_water (src/fluid/couple.rs:321-366)
+  R fastest_near_water(R dt, R slow) const {
+    size_t count = coupled.size();
+    const R big = R(std::numeric_limits<float>::max());
+    std::vector<R> fast(count, R(0));
+    std::vector<std::pair<V, V>> boxes(count, {V{big, big, big}, V{-big, -big, -big}});
+    for (size_t i = 0; i < movers.positions.size(); ++i) {
+      uint32_t o = movers.owners[i];
+      fast[o] = rmax(fast[o], length(movers.velocities[i]));
+      boxes[o] = {vmin(boxes[o].first, movers.positions[i]), vmax(boxes[o].second, movers.positions[i])};
+    }
+    R water = fastest();
+    R r = radius();
+    R most = R(0);
+    for (size_t b = 0; b < count; ++b) {
+      R speed = fast[b];
+      if (speed <= most) continue;
+      if (speed <= slow) {
+        most = speed;
+        continue;
+      }
+      R reach = r + (speed + water) * dt;
+      V splat{reach, reach, reach};
+      CellKey lo = cell_of(boxes[b].first - splat), hi = cell_of(boxes[b].second + splat);
+      bool unsorted = grid.cells.empty() && !positions.empty();
+      bool wet = unsorted;
+      for (size_t c = 0; !wet && c < grid.cells.size(); ++c) {
+        const CellKey& k = grid.cells[c].first;
+        wet = k.x >= lo.x && k.x <= hi.x && k.y >= lo.y && k.y <= hi.y && k.z >= lo.z && k.z <= hi.z;
+      }
+      if (wet) most = speed;
+    }
+    return most;
+  }
+
+  // mover_push (src/fluid/couple.rs:368-408)
+  void mover_push(size_t index, R mirrored, R damping, V& push, std::vector<std::pair<V, V>>& bodies) const {
+    uint32_t b = mover_neighbors.start[index], e = mover_neighbors.start[index + 1];
+    if (b == e) return;
+    V at = positions[index], velocity = velocities[index];
+    R density = densities[index];
+    R r = radius();
+    R soften = R(0.01f) * r * r;
+    R settles = length(cvt(settings.gravity)) * R(settings.spacing);
+    for (uint32_t k = b; k < e; ++k) {
+      uint32_t point = mover_neighbors.items[k];
+      V offset = at - movers.positions[point];
+      V relative = velocity - movers.velocities[point];
+      R toward = dot(relative, offset);
+      R damp = toward < R(0) ? -damping * toward / (length_squared(offset) + soften) / density : R(0);
+      V gradient = kernel.gradient(offset) * mass;
+      push = push - gradient * (mirrored + damp);
+      if (toward * toward < settles * length_squared(offset)) continue;
+      V force = gradient * (damp * mass);
+      uint32_t owner = movers.owners[point];
+      const Coupled& known = coupled[owner];
+      V arm = movers.positions[point] - (known.center + known.moved);
+      bodies[owner].first = bodies[owner].first + force;
+      bodies[owner].second = bodies[owner].second + cross(arm, force);
+    }
+  }
+
+  // Lift (src/fluid/lift.rs:14-22)
+  struct Lift {
+    size_t body;
+    V impulse, angular;
+    std::vector<uint32_t> near;
+  };
+
+  // lifts (src/fluid/lift.rs:25-152): buoyancy from the probes' depth under
+  // the water's level around the body, drag from the flow past them, and
+  // damping on the waterplane; DRAG = 1, READ_REACH = 3 (lift.rs:9-12).
+  std::vector<Lift> lifts(R dt, const std::vector<Pose>& poses) const {
+    std::vector<Lift> out;
+    V g = cvt(settings.gravity);
+    R gravity = length(g);
+    V up = gravity > R(0) ? (-g) / gravity : V{R(0), R(1), R(0)};
+    R reach = R(3) * radius();
+    R column = radius();
+    const R big = R(std::numeric_limits<float>::max());
+    for (size_t b = 0; b < coupled.size(); ++b) {
+      const Coupled& known = coupled[b];
+      const Pose* pose = pose_of(poses, known.index, known.generation);
+      if (!pose) continue;
+      V center = center_of(*pose);
+      std::vector<std::pair<V, V>> probes;
+      for (const Vec3<float>& local : known.probes) {
+        V arm = rotate(pose->rotation, cvt(local));
+        probes.push_back({arm, v3(pose->linear) + cross(v3(pose->angular), arm)});
+      }
+      V low{big, big, big}, high{-big, -big, -big};
+      for (auto& pr : probes) {
+        low = vmin(low, center + pr.first);
+        high = vmax(high, center + pr.first);
+      }
+      V splat{reach, reach, reach};
+      low = low - splat;
+      high = high + splat;
+      std::vector<uint32_t> near;
+      for (size_t i = 0; i < len(); ++i)
+        if (clamp(positions[i], low, high) == positions[i]) near.push_back((uint32_t)i);
+      if (near.empty()) continue;
+      // the level: mean of the middle half of the columns' tops
+      struct Top {
+        int32_t kx, kz;
+        R height;
+      };
+      std::vector<Top> tops;
+      for (uint32_t i : near) {
+        V at = positions[i];
+        tops.push_back({sat_i32((double)std::floor(at.x / column)), sat_i32((double)std::floor(at.z / column)),
+                        dot(at, up)});
+      }
+      std::sort(tops.begin(), tops.end(), [](const Top& a, const Top& b) {
+        if (a.kx != b.kx) return a.kx < b.kx;
+        if (a.kz != b.kz) return a.kz < b.kz;
+        return total_key(a.height) > total_key(b.height);
+      });
+      std::vector<R> heights;
+      for (size_t t = 0; t < tops.size(); ++t)
+        if (t == 0 || tops[t].kx != tops[t - 1].kx || tops[t].kz != tops[t - 1].kz) heights.push_back(tops[t].height);
+      std::sort(heights.begin(), heights.end(), [](R a, R b) { return total_key(a) < total_key(b); });
+      size_t quarter = heights.size() / 4;
+      R sum = R(0);
+      for (size_t t = quarter; t < heights.size() - quarter; ++t) sum += heights[t];
+      R level = sum / R(heights.size() - 2 * quarter) + R(settings.spacing / 2.0f);
+      V flow{R(0), R(0), R(0)};
+      uint32_t wet = 0;
+      for (uint32_t i : near)
+        if (dot(positions[i], up) <= level) {
+          flow = flow + velocities[i];
+          wet += 1;
+        }
+      flow = flow / R(std::max<uint32_t>(wet, 1));
+      R spacing = R(known.probe_spacing);
+      struct Under {
+        V arm, velocity;
+        R share;
+      };
+      std::vector<Under> under;
+      for (auto& pr : probes) {
+        R depth = level - dot(center + pr.first, up);
+        R share = clamp_rust(depth / spacing + R(0.5f), R(0), R(1));
+        if (share > R(0)) under.push_back({pr.first, pr.second, share});
+      }
+      R wetted = R(0);
+      for (auto& u : under) wetted += u.share;
+      if (wetted <= R(0)) continue;
+      R volume = wetted * (spacing * (spacing * spacing));  // powi(3)
+      R density = R(settings.rest_density);
+      V middle{R(0), R(0), R(0)};
+      for (auto& u : under) middle = middle + u.arm * u.share;
+      middle = middle / wetted;
+      V force{R(0), R(0), R(0)}, torque{R(0), R(0), R(0)};
+      V buoyancy = up * (density * gravity * volume);
+      force = force + buoyancy;
+      torque = torque + cross(middle, buoyancy);
+      R area = R(std::pow(volume, R(2.0f / 3.0f))) / wetted;
+      for (auto& u : under) {
+        V past = flow - u.velocity;
+        V drag = past * (R(0.5f) * R(1) * density * area * u.share * length(past));
+        force = force + drag;
+        torque = torque + cross(u.arm, drag);
+      }
+      size_t line = 0;
+      for (auto& u : under) line += u.share < R(1) ? 1 : 0;
+      R plane = R(line) * spacing * spacing;
+      R waves = density * plane * std::sqrt(gravity * std::sqrt(plane));
+      R share = waves / R(std::max<size_t>(line, 1));
+      for (auto& u : under) {
+        if (!(u.share < R(1))) continue;
+        V calm = up * (share * dot(flow - u.velocity, up));
+        force = force + calm;
+        torque = torque + cross(u.arm, calm);
+      }
+      out.push_back({b, force * dt, torque * dt, std::move(near)});
+    }
+    return out;
+  }
+
+  // give_back (src/fluid/lift.rs:154-166)
+  void give_back(const Lift& lift) {
+    V change = lift.impulse / (mass * R(lift.near.size()));
+    for (uint32_t i : lift.near) velocities[i] = velocities[i] - change;
+  }
+
+  // step_with_bodies (src/fluid/couple.rs:207-252): the bodies stand where
+  // `poses` has them; each coupled body's lift (taken when it is dynamic,
+  // and given back by the water) and damping impulse go to `pushes`, in
+  // handle order. Poses must be finite and name each body once.
+  int32_t step_with_bodies(float dt, const std::vector<Pose>& poses, Report& report, std::vector<BodyPush>& pushes) {
+    pushes.clear();
+    report = Report{};
+    if (settings.solver != kWcsph && !coupled.empty()) return kInvalidSettings;
+    for (size_t a = 0; a < poses.size(); ++a) {
+      const Pose& p = poses[a];
+      bool finite = true;
+      for (int c = 0; c < 3; ++c)
+        finite = finite && std::isfinite(p.position[c]) && std::isfinite((float)p.position[c]) &&
+                 std::isfinite(p.linear[c]) && std::isfinite(p.angular[c]);
+      for (int c = 0; c < 4; ++c) finite = finite && std::isfinite(p.rotation[c]);
+      if (!finite) {
+        detail = "fluid body pose";
+        return kNonFinite;
+      }
+      for (size_t b = 0; b < a; ++b)
+        if (poses[b].index == p.index && poses[b].generation == p.generation) return kInvalidHandle;
+    }
+    place_movers(poses);
+    int32_t st = step(dt, report);
+    movers = Movers{};
+    if (st != kOk) return st;
+    for (const Coupled& k : coupled) {
+      BodyPush bp{};
+      bp.index = k.index;
+      bp.generation = k.generation;
+      pushes.push_back(bp);
+    }
+    if (dt > 0.0f) {
+      for (const Lift& lift : lifts(R(dt), poses)) {
+        const Coupled& k = coupled[lift.body];
+        if (!pose_of(poses, k.index, k.generation)->dynamic) continue;
+        BodyPush& bp = pushes[lift.body];
+        bp.lifted = 1;
+        float li[3] = {(float)lift.impulse.x, (float)lift.impulse.y, (float)lift.impulse.z};
+        float la[3] = {(float)lift.angular.x, (float)lift.angular.y, (float)lift.angular.z};
+        std::memcpy(bp.lift_impulse, li, sizeof li);
+        std::memcpy(bp.lift_angular, la, sizeof la);
+        give_back(lift);
+      }
+      for (size_t b = 0; b < coupled.size(); ++b) {
+        Coupled& k = coupled[b];
+        float im[3] = {(float)k.impulse.x, (float)k.impulse.y, (float)k.impulse.z};
+        float an[3] = {(float)k.angular.x, (float)k.angular.y, (float)k.angular.z};
+        std::memcpy(pushes[b].impulse, im, sizeof im);
+        std::memcpy(pushes[b].angular, an, sizeof an);
+        k.impulse = V{R(0), R(0), R(0)};
+        k.angular = V{R(0), R(0), R(0)};
+      }
+    }
+    return kOk;
+  }
+
   // step (src/fluid/step.rs:58-90)
   int32_t step(float dt, Report& report) {
     report = Report{};
@@ -899,12 +1415,23 @@ class World {
     return kOk;
   }
 
-  // wcsph / wcsph_steps (src/fluid/step.rs:102-142); no coupled bodies, so
-  // the sound speed is the configured one.
+  // wcsph (src/fluid/step.rs:102-116): the sound speed of the step is
+  // raised to IMPACT_SOUND (4, step.rs:20) times the fastest coupled body
+  // near water.
   int32_t wcsph(float dt_in, Report& report) {
+    float configured = settings.sound_speed;
+    R mover = fastest_near_water(R(dt_in), R(configured) / R(4));
+    settings.sound_speed = (float)rmax(R(configured), R(4) * mover);
     report.sound_speed = settings.sound_speed;
+    int32_t st = wcsph_steps(dt_in, mover, report);
+    settings.sound_speed = configured;
+    return st;
+  }
+
+  // wcsph_steps (src/fluid/step.rs:118-142)
+  int32_t wcsph_steps(float dt_in, R mover, Report& report) {
     R dt = R(dt_in);
-    R fast = fastest();
+    R fast = rmax(fastest(), mover);
     R largest = R(settings.courant) * radius() / (R(settings.sound_speed) + fast);
     R wanted = rmax(std::ceil(dt / largest), R(1));
     R cap = R(settings.max_substeps);
@@ -913,6 +1440,7 @@ class World {
     R small = dt / R(substeps);
     for (uint32_t s = 0; s < substeps; ++s) {
       uint32_t held = 0;
+      if (!coupled.empty()) carry_movers(small * R(s));
       if (substep_hook) substep_hook(false, small);
       int32_t st = wcsph_substep(small, held);
       if (st != kOk) return st;
@@ -1000,8 +1528,14 @@ class World {
     R nu = viscosity(dt);
     for (size_t i = 0; i < n; ++i) scaled[i] = pressures[i] / (densities[i] * densities[i]);
     V gravity = cvt(settings.gravity);
+    // step.rs:163-164
+    R damping = rmin(R(1) * c * r, R(0.25f) * r * r / dt);
     std::vector<V> pushes(n);
+    size_t bodies = coupled.size();
+    std::vector<std::vector<std::pair<V, V>>> felt((n + Dispatch::kChunk - 1) / Dispatch::kChunk);
     bool ok = dispatch.run(n, [&](size_t b, size_t e) {
+      auto& mine = felt[b / Dispatch::kChunk];
+      mine.assign(bodies, {V{R(0), R(0), R(0)}, V{R(0), R(0), R(0)}});
       for (size_t i = b; i < e; ++i) {
         V at = positions[i], velocity = velocities[i];
         R density = densities[i];
@@ -1024,11 +1558,18 @@ class World {
           V offset = at - walls.positions[wall_neighbors.items[k]];
           push = push - kernel.gradient(offset) * (mass * mirrored);
         }
+        mover_push(i, mirrored, damping, push, mine);
         pushes[i] = push + wall_viscous(i, dt) + gravity;
       }
     });
     if (!ok) return kDispatchIncomplete;
     for (size_t i = 0; i < n; ++i) velocities[i] = velocities[i] + pushes[i] * dt;
+    // each chunk's damping on the bodies, in chunk order (step.rs:213-221)
+    for (auto& part : felt)
+      for (size_t k = 0; k < bodies; ++k) {
+        coupled[k].impulse = coupled[k].impulse + part[k].first * dt;
+        coupled[k].angular = coupled[k].angular + part[k].second * dt;
+      }
     forces = pushes;
     return advance(dt, held);
   }

@@ -20,8 +20,8 @@ from typing import Optional
 
 import numpy as np
 
-from solunar_amd.types import (CReport, CSettings, FluidReport, FluidSettings,
-                               PhysicsError, SnapshotError)
+from solunar_amd.types import (CBodyPose, CBodyPush, CReport, CSettings, CShape, FluidReport, FluidSettings,
+                               PhysicsError, Shape, SnapshotError, pose_array)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_fluid.so")
@@ -94,6 +94,17 @@ def library() -> ctypes.CDLL:
         "orc_tile_tree_sum": (ctypes.c_double, [fp, ctypes.POINTER(i32), sz]),
         "orc_grid_selftest": (i32, []),
         "orc_plant_mistake": (None, [i32]),
+        "orc_couple": (i32, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(CShape)]),
+        "orc_uncouple": (i32, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(i32)]),
+        "orc_coupled_len": (sz, [P]),
+        "orc_coupled_info": (i32, [P, sz, u32p, u32p, ctypes.POINTER(sz), ctypes.POINTER(sz), fp]),
+        "orc_read_coupled_points": (i32, [P, sz, i32, sz, fp]),
+        "orc_place_movers": (i32, [P, ctypes.POINTER(CBodyPose), sz]),
+        "orc_mover_len": (sz, [P]),
+        "orc_read_movers": (i32, [P, sz, fp, fp, u32p]),
+        "orc_lifts": (i32, [P, f, ctypes.POINTER(CBodyPose), sz, ctypes.POINTER(CBodyPush), sz, ctypes.POINTER(sz)]),
+        "orc_step_with_bodies": (i32, [P, f, ctypes.POINTER(CBodyPose), sz, ctypes.POINTER(CReport),
+                                       ctypes.POINTER(CBodyPush), sz, ctypes.POINTER(sz)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -289,6 +300,66 @@ class OracleWorld:
         w.set_residual_mode(residual_mode)
         return w
 
+    # ---- coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) ----
+
+    def couple(self, body, shape: Shape) -> None:
+        """Lets water and `body` (index, generation) push each other (couple.rs:165-198)."""
+        c = CShape.of(shape)
+        self._check(self._lib.orc_couple(self._h, int(body[0]), int(body[1]), ctypes.byref(c)))
+
+    def uncouple(self, body) -> bool:
+        o = ctypes.c_int32()
+        self._check(self._lib.orc_uncouple(self._h, int(body[0]), int(body[1]), ctypes.byref(o)))
+        return bool(o.value)
+
+    def coupled(self):
+        """[(body, points, probes, probe_spacing)] in handle order, points in body axes."""
+        out = []
+        for k in range(int(self._lib.orc_coupled_len(self._h))):
+            i, g = ctypes.c_uint32(), ctypes.c_uint32()
+            n, m, sp = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_float()
+            self._check(self._lib.orc_coupled_info(self._h, k, ctypes.byref(i), ctypes.byref(g), ctypes.byref(n),
+                                                   ctypes.byref(m), ctypes.byref(sp)))
+            pts = np.zeros((n.value, 3), np.float32)
+            prb = np.zeros((m.value, 3), np.float32)
+            self._check(self._lib.orc_read_coupled_points(self._h, k, 0, n.value, _p(pts, ctypes.c_float)))
+            self._check(self._lib.orc_read_coupled_points(self._h, k, 1, m.value, _p(prb, ctypes.c_float)))
+            out.append(((i.value, g.value), pts, prb, sp.value))
+        return out
+
+    def place_movers(self, poses) -> None:
+        arr = pose_array(poses)
+        self._check(self._lib.orc_place_movers(self._h, arr, len(poses)))
+
+    def movers(self):
+        """(positions, velocities, owners) of the placed body points, in sorted order."""
+        n = int(self._lib.orc_mover_len(self._h))
+        x, v = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        o = np.zeros(n, np.uint32)
+        self._check(self._lib.orc_read_movers(self._h, n, _p(x, ctypes.c_float), _p(v, ctypes.c_float),
+                                              _p(o, ctypes.c_uint32)))
+        return x, v, o
+
+    def lifts(self, dt: float, poses):
+        """lift.rs:25-152 over the present state, given back to no one."""
+        arr = pose_array(poses)
+        cap = int(self._lib.orc_coupled_len(self._h))
+        out = (CBodyPush * max(cap, 1))()
+        n = ctypes.c_size_t()
+        self._check(self._lib.orc_lifts(self._h, dt, arr, len(poses), out, cap, ctypes.byref(n)))
+        return [out[k].to_push() for k in range(n.value)]
+
+    def step_with_bodies(self, dt: float, poses):
+        """step_with_bodies (couple.rs:207-252): (report, [BodyPush] in handle order)."""
+        arr = pose_array(poses)
+        cap = int(self._lib.orc_coupled_len(self._h))
+        out = (CBodyPush * max(cap, 1))()
+        n = ctypes.c_size_t()
+        r = CReport()
+        self._check(self._lib.orc_step_with_bodies(self._h, dt, arr, len(poses), ctypes.byref(r), out, cap,
+                                                   ctypes.byref(n)))
+        return r.to_report(), [out[k].to_push() for k in range(n.value)]
+
     def run_flows(self):
         """Drains then emitters once, without a step: (emitted, drained)."""
         e, d = ctypes.c_uint32(), ctypes.c_uint32()
@@ -376,6 +447,9 @@ class OracleWorld:
 
     def wall_neighbors(self):
         return self._lists(1)
+
+    def mover_neighbors(self):
+        return self._lists(2)
 
     def neighbors_of(self, index: int) -> np.ndarray:
         s, it = self.neighbors()

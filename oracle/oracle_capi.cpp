@@ -275,7 +275,7 @@ int32_t orc_save_snapshot(void* w, uint8_t* out, size_t cap, size_t* len) {
 }
 
 // load_snapshot (src/fluid/save.rs:68-195): *kind 0 and a world, or the
-// refusal (1..6 as SnapshotError, 101 for coupled bodies or terrain)
+// refusal (1..6 as SnapshotError, 101 for terrain)
 int32_t orc_load_snapshot(const uint8_t* data, size_t len, const orc_settings* base, int32_t precision, void** out,
                           int32_t* kind, uint32_t* version, const char** what) {
   *out = nullptr;
@@ -298,6 +298,127 @@ int32_t orc_load_snapshot(const uint8_t* data, size_t len, const orc_settings* b
   }
   *out = h;
   return kOk;
+}
+
+// ---- coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) ----
+// orc_shape, Pose and BodyPush have the layouts of sfl_shape, sfl_body_pose
+// and sfl_body_push.
+struct orc_shape {
+  int32_t kind;
+  float a, b, c;
+};
+
+int32_t orc_couple(void* w, uint32_t index, uint32_t generation, const orc_shape* shape) {
+  Shape sh{shape->kind, shape->a, shape->b, shape->c};
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.couple(index, generation, sh); });
+}
+
+int32_t orc_uncouple(void* w, uint32_t index, uint32_t generation, int32_t* was) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    bool r = x.uncouple(index, generation);
+    if (was) *was = r ? 1 : 0;
+    return (int32_t)kOk;
+  });
+}
+
+size_t orc_coupled_len(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.coupled.size(); });
+}
+
+// body k: handle, point and probe counts, probe spacing
+int32_t orc_coupled_info(void* w, size_t k, uint32_t* index, uint32_t* generation, size_t* points, size_t* probes,
+                         float* probe_spacing) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    if (k >= x.coupled.size()) return (int32_t)kOutOfRange;
+    const auto& c = x.coupled[k];
+    *index = c.index;
+    *generation = c.generation;
+    *points = c.points.size();
+    *probes = c.probes.size();
+    *probe_spacing = c.probe_spacing;
+    return (int32_t)kOk;
+  });
+}
+
+// body k's points (which 0) or probes (which 1) in body axes
+int32_t orc_read_coupled_points(void* w, size_t k, int32_t which, size_t cap, float* xyz) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    if (k >= x.coupled.size()) return (int32_t)kOutOfRange;
+    const auto& v = which ? x.coupled[k].probes : x.coupled[k].points;
+    for (size_t i = 0; i < std::min(cap, v.size()); ++i) {
+      xyz[3 * i] = v[i].x;
+      xyz[3 * i + 1] = v[i].y;
+      xyz[3 * i + 2] = v[i].z;
+    }
+    return (int32_t)kOk;
+  });
+}
+
+// place_movers_at alone, for tests of the placement and the sort
+int32_t orc_place_movers(void* w, const Pose* poses, size_t n) {
+  std::vector<Pose> p(poses, poses + n);
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    x.place_movers(p);
+    return (int32_t)kOk;
+  });
+}
+
+size_t orc_mover_len(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.movers.positions.size(); });
+}
+
+int32_t orc_read_movers(void* w, size_t cap, float* xyz, float* vxyz, uint32_t* owners) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    size_t n = std::min(cap, x.movers.positions.size());
+    for (size_t i = 0; i < n; ++i) {
+      const auto& p = x.movers.positions[i];
+      const auto& v = x.movers.velocities[i];
+      float a[6] = {(float)p.x, (float)p.y, (float)p.z, (float)v.x, (float)v.y, (float)v.z};
+      std::memcpy(xyz + 3 * i, a, 12);
+      std::memcpy(vxyz + 3 * i, a + 3, 12);
+      owners[i] = x.movers.owners[i];
+    }
+    return (int32_t)kOk;
+  });
+}
+
+// lifts (src/fluid/lift.rs:25-152) over the present state, nothing given
+// back: one entry per coupled body, lifted 1 where the water holds a lift
+int32_t orc_lifts(void* w, float dt, const Pose* poses, size_t n, BodyPush* out, size_t cap, size_t* count) {
+  std::vector<Pose> p(poses, poses + n);
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    *count = x.coupled.size();
+    if (cap < x.coupled.size()) return (int32_t)kOutOfRange;
+    for (size_t b = 0; b < x.coupled.size(); ++b) {
+      out[b] = BodyPush{};
+      out[b].index = x.coupled[b].index;
+      out[b].generation = x.coupled[b].generation;
+    }
+    for (const auto& lift : x.lifts(dt, p)) {
+      BodyPush& bp = out[lift.body];
+      bp.lifted = 1;
+      float v[6] = {(float)lift.impulse.x, (float)lift.impulse.y, (float)lift.impulse.z,
+                    (float)lift.angular.x, (float)lift.angular.y, (float)lift.angular.z};
+      std::memcpy(bp.lift_impulse, v, 12);
+      std::memcpy(bp.lift_angular, v + 3, 12);
+    }
+    return (int32_t)kOk;
+  });
+}
+
+int32_t orc_step_with_bodies(void* w, float dt, const Pose* poses, size_t n, orc_report* report, BodyPush* out,
+                             size_t cap, size_t* count) {
+  std::vector<Pose> p(poses, poses + n);
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    Report r;
+    std::vector<BodyPush> pushes;
+    int32_t st = x.step_with_bodies(dt, p, r, pushes);
+    if (st != kOk) return st;
+    fill_report(r, report);
+    *count = pushes.size();
+    for (size_t b = 0; b < std::min(cap, pushes.size()); ++b) out[b] = pushes[b];
+    return (int32_t)kOk;
+  });
 }
 
 // run_flows alone (the reference's tests call it without a step)
@@ -416,7 +537,7 @@ int32_t orc_set_particles(void* w, size_t n, const uint32_t* ids, const float* x
 int32_t orc_read_neighbors(void* w, int32_t kind, size_t start_cap, uint32_t* start, size_t items_cap,
                            uint32_t* items, size_t* items_len) {
   return with(static_cast<Handle*>(w), [&](auto& x) {
-    const Lists& l = kind == 0 ? x.neighbors : x.wall_neighbors;
+    const Lists& l = kind == 0 ? x.neighbors : (kind == 1 ? x.wall_neighbors : x.mover_neighbors);
     if (items_len) *items_len = l.items.size();
     if (!x.neighbors_built) return (int32_t)kOutOfRange;
     if (start_cap < l.start.size() || items_cap < l.items.size()) return (int32_t)kOutOfRange;

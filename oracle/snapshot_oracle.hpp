@@ -23,7 +23,7 @@ enum SnapshotKind : int32_t {
   kSnapUnsupportedVersion = 4,
   kSnapBadChecksum = 5,
   kSnapInvalid = 6,
-  kSnapUnsupported = 101,  // coupled bodies or terrain: the oracle does not carry them
+  kSnapUnsupported = 101,  // terrain: the oracle does not carry it
 };
 
 struct SnapshotResult {
@@ -108,7 +108,7 @@ struct SnapReader {
   }
 };
 
-// What a snapshot holds (no coupled bodies, no terrain).
+// What a snapshot holds (no terrain).
 struct SnapshotState {
   Settings settings;
   std::vector<uint32_t> ids;
@@ -128,6 +128,12 @@ struct SnapshotState {
   uint32_t flow_next = 0;
   bool has_domain = false;
   Vec3<float> domain_low{0, 0, 0}, domain_high{0, 0, 0};
+  struct CoupledRec {
+    uint32_t index, generation;
+    std::vector<Vec3<float>> points, probes;
+    float probe_spacing;
+  };
+  std::vector<CoupledRec> coupled;
 };
 
 inline bool snap_box_ok(Vec3<float> lo, Vec3<float> hi) {
@@ -179,7 +185,17 @@ std::vector<uint8_t> save_snapshot(const World<R>& w) {
     }
   }
   out.u32(w.flow_next);
-  out.u32(0);  // coupled
+  // Coupled::save (src/fluid/couple.rs:50-55)
+  out.u32((uint32_t)w.coupled.size());
+  for (const auto& k : w.coupled) {
+    out.u32(k.index);
+    out.u32(k.generation);
+    out.u32((uint32_t)k.points.size());
+    for (const auto& p : k.points) out.vec3(p);
+    out.f32(k.probe_spacing);
+    out.u32((uint32_t)k.probes.size());
+    for (const auto& p : k.probes) out.vec3(p);
+  }
   out.u32(0);  // terrains
   if (w.has_domain) {
     out.u8(1);
@@ -285,9 +301,46 @@ inline SnapshotResult parse_snapshot(const uint8_t* data, size_t len, const Sett
   if (!r.u32(st.flow_next)) return res;
   for (const auto& fl : st.flows)
     if (fl.id >= st.flow_next) return {kSnapInvalid, 0, "fluid flow id"};
+  // Coupled::load (src/fluid/couple.rs:57-75), save.rs:158-171
   size_t coupled = 0;
   if (!r.count(12, uint64_t(1) << 16, coupled)) return res;
-  if (coupled) return {kSnapUnsupported, 0, "fluid coupled bodies"};
+  uint64_t room = max;
+  auto points = [&](uint64_t most, std::vector<Vec3<float>>& out) -> int {
+    size_t n = 0;
+    if (!r.count(12, most, n)) return 1;
+    for (size_t k = 0; k < n; ++k) {
+      Vec3<float> p;
+      if (!r.vec3(p)) return 1;
+      if (!p.finite()) return 2;
+      out.push_back(p);
+    }
+    return out.empty() ? 3 : 0;
+  };
+  for (size_t b = 0; b < coupled; ++b) {
+    SnapshotState::CoupledRec k{};
+    if (!r.u32(k.index) || !r.u32(k.generation)) return res;
+    int e = points(room, k.points);
+    if (e == 1) return res;
+    if (e == 2) return {kSnapInvalid, 0, "fluid body point"};
+    if (e == 3) return {kSnapInvalid, 0, "fluid body"};
+    if (version >= 3) {
+      if (!r.f32(k.probe_spacing)) return res;
+      if (!std::isfinite(k.probe_spacing) || k.probe_spacing <= 0.0f) return {kSnapInvalid, 0, "fluid body probes"};
+      e = points(kMaxProbes, k.probes);
+      if (e == 1) return res;
+      if (e == 2) return {kSnapInvalid, 0, "fluid body point"};
+      if (e == 3) return {kSnapInvalid, 0, "fluid body"};
+    } else {
+      k.probe_spacing = s.spacing;
+      k.probes = k.points;
+    }
+    room -= k.points.size();
+    st.coupled.push_back(std::move(k));
+  }
+  for (size_t b = 1; b < st.coupled.size(); ++b)
+    if (std::tie(st.coupled[b - 1].index, st.coupled[b - 1].generation) >=
+        std::tie(st.coupled[b].index, st.coupled[b].generation))
+      return {kSnapInvalid, 0, "fluid body order"};
   size_t terrains = 0;
   if (!r.count(32, uint64_t(1) << 16, terrains)) return res;
   if (terrains) return {kSnapUnsupported, 0, "fluid terrain"};
@@ -334,6 +387,11 @@ void apply_snapshot(World<R>& w, const SnapshotState& st) {
   w.has_domain = st.has_domain;
   w.domain_low = st.domain_low;
   w.domain_high = st.domain_high;
+  w.coupled.clear();
+  for (const auto& k : st.coupled) {
+    typename World<R>::Coupled c{k.index, k.generation, k.points, k.probes, k.probe_spacing};
+    w.coupled.push_back(std::move(c));
+  }
   w.neighbors_built = false;
   w.settle_walls();
 }

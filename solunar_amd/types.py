@@ -35,7 +35,9 @@ STATUS_NAMES = {
     100: "DeviceError",
     101: "Unsupported",
 }
+INVALID_HANDLE = 1
 NON_FINITE = 4
+INVALID_SHAPE = 5
 INVALID_SETTINGS = 7
 OUT_OF_RANGE = 10
 DISPATCH_INCOMPLETE = 11
@@ -230,3 +232,119 @@ class CBatchTuning(ctypes.Structure):
             "density_ms": [float(x) for x in self.density_ms[:n]],
             "force_ms": [float(x) for x in self.force_ms[:n]],
         }
+
+
+# ---- coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) ----
+
+CUBOID = 0
+SPHERE = 1
+CAPSULE = 2
+
+
+@dataclass(frozen=True)
+class Shape:
+    """A shape a body couples with (src/shape.rs:8-49): a cuboid (a, b, c its
+    half extents), a sphere (a its radius), a capsule along local y (a its
+    radius, b the half height of its cylinder)."""
+
+    kind: int
+    a: float
+    b: float = 0.0
+    c: float = 0.0
+
+    @staticmethod
+    def cuboid(half_extents) -> "Shape":
+        x, y, z = half_extents
+        return Shape(CUBOID, float(x), float(y), float(z))
+
+    @staticmethod
+    def sphere(radius: float) -> "Shape":
+        return Shape(SPHERE, float(radius))
+
+    @staticmethod
+    def capsule(radius: float, half_height: float) -> "Shape":
+        return Shape(CAPSULE, float(radius), float(half_height))
+
+
+class CShape(ctypes.Structure):
+    """`sfl_shape` of include/solunar_fluid.h."""
+
+    _fields_ = [("kind", ctypes.c_int32), ("a", ctypes.c_float), ("b", ctypes.c_float), ("c", ctypes.c_float)]
+
+    @classmethod
+    def of(cls, s: Shape) -> "CShape":
+        return cls(s.kind, s.a, s.b, s.c)
+
+
+@dataclass
+class BodyPose:
+    """Where the rigid world has a coupled body for a step (SubstepPose,
+    src/substep.rs): `body` is its handle (index, generation), rotation a
+    quaternion (x, y, z, w); `dynamic` says whether it takes pushes."""
+
+    body: Tuple[int, int]
+    position: Tuple[float, float, float]
+    rotation: Tuple[float, float, float, float] = (0.0, 0.0, 0.0, 1.0)
+    linear_velocity: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    angular_velocity: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    dynamic: bool = True
+
+
+class CBodyPose(ctypes.Structure):
+    """`sfl_body_pose` of include/solunar_fluid.h."""
+
+    _fields_ = [
+        ("index", ctypes.c_uint32),
+        ("generation", ctypes.c_uint32),
+        ("position", ctypes.c_double * 3),
+        ("rotation", ctypes.c_float * 4),
+        ("linear_velocity", ctypes.c_float * 3),
+        ("angular_velocity", ctypes.c_float * 3),
+        ("dynamic", ctypes.c_int32),
+    ]
+
+
+def pose_array(poses):
+    """A ctypes array of CBodyPose from BodyPose values."""
+    arr = (CBodyPose * max(len(poses), 1))()
+    for k, p in enumerate(poses):
+        c = arr[k]
+        c.index, c.generation = int(p.body[0]), int(p.body[1])
+        c.position[:] = [float(v) for v in p.position]
+        c.rotation[:] = [float(v) for v in p.rotation]
+        c.linear_velocity[:] = [float(v) for v in p.linear_velocity]
+        c.angular_velocity[:] = [float(v) for v in p.angular_velocity]
+        c.dynamic = 1 if p.dynamic else 0
+    return arr
+
+
+@dataclass
+class BodyPush:
+    """What the water gave a coupled body over a step, as impulses: the lift
+    of src/fluid/lift.rs (set when `lifted`, the body being dynamic) and the
+    damping of mover_push (src/fluid/couple.rs:368-408)."""
+
+    body: Tuple[int, int]
+    lifted: bool
+    lift_impulse: Tuple[float, float, float]
+    lift_angular: Tuple[float, float, float]
+    impulse: Tuple[float, float, float]
+    angular: Tuple[float, float, float]
+
+
+class CBodyPush(ctypes.Structure):
+    """`sfl_body_push` of include/solunar_fluid.h."""
+
+    _fields_ = [
+        ("index", ctypes.c_uint32),
+        ("generation", ctypes.c_uint32),
+        ("lifted", ctypes.c_int32),
+        ("lift_impulse", ctypes.c_float * 3),
+        ("lift_angular", ctypes.c_float * 3),
+        ("impulse", ctypes.c_float * 3),
+        ("angular", ctypes.c_float * 3),
+    ]
+
+    def to_push(self) -> BodyPush:
+        return BodyPush((self.index, self.generation), bool(self.lifted), tuple(self.lift_impulse),
+                        tuple(self.lift_angular), tuple(self.impulse), tuple(self.angular))

@@ -117,12 +117,13 @@ def test_the_domain_and_the_flows_travel_with_the_snapshot():
     assert e.value == SnapshotError(6, 0, "fluid domain flag")
 
 
-def test_a_snapshot_with_coupled_bodies_is_not_carried():
+def test_a_coupled_body_without_points_is_refused():
     b = bytearray(small().save_snapshot()[:-8])
     # the coupled count sits after the flows: 4 + 4 + 1 (no domain flag set) from the end
     assert b[-9:] == b"\0" * 9
     b[-9:-5] = struct.pack("<I", 1)
-    b += b"\0" * 12
+    b[-5:-5] = b"\0" * 12  # handle (0, 0), no points
     b += struct.pack("<Q", fnv1a(bytes(b)))
-    with pytest.raises((PhysicsError, SnapshotError)):
+    with pytest.raises(SnapshotError) as e:
         OracleWorld.load_snapshot(bytes(b))
+    assert e.value == SnapshotError(6, 0, "fluid body")
