@@ -179,24 +179,97 @@ typedef struct sfl_snapshot_error {
 
 /* FluidWorld::save_snapshot (src/fluid/save.rs:17-66): the SLFL version 3
  * bytes of the world (settings, particles in order, next id, walls, flows,
- * domain, FNV-1a checksum), the same bytes the reference writes. With `out`
- * null, only *len_out is set; with `cap` too small, *len_out is set and
- * SFL_OUT_OF_RANGE returned. Worlds here hold no coupled bodies or terrain,
- * so those counts are 0. */
+ * coupled bodies, domain, FNV-1a checksum), the same bytes the reference
+ * writes. With `out` null, only *len_out is set; with `cap` too small,
+ * *len_out is set and SFL_OUT_OF_RANGE returned. Worlds here hold no
+ * terrain, so that count is 0. */
 int32_t sfl_save_snapshot(sfl_world* world, uint8_t* out, size_t cap,
                           size_t* len_out);
 
 /* FluidWorld::load_snapshot (src/fluid/save.rs:68-195): a new world from
  * untrusted bytes, checked as the reference checks them (size, magic,
  * checksum, version 1..3, settings, counts against the bytes left, ids,
- * walls, flows, domain, no trailing bytes). A refused snapshot returns
- * SFL_SNAPSHOT_REFUSED with *error_out filled; one that holds coupled bodies
- * or terrain returns SFL_UNSUPPORTED. Settings the snapshot does not hold
+ * walls, flows, coupled bodies, domain, no trailing bytes). A refused
+ * snapshot returns SFL_SNAPSHOT_REFUSED with *error_out filled; one that
+ * holds terrain returns SFL_UNSUPPORTED. Settings the snapshot does not hold
  * (the extensions, device included) come from `base` (defaults when null).
  * Densities start at the rest density, as the reference's do. */
 int32_t sfl_load_snapshot(const uint8_t* data, size_t len,
                           const sfl_settings* base, sfl_world** world_out,
                           sfl_snapshot_error* error_out);
+
+/* ---- coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) ---- */
+
+/* The shapes a body couples with (src/shape.rs:8-49): a cuboid (a, b, c its
+ * half extents), a sphere (a its radius), a capsule along local y (a its
+ * radius, b the half height of its cylinder). */
+enum sfl_shape_kind { SFL_SHAPE_CUBOID = 0, SFL_SHAPE_SPHERE = 1, SFL_SHAPE_CAPSULE = 2 };
+typedef struct sfl_shape {
+  int32_t kind;
+  float a, b, c;
+} sfl_shape;
+
+/* Where the rigid world has a coupled body for one step (SubstepPose,
+ * src/substep.rs): the body's handle (BodyHandle::to_raw, src/body.rs:18-22),
+ * its centre in f64 (rounded to f32 as the reference rounds it), rotation as
+ * a quaternion (x, y, z, w), velocities, and whether the body is dynamic,
+ * i.e. takes the water's pushes (PhysicsWorld::apply_force fails on fixed
+ * and kinematic bodies). */
+typedef struct sfl_body_pose {
+  uint32_t index, generation;
+  double position[3];
+  float rotation[4];
+  float linear_velocity[3];
+  float angular_velocity[3];
+  int32_t dynamic;
+} sfl_body_pose;
+
+/* What the water gave one coupled body over a step, as impulses: the lift of
+ * src/fluid/lift.rs:25-152 (lifted = 1 when the body is dynamic and the water
+ * held a lift for it, which the water then gave back, lift.rs:154-166) and
+ * the damping summed by mover_push (src/fluid/couple.rs:368-408). The
+ * reference applies them as forces, impulse / dt, in step_with_bodies
+ * (couple.rs:236-251); a substep hook (couple.rs:128-158) pushes them as
+ * impulses. */
+typedef struct sfl_body_push {
+  uint32_t index, generation;
+  int32_t lifted;
+  float lift_impulse[3];
+  float lift_angular[3];
+  float impulse[3];
+  float angular[3];
+} sfl_body_push;
+
+/* FluidWorld::couple (src/fluid/couple.rs:165-198): water and the body push
+ * each other in sfl_step_with_bodies. Its wall points are the shape's
+ * lattice at the water's spacing as deep as the kernel reaches, its probes
+ * the lattice through its whole volume, coarser by 1.25 a time until at most
+ * 4096. SFL_INVALID_SHAPE for another kind, a non-finite or non-positive
+ * size, or a shape too small for a point; SFL_FLUID_FULL past max_particles
+ * body points. Coupling a coupled body again replaces its shape. */
+int32_t sfl_couple(sfl_world* world, uint32_t index, uint32_t generation,
+                   const sfl_shape* shape);
+/* FluidWorld::uncouple (couple.rs:200-205): *was_out = 0 for a body that was
+ * not coupled. */
+int32_t sfl_uncouple(sfl_world* world, uint32_t index, uint32_t generation,
+                     int32_t* was_out);
+/* Coupled bodies, in handle order. */
+size_t sfl_coupled_len(const sfl_world* world);
+
+/* FluidWorld::step_with_bodies (src/fluid/couple.rs:207-252) with the rigid
+ * world's part handed in: the coupled bodies stand where `poses` has them
+ * (a coupled body no pose names is uncoupled, as a body gone from the world
+ * is), their points move with them through the substeps, the step's sound
+ * speed follows the fastest body near water (step.rs:102-116), and the water
+ * feels them as walls. pushes[k] gets coupled body k's impulses, in handle
+ * order; *push_count_out the count (SFL_OUT_OF_RANGE, nothing stepped, when
+ * `push_cap` is smaller than the coupled bodies). SFL_INVALID_SETTINGS for a
+ * DFSPH world with coupled bodies, SFL_NON_FINITE for a non-finite pose,
+ * SFL_INVALID_HANDLE for a body named twice; otherwise as sfl_step. */
+int32_t sfl_step_with_bodies(sfl_world* world, float dt,
+                             const sfl_body_pose* poses, size_t pose_count,
+                             sfl_report* report_out, sfl_body_push* pushes,
+                             size_t push_cap, size_t* push_count_out);
 
 /* add_box_container / add_wall (src/fluid/boundary.rs:41-91). */
 int32_t sfl_add_box_container(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
@@ -269,7 +342,8 @@ enum sfl_phase {
   SFL_PHASE_VISCOUS = 6,  /* DFSPH viscosity + gravity */
   SFL_PHASE_ADVANCE = 7,  /* DFSPH advance */
   SFL_PHASE_FLOWS = 8,    /* drains, emitters and the domain after a step */
-  SFL_PHASE_COUNT = 9
+  SFL_PHASE_BODIES = 9,   /* coupled bodies: carry, damping sums, lifts */
+  SFL_PHASE_COUNT = 10
 };
 int32_t sfl_profile(sfl_world* world, int32_t enable);
 /* ms_out[SFL_PHASE_COUNT] summed milliseconds, launches_out[SFL_PHASE_COUNT]

@@ -175,6 +175,28 @@ int32_t sfl_remove_flow(sfl_world* world, uint32_t flow, int32_t* removed_out) {
   return guarded(world, [&](sfl::World& w) { return w.remove_flow(flow, removed_out); });
 }
 
+int32_t sfl_couple(sfl_world* world, uint32_t index, uint32_t generation, const sfl_shape* shape) {
+  return guarded(world, [&](sfl::World& w) {
+    if (!shape) return (int32_t)SFL_INVALID_SHAPE;
+    return w.couple(index, generation, *shape);
+  });
+}
+
+int32_t sfl_uncouple(sfl_world* world, uint32_t index, uint32_t generation, int32_t* was_out) {
+  return guarded(world, [&](sfl::World& w) { return w.uncouple(index, generation, was_out); });
+}
+
+size_t sfl_coupled_len(const sfl_world* world) { return world && world->w ? world->w->coupled_len() : 0; }
+
+int32_t sfl_step_with_bodies(sfl_world* world, float dt, const sfl_body_pose* poses, size_t pose_count,
+                             sfl_report* report_out, sfl_body_push* pushes, size_t push_cap,
+                             size_t* push_count_out) {
+  return guarded(world, [&](sfl::World& w) {
+    if ((pose_count && !poses) || (push_cap && !pushes)) return (int32_t)SFL_OUT_OF_RANGE;
+    return w.step_with_bodies(dt, poses, pose_count, report_out, pushes, push_cap, push_count_out);
+  });
+}
+
 int32_t sfl_add_box_container(sfl_world* world, sfl_vec3 low, sfl_vec3 high, uint32_t* boundary_out) {
   return guarded(world, [&](sfl::World& w) { return w.add_box_container(low, high, boundary_out); });
 }

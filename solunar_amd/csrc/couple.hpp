@@ -1,0 +1,94 @@
+// Coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs): the records the
+// coupling kernels (kernels_couple.hip) share with the host side.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace sfl {
+
+// One body's damping over one 512-particle chunk: the pairs' forces and
+// torques summed from +0 in the reference's order.
+struct FeltRecord {
+  uint32_t owner;
+  float force[3];
+  float torque[3];
+};
+
+// The fastest point and the box of each body (couple.rs:321-339), and
+// whether the water's last grid has a cell in reach of it.
+struct BodyExtent {
+  uint32_t speed_bits;
+  int32_t lo[3], hi[3];  // order-preserving keys of the box
+  uint32_t wet;
+};
+
+// A body's cell range (from the host) to test against the water's grid.
+struct WetQuery {
+  uint32_t body;
+  int32_t lo[3], hi[3];
+};
+
+// What the lift of one body reads from the water (lift.rs:25-152).
+struct LiftState {
+  int32_t box_lo[3], box_hi[3];  // order-preserving keys of the probes' box
+  float low[3], high[3];         // the box grown by READ_REACH radii
+  uint32_t near;                 // water particles inside
+  uint32_t near_offset;          // into the near-list buffer (set by the host)
+  int32_t col_lo[2];             // first column (x, z) of the box
+  uint32_t col_span[2];
+  uint32_t column_offset;  // into the column table
+  uint32_t sort_offset, sort_size;
+  float level;
+  float flow[3];
+  float wetted, volume;
+  float middle[3];  // sum of arm * share, not yet divided
+  uint32_t line;    // probes on the waterline
+  float area;       // volume^(2/3) / wetted, set by the host
+  float impulse[3], angular[3];
+  uint32_t lifted;
+};
+
+// The buffers of one carry (couple.rs:293-319).
+struct MoverCarry {
+  uint32_t m, nbodies, owner_bits;
+  const float4* ms;  // start (x, y, z, owner bits), current order
+  const float4* mv;  // velocity
+  float4* ms2;       // sorted start, velocity and carried position
+  float4* mv2;
+  float4* mp;
+  MoverGrid* grid;
+  uint64_t* keys_in;
+  uint32_t* perm_in;
+  uint64_t* keys;  // sorted keys, read by the WCSPH passes
+  uint32_t* perm;
+  void* temp;
+  size_t temp_bytes;
+  BodyState* bodies;
+};
+
+void launch_movers_place(hipStream_t s, const float4* local, uint32_t m, const BodyState* bodies, float4* MS,
+                         float4* MV);
+hipError_t mover_sort_bytes(uint32_t m, size_t* bytes);
+hipError_t launch_movers_carry(hipStream_t s, const MoverCarry& c, float elapsed, float radius, Control* ctl);
+void launch_felt(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, uint32_t* counts,
+                 uint32_t* offsets, uint32_t* tile_sums, uint32_t* total, FeltRecord* recs, uint32_t rec_cap,
+                 BodyState* bodies, uint32_t nbodies);
+void launch_extent(hipStream_t s, const float4* MP, const float4* MV, uint32_t m, BodyExtent* e, uint32_t nbodies);
+void launch_extent_wet(hipStream_t s, const uint32_t* start, const Control* ctl, const WetQuery* q, uint32_t nq,
+                       BodyExtent* e);
+void launch_lift_box(hipStream_t s, const float4* probes, const BodyState* bodies, uint32_t nbodies,
+                     uint32_t most_probes, float4* arm, float4* vel, LiftState* ls, const float4* P, uint32_t n,
+                     float reach);
+void launch_lift_near(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies, uint32_t* near,
+                      unsigned long long* columns, float column, const float up[3]);
+void launch_lift_level(hipStream_t s, const float4* P, const float4* V, LiftState* ls, const BodyState* bodies,
+                       uint32_t nbodies, const uint32_t* near, const unsigned long long* columns, uint32_t* sortbuf,
+                       const float4* arm, const float4* vel, float half_spacing, const float up[3]);
+void launch_lift_force(hipStream_t s, LiftState* ls, const BodyState* bodies, uint32_t nbodies, const float4* arm,
+                       const float4* vel, float rest, float gravity, const float up[3], float dt);
+void launch_give_back(hipStream_t s, const uint32_t* near, uint32_t count, const float change[3], float4* V);
+
+}  // namespace sfl

@@ -27,6 +27,8 @@ constexpr uint32_t kFlagBarrier = 4u;    // a grid barrier gave up waiting
 constexpr uint32_t kFlagPairs = 8u;     // a particle had more neighbours than the list holds
 constexpr uint32_t kFlagWide = 32u;     // a neighbour sat past the width of its cell's mask
 constexpr uint32_t kFlagTiles = 64u;    // the solve tiles outnumbered the stop sum's room
+constexpr uint32_t kFlagMoverBox = 128u;  // the body points' cell box outgrew the sort key
+constexpr uint32_t kFlagFeltRoom = 256u;  // the chunks' body sums outgrew their records
 
 // A drain's or the domain's box, and an emitter's lattice (flows after a
 // step, kernels_flow.hip).
@@ -114,6 +116,51 @@ struct CellBox {
   uint32_t span[3];
 };
 
+// A coupled body during a step (src/fluid/couple.rs:15-32): its pose (the
+// centre rounded from the rigid world's f64 position, rotation x, y, z, w),
+// how far its centre has gone in the current substep, the damping impulse
+// and angular impulse summed over the step, and its ranges of points and
+// probes in the concatenated body-axis arrays.
+struct BodyState {
+  float center[3];
+  float rotation[4];
+  float linear[3];
+  float angular[3];
+  float moved[3];
+  float impulse[3];
+  float angular_impulse[3];
+  uint32_t first_point, points;
+  uint32_t first_probe, probes;
+  float probe_spacing;
+};
+
+// The cell box of the body points after a carry (Movers::grid,
+// src/fluid/couple.rs:293-319). A point's sort key is its cell's slot in the
+// box, ((x - lo.x) * span.y + (y - lo.y)) * span.z + (z - lo.z), shifted up
+// past the owner bits, or-ed with its owner; `bad` is set when the box's
+// slots do not fit beside the owner bits in 64.
+struct MoverGrid {
+  int32_t lo[3];
+  int32_t hi[3];
+  uint64_t span[3];
+  uint32_t bad;
+};
+
+// The body points of a substep, sorted by (cell, owner, previous place), as
+// the WCSPH passes see them.
+struct MoverView {
+  const float4* pos;     // x, y, z, owner bits
+  const float4* vel;     // vx, vy, vz, 0
+  const uint64_t* keys;  // sort keys, ascending
+  uint32_t count;        // 0: no body points this step
+  uint32_t owner_bits;
+  const MoverGrid* grid;
+  const BodyState* bodies;
+  float damping;  // (1 c r).min(0.25 r r / dt), step.rs:163-164
+  float settles;  // |g| spacing, couple.rs:387
+  uint8_t* felt;  // per particle: 1 when a point's damping reached its body
+};
+
 // Scalars of one substep, computed on the host in the reference's f32
 // arithmetic.
 struct StepParams {
@@ -158,6 +205,7 @@ struct StepParams {
   uint16_t* near;
   float* factor;
   int32_t format;  // kMasks16, kMasks32, kLists or kCompact
+  MoverView movers;
 };
 
 // Rows the neighbour list arrays carry past their capacity: the solve's
@@ -296,6 +344,51 @@ __device__ __forceinline__ void visit_near(const float4* __restrict__ P, const u
         uint32_t s = start[c], e = start[c + 1];
         for (uint32_t j = s; j < e; ++j) {
           float4 pj = P[j];
+          float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+          float sq = dot3(ox, oy, oz, ox, oy, oz);
+          if (sq < reach) f(j, pj, ox, oy, oz, sq);
+        }
+      }
+}
+
+// First index in keys[0, n) whose key is not below `target`.
+__device__ __forceinline__ uint32_t lower_bound64(const uint64_t* __restrict__ keys, uint32_t n, uint64_t target) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = lo + (hi - lo) / 2;
+    if (keys[mid] < target)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Calls f(j, pj, ox, oy, oz, squared) for every body point j within reach
+// of `at`, o = at - pos[j], in the order `near` lists them over the body
+// points' grid (src/fluid/mod.rs:583-589): cells dz, dy, dx, then each
+// cell's points in sorted order.
+template <class F>
+__device__ __forceinline__ void visit_movers(const MoverView& mv, float4 at, float radius, float reach, F&& f) {
+  if (mv.count == 0) return;
+  const MoverGrid& g = *mv.grid;
+  int64_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
+  if (cx + 1 < g.lo[0] || cx - 1 > g.hi[0] || cy + 1 < g.lo[1] || cy - 1 > g.hi[1] || cz + 1 < g.lo[2] ||
+      cz - 1 > g.hi[2])
+    return;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int64_t x = cx + dx, y = cy + dy, z = cz + dz;
+        if (x < g.lo[0] || x > g.hi[0] || y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;
+        uint64_t slot = ((uint64_t)(x - g.lo[0]) * g.span[1] + (uint64_t)(y - g.lo[1])) * g.span[2] +
+                        (uint64_t)(z - g.lo[2]);
+        // the box's slots fit beside the owner bits (MoverGrid::bad unset), so
+        // slot + 1 shifted does too
+        uint32_t b = lower_bound64(mv.keys, mv.count, slot << mv.owner_bits);
+        uint32_t e = lower_bound64(mv.keys + b, mv.count - b, (slot + 1) << mv.owner_bits) + b;
+        for (uint32_t j = b; j < e; ++j) {
+          float4 pj = mv.pos[j];
           float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
           float sq = dot3(ox, oy, oz, ox, oy, oz);
           if (sq < reach) f(j, pj, ox, oy, oz, sq);

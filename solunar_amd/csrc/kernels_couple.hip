@@ -1,0 +1,775 @@
+// Coupled bodies on gfx950 (src/fluid/couple.rs, src/fluid/lift.rs): the
+// body points placed where their bodies stand and carried through the
+// substeps, sorted by (cell, owner, previous place) with a stable radix sort;
+// each 512-particle chunk's damping on each body summed pair by pair in the
+// reference's order and added to the bodies in chunk order; the fastest body
+// near water; the lift each body reads from the water around it, and what a
+// dynamic body's lift takes back out of that water.
+//
+// Arithmetic follows the reference's f32 operations in order, as the rest of
+// the library does (device.hpp). Sums the reference folds in order are
+// folded in order by one lane; only maxima, minima and counts use atomics.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "couple.hpp"
+#include "device.hpp"
+#include "launch.hpp"
+
+namespace sfl {
+
+namespace {
+
+__device__ __forceinline__ void cross3(float ax, float ay, float az, float bx, float by, float bz, float& cx,
+                                       float& cy, float& cz) {
+  cx = ay * bz - by * az;
+  cy = az * bx - bz * ax;
+  cz = ax * by - bx * ay;
+}
+
+// Quat::rotate_vec3, taken as glam's v (w^2 - |u|^2) + u (2 (v.u)) +
+// (u x v)(2 w), u = (x, y, z)
+__device__ __forceinline__ void rotate(const float q[4], float vx, float vy, float vz, float& ox, float& oy,
+                                       float& oz) {
+  float ux = q[0], uy = q[1], uz = q[2], w = q[3];
+  float b2 = dot3(ux, uy, uz, ux, uy, uz);
+  float s = w * w - b2;
+  float d = dot3(vx, vy, vz, ux, uy, uz) * 2.0f;
+  float cx, cy, cz;
+  cross3(ux, uy, uz, vx, vy, vz, cx, cy, cz);
+  float w2 = w * 2.0f;
+  ox = vx * s + ux * d + cx * w2;
+  oy = vy * s + uy * d + cy * w2;
+  oz = vz * s + uz * d + cz * w2;
+}
+
+// a point's arm and velocity from its body's pose: arm = R local,
+// velocity = linear + angular x arm (couple.rs:271-274)
+__device__ __forceinline__ void place(const BodyState& b, float4 local, float& ax, float& ay, float& az, float& vx,
+                                      float& vy, float& vz) {
+  rotate(b.rotation, local.x, local.y, local.z, ax, ay, az);
+  float cx, cy, cz;
+  cross3(b.angular[0], b.angular[1], b.angular[2], ax, ay, az, cx, cy, cz);
+  vx = b.linear[0] + cx;
+  vy = b.linear[1] + cy;
+  vz = b.linear[2] + cz;
+}
+
+// order-preserving int keys of floats for atomic min / max
+__device__ __forceinline__ int32_t ordered(float v) {
+  int32_t b = __float_as_int(v);
+  return b >= 0 ? b : b ^ 0x7fffffff;
+}
+__device__ __forceinline__ float unordered(int32_t k) { return __int_as_float(k >= 0 ? k : k ^ 0x7fffffff); }
+
+// f32::total_cmp as an unsigned key
+__device__ __forceinline__ uint32_t total_key(float v) {
+  uint32_t b = __float_as_uint(v);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float total_value(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__device__ __forceinline__ uint32_t owner_of(float4 p) { return __float_as_uint(p.w); }
+
+}  // namespace
+
+// ---- placement and carry (couple.rs:258-319) -----------------------------
+
+__global__ __launch_bounds__(256) void k_movers_place(const float4* __restrict__ local, uint32_t m,
+                                                      const BodyState* __restrict__ bodies, float4* __restrict__ MS,
+                                                      float4* __restrict__ MV) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  float4 p = local[i];
+  const BodyState& b = bodies[owner_of(p)];
+  float ax, ay, az, vx, vy, vz;
+  place(b, p, ax, ay, az, vx, vy, vz);
+  MS[i] = make_float4(b.center[0] + ax, b.center[1] + ay, b.center[2] + az, p.w);
+  MV[i] = make_float4(vx, vy, vz, 0.0f);
+}
+
+__global__ void k_movers_box_reset(MoverGrid* g, const Control* ctl) {
+  if (threadIdx.x != 0 || ctl->abort) return;
+  for (int a = 0; a < 3; ++a) {
+    g->lo[a] = 0x7fffffff;
+    g->hi[a] = (int32_t)0x80000000u;
+  }
+  g->bad = 0;
+}
+
+__device__ __forceinline__ void carried(const float4* MS, const float4* MV, uint32_t i, float elapsed, float& x,
+                                        float& y, float& z) {
+  float4 s = MS[i], v = MV[i];
+  x = s.x + v.x * elapsed;
+  y = s.y + v.y * elapsed;
+  z = s.z + v.z * elapsed;
+}
+
+__global__ __launch_bounds__(256) void k_movers_cells(const float4* __restrict__ MS, const float4* __restrict__ MV,
+                                                      uint32_t m, float elapsed, float radius, MoverGrid* g,
+                                                      const Control* ctl) {
+  if (ctl->abort) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
+  int32_t hi[3] = {(int32_t)0x80000000u, (int32_t)0x80000000u, (int32_t)0x80000000u};
+  if (i < m) {
+    float x, y, z;
+    carried(MS, MV, i, elapsed, x, y, z);
+    int32_t c[3] = {cell_coord(x, radius), cell_coord(y, radius), cell_coord(z, radius)};
+    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = c[a];
+  }
+  for (int a = 0; a < 3; ++a) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      lo[a] = min(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0 && blockIdx.x * blockDim.x + (threadIdx.x & ~63u) < m) {
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&g->lo[a], lo[a]);
+      atomicMax(&g->hi[a], hi[a]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_movers_keys(const float4* __restrict__ MS, const float4* __restrict__ MV,
+                                                     uint32_t m, float elapsed, float radius, uint32_t owner_bits,
+                                                     MoverGrid* g, uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ perm, Control* ctl) {
+  if (ctl->abort) return;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t span[3];
+  for (int a = 0; a < 3; ++a) span[a] = (uint64_t)((int64_t)g->hi[a] - (int64_t)g->lo[a] + 1);
+  // the slots fit beside the owner bits when volume <= (2^64 - 1) >> owner_bits
+  const uint64_t room = ~uint64_t(0) >> owner_bits;
+  bool fits = span[0] <= room && span[1] <= room / span[0] && span[2] <= room / (span[0] * span[1]);
+  if (i == 0) {
+    for (int a = 0; a < 3; ++a) g->span[a] = span[a];
+    if (!fits) {
+      g->bad = 1;
+      atomicOr(&ctl->flags, kFlagMoverBox);
+    }
+  }
+  if (i >= m) return;
+  float x, y, z;
+  carried(MS, MV, i, elapsed, x, y, z);
+  uint64_t key = 0;
+  if (fits) {
+    uint64_t cx = (uint64_t)((int64_t)cell_coord(x, radius) - g->lo[0]);
+    uint64_t cy = (uint64_t)((int64_t)cell_coord(y, radius) - g->lo[1]);
+    uint64_t cz = (uint64_t)((int64_t)cell_coord(z, radius) - g->lo[2]);
+    key = (((cx * span[1] + cy) * span[2] + cz) << owner_bits) | owner_of(MS[i]);
+  }
+  keys[i] = key;
+  perm[i] = i;
+}
+
+// The sorted order's start, velocity and carried position; each body's
+// `moved` = linear velocity * elapsed (couple.rs:315-318).
+__global__ __launch_bounds__(256) void k_movers_gather(const float4* __restrict__ MS, const float4* __restrict__ MV,
+                                                       const uint32_t* __restrict__ perm, uint32_t m, float elapsed,
+                                                       float4* __restrict__ MS2, float4* __restrict__ MV2,
+                                                       float4* __restrict__ MP, BodyState* __restrict__ bodies,
+                                                       uint32_t nbodies, const Control* ctl) {
+  if (ctl->abort) return;
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nbodies) {
+    BodyState& b = bodies[k];
+    for (int a = 0; a < 3; ++a) b.moved[a] = b.linear[a] * elapsed;
+  }
+  if (k >= m) return;
+  uint32_t i = perm[k];
+  float4 s = MS[i], v = MV[i];
+  MS2[k] = s;
+  MV2[k] = v;
+  MP[k] = make_float4(s.x + v.x * elapsed, s.y + v.y * elapsed, s.z + v.z * elapsed, s.w);
+}
+
+// ---- the damping each chunk gives each body (step.rs:173-221) ------------
+
+constexpr uint32_t kChunk = 512;  // CHUNK, src/fluid/mod.rs:36
+constexpr uint32_t kNoOwner = 0xffffffffu;
+
+// Walks the pairs of chunk c that reach a body, in the reference's order
+// (particles ascending, each particle's points as `near` lists them): f
+// gets (owner, force, torque) of each, force = gradient m (damp m),
+// torque = arm x force (couple.rs:396-407).
+template <class F>
+__device__ void chunk_reaches(const float4* __restrict__ PB, const float4* __restrict__ VB, const StepParams& prm,
+                              uint32_t c, F&& f) {
+  const MoverView& mv = prm.movers;
+  uint32_t end = min(prm.n, (c + 1) * kChunk);
+  for (uint32_t i = c * kChunk; i < end; ++i) {
+    if (!mv.felt[i]) continue;
+    float4 at = PB[i], vi = VB[i];
+    float density = vi.w;
+    visit_movers(mv, at, prm.radius, prm.reach, [&](uint32_t j, float4 p, float ox, float oy, float oz, float sq) {
+      float4 v = mv.vel[j];
+      float toward = dot3(vi.x - v.x, vi.y - v.y, vi.z - v.z, ox, oy, oz);
+      float damp = toward < 0.0f ? -mv.damping * toward / (sq + prm.soften) / density : 0.0f;
+      if (toward * toward < mv.settles * sq) return;
+      float gx, gy, gz;
+      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+      gx = gx * prm.mass;
+      gy = gy * prm.mass;
+      gz = gz * prm.mass;
+      float k = damp * prm.mass;
+      float fx = gx * k, fy = gy * k, fz = gz * k;
+      uint32_t owner = owner_of(p);
+      const BodyState& b = mv.bodies[owner];
+      float ax = p.x - (b.center[0] + b.moved[0]);
+      float ay = p.y - (b.center[1] + b.moved[1]);
+      float az = p.z - (b.center[2] + b.moved[2]);
+      float tx, ty, tz;
+      cross3(ax, ay, az, fx, fy, fz, tx, ty, tz);
+      f(owner, fx, fy, fz, tx, ty, tz);
+    });
+  }
+}
+
+// One lane per chunk: the bodies the chunk reaches, one walk per body in
+// ascending owner order plus one. WRITE = 0 counts them, 1 writes each
+// body's sums (pair by pair from +0) at the chunk's offset.
+template <int WRITE>
+__global__ __launch_bounds__(64) void k_felt(const float4* __restrict__ PB, const float4* __restrict__ VB,
+                                            StepParams prm, uint32_t chunks, uint32_t* __restrict__ counts,
+                                            const uint32_t* __restrict__ offsets, FeltRecord* __restrict__ out,
+                                            uint32_t cap) {
+  if (prm.ctl->abort) return;
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= chunks) return;
+  uint32_t target = kNoOwner;
+  chunk_reaches(PB, VB, prm, c, [&](uint32_t owner, float, float, float, float, float, float) {
+    target = min(target, owner);
+  });
+  uint32_t written = 0;
+  while (target != kNoOwner) {
+    uint32_t next = kNoOwner;
+    float f[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
+    chunk_reaches(PB, VB, prm, c, [&](uint32_t owner, float fx, float fy, float fz, float tx, float ty, float tz) {
+      if (owner == target) {
+        f[0] = f[0] + fx;
+        f[1] = f[1] + fy;
+        f[2] = f[2] + fz;
+        t[0] = t[0] + tx;
+        t[1] = t[1] + ty;
+        t[2] = t[2] + tz;
+      } else if (owner > target && owner < next) {
+        next = owner;
+      }
+    });
+    if (WRITE) {
+      uint32_t at = offsets[c] + written;
+      if (at < cap) {
+        FeltRecord& r = out[at];
+        r.owner = target;
+        for (int a = 0; a < 3; ++a) {
+          r.force[a] = f[a];
+          r.torque[a] = t[a];
+        }
+      } else {
+        atomicOr(&prm.ctl->flags, kFlagFeltRoom);
+      }
+    }
+    ++written;
+    target = next;
+  }
+  if (!WRITE) counts[c] = written;
+}
+
+// One lane per body: impulse += force dt over the chunks in order
+// (step.rs:218-221). Chunks that reach no body add +0, which changes
+// nothing: a sum from +0 is never -0.
+__global__ __launch_bounds__(64) void k_felt_apply(const FeltRecord* __restrict__ recs, const uint32_t* __restrict__ total,
+                                                   uint32_t cap, BodyState* __restrict__ bodies, uint32_t nbodies,
+                                                   float dt, const Control* ctl) {
+  if (ctl->abort) return;
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbodies) return;
+  BodyState& s = bodies[b];
+  float im[3] = {s.impulse[0], s.impulse[1], s.impulse[2]};
+  float an[3] = {s.angular_impulse[0], s.angular_impulse[1], s.angular_impulse[2]};
+  uint32_t n = min(*total, cap);
+  for (uint32_t r = 0; r < n; ++r) {
+    const FeltRecord& rec = recs[r];
+    if (rec.owner != b) continue;
+    for (int a = 0; a < 3; ++a) {
+      im[a] = im[a] + rec.force[a] * dt;
+      an[a] = an[a] + rec.torque[a] * dt;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    s.impulse[a] = im[a];
+    s.angular_impulse[a] = an[a];
+  }
+}
+
+// ---- the fastest body near water (couple.rs:321-366) ----------------------
+
+__global__ void k_extent_reset(BodyExtent* e, uint32_t nbodies) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbodies) return;
+  e[b].speed_bits = 0;
+  for (int a = 0; a < 3; ++a) {
+    e[b].lo[a] = ordered(3.40282347e38f);
+    e[b].hi[a] = ordered(-3.40282347e38f);
+  }
+  e[b].wet = 0;
+}
+
+// max |v| and the box of each body's points; max and min are exact in any
+// order (f32::max and min, couple.rs:335-338)
+__global__ __launch_bounds__(256) void k_extent(const float4* __restrict__ MP, const float4* __restrict__ MV,
+                                                uint32_t m, BodyExtent* __restrict__ e) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  float4 p = MP[i], v = MV[i];
+  BodyExtent& x = e[owner_of(p)];
+  float speed = sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z));
+  if (speed > 0.0f) atomicMax(&x.speed_bits, __float_as_uint(speed));
+  float c[3] = {p.x, p.y, p.z};
+  for (int a = 0; a < 3; ++a) {
+    atomicMin(&x.lo[a], ordered(c[a]));
+    atomicMax(&x.hi[a], ordered(c[a]));
+  }
+}
+
+// Any occupied cell of the fluid's last grid inside each body's cell range
+// (lo, hi from the host): the grid's table and box are the last sort's.
+__global__ __launch_bounds__(256) void k_extent_wet(const uint32_t* __restrict__ start, const Control* ctl,
+                                                    const WetQuery* __restrict__ q, uint32_t nq,
+                                                    BodyExtent* __restrict__ e) {
+  uint32_t b = blockIdx.y;
+  if (b >= nq) return;
+  const WetQuery& w = q[b];
+  int64_t lo[3], n[3];
+  uint64_t volume = 1;
+  for (int a = 0; a < 3; ++a) {
+    int64_t t0 = ctl->lo[a], t1 = (int64_t)ctl->lo[a] + ctl->span[a] - 1;
+    lo[a] = max((int64_t)w.lo[a], t0);
+    int64_t hi = min((int64_t)w.hi[a], t1);
+    n[a] = hi - lo[a] + 1;
+    if (n[a] <= 0) return;
+    volume *= (uint64_t)n[a];
+  }
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < volume;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    int64_t z = (int64_t)(k % n[2]), rest = (int64_t)(k / n[2]);
+    int64_t y = rest % n[1], x = rest / n[1];
+    int64_t slot = cell_slot(lo[0] + x, lo[1] + y, lo[2] + z, ctl->lo, ctl->span);
+    if (slot >= 0 && start[slot + 1] > start[slot]) {
+      e[w.body].wet = 1;
+      return;
+    }
+  }
+}
+
+// ---- lifts (src/fluid/lift.rs:25-152) -------------------------------------
+
+// probes' arms and velocities, and the box round them grown by the reach
+__global__ __launch_bounds__(256) void k_lift_probes(const float4* __restrict__ probes,
+                                                     const BodyState* __restrict__ bodies, uint32_t nbodies,
+                                                     float4* __restrict__ arm, float4* __restrict__ vel,
+                                                     LiftState* __restrict__ ls) {
+  uint32_t b = blockIdx.y;
+  if (b >= nbodies) return;
+  const BodyState& s = bodies[b];
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < s.probes; k += gridDim.x * blockDim.x) {
+    uint32_t i = s.first_probe + k;
+    float ax, ay, az, vx, vy, vz;
+    place(s, probes[i], ax, ay, az, vx, vy, vz);
+    arm[i] = make_float4(ax, ay, az, 0.0f);
+    vel[i] = make_float4(vx, vy, vz, 0.0f);
+    float c[3] = {s.center[0] + ax, s.center[1] + ay, s.center[2] + az};
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&ls[b].box_lo[a], ordered(c[a]));
+      atomicMax(&ls[b].box_hi[a], ordered(c[a]));
+    }
+  }
+}
+
+__global__ void k_lift_reset(LiftState* ls, uint32_t nbodies) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbodies) return;
+  LiftState& l = ls[b];
+  for (int a = 0; a < 3; ++a) {
+    l.box_lo[a] = ordered(3.40282347e38f);
+    l.box_hi[a] = ordered(-3.40282347e38f);
+  }
+  l.near = 0;
+}
+
+__device__ __forceinline__ bool in_box(const LiftState& l, float4 p) {
+  // at.clamp(low, high) == at
+  float lo[3] = {l.low[0], l.low[1], l.low[2]}, hi[3] = {l.high[0], l.high[1], l.high[2]};
+  float c[3] = {p.x, p.y, p.z};
+  for (int a = 0; a < 3; ++a)
+    if (rmin(rmax(c[a], lo[a]), hi[a]) != c[a]) return false;
+  return true;
+}
+
+// low = box - reach, high = box + reach (lift.rs:57-61); counts the water
+// inside
+__global__ __launch_bounds__(256) void k_lift_count(const float4* __restrict__ P, uint32_t n,
+                                                    LiftState* __restrict__ ls, uint32_t nbodies, float reach) {
+  uint32_t b = blockIdx.y;
+  if (b >= nbodies) return;
+  LiftState& l = ls[b];
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = unordered(l.box_lo[a]) - reach;
+    hi[a] = unordered(l.box_hi[a]) + reach;
+  }
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t inside = 0;
+  if (i < n) {
+    float4 p = P[i];
+    float c[3] = {p.x, p.y, p.z};
+    inside = 1;
+    for (int a = 0; a < 3; ++a)
+      if (rmin(rmax(c[a], lo[a]), hi[a]) != c[a]) inside = 0;
+  }
+  for (int off = 32; off >= 1; off >>= 1) inside += __shfl_xor(inside, off, 64);
+  if ((threadIdx.x & 63) == 0 && inside) atomicAdd(&l.near, inside);
+  if (i == 0)
+    for (int a = 0; a < 3; ++a) {
+      l.low[a] = lo[a];
+      l.high[a] = hi[a];
+    }
+}
+
+// One block per body: the water inside in particle order (a block-wide
+// compaction over every particle), and each column's highest water
+// (lift.rs:71-84) by total_cmp, as 1 << 32 | key in the column table.
+__global__ __launch_bounds__(1024) void k_lift_near(const float4* __restrict__ P, uint32_t n,
+                                                    LiftState* __restrict__ ls, uint32_t* __restrict__ near,
+                                                    unsigned long long* __restrict__ columns, float column,
+                                                    float upx, float upy, float upz) {
+  LiftState& l = ls[blockIdx.x];
+  if (l.near == 0) return;
+  __shared__ uint32_t wave_sums[16];
+  __shared__ uint32_t base_s;
+  if (threadIdx.x == 0) base_s = 0;
+  __syncthreads();
+  uint32_t* out = near + l.near_offset;
+  unsigned long long* table = columns + l.column_offset;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t first = 0; first < n; first += 1024) {
+    uint32_t i = first + threadIdx.x;
+    bool inside = false;
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < n) {
+      p = P[i];
+      inside = in_box(l, p);
+    }
+    unsigned long long ballot = __ballot(inside);
+    uint32_t before = __popcll(ballot & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_sums[wave] = __popcll(ballot);
+    __syncthreads();
+    uint32_t offset = base_s;
+    for (uint32_t w = 0; w < wave; ++w) offset += wave_sums[w];
+    if (inside) {
+      out[offset + before] = i;
+      int64_t kx = (int64_t)cell_coord(p.x, column) - l.col_lo[0];
+      int64_t kz = (int64_t)cell_coord(p.z, column) - l.col_lo[1];
+      float height = dot3(p.x, p.y, p.z, upx, upy, upz);
+      atomicMax(&table[kx * l.col_span[1] + kz], (1ull << 32) | total_key(height));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t all = 0;
+      for (uint32_t w = 0; w < 16; ++w) all += wave_sums[w];
+      base_s += all;
+    }
+    __syncthreads();
+  }
+}
+
+// One block per body: the columns' tops sorted (bitonic, total_cmp), the
+// level from the mean of their middle half; then lane 0 folds, in order,
+// the flow of the water under the level and the probes' shares
+// (lift.rs:85-126).
+__global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ P, const float4* __restrict__ V,
+                                                     LiftState* __restrict__ ls, const BodyState* __restrict__ bodies,
+                                                     const uint32_t* __restrict__ near,
+                                                     const unsigned long long* __restrict__ columns,
+                                                     uint32_t* __restrict__ sortbuf, const float4* __restrict__ arm,
+                                                     const float4* __restrict__ vel, float half_spacing, float upx,
+                                                     float upy, float upz) {
+  LiftState& l = ls[blockIdx.x];
+  if (l.near == 0) return;
+  const BodyState& body = bodies[blockIdx.x];
+  const unsigned long long* table = columns + l.column_offset;
+  uint32_t* keys = sortbuf + l.sort_offset;
+  const uint64_t cols = (uint64_t)l.col_span[0] * l.col_span[1];
+  __shared__ uint32_t wave_sums[16];
+  __shared__ uint32_t base_s;
+  if (threadIdx.x == 0) base_s = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint64_t first = 0; first < cols; first += 1024) {
+    uint64_t c = first + threadIdx.x;
+    unsigned long long t = c < cols ? table[c] : 0ull;
+    bool full = t != 0ull;
+    unsigned long long ballot = __ballot(full);
+    uint32_t before = __popcll(ballot & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_sums[wave] = __popcll(ballot);
+    __syncthreads();
+    uint32_t offset = base_s;
+    for (uint32_t w = 0; w < wave; ++w) offset += wave_sums[w];
+    if (full) keys[offset + before] = (uint32_t)t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t all = 0;
+      for (uint32_t w = 0; w < 16; ++w) all += wave_sums[w];
+      base_s += all;
+    }
+    __syncthreads();
+  }
+  const uint32_t count = base_s;
+  uint32_t size = 1;  // a power of two, at most the buffer's sort_size
+  while (size < count) size <<= 1;
+  for (uint32_t k = count + threadIdx.x; k < size; k += 1024) keys[k] = 0xffffffffu;
+  __syncthreads();
+  for (uint32_t k = 2; k <= size; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < size; i += 1024) {
+        uint32_t partner = i ^ j;
+        if (partner > i) {
+          uint32_t a = keys[i], b = keys[partner];
+          bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[partner] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  if (threadIdx.x != 0) return;
+  // the mean of the middle half, plus half a spacing
+  uint32_t quarter = count / 4;
+  float sum = 0.0f;
+  for (uint32_t k = quarter; k < count - quarter; ++k) sum = sum + total_value(keys[k]);
+  float level = sum / (float)(count - 2 * quarter) + half_spacing;
+  // the flow of the water under the level, in particle order
+  const uint32_t* mine = near + l.near_offset;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  uint32_t wet = 0;
+  for (uint32_t k = 0; k < l.near; ++k) {
+    uint32_t i = mine[k];
+    float4 p = P[i];
+    if (dot3(p.x, p.y, p.z, upx, upy, upz) <= level) {
+      float4 v = V[i];
+      fx = fx + v.x;
+      fy = fy + v.y;
+      fz = fz + v.z;
+      wet += 1;
+    }
+  }
+  float w = (float)(wet > 1 ? wet : 1);
+  l.flow[0] = fx / w;
+  l.flow[1] = fy / w;
+  l.flow[2] = fz / w;
+  l.level = level;
+  // the probes' shares under the level: wetted, middle
+  float spacing = body.probe_spacing;
+  float wetted = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+  uint32_t line = 0;
+  for (uint32_t k = 0; k < body.probes; ++k) {
+    float4 a = arm[body.first_probe + k];
+    float cx = body.center[0] + a.x, cy = body.center[1] + a.y, cz = body.center[2] + a.z;
+    float depth = level - dot3(cx, cy, cz, upx, upy, upz);
+    float share = depth / spacing + 0.5f;
+    if (share < 0.0f) share = 0.0f;
+    if (share > 1.0f) share = 1.0f;
+    if (!(share > 0.0f)) continue;
+    wetted = wetted + share;
+    mx = mx + a.x * share;
+    my = my + a.y * share;
+    mz = mz + a.z * share;
+    if (share < 1.0f) ++line;
+  }
+  l.wetted = wetted;
+  l.volume = wetted * (spacing * (spacing * spacing));
+  l.middle[0] = mx;
+  l.middle[1] = my;
+  l.middle[2] = mz;
+  l.line = line;
+}
+
+// One lane per body, after the host has set area = volume^(2/3) / wetted:
+// buoyancy, drag and the waterplane's damping, as impulses (lift.rs:127-150)
+__global__ void k_lift_force(LiftState* __restrict__ ls, const BodyState* __restrict__ bodies, uint32_t nbodies,
+                             const float4* __restrict__ arm, const float4* __restrict__ vel, float rest,
+                             float gravity, float upx, float upy, float upz, float dt) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbodies) return;
+  LiftState& l = ls[b];
+  l.lifted = 0;
+  if (l.near == 0 || l.wetted <= 0.0f) return;
+  const BodyState& body = bodies[b];
+  float spacing = body.probe_spacing;
+  float mx = l.middle[0] / l.wetted, my = l.middle[1] / l.wetted, mz = l.middle[2] / l.wetted;
+  float lift = rest * gravity * l.volume;
+  float bx = upx * lift, by = upy * lift, bz = upz * lift;
+  float f[3] = {0.0f + bx, 0.0f + by, 0.0f + bz};
+  float t[3];
+  cross3(mx, my, mz, bx, by, bz, t[0], t[1], t[2]);
+  t[0] = 0.0f + t[0];
+  t[1] = 0.0f + t[1];
+  t[2] = 0.0f + t[2];
+  auto shares = [&](auto&& each) {
+    for (uint32_t k = 0; k < body.probes; ++k) {
+      float4 a = arm[body.first_probe + k];
+      float cx = body.center[0] + a.x, cy = body.center[1] + a.y, cz = body.center[2] + a.z;
+      float depth = l.level - dot3(cx, cy, cz, upx, upy, upz);
+      float share = depth / spacing + 0.5f;
+      if (share < 0.0f) share = 0.0f;
+      if (share > 1.0f) share = 1.0f;
+      if (share > 0.0f) each(a, vel[body.first_probe + k], share);
+    }
+  };
+  // drag on each probe's share of the wetted area
+  shares([&](float4 a, float4 v, float share) {
+    float px = l.flow[0] - v.x, py = l.flow[1] - v.y, pz = l.flow[2] - v.z;
+    float k = 0.5f * 1.0f * rest * l.area * share * sqrtf(dot3(px, py, pz, px, py, pz));
+    float dx = px * k, dy = py * k, dz = pz * k;
+    float cx, cy, cz;
+    cross3(a.x, a.y, a.z, dx, dy, dz, cx, cy, cz);
+    f[0] = f[0] + dx;
+    f[1] = f[1] + dy;
+    f[2] = f[2] + dz;
+    t[0] = t[0] + cx;
+    t[1] = t[1] + cy;
+    t[2] = t[2] + cz;
+  });
+  // waves off the waterplane
+  float plane = (float)l.line * spacing * spacing;
+  float waves = rest * plane * sqrtf(gravity * sqrtf(plane));
+  float each = waves / (float)(l.line > 1 ? l.line : 1);
+  shares([&](float4 a, float4 v, float share) {
+    if (!(share < 1.0f)) return;
+    float k = each * dot3(l.flow[0] - v.x, l.flow[1] - v.y, l.flow[2] - v.z, upx, upy, upz);
+    float dx = upx * k, dy = upy * k, dz = upz * k;
+    float cx, cy, cz;
+    cross3(a.x, a.y, a.z, dx, dy, dz, cx, cy, cz);
+    f[0] = f[0] + dx;
+    f[1] = f[1] + dy;
+    f[2] = f[2] + dz;
+    t[0] = t[0] + cx;
+    t[1] = t[1] + cy;
+    t[2] = t[2] + cz;
+  });
+  for (int a = 0; a < 3; ++a) {
+    l.impulse[a] = f[a] * dt;
+    l.angular[a] = t[a] * dt;
+  }
+  l.lifted = 1;
+}
+
+// give_back (lift.rs:154-166): the water read loses what the body got
+__global__ __launch_bounds__(256) void k_give_back(const uint32_t* __restrict__ near, uint32_t count, float cx,
+                                                   float cy, float cz, float4* __restrict__ V) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  uint32_t i = near[k];
+  float4 v = V[i];
+  V[i] = make_float4(v.x - cx, v.y - cy, v.z - cz, v.w);
+}
+
+// ---- launchers ------------------------------------------------------------
+
+static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+void launch_movers_place(hipStream_t s, const float4* local, uint32_t m, const BodyState* bodies, float4* MS,
+                         float4* MV) {
+  if (m == 0) return;
+  hipLaunchKernelGGL(k_movers_place, blocks_for(m, 256), 256, 0, s, local, m, bodies, MS, MV);
+}
+
+hipError_t mover_sort_bytes(uint32_t m, size_t* bytes) {
+  *bytes = 0;
+  return rocprim::radix_sort_pairs(nullptr, *bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                   (const uint32_t*)nullptr, (uint32_t*)nullptr, m, 0, 64);
+}
+
+hipError_t launch_movers_carry(hipStream_t s, const MoverCarry& c, float elapsed, float radius, Control* ctl) {
+  if (c.m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_movers_box_reset, 1, 64, 0, s, c.grid, ctl);
+  hipLaunchKernelGGL(k_movers_cells, blocks_for(c.m, 256), 256, 0, s, c.ms, c.mv, c.m, elapsed, radius, c.grid, ctl);
+  hipLaunchKernelGGL(k_movers_keys, blocks_for(c.m, 256), 256, 0, s, c.ms, c.mv, c.m, elapsed, radius, c.owner_bits,
+                     c.grid, c.keys_in, c.perm_in, ctl);
+  size_t bytes = c.temp_bytes;
+  hipError_t e = rocprim::radix_sort_pairs(c.temp, bytes, c.keys_in, c.keys, c.perm_in, c.perm, c.m, 0, 64, s);
+  if (e != hipSuccess) return e;
+  uint32_t threads = c.m > c.nbodies ? c.m : c.nbodies;
+  hipLaunchKernelGGL(k_movers_gather, blocks_for(threads, 256), 256, 0, s, c.ms, c.mv, c.perm, c.m, elapsed, c.ms2,
+                     c.mv2, c.mp, c.bodies, c.nbodies, ctl);
+  return hipSuccess;
+}
+
+void launch_felt(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, uint32_t* counts,
+                 uint32_t* offsets, uint32_t* tile_sums, uint32_t* total, FeltRecord* recs, uint32_t rec_cap,
+                 BodyState* bodies, uint32_t nbodies) {
+  if (prm.n == 0 || prm.movers.count == 0) return;
+  uint32_t chunks = blocks_for(prm.n, kChunk);
+  hipLaunchKernelGGL(k_felt<0>, blocks_for(chunks, 64), 64, 0, s, PB, VB, prm, chunks, counts, nullptr, nullptr,
+                     rec_cap);
+  launch_scan(s, counts, offsets, tile_sums, total, nullptr, chunks, 0, prm.ctl);
+  hipLaunchKernelGGL(k_felt<1>, blocks_for(chunks, 64), 64, 0, s, PB, VB, prm, chunks, nullptr, offsets, recs,
+                     rec_cap);
+  hipLaunchKernelGGL(k_felt_apply, blocks_for(nbodies, 64), 64, 0, s, recs, total, rec_cap, bodies, nbodies, prm.dt,
+                     prm.ctl);
+}
+
+void launch_extent(hipStream_t s, const float4* MP, const float4* MV, uint32_t m, BodyExtent* e, uint32_t nbodies) {
+  if (nbodies == 0) return;
+  hipLaunchKernelGGL(k_extent_reset, blocks_for(nbodies, 64), 64, 0, s, e, nbodies);
+  if (m) hipLaunchKernelGGL(k_extent, blocks_for(m, 256), 256, 0, s, MP, MV, m, e);
+}
+
+void launch_extent_wet(hipStream_t s, const uint32_t* start, const Control* ctl, const WetQuery* q, uint32_t nq,
+                       BodyExtent* e) {
+  if (nq == 0) return;
+  hipLaunchKernelGGL(k_extent_wet, dim3(64, nq), 256, 0, s, start, ctl, q, nq, e);
+}
+
+void launch_lift_box(hipStream_t s, const float4* probes, const BodyState* bodies, uint32_t nbodies,
+                     uint32_t most_probes, float4* arm, float4* vel, LiftState* ls, const float4* P, uint32_t n,
+                     float reach) {
+  if (nbodies == 0) return;
+  hipLaunchKernelGGL(k_lift_reset, blocks_for(nbodies, 64), 64, 0, s, ls, nbodies);
+  hipLaunchKernelGGL(k_lift_probes, dim3(blocks_for(most_probes, 256), nbodies), 256, 0, s, probes, bodies, nbodies,
+                     arm, vel, ls);
+  if (n) hipLaunchKernelGGL(k_lift_count, dim3(blocks_for(n, 256), nbodies), 256, 0, s, P, n, ls, nbodies, reach);
+}
+
+void launch_lift_near(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies, uint32_t* near,
+                      unsigned long long* columns, float column, const float up[3]) {
+  if (nbodies == 0) return;
+  hipLaunchKernelGGL(k_lift_near, nbodies, 1024, 0, s, P, n, ls, near, columns, column, up[0], up[1], up[2]);
+}
+
+void launch_lift_level(hipStream_t s, const float4* P, const float4* V, LiftState* ls, const BodyState* bodies,
+                       uint32_t nbodies, const uint32_t* near, const unsigned long long* columns, uint32_t* sortbuf,
+                       const float4* arm, const float4* vel, float half_spacing, const float up[3]) {
+  if (nbodies == 0) return;
+  hipLaunchKernelGGL(k_lift_level, nbodies, 1024, 0, s, P, V, ls, bodies, near, columns, sortbuf, arm, vel,
+                     half_spacing, up[0], up[1], up[2]);
+}
+
+void launch_lift_force(hipStream_t s, LiftState* ls, const BodyState* bodies, uint32_t nbodies, const float4* arm,
+                       const float4* vel, float rest, float gravity, const float up[3], float dt) {
+  if (nbodies == 0) return;
+  hipLaunchKernelGGL(k_lift_force, blocks_for(nbodies, 64), 64, 0, s, ls, bodies, nbodies, arm, vel, rest, gravity,
+                     up[0], up[1], up[2], dt);
+}
+
+void launch_give_back(hipStream_t s, const uint32_t* near, uint32_t count, const float change[3], float4* V) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_give_back, blocks_for(count, 256), 256, 0, s, near, count, change[0], change[1], change[2], V);
+}
+
+}  // namespace sfl

@@ -12,6 +12,7 @@ namespace sfl {
 // compute_densities (src/fluid/mod.rs:621-675) for every particle, then the
 // pressure over density squared of step.rs:151-172. rho goes to V.w, p/rho^2
 // to `scaled`. Also folds max(rho/rest - 1, 0) for the report (step.rs:84-88).
+template <bool MOVERS>
 __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict__ P, float4* __restrict__ V,
                                                        float* __restrict__ scaled, StepParams prm) {
   Control* ctl = prm.ctl;
@@ -34,7 +35,17 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
                  [&](uint32_t, float4, float, float, float, float sq) {
                    wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
                  });
-    float density = prm.mass * (sum + wsum);
+    float density;
+    if (MOVERS) {
+      // the coupled bodies' points (mod.rs:648-656)
+      float msum = 0.0f;
+      visit_movers(prm.movers, at, prm.radius, prm.reach, [&](uint32_t, float4, float, float, float, float sq) {
+        msum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+      });
+      density = prm.mass * (sum + wsum + msum);
+    } else {
+      density = prm.mass * (sum + wsum);
+    }
     float pressure = tait_pressure(density, prm.rest, prm.stiffness);
     V[i].w = density;
     scaled[i] = pressure / (density * density);
@@ -43,11 +54,91 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
   wave_max_to(&stripe_of(ctl).comp_bits, comp);
 }
 
+// The push of wcsph_substep's force loop (step.rs:173-211): fluid pairs,
+// walls, the coupled bodies' points (mover_push, src/fluid/couple.rs:
+// 368-408), then + wall_viscous (step.rs:262-301) + gravity. Returns whether
+// some body point's damping reached its body (the pair counts in the body's
+// sum).
+template <bool MOVERS>
+__device__ __forceinline__ bool wcsph_push(const float4* __restrict__ PB, const float4* __restrict__ VB,
+                                           const float* __restrict__ scaled, const StepParams& prm, uint32_t i,
+                                           float4 at, float4 vi, float& ax, float& ay, float& az) {
+  const Control* ctl = prm.ctl;
+  int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+  uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+  float density = vi.w;
+  float own = scaled[i];
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+  visit_near(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+             [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+               if (sq <= 0.0f) return;
+               float distance = sqrtf(sq);
+               float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
+               float gx = ox * f, gy = oy * f, gz = oz * f;
+               float4 vj = VB[j];
+               float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
+               float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
+               float k = prm.mass * (own + scaled[j] + damp);
+               px = px - gx * k;
+               py = py - gy * k;
+               pz = pz - gz * k;
+             });
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  float mirrored = 0.0f;
+  if (MOVERS || prm.wall_cells) {
+    float pressure = tait_pressure(density, prm.rest, prm.stiffness);
+    mirrored = own + pressure / (prm.rest * prm.rest);
+  }
+  auto walls = [&](auto&& f) {
+    visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
+  };
+  float awx = 0.0f, awy = 0.0f, awz = 0.0f;
+  if (prm.wall_cells) {
+    walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
+      float gx, gy, gz;
+      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+      float k = prm.mass * mirrored;
+      px = px - gx * k;
+      py = py - gy * k;
+      pz = pz - gz * k;
+      awx = awx + -gx;
+      awy = awy + -gy;
+      awz = awz + -gz;
+    });
+  }
+  bool reached = false;
+  if (MOVERS) {
+    const MoverView& mv = prm.movers;
+    visit_movers(mv, at, prm.radius, prm.reach, [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+      float4 v = mv.vel[j];
+      float toward = dot3(vi.x - v.x, vi.y - v.y, vi.z - v.z, ox, oy, oz);
+      float damp = toward < 0.0f ? -mv.damping * toward / (sq + prm.soften) / density : 0.0f;
+      float gx, gy, gz;
+      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+      gx = gx * prm.mass;
+      gy = gy * prm.mass;
+      gz = gz * prm.mass;
+      float k = mirrored + damp;
+      px = px - gx * k;
+      py = py - gy * k;
+      pz = pz - gz * k;
+      if (!(toward * toward < mv.settles * sq)) reached = true;
+    });
+  }
+  if (prm.wall_cells) wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
+  // push + wall_viscous + gravity
+  ax = px + wx + prm.gx;
+  ay = py + wy + prm.gy;
+  az = pz + wz + prm.gz;
+  return reached;
+}
+
 // The force loop of wcsph_substep (step.rs:173-211) with wall_viscous
 // (step.rs:262-301), v += push * dt (step.rs:212-222) and advance
 // (step.rs:305-339), for every particle. Reads the sorted state PB/VB, writes
 // PA/VA; `forces` (optional) keeps the push, `clamps` (optional) the
 // clamped components of the advance (bits x, y, z).
+template <bool MOVERS>
 __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                      const float* __restrict__ scaled, StepParams prm,
                                                      float4* __restrict__ PA, float4* __restrict__ VA,
@@ -59,52 +150,12 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
   float speed = 0.0f;
   uint32_t held = 0;
   if (live) {
-    int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
-    uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
     float4 at = PB[i];
     float4 vi = VB[i];
     float density = vi.w;
-    float own = scaled[i];
-    float px = 0.0f, py = 0.0f, pz = 0.0f;
-    visit_near(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
-               [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-                 if (sq <= 0.0f) return;
-                 float distance = sqrtf(sq);
-                 float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
-                 float gx = ox * f, gy = oy * f, gz = oz * f;
-                 float4 vj = VB[j];
-                 float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
-                 float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
-                 float k = prm.mass * (own + scaled[j] + damp);
-                 px = px - gx * k;
-                 py = py - gy * k;
-                 pz = pz - gz * k;
-               });
-    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
-    if (prm.wall_cells) {
-      float pressure = tait_pressure(density, prm.rest, prm.stiffness);
-      float mirrored = own + pressure / (prm.rest * prm.rest);
-      float awx = 0.0f, awy = 0.0f, awz = 0.0f;
-      auto walls = [&](auto&& f) {
-        visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
-      };
-      walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
-        float gx, gy, gz;
-        kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
-        float k = prm.mass * mirrored;
-        px = px - gx * k;
-        py = py - gy * k;
-        pz = pz - gz * k;
-        awx = awx + -gx;
-        awy = awy + -gy;
-        awz = awz + -gz;
-      });
-      wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
-    }
-    // push + wall_viscous + gravity
-    float ax = px + wx + prm.gx;
-    float ay = py + wy + prm.gy;
-    float az = pz + wz + prm.gz;
+    float ax, ay, az;
+    bool reached = wcsph_push<MOVERS>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
+    if (MOVERS) prm.movers.felt[i] = reached ? 1 : 0;
     if (forces) {
       forces[3 * i] = ax;
       forces[3 * i + 1] = ay;
@@ -133,6 +184,7 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
 // one, undoing what the fused pass wrote past it: the reference stops its
 // advance there (step.rs:333-335), leaving later particles at their old
 // position with the pushed, unclamped velocity. Recomputes that velocity.
+template <bool MOVERS>
 __global__ __launch_bounds__(256) void k_wcsph_repair(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                       const float* __restrict__ scaled, StepParams prm,
                                                       float4* __restrict__ PA, float4* __restrict__ VA) {
@@ -145,50 +197,10 @@ __global__ __launch_bounds__(256) void k_wcsph_repair(const float4* __restrict__
     PA[i] = at;  // velocity keeps its zeroed components, position stays
     return;
   }
-  int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
-  uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
   float4 vi = VB[i];
   float density = vi.w;
-  float own = scaled[i];
-  float px = 0.0f, py = 0.0f, pz = 0.0f;
-  visit_near(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
-             [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-               if (sq <= 0.0f) return;
-               float distance = sqrtf(sq);
-               float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
-               float gx = ox * f, gy = oy * f, gz = oz * f;
-               float4 vj = VB[j];
-               float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
-               float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
-               float k = prm.mass * (own + scaled[j] + damp);
-               px = px - gx * k;
-               py = py - gy * k;
-               pz = pz - gz * k;
-             });
-  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
-  if (prm.wall_cells) {
-    float pressure = tait_pressure(density, prm.rest, prm.stiffness);
-    float mirrored = own + pressure / (prm.rest * prm.rest);
-    float awx = 0.0f, awy = 0.0f, awz = 0.0f;
-    auto walls = [&](auto&& f) {
-      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
-    };
-    walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
-      float gx, gy, gz;
-      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
-      float k = prm.mass * mirrored;
-      px = px - gx * k;
-      py = py - gy * k;
-      pz = pz - gz * k;
-      awx = awx + -gx;
-      awy = awy + -gy;
-      awz = awz + -gz;
-    });
-    wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
-  }
-  float ax = px + wx + prm.gx;
-  float ay = py + wy + prm.gy;
-  float az = pz + wz + prm.gz;
+  float ax, ay, az;
+  wcsph_push<MOVERS>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
   PA[i] = at;
   VA[i] = make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density);
 }
@@ -197,20 +209,30 @@ static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)(
 
 void launch_wcsph_density(hipStream_t s, const float4* P, float4* V, float* scaled, const StepParams& prm) {
   if (prm.n == 0) return;
-  hipLaunchKernelGGL(k_wcsph_density, blocks_for(prm.n, 256), 256, 0, s, P, V, scaled, prm);
+  if (prm.movers.count)
+    hipLaunchKernelGGL(k_wcsph_density<true>, blocks_for(prm.n, 256), 256, 0, s, P, V, scaled, prm);
+  else
+    hipLaunchKernelGGL(k_wcsph_density<false>, blocks_for(prm.n, 256), 256, 0, s, P, V, scaled, prm);
 }
 
 void launch_wcsph_force(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
                         const StepParams& prm, float4* PA, float4* VA, float* forces, uint8_t* clamps) {
   if (prm.n == 0) return;
-  hipLaunchKernelGGL(k_wcsph_force, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
-                     clamps);
+  if (prm.movers.count)
+    hipLaunchKernelGGL(k_wcsph_force<true>, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
+                       clamps);
+  else
+    hipLaunchKernelGGL(k_wcsph_force<false>, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
+                       clamps);
 }
 
 void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
                          const StepParams& prm, float4* PA, float4* VA) {
   if (prm.n == 0) return;
-  hipLaunchKernelGGL(k_wcsph_repair, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA);
+  if (prm.movers.count)
+    hipLaunchKernelGGL(k_wcsph_repair<true>, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA);
+  else
+    hipLaunchKernelGGL(k_wcsph_repair<false>, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA);
 }
 
 }  // namespace sfl

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <tuple>
 #include <vector>
 
 #include "world.hpp"
@@ -143,8 +144,11 @@ bool inside_out(const sfl_vec3& lo, const sfl_vec3& hi) { return hi.x <= lo.x ||
 size_t World::snapshot_size() const {
   size_t flows = 0;
   for (const Flow& f : flows_) flows += f.drain ? 29 : 41;
+  // a coupled body: handle, points, probe spacing, probes (couple.rs:50-55)
+  size_t bodies = 0;
+  for (const Coupled& k : coupled_) bodies += 8 + 4 + (k.points.size() / 3) * 12 + 4 + 4 + (k.probes.size() / 3) * 12;
   return 4 + 4 + 1 + 24 + 8 + 12 + 4 + 4 + n_ * 28 + 4 + 4 + wall_owner_.size() * 16 + 4 + 4 +
-         containers_.size() * 28 + 4 + flows + 4 + 4 + 4 + (has_domain_ ? 25 : 1) + kChecksumBytes;
+         containers_.size() * 28 + 4 + flows + 4 + 4 + bodies + 4 + (has_domain_ ? 25 : 1) + kChecksumBytes;
 }
 
 // save_snapshot (save.rs:17-66) into `bytes`, snapshot_size() of them
@@ -203,7 +207,17 @@ int32_t World::save_snapshot(uint8_t* bytes) {
     }
   }
   out.u32(flow_next_);
-  out.len(0);  // coupled bodies
+  // Coupled::save (couple.rs:50-55)
+  out.len(coupled_.size());
+  for (const Coupled& k : coupled_) {
+    out.u32(k.index);
+    out.u32(k.generation);
+    out.len(k.points.size() / 3);
+    for (size_t i = 0; i < k.points.size(); i += 3) out.vec3(k.points[i], k.points[i + 1], k.points[i + 2]);
+    out.f32(k.probe_spacing);
+    out.len(k.probes.size() / 3);
+    for (size_t i = 0; i < k.probes.size(); i += 3) out.vec3(k.probes[i], k.probes[i + 1], k.probes[i + 2]);
+  }
   out.len(0);  // terrain regions
   if (has_domain_) {
     out.u8(1);
@@ -361,13 +375,47 @@ int32_t World::load_snapshot(const uint8_t* data, size_t size, const sfl_setting
     r.invalid("fluid flow id");
     return SFL_SNAPSHOT_REFUSED;
   }
-  // coupled bodies and terrain regions: not carried by this world
+  // Coupled::load (couple.rs:57-75) in save.rs:158-171's order
   size_t coupled = 0;
   if (!r.count(12, kMaxParts, coupled)) return SFL_SNAPSHOT_REFUSED;
-  if (coupled > 0) {
-    why = "the snapshot holds coupled bodies, which the device world does not carry";
-    return SFL_UNSUPPORTED;
+  std::vector<Coupled> bodies;
+  size_t room = max;
+  auto points = [&](size_t most, std::vector<float>& pts) -> bool {
+    size_t n = 0;
+    if (!r.count(12, most, n)) return false;
+    for (size_t k = 0; k < n; ++k) {
+      sfl_vec3 p;
+      if (!r.vec3(p)) return false;
+      if (!finite3(p)) return r.invalid("fluid body point");
+      pts.insert(pts.end(), {p.x, p.y, p.z});
+    }
+    if (pts.empty()) return r.invalid("fluid body");
+    return true;
+  };
+  for (size_t b = 0; b < coupled; ++b) {
+    Coupled k{};
+    if (!r.u32(k.index) || !r.u32(k.generation)) return SFL_SNAPSHOT_REFUSED;
+    if (!points(room, k.points)) return SFL_SNAPSHOT_REFUSED;
+    if (version >= 3) {
+      if (!r.f32(k.probe_spacing)) return SFL_SNAPSHOT_REFUSED;
+      if (!std::isfinite(k.probe_spacing) || k.probe_spacing <= 0.0f) {
+        r.invalid("fluid body probes");
+        return SFL_SNAPSHOT_REFUSED;
+      }
+      if (!points(4096, k.probes)) return SFL_SNAPSHOT_REFUSED;  // MAX_PROBES, lift.rs:8
+    } else {
+      k.probe_spacing = s.spacing;
+      k.probes = k.points;
+    }
+    room -= k.points.size() / 3;
+    bodies.push_back(std::move(k));
   }
+  for (size_t b = 1; b < bodies.size(); ++b)
+    if (std::tie(bodies[b - 1].index, bodies[b - 1].generation) >= std::tie(bodies[b].index, bodies[b].generation)) {
+      r.invalid("fluid body order");
+      return SFL_SNAPSHOT_REFUSED;
+    }
+  // terrain regions: not carried by this world
   size_t terrains = 0;
   if (!r.count(32, kMaxParts, terrains)) return SFL_SNAPSHOT_REFUSED;
   if (terrains > 0) {
@@ -415,6 +463,8 @@ int32_t World::load_snapshot(const uint8_t* data, size_t size, const sfl_setting
   }
   w->flows_ = std::move(flows);
   w->flow_next_ = flow_next;
+  w->coupled_ = std::move(bodies);
+  w->coupled_dirty_ = true;
   w->has_domain_ = has_domain;
   w->domain_ = FlowBox{{dlo.x, dlo.y, dlo.z}, {dhi.x, dhi.y, dhi.z}};
   *world_out = w;

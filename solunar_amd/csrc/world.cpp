@@ -933,6 +933,7 @@ void World::collect_profile() {
 // from A into B.
 void World::enqueue_sort() {
   uint32_t n = (uint32_t)n_;
+  grid_any_ = n > 0;
   launch_substep_begin(stream_, ctl_);
   phase_begin(SFL_PHASE_BOUNDS);
   launch_bounds(stream_, pa_.ptr, n, radius_, ctl_);
@@ -949,8 +950,18 @@ void World::enqueue_sort() {
   phase_end(SFL_PHASE_REORDER);
 }
 
-// wcsph_substep (src/fluid/step.rs:144-224)
-void World::enqueue_wcsph_substep(const StepParams& prm) {
+// wcsph_substep (src/fluid/step.rs:144-224), substep `index` of the step;
+// with coupled bodies in the step their points are carried to index * small
+// first (step.rs:134-137), and each chunk's damping goes to the bodies after.
+int32_t World::enqueue_wcsph_substep(const StepParams& base, uint32_t index, float small) {
+  StepParams prm = base;
+  if (movers_m_) {
+    phase_begin(SFL_PHASE_BODIES);
+    int32_t st = carry_movers(small * (float)index);
+    phase_end(SFL_PHASE_BODIES);
+    if (st != SFL_OK) return st;
+    prm.movers = mover_view(prm.dt);
+  }
   enqueue_sort();
   phase_begin(SFL_PHASE_DENSITY);
   launch_wcsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm);
@@ -959,6 +970,14 @@ void World::enqueue_wcsph_substep(const StepParams& prm) {
   launch_wcsph_force(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm, pa_.ptr, va_.ptr,
                      keep_forces_ ? forces_.ptr : nullptr);
   phase_end(SFL_PHASE_FORCE);
+  if (movers_m_) {
+    phase_begin(SFL_PHASE_BODIES);
+    launch_felt(stream_, pb_.ptr, vb_.ptr, prm, felt_counts_.ptr, felt_offsets_.ptr, felt_tiles_.ptr,
+                felt_total_.ptr, felt_recs_.ptr, (uint32_t)felt_recs_.cap, bodies_d_.ptr,
+                (uint32_t)coupled_.size());
+    phase_end(SFL_PHASE_BODIES);
+  }
+  return SFL_OK;
 }
 
 // dfsph_substep (src/fluid/dfsph.rs:79-103)
@@ -1051,14 +1070,27 @@ int32_t World::step(float dt, sfl_report* out) {
   return SFL_OK;
 }
 
-// wcsph / wcsph_steps (src/fluid/step.rs:102-142); no coupled bodies, so the
-// sound speed is the configured one.
+// wcsph / wcsph_steps (src/fluid/step.rs:102-142): the step's sound speed
+// is raised to IMPACT_SOUND (4, step.rs:20) times the fastest coupled body
+// near water, and its substeps follow that body too.
 int32_t World::wcsph(float dt, sfl_report& r) {
   int32_t st = ensure_fastest();
   if (st != SFL_OK) return st;
+  const float configured = settings_.sound_speed;
+  float mover = 0.0f;
+  if (movers_m_) {
+    st = fastest_near_water(dt, configured / 4.0f, mover);
+    if (st != SFL_OK) return st;
+  }
+  struct Restore {
+    sfl_settings& s;
+    float c;
+    ~Restore() { s.sound_speed = c; }
+  } restore{settings_, configured};
+  settings_.sound_speed = fmax_rust(configured, 4.0f * mover);
   const sfl_settings& s = settings_;
   r.sound_speed = s.sound_speed;
-  float largest = s.courant * radius_ / (s.sound_speed + vmax_);
+  float largest = s.courant * radius_ / (s.sound_speed + fmax_rust(vmax_, mover));
   float wanted = fmax_rust(std::ceil(dt / largest), 1.0f);
   float cap = (float)s.max_substeps;
   float take = fmin_rust(wanted, cap);
@@ -1070,12 +1102,39 @@ int32_t World::wcsph(float dt, sfl_report& r) {
   StepParams prm = params(small);
   uint32_t done = 0, held = 0;
   lists_valid_ = false;
+  if (movers_m_) {
+    // a flag per particle, and per 512-particle chunk the bodies it reaches
+    const size_t chunks = (n_ + 511) / 512, nb = coupled_.size();
+    SFL_HIP(felt_.reserve(n_, 0, stream_), "hipMalloc(body reach flags)");
+    SFL_HIP(felt_counts_.reserve(chunks, 0, stream_), "hipMalloc(body sums)");
+    SFL_HIP(felt_offsets_.reserve(chunks + 1, 0, stream_), "hipMalloc(body sums)");
+    SFL_HIP(felt_tiles_.reserve(chunks / 256 + 2, 0, stream_), "hipMalloc(body sums)");
+    SFL_HIP(felt_total_.reserve(1, 0, stream_), "hipMalloc(body sums)");
+    SFL_HIP(felt_recs_.reserve(chunks * std::min<size_t>(nb, 64), 0, stream_), "hipMalloc(body sums)");
+  }
   while (done < substeps) {
     launch_step_begin(stream_, ctl_);
-    for (uint32_t k = done; k < substeps; ++k) enqueue_wcsph_substep(prm);
+    if (done > 0 && movers_m_) {
+      // the points' order after `done` substeps: carried again from where
+      // they were placed, as the aborted launches left them behind
+      st = replay_movers(done, small);
+      if (st != SFL_OK) return st;
+    }
+    for (uint32_t k = done; k < substeps; ++k) {
+      st = enqueue_wcsph_substep(prm, k, small);
+      if (st != SFL_OK) return st;
+    }
     st = read_control();
     if (st != SFL_OK) return st;
     const Control& c = *ctl_host_;
+    if (c.flags & kFlagMoverBox) {
+      detail_ = "the coupled bodies' points spread over more cells than their sort key holds";
+      return SFL_UNSUPPORTED;
+    }
+    if (c.flags & kFlagFeltRoom) {
+      detail_ = "a chunk of water reached more coupled bodies than its records hold";
+      return SFL_UNSUPPORTED;
+    }
     if (c.flags & kFlagOverflow) {
       uint32_t completed = c.substeps_done ? c.substeps_done - 1 : 0;
       held += totals().held;
@@ -1230,6 +1289,7 @@ int32_t World::update_neighbors() {
     lists_valid_ = true;
     lists_on_b_ = false;
     lists_emitted_ = false;
+    grid_any_ = false;
     return SFL_OK;
   }
   int32_t st = ensure_cells_for_bounds();

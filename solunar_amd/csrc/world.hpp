@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/solunar_fluid.h"
+#include "couple.hpp"
 #include "device.hpp"
 
 namespace sfl {
@@ -102,6 +103,12 @@ class World {
   int32_t remove_boundary(uint32_t id, int32_t* removed);
 
   int32_t step(float dt, sfl_report* report);
+  // coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs), couple.cpp
+  int32_t couple(uint32_t index, uint32_t generation, const sfl_shape& shape);
+  int32_t uncouple(uint32_t index, uint32_t generation, int32_t* was);
+  size_t coupled_len() const { return coupled_.size(); }
+  int32_t step_with_bodies(float dt, const sfl_body_pose* poses, size_t count, sfl_report* report,
+                           sfl_body_push* pushes, size_t cap, size_t* out_count);
   int32_t update_neighbors();
   int32_t compute_densities();
 
@@ -149,7 +156,7 @@ class World {
   int32_t wcsph(float dt, sfl_report& r);
   int32_t dfsph(float dt, sfl_report& r);
   void enqueue_sort();
-  void enqueue_wcsph_substep(const StepParams& prm);
+  int32_t enqueue_wcsph_substep(const StepParams& prm, uint32_t index, float small);
   int32_t enqueue_dfsph_substep(const StepParams& prm);
   int32_t emit_lists();
   int32_t ensure_fastest();
@@ -164,6 +171,13 @@ class World {
   int32_t keep_only(const FlowBox& box, bool domain, uint32_t& gone);
   int32_t emit(const sfl_vec3& low, const sfl_vec3& high, const sfl_vec3& velocity, uint32_t& added);
   int32_t read_count(const uint32_t* dev, uint32_t* out);
+  size_t coupled_points() const;
+  int32_t place_movers(const sfl_body_pose* poses, size_t count);
+  int32_t carry_movers(float elapsed);
+  int32_t replay_movers(uint32_t done, float small);
+  MoverView mover_view(float dt) const;
+  int32_t fastest_near_water(float dt, float slow, float& most);
+  int32_t lifts(float dt, const sfl_body_pose* poses, size_t count, sfl_body_push* pushes);
   void phase_begin(int phase);
   void phase_end(int phase);
   void collect_profile();
@@ -248,6 +262,38 @@ class World {
   bool lists_on_b_ = false;  // true: the build's positions are PB (after a step)
   DeviceBuffer<uint32_t> fstart_, fitems_, wstart_, witems_, list_counts_;
   size_t fitems_len_ = 0, witems_len_ = 0;
+
+  // coupled bodies (src/fluid/couple.rs:15-47), in handle order: wall
+  // points and probes in body axes
+  struct Coupled {
+    uint32_t index, generation;
+    std::vector<float> points, probes;
+    float probe_spacing;
+  };
+  std::vector<Coupled> coupled_;
+  bool coupled_dirty_ = true;  // the device copies of the points are stale
+  uint32_t most_probes_ = 0;
+  std::vector<BodyState> bodies_h_;
+  uint32_t movers_m_ = 0;  // body points placed for this step; 0 outside step_with_bodies
+  uint32_t owner_bits_ = 0;
+  int mcur_ = 0;           // which of ms_/mv_ holds the current order
+  bool grid_any_ = false;  // the water's last grid (update_neighbors) holds a cell
+  DeviceBuffer<float4> body_local_, probe_local_;
+  DeviceBuffer<BodyState> bodies_d_;
+  DeviceBuffer<float4> ms_[2], mv_[2], mp_;
+  DeviceBuffer<uint64_t> mkeys_in_, mkeys_;
+  DeviceBuffer<uint32_t> mperm_in_, mperm_;
+  DeviceBuffer<uint8_t> msort_temp_;
+  DeviceBuffer<MoverGrid> mgrid_;
+  DeviceBuffer<uint8_t> felt_;
+  DeviceBuffer<uint32_t> felt_counts_, felt_offsets_, felt_tiles_, felt_total_;
+  DeviceBuffer<FeltRecord> felt_recs_;
+  DeviceBuffer<BodyExtent> extent_;
+  DeviceBuffer<WetQuery> wet_;
+  DeviceBuffer<LiftState> lift_;
+  DeviceBuffer<float4> probe_arm_, probe_vel_;
+  DeviceBuffer<uint32_t> lift_near_, lift_sort_;
+  DeviceBuffer<unsigned long long> lift_columns_;
 
   // measurement
   bool profiling_ = false;

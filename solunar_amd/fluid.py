@@ -17,14 +17,14 @@ from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
-from .types import (CBatchTuning, CReport, CSettings, CSnapshotError, CVec3,
-                    FluidReport, FluidSettings, PhysicsError)
+from .types import (CBatchTuning, CBodyPose, CBodyPush, CReport, CSettings, CShape, CSnapshotError, CVec3,
+                    FluidReport, FluidSettings, PhysicsError, Shape, pose_array)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsolunar_fluid.so")
 
 _lib = None
 
-PHASES = ("bounds", "sort", "reorder", "density", "force", "solve", "viscous", "advance", "flows")
+PHASES = ("bounds", "sort", "reorder", "density", "force", "solve", "viscous", "advance", "flows", "bodies")
 
 
 SNAPSHOT_REFUSED = 102  # SFL_SNAPSHOT_REFUSED
@@ -76,6 +76,12 @@ def library() -> ctypes.CDLL:
         "sfl_wall_len": (sz, [P]),
         "sfl_read_wall_positions": (i32, [P, sz, fp]),
         "sfl_step": (i32, [P, ctypes.c_float, ctypes.POINTER(CReport)]),
+        "sfl_couple": (i32, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(CShape)]),
+        "sfl_uncouple": (i32, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(i32)]),
+        "sfl_coupled_len": (ctypes.c_size_t, [P]),
+        "sfl_step_with_bodies": (i32, [P, ctypes.c_float, ctypes.POINTER(CBodyPose), ctypes.c_size_t,
+                                       ctypes.POINTER(CReport), ctypes.POINTER(CBodyPush), ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t)]),
         "sfl_update_neighbors": (i32, [P]),
         "sfl_compute_densities": (i32, [P]),
         "sfl_read_ids": (i32, [P, sz, u32p]),
@@ -301,6 +307,34 @@ class FluidWorld:
         rep = CReport()
         self._check(self._lib.sfl_step(self._h, dt, ctypes.byref(rep)))
         return rep.to_report()
+
+    # ---- coupled bodies (couple.rs, lift.rs) ----
+    def couple(self, body, shape: Shape) -> None:
+        """Lets water and `body` (index, generation) push each other in
+        step_with_bodies (src/fluid/couple.rs:165-198)."""
+        c = CShape.of(shape)
+        self._check(self._lib.sfl_couple(self._h, int(body[0]), int(body[1]), ctypes.byref(c)))
+
+    def uncouple(self, body) -> bool:
+        out = ctypes.c_int32()
+        self._check(self._lib.sfl_uncouple(self._h, int(body[0]), int(body[1]), ctypes.byref(out)))
+        return out.value != 0
+
+    def coupled_len(self) -> int:
+        return int(self._lib.sfl_coupled_len(self._h))
+
+    def step_with_bodies(self, dt: float, poses):
+        """step_with_bodies (src/fluid/couple.rs:207-252) with the bodies
+        where `poses` (BodyPose) has them: (FluidReport, [BodyPush]) with each
+        coupled body's lift and damping impulses, in handle order."""
+        arr = pose_array(poses)
+        cap = self.coupled_len()
+        out = (CBodyPush * max(cap, 1))()
+        n = ctypes.c_size_t()
+        rep = CReport()
+        self._check(self._lib.sfl_step_with_bodies(self._h, dt, arr, len(poses), ctypes.byref(rep), out, cap,
+                                                   ctypes.byref(n)))
+        return rep.to_report(), [out[k].to_push() for k in range(n.value)]
 
     def update_neighbors(self) -> None:
         self._check(self._lib.sfl_update_neighbors(self._h))

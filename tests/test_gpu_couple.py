@@ -1,0 +1,131 @@
+"""Coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) on the device,
+against the oracle on the same poses: reports, every body's lift and damping
+impulses, and the particles, bit for bit, step after step; refusals as the
+reference refuses; snapshots with coupled bodies crossing between the two.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleWorld
+from solunar_amd.fluid import FluidWorld
+from solunar_amd.types import DFSPH, BodyPose, FluidSettings, PhysicsError, Shape
+
+pytestmark = pytest.mark.gpu
+
+TICK = 1.0 / 60.0
+
+
+def pool(w, size=(1.0, 1.0, 0.8), water=(1.0, 0.4, 0.8)):
+    w.add_box_container((0, 0, 0), size)
+    w.fill_box((0, 0, 0), water)
+    return w
+
+
+def pair(settings=None, **kw):
+    s = settings or FluidSettings()
+    return pool(FluidWorld(s), **kw), pool(OracleWorld(s, threads=8), **kw)
+
+
+def same(gpu, cpu, poses, k):
+    g = gpu.step_with_bodies(TICK, poses)
+    c = cpu.step_with_bodies(TICK, poses)
+    assert g[0] == c[0], k
+    assert g[1] == c[1], k
+    assert np.array_equal(gpu.ids(), cpu.ids()), k
+    assert np.array_equal(gpu.positions(), cpu.positions()), k
+    assert np.array_equal(gpu.velocities(), cpu.velocities()), k
+    return g
+
+
+def test_a_turning_box_and_a_kinematic_ball_push_the_water_as_the_oracle_does():
+    print("case: box + ball, 6 ticks", flush=True)
+    gpu, cpu = pair()
+    for w in (gpu, cpu):
+        w.couple((3, 0), Shape.cuboid((0.12, 0.1, 0.15)))
+        w.couple((1, 2), Shape.sphere(0.08))
+    lifted = False
+    for k in range(6):
+        poses = [BodyPose((3, 0), (0.5, 0.5 - 0.04 * k, 0.4), (0.0, 0.0998334, 0.0, 0.9950042),
+                          (0.0, -2.4, 0.0), (0.0, 1.0, 0.3)),
+                 BodyPose((1, 2), (0.2, 0.35, 0.3), linear_velocity=(0.6, 0.0, 0.0), dynamic=False)]
+        _, pushes = same(gpu, cpu, poses, k)
+        # handle order: the ball (1, 2), then the box (3, 0)
+        lifted = lifted or pushes[1].lifted
+    assert lifted and not pushes[0].lifted
+    assert gpu.state_hash() == cpu.state_hash()
+
+
+def test_many_bodies_across_many_chunks():
+    print("case: 12 bodies, 64k water", flush=True)
+    s = FluidSettings(spacing=0.02)
+    gpu, cpu = pair(s, size=(1.6, 1.0, 0.8), water=(1.6, 0.4, 0.8))
+    shapes = [Shape.sphere(0.05), Shape.cuboid((0.06, 0.03, 0.05)), Shape.capsule(0.03, 0.05)]
+    bodies = [(b, 1) for b in range(12)]
+    for w in (gpu, cpu):
+        for b in bodies:
+            w.couple(b, shapes[b[0] % 3])
+    for k in range(3):
+        poses = [BodyPose(b, (0.12 + 0.12 * b[0], 0.38 + 0.01 * k, 0.2 + 0.03 * b[0]),
+                          (0.0, np.sin(0.1 * b[0]), 0.0, np.cos(0.1 * b[0])),
+                          (0.3 - 0.05 * b[0], -0.2, 0.1), (0.0, 0.5, 0.0), dynamic=b[0] % 2 == 0)
+                 for b in bodies]
+        _, pushes = same(gpu, cpu, poses, k)
+    assert sum(p.impulse != (0.0, 0.0, 0.0) for p in pushes) >= 6
+
+
+def test_a_fast_ball_raises_the_sound_speed_only_near_water():
+    print("case: 24 m/s ball", flush=True)
+    gpu, cpu = pair(size=(1.0, 1.2, 1.0), water=(1.0, 0.6, 1.0))
+    for w in (gpu, cpu):
+        w.couple((1, 0), Shape.sphere(0.1))
+    speed = 24.0
+    far = [BodyPose((1, 0), (0.5, 5.0, 0.5), linear_velocity=(0, -speed, 0))]
+    r, _ = same(gpu, cpu, far, "unsorted")
+    assert r.sound_speed == np.float32(4 * speed)
+    r, _ = same(gpu, cpu, far, "far")
+    assert r.sound_speed == np.float32(20.0)
+    for k in range(2):
+        y = 0.75 - speed * TICK * k
+        r, _ = same(gpu, cpu, [BodyPose((1, 0), (0.5, y, 0.5), linear_velocity=(0, -speed, 0))], k)
+    assert r.sound_speed == np.float32(4 * speed)
+
+
+def test_coupled_steps_are_refused_as_the_reference_refuses_them():
+    print("case: refusals", flush=True)
+    w = pool(FluidWorld(FluidSettings(solver=DFSPH)))
+    w.couple((1, 0), Shape.sphere(0.1))
+    with pytest.raises(PhysicsError) as e:
+        w.step_with_bodies(TICK, [BodyPose((1, 0), (0.5, 0.3, 0.4))])
+    assert e.value.name == "InvalidSettings"
+    w = pool(FluidWorld(FluidSettings()))
+    with pytest.raises(PhysicsError) as e:
+        w.couple((1, 0), Shape.sphere(0.001))
+    assert e.value.name == "InvalidShape"
+    w.couple((1, 0), Shape.sphere(0.1))
+    with pytest.raises(PhysicsError) as e:
+        w.step_with_bodies(TICK, [BodyPose((1, 0), (0.5, 0.3, 0.4))] * 2)
+    assert e.value.name == "InvalidHandle"
+    with pytest.raises(PhysicsError) as e:
+        w.step_with_bodies(TICK, [BodyPose((1, 0), (0.5, float("inf"), 0.4))])
+    assert e.value.name == "NonFinite"
+    assert w.coupled_len() == 1
+    r, pushes = w.step_with_bodies(TICK, [])
+    assert pushes == [] and w.coupled_len() == 0
+    assert w.uncouple((1, 0)) is False
+
+
+def test_coupled_bodies_cross_between_device_and_oracle_snapshots():
+    print("case: snapshots with bodies", flush=True)
+    gpu, cpu = pair()
+    for w in (gpu, cpu):
+        w.couple((3, 0), Shape.cuboid((0.15, 0.15, 0.15)))
+        w.couple((4, 1), Shape.cuboid((1.0, 0.05, 0.05)))
+    poses = lambda k: [BodyPose((3, 0), (0.5, 0.45 - 0.02 * k, 0.4), linear_velocity=(0.0, -1.2, 0.0)),  # noqa
+                       BodyPose((4, 1), (0.5, 0.1, 0.6), dynamic=False)]
+    same(gpu, cpu, poses(0), 0)
+    assert gpu.save_snapshot() == cpu.save_snapshot()
+    back = FluidWorld.load_snapshot(cpu.save_snapshot())
+    assert back.coupled_len() == 2
+    for k in range(1, 3):
+        assert back.step_with_bodies(TICK, poses(k)) == cpu.step_with_bodies(TICK, poses(k))
+    assert back.state_hash() == cpu.state_hash()
