@@ -324,6 +324,83 @@ def run_flows(args) -> dict:
     }
 
 
+def run_bodies(args) -> dict:
+    """bodies_1m: the dam break under WCSPH with 200 coupled spheres, each
+    step taking the bodies' poses (step_with_bodies) and handing back every
+    body's lift and damping impulses; then the same world stepped without
+    poses (plain WCSPH) for the cost of the coupling."""
+    from solunar_amd import scenes
+    from solunar_amd.fluid import FluidWorld
+    from solunar_amd.types import Shape
+
+    world = FluidWorld(scenes.settings_bodies_1m())
+    n = scenes.dam_break(world)
+    field = scenes.body_field()
+    for handle, r, *_ in field:
+        world.couple(handle, Shape.sphere(r))
+    log(f"bodies_1m: {n} fluid particles, {len(field)} coupled spheres; warmup {args.warmup} steps")
+    t = 0.0
+    for k in range(args.warmup):
+        r, _ = world.step_with_bodies(DT, scenes.body_poses(field, t))
+        t += DT
+        log(f"warmup step {k}: {r.substeps} substeps")
+    world.synchronize()
+    world.profile(True)
+    world.profile_read(reset=True)
+    particle_steps = substeps = 0
+    lifted = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        particle_steps += len(world)
+        r, pushes = world.step_with_bodies(DT, scenes.body_poses(field, t))
+        t += DT
+        substeps += r.substeps
+        lifted += sum(p.lifted for p in pushes)
+    world.synchronize()
+    elapsed = time.perf_counter() - t0
+    phases = world.profile_read(reset=True)
+    world.profile(False)
+    log(f"coupled: {elapsed * 1e3 / args.steps:.2f} ms a step; timing plain steps")
+    world.synchronize()
+    plain_steps = 0
+    t1 = time.perf_counter()
+    for k in range(args.steps):
+        plain_steps += len(world)
+        world.step(DT)
+    world.synchronize()
+    plain = time.perf_counter() - t1
+    points = sum(len(scenes.sphere_points((0, 0, 0), r, 0.005, 0.01)) for _, r, *_ in field)
+    world.close()
+    bodies_ms, bodies_launches = phases.get("bodies", (0.0, 0))
+    return {
+        "metric": "particle-steps/s at 1M WCSPH with 200 coupled bodies",
+        "value": particle_steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "bodies_1m: dam_1m's 1,024,000 fluid particles under WCSPH (spacing 0.005 m, "
+                               "step 5e-4 s) with 200 coupled spheres of radius 0.02 to 0.08 m, 100 settling "
+                               "onto the water's top at 0.1 m/s and 100 on the dry floor downstream, drifting "
+                               "and turning about y, every other one dynamic",
+                   "fluid_particles": n, "bodies": len(field), "body_points_approx": points,
+                   "substeps_per_step": substeps / max(args.steps, 1),
+                   "lifts_per_step": lifted / max(args.steps, 1)},
+        "bodies": {"ms_per_step": bodies_ms / max(args.steps, 1), "share_of_step": bodies_ms * 1e-3 / elapsed,
+                   "launches": bodies_launches},
+        "plain_wcsph": {"value": plain_steps / plain, "ms_per_step": plain * 1e3 / args.steps},
+        "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def run_snapshot(args) -> dict:
     """snapshot_1m: save and load of the dam_1m world after `warmup` steps
     (1,024,000 particles at 28 B and 832,064 walls at 16 B in the SLFL
@@ -383,7 +460,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256", "adjoint_64k", "flows_1m",
-                                           "snapshot_1m"],
+                                           "snapshot_1m", "bodies_1m"],
                     default="dam_1m",
                     help="dam_16m: 16,000,000 fluid particles at spacing 0.002 with 200 sphere obstacles; "
                          "batch_256: 256 WCSPH scenes in one batch; flows_1m: dam_1m with an emitter, a "
@@ -419,6 +496,9 @@ def main() -> int:
         return 0
     if args.workload == "snapshot_1m":
         print(json.dumps(run_snapshot(args)), flush=True)
+        return 0
+    if args.workload == "bodies_1m":
+        print(json.dumps(run_bodies(args)), flush=True)
         return 0
 
     from solunar_amd import scenes

@@ -171,7 +171,7 @@ const sfl_body_pose* pose_of(const sfl_body_pose* poses, size_t count, uint32_t 
 // place_movers_at (src/fluid/couple.rs:258-291): bodies no pose names are
 // uncoupled; the others' points placed where their poses put them, then
 // carried to 0 (sorted).
-int32_t World::place_movers(const sfl_body_pose* poses, size_t count) {
+int32_t World::place_movers(const sfl_body_pose* poses, size_t count, float dt) {
   size_t before = coupled_.size();
   coupled_.erase(std::remove_if(coupled_.begin(), coupled_.end(),
                                 [&](const Coupled& k) { return !pose_of(poses, count, k.index, k.generation); }),
@@ -183,12 +183,15 @@ int32_t World::place_movers(const sfl_body_pose* poses, size_t count) {
   if (coupled_dirty_) {
     std::vector<float4> local, probes;
     most_probes_ = 0;
+    body_reach_.assign(nb, 0.0f);
     for (uint32_t b = 0; b < nb; ++b) {
       const Coupled& k = coupled_[b];
       float owner;
       std::memcpy(&owner, &b, 4);
-      for (size_t i = 0; i < k.points.size(); i += 3)
+      for (size_t i = 0; i < k.points.size(); i += 3) {
         local.push_back(make_float4(k.points[i], k.points[i + 1], k.points[i + 2], owner));
+        body_reach_[b] = std::max(body_reach_[b], length3(k.points[i], k.points[i + 1], k.points[i + 2]));
+      }
       for (size_t i = 0; i < k.probes.size(); i += 3)
         probes.push_back(make_float4(k.probes[i], k.probes[i + 1], k.probes[i + 2], 0.0f));
       most_probes_ = std::max<uint32_t>(most_probes_, (uint32_t)(k.probes.size() / 3));
@@ -241,12 +244,36 @@ int32_t World::place_movers(const sfl_body_pose* poses, size_t count) {
   SFL_HIP(mperm_in_.reserve(m, 0, stream_), "hipMalloc(body point order)");
   SFL_HIP(mperm_.reserve(m, 0, stream_), "hipMalloc(body point order)");
   SFL_HIP(mgrid_.reserve(1, 0, stream_), "hipMalloc(body point grid)");
+  // the dense cell table over the box the points may reach this step: each
+  // body's centre, give or take its reach, what it moves in dt and a cell
+  {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (uint32_t b = 0; b < nb; ++b) {
+      const BodyState& st = bodies_h_[b];
+      double v = std::sqrt((double)st.linear[0] * st.linear[0] + (double)st.linear[1] * st.linear[1] +
+                           (double)st.linear[2] * st.linear[2]);
+      double w = std::sqrt((double)st.angular[0] * st.angular[0] + (double)st.angular[1] * st.angular[1] +
+                           (double)st.angular[2] * st.angular[2]);
+      double reach = body_reach_[b] * 1.001 + (v + w * body_reach_[b]) * std::fabs((double)dt) + 2.0 * radius_;
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], (double)st.center[a] - reach);
+        hi[a] = std::max(hi[a], (double)st.center[a] + reach);
+      }
+    }
+    double volume = 1.0;
+    for (int a = 0; a < 3; ++a) volume *= std::floor(hi[a] / radius_) - std::floor(lo[a] / radius_) + 3.0;
+    if (std::isfinite(volume) && volume < (double)(uint64_t(1) << 27))
+      SFL_HIP(mtable_.reserve((size_t)volume + 2, 0, stream_), "hipMalloc(body point table)");
+  }
   size_t temp = 0;
   SFL_HIP(mover_sort_bytes(m, &temp), "radix sort size");
   SFL_HIP(msort_temp_.reserve(std::max<size_t>(temp, 1), 0, stream_), "hipMalloc(radix sort)");
   mcur_ = 0;
   movers_m_ = m;
   launch_movers_place(stream_, body_local_.ptr, m, bodies_d_.ptr, ms_[0].ptr, mv_[0].ptr);
+  // each body's fastest point and box, for fastest_near_water
+  SFL_HIP(extent_.reserve(nb, 0, stream_), "hipMalloc(body extents)");
+  launch_extent(stream_, ms_[0].ptr, mv_[0].ptr, bodies_d_.ptr, extent_.ptr, nb);
   return carry_movers(0.0f);
 }
 
@@ -269,6 +296,8 @@ int32_t World::carry_movers(float elapsed) {
   c.temp = msort_temp_.ptr;
   c.temp_bytes = msort_temp_.cap;
   c.bodies = bodies_d_.ptr;
+  c.table = mtable_.ptr;
+  c.table_cap = mtable_.cap;
   SFL_HIP(launch_movers_carry(stream_, c, elapsed, radius_, ctl_), "radix sort (body points)");
   mcur_ = 1 - mcur_;
   return SFL_OK;
@@ -292,13 +321,55 @@ MoverView World::mover_view(float dt) const {
   v.count = movers_m_;
   v.owner_bits = owner_bits_;
   v.grid = mgrid_.ptr;
+  v.table = mtable_.ptr;
   v.bodies = bodies_d_.ptr;
   const float c = settings_.sound_speed, r = radius_;
   v.damping = fmin_r(1.0f * c * r, 0.25f * r * r / dt);  // step.rs:163-164
   const sfl_vec3& g = settings_.gravity;
   v.settles = length3(g.x, g.y, g.z) * settings_.spacing;  // couple.rs:387
-  v.felt = felt_.ptr;
+  v.pairs.counts = pair_counts_b_.ptr;
+  v.pairs.offsets = pair_offsets_b_.ptr;
+  v.pairs.key = pair_key_[0].ptr;
+  v.pairs.force = pair_force_.ptr;
+  v.pairs.torque = pair_torque_.ptr;
+  v.pairs.none = (uint32_t)coupled_.size();
   return v;
+}
+
+int32_t World::ensure_pair_room(uint64_t cap) {
+  cap = std::min<uint64_t>(cap, 0xffffffffu);
+  if (cap <= body_pair_cap_) return SFL_OK;
+  for (int k = 0; k < 2; ++k) {
+    SFL_HIP(pair_key_[k].reserve(cap, 0, stream_), "hipMalloc(body pair keys)");
+    SFL_HIP(pair_value_[k].reserve(cap, 0, stream_), "hipMalloc(body pair order)");
+  }
+  SFL_HIP(pair_force_.reserve(cap, 0, stream_), "hipMalloc(body pair forces)");
+  SFL_HIP(pair_torque_.reserve(cap, 0, stream_), "hipMalloc(body pair torques)");
+  size_t temp = 0;
+  SFL_HIP(pair_sort_bytes((uint32_t)cap, &temp), "radix sort size");
+  SFL_HIP(pair_temp_.reserve(std::max<size_t>(temp, 1), 0, stream_), "hipMalloc(radix sort)");
+  body_pair_cap_ = (uint32_t)cap;
+  return SFL_OK;
+}
+
+PairRoom World::pair_room() const {
+  PairRoom r{};
+  r.counts = pair_counts_b_.ptr;
+  r.offsets = pair_offsets_b_.ptr;
+  r.tiles = pair_tiles_b_.ptr;
+  r.total = pair_total_b_.ptr;
+  r.cap = body_pair_cap_;
+  r.none = (uint32_t)coupled_.size();
+  r.key_bits = owner_bits_ + 1;
+  r.key = pair_key_[0].ptr;
+  r.value = pair_value_[0].ptr;
+  r.key_sorted = pair_key_[1].ptr;
+  r.value_sorted = pair_value_[1].ptr;
+  r.force = pair_force_.ptr;
+  r.torque = pair_torque_.ptr;
+  r.temp = pair_temp_.ptr;
+  r.temp_bytes = pair_temp_.cap;
+  return r;
 }
 
 // fastest_near_water (src/fluid/couple.rs:321-366): the fastest point of
@@ -307,9 +378,7 @@ int32_t World::fastest_near_water(float dt, float slow, float& most) {
   most = 0.0f;
   const uint32_t nb = (uint32_t)coupled_.size();
   if (nb == 0) return SFL_OK;
-  SFL_HIP(extent_.reserve(nb, 0, stream_), "hipMalloc(body extents)");
   SFL_HIP(wet_.reserve(nb, 0, stream_), "hipMalloc(body reach)");
-  launch_extent(stream_, mp_.ptr, mv_[mcur_].ptr, movers_m_, extent_.ptr, nb);
   std::vector<BodyExtent> ext(nb);
   SFL_HIP(hipMemcpyAsync(ext.data(), extent_.ptr, nb * sizeof(BodyExtent), hipMemcpyDeviceToHost, stream_),
           "read body extents");
@@ -369,8 +438,17 @@ int32_t World::lifts(float dt, const sfl_body_pose* poses, size_t count, sfl_bod
   SFL_HIP(lift_.reserve(nb, 0, stream_), "hipMalloc(lifts)");
   SFL_HIP(probe_arm_.reserve(total_probes, 0, stream_), "hipMalloc(probe arms)");
   SFL_HIP(probe_vel_.reserve(total_probes, 0, stream_), "hipMalloc(probe velocities)");
+  // the water in each body's box: counted per (body, 1024-particle chunk),
+  // chunks whose box misses the body's skipped, and scanned into offsets
+  const uint64_t chunks = (n_ + 1023) / 1024, cells = (uint64_t)nb * chunks;
+  SFL_HIP(lift_boxes_.reserve(2 * chunks, 0, stream_), "hipMalloc(lift chunks)");
+  SFL_HIP(lift_counts_.reserve(cells + 1, 0, stream_), "hipMalloc(lift counts)");
+  SFL_HIP(lift_offsets_.reserve(cells + 1, 0, stream_), "hipMalloc(lift offsets)");
+  SFL_HIP(lift_tiles_.reserve(cells / 2048 + 2, 0, stream_), "hipMalloc(lift offsets)");
   launch_lift_box(stream_, probe_local_.ptr, bodies_d_.ptr, nb, most_probes_, probe_arm_.ptr, probe_vel_.ptr,
-                  lift_.ptr, pa_.ptr, (uint32_t)n_, reach);
+                  lift_.ptr, reach);
+  launch_lift_count(stream_, pa_.ptr, (uint32_t)n_, lift_.ptr, nb, lift_boxes_.ptr, lift_counts_.ptr,
+                    lift_offsets_.ptr, lift_tiles_.ptr, lift_tiles_.ptr + cells / 2048 + 1);
   std::vector<LiftState> ls(nb);
   auto pull = [&]() -> int32_t {
     SFL_HIP(hipMemcpyAsync(ls.data(), lift_.ptr, nb * sizeof(LiftState), hipMemcpyDeviceToHost, stream_),
@@ -389,7 +467,6 @@ int32_t World::lifts(float dt, const sfl_body_pose* poses, size_t count, sfl_bod
   uint64_t near = 0, cols = 0, sort = 0;
   for (uint32_t b = 0; b < nb; ++b) {
     LiftState& l = ls[b];
-    l.near_offset = (uint32_t)near;
     near += l.near;
     int64_t span[2] = {1, 1};
     if (l.near) {
@@ -419,7 +496,8 @@ int32_t World::lifts(float dt, const sfl_body_pose* poses, size_t count, sfl_bod
   SFL_HIP(hipMemsetAsync(lift_columns_.ptr, 0, cols * sizeof(unsigned long long), stream_), "clear lift columns");
   st = push();
   if (st != SFL_OK) return st;
-  launch_lift_near(stream_, pa_.ptr, (uint32_t)n_, lift_.ptr, nb, lift_near_.ptr, lift_columns_.ptr, column, up);
+  launch_lift_near(stream_, pa_.ptr, (uint32_t)n_, lift_.ptr, nb, lift_boxes_.ptr, lift_offsets_.ptr, lift_near_.ptr,
+                   lift_columns_.ptr, column, up);
   launch_lift_level(stream_, pa_.ptr, va_.ptr, lift_.ptr, bodies_d_.ptr, nb, lift_near_.ptr, lift_columns_.ptr,
                     lift_sort_.ptr, probe_arm_.ptr, probe_vel_.ptr, settings_.spacing / 2.0f, up);
   st = pull();
@@ -475,7 +553,7 @@ int32_t World::step_with_bodies(float dt, const sfl_body_pose* poses, size_t cou
   if (cap < kept) return SFL_OUT_OF_RANGE;
   ++version_;
   launch_step_begin(stream_, ctl_);
-  int32_t st = place_movers(poses, count);
+  int32_t st = place_movers(poses, count, dt);
   if (st == SFL_OK) st = step(dt, report);
   movers_m_ = 0;
   if (st != SFL_OK) return st;

@@ -9,12 +9,23 @@
 
 namespace sfl {
 
-// One body's damping over one 512-particle chunk: the pairs' forces and
-// torques summed from +0 in the reference's order.
-struct FeltRecord {
-  uint32_t owner;
-  float force[3];
-  float torque[3];
+// The pair records of one step and their sort (PairRecords, device.hpp).
+struct PairRoom {
+  uint32_t* counts;  // n
+  uint32_t* offsets;  // n + 1
+  uint32_t* tiles;
+  uint32_t* total;
+  uint32_t cap;
+  uint32_t none;
+  uint32_t key_bits;
+  uint32_t* key;
+  uint32_t* value;
+  uint32_t* key_sorted;
+  uint32_t* value_sorted;
+  float4* force;
+  float4* torque;
+  void* temp;
+  size_t temp_bytes;
 };
 
 // The fastest point and the box of each body (couple.rs:321-339), and
@@ -67,23 +78,29 @@ struct MoverCarry {
   void* temp;
   size_t temp_bytes;
   BodyState* bodies;
+  uint32_t* table;  // dense cell table, table_cap slots (0: none)
+  uint64_t table_cap;
 };
 
 void launch_movers_place(hipStream_t s, const float4* local, uint32_t m, const BodyState* bodies, float4* MS,
                          float4* MV);
 hipError_t mover_sort_bytes(uint32_t m, size_t* bytes);
 hipError_t launch_movers_carry(hipStream_t s, const MoverCarry& c, float elapsed, float radius, Control* ctl);
-void launch_felt(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, uint32_t* counts,
-                 uint32_t* offsets, uint32_t* tile_sums, uint32_t* total, FeltRecord* recs, uint32_t rec_cap,
-                 BodyState* bodies, uint32_t nbodies);
-void launch_extent(hipStream_t s, const float4* MP, const float4* MV, uint32_t m, BodyExtent* e, uint32_t nbodies);
+void launch_pairs_prepare(hipStream_t s, const PairRoom& room, uint32_t n, Control* ctl);
+hipError_t pair_sort_bytes(uint32_t cap, size_t* bytes);
+hipError_t launch_pairs_fold(hipStream_t s, const PairRoom& room, BodyState* bodies, uint32_t nbodies, float dt,
+                             const Control* ctl);
+void launch_extent(hipStream_t s, const float4* MS, const float4* MV, const BodyState* bodies, BodyExtent* e,
+                   uint32_t nbodies);
 void launch_extent_wet(hipStream_t s, const uint32_t* start, const Control* ctl, const WetQuery* q, uint32_t nq,
                        BodyExtent* e);
 void launch_lift_box(hipStream_t s, const float4* probes, const BodyState* bodies, uint32_t nbodies,
-                     uint32_t most_probes, float4* arm, float4* vel, LiftState* ls, const float4* P, uint32_t n,
-                     float reach);
-void launch_lift_near(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies, uint32_t* near,
-                      unsigned long long* columns, float column, const float up[3]);
+                     uint32_t most_probes, float4* arm, float4* vel, LiftState* ls, float reach);
+void launch_lift_count(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies, float4* boxes,
+                       uint32_t* counts, uint32_t* offsets, uint32_t* tiles, uint32_t* total);
+void launch_lift_near(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies,
+                      const float4* boxes, const uint32_t* offsets, uint32_t* near, unsigned long long* columns,
+                      float column, const float up[3]);
 void launch_lift_level(hipStream_t s, const float4* P, const float4* V, LiftState* ls, const BodyState* bodies,
                        uint32_t nbodies, const uint32_t* near, const unsigned long long* columns, uint32_t* sortbuf,
                        const float4* arm, const float4* vel, float half_spacing, const float up[3]);

@@ -28,7 +28,7 @@ constexpr uint32_t kFlagPairs = 8u;     // a particle had more neighbours than t
 constexpr uint32_t kFlagWide = 32u;     // a neighbour sat past the width of its cell's mask
 constexpr uint32_t kFlagTiles = 64u;    // the solve tiles outnumbered the stop sum's room
 constexpr uint32_t kFlagMoverBox = 128u;  // the body points' cell box outgrew the sort key
-constexpr uint32_t kFlagFeltRoom = 256u;  // the chunks' body sums outgrew their records
+constexpr uint32_t kFlagPairRoom = 256u;  // the body-point pairs outgrew their records
 
 // A drain's or the domain's box, and an emitter's lattice (flows after a
 // step, kernels_flow.hip).
@@ -71,6 +71,7 @@ struct Control {
   uint32_t abort;        // non-zero: every later launch of the step is a no-op
   uint32_t pad0;
   uint64_t need_volume;  // cells the box needed when it overflowed
+  uint64_t need_pairs;   // body-point pair records a substep needed past their room
   uint32_t bad_index;    // smallest sorted index whose advance went non-finite
   uint32_t substeps_done;
   uint32_t iterations[2];        // last solve's pushes: divergence, density
@@ -138,27 +139,44 @@ struct BodyState {
 // src/fluid/couple.rs:293-319). A point's sort key is its cell's slot in the
 // box, ((x - lo.x) * span.y + (y - lo.y)) * span.z + (z - lo.z), shifted up
 // past the owner bits, or-ed with its owner; `bad` is set when the box's
-// slots do not fit beside the owner bits in 64.
+// slots do not fit beside the owner bits in 64. When the box's volume + 1
+// fits the table (`dense`), slot c's points are table[c] .. table[c + 1].
 struct MoverGrid {
   int32_t lo[3];
   int32_t hi[3];
   uint64_t span[3];
+  uint64_t volume;
   uint32_t bad;
+  uint32_t dense;
+};
+
+// One body-point pair of the WCSPH force pass, for the damping the body
+// feels (mover_push, src/fluid/couple.rs:396-407): slot offsets[i] + k holds
+// particle i's k-th pair; key = owner when the pair reaches its body, else
+// the body count (never summed).
+struct PairRecords {
+  uint32_t* counts;         // per particle: body points within reach (density pass)
+  const uint32_t* offsets;  // exclusive scan of counts
+  uint32_t* key;
+  float4* force;   // x, y, z, chunk (bits)
+  float4* torque;  // x, y, z, 0
+  uint32_t none;   // the key of a pair that does not reach
 };
 
 // The body points of a substep, sorted by (cell, owner, previous place), as
 // the WCSPH passes see them.
 struct MoverView {
-  const float4* pos;     // x, y, z, owner bits
-  const float4* vel;     // vx, vy, vz, 0
-  const uint64_t* keys;  // sort keys, ascending
-  uint32_t count;        // 0: no body points this step
+  const float4* pos;      // x, y, z, owner bits
+  const float4* vel;      // vx, vy, vz, 0
+  const uint64_t* keys;   // sort keys, ascending
+  const uint32_t* table;  // dense cell table, when the grid says so
+  uint32_t count;         // 0: no body points this step
   uint32_t owner_bits;
   const MoverGrid* grid;
   const BodyState* bodies;
   float damping;  // (1 c r).min(0.25 r r / dt), step.rs:163-164
   float settles;  // |g| spacing, couple.rs:387
-  uint8_t* felt;  // per particle: 1 when a point's damping reached its body
+  PairRecords pairs;
 };
 
 // Scalars of one substep, computed on the host in the reference's f32
@@ -383,10 +401,16 @@ __device__ __forceinline__ void visit_movers(const MoverView& mv, float4 at, flo
         if (x < g.lo[0] || x > g.hi[0] || y < g.lo[1] || y > g.hi[1] || z < g.lo[2] || z > g.hi[2]) continue;
         uint64_t slot = ((uint64_t)(x - g.lo[0]) * g.span[1] + (uint64_t)(y - g.lo[1])) * g.span[2] +
                         (uint64_t)(z - g.lo[2]);
-        // the box's slots fit beside the owner bits (MoverGrid::bad unset), so
-        // slot + 1 shifted does too
-        uint32_t b = lower_bound64(mv.keys, mv.count, slot << mv.owner_bits);
-        uint32_t e = lower_bound64(mv.keys + b, mv.count - b, (slot + 1) << mv.owner_bits) + b;
+        uint32_t b, e;
+        if (g.dense) {
+          b = mv.table[slot];
+          e = mv.table[slot + 1];
+        } else {
+          // the box's slots fit beside the owner bits (MoverGrid::bad unset),
+          // so slot + 1 shifted does too
+          b = lower_bound64(mv.keys, mv.count, slot << mv.owner_bits);
+          e = lower_bound64(mv.keys + b, mv.count - b, (slot + 1) << mv.owner_bits) + b;
+        }
         for (uint32_t j = b; j < e; ++j) {
           float4 pj = mv.pos[j];
           float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;

@@ -96,6 +96,8 @@ __global__ void k_movers_box_reset(MoverGrid* g, const Control* ctl) {
     g->hi[a] = (int32_t)0x80000000u;
   }
   g->bad = 0;
+  g->dense = 0;
+  g->volume = 0;
 }
 
 __device__ __forceinline__ void carried(const float4* MS, const float4* MV, uint32_t i, float elapsed, float& x,
@@ -110,14 +112,16 @@ __global__ __launch_bounds__(256) void k_movers_cells(const float4* __restrict__
                                                       uint32_t m, float elapsed, float radius, MoverGrid* g,
                                                       const Control* ctl) {
   if (ctl->abort) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   int32_t lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
   int32_t hi[3] = {(int32_t)0x80000000u, (int32_t)0x80000000u, (int32_t)0x80000000u};
-  if (i < m) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
     float x, y, z;
     carried(MS, MV, i, elapsed, x, y, z);
     int32_t c[3] = {cell_coord(x, radius), cell_coord(y, radius), cell_coord(z, radius)};
-    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = c[a];
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = min(lo[a], c[a]);
+      hi[a] = max(hi[a], c[a]);
+    }
   }
   for (int a = 0; a < 3; ++a) {
     for (int off = 32; off >= 1; off >>= 1) {
@@ -125,10 +129,24 @@ __global__ __launch_bounds__(256) void k_movers_cells(const float4* __restrict__
       hi[a] = max(hi[a], __shfl_xor(hi[a], off, 64));
     }
   }
-  if ((threadIdx.x & 63) == 0 && blockIdx.x * blockDim.x + (threadIdx.x & ~63u) < m) {
+  // the block's box, then one set of atomics a block
+  __shared__ int32_t wlo[4][3], whi[4][3];
+  const uint32_t wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&g->lo[a], lo[a]);
-      atomicMax(&g->hi[a], hi[a]);
+      wlo[wave][a] = lo[a];
+      whi[wave][a] = hi[a];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0 && blockIdx.x * blockDim.x < m) {
+    for (int a = 0; a < 3; ++a) {
+      int32_t l = wlo[0][a], h = whi[0][a];
+      for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+        l = min(l, wlo[w][a]);
+        h = max(h, whi[w][a]);
+      }
+      atomicMin(&g->lo[a], l);
+      atomicMax(&g->hi[a], h);
     }
   }
 }
@@ -146,6 +164,7 @@ __global__ __launch_bounds__(256) void k_movers_keys(const float4* __restrict__ 
   bool fits = span[0] <= room && span[1] <= room / span[0] && span[2] <= room / (span[0] * span[1]);
   if (i == 0) {
     for (int a = 0; a < 3; ++a) g->span[a] = span[a];
+    g->volume = fits ? span[0] * span[1] * span[2] : 0;
     if (!fits) {
       g->bad = 1;
       atomicOr(&ctl->flags, kFlagMoverBox);
@@ -186,119 +205,115 @@ __global__ __launch_bounds__(256) void k_movers_gather(const float4* __restrict_
   MP[k] = make_float4(s.x + v.x * elapsed, s.y + v.y * elapsed, s.z + v.z * elapsed, s.w);
 }
 
+// Slot c of the points' dense cell table: the first point of slot c or later.
+__global__ __launch_bounds__(256) void k_mover_table(const uint64_t* __restrict__ keys, uint32_t m,
+                                                     uint32_t owner_bits, MoverGrid* g, uint32_t* __restrict__ table,
+                                                     uint64_t cap, const Control* ctl) {
+  if (ctl->abort) return;
+  const uint64_t volume = g->volume;
+  const bool dense = !g->bad && volume + 1 <= cap;
+  if (blockIdx.x == 0 && threadIdx.x == 0) g->dense = dense ? 1u : 0u;
+  if (!dense) return;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= volume;
+       c += (uint64_t)gridDim.x * blockDim.x)
+    table[c] = c == volume ? m : lower_bound64(keys, m, c << owner_bits);
+}
+
 // ---- the damping each chunk gives each body (step.rs:173-221) ------------
 
-constexpr uint32_t kChunk = 512;  // CHUNK, src/fluid/mod.rs:36
-constexpr uint32_t kNoOwner = 0xffffffffu;
-
-// Walks the pairs of chunk c that reach a body, in the reference's order
-// (particles ascending, each particle's points as `near` lists them): f
-// gets (owner, force, torque) of each, force = gradient m (damp m),
-// torque = arm x force (couple.rs:396-407).
-template <class F>
-__device__ void chunk_reaches(const float4* __restrict__ PB, const float4* __restrict__ VB, const StepParams& prm,
-                              uint32_t c, F&& f) {
-  const MoverView& mv = prm.movers;
-  uint32_t end = min(prm.n, (c + 1) * kChunk);
-  for (uint32_t i = c * kChunk; i < end; ++i) {
-    if (!mv.felt[i]) continue;
-    float4 at = PB[i], vi = VB[i];
-    float density = vi.w;
-    visit_movers(mv, at, prm.radius, prm.reach, [&](uint32_t j, float4 p, float ox, float oy, float oz, float sq) {
-      float4 v = mv.vel[j];
-      float toward = dot3(vi.x - v.x, vi.y - v.y, vi.z - v.z, ox, oy, oz);
-      float damp = toward < 0.0f ? -mv.damping * toward / (sq + prm.soften) / density : 0.0f;
-      if (toward * toward < mv.settles * sq) return;
-      float gx, gy, gz;
-      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
-      gx = gx * prm.mass;
-      gy = gy * prm.mass;
-      gz = gz * prm.mass;
-      float k = damp * prm.mass;
-      float fx = gx * k, fy = gy * k, fz = gz * k;
-      uint32_t owner = owner_of(p);
-      const BodyState& b = mv.bodies[owner];
-      float ax = p.x - (b.center[0] + b.moved[0]);
-      float ay = p.y - (b.center[1] + b.moved[1]);
-      float az = p.z - (b.center[2] + b.moved[2]);
-      float tx, ty, tz;
-      cross3(ax, ay, az, fx, fy, fz, tx, ty, tz);
-      f(owner, fx, fy, fz, tx, ty, tz);
-    });
-  }
-}
-
-// One lane per chunk: the bodies the chunk reaches, one walk per body in
-// ascending owner order plus one. WRITE = 0 counts them, 1 writes each
-// body's sums (pair by pair from +0) at the chunk's offset.
-template <int WRITE>
-__global__ __launch_bounds__(64) void k_felt(const float4* __restrict__ PB, const float4* __restrict__ VB,
-                                            StepParams prm, uint32_t chunks, uint32_t* __restrict__ counts,
-                                            const uint32_t* __restrict__ offsets, FeltRecord* __restrict__ out,
-                                            uint32_t cap) {
-  if (prm.ctl->abort) return;
-  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= chunks) return;
-  uint32_t target = kNoOwner;
-  chunk_reaches(PB, VB, prm, c, [&](uint32_t owner, float, float, float, float, float, float) {
-    target = min(target, owner);
-  });
-  uint32_t written = 0;
-  while (target != kNoOwner) {
-    uint32_t next = kNoOwner;
-    float f[3] = {0.0f, 0.0f, 0.0f}, t[3] = {0.0f, 0.0f, 0.0f};
-    chunk_reaches(PB, VB, prm, c, [&](uint32_t owner, float fx, float fy, float fz, float tx, float ty, float tz) {
-      if (owner == target) {
-        f[0] = f[0] + fx;
-        f[1] = f[1] + fy;
-        f[2] = f[2] + fz;
-        t[0] = t[0] + tx;
-        t[1] = t[1] + ty;
-        t[2] = t[2] + tz;
-      } else if (owner > target && owner < next) {
-        next = owner;
-      }
-    });
-    if (WRITE) {
-      uint32_t at = offsets[c] + written;
-      if (at < cap) {
-        FeltRecord& r = out[at];
-        r.owner = target;
-        for (int a = 0; a < 3; ++a) {
-          r.force[a] = f[a];
-          r.torque[a] = t[a];
-        }
-      } else {
-        atomicOr(&prm.ctl->flags, kFlagFeltRoom);
-      }
-    }
-    ++written;
-    target = next;
-  }
-  if (!WRITE) counts[c] = written;
-}
-
-// One lane per body: impulse += force dt over the chunks in order
-// (step.rs:218-221). Chunks that reach no body add +0, which changes
-// nothing: a sum from +0 is never -0.
-__global__ __launch_bounds__(64) void k_felt_apply(const FeltRecord* __restrict__ recs, const uint32_t* __restrict__ total,
-                                                   uint32_t cap, BodyState* __restrict__ bodies, uint32_t nbodies,
-                                                   float dt, const Control* ctl) {
+// The pair records' room against what the density pass counted: past it the
+// substep aborts before its force pass (the step runs it again with more).
+// Slots past the count take the key of no body; values number the slots.
+__global__ __launch_bounds__(256) void k_pairs_check(const uint32_t* __restrict__ total, uint32_t cap,
+                                                     uint32_t none, uint32_t* __restrict__ key,
+                                                     uint32_t* __restrict__ value, Control* ctl) {
   if (ctl->abort) return;
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbodies) return;
+  const uint32_t used = *total;
+  if (used > cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      atomicOr(&ctl->flags, kFlagPairRoom);
+      ctl->need_pairs = used;
+      ctl->abort = 1;
+    }
+    return;
+  }
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < cap; k += gridDim.x * blockDim.x) {
+    if (k >= used) key[k] = none;
+    value[k] = k;
+  }
+}
+
+// One wave per body: its reaching pairs, in particle order (a stable sort by
+// body kept it), summed per chunk from +0 and added to the body's impulse as
+// force * dt chunk by chunk (step.rs:213-221). The wave stages 64 records at
+// a time in LDS; lane 0 folds them in order.
+__global__ __launch_bounds__(64) void k_body_fold(const uint32_t* __restrict__ key, const uint32_t* __restrict__ value,
+                                                  uint32_t cap, const float4* __restrict__ force,
+                                                  const float4* __restrict__ torque, BodyState* __restrict__ bodies,
+                                                  float dt, const Control* ctl) {
+  if (ctl->abort) return;
+  const uint32_t b = blockIdx.x;
+  __shared__ float4 fs[64], ts[64];
+  uint32_t first = 0, last = 0;
+  {
+    uint32_t lo = 0, hi = cap;
+    while (lo < hi) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      if (key[mid] < b) lo = mid + 1; else hi = mid;
+    }
+    first = lo;
+    hi = cap;
+    while (lo < hi) {
+      uint32_t mid = lo + (hi - lo) / 2;
+      if (key[mid] < b + 1) lo = mid + 1; else hi = mid;
+    }
+    last = lo;
+  }
   BodyState& s = bodies[b];
   float im[3] = {s.impulse[0], s.impulse[1], s.impulse[2]};
   float an[3] = {s.angular_impulse[0], s.angular_impulse[1], s.angular_impulse[2]};
-  uint32_t n = min(*total, cap);
-  for (uint32_t r = 0; r < n; ++r) {
-    const FeltRecord& rec = recs[r];
-    if (rec.owner != b) continue;
-    for (int a = 0; a < 3; ++a) {
-      im[a] = im[a] + rec.force[a] * dt;
-      an[a] = an[a] + rec.torque[a] * dt;
+  float af[3] = {0.0f, 0.0f, 0.0f}, at[3] = {0.0f, 0.0f, 0.0f};
+  uint32_t chunk = 0xffffffffu;
+  bool open = false;
+  for (uint32_t base = first; base < last; base += 64) {
+    uint32_t k = base + threadIdx.x;
+    if (k < last) {
+      uint32_t r = value[k];
+      fs[threadIdx.x] = force[r];
+      ts[threadIdx.x] = torque[r];
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t n = min(64u, last - base);
+      for (uint32_t j = 0; j < n; ++j) {
+        float4 f = fs[j], t = ts[j];
+        uint32_t c = __float_as_uint(f.w);
+        if (!open || c != chunk) {
+          if (open)
+            for (int a = 0; a < 3; ++a) {
+              im[a] = im[a] + af[a] * dt;
+              an[a] = an[a] + at[a] * dt;
+            }
+          open = true;
+          chunk = c;
+          for (int a = 0; a < 3; ++a) af[a] = at[a] = 0.0f;
+        }
+        af[0] = af[0] + f.x;
+        af[1] = af[1] + f.y;
+        af[2] = af[2] + f.z;
+        at[0] = at[0] + t.x;
+        at[1] = at[1] + t.y;
+        at[2] = at[2] + t.z;
+      }
+    }
+    __syncthreads();
   }
+  if (threadIdx.x != 0) return;
+  if (open)
+    for (int a = 0; a < 3; ++a) {
+      im[a] = im[a] + af[a] * dt;
+      an[a] = an[a] + at[a] * dt;
+    }
   for (int a = 0; a < 3; ++a) {
     s.impulse[a] = im[a];
     s.angular_impulse[a] = an[a];
@@ -307,32 +322,59 @@ __global__ __launch_bounds__(64) void k_felt_apply(const FeltRecord* __restrict_
 
 // ---- the fastest body near water (couple.rs:321-366) ----------------------
 
-__global__ void k_extent_reset(BodyExtent* e, uint32_t nbodies) {
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbodies) return;
-  e[b].speed_bits = 0;
-  for (int a = 0; a < 3; ++a) {
-    e[b].lo[a] = ordered(3.40282347e38f);
-    e[b].hi[a] = ordered(-3.40282347e38f);
+// One block per body over its points in placement order (its own
+// contiguous range): max |v| and the box of the points where carry(0) puts
+// them (start + v 0, the start up to the sign of a zero, which no cell of
+// the box minus or plus a positive reach tells apart). Max and min are f32
+// max and min, exact in any order (couple.rs:335-338).
+__global__ __launch_bounds__(256) void k_extent_bodies(const float4* __restrict__ MS, const float4* __restrict__ MV,
+                                                       const BodyState* __restrict__ bodies,
+                                                       BodyExtent* __restrict__ e) {
+  const BodyState& b = bodies[blockIdx.x];
+  float speed = 0.0f;
+  float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+  float hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+  for (uint32_t k = threadIdx.x; k < b.points; k += blockDim.x) {
+    float4 p = MS[b.first_point + k], v = MV[b.first_point + k];
+    speed = fmaxf(speed, sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z)));
+    float c[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], c[a]);
+      hi[a] = fmaxf(hi[a], c[a]);
+    }
   }
-  e[b].wet = 0;
-}
-
-// max |v| and the box of each body's points; max and min are exact in any
-// order (f32::max and min, couple.rs:335-338)
-__global__ __launch_bounds__(256) void k_extent(const float4* __restrict__ MP, const float4* __restrict__ MV,
-                                                uint32_t m, BodyExtent* __restrict__ e) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  float4 p = MP[i], v = MV[i];
-  BodyExtent& x = e[owner_of(p)];
-  float speed = sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z));
-  if (speed > 0.0f) atomicMax(&x.speed_bits, __float_as_uint(speed));
-  float c[3] = {p.x, p.y, p.z};
-  for (int a = 0; a < 3; ++a) {
-    atomicMin(&x.lo[a], ordered(c[a]));
-    atomicMax(&x.hi[a], ordered(c[a]));
+  for (int off = 32; off >= 1; off >>= 1) {
+    speed = fmaxf(speed, __shfl_xor(speed, off, 64));
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+    }
   }
+  __shared__ float ws[4][7];
+  const uint32_t wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    ws[wave][0] = speed;
+    for (int a = 0; a < 3; ++a) {
+      ws[wave][1 + a] = lo[a];
+      ws[wave][4 + a] = hi[a];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (uint32_t w = 1; w < blockDim.x / 64; ++w) {
+    speed = fmaxf(speed, ws[w][0]);
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], ws[w][1 + a]);
+      hi[a] = fmaxf(hi[a], ws[w][4 + a]);
+    }
+  }
+  BodyExtent& x = e[blockIdx.x];
+  x.speed_bits = __float_as_uint(speed);
+  for (int a = 0; a < 3; ++a) {
+    x.lo[a] = ordered(lo[a]);
+    x.hi[a] = ordered(hi[a]);
+  }
+  x.wet = 0;
 }
 
 // Any occupied cell of the fluid's last grid inside each body's cell range
@@ -409,81 +451,135 @@ __device__ __forceinline__ bool in_box(const LiftState& l, float4 p) {
   return true;
 }
 
-// low = box - reach, high = box + reach (lift.rs:57-61); counts the water
-// inside
-__global__ __launch_bounds__(256) void k_lift_count(const float4* __restrict__ P, uint32_t n,
-                                                    LiftState* __restrict__ ls, uint32_t nbodies, float reach) {
-  uint32_t b = blockIdx.y;
+// low = box - reach, high = box + reach (lift.rs:57-61), one lane a body
+__global__ void k_lift_reach(LiftState* __restrict__ ls, uint32_t nbodies, float reach) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nbodies) return;
   LiftState& l = ls[b];
-  float lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
-    lo[a] = unordered(l.box_lo[a]) - reach;
-    hi[a] = unordered(l.box_hi[a]) + reach;
+    l.low[a] = unordered(l.box_lo[a]) - reach;
+    l.high[a] = unordered(l.box_hi[a]) + reach;
   }
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t inside = 0;
-  if (i < n) {
-    float4 p = P[i];
-    float c[3] = {p.x, p.y, p.z};
-    inside = 1;
-    for (int a = 0; a < 3; ++a)
-      if (rmin(rmax(c[a], lo[a]), hi[a]) != c[a]) inside = 0;
-  }
-  for (int off = 32; off >= 1; off >>= 1) inside += __shfl_xor(inside, off, 64);
-  if ((threadIdx.x & 63) == 0 && inside) atomicAdd(&l.near, inside);
-  if (i == 0)
-    for (int a = 0; a < 3; ++a) {
-      l.low[a] = lo[a];
-      l.high[a] = hi[a];
-    }
 }
 
-// One block per body: the water inside in particle order (a block-wide
-// compaction over every particle), and each column's highest water
+constexpr uint32_t kLiftChunk = 1024;
+
+// The box of each 1024 particles of the water (its order after the step is
+// the last sort's, so a chunk is a compact piece of it).
+__global__ __launch_bounds__(256) void k_chunk_boxes(const float4* __restrict__ P, uint32_t n,
+                                                     float4* __restrict__ boxes) {
+  const uint32_t c = blockIdx.x;
+  float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+  float hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+  for (uint32_t k = threadIdx.x; k < kLiftChunk; k += blockDim.x) {
+    uint32_t i = c * kLiftChunk + k;
+    if (i >= n) break;
+    float4 p = P[i];
+    float q[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], q[a]);
+      hi[a] = fmaxf(hi[a], q[a]);
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+    }
+  __shared__ float ws[4][6];
+  const uint32_t wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) {
+      ws[wave][a] = lo[a];
+      ws[wave][3 + a] = hi[a];
+    }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (uint32_t w = 1; w < 4; ++w)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], ws[w][a]);
+      hi[a] = fmaxf(hi[a], ws[w][3 + a]);
+    }
+  boxes[2 * c] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+  boxes[2 * c + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+}
+
+__device__ __forceinline__ bool chunk_meets(const LiftState& l, const float4* boxes, uint32_t c) {
+  float4 lo = boxes[2 * c], hi = boxes[2 * c + 1];
+  return !(hi.x < l.low[0] || lo.x > l.high[0] || hi.y < l.low[1] || lo.y > l.high[1] || hi.z < l.low[2] ||
+           lo.z > l.high[2]);
+}
+
+// Block (body, chunk): the chunk's water inside the body's box, counted into
+// counts[body * chunks + chunk].
+__global__ __launch_bounds__(256) void k_lift_count(const float4* __restrict__ P, uint32_t n,
+                                                    const LiftState* __restrict__ ls, const float4* __restrict__ boxes,
+                                                    uint32_t chunks, uint32_t* __restrict__ counts) {
+  const uint32_t b = blockIdx.x, c = blockIdx.y;
+  const LiftState& l = ls[b];
+  __shared__ uint32_t sum_s;
+  if (!chunk_meets(l, boxes, c)) {
+    if (threadIdx.x == 0) counts[(size_t)b * chunks + c] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) sum_s = 0;
+  __syncthreads();
+  uint32_t inside = 0;
+  for (uint32_t k = threadIdx.x; k < kLiftChunk; k += blockDim.x) {
+    uint32_t i = c * kLiftChunk + k;
+    if (i < n && in_box(l, P[i])) ++inside;
+  }
+  for (int off = 32; off >= 1; off >>= 1) inside += __shfl_xor(inside, off, 64);
+  if ((threadIdx.x & 63) == 0 && inside) atomicAdd(&sum_s, inside);
+  __syncthreads();
+  if (threadIdx.x == 0) counts[(size_t)b * chunks + c] = sum_s;
+}
+
+// Each body's water count and its offset in the near lists, from the scan
+// of the (body, chunk) counts.
+__global__ void k_lift_offsets(LiftState* __restrict__ ls, uint32_t nbodies, const uint32_t* __restrict__ offsets,
+                               uint32_t chunks) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbodies) return;
+  uint32_t first = offsets[(size_t)b * chunks], last = offsets[(size_t)(b + 1) * chunks];
+  ls[b].near = last - first;
+  ls[b].near_offset = first;
+}
+
+// Block (body, chunk) of 1024: the chunk's water inside the body's box in
+// particle order at its offset, and each column's highest water
 // (lift.rs:71-84) by total_cmp, as 1 << 32 | key in the column table.
 __global__ __launch_bounds__(1024) void k_lift_near(const float4* __restrict__ P, uint32_t n,
-                                                    LiftState* __restrict__ ls, uint32_t* __restrict__ near,
-                                                    unsigned long long* __restrict__ columns, float column,
-                                                    float upx, float upy, float upz) {
-  LiftState& l = ls[blockIdx.x];
-  if (l.near == 0) return;
+                                                    const LiftState* __restrict__ ls, const float4* __restrict__ boxes,
+                                                    uint32_t chunks, const uint32_t* __restrict__ offsets,
+                                                    uint32_t* __restrict__ near, unsigned long long* __restrict__ columns,
+                                                    float column, float upx, float upy, float upz) {
+  const uint32_t b = blockIdx.x, c = blockIdx.y;
+  const LiftState& l = ls[b];
+  const size_t at = (size_t)b * chunks + c;
+  if (offsets[at + 1] == offsets[at]) return;
   __shared__ uint32_t wave_sums[16];
-  __shared__ uint32_t base_s;
-  if (threadIdx.x == 0) base_s = 0;
-  __syncthreads();
-  uint32_t* out = near + l.near_offset;
-  unsigned long long* table = columns + l.column_offset;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t first = 0; first < n; first += 1024) {
-    uint32_t i = first + threadIdx.x;
-    bool inside = false;
-    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (i < n) {
-      p = P[i];
-      inside = in_box(l, p);
-    }
-    unsigned long long ballot = __ballot(inside);
-    uint32_t before = __popcll(ballot & ((1ull << lane) - 1ull));
-    if (lane == 0) wave_sums[wave] = __popcll(ballot);
-    __syncthreads();
-    uint32_t offset = base_s;
-    for (uint32_t w = 0; w < wave; ++w) offset += wave_sums[w];
-    if (inside) {
-      out[offset + before] = i;
-      int64_t kx = (int64_t)cell_coord(p.x, column) - l.col_lo[0];
-      int64_t kz = (int64_t)cell_coord(p.z, column) - l.col_lo[1];
-      float height = dot3(p.x, p.y, p.z, upx, upy, upz);
-      atomicMax(&table[kx * l.col_span[1] + kz], (1ull << 32) | total_key(height));
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t all = 0;
-      for (uint32_t w = 0; w < 16; ++w) all += wave_sums[w];
-      base_s += all;
-    }
-    __syncthreads();
+  const uint32_t i = c * kLiftChunk + threadIdx.x;
+  bool inside = false;
+  float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (i < n) {
+    p = P[i];
+    inside = in_box(l, p);
   }
+  unsigned long long ballot = __ballot(inside);
+  uint32_t before = __popcll(ballot & ((1ull << lane) - 1ull));
+  if (lane == 0) wave_sums[wave] = __popcll(ballot);
+  __syncthreads();
+  if (!inside) return;
+  uint32_t offset = offsets[at];
+  for (uint32_t w = 0; w < wave; ++w) offset += wave_sums[w];
+  near[offset + before] = i;
+  unsigned long long* table = columns + l.column_offset;
+  int64_t kx = (int64_t)cell_coord(p.x, column) - l.col_lo[0];
+  int64_t kz = (int64_t)cell_coord(p.z, column) - l.col_lo[1];
+  float height = dot3(p.x, p.y, p.z, upx, upy, upz);
+  atomicMax(&table[kx * l.col_span[1] + kz], (1ull << 32) | total_key(height));
 }
 
 // One block per body: the columns' tops sorted (bitonic, total_cmp), the
@@ -547,50 +643,83 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
       }
       __syncthreads();
     }
-  if (threadIdx.x != 0) return;
   // the mean of the middle half, plus half a spacing
-  uint32_t quarter = count / 4;
-  float sum = 0.0f;
-  for (uint32_t k = quarter; k < count - quarter; ++k) sum = sum + total_value(keys[k]);
-  float level = sum / (float)(count - 2 * quarter) + half_spacing;
-  // the flow of the water under the level, in particle order
+  __shared__ float level_s;
+  __shared__ float4 buf[1024];
+  if (threadIdx.x == 0) {
+    uint32_t quarter = count / 4;
+    float sum = 0.0f;
+    for (uint32_t k = quarter; k < count - quarter; ++k) sum = sum + total_value(keys[k]);
+    level_s = sum / (float)(count - 2 * quarter) + half_spacing;
+  }
+  __syncthreads();
+  const float level = level_s;
+  // the flow of the water under the level, in particle order: the block
+  // stages 1024 particles at a time, lane 0 folds them
   const uint32_t* mine = near + l.near_offset;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   uint32_t wet = 0;
-  for (uint32_t k = 0; k < l.near; ++k) {
-    uint32_t i = mine[k];
-    float4 p = P[i];
-    if (dot3(p.x, p.y, p.z, upx, upy, upz) <= level) {
+  for (uint32_t base = 0; base < l.near; base += 1024) {
+    uint32_t k = base + threadIdx.x;
+    if (k < l.near) {
+      uint32_t i = mine[k];
+      float4 p = P[i];
       float4 v = V[i];
-      fx = fx + v.x;
-      fy = fy + v.y;
-      fz = fz + v.z;
-      wet += 1;
+      buf[threadIdx.x] = make_float4(v.x, v.y, v.z, dot3(p.x, p.y, p.z, upx, upy, upz) <= level ? 1.0f : 0.0f);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t n = min(1024u, l.near - base);
+      for (uint32_t j = 0; j < n; ++j) {
+        float4 b = buf[j];
+        if (b.w != 0.0f) {
+          fx = fx + b.x;
+          fy = fy + b.y;
+          fz = fz + b.z;
+          wet += 1;
+        }
+      }
+    }
+    __syncthreads();
   }
+  // the probes' shares under the level: wetted, middle, the waterline
+  float spacing = body.probe_spacing;
+  float wetted = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+  uint32_t line = 0;
+  for (uint32_t base = 0; base < body.probes; base += 1024) {
+    uint32_t k = base + threadIdx.x;
+    if (k < body.probes) {
+      float4 a = arm[body.first_probe + k];
+      float cx = body.center[0] + a.x, cy = body.center[1] + a.y, cz = body.center[2] + a.z;
+      float depth = level - dot3(cx, cy, cz, upx, upy, upz);
+      float share = depth / spacing + 0.5f;
+      if (share < 0.0f) share = 0.0f;
+      if (share > 1.0f) share = 1.0f;
+      // a share that is not > 0 (0 or NaN) leaves the probe out
+      buf[threadIdx.x] = share > 0.0f ? make_float4(a.x * share, a.y * share, a.z * share, share)
+                                      : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t n = min(1024u, body.probes - base);
+      for (uint32_t j = 0; j < n; ++j) {
+        float4 b = buf[j];
+        if (!(b.w > 0.0f)) continue;
+        wetted = wetted + b.w;
+        mx = mx + b.x;
+        my = my + b.y;
+        mz = mz + b.z;
+        if (b.w < 1.0f) ++line;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   float w = (float)(wet > 1 ? wet : 1);
   l.flow[0] = fx / w;
   l.flow[1] = fy / w;
   l.flow[2] = fz / w;
   l.level = level;
-  // the probes' shares under the level: wetted, middle
-  float spacing = body.probe_spacing;
-  float wetted = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
-  uint32_t line = 0;
-  for (uint32_t k = 0; k < body.probes; ++k) {
-    float4 a = arm[body.first_probe + k];
-    float cx = body.center[0] + a.x, cy = body.center[1] + a.y, cz = body.center[2] + a.z;
-    float depth = level - dot3(cx, cy, cz, upx, upy, upz);
-    float share = depth / spacing + 0.5f;
-    if (share < 0.0f) share = 0.0f;
-    if (share > 1.0f) share = 1.0f;
-    if (!(share > 0.0f)) continue;
-    wetted = wetted + share;
-    mx = mx + a.x * share;
-    my = my + a.y * share;
-    mz = mz + a.z * share;
-    if (share < 1.0f) ++line;
-  }
   l.wetted = wetted;
   l.volume = wetted * (spacing * (spacing * spacing));
   l.middle[0] = mx;
@@ -599,69 +728,96 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
   l.line = line;
 }
 
-// One lane per body, after the host has set area = volume^(2/3) / wetted:
-// buoyancy, drag and the waterplane's damping, as impulses (lift.rs:127-150)
-__global__ void k_lift_force(LiftState* __restrict__ ls, const BodyState* __restrict__ bodies, uint32_t nbodies,
-                             const float4* __restrict__ arm, const float4* __restrict__ vel, float rest,
-                             float gravity, float upx, float upy, float upz, float dt) {
-  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbodies) return;
+// One block per body, after the host has set area = volume^(2/3) / wetted:
+// buoyancy, then the drag on every probe under, then the waterplane's
+// damping on every probe on the waterline, as impulses (lift.rs:127-150).
+// The block works each probe's term out in parallel; lane 0 sums them in the
+// reference's order.
+__global__ __launch_bounds__(256) void k_lift_force(LiftState* __restrict__ ls, const BodyState* __restrict__ bodies,
+                                                    const float4* __restrict__ arm, const float4* __restrict__ vel,
+                                                    float rest, float gravity, float upx, float upy, float upz,
+                                                    float dt) {
+  const uint32_t b = blockIdx.x;
   LiftState& l = ls[b];
-  l.lifted = 0;
-  if (l.near == 0 || l.wetted <= 0.0f) return;
+  if (l.near == 0 || l.wetted <= 0.0f) {
+    if (threadIdx.x == 0) l.lifted = 0;
+    return;
+  }
   const BodyState& body = bodies[b];
-  float spacing = body.probe_spacing;
-  float mx = l.middle[0] / l.wetted, my = l.middle[1] / l.wetted, mz = l.middle[2] / l.wetted;
-  float lift = rest * gravity * l.volume;
-  float bx = upx * lift, by = upy * lift, bz = upz * lift;
-  float f[3] = {0.0f + bx, 0.0f + by, 0.0f + bz};
-  float t[3];
-  cross3(mx, my, mz, bx, by, bz, t[0], t[1], t[2]);
-  t[0] = 0.0f + t[0];
-  t[1] = 0.0f + t[1];
-  t[2] = 0.0f + t[2];
-  auto shares = [&](auto&& each) {
-    for (uint32_t k = 0; k < body.probes; ++k) {
-      float4 a = arm[body.first_probe + k];
-      float cx = body.center[0] + a.x, cy = body.center[1] + a.y, cz = body.center[2] + a.z;
-      float depth = l.level - dot3(cx, cy, cz, upx, upy, upz);
-      float share = depth / spacing + 0.5f;
-      if (share < 0.0f) share = 0.0f;
-      if (share > 1.0f) share = 1.0f;
-      if (share > 0.0f) each(a, vel[body.first_probe + k], share);
+  const float spacing = body.probe_spacing;
+  __shared__ float4 fs[256], ts[256];
+  float f[3], t[3];
+  if (threadIdx.x == 0) {
+    float mx = l.middle[0] / l.wetted, my = l.middle[1] / l.wetted, mz = l.middle[2] / l.wetted;
+    float lift = rest * gravity * l.volume;
+    float bx = upx * lift, by = upy * lift, bz = upz * lift;
+    f[0] = 0.0f + bx;
+    f[1] = 0.0f + by;
+    f[2] = 0.0f + bz;
+    cross3(mx, my, mz, bx, by, bz, t[0], t[1], t[2]);
+    t[0] = 0.0f + t[0];
+    t[1] = 0.0f + t[1];
+    t[2] = 0.0f + t[2];
+  }
+  const float plane = (float)l.line * spacing * spacing;
+  const float waves = rest * plane * sqrtf(gravity * sqrtf(plane));
+  const float each = waves / (float)(l.line > 1 ? l.line : 1);
+  for (int pass = 0; pass < 2; ++pass) {
+    for (uint32_t base = 0; base < body.probes; base += 256) {
+      uint32_t k = base + threadIdx.x;
+      float4 fo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), to = fo;
+      if (k < body.probes) {
+        float4 a = arm[body.first_probe + k], v = vel[body.first_probe + k];
+        float cx = body.center[0] + a.x, cy = body.center[1] + a.y, cz = body.center[2] + a.z;
+        float depth = l.level - dot3(cx, cy, cz, upx, upy, upz);
+        float share = depth / spacing + 0.5f;
+        if (share < 0.0f) share = 0.0f;
+        if (share > 1.0f) share = 1.0f;
+        float dx, dy, dz;
+        bool take;
+        if (pass == 0) {
+          // drag on the probe's share of the wetted area
+          float px = l.flow[0] - v.x, py = l.flow[1] - v.y, pz = l.flow[2] - v.z;
+          float kk = 0.5f * 1.0f * rest * l.area * share * sqrtf(dot3(px, py, pz, px, py, pz));
+          dx = px * kk;
+          dy = py * kk;
+          dz = pz * kk;
+          take = share > 0.0f;
+        } else {
+          // waves off the waterplane
+          float kk = each * dot3(l.flow[0] - v.x, l.flow[1] - v.y, l.flow[2] - v.z, upx, upy, upz);
+          dx = upx * kk;
+          dy = upy * kk;
+          dz = upz * kk;
+          take = share > 0.0f && share < 1.0f;
+        }
+        if (take) {
+          float qx, qy, qz;
+          cross3(a.x, a.y, a.z, dx, dy, dz, qx, qy, qz);
+          fo = make_float4(dx, dy, dz, 1.0f);
+          to = make_float4(qx, qy, qz, 0.0f);
+        }
+      }
+      fs[threadIdx.x] = fo;
+      ts[threadIdx.x] = to;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t n = min(256u, body.probes - base);
+        for (uint32_t j = 0; j < n; ++j) {
+          float4 a = fs[j], c = ts[j];
+          if (a.w == 0.0f) continue;
+          f[0] = f[0] + a.x;
+          f[1] = f[1] + a.y;
+          f[2] = f[2] + a.z;
+          t[0] = t[0] + c.x;
+          t[1] = t[1] + c.y;
+          t[2] = t[2] + c.z;
+        }
+      }
+      __syncthreads();
     }
-  };
-  // drag on each probe's share of the wetted area
-  shares([&](float4 a, float4 v, float share) {
-    float px = l.flow[0] - v.x, py = l.flow[1] - v.y, pz = l.flow[2] - v.z;
-    float k = 0.5f * 1.0f * rest * l.area * share * sqrtf(dot3(px, py, pz, px, py, pz));
-    float dx = px * k, dy = py * k, dz = pz * k;
-    float cx, cy, cz;
-    cross3(a.x, a.y, a.z, dx, dy, dz, cx, cy, cz);
-    f[0] = f[0] + dx;
-    f[1] = f[1] + dy;
-    f[2] = f[2] + dz;
-    t[0] = t[0] + cx;
-    t[1] = t[1] + cy;
-    t[2] = t[2] + cz;
-  });
-  // waves off the waterplane
-  float plane = (float)l.line * spacing * spacing;
-  float waves = rest * plane * sqrtf(gravity * sqrtf(plane));
-  float each = waves / (float)(l.line > 1 ? l.line : 1);
-  shares([&](float4 a, float4 v, float share) {
-    if (!(share < 1.0f)) return;
-    float k = each * dot3(l.flow[0] - v.x, l.flow[1] - v.y, l.flow[2] - v.z, upx, upy, upz);
-    float dx = upx * k, dy = upy * k, dz = upz * k;
-    float cx, cy, cz;
-    cross3(a.x, a.y, a.z, dx, dy, dz, cx, cy, cz);
-    f[0] = f[0] + dx;
-    f[1] = f[1] + dy;
-    f[2] = f[2] + dz;
-    t[0] = t[0] + cx;
-    t[1] = t[1] + cy;
-    t[2] = t[2] + cz;
-  });
+  }
+  if (threadIdx.x != 0) return;
   for (int a = 0; a < 3; ++a) {
     l.impulse[a] = f[a] * dt;
     l.angular[a] = t[a] * dt;
@@ -698,7 +854,8 @@ hipError_t mover_sort_bytes(uint32_t m, size_t* bytes) {
 hipError_t launch_movers_carry(hipStream_t s, const MoverCarry& c, float elapsed, float radius, Control* ctl) {
   if (c.m == 0) return hipSuccess;
   hipLaunchKernelGGL(k_movers_box_reset, 1, 64, 0, s, c.grid, ctl);
-  hipLaunchKernelGGL(k_movers_cells, blocks_for(c.m, 256), 256, 0, s, c.ms, c.mv, c.m, elapsed, radius, c.grid, ctl);
+  const uint32_t cell_blocks = blocks_for(c.m, 256) < 256 ? blocks_for(c.m, 256) : 256;
+  hipLaunchKernelGGL(k_movers_cells, cell_blocks, 256, 0, s, c.ms, c.mv, c.m, elapsed, radius, c.grid, ctl);
   hipLaunchKernelGGL(k_movers_keys, blocks_for(c.m, 256), 256, 0, s, c.ms, c.mv, c.m, elapsed, radius, c.owner_bits,
                      c.grid, c.keys_in, c.perm_in, ctl);
   size_t bytes = c.temp_bytes;
@@ -707,27 +864,39 @@ hipError_t launch_movers_carry(hipStream_t s, const MoverCarry& c, float elapsed
   uint32_t threads = c.m > c.nbodies ? c.m : c.nbodies;
   hipLaunchKernelGGL(k_movers_gather, blocks_for(threads, 256), 256, 0, s, c.ms, c.mv, c.perm, c.m, elapsed, c.ms2,
                      c.mv2, c.mp, c.bodies, c.nbodies, ctl);
+  if (c.table_cap)
+    hipLaunchKernelGGL(k_mover_table, 1024, 256, 0, s, c.keys, c.m, c.owner_bits, c.grid, c.table, c.table_cap, ctl);
   return hipSuccess;
 }
 
-void launch_felt(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, uint32_t* counts,
-                 uint32_t* offsets, uint32_t* tile_sums, uint32_t* total, FeltRecord* recs, uint32_t rec_cap,
-                 BodyState* bodies, uint32_t nbodies) {
-  if (prm.n == 0 || prm.movers.count == 0) return;
-  uint32_t chunks = blocks_for(prm.n, kChunk);
-  hipLaunchKernelGGL(k_felt<0>, blocks_for(chunks, 64), 64, 0, s, PB, VB, prm, chunks, counts, nullptr, nullptr,
-                     rec_cap);
-  launch_scan(s, counts, offsets, tile_sums, total, nullptr, chunks, 0, prm.ctl);
-  hipLaunchKernelGGL(k_felt<1>, blocks_for(chunks, 64), 64, 0, s, PB, VB, prm, chunks, nullptr, offsets, recs,
-                     rec_cap);
-  hipLaunchKernelGGL(k_felt_apply, blocks_for(nbodies, 64), 64, 0, s, recs, total, rec_cap, bodies, nbodies, prm.dt,
-                     prm.ctl);
+void launch_pairs_prepare(hipStream_t s, const PairRoom& room, uint32_t n, Control* ctl) {
+  if (n == 0) return;
+  launch_scan(s, room.counts, room.offsets, room.tiles, room.total, nullptr, n, 0, ctl);
+  hipLaunchKernelGGL(k_pairs_check, 1024, 256, 0, s, room.total, room.cap, room.none, room.key, room.value, ctl);
 }
 
-void launch_extent(hipStream_t s, const float4* MP, const float4* MV, uint32_t m, BodyExtent* e, uint32_t nbodies) {
+hipError_t pair_sort_bytes(uint32_t cap, size_t* bytes) {
+  *bytes = 0;
+  return rocprim::radix_sort_pairs(nullptr, *bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                   (const uint32_t*)nullptr, (uint32_t*)nullptr, cap, 0, 32);
+}
+
+hipError_t launch_pairs_fold(hipStream_t s, const PairRoom& room, BodyState* bodies, uint32_t nbodies, float dt,
+                             const Control* ctl) {
+  if (nbodies == 0) return hipSuccess;
+  size_t bytes = room.temp_bytes;
+  hipError_t e = rocprim::radix_sort_pairs(room.temp, bytes, room.key, room.key_sorted, room.value,
+                                           room.value_sorted, room.cap, 0, (int)room.key_bits, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_body_fold, nbodies, 64, 0, s, room.key_sorted, room.value_sorted, room.cap, room.force,
+                     room.torque, bodies, dt, ctl);
+  return hipSuccess;
+}
+
+void launch_extent(hipStream_t s, const float4* MS, const float4* MV, const BodyState* bodies, BodyExtent* e,
+                   uint32_t nbodies) {
   if (nbodies == 0) return;
-  hipLaunchKernelGGL(k_extent_reset, blocks_for(nbodies, 64), 64, 0, s, e, nbodies);
-  if (m) hipLaunchKernelGGL(k_extent, blocks_for(m, 256), 256, 0, s, MP, MV, m, e);
+  hipLaunchKernelGGL(k_extent_bodies, nbodies, 256, 0, s, MS, MV, bodies, e);
 }
 
 void launch_extent_wet(hipStream_t s, const uint32_t* start, const Control* ctl, const WetQuery* q, uint32_t nq,
@@ -737,19 +906,31 @@ void launch_extent_wet(hipStream_t s, const uint32_t* start, const Control* ctl,
 }
 
 void launch_lift_box(hipStream_t s, const float4* probes, const BodyState* bodies, uint32_t nbodies,
-                     uint32_t most_probes, float4* arm, float4* vel, LiftState* ls, const float4* P, uint32_t n,
-                     float reach) {
+                     uint32_t most_probes, float4* arm, float4* vel, LiftState* ls, float reach) {
   if (nbodies == 0) return;
   hipLaunchKernelGGL(k_lift_reset, blocks_for(nbodies, 64), 64, 0, s, ls, nbodies);
   hipLaunchKernelGGL(k_lift_probes, dim3(blocks_for(most_probes, 256), nbodies), 256, 0, s, probes, bodies, nbodies,
                      arm, vel, ls);
-  if (n) hipLaunchKernelGGL(k_lift_count, dim3(blocks_for(n, 256), nbodies), 256, 0, s, P, n, ls, nbodies, reach);
+  hipLaunchKernelGGL(k_lift_reach, blocks_for(nbodies, 64), 64, 0, s, ls, nbodies, reach);
 }
 
-void launch_lift_near(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies, uint32_t* near,
-                      unsigned long long* columns, float column, const float up[3]) {
-  if (nbodies == 0) return;
-  hipLaunchKernelGGL(k_lift_near, nbodies, 1024, 0, s, P, n, ls, near, columns, column, up[0], up[1], up[2]);
+void launch_lift_count(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies, float4* boxes,
+                       uint32_t* counts, uint32_t* offsets, uint32_t* tiles, uint32_t* total) {
+  if (nbodies == 0 || n == 0) return;
+  const uint32_t chunks = blocks_for(n, kLiftChunk);
+  hipLaunchKernelGGL(k_chunk_boxes, chunks, 256, 0, s, P, n, boxes);
+  hipLaunchKernelGGL(k_lift_count, dim3(nbodies, chunks), 256, 0, s, P, n, ls, boxes, chunks, counts);
+  launch_scan(s, counts, offsets, tiles, total, nullptr, (uint64_t)nbodies * chunks, 0, nullptr);
+  hipLaunchKernelGGL(k_lift_offsets, blocks_for(nbodies, 64), 64, 0, s, ls, nbodies, offsets, chunks);
+}
+
+void launch_lift_near(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies,
+                      const float4* boxes, const uint32_t* offsets, uint32_t* near, unsigned long long* columns,
+                      float column, const float up[3]) {
+  if (nbodies == 0 || n == 0) return;
+  const uint32_t chunks = blocks_for(n, kLiftChunk);
+  hipLaunchKernelGGL(k_lift_near, dim3(nbodies, chunks), 1024, 0, s, P, n, ls, boxes, chunks, offsets, near, columns,
+                     column, up[0], up[1], up[2]);
 }
 
 void launch_lift_level(hipStream_t s, const float4* P, const float4* V, LiftState* ls, const BodyState* bodies,
@@ -763,8 +944,7 @@ void launch_lift_level(hipStream_t s, const float4* P, const float4* V, LiftStat
 void launch_lift_force(hipStream_t s, LiftState* ls, const BodyState* bodies, uint32_t nbodies, const float4* arm,
                        const float4* vel, float rest, float gravity, const float up[3], float dt) {
   if (nbodies == 0) return;
-  hipLaunchKernelGGL(k_lift_force, blocks_for(nbodies, 64), 64, 0, s, ls, bodies, nbodies, arm, vel, rest, gravity,
-                     up[0], up[1], up[2], dt);
+  hipLaunchKernelGGL(k_lift_force, nbodies, 256, 0, s, ls, bodies, arm, vel, rest, gravity, up[0], up[1], up[2], dt);
 }
 
 void launch_give_back(hipStream_t s, const uint32_t* near, uint32_t count, const float change[3], float4* V) {

@@ -37,11 +37,15 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
                  });
     float density;
     if (MOVERS) {
-      // the coupled bodies' points (mod.rs:648-656)
+      // the coupled bodies' points (mod.rs:648-656); their count sizes the
+      // force pass's pair records
       float msum = 0.0f;
+      uint32_t count = 0;
       visit_movers(prm.movers, at, prm.radius, prm.reach, [&](uint32_t, float4, float, float, float, float sq) {
         msum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+        ++count;
       });
+      prm.movers.pairs.counts[i] = count;
       density = prm.mass * (sum + wsum + msum);
     } else {
       density = prm.mass * (sum + wsum);
@@ -56,11 +60,10 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
 
 // The push of wcsph_substep's force loop (step.rs:173-211): fluid pairs,
 // walls, the coupled bodies' points (mover_push, src/fluid/couple.rs:
-// 368-408), then + wall_viscous (step.rs:262-301) + gravity. Returns whether
-// some body point's damping reached its body (the pair counts in the body's
-// sum).
-template <bool MOVERS>
-__device__ __forceinline__ bool wcsph_push(const float4* __restrict__ PB, const float4* __restrict__ VB,
+// 368-408), then + wall_viscous (step.rs:262-301) + gravity. With RECORD,
+// each body-point pair's damping on its body goes to the pair records.
+template <bool MOVERS, bool RECORD>
+__device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                            const float* __restrict__ scaled, const StepParams& prm, uint32_t i,
                                            float4 at, float4 vi, float& ax, float& ay, float& az) {
   const Control* ctl = prm.ctl;
@@ -106,10 +109,11 @@ __device__ __forceinline__ bool wcsph_push(const float4* __restrict__ PB, const 
       awz = awz + -gz;
     });
   }
-  bool reached = false;
   if (MOVERS) {
     const MoverView& mv = prm.movers;
-    visit_movers(mv, at, prm.radius, prm.reach, [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+    uint32_t slot = RECORD ? mv.pairs.offsets[i] : 0;
+    const float chunk = __uint_as_float(i / 512u);  // CHUNK, src/fluid/mod.rs:36
+    visit_movers(mv, at, prm.radius, prm.reach, [&](uint32_t j, float4 p, float ox, float oy, float oz, float sq) {
       float4 v = mv.vel[j];
       float toward = dot3(vi.x - v.x, vi.y - v.y, vi.z - v.z, ox, oy, oz);
       float damp = toward < 0.0f ? -mv.damping * toward / (sq + prm.soften) / density : 0.0f;
@@ -122,7 +126,24 @@ __device__ __forceinline__ bool wcsph_push(const float4* __restrict__ PB, const 
       px = px - gx * k;
       py = py - gy * k;
       pz = pz - gz * k;
-      if (!(toward * toward < mv.settles * sq)) reached = true;
+      if (!RECORD) return;
+      if (toward * toward < mv.settles * sq) {
+        mv.pairs.key[slot++] = mv.pairs.none;
+        return;
+      }
+      // force = gradient m (damp m), torque = arm x force, arm from the
+      // body's centre moved on (couple.rs:400-407)
+      float km = damp * prm.mass;
+      float fx = gx * km, fy = gy * km, fz = gz * km;
+      uint32_t owner = __float_as_uint(p.w);
+      const BodyState& b = mv.bodies[owner];
+      float rx = p.x - (b.center[0] + b.moved[0]);
+      float ry = p.y - (b.center[1] + b.moved[1]);
+      float rz = p.z - (b.center[2] + b.moved[2]);
+      mv.pairs.key[slot] = owner;
+      mv.pairs.force[slot] = make_float4(fx, fy, fz, chunk);
+      mv.pairs.torque[slot] = make_float4(ry * fz - fy * rz, rz * fx - fz * rx, rx * fy - fx * ry, 0.0f);
+      ++slot;
     });
   }
   if (prm.wall_cells) wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
@@ -130,7 +151,6 @@ __device__ __forceinline__ bool wcsph_push(const float4* __restrict__ PB, const 
   ax = px + wx + prm.gx;
   ay = py + wy + prm.gy;
   az = pz + wz + prm.gz;
-  return reached;
 }
 
 // The force loop of wcsph_substep (step.rs:173-211) with wall_viscous
@@ -154,8 +174,7 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
     float4 vi = VB[i];
     float density = vi.w;
     float ax, ay, az;
-    bool reached = wcsph_push<MOVERS>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
-    if (MOVERS) prm.movers.felt[i] = reached ? 1 : 0;
+    wcsph_push<MOVERS, MOVERS>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
     if (forces) {
       forces[3 * i] = ax;
       forces[3 * i + 1] = ay;
@@ -200,7 +219,7 @@ __global__ __launch_bounds__(256) void k_wcsph_repair(const float4* __restrict__
   float4 vi = VB[i];
   float density = vi.w;
   float ax, ay, az;
-  wcsph_push<MOVERS>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
+  wcsph_push<MOVERS, false>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
   PA[i] = at;
   VA[i] = make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density);
 }

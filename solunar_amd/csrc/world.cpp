@@ -966,16 +966,20 @@ int32_t World::enqueue_wcsph_substep(const StepParams& base, uint32_t index, flo
   phase_begin(SFL_PHASE_DENSITY);
   launch_wcsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm);
   phase_end(SFL_PHASE_DENSITY);
+  if (movers_m_) {
+    phase_begin(SFL_PHASE_BODIES);
+    launch_pairs_prepare(stream_, pair_room(), prm.n, ctl_);
+    phase_end(SFL_PHASE_BODIES);
+  }
   phase_begin(SFL_PHASE_FORCE);
   launch_wcsph_force(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm, pa_.ptr, va_.ptr,
                      keep_forces_ ? forces_.ptr : nullptr);
   phase_end(SFL_PHASE_FORCE);
   if (movers_m_) {
     phase_begin(SFL_PHASE_BODIES);
-    launch_felt(stream_, pb_.ptr, vb_.ptr, prm, felt_counts_.ptr, felt_offsets_.ptr, felt_tiles_.ptr,
-                felt_total_.ptr, felt_recs_.ptr, (uint32_t)felt_recs_.cap, bodies_d_.ptr,
-                (uint32_t)coupled_.size());
+    hipError_t e = launch_pairs_fold(stream_, pair_room(), bodies_d_.ptr, (uint32_t)coupled_.size(), prm.dt, ctl_);
     phase_end(SFL_PHASE_BODIES);
+    if (e != hipSuccess) return device_error("radix sort (body pairs)", e);
   }
   return SFL_OK;
 }
@@ -1103,14 +1107,13 @@ int32_t World::wcsph(float dt, sfl_report& r) {
   uint32_t done = 0, held = 0;
   lists_valid_ = false;
   if (movers_m_) {
-    // a flag per particle, and per 512-particle chunk the bodies it reaches
-    const size_t chunks = (n_ + 511) / 512, nb = coupled_.size();
-    SFL_HIP(felt_.reserve(n_, 0, stream_), "hipMalloc(body reach flags)");
-    SFL_HIP(felt_counts_.reserve(chunks, 0, stream_), "hipMalloc(body sums)");
-    SFL_HIP(felt_offsets_.reserve(chunks + 1, 0, stream_), "hipMalloc(body sums)");
-    SFL_HIP(felt_tiles_.reserve(chunks / 256 + 2, 0, stream_), "hipMalloc(body sums)");
-    SFL_HIP(felt_total_.reserve(1, 0, stream_), "hipMalloc(body sums)");
-    SFL_HIP(felt_recs_.reserve(chunks * std::min<size_t>(nb, 64), 0, stream_), "hipMalloc(body sums)");
+    // per particle its body points in reach, and room for their pair records
+    SFL_HIP(pair_counts_b_.reserve(n_, 0, stream_), "hipMalloc(body pairs)");
+    SFL_HIP(pair_offsets_b_.reserve(n_ + 1, 0, stream_), "hipMalloc(body pairs)");
+    SFL_HIP(pair_tiles_b_.reserve(n_ / 2048 + 2, 0, stream_), "hipMalloc(body pairs)");
+    SFL_HIP(pair_total_b_.reserve(1, 0, stream_), "hipMalloc(body pairs)");
+    st = ensure_pair_room(std::max<uint64_t>(body_pair_cap_, std::max<uint64_t>((uint64_t)movers_m_ * 16, 65536)));
+    if (st != SFL_OK) return st;
   }
   while (done < substeps) {
     launch_step_begin(stream_, ctl_);
@@ -1131,11 +1134,7 @@ int32_t World::wcsph(float dt, sfl_report& r) {
       detail_ = "the coupled bodies' points spread over more cells than their sort key holds";
       return SFL_UNSUPPORTED;
     }
-    if (c.flags & kFlagFeltRoom) {
-      detail_ = "a chunk of water reached more coupled bodies than its records hold";
-      return SFL_UNSUPPORTED;
-    }
-    if (c.flags & kFlagOverflow) {
+    if (c.flags & (kFlagOverflow | kFlagPairRoom)) {
       uint32_t completed = c.substeps_done ? c.substeps_done - 1 : 0;
       held += totals().held;
       done += completed;
@@ -1145,7 +1144,8 @@ int32_t World::wcsph(float dt, sfl_report& r) {
         lists_on_b_ = true;
         lists_emitted_ = false;
       }
-      st = ensure_cells(c.need_volume);
+      if (c.flags & kFlagOverflow) st = ensure_cells(c.need_volume);
+      if (st == SFL_OK && (c.flags & kFlagPairRoom)) st = ensure_pair_room(c.need_pairs + c.need_pairs / 4 + 1024);
       if (st != SFL_OK) return st;
       prm = params(small);
       continue;
@@ -1502,7 +1502,14 @@ size_t World::device_bytes() const {
          b(slot_rank_) + b(scatter_) + b(counts_) + b(cell_start_) + b(tile_sums_) + b(scan_total_) + b(pairs_) +
          b(masks_) + b(own_slot_) + b(near_) + b(factor_) +
          b(pair_counts_) + b(stop_sum_) + b(stop_count_) + b(walls_d_) + b(wall_start_d_) + b(fstart_) + b(fitems_) +
-         b(wstart_) + b(witems_) + b(list_counts_);
+         b(wstart_) + b(witems_) + b(list_counts_) + b(flow_flags_) + b(flow_start_) + b(flow_tiles_) +
+         b(flow_total_) + b(body_local_) + b(probe_local_) + b(bodies_d_) + b(ms_[0]) + b(ms_[1]) + b(mv_[0]) +
+         b(mv_[1]) + b(mp_) + b(mkeys_in_) + b(mkeys_) + b(mperm_in_) + b(mperm_) + b(msort_temp_) + b(mgrid_) +
+         b(mtable_) + b(pair_counts_b_) + b(pair_offsets_b_) + b(pair_tiles_b_) + b(pair_total_b_) +
+         b(pair_key_[0]) + b(pair_key_[1]) + b(pair_value_[0]) + b(pair_value_[1]) + b(pair_force_) +
+         b(pair_torque_) + b(pair_temp_) +
+         b(extent_) + b(wet_) + b(lift_) + b(probe_arm_) + b(probe_vel_) + b(lift_near_) + b(lift_sort_) +
+         b(lift_columns_) + b(lift_counts_) + b(lift_offsets_) + b(lift_tiles_) + b(lift_boxes_);
 }
 
 int32_t World::synchronize() {

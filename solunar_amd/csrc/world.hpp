@@ -172,7 +172,7 @@ class World {
   int32_t emit(const sfl_vec3& low, const sfl_vec3& high, const sfl_vec3& velocity, uint32_t& added);
   int32_t read_count(const uint32_t* dev, uint32_t* out);
   size_t coupled_points() const;
-  int32_t place_movers(const sfl_body_pose* poses, size_t count);
+  int32_t place_movers(const sfl_body_pose* poses, size_t count, float dt);
   int32_t carry_movers(float elapsed);
   int32_t replay_movers(uint32_t done, float small);
   MoverView mover_view(float dt) const;
@@ -270,6 +270,7 @@ class World {
     std::vector<float> points, probes;
     float probe_spacing;
   };
+  std::vector<float> body_reach_;  // each body's farthest point from its centre
   std::vector<Coupled> coupled_;
   bool coupled_dirty_ = true;  // the device copies of the points are stale
   uint32_t most_probes_ = 0;
@@ -285,14 +286,21 @@ class World {
   DeviceBuffer<uint32_t> mperm_in_, mperm_;
   DeviceBuffer<uint8_t> msort_temp_;
   DeviceBuffer<MoverGrid> mgrid_;
-  DeviceBuffer<uint8_t> felt_;
-  DeviceBuffer<uint32_t> felt_counts_, felt_offsets_, felt_tiles_, felt_total_;
-  DeviceBuffer<FeltRecord> felt_recs_;
+  DeviceBuffer<uint32_t> mtable_;  // the points' dense cell table
+  // the force pass's body-point pair records and their sort by body
+  DeviceBuffer<uint32_t> pair_counts_b_, pair_offsets_b_, pair_tiles_b_, pair_total_b_;
+  DeviceBuffer<uint32_t> pair_key_[2], pair_value_[2];
+  DeviceBuffer<float4> pair_force_, pair_torque_;
+  DeviceBuffer<uint8_t> pair_temp_;
+  uint32_t body_pair_cap_ = 0;
+  int32_t ensure_pair_room(uint64_t cap);
+  PairRoom pair_room() const;
   DeviceBuffer<BodyExtent> extent_;
   DeviceBuffer<WetQuery> wet_;
   DeviceBuffer<LiftState> lift_;
   DeviceBuffer<float4> probe_arm_, probe_vel_;
-  DeviceBuffer<uint32_t> lift_near_, lift_sort_;
+  DeviceBuffer<uint32_t> lift_near_, lift_sort_, lift_counts_, lift_offsets_, lift_tiles_;
+  DeviceBuffer<float4> lift_boxes_;
   DeviceBuffer<unsigned long long> lift_columns_;
 
   // measurement

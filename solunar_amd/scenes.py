@@ -11,6 +11,11 @@ low corner, lattice sites nudged by up to +-0.05 spacing per axis):
   adjoint_64k WCSPH, spacing 0.01, 0.4 x 0.4 x 0.4 block: 64,000 particles with
               seeded N(0, 0.1 m/s) initial velocities, 200 steps of 1e-4 s,
               loss = mean |x_T - x_T*|^2 against the viscosity-0.08 rollout
+  bodies_1m   dam_1m under WCSPH with 200 coupled spheres (radii uniform in
+              0.02 to 0.08 m): 100 resting on the water block's top, half a
+              spacing clear of it, settling in at 0.1 m/s, and 100 on the dry
+              floor downstream; drifting up to 0.1 m/s and turning about y at
+              up to 1 rad/s; every other one dynamic
   batch_256   256 WCSPH dam breaks at spacing 0.02, block sides uniform in
               [0.2, 0.8] m per axis, redrawn until the block holds 2,000 to
               64,000 particles
@@ -118,6 +123,47 @@ def obstacle_field(count: int, spacing: float, seed: int = 7, region=((0.5, 0.05
         c = rng.uniform(lo + r, hi - r)
         parts.append(sphere_points(c, r, spacing, 2.0 * spacing))
     return np.concatenate(parts) if parts else np.zeros((0, 3), np.float32)
+
+
+def settings_bodies_1m() -> FluidSettings:
+    """dam_1m's water under WCSPH: the reference couples bodies with the
+    weakly compressible solver only (src/fluid/couple.rs:223-225)."""
+    return replace(settings_dam_1m(), solver=WCSPH)
+
+
+def body_field(count: int = 200, seed: int = 11, spacing: float = 0.005, block=BLOCK, box=BOX,
+               radii=(0.02, 0.08)):
+    """[(handle, radius, centre, velocity, spin)] of `count` spheres: the
+    first half on the block's top surface (its particles' centres reach
+    block_y - spacing / 2), the rest on the floor past the block."""
+    rng = np.random.default_rng(seed)
+    out = []
+    top = block[1] - 0.5 * spacing
+    for k in range(count):
+        r = float(rng.uniform(*radii))
+        if k < count // 2:
+            c = np.array([rng.uniform(0.0, block[0]), top + 0.5 * spacing + r + 0.5 * spacing,
+                          rng.uniform(0.0, block[2])])
+            v = np.array([rng.uniform(-0.1, 0.1), -0.1, rng.uniform(-0.1, 0.1)])
+        else:
+            c = np.array([rng.uniform(block[0] + 0.1 + r, box[0] - r), r + spacing, rng.uniform(r, box[2] - r)])
+            v = np.array([rng.uniform(-0.1, 0.1), 0.0, rng.uniform(-0.1, 0.1)])
+        spin = float(rng.uniform(-1.0, 1.0))
+        out.append(((k, 0), r, c, v, spin))
+    return out
+
+
+def body_poses(field, t: float):
+    """The field's poses at time t: centres moved on, turned about y."""
+    from .types import BodyPose
+
+    poses = []
+    for handle, _, c, v, spin in field:
+        half = 0.5 * spin * t
+        poses.append(BodyPose(handle, tuple(float(x) for x in c + v * t), (0.0, float(np.sin(half)), 0.0,
+                                                                              float(np.cos(half))),
+                              tuple(float(x) for x in v), (0.0, spin, 0.0), dynamic=handle[0] % 2 == 0))
+    return poses
 
 
 def dam_with_obstacles(world, obstacles: int = 200, seed: int = 7) -> int:

@@ -129,3 +129,28 @@ def test_coupled_bodies_cross_between_device_and_oracle_snapshots():
     for k in range(1, 3):
         assert back.step_with_bodies(TICK, poses(k)) == cpu.step_with_bodies(TICK, poses(k))
     assert back.state_hash() == cpu.state_hash()
+
+
+def test_a_step_that_outgrows_the_cell_table_carries_the_bodies_as_the_oracle_does():
+    # no container, so the device's cell table is sized by the particle count
+    # (max(4096, 2n) cells and a quarter more): water thrown apart at 80 m/s
+    # spreads over many times that within the tick, the step's launches
+    # abort and run again from the last finished substep, and the body
+    # points must stand where the oracle's are after every substep
+    print("case: exploding block beside a sphere", flush=True)
+    s = FluidSettings()
+    rng = np.random.default_rng(7)
+    x = np.stack(np.meshgrid(*[np.arange(20) * 0.05 + 0.025] * 3, indexing="ij"), -1).reshape(-1, 3)
+    v = rng.uniform(-80.0, 80.0, x.shape)
+    gpu, cpu = FluidWorld(s), OracleWorld(s, threads=8)
+    for w in (gpu, cpu):
+        w.add_particles(x, v)
+        w.couple((2, 0), Shape.sphere(0.2))
+    poses = [BodyPose((2, 0), (0.5, 0.5, 1.25), linear_velocity=(0.0, 0.0, -3.0))]
+    r, _ = same(gpu, cpu, poses, "explode")
+    assert r.substeps > 40
+    p = gpu.positions()
+    cells = np.floor(p / np.float32(s.radius())).astype(np.int64)
+    volume = np.prod(cells.max(axis=0) - cells.min(axis=0) + 1)
+    assert volume > 2 * (2 * len(x) * 1.25 + 4096)
+    same(gpu, cpu, poses, "after")
