@@ -466,6 +466,127 @@ inline int32_t shape_points(const Shape& s, float spacing, float reach, size_t l
   return kOk;
 }
 
+// ---- terrain ground (src/fluid/terrain.rs) ----
+
+// Most lattice nodes one terrain region may sample (terrain.rs:9), and the
+// samples of the ground's damping table (terrain.rs:11).
+constexpr size_t kMaxTerrainNodes = size_t(1) << 24;
+constexpr int kMapSamples = 256;
+
+// TerrainField (terrain.rs:35-42): the signed distance at nodes `step`
+// apart from `low`, node (x, y, z) at (y * nz + z) * nx + x.
+struct TerrainField {
+  uint32_t owner;
+  Vec3<float> low;
+  size_t nodes[3];
+  float step;
+  std::vector<float> distances;
+
+  // read (terrain.rs:101-136): trilinear value and its change per axis
+  template <class R>
+  bool read(Vec3<R> point, R& value, Vec3<R>& change) const {
+    R st = R(step);
+    Vec3<R> local = Vec3<R>{point.x - R(low.x), point.y - R(low.y), point.z - R(low.z)} / st;
+    auto inside = [](R v, size_t n) { return v >= R(0) && v <= R(n - 1); };
+    if (!(inside(local.x, nodes[0]) && inside(local.y, nodes[1]) && inside(local.z, nodes[2]))) return false;
+    auto base = [](R v, size_t n) { return std::min<size_t>((size_t)std::floor(v), n - 2); };
+    size_t x = base(local.x, nodes[0]), y = base(local.y, nodes[1]), z = base(local.z, nodes[2]);
+    R fx = local.x - R(x), fy = local.y - R(y), fz = local.z - R(z);
+    auto corner = [&](size_t dx, size_t dy, size_t dz) {
+      return R(distances[((y + dy) * nodes[2] + (z + dz)) * nodes[0] + (x + dx)]);
+    };
+    auto lerp = [](R a, R b, R t) { return a + (b - a) * t; };
+    auto face = [&](size_t dy, size_t dz, R& v, R& g) {
+      R a = corner(0, dy, dz), b = corner(1, dy, dz);
+      v = lerp(a, b, fx);
+      g = b - a;
+    };
+    R v00, g00, v10, g10, v01, g01, v11, g11;
+    face(0, 0, v00, g00);
+    face(1, 0, v10, g10);
+    face(0, 1, v01, g01);
+    face(1, 1, v11, g11);
+    R v0 = lerp(v00, v10, fy), v1 = lerp(v01, v11, fy);
+    value = lerp(v0, v1, fz);
+    change = Vec3<R>{lerp(lerp(g00, g10, fy), lerp(g01, g11, fy), fz), lerp(v10 - v00, v11 - v01, fz), v1 - v0} / st;
+    return true;
+  }
+
+  // sample (terrain.rs:138-142): distance and the way out of the ground
+  template <class R>
+  bool sample(Vec3<R> point, R& distance, Vec3<R>& normal) const {
+    Vec3<R> change;
+    if (!read(point, distance, change)) return false;
+    normal = normalize_or(change, Vec3<R>{R(0), R(1), R(0)});
+    return true;
+  }
+};
+
+// GroundSites (terrain.rs:173-247): the water's lattice around a particle
+// within the kernel with the density each site adds, and the damping weight
+// into flat ground by distance.
+template <class R>
+struct GroundSites {
+  R spacing, reach, low, step;
+  struct Site {
+    Vec3<R> offset;
+    R weight, far;
+  };
+  std::vector<Site> offsets;
+  std::vector<R> damping;
+
+  template <class K>
+  GroundSites(const K& kernel, R mass, R spacing_, R radius) : spacing(spacing_) {
+    int r = (int)std::ceil(radius / spacing);
+    for (int z = -r; z <= r; ++z)
+      for (int y = -r; y <= r; ++y)
+        for (int x = -r; x <= r; ++x) {
+          Vec3<R> offset = Vec3<R>{R(x), R(y), R(z)} * spacing;
+          R far = length(offset);
+          if (far < radius) offsets.push_back({offset, mass * kernel.value(far), far});
+        }
+    low = -spacing / R(2);
+    step = (radius - low) / R(kMapSamples - 1);
+    R soften = R(0.01f) * radius * radius;
+    int across = r + 1;
+    for (int sample = 0; sample < kMapSamples; ++sample) {
+      R distance = low + step * R(sample);
+      R weight = R(0);
+      for (int layer = 0;; ++layer) {
+        R depth = distance + spacing * (R(layer) + R(0.5f));
+        if (depth >= radius) break;
+        for (int a = -across; a <= across; ++a)
+          for (int b = -across; b <= across; ++b) {
+            R flat = R(a * a + b * b) * spacing * spacing;
+            R squared = flat + depth * depth;
+            R far = std::sqrt(squared);
+            if (far >= radius || far <= R(0)) continue;
+            weight += mass * depth * depth * -kernel.slope(far) / (far * (squared + soften));
+          }
+      }
+      damping.push_back(weight);
+    }
+    reach = radius + spacing / R(2);
+  }
+
+  // damping (terrain.rs:236-245)
+  R damping_at(R distance) const {
+    R at = (rmax(distance, low) - low) / step;
+    R f = std::floor(at);
+    size_t below = std::isnan(f) ? 0 : (f >= R(1e18) ? SIZE_MAX : (size_t)f);
+    if (below >= size_t(kMapSamples) - 1) return R(0);
+    R part = at - R(below);
+    return damping[below] + (damping[below + 1] - damping[below]) * part;
+  }
+};
+
+// solid (terrain.rs:249-252)
+template <class R>
+inline R solid(R distance, R size) {
+  R v = R(0.5f) - distance / size;
+  return clamp_rust(v, R(0), R(1));
+}
+
 // FluidReport (src/fluid/step.rs:28-49), iterations split by solve.
 struct Report {
   uint32_t substeps = 0;
@@ -569,6 +690,7 @@ template <class R>
 class World {
  public:
   using V = Vec3<R>;
+  using Real = R;
   static V cvt(Vec3<float> v) { return {R(v.x), R(v.y), R(v.z)}; }
 
   Settings settings;
@@ -611,6 +733,10 @@ class World {
     Grid grid;
   } movers;
   Lists mover_neighbors;
+  // terrain ground (src/fluid/mod.rs:320-335)
+  std::vector<TerrainField> terrains;
+  GroundSites<R> ground_sites;
+  std::vector<V> ground_slopes;
 
   Dispatch dispatch;
   // called before (false) and after (true) every WCSPH substep with its
@@ -621,12 +747,11 @@ class World {
   bool log_solves = false;
   std::string detail;
 
-  explicit World(const Settings& s) : settings(s), kernel(R(s.spacing) * R(s.support)) {
-    // radius() is computed in f32 (mod.rs:120); the f64 world takes the f32
-    // product exactly
-    kernel = Kernel<R>(R(s.radius()));
-    mass = rest_mass();
-  }
+  explicit World(const Settings& s)
+      : settings(s),
+        kernel(R(s.radius())),
+        mass(rest_mass()),
+        ground_sites(kernel, mass, R(s.spacing), R(s.radius())) {}
 
   R radius() const { return R(settings.radius()); }
   size_t len() const { return ids.size(); }
@@ -972,6 +1097,7 @@ class World {
   int32_t compute_densities() {
     if (!neighbors_built || neighbors.start.size() != len() + 1) return kOutOfRange;
     std::vector<R> out(len());
+    std::vector<V> slopes(len());
     bool ok = dispatch.run(len(), [&](size_t b, size_t e) {
       for (size_t i = b; i < e; ++i) {
         V at = positions[i];
@@ -984,11 +1110,92 @@ class World {
         R msum = R(0);
         for (uint32_t k = mover_neighbors.start[i]; k < mover_neighbors.start[i + 1]; ++k)
           msum += kernel.value(length(movers.positions[mover_neighbors.items[k]] - at));
-        out[i] = mass * (sum + wsum + msum);
+        R ground = R(0);
+        V slope{R(0), R(0), R(0)};
+        for (const TerrainField& field : terrains) {
+          R gd;
+          V gs;
+          if (ground_at(field, at, gd, gs)) {
+            ground = ground + gd;
+            slope = slope + gs;
+          }
+        }
+        out[i] = mass * (sum + wsum + msum) + ground;
+        slopes[i] = slope;
       }
     });
     if (!ok) return kDispatchIncomplete;
     densities.swap(out);
+    if (terrains.empty())
+      ground_slopes.clear();
+    else
+      ground_slopes.swap(slopes);
+    return kOk;
+  }
+
+  // TerrainField::ground (terrain.rs:144-170): the density the ground's
+  // sites add at `point` and how it changes; false outside the region
+  bool ground_at(const TerrainField& field, V point, R& density, V& gradient) const {
+    R distance;
+    V change;
+    if (!field.read(point, distance, change)) return false;
+    density = R(0);
+    gradient = V{R(0), R(0), R(0)};
+    const GroundSites<R>& g = ground_sites;
+    if (distance < g.reach) {
+      R open = distance - g.spacing / R(2);
+      for (const auto& site : g.offsets) {
+        if (site.far <= open) continue;
+        R there;
+        V toward;
+        if (!field.read(point + site.offset, there, toward)) continue;
+        R share = solid(there, g.spacing);
+        density += site.weight * share;
+        if (share > R(0) && share < R(1)) gradient = gradient - toward * (site.weight / g.spacing);
+      }
+    }
+    return true;
+  }
+
+  // The lattice add_terrain samples over a box (terrain.rs:280-286):
+  // ceil(extent / step) + 1 nodes per axis, step = spacing / 2.
+  bool terrain_nodes(Vec3<float> low, Vec3<float> high, size_t nodes[3]) const {
+    float step = settings.spacing / 2.0f;
+    Vec3<float> size = high - low;
+    float e[3] = {size.x, size.y, size.z};
+    for (int a = 0; a < 3; ++a) {
+      float c = std::fmin(std::ceil(e[a] / step), 1e7f);  // `as usize`: NaN and below 0 give 0
+      nodes[a] = (std::isnan(c) || c <= 0.0f ? 0 : (size_t)c) + 1;
+    }
+    return true;
+  }
+
+  // add_terrain (terrain.rs:262-305) over distances the caller sampled at
+  // low + (x, y, z) step, loops y, z, x (its signed-distance volume is not
+  // part of this path)
+  int32_t add_terrain(Vec3<float> low, Vec3<float> high, const std::vector<float>& distances, uint32_t* id_out) {
+    if (!low.finite() || !high.finite()) {
+      detail = "terrain box";
+      return kNonFinite;
+    }
+    if (high.x <= low.x || high.y <= low.y || high.z <= low.z) return kOutOfRange;
+    TerrainField f{};
+    terrain_nodes(low, high, f.nodes);
+    uint64_t total = 1;
+    for (size_t n : f.nodes) total = total > UINT64_MAX / n ? UINT64_MAX : total * n;
+    if (total > kMaxTerrainNodes) return kFluidFull;
+    if (distances.size() != total) return kOutOfRange;
+    for (float d : distances)
+      if (!std::isfinite(d)) {
+        detail = "terrain distance";
+        return kNonFinite;
+      }
+    f.low = low;
+    f.step = settings.spacing / 2.0f;
+    f.distances = distances;
+    f.owner = add_wall_points({});
+    terrains.push_back(std::move(f));
+    if (id_out) *id_out = terrains.back().owner;
     return kOk;
   }
 
@@ -1471,6 +1678,16 @@ class World {
     V normal = normalize_or(away, V{R(0), R(0), R(0)});
     V into = normal * rmin(dot(velocity, normal), R(0));
     R wall_viscosity = rmin(R(1) * R(settings.sound_speed) * r, R(0.25f) * stable);
+    for (const TerrainField& field : terrains) {
+      R distance;
+      V normal;
+      if (!field.sample(at, distance, normal)) continue;
+      R into_ground = dot(velocity, normal);
+      if (into_ground < R(0)) {
+        R weight = ground_sites.damping_at(distance);
+        push = push - normal * (wall_viscosity * into_ground * weight / density);
+      }
+    }
     for (uint32_t k = wall_neighbors.start[index]; k < wall_neighbors.start[index + 1]; ++k) {
       V offset = at - walls.positions[wall_neighbors.items[k]];
       R toward = dot(into, offset);
@@ -1558,6 +1775,7 @@ class World {
           V offset = at - walls.positions[wall_neighbors.items[k]];
           push = push - kernel.gradient(offset) * (mass * mirrored);
         }
+        if (i < ground_slopes.size()) push = push - ground_slopes[i] * mirrored;
         mover_push(i, mirrored, damping, push, mine);
         pushes[i] = push + wall_viscous(i, dt) + gravity;
       }
@@ -1574,7 +1792,7 @@ class World {
     return advance(dt, held);
   }
 
-  // advance (src/fluid/step.rs:305-339), no terrains.
+  // advance (src/fluid/step.rs:305-339)
   int32_t advance(R dt, uint32_t& held) {
     held = 0;
     R half = R(settings.spacing / 2.0f);
@@ -1594,6 +1812,7 @@ class World {
           next = inside;
         }
       }
+      if (terrain_contact(half, next, velocity)) held += 1;
       if (!next.finite() || !velocity.finite()) {
         detail = "fluid particle";
         return kNonFinite;
@@ -1603,8 +1822,29 @@ class World {
     return kOk;
   }
 
+  // terrain_contact (terrain.rs:315-343): water closer than `rest` to a
+  // terrain's surface is put back there and stops moving into it; true when
+  // it got under the surface
+  bool terrain_contact(R rest, V& at, V& velocity) const {
+    bool under = false;
+    for (const TerrainField& field : terrains) {
+      R distance;
+      V normal;
+      if (!field.sample(at, distance, normal)) continue;
+      if (distance < rest) {
+        under = under || distance < R(0);
+        at = at + normal * (rest - distance);
+        R into = dot(velocity, normal);
+        if (into < R(0)) velocity = velocity - normal * into;
+      }
+    }
+    return under;
+  }
+
   // dfsph (src/fluid/dfsph.rs:46-77)
   int32_t dfsph(float dt_in, Report& report) {
+    // ground pushes through the weakly compressible solver's pressure only
+    if (!terrains.empty()) return kInvalidSettings;
     R spacing = R(settings.spacing);
     R courant = R(settings.courant);
     R pull = length(cvt(settings.gravity));
@@ -1878,7 +2118,7 @@ class World {
     return id;
   }
 
-  // remove_boundary (boundary.rs:94-115), no terrains.
+  // remove_boundary (boundary.rs:94-115)
   bool remove_boundary(uint32_t id) {
     size_t before = walls.positions.size();
     std::vector<V> kp;
@@ -1900,7 +2140,11 @@ class World {
     }
     walls.containers.swap(kc);
     walls.bounds = union_of(walls.containers);
-    bool gone = walls.positions.size() != before || had;
+    size_t fields = terrains.size();
+    terrains.erase(std::remove_if(terrains.begin(), terrains.end(),
+                                  [&](const TerrainField& f) { return f.owner == id; }),
+                   terrains.end());
+    bool gone = walls.positions.size() != before || had || terrains.size() != fields;
     if (gone) settle_walls();
     return gone;
   }

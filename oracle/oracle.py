@@ -21,7 +21,7 @@ from typing import Optional
 import numpy as np
 
 from solunar_amd.types import (CBodyPose, CBodyPush, CReport, CSettings, CShape, FluidReport, FluidSettings,
-                               PhysicsError, Shape, SnapshotError, pose_array)
+                               PhysicsError, Shape, SnapshotError, pose_array, terrain_samples)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_fluid.so")
@@ -94,6 +94,13 @@ def library() -> ctypes.CDLL:
         "orc_tile_tree_sum": (ctypes.c_double, [fp, ctypes.POINTER(i32), sz]),
         "orc_grid_selftest": (i32, []),
         "orc_plant_mistake": (None, [i32]),
+        "orc_terrain_nodes": (i32, [P, fp, fp, ctypes.POINTER(sz)]),
+        "orc_add_terrain": (i32, [P, fp, fp, fp, sz, u32p]),
+        "orc_terrain_len": (sz, [P]),
+        "orc_terrain_sample": (i32, [P, sz, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]),
+        "orc_ground_near": (i32, [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)]),
         "orc_couple": (i32, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(CShape)]),
         "orc_uncouple": (i32, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(i32)]),
         "orc_coupled_len": (sz, [P]),
@@ -299,6 +306,47 @@ class OracleWorld:
         w.set_threads(threads)
         w.set_residual_mode(residual_mode)
         return w
+
+    # ---- terrain ground (src/fluid/terrain.rs) ----
+
+    def terrain_nodes(self, low, high):
+        """The lattice add_terrain samples over the box: nodes per axis (x, y, z)."""
+        lo, hi = _v3(low), _v3(high)
+        n = (ctypes.c_size_t * 3)()
+        self._check(self._lib.orc_terrain_nodes(self._h, _p(lo, ctypes.c_float), _p(hi, ctypes.c_float), n))
+        return tuple(n)
+
+    def add_terrain(self, low, high, distance) -> int:
+        """add_terrain (terrain.rs:262-305): `distance` is the terrain's signed
+        distance, a function of (N, 3) float32 points, or the sampled array."""
+        lo, hi = _v3(low), _v3(high)
+        d = terrain_samples(self, lo, hi, distance)
+        o = ctypes.c_uint32()
+        self._check(self._lib.orc_add_terrain(self._h, _p(lo, ctypes.c_float), _p(hi, ctypes.c_float),
+                                              _p(d, ctypes.c_float), d.size, ctypes.byref(o)))
+        return o.value
+
+    def terrain_len(self) -> int:
+        return int(self._lib.orc_terrain_len(self._h))
+
+    def terrain_sample(self, k: int, point):
+        """(distance, normal) of field k at `point`, or None outside it."""
+        p = np.asarray(point, np.float64)
+        d = ctypes.c_double()
+        n = (ctypes.c_double * 3)()
+        st = self._lib.orc_terrain_sample(self._h, k, _p(p, ctypes.c_double), ctypes.byref(d), n)
+        if st == 10:
+            return None
+        self._check(st)
+        return d.value, np.array(n[:])
+
+    def ground_near(self, point):
+        """(density, gradient) the ground adds at `point` (terrain.rs:144-170, 307-313)."""
+        p = np.asarray(point, np.float64)
+        d = ctypes.c_double()
+        g = (ctypes.c_double * 3)()
+        self._check(self._lib.orc_ground_near(self._h, _p(p, ctypes.c_double), ctypes.byref(d), g))
+        return d.value, np.array(g[:])
 
     # ---- coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) ----
 

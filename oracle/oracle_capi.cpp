@@ -275,7 +275,7 @@ int32_t orc_save_snapshot(void* w, uint8_t* out, size_t cap, size_t* len) {
 }
 
 // load_snapshot (src/fluid/save.rs:68-195): *kind 0 and a world, or the
-// refusal (1..6 as SnapshotError, 101 for terrain)
+// refusal (1..6 as SnapshotError)
 int32_t orc_load_snapshot(const uint8_t* data, size_t len, const orc_settings* base, int32_t precision, void** out,
                           int32_t* kind, uint32_t* version, const char** what) {
   *out = nullptr;
@@ -417,6 +417,61 @@ int32_t orc_step_with_bodies(void* w, float dt, const Pose* poses, size_t n, orc
     fill_report(r, report);
     *count = pushes.size();
     for (size_t b = 0; b < std::min(cap, pushes.size()); ++b) out[b] = pushes[b];
+    return (int32_t)kOk;
+  });
+}
+
+// ---- terrain ground (src/fluid/terrain.rs) ----
+
+int32_t orc_terrain_nodes(void* w, const float* low, const float* high, size_t* nodes) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    x.terrain_nodes({low[0], low[1], low[2]}, {high[0], high[1], high[2]}, nodes);
+    return (int32_t)kOk;
+  });
+}
+
+int32_t orc_add_terrain(void* w, const float* low, const float* high, const float* distances, size_t count,
+                        uint32_t* id) {
+  std::vector<float> d(distances, distances + count);
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    return x.add_terrain({low[0], low[1], low[2]}, {high[0], high[1], high[2]}, d, id);
+  });
+}
+
+size_t orc_terrain_len(void* w) {
+  return with(static_cast<Handle*>(w), [&](auto& x) { return x.terrains.size(); });
+}
+
+// field k's distance and way out at `point` (TerrainField::sample); kOutOfRange outside
+int32_t orc_terrain_sample(void* w, size_t k, const double* point, double* distance, double* normal) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = typename std::decay_t<decltype(x)>::Real;
+    if (k >= x.terrains.size()) return (int32_t)kOutOfRange;
+    R d;
+    Vec3<R> n;
+    if (!x.terrains[k].sample(Vec3<R>{R(point[0]), R(point[1]), R(point[2])}, d, n)) return (int32_t)kOutOfRange;
+    *distance = d;
+    normal[0] = n.x, normal[1] = n.y, normal[2] = n.z;
+    return (int32_t)kOk;
+  });
+}
+
+// ground_near (terrain.rs:307-313) summed over the fields as compute_densities sums it
+int32_t orc_ground_near(void* w, const double* point, double* density, double* gradient) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = typename std::decay_t<decltype(x)>::Real;
+    R total = R(0);
+    Vec3<R> slope{R(0), R(0), R(0)};
+    for (const auto& f : x.terrains) {
+      R d;
+      Vec3<R> g;
+      if (x.ground_at(f, Vec3<R>{R(point[0]), R(point[1]), R(point[2])}, d, g)) {
+        total = total + d;
+        slope = slope + g;
+      }
+    }
+    *density = total;
+    gradient[0] = slope.x, gradient[1] = slope.y, gradient[2] = slope.z;
     return (int32_t)kOk;
   });
 }

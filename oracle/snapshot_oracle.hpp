@@ -23,7 +23,7 @@ enum SnapshotKind : int32_t {
   kSnapUnsupportedVersion = 4,
   kSnapBadChecksum = 5,
   kSnapInvalid = 6,
-  kSnapUnsupported = 101,  // terrain: the oracle does not carry it
+  kSnapUnsupported = 101,
 };
 
 struct SnapshotResult {
@@ -108,7 +108,7 @@ struct SnapReader {
   }
 };
 
-// What a snapshot holds (no terrain).
+// What a snapshot holds.
 struct SnapshotState {
   Settings settings;
   std::vector<uint32_t> ids;
@@ -134,6 +134,7 @@ struct SnapshotState {
     float probe_spacing;
   };
   std::vector<CoupledRec> coupled;
+  std::vector<TerrainField> terrains;
 };
 
 inline bool snap_box_ok(Vec3<float> lo, Vec3<float> hi) {
@@ -196,7 +197,16 @@ std::vector<uint8_t> save_snapshot(const World<R>& w) {
     out.u32((uint32_t)k.probes.size());
     for (const auto& p : k.probes) out.vec3(p);
   }
-  out.u32(0);  // terrains
+  // TerrainField::save (terrain.rs:45-56)
+  out.u32((uint32_t)w.terrains.size());
+  for (const auto& t : w.terrains) {
+    out.u32(t.owner);
+    out.vec3(t.low);
+    for (size_t n : t.nodes) out.u32((uint32_t)n);
+    out.f32(t.step);
+    out.u32((uint32_t)t.distances.size());
+    for (float d : t.distances) out.f32(d);
+  }
   if (w.has_domain) {
     out.u8(1);
     out.vec3(w.domain_low);
@@ -341,9 +351,29 @@ inline SnapshotResult parse_snapshot(const uint8_t* data, size_t len, const Sett
     if (std::tie(st.coupled[b - 1].index, st.coupled[b - 1].generation) >=
         std::tie(st.coupled[b].index, st.coupled[b].generation))
       return {kSnapInvalid, 0, "fluid body order"};
+  // TerrainField::load (terrain.rs:58-95), save.rs:172-179
   size_t terrains = 0;
   if (!r.count(32, uint64_t(1) << 16, terrains)) return res;
-  if (terrains) return {kSnapUnsupported, 0, "fluid terrain"};
+  for (size_t t = 0; t < terrains; ++t) {
+    TerrainField f{};
+    uint32_t n[3];
+    if (!r.u32(f.owner) || !r.vec3(f.low) || !r.u32(n[0]) || !r.u32(n[1]) || !r.u32(n[2]) || !r.f32(f.step))
+      return res;
+    size_t count = 0;
+    if (!r.count(4, kMaxTerrainNodes, count)) return res;
+    unsigned __int128 total = (unsigned __int128)n[0] * n[1] * n[2];
+    if (!f.low.finite() || !std::isfinite(f.step) || f.step <= 0.0f || n[0] < 2 || n[1] < 2 || n[2] < 2 ||
+        total != count)
+      return {kSnapInvalid, 0, "fluid terrain"};
+    for (int a = 0; a < 3; ++a) f.nodes[a] = n[a];
+    f.distances.resize(count);
+    for (size_t k = 0; k < count; ++k) {
+      if (!r.f32(f.distances[k])) return res;
+      if (!std::isfinite(f.distances[k])) return {kSnapInvalid, 0, "fluid terrain distance"};
+    }
+    if (f.owner >= st.wall_next) return {kSnapInvalid, 0, "fluid terrain owner"};
+    st.terrains.push_back(std::move(f));
+  }
   if (version >= 2) {
     uint8_t flag = 0;
     if (!r.u8(flag)) return res;
@@ -392,6 +422,7 @@ void apply_snapshot(World<R>& w, const SnapshotState& st) {
     typename World<R>::Coupled c{k.index, k.generation, k.points, k.probes, k.probe_spacing};
     w.coupled.push_back(std::move(c));
   }
+  w.terrains = st.terrains;
   w.neighbors_built = false;
   w.settle_walls();
 }

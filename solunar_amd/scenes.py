@@ -257,3 +257,28 @@ def adjoint_velocities(n: int, seed: int = ADJOINT_SEED) -> np.ndarray:
 
 def container(world) -> None:
     world.add_box_container((0.0, 0.0, 0.0), BOX)
+
+
+# ---- terrain ground (src/fluid/terrain.rs tests) ----
+
+# The region the reference's trench tests sample (terrain.rs:356-357).
+TRENCH_LOW = (-0.4, -0.7, -0.5)
+TRENCH_HIGH = (2.2, 0.3, 0.5)
+
+
+def trench_distance(points) -> np.ndarray:
+    """Signed distance of flat ground at y = 0 with a trench along x from
+    -0.1 to 1.9, its bed at y = -0.4, 0.6 m wide (the reference's test
+    terrain, terrain.rs:360-381, as an exact field rather than its 10 cm
+    voxels): max(plane, -box), positive above the ground."""
+    p = np.asarray(points, np.float64)
+    plane = p[:, 1]
+    c, h = np.array([0.9, -0.2, 0.0]), np.array([1.0, 0.2, 0.3])
+    q = np.abs(p - c) - h
+    box = np.linalg.norm(np.maximum(q, 0.0), axis=1) + np.minimum(q.max(axis=1), 0.0)
+    return np.maximum(plane, -box).astype(np.float32)
+
+
+def in_trench(world) -> int:
+    """add_terrain over the trench region; returns the terrain's boundary id."""
+    return world.add_terrain(TRENCH_LOW, TRENCH_HIGH, trench_distance)

@@ -348,3 +348,25 @@ class CBodyPush(ctypes.Structure):
     def to_push(self) -> BodyPush:
         return BodyPush((self.index, self.generation), bool(self.lifted), tuple(self.lift_impulse),
                         tuple(self.lift_angular), tuple(self.impulse), tuple(self.angular))
+
+
+# ---- terrain ground (src/fluid/terrain.rs) ----
+
+def terrain_samples(world, low, high, distance):
+    """The distances add_terrain takes over the box: `distance` (a function
+    of (N, 3) float32 points, or the array itself) at low + (x, y, z) step,
+    step = spacing / 2 in f32, loops y, z, x (terrain.rs:287-295)."""
+    import numpy as np
+
+    nx, ny, nz = world.terrain_nodes(low, high)
+    if callable(distance) and (nx * ny * nz > 1 << 24 or not np.isfinite(np.asarray(low, np.float64)).all()):
+        return np.zeros(0, np.float32)  # the library refuses the box before it reads any distance
+    if not callable(distance):
+        return np.ascontiguousarray(np.asarray(distance, np.float32).reshape(-1))
+    f = np.float32
+    step = f(world.settings.spacing) / f(2.0)
+    lo = np.asarray(low, np.float32)
+    y, z, x = np.meshgrid(np.arange(ny, dtype=np.float32), np.arange(nz, dtype=np.float32),
+                          np.arange(nx, dtype=np.float32), indexing="ij")
+    at = np.stack([lo[0] + x.ravel() * step, lo[1] + y.ravel() * step, lo[2] + z.ravel() * step], axis=1)
+    return np.ascontiguousarray(np.asarray(distance(at.astype(np.float32)), np.float32).reshape(-1))
