@@ -179,24 +179,47 @@ typedef struct sfl_snapshot_error {
 
 /* FluidWorld::save_snapshot (src/fluid/save.rs:17-66): the SLFL version 3
  * bytes of the world (settings, particles in order, next id, walls, flows,
- * coupled bodies, domain, FNV-1a checksum), the same bytes the reference
- * writes. With `out` null, only *len_out is set; with `cap` too small,
- * *len_out is set and SFL_OUT_OF_RANGE returned. Worlds here hold no
- * terrain, so that count is 0. */
+ * coupled bodies, terrain regions, domain, FNV-1a checksum), the same bytes
+ * the reference writes. With `out` null, only *len_out is set; with `cap`
+ * too small, *len_out is set and SFL_OUT_OF_RANGE returned. */
 int32_t sfl_save_snapshot(sfl_world* world, uint8_t* out, size_t cap,
                           size_t* len_out);
 
 /* FluidWorld::load_snapshot (src/fluid/save.rs:68-195): a new world from
  * untrusted bytes, checked as the reference checks them (size, magic,
  * checksum, version 1..3, settings, counts against the bytes left, ids,
- * walls, flows, coupled bodies, domain, no trailing bytes). A refused
- * snapshot returns SFL_SNAPSHOT_REFUSED with *error_out filled; one that
- * holds terrain returns SFL_UNSUPPORTED. Settings the snapshot does not hold
+ * walls, flows, coupled bodies, terrain regions, domain, no trailing
+ * bytes). A refused snapshot returns SFL_SNAPSHOT_REFUSED with *error_out
+ * filled. Settings the snapshot does not hold
  * (the extensions, device included) come from `base` (defaults when null).
  * Densities start at the rest density, as the reference's do. */
 int32_t sfl_load_snapshot(const uint8_t* data, size_t len,
                           const sfl_settings* base, sfl_world** world_out,
                           sfl_snapshot_error* error_out);
+
+/* ---- terrain ground (src/fluid/terrain.rs) ---- */
+
+/* The lattice FluidWorld::add_terrain samples over the box low..high
+ * (terrain.rs:280-286): nodes_out[a] = ceil(extent_a / step) + 1 nodes per
+ * axis, step = spacing / 2, node (x, y, z) at low + (x, y, z) * step in f32.
+ * The caller evaluates its signed-distance field (the reference's
+ * crate::sdf volume, outside this library) there. */
+int32_t sfl_terrain_nodes(const sfl_world* world, sfl_vec3 low, sfl_vec3 high,
+                          uint64_t* nodes_out);
+/* FluidWorld::add_terrain (terrain.rs:262-305): water rests on and flows
+ * over the ground whose signed distance (positive above it) `distances`
+ * holds at the nodes of sfl_terrain_nodes, in the reference's order (loops
+ * y, z, x: index (y * nz + z) * nx + x); `count` is their number. The
+ * ground enters the WCSPH density, force, wall viscosity and advance; a
+ * DFSPH step of a world with terrain returns SFL_INVALID_SETTINGS, as the
+ * reference's does. SFL_NON_FINITE for a non-finite box or distance,
+ * SFL_OUT_OF_RANGE for an inside-out box or a count that is not the
+ * lattice's, SFL_FLUID_FULL past 2^24 nodes. *boundary_out is the id
+ * sfl_remove_boundary takes it away with. */
+int32_t sfl_add_terrain(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
+                        const float* distances, size_t count,
+                        uint32_t* boundary_out);
+size_t sfl_terrain_len(const sfl_world* world);
 
 /* ---- coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs) ---- */
 

@@ -61,6 +61,10 @@ int32_t Tape::record(World* w, float dt, uint32_t steps, uint32_t every, Tape** 
     why = "the adjoint differentiates steps that keep the particles: no flows or domain";
     return SFL_UNSUPPORTED;
   }
+  if (!w->terrains_.empty()) {
+    why = "the adjoint's reverse pass has no ground terms";
+    return SFL_UNSUPPORTED;
+  }
   std::unique_ptr<Tape> t(new Tape());
   t->w_ = w;
   t->dt_ = dt;

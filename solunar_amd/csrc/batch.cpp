@@ -78,8 +78,8 @@ int32_t Batch::create(World* const* worlds, size_t count, Batch** out, std::stri
       why = "the worlds of a batch need equal settings";
       return SFL_INVALID_SETTINGS;
     }
-    if (!w.flows_.empty() || w.has_domain_) {
-      why = "a batch steps worlds without flows or a domain";
+    if (!w.flows_.empty() || w.has_domain_ || !w.terrains_.empty()) {
+      why = "a batch steps worlds without flows, a domain or terrain";
       return SFL_UNSUPPORTED;
     }
     total += w.n_;

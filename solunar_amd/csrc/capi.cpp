@@ -175,6 +175,20 @@ int32_t sfl_remove_flow(sfl_world* world, uint32_t flow, int32_t* removed_out) {
   return guarded(world, [&](sfl::World& w) { return w.remove_flow(flow, removed_out); });
 }
 
+int32_t sfl_terrain_nodes(const sfl_world* world, sfl_vec3 low, sfl_vec3 high, uint64_t* nodes_out) {
+  if (!world || !world->w) return SFL_INVALID_HANDLE;
+  if (!nodes_out) return SFL_OUT_OF_RANGE;
+  world->w->terrain_nodes(low, high, nodes_out);
+  return SFL_OK;
+}
+
+int32_t sfl_add_terrain(sfl_world* world, sfl_vec3 low, sfl_vec3 high, const float* distances, size_t count,
+                        uint32_t* boundary_out) {
+  return guarded(world, [&](sfl::World& w) { return w.add_terrain(low, high, distances, count, boundary_out); });
+}
+
+size_t sfl_terrain_len(const sfl_world* world) { return world && world->w ? world->w->terrain_len() : 0; }
+
 int32_t sfl_couple(sfl_world* world, uint32_t index, uint32_t generation, const sfl_shape* shape) {
   return guarded(world, [&](sfl::World& w) {
     if (!shape) return (int32_t)SFL_INVALID_SHAPE;

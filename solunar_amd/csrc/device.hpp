@@ -179,6 +179,35 @@ struct MoverView {
   PairRecords pairs;
 };
 
+// A terrain region (TerrainField, src/fluid/terrain.rs:35-42): signed
+// distances at nodes `step` apart from `low`, node (x, y, z) at
+// first + (y * nodes.z + z) * nodes.x + x of the distance array.
+struct TerrainDesc {
+  float low[3];
+  uint32_t nodes[3];
+  float step;
+  uint32_t first;
+};
+
+// The terrains of a world and its GroundSites (terrain.rs:173-247): the
+// lattice sites within the kernel (offset, density weight) and how far
+// each is, and the damping table into flat ground. `slopes` holds each
+// particle's ground slope from the density pass, read by the force pass.
+struct TerrainView {
+  const TerrainDesc* fields;
+  const float* distances;
+  uint32_t count;  // 0: no terrain
+  uint32_t nsites;
+  const float4* sites;  // x, y, z, weight
+  const float* far;
+  const float* damping;  // kMapSamples values
+  float low, step;       // of the damping table
+  float reach;           // radius + spacing / 2
+  float spacing;
+  float4* slopes;
+};
+constexpr int kMapSamples = 256;  // MAP_SAMPLES, terrain.rs:11
+
 // Scalars of one substep, computed on the host in the reference's f32
 // arithmetic.
 struct StepParams {
@@ -224,6 +253,7 @@ struct StepParams {
   float* factor;
   int32_t format;  // kMasks16, kMasks32, kLists or kCompact
   MoverView movers;
+  TerrainView terrain;
 };
 
 // Rows the neighbour list arrays carry past their capacity: the solve's
@@ -432,6 +462,134 @@ __device__ __forceinline__ void wave_sum_to(uint32_t* dst, uint32_t v) {
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
+// ---- terrain ground (src/fluid/terrain.rs) --------------------------------
+
+// TerrainField::read (terrain.rs:101-136): the trilinear distance at
+// (px, py, pz) and its change per axis; false outside the region.
+__device__ __forceinline__ bool terrain_read(const TerrainDesc& f, const float* __restrict__ dist, float px, float py,
+                                             float pz, float& value, float& cx, float& cy, float& cz) {
+  float lx = (px - f.low[0]) / f.step, ly = (py - f.low[1]) / f.step, lz = (pz - f.low[2]) / f.step;
+  if (!(lx >= 0.0f && lx <= (float)(f.nodes[0] - 1) && ly >= 0.0f && ly <= (float)(f.nodes[1] - 1) && lz >= 0.0f &&
+        lz <= (float)(f.nodes[2] - 1)))
+    return false;
+  uint32_t x = min((uint32_t)floorf(lx), f.nodes[0] - 2);
+  uint32_t y = min((uint32_t)floorf(ly), f.nodes[1] - 2);
+  uint32_t z = min((uint32_t)floorf(lz), f.nodes[2] - 2);
+  float fx = lx - (float)x, fy = ly - (float)y, fz = lz - (float)z;
+  const uint32_t nx = f.nodes[0], nz = f.nodes[2];
+  auto corner = [&](uint32_t dx, uint32_t dy, uint32_t dz) {
+    return dist[f.first + ((size_t)(y + dy) * nz + (z + dz)) * nx + (x + dx)];
+  };
+  auto lerp = [](float a, float b, float t) { return a + (b - a) * t; };
+  float a00 = corner(0, 0, 0), b00 = corner(1, 0, 0);
+  float a10 = corner(0, 1, 0), b10 = corner(1, 1, 0);
+  float a01 = corner(0, 0, 1), b01 = corner(1, 0, 1);
+  float a11 = corner(0, 1, 1), b11 = corner(1, 1, 1);
+  float v00 = lerp(a00, b00, fx), g00 = b00 - a00;
+  float v10 = lerp(a10, b10, fx), g10 = b10 - a10;
+  float v01 = lerp(a01, b01, fx), g01 = b01 - a01;
+  float v11 = lerp(a11, b11, fx), g11 = b11 - a11;
+  float v0 = lerp(v00, v10, fy), v1 = lerp(v01, v11, fy);
+  value = lerp(v0, v1, fz);
+  cx = lerp(lerp(g00, g10, fy), lerp(g01, g11, fy), fz) / f.step;
+  cy = lerp(v10 - v00, v11 - v01, fz) / f.step;
+  cz = (v1 - v0) / f.step;
+  return true;
+}
+
+// TerrainField::sample (terrain.rs:138-142): distance and the way out of the
+// ground, normalize_or(change, Y).
+__device__ __forceinline__ bool terrain_sample(const TerrainDesc& f, const float* __restrict__ dist, float px, float py,
+                                               float pz, float& distance, float& nx, float& ny, float& nz) {
+  float cx, cy, cz;
+  if (!terrain_read(f, dist, px, py, pz, distance, cx, cy, cz)) return false;
+  float rcp = 1.0f / sqrtf(dot3(cx, cy, cz, cx, cy, cz));
+  if (isfinite(rcp) && rcp > 0.0f) {
+    nx = cx * rcp;
+    ny = cy * rcp;
+    nz = cz * rcp;
+  } else {
+    nx = 0.0f;
+    ny = 1.0f;
+    nz = 0.0f;
+  }
+  return true;
+}
+
+// TerrainField::ground (terrain.rs:144-170) summed over the fields from +0
+// as compute_densities folds them (mod.rs:657-662): the density the ground's
+// sites add and its slope.
+__device__ __forceinline__ void ground_near(const TerrainView& t, float px, float py, float pz, float& density,
+                                            float& sx, float& sy, float& sz) {
+  density = 0.0f;
+  sx = sy = sz = 0.0f;
+  for (uint32_t k = 0; k < t.count; ++k) {
+    const TerrainDesc& f = t.fields[k];
+    float distance, cx, cy, cz;
+    if (!terrain_read(f, t.distances, px, py, pz, distance, cx, cy, cz)) continue;
+    float d = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    if (distance < t.reach) {
+      float open = distance - t.spacing / 2.0f;
+      for (uint32_t q = 0; q < t.nsites; ++q) {
+        if (t.far[q] <= open) continue;
+        float4 site = t.sites[q];
+        float there, tx, ty, tz;
+        if (!terrain_read(f, t.distances, px + site.x, py + site.y, pz + site.z, there, tx, ty, tz)) continue;
+        float share = 0.5f - there / t.spacing;  // solid, terrain.rs:249-252
+        if (share < 0.0f) share = 0.0f;
+        if (share > 1.0f) share = 1.0f;
+        d += site.w * share;
+        if (share > 0.0f && share < 1.0f) {
+          float k2 = site.w / t.spacing;
+          gx = gx - tx * k2;
+          gy = gy - ty * k2;
+          gz = gz - tz * k2;
+        }
+      }
+    }
+    density = density + d;
+    sx = sx + gx;
+    sy = sy + gy;
+    sz = sz + gz;
+  }
+}
+
+// GroundSites::damping (terrain.rs:236-245)
+__device__ __forceinline__ float ground_damping(const TerrainView& t, float distance) {
+  float at = (fmaxf(distance, t.low) - t.low) / t.step;
+  float f = floorf(at);
+  if (!(f < (float)(kMapSamples - 1))) return 0.0f;  // below + 1 >= MAP_SAMPLES
+  uint32_t below = (uint32_t)f;
+  float part = at - (float)below;
+  return t.damping[below] + (t.damping[below + 1] - t.damping[below]) * part;
+}
+
+// terrain_contact (terrain.rs:315-343): water closer than `rest` to a
+// surface is put back there and stops moving into it; true when it got
+// under the surface.
+__device__ __forceinline__ bool terrain_contact(const TerrainView& t, float rest, float& px, float& py, float& pz,
+                                                float& vx, float& vy, float& vz) {
+  bool under = false;
+  for (uint32_t k = 0; k < t.count; ++k) {
+    float distance, nx, ny, nz;
+    if (!terrain_sample(t.fields[k], t.distances, px, py, pz, distance, nx, ny, nz)) continue;
+    if (distance < rest) {
+      under = under || distance < 0.0f;
+      float lift = rest - distance;
+      px = px + nx * lift;
+      py = py + ny * lift;
+      pz = pz + nz * lift;
+      float into = dot3(vx, vy, vz, nx, ny, nz);
+      if (into < 0.0f) {
+        vx = vx - nx * into;
+        vy = vy - ny * into;
+        vz = vz - nz * into;
+      }
+    }
+  }
+  return under;
+}
+
 // advance (src/fluid/step.rs:305-339) of one particle: next position, the
 // clamped velocity components zeroed, held-back count, finiteness. `clamped`
 // gets bit c set when component c was clamped inside the container.
@@ -469,6 +627,7 @@ __device__ __forceinline__ bool advance_one(const StepParams& prm, float4 p, flo
       nz = iz;
     }
   }
+  if (prm.terrain.count && terrain_contact(prm.terrain, prm.half_spacing, nx, ny, nz, vx, vy, vz)) held += 1;
   return finite3(nx, ny, nz) && finite3(vx, vy, vz);
 }
 
@@ -499,6 +658,20 @@ __device__ __forceinline__ void wall_viscous(const StepParams& prm, float4 at, f
   float t = rmin(dot3(vx, vy, vz, nx, ny, nz), 0.0f);
   float ix = nx * t, iy = ny * t, iz = nz * t;
   float wnu = prm.wall_nu;
+  // the ground's part, on motion into it (step.rs:284-290)
+  for (uint32_t k = 0; k < prm.terrain.count; ++k) {
+    float distance, gx, gy, gz;
+    if (!terrain_sample(prm.terrain.fields[k], prm.terrain.distances, at.x, at.y, at.z, distance, gx, gy, gz))
+      continue;
+    float into = dot3(vx, vy, vz, gx, gy, gz);
+    if (into < 0.0f) {
+      float weight = ground_damping(prm.terrain, distance);
+      float k2 = wnu * into * weight / density;
+      px = px - gx * k2;
+      py = py - gy * k2;
+      pz = pz - gz * k2;
+    }
+  }
   visit_walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
     float toward = dot3(ix, iy, iz, ox, oy, oz);
     if (toward < 0.0f) {

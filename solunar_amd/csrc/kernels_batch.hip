@@ -52,6 +52,8 @@ __device__ __forceinline__ uint32_t scene_slot(const SceneState& st, float4 p, f
 // fills, src/fluid/step.rs:149-165, 254-259, 282-283).
 __device__ __forceinline__ StepParams scene_params(const BatchArgs& b, const SceneState& st, const SceneDesc& d) {
   StepParams p;
+  p.movers = MoverView{};
+  p.terrain = TerrainView{};
   p.n = b.n;
   p.radius = b.radius;
   p.reach = b.reach;

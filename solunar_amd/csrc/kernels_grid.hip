@@ -370,7 +370,14 @@ __global__ __launch_bounds__(256) void k_list_density(const float4* __restrict__
     float ox = o.x - at.x, oy = o.y - at.y, oz = o.z - at.z;
     wsum += kernel_value(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), prm.radius, prm.scale);
   }
-  V[i].w = prm.mass * (sum + wsum);
+  float density = prm.mass * (sum + wsum);
+  if (prm.terrain.count) {
+    // the ground's sites (mod.rs:657-663)
+    float ground, sx, sy, sz;
+    ground_near(prm.terrain, at.x, at.y, at.z, ground, sx, sy, sz);
+    density = density + ground;
+  }
+  V[i].w = density;
 }
 
 // ---- launchers ------------------------------------------------------------

@@ -50,6 +50,13 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
     } else {
       density = prm.mass * (sum + wsum);
     }
+    if (prm.terrain.count) {
+      // the ground's sites and slope (mod.rs:657-663)
+      float ground, sx, sy, sz;
+      ground_near(prm.terrain, at.x, at.y, at.z, ground, sx, sy, sz);
+      density = density + ground;
+      prm.terrain.slopes[i] = make_float4(sx, sy, sz, 0.0f);
+    }
     float pressure = tait_pressure(density, prm.rest, prm.stiffness);
     V[i].w = density;
     scaled[i] = pressure / (density * density);
@@ -88,12 +95,13 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
              });
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
   float mirrored = 0.0f;
-  if (MOVERS || prm.wall_cells) {
+  if (MOVERS || prm.wall_cells || prm.terrain.count) {
     float pressure = tait_pressure(density, prm.rest, prm.stiffness);
     mirrored = own + pressure / (prm.rest * prm.rest);
   }
   auto walls = [&](auto&& f) {
-    visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
+    if (prm.wall_cells)
+      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
   };
   float awx = 0.0f, awy = 0.0f, awz = 0.0f;
   if (prm.wall_cells) {
@@ -108,6 +116,13 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
       awy = awy + -gy;
       awz = awz + -gz;
     });
+  }
+  if (prm.terrain.count) {
+    // push -= slope * mirrored (step.rs:203-205)
+    float4 g = prm.terrain.slopes[i];
+    px = px - g.x * mirrored;
+    py = py - g.y * mirrored;
+    pz = pz - g.z * mirrored;
   }
   if (MOVERS) {
     const MoverView& mv = prm.movers;
@@ -146,7 +161,8 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
       ++slot;
     });
   }
-  if (prm.wall_cells) wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
+  if (prm.wall_cells || prm.terrain.count)
+    wall_viscous(prm, at, vi.x, vi.y, vi.z, density, awx, awy, awz, walls, wx, wy, wz);
   // push + wall_viscous + gravity
   ax = px + wx + prm.gx;
   ay = py + wy + prm.gy;

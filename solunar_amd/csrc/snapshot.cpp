@@ -147,8 +147,11 @@ size_t World::snapshot_size() const {
   // a coupled body: handle, points, probe spacing, probes (couple.rs:50-55)
   size_t bodies = 0;
   for (const Coupled& k : coupled_) bodies += 8 + 4 + (k.points.size() / 3) * 12 + 4 + 4 + (k.probes.size() / 3) * 12;
+  // a terrain region: owner, low, nodes, step, distances (terrain.rs:45-56)
+  size_t ground = 0;
+  for (const Terrain& t : terrains_) ground += 4 + 12 + 12 + 4 + 4 + t.distances.size() * 4;
   return 4 + 4 + 1 + 24 + 8 + 12 + 4 + 4 + n_ * 28 + 4 + 4 + wall_owner_.size() * 16 + 4 + 4 +
-         containers_.size() * 28 + 4 + flows + 4 + 4 + bodies + 4 + (has_domain_ ? 25 : 1) + kChecksumBytes;
+         containers_.size() * 28 + 4 + flows + 4 + 4 + bodies + 4 + ground + (has_domain_ ? 25 : 1) + kChecksumBytes;
 }
 
 // save_snapshot (save.rs:17-66) into `bytes`, snapshot_size() of them
@@ -218,7 +221,16 @@ int32_t World::save_snapshot(uint8_t* bytes) {
     out.len(k.probes.size() / 3);
     for (size_t i = 0; i < k.probes.size(); i += 3) out.vec3(k.probes[i], k.probes[i + 1], k.probes[i + 2]);
   }
-  out.len(0);  // terrain regions
+  // TerrainField::save (terrain.rs:45-56)
+  out.len(terrains_.size());
+  for (const Terrain& t : terrains_) {
+    out.u32(t.owner);
+    out.vec3(t.low[0], t.low[1], t.low[2]);
+    for (uint32_t n : t.nodes) out.u32(n);
+    out.f32(t.step);
+    out.len(t.distances.size());
+    for (float d : t.distances) out.f32(d);
+  }
   if (has_domain_) {
     out.u8(1);
     out.vec3(domain_.lo[0], domain_.lo[1], domain_.lo[2]);
@@ -415,12 +427,38 @@ int32_t World::load_snapshot(const uint8_t* data, size_t size, const sfl_setting
       r.invalid("fluid body order");
       return SFL_SNAPSHOT_REFUSED;
     }
-  // terrain regions: not carried by this world
+  // TerrainField::load (terrain.rs:58-95) in save.rs:172-179's order
   size_t terrains = 0;
   if (!r.count(32, kMaxParts, terrains)) return SFL_SNAPSHOT_REFUSED;
-  if (terrains > 0) {
-    why = "the snapshot holds terrain, which the device world does not carry";
-    return SFL_UNSUPPORTED;
+  std::vector<Terrain> ground;
+  for (size_t t = 0; t < terrains; ++t) {
+    Terrain f{};
+    sfl_vec3 low;
+    if (!r.u32(f.owner) || !r.vec3(low) || !r.u32(f.nodes[0]) || !r.u32(f.nodes[1]) || !r.u32(f.nodes[2]) ||
+        !r.f32(f.step))
+      return SFL_SNAPSHOT_REFUSED;
+    size_t count = 0;
+    if (!r.count(4, size_t(1) << 24, count)) return SFL_SNAPSHOT_REFUSED;  // MAX_NODES, terrain.rs:9
+    const unsigned __int128 total = (unsigned __int128)f.nodes[0] * f.nodes[1] * f.nodes[2];
+    if (!finite3(low) || !std::isfinite(f.step) || f.step <= 0.0f || f.nodes[0] < 2 || f.nodes[1] < 2 ||
+        f.nodes[2] < 2 || total != count) {
+      r.invalid("fluid terrain");
+      return SFL_SNAPSHOT_REFUSED;
+    }
+    f.low[0] = low.x, f.low[1] = low.y, f.low[2] = low.z;
+    f.distances.resize(count);
+    for (size_t k = 0; k < count; ++k) {
+      if (!r.f32(f.distances[k])) return SFL_SNAPSHOT_REFUSED;
+      if (!std::isfinite(f.distances[k])) {
+        r.invalid("fluid terrain distance");
+        return SFL_SNAPSHOT_REFUSED;
+      }
+    }
+    if (f.owner >= wall_next) {
+      r.invalid("fluid terrain owner");
+      return SFL_SNAPSHOT_REFUSED;
+    }
+    ground.push_back(std::move(f));
   }
   bool has_domain = false;
   sfl_vec3 dlo{0, 0, 0}, dhi{0, 0, 0};
@@ -465,6 +503,8 @@ int32_t World::load_snapshot(const uint8_t* data, size_t size, const sfl_setting
   w->flow_next_ = flow_next;
   w->coupled_ = std::move(bodies);
   w->coupled_dirty_ = true;
+  w->terrains_ = std::move(ground);
+  w->terrain_dirty_ = true;
   w->has_domain_ = has_domain;
   w->domain_ = FlowBox{{dlo.x, dlo.y, dlo.z}, {dhi.x, dhi.y, dhi.z}};
   *world_out = w;

@@ -607,7 +607,7 @@ int32_t World::add_wall_points(const std::vector<float>& pts, uint32_t* id_out) 
   return SFL_OK;
 }
 
-// remove_boundary (boundary.rs:94-115), no terrains.
+// remove_boundary (boundary.rs:94-115)
 int32_t World::remove_boundary(uint32_t id, int32_t* removed) {
   ++version_;
   DeviceGuard g(device_);
@@ -627,7 +627,11 @@ int32_t World::remove_boundary(uint32_t id, int32_t* removed) {
     else
       keep_c.push_back(c);
   }
-  bool gone = keep_owner.size() != before || had;
+  const size_t fields = terrains_.size();
+  terrains_.erase(std::remove_if(terrains_.begin(), terrains_.end(), [&](const Terrain& t) { return t.owner == id; }),
+                  terrains_.end());
+  if (terrains_.size() != fields) terrain_dirty_ = true;
+  bool gone = keep_owner.size() != before || had || terrains_.size() != fields;
   if (gone) {
     wall_pos_.swap(keep_pos);
     wall_owner_.swap(keep_owner);
@@ -780,6 +784,7 @@ StepParams World::params(float dt) const {
   p.near = near_.ptr;
   p.factor = factor_.ptr;
   p.format = format_;
+  p.terrain = terrain_view();
   return p;
 }
 
@@ -1103,6 +1108,9 @@ int32_t World::wcsph(float dt, sfl_report& r) {
   float small = dt / (float)substeps;
   st = ensure_cells_for_bounds();
   if (st != SFL_OK) return st;
+  st = upload_terrains();
+  if (st != SFL_OK) return st;
+  if (!terrains_.empty()) SFL_HIP(slopes_.reserve(n_, 0, stream_), "hipMalloc(ground slopes)");
   StepParams prm = params(small);
   uint32_t done = 0, held = 0;
   lists_valid_ = false;
@@ -1172,8 +1180,10 @@ int32_t World::wcsph(float dt, sfl_report& r) {
   return SFL_OK;
 }
 
-// dfsph (src/fluid/dfsph.rs:46-77)
+// dfsph (src/fluid/dfsph.rs:46-77); ground pushes through the weakly
+// compressible solver's pressure only (dfsph.rs:52-55)
 int32_t World::dfsph(float dt, sfl_report& r) {
+  if (!terrains_.empty()) return SFL_INVALID_SETTINGS;
   int32_t st = ensure_fastest();
   if (st != SFL_OK) return st;
   const sfl_settings& s = settings_;
@@ -1357,6 +1367,8 @@ int32_t World::compute_densities() {
   if (!lists_valid_) return SFL_OUT_OF_RANGE;
   int32_t st = emit_lists();
   if (st != SFL_OK) return st;
+  st = upload_terrains();
+  if (st != SFL_OK) return st;
   StepParams prm = params(1.0f);
   launch_list_density(stream_, pa_.ptr, va_.ptr, prm, fstart_.ptr, fitems_.ptr, wstart_.ptr, witems_.ptr);
   SFL_HIP(hipStreamSynchronize(stream_), "densities");
@@ -1509,7 +1521,8 @@ size_t World::device_bytes() const {
          b(pair_key_[0]) + b(pair_key_[1]) + b(pair_value_[0]) + b(pair_value_[1]) + b(pair_force_) +
          b(pair_torque_) + b(pair_temp_) +
          b(extent_) + b(wet_) + b(lift_) + b(probe_arm_) + b(probe_vel_) + b(lift_near_) + b(lift_sort_) +
-         b(lift_columns_) + b(lift_counts_) + b(lift_offsets_) + b(lift_tiles_) + b(lift_boxes_);
+         b(lift_columns_) + b(lift_counts_) + b(lift_offsets_) + b(lift_tiles_) + b(lift_boxes_) +
+         b(terrain_desc_) + b(terrain_dist_) + b(far_d_) + b(damping_d_) + b(sites_d_) + b(slopes_);
 }
 
 int32_t World::synchronize() {

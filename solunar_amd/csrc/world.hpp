@@ -109,6 +109,10 @@ class World {
   size_t coupled_len() const { return coupled_.size(); }
   int32_t step_with_bodies(float dt, const sfl_body_pose* poses, size_t count, sfl_report* report,
                            sfl_body_push* pushes, size_t cap, size_t* out_count);
+  // terrain ground (src/fluid/terrain.rs), terrain.cpp
+  void terrain_nodes(sfl_vec3 low, sfl_vec3 high, uint64_t nodes[3]) const;
+  int32_t add_terrain(sfl_vec3 low, sfl_vec3 high, const float* distances, size_t count, uint32_t* id);
+  size_t terrain_len() const { return terrains_.size(); }
   int32_t update_neighbors();
   int32_t compute_densities();
 
@@ -271,6 +275,26 @@ class World {
     float probe_spacing;
   };
   std::vector<float> body_reach_;  // each body's farthest point from its centre
+  // terrain regions (TerrainField, src/fluid/terrain.rs:35-42) and the
+  // world's GroundSites (terrain.rs:173-247), built with the first region
+  struct Terrain {
+    uint32_t owner;
+    float low[3];
+    uint32_t nodes[3];
+    float step;
+    std::vector<float> distances;
+  };
+  std::vector<Terrain> terrains_;
+  bool terrain_dirty_ = false;
+  std::vector<float4> sites_h_;
+  std::vector<float> far_h_, damping_h_;
+  float ground_low_ = 0, ground_step_ = 0, ground_reach_ = 0;
+  DeviceBuffer<TerrainDesc> terrain_desc_;
+  DeviceBuffer<float> terrain_dist_, far_d_, damping_d_;
+  DeviceBuffer<float4> sites_d_, slopes_;
+  int32_t upload_terrains();
+  void build_ground_sites();
+  TerrainView terrain_view() const;
   std::vector<Coupled> coupled_;
   bool coupled_dirty_ = true;  // the device copies of the points are stale
   uint32_t most_probes_ = 0;

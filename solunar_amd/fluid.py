@@ -18,7 +18,7 @@ from typing import Optional, Sequence, Tuple
 import numpy as np
 
 from .types import (CBatchTuning, CBodyPose, CBodyPush, CReport, CSettings, CShape, CSnapshotError, CVec3,
-                    FluidReport, FluidSettings, PhysicsError, Shape, pose_array)
+                    FluidReport, FluidSettings, PhysicsError, Shape, pose_array, terrain_samples)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsolunar_fluid.so")
 
@@ -76,6 +76,10 @@ def library() -> ctypes.CDLL:
         "sfl_wall_len": (sz, [P]),
         "sfl_read_wall_positions": (i32, [P, sz, fp]),
         "sfl_step": (i32, [P, ctypes.c_float, ctypes.POINTER(CReport)]),
+        "sfl_terrain_nodes": (i32, [P, CVec3, CVec3, ctypes.POINTER(ctypes.c_uint64)]),
+        "sfl_add_terrain": (i32, [P, CVec3, CVec3, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t,
+                                  ctypes.POINTER(ctypes.c_uint32)]),
+        "sfl_terrain_len": (ctypes.c_size_t, [P]),
         "sfl_couple": (i32, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(CShape)]),
         "sfl_uncouple": (i32, [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(i32)]),
         "sfl_coupled_len": (ctypes.c_size_t, [P]),
@@ -307,6 +311,26 @@ class FluidWorld:
         rep = CReport()
         self._check(self._lib.sfl_step(self._h, dt, ctypes.byref(rep)))
         return rep.to_report()
+
+    # ---- terrain ground (terrain.rs) ----
+    def terrain_nodes(self, low, high):
+        """The lattice add_terrain samples over the box: nodes per axis (x, y, z)."""
+        n = (ctypes.c_uint64 * 3)()
+        self._check(self._lib.sfl_terrain_nodes(self._h, CVec3.of(low), CVec3.of(high), n))
+        return tuple(int(v) for v in n)
+
+    def add_terrain(self, low, high, distance) -> int:
+        """add_terrain (src/fluid/terrain.rs:262-305): `distance` is the
+        terrain's signed distance (positive above it), a function of (N, 3)
+        float32 points or the sampled array."""
+        d = terrain_samples(self, low, high, distance)
+        out = ctypes.c_uint32()
+        self._check(self._lib.sfl_add_terrain(self._h, CVec3.of(low), CVec3.of(high), _ptr(d, ctypes.c_float),
+                                              d.size, ctypes.byref(out)))
+        return out.value
+
+    def terrain_len(self) -> int:
+        return int(self._lib.sfl_terrain_len(self._h))
 
     # ---- coupled bodies (couple.rs, lift.rs) ----
     def couple(self, body, shape: Shape) -> None:
