@@ -401,6 +401,57 @@ def run_bodies(args) -> dict:
     }
 
 
+def run_trench(args) -> dict:
+    """trench_720k: a WCSPH dam break down a trench in terrain ground: the
+    ground's density sites, slope, viscosity and contact in every substep."""
+    from solunar_amd import scenes
+    from solunar_amd.fluid import FluidWorld
+
+    world = FluidWorld(scenes.settings_trench_720k())
+    n = scenes.trench_dam(world)
+    log(f"trench_720k: {n} fluid particles; warmup {args.warmup} steps")
+    for k in range(args.warmup):
+        r = world.step(DT)
+        log(f"warmup step {k}: {r.substeps} substeps")
+    world.synchronize()
+    world.profile(True)
+    world.profile_read(reset=True)
+    particle_steps = substeps = held = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        particle_steps += len(world)
+        r = world.step(DT)
+        substeps += r.substeps
+        held += r.held_back
+    world.synchronize()
+    elapsed = time.perf_counter() - t0
+    phases = world.profile_read(reset=True)
+    world.profile(False)
+    world.close()
+    return {
+        "metric": "particle-steps/s of a WCSPH dam break in terrain",
+        "value": particle_steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "trench_720k: WCSPH, spacing 0.005 m, step 5e-4 s; 720,000 particles (1 x 0.3 x "
+                               "0.3 m) released down a 2 m trench 0.3 m deep and wide in flat ground, one terrain "
+                               "region of 833 x 137 x 137 nodes",
+                   "fluid_particles": n, "substeps_per_step": substeps / max(args.steps, 1),
+                   "held_back": held},
+        "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def run_snapshot(args) -> dict:
     """snapshot_1m: save and load of the dam_1m world after `warmup` steps
     (1,024,000 particles at 28 B and 832,064 walls at 16 B in the SLFL
@@ -460,7 +511,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256", "adjoint_64k", "flows_1m",
-                                           "snapshot_1m", "bodies_1m"],
+                                           "snapshot_1m", "bodies_1m", "trench_720k"],
                     default="dam_1m",
                     help="dam_16m: 16,000,000 fluid particles at spacing 0.002 with 200 sphere obstacles; "
                          "batch_256: 256 WCSPH scenes in one batch; flows_1m: dam_1m with an emitter, a "
@@ -499,6 +550,9 @@ def main() -> int:
         return 0
     if args.workload == "bodies_1m":
         print(json.dumps(run_bodies(args)), flush=True)
+        return 0
+    if args.workload == "trench_720k":
+        print(json.dumps(run_trench(args)), flush=True)
         return 0
 
     from solunar_amd import scenes

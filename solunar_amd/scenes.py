@@ -11,6 +11,9 @@ low corner, lattice sites nudged by up to +-0.05 spacing per axis):
   adjoint_64k WCSPH, spacing 0.01, 0.4 x 0.4 x 0.4 block: 64,000 particles with
               seeded N(0, 0.1 m/s) initial velocities, 200 steps of 1e-4 s,
               loss = mean |x_T - x_T*|^2 against the viscosity-0.08 rollout
+  trench_720k WCSPH at spacing 0.005: a dam break of 720,000 particles (1 x
+              0.3 x 0.3 m) down a 2 m trench 0.3 m deep and wide in flat
+              ground, one terrain region of 15.6M nodes
   bodies_1m   dam_1m under WCSPH with 200 coupled spheres (radii uniform in
               0.02 to 0.08 m): 100 resting on the water block's top, half a
               spacing clear of it, settling in at 0.1 m/s, and 100 on the dry
@@ -282,3 +285,30 @@ def trench_distance(points) -> np.ndarray:
 def in_trench(world) -> int:
     """add_terrain over the trench region; returns the terrain's boundary id."""
     return world.add_terrain(TRENCH_LOW, TRENCH_HIGH, trench_distance)
+
+
+# The trench_720k terrain: a 2 m trench, bed at y = -0.3, 0.3 m wide, in
+# flat ground at y = 0; its region a little round it, under 2^24 nodes at a
+# spacing of 0.005.
+LONG_TRENCH_LOW = (-0.04, -0.32, -0.17)
+LONG_TRENCH_HIGH = (2.04, 0.02, 0.17)
+
+
+def long_trench_distance(points) -> np.ndarray:
+    p = np.asarray(points, np.float64)
+    c, h = np.array([1.0, -0.15, 0.0]), np.array([1.0, 0.15, 0.15])
+    q = np.abs(p - c) - h
+    box = np.linalg.norm(np.maximum(q, 0.0), axis=1) + np.minimum(q.max(axis=1), 0.0)
+    return np.maximum(p[:, 1], -box).astype(np.float32)
+
+
+def settings_trench_720k() -> FluidSettings:
+    return FluidSettings(solver=WCSPH, spacing=0.005, rest_density=1000.0, support=2.0, sound_speed=20.0,
+                         viscosity=0.05, gravity=(0.0, -9.81, 0.0), max_particles=1 << 22)
+
+
+def trench_dam(world) -> int:
+    """The trench_720k scene: the terrain, then the water block at the
+    trench's closed end; returns the particle count."""
+    world.add_terrain(LONG_TRENCH_LOW, LONG_TRENCH_HIGH, long_trench_distance)
+    return world.fill_box((0.0, -0.3, -0.15), (1.0, 0.0, 0.15))
