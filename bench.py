@@ -77,6 +77,29 @@ def cpu_baseline(steps: int, warmup: int):
     }
 
 
+def cpu_sample(name: str, build, step, steps: int):
+    """The CPU oracle timed on `steps` steps of a workload from its initial
+    state: `build(world)` makes the scene on an OracleWorld and returns its
+    particle count, `step(world, k)` takes step k and returns its report."""
+    from oracle.oracle import OracleWorld
+
+    threads = min(16, os.cpu_count() or 1)
+    w, n = build(lambda settings: OracleWorld(settings, threads=threads))
+    t0 = time.perf_counter()
+    substeps = 0
+    for k in range(steps):
+        substeps += step(w, k).substeps
+    dt = time.perf_counter() - t0
+    return {
+        "value": n * steps / dt,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"steps 0..{steps - 1} of {name} ({n} particles, {substeps} substeps), "
+                  f"{dt:.1f} s wall on {threads} host threads",
+    }
+
+
 # Batched WCSPH: algorithmic bytes per fluid particle and substep of each
 # kernel group (FP32 state in float4 pairs (x, id) and (v, rho))
 #   sort    : bounds 16, count 16 + 8, scatter 8 + 16 + 8, gather 8 + 32 + 32 = 144
@@ -397,8 +420,26 @@ def run_bodies(args) -> dict:
         "plain_wcsph": {"value": plain_steps / plain, "ms_per_step": plain * 1e3 / args.steps},
         "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else bodies_cpu(args.cpu_steps),
     }
+
+
+def bodies_cpu(steps: int):
+    from solunar_amd import scenes
+    from solunar_amd.types import Shape
+
+    log("cpu baseline (bodies_1m)")
+    field = scenes.body_field()
+
+    def build(make):
+        w = make(scenes.settings_bodies_1m())
+        n = scenes.dam_break(w)
+        for handle, r, *_ in field:
+            w.couple(handle, Shape.sphere(r))
+        return w, n
+
+    return cpu_sample("bodies_1m", build, lambda w, k: w.step_with_bodies(DT, scenes.body_poses(field, k * DT))[0],
+                      steps)
 
 
 def run_trench(args) -> dict:
@@ -448,8 +489,20 @@ def run_trench(args) -> dict:
                    "held_back": held},
         "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else trench_cpu(args.cpu_steps),
     }
+
+
+def trench_cpu(steps: int):
+    from solunar_amd import scenes
+
+    log("cpu baseline (trench_720k)")
+
+    def build(make):
+        w = make(scenes.settings_trench_720k())
+        return w, scenes.trench_dam(w)
+
+    return cpu_sample("trench_720k", build, lambda w, k: w.step(DT), steps)
 
 
 def run_snapshot(args) -> dict:
