@@ -364,7 +364,7 @@ def run_bodies(args) -> dict:
     log(f"bodies_1m: {n} fluid particles, {len(field)} coupled spheres; warmup {args.warmup} steps")
     t = 0.0
     for k in range(args.warmup):
-        r, _ = world.step_with_bodies(DT, scenes.body_poses(field, t))
+        r, _ = world.step_with_bodies(DT, scenes.body_poses_array(field, t))
         t += DT
         log(f"warmup step {k}: {r.substeps} substeps")
     world.synchronize()
@@ -375,10 +375,10 @@ def run_bodies(args) -> dict:
     t0 = time.perf_counter()
     for k in range(args.steps):
         particle_steps += len(world)
-        r, pushes = world.step_with_bodies(DT, scenes.body_poses(field, t))
+        r, pushes = world.step_with_bodies(DT, scenes.body_poses_array(field, t))
         t += DT
         substeps += r.substeps
-        lifted += sum(p.lifted for p in pushes)
+        lifted += int(pushes["lifted"].sum())
     world.synchronize()
     elapsed = time.perf_counter() - t0
     phases = world.profile_read(reset=True)

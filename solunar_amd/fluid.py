@@ -17,8 +17,9 @@ from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
-from .types import (CBatchTuning, CBodyPose, CBodyPush, CReport, CSettings, CShape, CSnapshotError, CVec3,
-                    FluidReport, FluidSettings, PhysicsError, Shape, pose_array, terrain_samples)
+from .types import (POSE_DTYPE, PUSH_DTYPE, CBatchTuning, CBodyPose, CBodyPush, CReport, CSettings, CShape,
+                    CSnapshotError, CVec3, FluidReport, FluidSettings, PhysicsError, Shape, pose_array,
+                    terrain_samples)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsolunar_fluid.so")
 
@@ -349,13 +350,24 @@ class FluidWorld:
 
     def step_with_bodies(self, dt: float, poses):
         """step_with_bodies (src/fluid/couple.rs:207-252) with the bodies
-        where `poses` (BodyPose) has them: (FluidReport, [BodyPush]) with each
-        coupled body's lift and damping impulses, in handle order."""
-        arr = pose_array(poses)
+        where `poses` has them: (FluidReport, pushes) with each coupled body's
+        lift and damping impulses, in handle order. Poses as BodyPose values
+        give BodyPush values back; poses as a numpy array of POSE_DTYPE give a
+        numpy array of PUSH_DTYPE back, with no per-body Python objects."""
         cap = self.coupled_len()
-        out = (CBodyPush * max(cap, 1))()
         n = ctypes.c_size_t()
         rep = CReport()
+        if isinstance(poses, np.ndarray):
+            if poses.dtype != POSE_DTYPE:
+                raise TypeError("poses as an array need POSE_DTYPE")
+            arr = np.ascontiguousarray(poses)
+            out = np.zeros(max(cap, 1), PUSH_DTYPE)
+            self._check(self._lib.sfl_step_with_bodies(
+                self._h, dt, arr.ctypes.data_as(ctypes.POINTER(CBodyPose)), len(arr), ctypes.byref(rep),
+                out.ctypes.data_as(ctypes.POINTER(CBodyPush)), cap, ctypes.byref(n)))
+            return rep.to_report(), out[: n.value]
+        arr = pose_array(poses)
+        out = (CBodyPush * max(cap, 1))()
         self._check(self._lib.sfl_step_with_bodies(self._h, dt, arr, len(poses), ctypes.byref(rep), out, cap,
                                                    ctypes.byref(n)))
         return rep.to_report(), [out[k].to_push() for k in range(n.value)]

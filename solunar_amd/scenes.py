@@ -169,6 +169,26 @@ def body_poses(field, t: float):
     return poses
 
 
+def body_poses_array(field, t: float):
+    """body_poses as one numpy array of POSE_DTYPE (the C layout)."""
+    from .types import POSE_DTYPE
+
+    out = np.zeros(len(field), POSE_DTYPE)
+    idx = np.array([h for h, *_ in field], np.uint32)
+    c = np.array([f[2] for f in field], np.float64)
+    v = np.array([f[3] for f in field], np.float64)
+    spin = np.array([f[4] for f in field], np.float64)
+    half = 0.5 * spin * t
+    out["index"], out["generation"] = idx[:, 0], idx[:, 1]
+    out["position"] = c + v * t
+    out["rotation"][:, 1] = np.sin(half)
+    out["rotation"][:, 3] = np.cos(half)
+    out["linear_velocity"] = v
+    out["angular_velocity"][:, 1] = spin
+    out["dynamic"] = (idx[:, 0] % 2 == 0).astype(np.int32)
+    return out
+
+
 def dam_with_obstacles(world, obstacles: int = 200, seed: int = 7) -> int:
     """The dam break plus `obstacles` spheres added as one boundary."""
     n = dam_break(world)

@@ -304,6 +304,29 @@ class CBodyPose(ctypes.Structure):
     ]
 
 
+def _pose_dtype():
+    import numpy as np
+
+    return np.dtype({"names": ["index", "generation", "position", "rotation", "linear_velocity",
+                               "angular_velocity", "dynamic"],
+                     "formats": ["<u4", "<u4", ("<f8", 3), ("<f4", 4), ("<f4", 3), ("<f4", 3), "<i4"],
+                     "offsets": [0, 4, 8, 32, 48, 60, 72], "itemsize": 80})
+
+
+def _push_dtype():
+    import numpy as np
+
+    return np.dtype({"names": ["index", "generation", "lifted", "lift_impulse", "lift_angular", "impulse", "angular"],
+                     "formats": ["<u4", "<u4", "<i4", ("<f4", 3), ("<f4", 3), ("<f4", 3), ("<f4", 3)],
+                     "offsets": [0, 4, 8, 12, 24, 36, 48], "itemsize": 60})
+
+
+# numpy record layouts of sfl_body_pose and sfl_body_push, for callers that
+# hand poses over and take pushes back as arrays
+POSE_DTYPE = _pose_dtype()
+PUSH_DTYPE = _push_dtype()
+
+
 def pose_array(poses):
     """A ctypes array of CBodyPose from BodyPose values."""
     arr = (CBodyPose * max(len(poses), 1))()

@@ -154,3 +154,26 @@ def test_a_step_that_outgrows_the_cell_table_carries_the_bodies_as_the_oracle_do
     volume = np.prod(cells.max(axis=0) - cells.min(axis=0) + 1)
     assert volume > 2 * (2 * len(x) * 1.25 + 4096)
     same(gpu, cpu, poses, "after")
+
+
+def test_poses_as_an_array_step_as_poses_as_values():
+    print("case: array poses", flush=True)
+    from solunar_amd import scenes
+    from solunar_amd.types import POSE_DTYPE
+
+    field = scenes.body_field(count=12, spacing=0.05, block=(1.0, 0.4, 0.8), box=(1.0, 1.0, 0.8))
+    worlds = []
+    for _ in range(2):
+        w = pool(FluidWorld(FluidSettings()))
+        for handle, r, *_ in field:
+            w.couple(handle, Shape.sphere(r))
+        worlds.append(w)
+    for k in range(3):
+        a = worlds[0].step_with_bodies(TICK, scenes.body_poses(field, k * TICK))
+        arr = scenes.body_poses_array(field, k * TICK)
+        assert arr.dtype == POSE_DTYPE
+        b = worlds[1].step_with_bodies(TICK, arr)
+        assert a[0] == b[0]
+        assert [p.impulse for p in a[1]] == [tuple(x) for x in b[1]["impulse"].tolist()]
+        assert [p.lift_impulse for p in a[1]] == [tuple(x) for x in b[1]["lift_impulse"].tolist()]
+    assert worlds[0].state_hash() == worlds[1].state_hash()
