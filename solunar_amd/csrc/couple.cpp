@@ -228,9 +228,10 @@ int32_t World::place_movers(const sfl_body_pose* poses, size_t count, float dt) 
     probes += s.probes;
   }
   SFL_HIP(bodies_d_.reserve(nb, 0, stream_), "hipMalloc(bodies)");
+  // bodies_h_ stays as it is until the next step, which reads back (and so
+  // waits on) this stream before it writes it again
   SFL_HIP(hipMemcpyAsync(bodies_d_.ptr, bodies_h_.data(), nb * sizeof(BodyState), hipMemcpyHostToDevice, stream_),
           "upload bodies");
-  SFL_HIP(hipStreamSynchronize(stream_), "upload bodies");
   const uint32_t m = points;
   owner_bits_ = 0;
   while ((uint64_t(1) << owner_bits_) < nb) ++owner_bits_;
@@ -456,10 +457,11 @@ int32_t World::lifts(float dt, const sfl_body_pose* poses, size_t count, sfl_bod
     SFL_HIP(hipStreamSynchronize(stream_), "lifts");
     return SFL_OK;
   };
+  // `ls` is next written by a pull, whose wait on the stream comes after
+  // this copy in stream order
   auto push = [&]() -> int32_t {
     SFL_HIP(hipMemcpyAsync(lift_.ptr, ls.data(), nb * sizeof(LiftState), hipMemcpyHostToDevice, stream_),
             "write lifts");
-    SFL_HIP(hipStreamSynchronize(stream_), "lifts");
     return SFL_OK;
   };
   int32_t st = pull();
@@ -523,7 +525,7 @@ int32_t World::lifts(float dt, const sfl_body_pose* poses, size_t count, sfl_bod
     }
     launch_give_back(stream_, lift_near_.ptr + l.near_offset, l.near, change, va_.ptr);
   }
-  SFL_HIP(hipStreamSynchronize(stream_), "give back");
+  // the impulses' read-back after this waits on the give-back launches
   vmax_valid_ = false;
   return SFL_OK;
 }

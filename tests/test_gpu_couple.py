@@ -161,7 +161,7 @@ def test_poses_as_an_array_step_as_poses_as_values():
     from solunar_amd import scenes
     from solunar_amd.types import POSE_DTYPE
 
-    field = scenes.body_field(count=12, spacing=0.05, block=(0.5, 0.4, 0.8), box=(1.0, 1.0, 0.8))
+    field = scenes.body_field(count=12, spacing=0.05, block=(0.5, 0.4, 0.8), box=(1.0, 1.0, 0.8), radii=(0.1, 0.2))
     worlds = []
     for _ in range(2):
         w = pool(FluidWorld(FluidSettings()))
