@@ -1123,14 +1123,17 @@ int32_t World::wcsph(float dt, sfl_report& r) {
     st = ensure_pair_room(std::max<uint64_t>(body_pair_cap_, std::max<uint64_t>((uint64_t)movers_m_ * 16, 65536)));
     if (st != SFL_OK) return st;
   }
+  bool retry = false;
   while (done < substeps) {
     launch_step_begin(stream_, ctl_);
-    if (done > 0 && movers_m_) {
-      // the points' order after `done` substeps: carried again from where
-      // they were placed, as the aborted launches left them behind
+    if (retry && movers_m_) {
+      // the points' order after `done` substeps, carried again from where
+      // they were placed: the aborted launches left the buffers behind the
+      // host's count of carries
       st = replay_movers(done, small);
       if (st != SFL_OK) return st;
     }
+    retry = true;
     for (uint32_t k = done; k < substeps; ++k) {
       st = enqueue_wcsph_substep(prm, k, small);
       if (st != SFL_OK) return st;
@@ -1153,7 +1156,13 @@ int32_t World::wcsph(float dt, sfl_report& r) {
         lists_emitted_ = false;
       }
       if (c.flags & kFlagOverflow) st = ensure_cells(c.need_volume);
-      if (st == SFL_OK && (c.flags & kFlagPairRoom)) st = ensure_pair_room(c.need_pairs + c.need_pairs / 4 + 1024);
+      if (st == SFL_OK && (c.flags & kFlagPairRoom)) {
+        st = ensure_pair_room(c.need_pairs + c.need_pairs / 4 + 1024);
+        if (st == SFL_OK && body_pair_cap_ < c.need_pairs) {
+          detail_ = "the coupled bodies' points pair with more water than 2^32 records hold";
+          return SFL_UNSUPPORTED;
+        }
+      }
       if (st != SFL_OK) return st;
       prm = params(small);
       continue;

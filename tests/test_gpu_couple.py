@@ -177,3 +177,25 @@ def test_poses_as_an_array_step_as_poses_as_values():
         assert [p.impulse for p in a[1]] == [tuple(x) for x in b[1]["impulse"].tolist()]
         assert [p.lift_impulse for p in a[1]] == [tuple(x) for x in b[1]["lift_impulse"].tolist()]
     assert worlds[0].state_hash() == worlds[1].state_hash()
+
+
+def test_a_step_whose_first_substep_outgrows_the_cell_table():
+    # a cluster round a sphere and a few particles 6 m out: the device sizes
+    # its cell table by the particle count (no container), so the very first
+    # substep overflows it and the step runs again from there, carrying the
+    # body points again from their placement
+    print("case: first substep overflows", flush=True)
+    s = FluidSettings()
+    cluster = np.stack(np.meshgrid(*[np.arange(10) * 0.05 + 0.275] * 3, indexing="ij"), -1).reshape(-1, 3)
+    far = np.array([[x, y, z] for x in (-3.0, 3.0) for y in (-3.0, 3.0) for z in (-3.0, 3.0)])
+    x = np.concatenate([cluster, far]).astype(np.float32)
+    gpu, cpu = FluidWorld(s), OracleWorld(s, threads=8)
+    for w in (gpu, cpu):
+        w.add_particles(x, np.zeros_like(x))
+        w.couple((2, 0), Shape.sphere(0.15))
+    cells = np.floor(x / np.float32(s.radius())).astype(np.int64)
+    volume = np.prod(cells.max(axis=0) - cells.min(axis=0) + 1)
+    assert volume > 4 * (2 * len(x) * 1.25 + 4096)
+    for k in range(2):
+        poses = [BodyPose((2, 0), (0.5, 0.5, 0.5 - 0.02 * k), linear_velocity=(0.0, 0.0, -1.5))]
+        same(gpu, cpu, poses, k)
