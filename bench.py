@@ -77,6 +77,28 @@ def cpu_baseline(steps: int, warmup: int):
     }
 
 
+# WCSPH force pass, algorithmic bytes per fluid particle and launch (float4
+# state): x 16, v (with rho) 16 and p/rho^2 4 read, x 16 and v 16 written;
+# with terrain the slope (float4) 16 more read. Neighbour re-reads are not
+# counted: the cell order keeps them in L2.
+FORCE_BYTES = 68
+FORCE_SLOPE_BYTES = 16
+
+
+def force_roofline(phases, particles: int, per_particle: int):
+    """The WCSPH force pass against HBM: algorithmic bytes per launch over
+    its average launch time (HIP events round each launch on the world's
+    stream)."""
+    ms, launches = phases.get("force", (0.0, 0))
+    if not launches or ms <= 0:
+        return None
+    seconds = ms * 1e-3 / launches
+    achieved = particles * per_particle / seconds / 1e9
+    return {"kernel": "k_wcsph_force", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "launch_us": seconds * 1e6, "bytes_per_particle": per_particle}
+
+
 def cpu_sample(name: str, build, step, steps: int):
     """The CPU oracle timed on `steps` steps of a workload from its initial
     state: `build(world)` makes the scene on an OracleWorld and returns its
@@ -419,7 +441,7 @@ def run_bodies(args) -> dict:
                    "launches": bodies_launches},
         "plain_wcsph": {"value": plain_steps / plain, "ms_per_step": plain * 1e3 / args.steps},
         "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
-        "roofline": None,
+        "roofline": force_roofline(phases, n, FORCE_BYTES),
         "cpu_baseline": None if args.no_cpu else bodies_cpu(args.cpu_steps),
     }
 
@@ -488,7 +510,7 @@ def run_trench(args) -> dict:
                    "fluid_particles": n, "substeps_per_step": substeps / max(args.steps, 1),
                    "held_back": held},
         "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
-        "roofline": None,
+        "roofline": force_roofline(phases, n, FORCE_BYTES + FORCE_SLOPE_BYTES),
         "cpu_baseline": None if args.no_cpu else trench_cpu(args.cpu_steps),
     }
 
