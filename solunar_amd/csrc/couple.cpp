@@ -359,7 +359,7 @@ PairRoom World::pair_room() const {
   r.offsets = pair_offsets_b_.ptr;
   r.tiles = pair_tiles_b_.ptr;
   r.total = pair_total_b_.ptr;
-  r.cap = body_pair_cap_;
+  r.cap = body_pair_use_ ? std::min(body_pair_use_, body_pair_cap_) : body_pair_cap_;
   r.none = (uint32_t)coupled_.size();
   r.key_bits = owner_bits_ + 1;
   r.key = pair_key_[0].ptr;

@@ -72,6 +72,7 @@ struct Control {
   uint32_t pad0;
   uint64_t need_volume;  // cells the box needed when it overflowed
   uint64_t need_pairs;   // body-point pair records a substep needed past their room
+  uint32_t pairs_used;   // most body-point pair records a substep of this step used
   uint32_t bad_index;    // smallest sorted index whose advance went non-finite
   uint32_t substeps_done;
   uint32_t iterations[2];        // last solve's pushes: divergence, density

@@ -229,6 +229,7 @@ __global__ __launch_bounds__(256) void k_pairs_check(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ value, Control* ctl) {
   if (ctl->abort) return;
   const uint32_t used = *total;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(&ctl->pairs_used, used);
   if (used > cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       atomicOr(&ctl->flags, kFlagPairRoom);

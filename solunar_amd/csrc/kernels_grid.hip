@@ -21,6 +21,7 @@ __global__ void k_step_begin(Control* ctl) {
   for (int k = 0; k < kStripes; ++k) ctl->stripes[k].held = 0;
   ctl->flags = 0;
   ctl->abort = 0;
+  ctl->pairs_used = 0;
   ctl->bad_index = 0xffffffffu;
   ctl->substeps_done = 0;
   ctl->iterations_step[0] = 0;

@@ -1141,6 +1141,11 @@ int32_t World::wcsph(float dt, sfl_report& r) {
     st = read_control();
     if (st != SFL_OK) return st;
     const Control& c = *ctl_host_;
+    if (movers_m_ && !(c.flags & (kFlagOverflow | kFlagPairRoom))) {
+      // the next step sorts what this one used at most, and a quarter more
+      const uint64_t want = (uint64_t)c.pairs_used + c.pairs_used / 4 + 65536;
+      body_pair_use_ = (uint32_t)std::min<uint64_t>(want, body_pair_cap_);
+    }
     if (c.flags & kFlagMoverBox) {
       detail_ = "the coupled bodies' points spread over more cells than their sort key holds";
       return SFL_UNSUPPORTED;
@@ -1157,6 +1162,7 @@ int32_t World::wcsph(float dt, sfl_report& r) {
       }
       if (c.flags & kFlagOverflow) st = ensure_cells(c.need_volume);
       if (st == SFL_OK && (c.flags & kFlagPairRoom)) {
+        body_pair_use_ = 0;  // sort the whole room until a step says how much it needs
         st = ensure_pair_room(c.need_pairs + c.need_pairs / 4 + 1024);
         if (st == SFL_OK && body_pair_cap_ < c.need_pairs) {
           detail_ = "the coupled bodies' points pair with more water than 2^32 records hold";

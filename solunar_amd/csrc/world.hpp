@@ -316,7 +316,8 @@ class World {
   DeviceBuffer<uint32_t> pair_key_[2], pair_value_[2];
   DeviceBuffer<float4> pair_force_, pair_torque_;
   DeviceBuffer<uint8_t> pair_temp_;
-  uint32_t body_pair_cap_ = 0;
+  uint32_t body_pair_cap_ = 0;  // records the buffers hold
+  uint32_t body_pair_use_ = 0;  // records a substep sorts: the last step's most, and a quarter
   int32_t ensure_pair_room(uint64_t cap);
   PairRoom pair_room() const;
   DeviceBuffer<BodyExtent> extent_;
