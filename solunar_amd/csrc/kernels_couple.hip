@@ -656,9 +656,11 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
   __syncthreads();
   const float level = level_s;
   // the flow of the water under the level, in particle order: the block
-  // stages 1024 particles at a time, lane 0 folds them
+  // stages 1024 particles at a time; thread a < 3 folds component a, each in
+  // particle order, the same sums one lane would make, side by side
   const uint32_t* mine = near + l.near_offset;
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  const uint32_t who = threadIdx.x;
+  float acc = 0.0f;
   uint32_t wet = 0;
   for (uint32_t base = 0; base < l.near; base += 1024) {
     uint32_t k = base + threadIdx.x;
@@ -669,14 +671,12 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
       buf[threadIdx.x] = make_float4(v.x, v.y, v.z, dot3(p.x, p.y, p.z, upx, upy, upz) <= level ? 1.0f : 0.0f);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (who < 4) {
       uint32_t n = min(1024u, l.near - base);
       for (uint32_t j = 0; j < n; ++j) {
         float4 b = buf[j];
         if (b.w != 0.0f) {
-          fx = fx + b.x;
-          fy = fy + b.y;
-          fz = fz + b.z;
+          acc = acc + (who == 0 ? b.x : (who == 1 ? b.y : b.z));
           wet += 1;
         }
       }
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
   }
   // the probes' shares under the level: wetted, middle, the waterline
   float spacing = body.probe_spacing;
-  float wetted = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f;
+  float part = 0.0f;  // lane 0: wetted; lanes 1..3: middle x, y, z
   uint32_t line = 0;
   for (uint32_t base = 0; base < body.probes; base += 1024) {
     uint32_t k = base + threadIdx.x;
@@ -701,32 +701,27 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
                                       : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (who < 4) {
       uint32_t n = min(1024u, body.probes - base);
       for (uint32_t j = 0; j < n; ++j) {
         float4 b = buf[j];
         if (!(b.w > 0.0f)) continue;
-        wetted = wetted + b.w;
-        mx = mx + b.x;
-        my = my + b.y;
-        mz = mz + b.z;
+        part = part + (who == 0 ? b.w : (who == 1 ? b.x : (who == 2 ? b.y : b.z)));
         if (b.w < 1.0f) ++line;
       }
     }
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
-  float w = (float)(wet > 1 ? wet : 1);
-  l.flow[0] = fx / w;
-  l.flow[1] = fy / w;
-  l.flow[2] = fz / w;
-  l.level = level;
-  l.wetted = wetted;
-  l.volume = wetted * (spacing * (spacing * spacing));
-  l.middle[0] = mx;
-  l.middle[1] = my;
-  l.middle[2] = mz;
-  l.line = line;
+  if (who >= 4) return;
+  if (who < 3) l.flow[who] = acc / (float)(wet > 1 ? wet : 1);
+  if (who == 0) {
+    l.level = level;
+    l.wetted = part;
+    l.volume = part * (spacing * (spacing * spacing));
+    l.line = line;
+  } else {
+    l.middle[who - 1] = part;
+  }
 }
 
 // One block per body, after the host has set area = volume^(2/3) / wetted:
@@ -747,18 +742,18 @@ __global__ __launch_bounds__(256) void k_lift_force(LiftState* __restrict__ ls, 
   const BodyState& body = bodies[b];
   const float spacing = body.probe_spacing;
   __shared__ float4 fs[256], ts[256];
-  float f[3], t[3];
-  if (threadIdx.x == 0) {
+  // lane a < 3 folds force component a, lane 3 + a torque component a, each
+  // in probe order, from buoyancy and its torque
+  const uint32_t lane = threadIdx.x;
+  float acc = 0.0f;
+  {
     float mx = l.middle[0] / l.wetted, my = l.middle[1] / l.wetted, mz = l.middle[2] / l.wetted;
     float lift = rest * gravity * l.volume;
     float bx = upx * lift, by = upy * lift, bz = upz * lift;
-    f[0] = 0.0f + bx;
-    f[1] = 0.0f + by;
-    f[2] = 0.0f + bz;
-    cross3(mx, my, mz, bx, by, bz, t[0], t[1], t[2]);
-    t[0] = 0.0f + t[0];
-    t[1] = 0.0f + t[1];
-    t[2] = 0.0f + t[2];
+    float tx, ty, tz;
+    cross3(mx, my, mz, bx, by, bz, tx, ty, tz);
+    float start[6] = {bx, by, bz, tx, ty, tz};
+    acc = 0.0f + start[lane < 6 ? lane : 0];
   }
   const float plane = (float)l.line * spacing * spacing;
   const float waves = rest * plane * sqrtf(gravity * sqrtf(plane));
@@ -802,28 +797,24 @@ __global__ __launch_bounds__(256) void k_lift_force(LiftState* __restrict__ ls, 
       fs[threadIdx.x] = fo;
       ts[threadIdx.x] = to;
       __syncthreads();
-      if (threadIdx.x == 0) {
+      if (lane < 6) {
         uint32_t n = min(256u, body.probes - base);
         for (uint32_t j = 0; j < n; ++j) {
           float4 a = fs[j], c = ts[j];
           if (a.w == 0.0f) continue;
-          f[0] = f[0] + a.x;
-          f[1] = f[1] + a.y;
-          f[2] = f[2] + a.z;
-          t[0] = t[0] + c.x;
-          t[1] = t[1] + c.y;
-          t[2] = t[2] + c.z;
+          float x = lane == 0 ? a.x : lane == 1 ? a.y : lane == 2 ? a.z : lane == 3 ? c.x : lane == 4 ? c.y : c.z;
+          acc = acc + x;
         }
       }
       __syncthreads();
     }
   }
-  if (threadIdx.x != 0) return;
-  for (int a = 0; a < 3; ++a) {
-    l.impulse[a] = f[a] * dt;
-    l.angular[a] = t[a] * dt;
-  }
-  l.lifted = 1;
+  if (lane >= 6) return;
+  if (lane < 3)
+    l.impulse[lane] = acc * dt;
+  else
+    l.angular[lane - 3] = acc * dt;
+  if (lane == 0) l.lifted = 1;
 }
 
 // give_back (lift.rs:154-166): the water read loses what the body got
