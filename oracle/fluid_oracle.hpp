@@ -31,9 +31,14 @@
 // search, brute-force neighbours, rest lattice density, thread-count
 // invariance, sort invariance of the hash, refused input, Tait hydrostatic
 // squeeze, ideal dam-break front, Martin-Moyce front, gate), plus FNV-1a
-// known answers. The DFSPH restatement has no physical known answer in the
-// reference (it is marked "Not ready" there): parity unpinned beyond the
-// invariants (thread invariance, f32 versus f64 agreement, divergence drop).
+// known answers. The DFSPH restatement has no known answer in the reference
+// (it is marked "Not ready" there); it is pinned from outside: the density
+// rate is zero under rigid motion and follows the continuity equation under
+// a dilation, push_apart keeps momentum and angular momentum, factors, rates
+// and velocity changes equal hand-worked FP64 values for two and three
+// particles and a particle by one wall point, the divergence term is zero
+// below 20 neighbours and for a negative rate, and the solves end under
+// their tolerance; planted mistakes 4-8 each fail one of these checks.
 #pragma once
 
 #include <algorithm>
@@ -55,7 +60,10 @@ namespace oracle {
 // Planted mistakes, for tests that show the oracle's checks are not blind.
 // 0 = none; 1 = kernel value without its (1 + 4q) factor; 2 = pressure force
 // with the sign of the pair gradient flipped; 3 = neighbour cells visited dx
-// outer instead of dz outer.
+// outer instead of dz outer. DFSPH: 4 = push_apart with the particle's own
+// stiffness only (kappa_i for kappa_i + kappa_j); 5 = factors without the
+// sum of squares; 6 = divergence rate not clamped at zero; 7 = no free-surface
+// rule (crowd < 20); 8 = density rate without the walls' term.
 inline int& planted_mistake() {
   static int m = 0;
   return m;
@@ -1897,7 +1905,7 @@ class World {
         for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k) {
           V part = gfluid[k] * mass;
           sum = sum + part;
-          squares += length_squared(part);
+          if (planted_mistake() != 5) squares += length_squared(part);
         }
         for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
           sum = sum + gwalls[k] * mass;
@@ -1914,31 +1922,89 @@ class World {
     R rate = R(0);
     for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k)
       rate += dot(velocity - velocities[neighbors.items[k]], gfluid[k]);
-    for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
-      rate += dot(velocity, gwalls[k]);
+    if (planted_mistake() != 8)
+      for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
+        rate += dot(velocity, gwalls[k]);
     return rate * mass;
   }
 
-  // push_apart (src/fluid/dfsph.rs:184-216)
-  void push_apart(const std::vector<R>& stiffness, R dt) {
+  // push_apart (src/fluid/dfsph.rs:184-216): the velocity changes, then
+  // their addition to the velocities.
+  std::vector<V> push_changes(const std::vector<R>& stiffness, R dt) const {
     std::vector<V> changes(len());
     dispatch.run(len(), [&](size_t b, size_t e) {
       for (size_t i = b; i < e; ++i) {
         R own = stiffness[i];
         V change{R(0), R(0), R(0)};
-        for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k)
-          change = change - gfluid[k] * (mass * (own + stiffness[neighbors.items[k]]));
+        for (uint32_t k = neighbors.start[i]; k < neighbors.start[i + 1]; ++k) {
+          R other = planted_mistake() == 4 ? R(0) : stiffness[neighbors.items[k]];
+          change = change - gfluid[k] * (mass * (own + other));
+        }
         for (uint32_t k = wall_neighbors.start[i]; k < wall_neighbors.start[i + 1]; ++k)
           change = change - gwalls[k] * (mass * own);
         changes[i] = change * dt;
       }
     });
+    return changes;
+  }
+  void push_apart(const std::vector<R>& stiffness, R dt) {
+    std::vector<V> changes = push_changes(stiffness, dt);
     for (size_t i = 0; i < len(); ++i) velocities[i] = velocities[i] + changes[i];
+  }
+
+  // One particle's term of a solve iteration (src/fluid/dfsph.rs:236-256):
+  // its stiffness (target times factor) and its error. `which` 0 =
+  // divergence, 1 = density.
+  void solve_term(int which, R dt, R alpha, size_t i, R& stiffness, R& error) const {
+    R rest = R(settings.rest_density);
+    R rate = density_rate(i);
+    R target;
+    if (which == 1) {
+      R over = rmax(densities[i] + dt * rate - rest, R(0));
+      target = over / (dt * dt);
+      error = over / rest;
+    } else {
+      size_t crowd = neighbors.count(i) + wall_neighbors.count(i);
+      R r = planted_mistake() == 6 ? rate : rmax(rate, R(0));
+      if (crowd < 20 && planted_mistake() != 7) r = R(0);  // SURFACE_NEIGHBORS
+      target = r / dt;
+      error = dt * r / rest;
+    }
+    stiffness = target * alpha;
+  }
+
+  // Test hook: the DFSPH quantities of one substep's first solve iteration
+  // at the current state, with the lists already built: densities, factors,
+  // density rates, both solves' stiffness and error, and the velocity
+  // changes push_apart makes for the caller's stiffness (velocities are left
+  // as they are).
+  struct Probe {
+    std::vector<R> alpha, rate, div_stiffness, div_error, den_stiffness, den_error;
+    std::vector<V> change;
+  };
+  int32_t dfsph_probe(R dt, const std::vector<R>* stiffness, Probe& out) {
+    if (!neighbors_built) return kOutOfRange;
+    int32_t st = compute_densities();
+    if (st != kOk) return st;
+    gradients();
+    out.alpha = factors();
+    size_t n = len();
+    out.rate.resize(n);
+    out.div_stiffness.resize(n);
+    out.div_error.resize(n);
+    out.den_stiffness.resize(n);
+    out.den_error.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      out.rate[i] = density_rate(i);
+      solve_term(0, dt, out.alpha[i], i, out.div_stiffness[i], out.div_error[i]);
+      solve_term(1, dt, out.alpha[i], i, out.den_stiffness[i], out.den_error[i]);
+    }
+    if (stiffness) out.change = push_changes(*stiffness, dt);
+    return kOk;
   }
 
   // solve (src/fluid/dfsph.rs:222-275); `which` 0 = divergence, 1 = density.
   uint32_t solve(int which, R dt, const std::vector<R>& alpha) {
-    R rest = R(settings.rest_density);
     float tolerance = which == 0 ? settings.divergence_tolerance : settings.density_tolerance;
     uint32_t iterations = 0;
     size_t n = len();
@@ -1952,22 +2018,7 @@ class World {
     }
     for (;;) {
       dispatch.run(n, [&](size_t b, size_t e) {
-        for (size_t i = b; i < e; ++i) {
-          R rate = density_rate(i);
-          R target, error;
-          if (which == 1) {
-            R over = rmax(densities[i] + dt * rate - rest, R(0));
-            target = over / (dt * dt);
-            error = over / rest;
-          } else {
-            size_t crowd = neighbors.count(i) + wall_neighbors.count(i);
-            R r = crowd < 20 ? R(0) : rmax(rate, R(0));  // SURFACE_NEIGHBORS
-            target = r / dt;
-            error = dt * r / rest;
-          }
-          stiffness[i] = target * alpha[i];
-          errors[i] = error;
-        }
+        for (size_t i = b; i < e; ++i) solve_term(which, dt, alpha[i], i, stiffness[i], errors[i]);
       });
       uint32_t loaded = 0;
       for (R e : errors) loaded += e > R(0);

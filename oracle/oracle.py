@@ -89,6 +89,7 @@ def library() -> ctypes.CDLL:
         "orc_set_particles": (i32, [P, sz, u32p, fp, fp]),
         "orc_read_neighbors": (i32, [P, i32, sz, u32p, sz, u32p, ctypes.POINTER(sz)]),
         "orc_state_hash": (ctypes.c_uint64, [P]),
+        "orc_dfsph_probe": (i32, [P, f] + [ctypes.POINTER(ctypes.c_double)] * 8),
         "orc_error_detail": (ctypes.c_char_p, [P]),
         "orc_fnv1a": (ctypes.c_uint64, [ctypes.c_char_p, sz]),
         "orc_tile_tree_sum": (ctypes.c_double, [fp, ctypes.POINTER(i32), sz]),
@@ -502,6 +503,26 @@ class OracleWorld:
     def neighbors_of(self, index: int) -> np.ndarray:
         s, it = self.neighbors()
         return it[s[index]:s[index + 1]]
+
+    def dfsph_probe(self, dt: float, stiffness=None) -> dict:
+        """The DFSPH quantities of a substep's first solve iteration at the
+        current state, in storage order (call update_neighbors first):
+        factors `alpha`, density `rate`, `div_stiffness`, `div_error`,
+        `den_stiffness`, `den_error`, and, for a given stiffness, the velocity
+        `change` push_apart makes (src/fluid/dfsph.rs:138-256)."""
+        n = len(self)
+        d = ctypes.POINTER(ctypes.c_double)
+        out = {k: np.zeros(n) for k in ("alpha", "rate", "div_stiffness", "div_error", "den_stiffness", "den_error")}
+        change = np.zeros((n, 3))
+        k = None if stiffness is None else np.ascontiguousarray(np.asarray(stiffness, np.float64).reshape(n))
+        self._check(self._lib.orc_dfsph_probe(
+            self._h, dt, None if k is None else _p(k, ctypes.c_double),
+            *[_p(out[name], ctypes.c_double) for name in ("alpha", "rate", "div_stiffness", "div_error",
+                                                          "den_stiffness", "den_error")],
+            _p(change, ctypes.c_double) if k is not None else ctypes.cast(None, d)))
+        if k is not None:
+            out["change"] = change
+        return out
 
     def state_hash(self) -> int:
         return int(self._lib.orc_state_hash(self._h))

@@ -602,6 +602,38 @@ int32_t orc_read_neighbors(void* w, int32_t kind, size_t start_cap, uint32_t* st
   });
 }
 
+// The DFSPH quantities of a substep's first solve iteration at the current
+// state (World::dfsph_probe), lists already built, as doubles in storage
+// order: factors, density rates, divergence and density stiffness and
+// error (n each), and the velocity changes (3n) push_apart makes for the
+// given stiffness (skipped when `stiffness` is null).
+int32_t orc_dfsph_probe(void* w, float dt, const double* stiffness, double* alpha, double* rate, double* div_k,
+                        double* div_e, double* den_k, double* den_e, double* change) {
+  return with(static_cast<Handle*>(w), [&](auto& x) {
+    using R = std::decay_t<decltype(x.mass)>;
+    using WorldT = std::decay_t<decltype(x)>;
+    typename WorldT::Probe pr;
+    std::vector<R> k;
+    if (stiffness) k.assign(stiffness, stiffness + x.len());
+    int32_t st = x.dfsph_probe(R(dt), stiffness ? &k : nullptr, pr);
+    if (st != kOk) return st;
+    for (size_t i = 0; i < x.len(); ++i) {
+      alpha[i] = pr.alpha[i];
+      rate[i] = pr.rate[i];
+      div_k[i] = pr.div_stiffness[i];
+      div_e[i] = pr.div_error[i];
+      den_k[i] = pr.den_stiffness[i];
+      den_e[i] = pr.den_error[i];
+      if (stiffness) {
+        change[3 * i] = pr.change[i].x;
+        change[3 * i + 1] = pr.change[i].y;
+        change[3 * i + 2] = pr.change[i].z;
+      }
+    }
+    return (int32_t)kOk;
+  });
+}
+
 uint64_t orc_state_hash(void* w) {
   return with(static_cast<Handle*>(w), [&](auto& x) { return x.state_hash(); });
 }
