@@ -315,6 +315,23 @@ int32_t sfl_read_wall_positions(const sfl_world* world, size_t cap,
  * reference has no counterpart: the device path has one schedule. */
 int32_t sfl_step(sfl_world* world, float dt, sfl_report* report_out);
 
+/* No counterpart in the reference: sfl_step for callers that cannot
+ * allocate device memory (HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same results and report as sfl_step, and it calls
+ * no hipMalloc or hipFree: the step's control block and cell table live in a
+ * 32 MiB scratch array that each GPU holds from the first call until the
+ * process ends and that every call on that GPU shares (calls overlapping on
+ * one GPU race; each returns after its last kernel). WCSPH worlds without
+ * coupled bodies, terrain, flows or a domain (else SFL_UNSUPPORTED); a sort
+ * needing more cells than the scratch holds (about 4.19M, less 1 per 2,048
+ * particles) returns SFL_OUT_OF_RANGE with the last finished substep's state.
+ * The last grid is kept when the world's own cell table holds it (it does
+ * after any sfl_step or sfl_update_neighbors at least as large); otherwise
+ * the neighbour lists go stale (sfl_compute_densities and sfl_read_neighbors
+ * return SFL_OUT_OF_RANGE until sfl_update_neighbors), and a coupled step
+ * whose fast body needs that grid returns SFL_UNSUPPORTED. */
+int32_t sfl_step_no_alloc(sfl_world* world, float dt, sfl_report* report_out);
+
 /* update_neighbors / compute_densities (src/fluid/mod.rs:540-675): the
  * neighbour build and the density pass alone, for parity checks. */
 int32_t sfl_update_neighbors(sfl_world* world);

@@ -242,6 +242,10 @@ int32_t sfl_step(sfl_world* world, float dt, sfl_report* report_out) {
   return guarded(world, [&](sfl::World& w) { return w.step(dt, report_out); });
 }
 
+int32_t sfl_step_no_alloc(sfl_world* world, float dt, sfl_report* report_out) {
+  return guarded(world, [&](sfl::World& w) { return w.step_no_alloc(dt, report_out); });
+}
+
 int32_t sfl_update_neighbors(sfl_world* world) {
   return guarded(world, [&](sfl::World& w) { return w.update_neighbors(); });
 }

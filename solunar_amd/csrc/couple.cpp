@@ -391,6 +391,11 @@ int32_t World::fastest_near_water(float dt, float slow, float& most) {
     float speed;
     std::memcpy(&speed, &ext[b].speed_bits, 4);
     if (!(speed > slow) || !grid_any_) continue;
+    if (grid_dropped_) {
+      detail_ = "a fast body needs the water's last grid, which an allocation-free step could not keep "
+                "(sfl_step or sfl_update_neighbors builds one)";
+      return SFL_UNSUPPORTED;
+    }
     float reach = r + (speed + water) * dt;
     WetQuery w{b, {}, {}};
     for (int a = 0; a < 3; ++a) {

@@ -833,6 +833,10 @@ int32_t World::ensure_pairs(uint32_t cap) {
 
 int32_t World::ensure_cells(uint64_t volume) {
   if (volume <= cell_cap_ && counts_.ptr) return SFL_OK;
+  if (no_alloc_) {
+    detail_ = "the fluid spreads over more cells than the allocation-free step's scratch holds";
+    return SFL_OUT_OF_RANGE;
+  }
   if (volume > kMaxCells) {
     detail_ = "the fluid spread over more cells than the device cell table holds";
     return SFL_OUT_OF_RANGE;
@@ -894,7 +898,7 @@ int32_t World::ensure_fastest() {
 }
 
 void World::phase_begin(int phase) {
-  if (!profiling_) return;
+  if (!profiling_ || no_alloc_) return;
   hipEvent_t e;
   if (free_events_.empty()) {
     if (hipEventCreate(&e) != hipSuccess) return;
@@ -908,7 +912,7 @@ void World::phase_begin(int phase) {
 }
 
 void World::phase_end(int phase) {
-  if (!profiling_ || !open_) return;
+  if (!profiling_ || no_alloc_ || !open_) return;
   hipEvent_t e;
   if (free_events_.empty()) {
     if (hipEventCreate(&e) != hipSuccess) return;
@@ -939,6 +943,8 @@ void World::collect_profile() {
 void World::enqueue_sort() {
   uint32_t n = (uint32_t)n_;
   grid_any_ = n > 0;
+  grid_dropped_ = false;
+  ++sorts_;
   launch_substep_begin(stream_, ctl_);
   phase_begin(SFL_PHASE_BOUNDS);
   launch_bounds(stream_, pa_.ptr, n, radius_, ctl_);
@@ -1077,6 +1083,111 @@ int32_t World::step(float dt, sfl_report* out) {
   if (flowing) phase_end(SFL_PHASE_FLOWS);
   if (out) *out = r;
   return SFL_OK;
+}
+
+// The allocation-free step (sfl_step_no_alloc): World::step with the
+// control block and the sort's cell table (counts, their scan, the scan's
+// tile sums and total) lent from the shared scratch of this GPU. The particle
+// state and its sorted copy are the world's own, sized when the particles
+// were added, so nothing needs to grow; a buffer that would have to fails
+// the call (t_no_alloc) instead of allocating. The last grid is copied into
+// the world's own table when it has one large enough, else it is dropped
+// with the lists that refer to it.
+int32_t World::step_no_alloc(float dt, sfl_report* out) {
+  DeviceGuard g(device_);
+  if (settings_.solver != SFL_WCSPH) {
+    detail_ = "the allocation-free step runs WCSPH worlds: a DFSPH step keeps neighbour lists that grow with the water";
+    return SFL_UNSUPPORTED;
+  }
+  if (!coupled_.empty() || !terrains_.empty() || !flows_.empty() || has_domain_) {
+    detail_ = "the allocation-free step runs worlds without coupled bodies, terrain, flows or a domain";
+    return SFL_UNSUPPORTED;
+  }
+  if (keep_forces_ && forces_.cap < 3 * n_) {
+    detail_ = "the kept forces need a larger buffer than the world holds";
+    return SFL_UNSUPPORTED;
+  }
+  if (n_ == 0 || !(dt > 0.0f) || !std::isfinite(dt)) {
+    // no device work: the refusals and the empty step of World::step
+    t_no_alloc = true;
+    int32_t st = step(dt, out);
+    t_no_alloc = false;
+    return st;
+  }
+  unsigned char* base = nullptr;
+  SFL_HIP(no_alloc_scratch(device_, &base), "hipGetSymbolAddress(scratch)");
+  // layout, each piece on 256 bytes: control block, scan total, cell counts,
+  // cell table (cap + 1), scan tile sums
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t head = up(sizeof(Control)) + up(sizeof(uint32_t));
+  const size_t room = kNoAllocScratchBytes - head;
+  auto tiles_for = [&](uint64_t c) { return std::max<uint64_t>((c + 2047) / 2048 + 1, ((uint64_t)n_ + 2048) / 2048); };
+  auto need = [&](uint64_t c) { return up(4 * c) + up(4 * (c + 1)) + up(4 * tiles_for(c)); };
+  uint64_t cap = room / 8;
+  while (cap > 0 && need(cap) > room) cap -= std::min<uint64_t>(cap, 4096);
+  if (cap < 4096) {
+    detail_ = "the world holds more particles than the allocation-free step's scratch can sort";
+    return SFL_OUT_OF_RANGE;
+  }
+  uint32_t* counts = reinterpret_cast<uint32_t*>(base + head);
+  uint32_t* starts = reinterpret_cast<uint32_t*>(base + head + up(4 * cap));
+  uint32_t* tiles = reinterpret_cast<uint32_t*>(base + head + up(4 * cap) + up(4 * (cap + 1)));
+
+  struct Lease {
+    World& w;
+    Control* ctl;
+    uint64_t cell_cap;
+    DeviceBuffer<uint32_t> counts, starts, tiles, total;  // the world's own, set aside
+    static void lend(DeviceBuffer<uint32_t>& member, DeviceBuffer<uint32_t>& keep, uint32_t* view, size_t cap) {
+      keep.swap(member);
+      member.ptr = view;
+      member.cap = cap;
+    }
+    static void give_back(DeviceBuffer<uint32_t>& member, DeviceBuffer<uint32_t>& keep) {
+      member.ptr = nullptr;  // a view into the scratch: never freed
+      member.cap = 0;
+      member.swap(keep);
+    }
+    ~Lease() {
+      give_back(w.counts_, counts);
+      give_back(w.cell_start_, starts);
+      give_back(w.tile_sums_, tiles);
+      give_back(w.scan_total_, total);
+      w.ctl_ = ctl;
+      w.cell_cap_ = cell_cap;
+      w.no_alloc_ = false;
+      t_no_alloc = false;
+    }
+  } lease{*this, ctl_, cell_cap_, {}, {}, {}, {}};
+  Lease::lend(counts_, lease.counts, counts, cap);
+  Lease::lend(cell_start_, lease.starts, starts, cap + 1);
+  Lease::lend(tile_sums_, lease.tiles, tiles, tiles_for(cap));
+  Lease::lend(scan_total_, lease.total, reinterpret_cast<uint32_t*>(base + up(sizeof(Control))), 1);
+  ctl_ = reinterpret_cast<Control*>(base);
+  cell_cap_ = cap;
+  no_alloc_ = true;
+  t_no_alloc = true;
+  // the control block and the counts start at zero; the rest is written
+  // before it is read
+  launch_fill_zero(stream_, base, head + up(4 * cap));
+  const uint64_t sorts = sorts_;
+  int32_t st = step(dt, out);
+  if (sorts_ != sorts) {
+    // the last sort's grid: the cell box (lo, span, volume) and the table
+    const uint64_t volume = ctl_host_->volume;
+    if (st == SFL_OK && lists_valid_ && lease.counts.ptr && volume <= lease.cell_cap) {
+      SFL_HIP(hipMemcpyAsync(lease.starts.ptr, starts, (volume + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                             stream_),
+              "keep the cell table");
+      SFL_HIP(hipMemcpyAsync(lease.ctl, ctl_, offsetof(Control, flags), hipMemcpyDeviceToDevice, stream_),
+              "keep the cell box");
+      SFL_HIP(hipStreamSynchronize(stream_), "keep the grid");
+    } else {
+      invalidate_lists();
+      grid_dropped_ = true;
+    }
+  }
+  return st;
 }
 
 // wcsph / wcsph_steps (src/fluid/step.rs:102-142): the step's sound speed

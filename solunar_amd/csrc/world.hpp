@@ -17,6 +17,16 @@
 
 namespace sfl {
 
+// Set while an allocation-free step runs on this thread: a buffer that would
+// have to grow then fails instead of calling hipMalloc.
+inline thread_local bool t_no_alloc = false;
+
+// The allocation-free step's scratch (no_alloc.hip): its size, the device
+// address of one GPU's copy, and a kernel that clears part of it.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+hipError_t no_alloc_scratch(int device, unsigned char** out);
+void launch_fill_zero(hipStream_t s, void* p, size_t bytes);
+
 template <class T>
 struct DeviceBuffer {
   T* ptr = nullptr;
@@ -41,6 +51,7 @@ struct DeviceBuffer {
 template <class T>
 hipError_t DeviceBuffer<T>::reserve(size_t n, size_t keep, hipStream_t s) {
   if (n <= cap) return hipSuccess;
+  if (t_no_alloc) return hipErrorOutOfMemory;
   size_t want = std::max(n, cap + cap / 2);
   T* fresh = nullptr;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&fresh), want * sizeof(T));
@@ -103,6 +114,9 @@ class World {
   int32_t remove_boundary(uint32_t id, int32_t* removed);
 
   int32_t step(float dt, sfl_report* report);
+  // step of a WCSPH world with its control block and cell table in the
+  // shared scratch (no_alloc.hip): calls no hipMalloc or hipFree
+  int32_t step_no_alloc(float dt, sfl_report* report);
   // coupled bodies (src/fluid/couple.rs, src/fluid/lift.rs), couple.cpp
   int32_t couple(uint32_t index, uint32_t generation, const sfl_shape& shape);
   int32_t uncouple(uint32_t index, uint32_t generation, int32_t* was);
@@ -303,6 +317,12 @@ class World {
   uint32_t owner_bits_ = 0;
   int mcur_ = 0;           // which of ms_/mv_ holds the current order
   bool grid_any_ = false;  // the water's last grid (update_neighbors) holds a cell
+  // an allocation-free step left its last grid in the shared scratch, where
+  // the world could not keep it: the lists are stale and the fast-body rule
+  // has no grid to read until the next sort
+  bool grid_dropped_ = false;
+  bool no_alloc_ = false;  // inside step_no_alloc
+  uint64_t sorts_ = 0;     // sorts enqueued so far
   DeviceBuffer<float4> body_local_, probe_local_;
   DeviceBuffer<BodyState> bodies_d_;
   DeviceBuffer<float4> ms_[2], mv_[2], mp_;

@@ -77,6 +77,7 @@ def library() -> ctypes.CDLL:
         "sfl_wall_len": (sz, [P]),
         "sfl_read_wall_positions": (i32, [P, sz, fp]),
         "sfl_step": (i32, [P, ctypes.c_float, ctypes.POINTER(CReport)]),
+        "sfl_step_no_alloc": (i32, [P, ctypes.c_float, ctypes.POINTER(CReport)]),
         "sfl_terrain_nodes": (i32, [P, CVec3, CVec3, ctypes.POINTER(ctypes.c_uint64)]),
         "sfl_add_terrain": (i32, [P, CVec3, CVec3, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t,
                                   ctypes.POINTER(ctypes.c_uint32)]),
@@ -311,6 +312,13 @@ class FluidWorld:
     def step(self, dt: float) -> FluidReport:
         rep = CReport()
         self._check(self._lib.sfl_step(self._h, dt, ctypes.byref(rep)))
+        return rep.to_report()
+
+    def step_no_alloc(self, dt: float) -> FluidReport:
+        """`step` without device allocation (sfl_step_no_alloc): WCSPH
+        worlds, scratch shared by every such call on the GPU."""
+        rep = CReport()
+        self._check(self._lib.sfl_step_no_alloc(self._h, dt, ctypes.byref(rep)))
         return rep.to_report()
 
     # ---- terrain ground (terrain.rs) ----
