@@ -400,6 +400,14 @@ int32_t sfl_work_read(sfl_world* world, uint64_t* particle_substeps,
  * tables, neighbour lists, walls). */
 size_t sfl_memory_bytes(const sfl_world* world);
 
+/* No counterpart in the reference (a parity hook, like the oracle's solve
+ * log): the average error every DFSPH stop test of the last step saw, in
+ * order, positive for a divergence solve and negated for a density solve,
+ * and how many particles it averaged over (dfsph.rs:263-268). At most 256
+ * entries; *len_out gets the count held. */
+int32_t sfl_read_solve_log(sfl_world* world, size_t cap, double* average,
+                           uint32_t* loaded, size_t* len_out);
+
 /* Blocks of 1024 threads the persistent DFSPH solve runs on (every one
  * resident for its grid barrier). */
 int32_t sfl_solve_grid(const sfl_world* world);

@@ -353,6 +353,10 @@ int32_t sfl_work_read(sfl_world* world, uint64_t* particle_substeps, uint64_t* p
                  [&](sfl::World& w) { return w.work_read(particle_substeps, particle_pushes, particle_rates, reset != 0); });
 }
 
+int32_t sfl_read_solve_log(sfl_world* world, size_t cap, double* average, uint32_t* loaded, size_t* len_out) {
+  return guarded(world, [&](sfl::World& w) { return w.read_solve_log(cap, average, loaded, len_out); });
+}
+
 size_t sfl_memory_bytes(const sfl_world* world) { return world && world->w ? world->w->device_bytes() : 0; }
 
 int32_t sfl_solve_grid(const sfl_world* world) { return world && world->w ? world->w->solve_grid() : 0; }

@@ -1681,6 +1681,24 @@ int32_t World::profile_read(double* ms, uint64_t* launches, bool reset) {
   return SFL_OK;
 }
 
+int32_t World::read_solve_log(size_t cap, double* average, uint32_t* loaded, size_t* len) {
+  DeviceGuard g(device_);
+  uint32_t n = 0;
+  SFL_HIP(hipMemcpyAsync(&n, &ctl_->log_len, sizeof(n), hipMemcpyDeviceToHost, stream_), "read solve log");
+  SFL_HIP(hipStreamSynchronize(stream_), "read solve log");
+  n = std::min<uint32_t>(n, (uint32_t)kResidualLogCap);
+  if (len) *len = n;
+  const size_t take = std::min<size_t>(cap, n);
+  if (take && average)
+    SFL_HIP(hipMemcpyAsync(average, ctl_->residual_log, take * sizeof(double), hipMemcpyDeviceToHost, stream_),
+            "read solve log");
+  if (take && loaded)
+    SFL_HIP(hipMemcpyAsync(loaded, ctl_->loaded_log, take * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+            "read solve log");
+  SFL_HIP(hipStreamSynchronize(stream_), "read solve log");
+  return SFL_OK;
+}
+
 int32_t World::work_read(uint64_t* substeps, uint64_t* pushes, uint64_t* rates, bool reset) {
   if (substeps) *substeps = work_substeps_;
   if (pushes) *pushes = work_pushes_;

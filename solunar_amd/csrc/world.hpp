@@ -145,6 +145,7 @@ class World {
   int32_t profile(bool enable);
   int32_t profile_read(double* ms, uint64_t* launches, bool reset);
   int32_t work_read(uint64_t* substeps, uint64_t* pushes, uint64_t* rates, bool reset);
+  int32_t read_solve_log(size_t cap, double* average, uint32_t* loaded, size_t* len);
 
   const sfl_settings& settings() const { return settings_; }
   float mass() const { return mass_; }

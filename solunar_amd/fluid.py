@@ -107,6 +107,8 @@ def library() -> ctypes.CDLL:
         "sfl_synchronize": (i32, [P]),
         "sfl_solve_grid": (i32, [P]),
         "sfl_memory_bytes": (sz, [P]),
+        "sfl_read_solve_log": (i32, [P, sz, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32),
+                                     ctypes.POINTER(sz)]),
         "sfl_batch_create": (i32, [ctypes.POINTER(P), sz, ctypes.POINTER(P)]),
         "sfl_batch_destroy": (None, [P]),
         "sfl_batch_scenes": (sz, [P]),
@@ -302,6 +304,9 @@ class FluidWorld:
         self._check(self._lib.sfl_remove_boundary(self._h, boundary, ctypes.byref(out)))
         return bool(out.value)
 
+    def wall_len(self) -> int:
+        return int(self._lib.sfl_wall_len(self._h))
+
     def wall_positions(self) -> np.ndarray:
         n = int(self._lib.sfl_wall_len(self._h))
         out = np.zeros((n, 3), dtype=np.float32)
@@ -465,6 +470,19 @@ class FluidWorld:
 
     def memory_bytes(self) -> int:
         return int(self._lib.sfl_memory_bytes(self._h))
+
+    def solve_log(self):
+        """The last step's DFSPH stop tests: signed averages (+ divergence,
+        - density) and the particles each averaged over."""
+        n = ctypes.c_size_t()
+        self._check(self._lib.sfl_read_solve_log(self._h, 0, None, None, ctypes.byref(n)))
+        avg = np.zeros(n.value, np.float64)
+        loaded = np.zeros(n.value, np.uint32)
+        if n.value:
+            self._check(self._lib.sfl_read_solve_log(self._h, n.value, avg.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                     loaded.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                     ctypes.byref(n)))
+        return avg, loaded
 
     def solve_grid(self) -> int:
         return int(self._lib.sfl_solve_grid(self._h))
