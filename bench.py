@@ -663,7 +663,6 @@ def main() -> int:
     world.profile(False)
     solve_blocks = world.solve_grid()
     memory = world.memory_bytes()
-    world.close()  # free device memory before the runtime (or a profiler) shuts down
     log(f"timed: {elapsed:.3f} s, {substeps} substeps, {iterations} iterations")
 
     value = n * args.steps / elapsed

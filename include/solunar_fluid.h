@@ -141,6 +141,19 @@ int32_t sfl_fill_box_jittered(sfl_world* world, sfl_vec3 low, sfl_vec3 high,
 int32_t sfl_set_domain(sfl_world* world, const sfl_vec3* low,
                        const sfl_vec3* high);
 
+/* FluidWorld::domain (src/fluid/mod.rs:446-448): *has_out 1 and the box
+ * when a domain is set, else 0 (the box is left alone). */
+int32_t sfl_get_domain(const sfl_world* world, int32_t* has_out,
+                       sfl_vec3* low_out, sfl_vec3* high_out);
+
+/* FluidWorld::kernel (mod.rs:376-378): the world's Wendland C2 kernel,
+ * Kernel::value, slope and gradient (mod.rs:140-164) in the f32 arithmetic
+ * of the device passes. A null world gives 0. */
+float sfl_kernel_value(const sfl_world* world, float distance);
+float sfl_kernel_slope(const sfl_world* world, float distance);
+int32_t sfl_kernel_gradient(const sfl_world* world, sfl_vec3 offset,
+                            sfl_vec3* out);
+
 /* FluidWorld::add_emitter / add_drain (src/fluid/flow.rs:92-111). After each
  * step an emitter fills every lattice slot of its box (spacing apart, half a
  * spacing in from its low corner) that no water is nearer to than 0.9
@@ -357,6 +370,12 @@ int32_t sfl_read_forces(const sfl_world* world, size_t cap, float* xyz);
 int32_t sfl_read_neighbors(sfl_world* world, int32_t kind, size_t start_cap,
                            uint32_t* start, size_t items_cap, uint32_t* items,
                            size_t* items_len_out);
+
+/* FluidWorld::neighbors_of (src/fluid/mod.rs:612-614): the fluid list of
+ * particle `index` (storage order) from the last build. *len_out receives
+ * its length even if `cap` is too small (then SFL_OUT_OF_RANGE). */
+int32_t sfl_neighbors_of(sfl_world* world, size_t index, size_t cap,
+                         uint32_t* out, size_t* len_out);
 
 /* FluidWorld::state_hash (src/fluid/mod.rs:678-691): FNV-1a 64
  * (src/snapshot/bytes.rs:6-13) over (id, x, v) in id order. */

@@ -242,6 +242,32 @@ int32_t sfl_step(sfl_world* world, float dt, sfl_report* report_out) {
   return guarded(world, [&](sfl::World& w) { return w.step(dt, report_out); });
 }
 
+int32_t sfl_get_domain(const sfl_world* world, int32_t* has_out, sfl_vec3* low_out, sfl_vec3* high_out) {
+  if (!world || !world->w) return SFL_INVALID_HANDLE;
+  const bool has = world->w->domain(low_out, high_out);
+  if (has_out) *has_out = has ? 1 : 0;
+  return SFL_OK;
+}
+
+float sfl_kernel_value(const sfl_world* world, float distance) {
+  return world && world->w ? world->w->kernel_value_at(distance) : 0.0f;
+}
+
+float sfl_kernel_slope(const sfl_world* world, float distance) {
+  return world && world->w ? world->w->kernel_slope_at(distance) : 0.0f;
+}
+
+int32_t sfl_kernel_gradient(const sfl_world* world, sfl_vec3 offset, sfl_vec3* out) {
+  if (!world || !world->w) return SFL_INVALID_HANDLE;
+  if (!out) return SFL_OUT_OF_RANGE;
+  *out = world->w->kernel_gradient_at(offset);
+  return SFL_OK;
+}
+
+int32_t sfl_neighbors_of(sfl_world* world, size_t index, size_t cap, uint32_t* out, size_t* len_out) {
+  return guarded(world, [&](sfl::World& w) { return w.neighbors_of(index, cap, out, len_out); });
+}
+
 int32_t sfl_step_no_alloc(sfl_world* world, float dt, sfl_report* report_out) {
   return guarded(world, [&](sfl::World& w) { return w.step_no_alloc(dt, report_out); });
 }

@@ -145,6 +145,7 @@ int32_t World::couple(uint32_t index, uint32_t generation, const sfl_shape& shap
     return std::tie(a.index, a.generation) < std::tie(b.index, b.generation);
   });
   coupled_dirty_ = true;
+  body_pair_use_ = 0;  // the last step's pair count says nothing of the new set
   return SFL_OK;
 }
 
@@ -155,7 +156,10 @@ int32_t World::uncouple(uint32_t index, uint32_t generation, int32_t* was) {
                                 [&](const Coupled& k) { return k.index == index && k.generation == generation; }),
                  coupled_.end());
   bool gone = coupled_.size() != before;
-  if (gone) coupled_dirty_ = true;
+  if (gone) {
+    coupled_dirty_ = true;
+    body_pair_use_ = 0;
+  }
   if (was) *was = gone ? 1 : 0;
   return SFL_OK;
 }
@@ -176,7 +180,10 @@ int32_t World::place_movers(const sfl_body_pose* poses, size_t count, float dt) 
   coupled_.erase(std::remove_if(coupled_.begin(), coupled_.end(),
                                 [&](const Coupled& k) { return !pose_of(poses, count, k.index, k.generation); }),
                  coupled_.end());
-  if (coupled_.size() != before) coupled_dirty_ = true;
+  if (coupled_.size() != before) {
+    coupled_dirty_ = true;
+    body_pair_use_ = 0;
+  }
   const uint32_t nb = (uint32_t)coupled_.size();
   movers_m_ = 0;
   if (nb == 0) return SFL_OK;

@@ -296,7 +296,7 @@ __host__ __device__ __forceinline__ uint32_t tile_depth(uint32_t sz) {
 
 // ---- Wendland C2 (src/fluid/mod.rs:126-165) ------------------------------
 
-__device__ __forceinline__ float kernel_value(float distance, float radius, float scale) {
+__host__ __device__ __forceinline__ float kernel_value(float distance, float radius, float scale) {
   float q = distance / radius;
   if (q >= 1.0f) return 0.0f;
   float t = 1.0f - q;
@@ -305,7 +305,7 @@ __device__ __forceinline__ float kernel_value(float distance, float radius, floa
   return scale * t4 * (1.0f + 4.0f * q);
 }
 
-__device__ __forceinline__ float kernel_slope(float distance, float radius, float scale) {
+__host__ __device__ __forceinline__ float kernel_slope(float distance, float radius, float scale) {
   float q = distance / radius;
   if (q >= 1.0f) return 0.0f;
   float t = 1.0f - q;
@@ -314,7 +314,7 @@ __device__ __forceinline__ float kernel_slope(float distance, float radius, floa
 }
 
 // Kernel::gradient (mod.rs:158-164) given the offset and its squared length.
-__device__ __forceinline__ void kernel_gradient(float ox, float oy, float oz, float squared, float radius, float scale,
+__host__ __device__ __forceinline__ void kernel_gradient(float ox, float oy, float oz, float squared, float radius, float scale,
                                                 float& gx, float& gy, float& gz) {
   float distance = sqrtf(squared);
   if (distance <= 0.0f) {
@@ -330,7 +330,7 @@ __device__ __forceinline__ void kernel_gradient(float ox, float oy, float oz, fl
 }
 
 // (x*x' + y*y') + z*z', no contraction
-__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+__host__ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
   return ax * bx + ay * by + az * bz;
 }
 

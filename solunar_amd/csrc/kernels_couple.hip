@@ -584,8 +584,9 @@ __global__ __launch_bounds__(1024) void k_lift_near(const float4* __restrict__ P
 }
 
 // One block per body: the columns' tops sorted (bitonic, total_cmp), the
-// level from the mean of their middle half; then lane 0 folds, in order,
-// the flow of the water under the level and the probes' shares
+// level from the mean of their middle half; then lanes 0-3 fold the flow of
+// the water under the level and the probes' wetted share and middle, one
+// component a lane, each component's sum in the reference's order
 // (lift.rs:85-126).
 __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ P, const float4* __restrict__ V,
                                                      LiftState* __restrict__ ls, const BodyState* __restrict__ bodies,
@@ -727,8 +728,8 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
 // One block per body, after the host has set area = volume^(2/3) / wetted:
 // buoyancy, then the drag on every probe under, then the waterplane's
 // damping on every probe on the waterline, as impulses (lift.rs:127-150).
-// The block works each probe's term out in parallel; lane 0 sums them in the
-// reference's order.
+// The block works each probe's term out in parallel; lanes 0-5 sum them, one
+// force or torque component a lane, each in the reference's order.
 __global__ __launch_bounds__(256) void k_lift_force(LiftState* __restrict__ ls, const BodyState* __restrict__ bodies,
                                                     const float4* __restrict__ arm, const float4* __restrict__ vel,
                                                     float rest, float gravity, float upx, float upy, float upz,

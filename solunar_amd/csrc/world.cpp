@@ -189,6 +189,11 @@ int32_t World::init_device() {
   format_ = solve_lds_ ? kCompact : kLists;
   if (fmt) format_ = std::min(std::max(std::atoi(fmt), (int)kMasks16), (int)kCompact);
   if (format_ == kCompact && !solve_lds_) format_ = kLists;  // compact lists need the tiled solve
+  // SOLUNAR_BODY_PAIR_SLACK (default 65536) is the smallest room for
+  // body-point pair records and the slack a step adds to the last step's
+  // count; a small value makes coupled steps outgrow their room (tests)
+  const char* slack = std::getenv("SOLUNAR_BODY_PAIR_SLACK");
+  if (slack) pair_slack_ = (uint32_t)std::min<long>(std::max(std::atol(slack), 0L), 1L << 30);
   if (solve_grid_ <= 0) {
     detail_ = "occupancy query for the persistent solve failed";
     return SFL_DEVICE_ERROR;
@@ -343,6 +348,19 @@ int32_t World::set_domain(const sfl_vec3* low, const sfl_vec3* high) {
   has_domain_ = true;
   domain_ = FlowBox{{low->x, low->y, low->z}, {high->x, high->y, high->z}};
   return SFL_OK;
+}
+
+bool World::domain(sfl_vec3* low, sfl_vec3* high) const {
+  if (!has_domain_) return false;
+  if (low) *low = sfl_vec3{domain_.lo[0], domain_.lo[1], domain_.lo[2]};
+  if (high) *high = sfl_vec3{domain_.hi[0], domain_.hi[1], domain_.hi[2]};
+  return true;
+}
+
+sfl_vec3 World::kernel_gradient_at(sfl_vec3 o) const {
+  sfl_vec3 g{0.0f, 0.0f, 0.0f};
+  kernel_gradient(o.x, o.y, o.z, dot3(o.x, o.y, o.z, o.x, o.y, o.z), radius_, scale_, g.x, g.y, g.z);
+  return g;
 }
 
 // ---- flows (src/fluid/flow.rs) ----------------------------------------------
@@ -1231,7 +1249,7 @@ int32_t World::wcsph(float dt, sfl_report& r) {
     SFL_HIP(pair_offsets_b_.reserve(n_ + 1, 0, stream_), "hipMalloc(body pairs)");
     SFL_HIP(pair_tiles_b_.reserve(n_ / 2048 + 2, 0, stream_), "hipMalloc(body pairs)");
     SFL_HIP(pair_total_b_.reserve(1, 0, stream_), "hipMalloc(body pairs)");
-    st = ensure_pair_room(std::max<uint64_t>(body_pair_cap_, std::max<uint64_t>((uint64_t)movers_m_ * 16, 65536)));
+    st = ensure_pair_room(std::max<uint64_t>(body_pair_cap_, std::max<uint64_t>((uint64_t)movers_m_ * 16, pair_slack_)));
     if (st != SFL_OK) return st;
   }
   bool retry = false;
@@ -1254,7 +1272,7 @@ int32_t World::wcsph(float dt, sfl_report& r) {
     const Control& c = *ctl_host_;
     if (movers_m_ && !(c.flags & (kFlagOverflow | kFlagPairRoom))) {
       // the next step sorts what this one used at most, and a quarter more
-      const uint64_t want = (uint64_t)c.pairs_used + c.pairs_used / 4 + 65536;
+      const uint64_t want = (uint64_t)c.pairs_used + c.pairs_used / 4 + pair_slack_;
       body_pair_use_ = (uint32_t)std::min<uint64_t>(want, body_pair_cap_);
     }
     if (c.flags & kFlagMoverBox) {
@@ -1589,6 +1607,26 @@ int32_t World::read_neighbors(int kind, size_t start_cap, uint32_t* start, size_
   SFL_HIP(hipMemcpyAsync(start, s.ptr, (n_ + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "read lists");
   if (len) SFL_HIP(hipMemcpyAsync(items, it.ptr, len * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_), "read lists");
   SFL_HIP(hipStreamSynchronize(stream_), "read lists");
+  return SFL_OK;
+}
+
+int32_t World::neighbors_of(size_t index, size_t cap, uint32_t* out, size_t* len) {
+  DeviceGuard g(device_);
+  if (!lists_valid_ || index >= n_) return SFL_OUT_OF_RANGE;
+  int32_t st = emit_lists();
+  if (st != SFL_OK) return st;
+  uint32_t range[2] = {0, 0};
+  SFL_HIP(hipMemcpyAsync(range, fstart_.ptr + index, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+          "read list");
+  SFL_HIP(hipStreamSynchronize(stream_), "read list");
+  const size_t count = range[1] - range[0];
+  if (len) *len = count;
+  if (cap < count) return SFL_OUT_OF_RANGE;
+  if (count) {
+    SFL_HIP(hipMemcpyAsync(out, fitems_.ptr + range[0], count * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+            "read list");
+    SFL_HIP(hipStreamSynchronize(stream_), "read list");
+  }
   return SFL_OK;
 }
 

@@ -148,6 +148,16 @@ class World {
   int32_t read_solve_log(size_t cap, double* average, uint32_t* loaded, size_t* len);
 
   const sfl_settings& settings() const { return settings_; }
+  // domain (src/fluid/mod.rs:446-448): false when none is set
+  bool domain(sfl_vec3* low, sfl_vec3* high) const;
+  // kernel (mod.rs:376-378) with Kernel::value / slope / gradient
+  // (mod.rs:140-164), in the device's arithmetic
+  float kernel_value_at(float distance) const { return kernel_value(distance, radius_, scale_); }
+  float kernel_slope_at(float distance) const { return kernel_slope(distance, radius_, scale_); }
+  sfl_vec3 kernel_gradient_at(sfl_vec3 offset) const;
+  // neighbors_of (mod.rs:612-614): particle `index`'s fluid list of the last
+  // build
+  int32_t neighbors_of(size_t index, size_t cap, uint32_t* out, size_t* len);
   float mass() const { return mass_; }
   size_t len() const { return n_; }
   size_t wall_len() const { return wall_owner_.size(); }
@@ -339,6 +349,7 @@ class World {
   DeviceBuffer<uint8_t> pair_temp_;
   uint32_t body_pair_cap_ = 0;  // records the buffers hold
   uint32_t body_pair_use_ = 0;  // records a substep sorts: the last step's most, and a quarter
+  uint32_t pair_slack_ = 65536;  // smallest room, and the slack over the last step's most
   int32_t ensure_pair_room(uint64_t cap);
   PairRoom pair_room() const;
   DeviceBuffer<BodyExtent> extent_;

@@ -11,8 +11,11 @@ first use.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 from typing import Optional, Sequence, Tuple
 
 import numpy as np
@@ -29,6 +32,34 @@ PHASES = ("bounds", "sort", "reorder", "density", "force", "solve", "viscous", "
 
 
 SNAPSHOT_REFUSED = 102  # SFL_SNAPSHOT_REFUSED
+
+# Every live handle, closed by an exit hook before the interpreter tears down:
+# freeing device memory after the HIP runtime (or a profiler attached to it)
+# has finalised crashes the process at exit. Tapes go first (they read their
+# world's arrays), then batches, then worlds.
+_live = {"tape": weakref.WeakSet(), "batch": weakref.WeakSet(), "world": weakref.WeakSet()}
+
+
+@atexit.register
+def close_all() -> None:
+    for kind in ("tape", "batch", "world"):
+        for obj in list(_live[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+def _release(obj, destroy) -> None:
+    """Frees obj's handle once; skipped while the interpreter finalises,
+    when the exit hook has already run and the runtime may be gone."""
+    h = getattr(obj, "_h", None)
+    if not h:
+        return
+    obj._h = None
+    if sys.is_finalizing():
+        return
+    destroy(h)
 
 
 def settings_of(c: CSettings) -> FluidSettings:
@@ -64,6 +95,11 @@ def library() -> ctypes.CDLL:
         "sfl_fill_box": (i32, [P, CVec3, CVec3, ctypes.POINTER(sz)]),
         "sfl_fill_box_jittered": (i32, [P, CVec3, CVec3, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(sz)]),
         "sfl_set_domain": (i32, [P, ctypes.POINTER(CVec3), ctypes.POINTER(CVec3)]),
+        "sfl_get_domain": (i32, [P, ctypes.POINTER(i32), ctypes.POINTER(CVec3), ctypes.POINTER(CVec3)]),
+        "sfl_kernel_value": (ctypes.c_float, [P, ctypes.c_float]),
+        "sfl_kernel_slope": (ctypes.c_float, [P, ctypes.c_float]),
+        "sfl_kernel_gradient": (i32, [P, CVec3, ctypes.POINTER(CVec3)]),
+        "sfl_neighbors_of": (i32, [P, sz, sz, u32p, ctypes.POINTER(sz)]),
         "sfl_add_emitter": (i32, [P, CVec3, CVec3, CVec3, u32p]),
         "sfl_add_drain": (i32, [P, CVec3, CVec3, u32p]),
         "sfl_remove_flow": (i32, [P, ctypes.c_uint32, ctypes.POINTER(i32)]),
@@ -161,11 +197,10 @@ class FluidWorld:
         if st:
             raise PhysicsError(st, self._lib.sfl_error_detail(None).decode())
         self._h = handle
+        _live["world"].add(self)
 
     def close(self) -> None:
-        if getattr(self, "_h", None):
-            self._lib.sfl_destroy(self._h)
-            self._h = None
+        _release(self, self._lib.sfl_destroy)
 
     def __del__(self):
         try:
@@ -228,6 +263,24 @@ class FluidWorld:
             lo, hi = CVec3.of(domain[0]), CVec3.of(domain[1])
             self._check(self._lib.sfl_set_domain(self._h, ctypes.byref(lo), ctypes.byref(hi)))
 
+    def domain(self):
+        """FluidWorld::domain (mod.rs:446-448): (low, high) or None."""
+        has, lo, hi = ctypes.c_int32(), CVec3(), CVec3()
+        self._check(self._lib.sfl_get_domain(self._h, ctypes.byref(has), ctypes.byref(lo), ctypes.byref(hi)))
+        return ((lo.x, lo.y, lo.z), (hi.x, hi.y, hi.z)) if has.value else None
+
+    # ---- the kernel (mod.rs:126-165, 376-378) ----
+    def kernel_value(self, distance: float) -> float:
+        return float(self._lib.sfl_kernel_value(self._h, distance))
+
+    def kernel_slope(self, distance: float) -> float:
+        return float(self._lib.sfl_kernel_slope(self._h, distance))
+
+    def kernel_gradient(self, offset) -> Tuple[float, float, float]:
+        out = CVec3()
+        self._check(self._lib.sfl_kernel_gradient(self._h, CVec3.of(offset), ctypes.byref(out)))
+        return (out.x, out.y, out.z)
+
     # ---- flows (flow.rs) ----
     def add_emitter(self, low, high, velocity) -> int:
         out = ctypes.c_uint32()
@@ -276,6 +329,7 @@ class FluidWorld:
         w = cls.__new__(cls)
         w._lib = lib
         w._h = handle
+        _live["world"].add(w)
         c = CSettings()
         lib.sfl_get_settings(handle, ctypes.byref(c))
         w._settings = settings_of(c)
@@ -444,8 +498,14 @@ class FluidWorld:
         return self._lists(1)
 
     def neighbors_of(self, index: int) -> np.ndarray:
-        start, items = self.neighbors()
-        return items[start[index]:start[index + 1]]
+        """neighbors_of (mod.rs:612-614): one particle's fluid list."""
+        n = ctypes.c_size_t()
+        st = self._lib.sfl_neighbors_of(self._h, index, 0, None, ctypes.byref(n))
+        if st and n.value == 0:
+            self._check(st)
+        out = np.zeros(n.value, np.uint32)
+        self._check(self._lib.sfl_neighbors_of(self._h, index, n.value, _ptr(out, ctypes.c_uint32), ctypes.byref(n)))
+        return out
 
     def state_hash(self) -> int:
         out = ctypes.c_uint64()
@@ -504,11 +564,10 @@ class Tape:
             raise PhysicsError(st, self._lib.sfl_error_detail(world._h).decode())
         self._h = h
         self.n = len(world)
+        _live["tape"].add(self)
 
     def close(self) -> None:
-        if getattr(self, "_h", None):
-            self._lib.sfl_tape_destroy(self._h)
-            self._h = None
+        _release(self, self._lib.sfl_tape_destroy)
 
     def __del__(self):
         try:
@@ -552,11 +611,10 @@ class FluidBatch:
         if st:
             raise PhysicsError(st, self._lib.sfl_batch_error_detail(None).decode())
         self._h = h
+        _live["batch"].add(self)
 
     def close(self) -> None:
-        if getattr(self, "_h", None):
-            self._lib.sfl_batch_destroy(self._h)
-            self._h = None
+        _release(self, self._lib.sfl_batch_destroy)
 
     def __del__(self):
         try:

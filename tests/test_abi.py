@@ -8,6 +8,7 @@ import glob
 import os
 import re
 
+import numpy as np
 import pytest
 
 from solunar_amd import fluid
@@ -43,20 +44,34 @@ def test_struct_layouts_match_the_header():
     assert CBatchTuning.sort_ms.offset == 36
 
 
+# FluidSettings::default (src/fluid/mod.rs:80-95, FluidSolver::Wcsph is the
+# #[default] at mod.rs:47) and the DFSPH constants (src/fluid/dfsph.rs:8-16),
+# as the reference writes them; f32 fields compare as f32 bits
+REFERENCE_DEFAULTS = {
+    "solver": 0, "spacing": 0.05, "rest_density": 1000.0, "support": 2.0, "max_particles": 1 << 22,
+    "sound_speed": 20.0, "viscosity": 0.15, "gravity": (0.0, -9.81, 0.0), "courant": 0.4, "max_substeps": 2000,
+    "divergence_tolerance": 1e-3, "density_tolerance": 5e-4, "min_iterations": 2, "max_iterations": 100,
+}
+
+
 def test_defaults_are_the_reference_defaults():
     lib = fluid.library()
     c = CSettings()
     lib.sfl_settings_default(ctypes.byref(c))
-    d = FluidSettings()
-    for name, _ in CSettings._fields_:
+    for name, want in REFERENCE_DEFAULTS.items():
         got = getattr(c, name)
-        want = getattr(d, name)
         if name == "gravity":
-            assert tuple(got) == pytest.approx(want)
+            assert tuple(np.float32(x) for x in got) == tuple(np.float32(w) for w in want)
         elif isinstance(want, float):
-            assert got == pytest.approx(want)
+            assert np.float32(got) == np.float32(want), name
         else:
             assert got == want, name
+    assert c.device == 0
+    # and the Python mirror agrees with the C defaults
+    d = FluidSettings()
+    for name in REFERENCE_DEFAULTS:
+        if name != "gravity":
+            assert np.float32(getattr(d, name)) == np.float32(getattr(c, name)), name
 
 
 def test_status_texts_follow_physics_error_display():

@@ -159,16 +159,22 @@ def test_the_64k_rollout_gradients():
     fd_a = (L(ap, v0) - L(am, v0)) / (ap - am)
     d = np.random.default_rng(7).normal(size=v0.shape)
     d /= np.linalg.norm(d)
-    eps = 1e-5
-    fd_v = (L(0.05, v0 + eps * d) - L(0.05, v0 - eps * d)) / (2 * eps)
     err_fa = abs(ga_own - fd_a) / abs(fd_a)
-    err_fv64 = abs(float(np.sum(gv64 * d)) - fd_v) / abs(fd_v)
-    err_fv = abs(float(np.sum(gv_own * d)) - fd_v) / abs(fd_v)
-    print(f"FP64 central differences: device alpha {err_fa:.2e}; FP64 adjoint v0 along d {err_fv64:.2e}; "
-          f"device v0 along d {err_fv:.2e}", flush=True)
-    assert err_fa < 1e-3 and err_fv64 < 1e-3
-    # the device's velocity gradient is that of the f32 rollout; against the
-    # FP64 rollout's difference quotient it carries the trajectory split
+    for eps in (1e-3, 1e-5):  # the configured step, and one inside the kinks' spacing
+        fd_v = (L(0.05, v0 + eps * d) - L(0.05, v0 - eps * d)) / (2 * eps)
+        err_fv64 = abs(float(np.sum(gv64 * d)) - fd_v) / abs(fd_v)
+        err_fv = abs(float(np.sum(gv_own * d)) - fd_v) / abs(fd_v)
+        print(f"FP64 central differences, eps {eps:g}: device alpha {err_fa:.2e}; FP64 adjoint v0 along d "
+              f"{err_fv64:.2e}; device v0 along d {err_fv:.2e}", flush=True)
+    assert err_fa < 1e-3
+    # at eps 1e-5 the FP64 adjoint meets the 1e-3 bar; at 1e-3 the quotient
+    # steps over the viscosity's and the equation of state's kinks and has
+    # not converged for either gradient
+    assert err_fv64 < 1e-3
+    # Known shortfall, not a pass at 1e-3: the device's velocity gradient is
+    # the exact gradient of the f32 rollout (3e-6 from the FP64 reverse pass
+    # of that rollout, above); against the FP64 rollout's quotient it carries
+    # the split between the two trajectories (about 1.5e-2)
     assert err_fv < 5e-2
 
 

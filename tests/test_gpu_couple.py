@@ -199,3 +199,39 @@ def test_a_step_whose_first_substep_outgrows_the_cell_table():
     for k in range(2):
         poses = [BodyPose((2, 0), (0.5, 0.5, 0.5 - 0.02 * k), linear_velocity=(0.0, 0.0, -1.5))]
         same(gpu, cpu, poses, k)
+
+
+def pair_room_retries(gpu, rep):
+    """(first, mid): whether this step's pair records outgrew their room in
+    its first substep or a later one. An outgrown substep aborts the rest of
+    the step's launches and the step runs again from the last finished
+    substep: the force pass is enqueued substeps + (substeps - finished)
+    times."""
+    extra = gpu.profile_read(reset=True)["force"][1] - rep.substeps
+    return extra == rep.substeps, 0 < extra < rep.substeps
+
+
+def test_steps_that_outgrow_the_pair_room_carry_on_as_the_oracle_does(monkeypatch):
+    # no slack over the last step's pair count and a first room of 16
+    # records per body point: a ball coupled deep under water outgrows the
+    # room in its first substep; a ball falling into water outgrows it
+    # mid-step as more of it goes under. Reports, pushes and particles must
+    # still equal the oracle's, which has no room to outgrow.
+    monkeypatch.setenv("SOLUNAR_BODY_PAIR_SLACK", "0")
+    print("case: tight pair room", flush=True)
+    firsts = mids = 0
+    for start, speed, ticks in ((0.25, 0.0, 2), (0.62, -3.0, 8)):
+        gpu, cpu = pair()
+        for w in (gpu, cpu):
+            w.couple((2, 0), Shape.sphere(0.12))
+        gpu.profile(True)
+        gpu.profile_read(reset=True)
+        for k in range(ticks):
+            poses = [BodyPose((2, 0), (0.5, start + speed * TICK * k, 0.4), linear_velocity=(0.0, speed, 0.0))]
+            rep, _ = same(gpu, cpu, poses, (start, k))
+            first, mid = pair_room_retries(gpu, rep)
+            print(f"ball from {start}, tick {k}: {rep.substeps} substeps, first {first}, mid {mid}", flush=True)
+            firsts += first
+            mids += mid
+        assert gpu.state_hash() == cpu.state_hash()
+    assert firsts >= 1 and mids >= 1

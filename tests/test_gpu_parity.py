@@ -352,3 +352,32 @@ def test_compact_lists_widen_past_their_offsets():
     rg, rc = gpu.step(1e-4), cpu.step(1e-4)
     assert rg == rc, (rg, rc)
     assert gpu.state_hash() == cpu.state_hash()
+
+
+def test_the_kernel_domain_and_one_particles_list_read_back_as_the_oracle_has_them():
+    # FluidWorld::kernel with value / slope / gradient (mod.rs:140-164,
+    # 376-378), domain (mod.rs:446-448), neighbors_of (mod.rs:612-614)
+    s = scenes.settings_dam_small()
+    gpu, cpu = pair(s)
+    for d in np.linspace(0.0, 0.05, 101, dtype=np.float32):
+        assert np.float32(gpu.kernel_value(float(d))) == np.float32(cpu.kernel_value(float(d)))
+        assert np.float32(gpu.kernel_slope(float(d))) == np.float32(cpu.kernel_slope(float(d)))
+    for off in [(0.0, 0.0, 0.0), (0.01, -0.02, 0.005), (0.03, 0.0, 0.0), (0.05, 0.0, 0.0)]:
+        o = np.asarray(off, np.float32)
+        d = np.sqrt(np.float32(o[0] * o[0] + o[1] * o[1]) + o[2] * o[2])
+        want = o * (np.float32(cpu.kernel_slope(float(d))) / d) if d > 0 else np.zeros(3, np.float32)
+        assert np.array_equal(np.asarray(gpu.kernel_gradient(off), np.float32), want.astype(np.float32))
+    assert gpu.domain() is None
+    box = ((0.0, -1.0, 0.0), (1.6, 2.0, 0.8))
+    gpu.set_domain(box)
+    got = gpu.domain()
+    assert [tuple(np.float32(c) for c in v) for v in got] == [tuple(np.float32(c) for c in v) for v in box]
+    gpu.set_domain(None)
+    assert gpu.domain() is None
+    for w in (gpu, cpu):
+        scenes.dam_break(w)
+        w.update_neighbors()
+    for i in (0, 1, 777, len(gpu) - 1):
+        assert np.array_equal(gpu.neighbors_of(i), cpu.neighbors_of(i)), i
+    with pytest.raises(PhysicsError):
+        gpu.neighbors_of(len(gpu))
