@@ -28,6 +28,7 @@
 // it goes, so after the pass's barrier every wave folds the tile sums itself
 // and all blocks take the same decision with no further barrier.
 #include <algorithm>
+#include <cstdlib>
 
 #include "device.hpp"
 #include "launch.hpp"
@@ -1742,20 +1743,36 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   if (want < blocks) blocks = want;
   blocks = blocks / kXcds * kXcds;
   if (blocks == 0) blocks = kXcds;
-  // A plain launch: the grid is sized to fit the device at once (two blocks
-  // per CU at most), so every block is resident for the barrier; it also
-  // keeps the launch capturable in a graph.
-  if (tiled && prm.format == kCompact)
-    hipLaunchKernelGGL((k_dfsph_solve<kCompact, true>), dim3(blocks), dim3(kSolveThreads), tile_bytes, s, a);
-  else if (tiled)
-    hipLaunchKernelGGL((k_dfsph_solve<kLists, true>), dim3(blocks), dim3(kSolveThreads), tile_bytes, s, a);
-  else if (prm.format == kMasks16)
-    hipLaunchKernelGGL((k_dfsph_solve<kMasks16, false>), dim3(blocks), dim3(kSolveThreads), 0, s, a);
-  else if (prm.format == kMasks32)
-    hipLaunchKernelGGL((k_dfsph_solve<kMasks32, false>), dim3(blocks), dim3(kSolveThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_dfsph_solve<kLists, false>), dim3(blocks), dim3(kSolveThreads), 0, s, a);
-  return hipGetLastError();
+  // A cooperative launch: the runtime refuses up front a grid whose blocks
+  // cannot all be resident at once, which the grid barrier needs (the grid
+  // is sized by the occupancy query, so that refusal means the device is
+  // not the one the query saw); the barrier's 2 s bound stays as the last
+  // guard.
+  const void* fn;
+  size_t lds = 0;
+  if (tiled && prm.format == kCompact) {
+    fn = reinterpret_cast<const void*>(k_dfsph_solve<kCompact, true>);
+    lds = tile_bytes;
+  } else if (tiled) {
+    fn = reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>);
+    lds = tile_bytes;
+  } else if (prm.format == kMasks16) {
+    fn = reinterpret_cast<const void*>(k_dfsph_solve<kMasks16, false>);
+  } else if (prm.format == kMasks32) {
+    fn = reinterpret_cast<const void*>(k_dfsph_solve<kMasks32, false>);
+  } else {
+    fn = reinterpret_cast<const void*>(k_dfsph_solve<kLists, false>);
+  }
+  // SOLUNAR_SOLVE_COOPERATIVE=0 launches plainly instead (same grid, same
+  // kernel), for tools that do not follow cooperative launches
+  static const bool cooperative = [] {
+    const char* v = std::getenv("SOLUNAR_SOLVE_COOPERATIVE");
+    return !(v && std::atoi(v) == 0);
+  }();
+  void* args[] = {&a};
+  if (cooperative)
+    return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(kSolveThreads), args, (unsigned int)lds, s);
+  return hipLaunchKernel(fn, dim3(blocks), dim3(kSolveThreads), args, lds, s);
 }
 
 void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,
