@@ -10,13 +10,17 @@ FluidWorld::step over all particles, state resident in HBM.
 Printed: one JSON line with particle-steps/s, the HBM roofline of the
 dominant kernel (algorithmic bytes from HIP events on the world's stream over
 the timed region) and the CPU oracle timed on this host on a bounded sample.
+After the timed steps (outside `value`), the same line carries the whole
+200-step rollout of a fresh dam_1m world (`rollout_200`) and the 64k WCSPH
+adjoint's reverse:forward time ratio (`adjoint_64k`); --no-extras skips
+them.
 
 With --workload batch_256: 256 WCSPH dam breaks (spacing 0.02 m, block sides
 uniform in [0.2, 0.8] m, 2,000-64,000 particles each, 3.86M in all) stepped
 as one batch, `steps` steps of 1e-4 s in one graph launch, next to the same
 256 worlds stepped one by one on the single-world path.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S] [--no-cpu] [--no-extras]
                        [--workload dam_1m|dam_16m|batch_256|adjoint_64k|flows_1m|snapshot_1m]
 """
 from __future__ import annotations
@@ -208,12 +212,12 @@ def run_batch(args) -> dict:
     }
 
 
-def run_adjoint(args) -> dict:
-    """Differentiable WCSPH: 64,000 particles, 200 steps of 1e-4 s, loss =
-    mean |x_T - x_T*|^2 against the viscosity-0.08 rollout. Times the plain
-    forward rollout, the recording forward (checkpoints every 20 steps) and
-    the reverse pass (segment replay + adjoint kernels); the metric is
-    reverse / forward."""
+def adjoint_times(repeats: int) -> dict:
+    """The 64,000-particle WCSPH rollout of 200 steps of 1e-4 s, loss =
+    mean |x_T - x_T*|^2 against the viscosity-0.08 rollout: seconds of the
+    plain forward rollout, of the recording forward (checkpoints every 20
+    steps) and of the reverse pass (segment replay + adjoint kernels, best of
+    `repeats` after one untimed), with the tape's footprint."""
     import numpy as np
 
     from solunar_amd import scenes
@@ -255,10 +259,10 @@ def run_adjoint(args) -> dict:
     xT[w.ids()] = w.positions()
     seed = (2.0 * (xT.astype(np.float64) - target) / n).astype(np.float32)
     loss = float(np.sum((xT.astype(np.float64) - target) ** 2) / n)
-    log(f"record {record * 1e3:.1f} ms; reverse pass (warm-up, then {max(args.steps, 1)} timed)")
+    log(f"record {record * 1e3:.1f} ms; reverse pass (warm-up, then {max(repeats, 1)} timed)")
     tape.adjoint(seed)
     runs = []
-    for _ in range(max(args.steps, 1)):
+    for _ in range(max(repeats, 1)):
         t0 = time.perf_counter()
         gx, gv, ga = tape.adjoint(seed)
         runs.append(time.perf_counter() - t0)
@@ -267,6 +271,20 @@ def run_adjoint(args) -> dict:
     tape.close()
     w.close()
     log(f"reverse {reverse * 1e3:.1f} ms (best of {len(runs)})")
+    return {"particles": n, "steps": steps, "checkpoint_every": every, "forward_s": forward, "record_s": record,
+            "reverse_s": reverse, "reverse_runs": len(runs), "loss": loss, "dloss_dviscosity": ga,
+            "tape_bytes": memory}
+
+
+def run_adjoint(args) -> dict:
+    """Differentiable WCSPH: 64,000 particles, 200 steps of 1e-4 s, loss =
+    mean |x_T - x_T*|^2 against the viscosity-0.08 rollout. Times the plain
+    forward rollout, the recording forward (checkpoints every 20 steps) and
+    the reverse pass (segment replay + adjoint kernels); the metric is
+    reverse / forward."""
+    a = adjoint_times(args.steps)
+    forward, record, reverse = a["forward_s"], a["record_s"], a["reverse_s"]
+    n, steps = a["particles"], a["steps"]
     return {
         "metric": "adjoint:forward time ratio, 64k WCSPH rollout",
         "value": reverse / forward,
@@ -284,12 +302,11 @@ def run_adjoint(args) -> dict:
                                "0.8 m container, v0 ~ N(0, 0.1 m/s), 200 steps of 1e-4 s, checkpoints every 20 "
                                "steps, loss = mean |x_T - x_T*|^2 against viscosity 0.08, gradients in v0 and alpha",
                    "forward_ms": forward * 1e3, "record_ms": record * 1e3, "reverse_ms": reverse * 1e3,
-                   "reverse_runs_ms": [r * 1e3 for r in runs],
+                   "reverse_runs": a["reverse_runs"],
                    "record_plus_reverse_over_forward": (record + reverse) / forward,
                    "forward_particle_steps_per_s": n * steps / forward,
                    "reverse_particle_steps_per_s": n * steps / reverse,
-                   "loss": loss, "dL_dalpha": ga, "dL_dv0_norm": float(np.linalg.norm(gv)),
-                   "tape_device_bytes": memory},
+                   "loss": a["loss"], "dL_dalpha": a["dloss_dviscosity"], "tape_device_bytes": a["tape_bytes"]},
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -585,6 +602,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="dam_1m: skip the 200-step rollout and the adjoint ratio after the timed steps")
     ap.add_argument("--workload", choices=["dam_1m", "dam_16m", "batch_256", "adjoint_64k", "flows_1m",
                                            "snapshot_1m", "bodies_1m", "trench_720k"],
                     default="dam_1m",
@@ -720,6 +739,15 @@ def main() -> int:
                           "unit": "GB/s", "bytes": step_bytes},
         "phase_ms": {p: round(v[0], 3) for p, v in phases.items()},
     }
+    if not large and not args.no_extras:
+        out["rollout_200"] = full_rollout(settings)
+        a = adjoint_times(2)
+        out["adjoint_64k"] = {
+            "reverse_over_forward": a["reverse_s"] / a["forward_s"],
+            "record_and_reverse_over_forward": (a["record_s"] + a["reverse_s"]) / a["forward_s"],
+            "forward_ms": a["forward_s"] * 1e3, "record_ms": a["record_s"] * 1e3, "reverse_ms": a["reverse_s"] * 1e3,
+            "particles": a["particles"], "steps": a["steps"], "checkpoint_every": a["checkpoint_every"],
+        }
     if not args.no_cpu and not large:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps, args.warmup)
@@ -727,6 +755,36 @@ def main() -> int:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
     return 0
+
+
+def full_rollout(settings) -> dict:
+    """The whole 200-step dam_1m rollout from a fresh world, steps 0-199
+    timed once (the first step's buffer set-up included), and the rate
+    without step 0."""
+    from solunar_amd import scenes
+    from solunar_amd.fluid import FluidWorld
+
+    w = FluidWorld(settings)
+    n = scenes.dam_break(w)
+    log("rollout: 200 steps of a fresh dam_1m world")
+    w.synchronize()
+    t0 = time.perf_counter()
+    w.step(DT)
+    w.synchronize()
+    t1 = time.perf_counter()
+    substeps = iterations = 0
+    for k in range(1, 200):
+        r = w.step(DT)
+        substeps += r.substeps
+        iterations += r.iterations
+        if k % 50 == 49:
+            log(f"rollout step {k + 1}/200")
+    w.synchronize()
+    t2 = time.perf_counter()
+    w.close()
+    return {"steps": 200, "seconds": t2 - t0, "value": n * 200 / (t2 - t0), "unit": "particle-steps/s",
+            "first_step_ms": (t1 - t0) * 1e3, "value_steps_1_to_199": n * 199 / (t2 - t1),
+            "substeps_1_to_199": substeps, "jacobi_iterations_1_to_199": iterations}
 
 
 if __name__ == "__main__":
