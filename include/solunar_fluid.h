@@ -325,7 +325,13 @@ int32_t sfl_read_wall_positions(const sfl_world* world, size_t cap,
                                 float* xyz);
 
 /* FluidWorld::step (src/fluid/step.rs:58-90). The dispatcher argument of the
- * reference has no counterpart: the device path has one schedule. */
+ * reference has no counterpart: the device path has one schedule.
+ * Like the reference's Grid (src/fluid/mod.rs:214-236), the cell table is
+ * dense while the cells' box is small enough (at most max(64 n + 4096, 2^26)
+ * cells) and searches the occupied cells past that; either gives the same
+ * neighbours in the same order. A DFSPH world keeps a dense table and
+ * returns SFL_OUT_OF_RANGE, its state untouched, when its water spreads over
+ * more than 2^31 cells. */
 int32_t sfl_step(sfl_world* world, float dt, sfl_report* report_out);
 
 /* No counterpart in the reference: sfl_step for callers that cannot
@@ -461,7 +467,9 @@ typedef struct sfl_batch_tuning {
 
 /* Copies the worlds' particles, walls and containers. Every world must be
  * WCSPH, on one device, with equal settings apart from max_particles. The
- * worlds are not changed. SFL_INVALID_SETTINGS otherwise. */
+ * worlds are not changed. SFL_INVALID_SETTINGS otherwise; SFL_UNSUPPORTED
+ * for a world whose walls or water went onto a searched cell table (see
+ * sfl_step): the batch keeps dense tables. */
 int32_t sfl_batch_create(sfl_world* const* worlds, size_t count,
                          sfl_batch** out);
 void sfl_batch_destroy(sfl_batch* batch);
@@ -517,7 +525,9 @@ int32_t sfl_batch_synchronize(sfl_batch* batch);
  * sets, substep counts and clamp decisions are those of the recorded run.
  * The world must outlive the tape and must not change between recording
  * and the reverse pass (SFL_OUT_OF_RANGE otherwise); its ids must be
- * 0..len-1 (a world that only ever gained particles).
+ * 0..len-1 (a world that only ever gained particles). Recording stops with
+ * SFL_UNSUPPORTED when the water goes onto a searched cell table (see
+ * sfl_step): segments replay on dense tables.
  */
 typedef struct sfl_tape sfl_tape;
 

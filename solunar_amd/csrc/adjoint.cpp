@@ -101,6 +101,10 @@ int32_t Tape::record(World* w, float dt, uint32_t steps, uint32_t every, Tape** 
       why = w->detail_;
       return st;
     }
+    if (w->searched_) {
+      why = "the tape replays its segments on dense cell tables; the water spread onto a searched one";
+      return SFL_UNSUPPORTED;
+    }
     t->substeps_.push_back(r.substeps);
     t->small_.push_back(dt / (float)r.substeps);  // wcsph_steps (step.rs:131)
   }
@@ -169,6 +173,7 @@ int32_t Tape::replay(uint32_t c, uint32_t* count) {
       for (uint32_t j = 0; j < substeps_[k]; ++j, ++slot) {
         Slot& sl = *slots_[slot];
         prm.cell_start = sl.cells.ptr;
+        prm.cell_keys = nullptr;  // the tape's tables are dense
         launch_substep_begin(s, ctl_);
         launch_bounds(s, ra_p_.ptr, n, w.radius_, ctl_);
         launch_grid_setup(s, ctl_, n, cell_cap_);
@@ -258,6 +263,8 @@ int32_t Tape::reverse(uint32_t c, uint32_t count) {
     a.box = boxes_.ptr + slot;
     a.walls = prm.walls;
     a.wall_start = prm.wall_start;
+    a.wall_keys = prm.wall_keys;
+    a.wall_keyed = prm.wall_keyed;
     a.wall_box = prm.wall_box;
     a.wall_cells = prm.wall_cells;
     launch_adj_advance(s, a, xb_.ptr, vb_.ptr, xb2_.ptr, vb2_.ptr, ab_.ptr);

@@ -33,6 +33,8 @@ struct AdjointArgs {
   const CellBox* box;
   const float4* walls;
   const uint32_t* wall_start;
+  const uint64_t* wall_keys;  // searched wall table, or null
+  uint32_t wall_keyed;
   CellBox wall_box;
   uint32_t wall_cells;
 };

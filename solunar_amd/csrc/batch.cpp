@@ -82,6 +82,10 @@ int32_t Batch::create(World* const* worlds, size_t count, Batch** out, std::stri
       why = "a batch steps worlds without flows, a domain or terrain";
       return SFL_UNSUPPORTED;
     }
+    if (w.wall_keyed_ || w.searched_) {
+      why = "a batch keeps dense cell tables: a world's walls or water are too spread out for one";
+      return SFL_UNSUPPORTED;
+    }
     total += w.n_;
   }
   if (total >= (uint64_t(1) << 31)) {

@@ -414,7 +414,8 @@ int32_t World::fastest_near_water(float dt, float slow, float& most) {
   if (!q.empty()) {
     SFL_HIP(hipMemcpyAsync(wet_.ptr, q.data(), q.size() * sizeof(WetQuery), hipMemcpyHostToDevice, stream_),
             "body reach");
-    launch_extent_wet(stream_, cell_start_.ptr, ctl_, wet_.ptr, (uint32_t)q.size(), extent_.ptr);
+    launch_extent_wet(stream_, searched_ ? sstart_.ptr : cell_start_.ptr, searched_ ? cell_keys_.ptr : nullptr, ctl_,
+                      wet_.ptr, (uint32_t)q.size(), extent_.ptr);
     SFL_HIP(hipMemcpyAsync(ext.data(), extent_.ptr, nb * sizeof(BodyExtent), hipMemcpyDeviceToHost, stream_),
             "read body reach");
     SFL_HIP(hipStreamSynchronize(stream_), "body reach");

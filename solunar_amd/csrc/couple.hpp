@@ -92,7 +92,8 @@ hipError_t launch_pairs_fold(hipStream_t s, const PairRoom& room, BodyState* bod
                              const Control* ctl);
 void launch_extent(hipStream_t s, const float4* MS, const float4* MV, const BodyState* bodies, BodyExtent* e,
                    uint32_t nbodies);
-void launch_extent_wet(hipStream_t s, const uint32_t* start, const Control* ctl, const WetQuery* q, uint32_t nq,
+void launch_extent_wet(hipStream_t s, const uint32_t* start, const uint64_t* keys, const Control* ctl,
+                       const WetQuery* q, uint32_t nq,
                        BodyExtent* e);
 void launch_lift_box(hipStream_t s, const float4* probes, const BodyState* bodies, uint32_t nbodies,
                      uint32_t most_probes, float4* arm, float4* vel, LiftState* ls, float reach);

@@ -72,6 +72,8 @@ struct Control {
   uint32_t pad0;
   uint64_t need_volume;  // cells the box needed when it overflowed
   uint64_t need_pairs;   // body-point pair records a substep needed past their room
+  uint32_t cells;        // searched table: occupied cells of the last sort
+  uint32_t pad1;
   uint32_t pairs_used;   // most body-point pair records a substep of this step used
   uint32_t bad_index;    // smallest sorted index whose advance went non-finite
   uint32_t substeps_done;
@@ -227,13 +229,18 @@ struct StepParams {
   float half_spacing;
   int32_t has_bounds;
   float blo[3], bhi[3];
-  // walls (static, sorted by cell)
+  // walls (static, sorted by cell); the table is searched when wall_keys is
+  // set (wall_keyed occupied cells)
   const float4* walls;
   const uint32_t* wall_start;
+  const uint64_t* wall_keys;
+  uint32_t wall_keyed;
   CellBox wall_box;
   uint32_t wall_cells;  // 0: no walls
-  // fluid cell table (box read from the control block)
+  // fluid cell table (box read from the control block); searched when
+  // cell_keys is set (the occupied cells' slots, count in the control block)
   const uint32_t* cell_start;
+  const uint64_t* cell_keys;
   Control* ctl;
   // per-substep neighbour lists (DFSPH): slot k of particle i at
   // pairs[k * pair_stride + i] = (neighbour index, bits of slope(d)/d);
@@ -375,22 +382,82 @@ __device__ __forceinline__ float tait_pressure(float density, float rest, float 
 
 namespace sfl {
 
+// A cell table (Grid, src/fluid/mod.rs:168-236) over a box of cells, slot
+// ((x - lo.x) * span.y + (y - lo.y)) * span.z + (z - lo.z). Dense (keys
+// null): start[slot] .. start[slot + 1] is the cell's range. Searched:
+// keys[0, cells) are the occupied cells' slots in ascending order and
+// start[k] .. start[k + 1] the k-th one's range, found by binary search as
+// the reference's Grid::range does without its dense index. Either gives the
+// same ranges, so the same neighbours in the same order.
+struct GridView {
+  int32_t lo[3];
+  uint32_t span[3];
+  const uint32_t* start;
+  const uint64_t* keys;
+  uint32_t cells;
+};
+
+__device__ __forceinline__ GridView dense_grid(const int32_t lo[3], const uint32_t span[3],
+                                               const uint32_t* start) {
+  return GridView{{lo[0], lo[1], lo[2]}, {span[0], span[1], span[2]}, start, nullptr, 0u};
+}
+
+// The fluid's table of the last sort (box in the control block).
+__device__ __forceinline__ GridView fluid_grid(const StepParams& prm) {
+  const Control* ctl = prm.ctl;
+  return GridView{{ctl->lo[0], ctl->lo[1], ctl->lo[2]},
+                  {ctl->span[0], ctl->span[1], ctl->span[2]},
+                  prm.cell_start,
+                  prm.cell_keys,
+                  prm.cell_keys ? ctl->cells : 0u};
+}
+
+__device__ __forceinline__ GridView wall_grid(const StepParams& prm) {
+  return GridView{{prm.wall_box.lo[0], prm.wall_box.lo[1], prm.wall_box.lo[2]},
+                  {prm.wall_box.span[0], prm.wall_box.span[1], prm.wall_box.span[2]},
+                  prm.wall_start,
+                  prm.wall_keys,
+                  prm.wall_keyed};
+}
+
+// The range [s, e) of cell (x, y, z); false when the cell holds nothing.
+__device__ __forceinline__ bool grid_range(const GridView& g, int64_t x, int64_t y, int64_t z, uint32_t& s,
+                                           uint32_t& e) {
+  const int64_t c = cell_slot(x, y, z, g.lo, g.span);
+  if (c < 0) return false;
+  if (!g.keys) {
+    s = g.start[c];
+    e = g.start[c + 1];
+    return s < e;
+  }
+  uint32_t a = 0, b = g.cells;
+  while (a < b) {
+    const uint32_t mid = a + (b - a) / 2;
+    if (g.keys[mid] < (uint64_t)c)
+      a = mid + 1;
+    else
+      b = mid;
+  }
+  if (a >= g.cells || g.keys[a] != (uint64_t)c) return false;
+  s = g.start[a];
+  e = g.start[a + 1];
+  return true;
+}
+
 // Calls f(j, pj, ox, oy, oz, squared) for every particle j of `P` within
 // reach of `at`, o = at - P[j], in the reference's neighbour order: cells
 // dz, dy, dx (Grid::around, src/fluid/mod.rs:250-261), then the cell's range
 // in sorted order; membership |o|^2 < reach as in `near` (mod.rs:264-277),
 // whose |x_j - x_i|^2 has the same bits.
 template <class F>
-__device__ __forceinline__ void visit_near(const float4* __restrict__ P, const uint32_t* __restrict__ start,
-                                           const int32_t lo[3], const uint32_t span[3], float4 at, float radius,
+__device__ __forceinline__ void visit_near(const float4* __restrict__ P, const GridView& g, float4 at, float radius,
                                            float reach, F&& f) {
   int32_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
       for (int dx = -1; dx <= 1; ++dx) {
-        int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
-        if (c < 0) continue;
-        uint32_t s = start[c], e = start[c + 1];
+        uint32_t s, e;
+        if (!grid_range(g, (int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, s, e)) continue;
         for (uint32_t j = s; j < e; ++j) {
           float4 pj = P[j];
           float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;

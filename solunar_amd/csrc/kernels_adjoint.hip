@@ -24,6 +24,14 @@
 
 namespace sfl {
 
+__device__ __forceinline__ GridView adj_wall_grid(const AdjointArgs& a) {
+  return GridView{{a.wall_box.lo[0], a.wall_box.lo[1], a.wall_box.lo[2]},
+                  {a.wall_box.span[0], a.wall_box.span[1], a.wall_box.span[2]},
+                  a.wall_start,
+                  a.wall_keys,
+                  a.wall_keyed};
+}
+
 namespace {
 
 struct Kern {
@@ -135,7 +143,7 @@ __global__ __launch_bounds__(256) void k_adj_force(AdjointArgs a, const float4* 
     float3 ai = f3(Ab[i]);
     float3 xb = f3(Xb[i]), vb = f3(Vb[i]);
     float sbar = 0.0f, pbar = 0.0f, rbar = 0.0f, nsum = 0.0f;
-    visit_near(a.P, a.cell_start, box.lo, box.span, pi4, r, a.reach,
+    visit_near(a.P, dense_grid(box.lo, box.span, a.cell_start), pi4, r, a.reach,
                [&](uint32_t j, float4 pj, float ox, float oy, float oz, float sq) {
                  if (sq <= 0.0f) return;
                  float d = sqrtf(sq);
@@ -169,7 +177,7 @@ __global__ __launch_bounds__(256) void k_adj_force(AdjointArgs a, const float4* 
       tait(rho, a.rest, a.stiffness, p, dp);
       float S = si + p / (a.rest * a.rest);
       auto walls = [&](auto&& f) {
-        visit_near(a.walls, a.wall_start, a.wall_box.lo, a.wall_box.span, pi4, r, a.reach, f);
+        visit_near(a.walls, adj_wall_grid(a), pi4, r, a.reach, f);
       };
       // forward: away, normal, the velocity into the wall
       float3 away = make_float3(0.0f, 0.0f, 0.0f);
@@ -253,14 +261,14 @@ __global__ __launch_bounds__(256) void k_adj_density(AdjointArgs a, const float*
   float4 pi4 = a.P[i];
   float rbi = Rb[i];
   float3 xb = f3(Xb[i]);
-  visit_near(a.P, a.cell_start, box.lo, box.span, pi4, r, a.reach,
+  visit_near(a.P, dense_grid(box.lo, box.span, a.cell_start), pi4, r, a.reach,
              [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
                if (sq <= 0.0f) return;
                Kern k = kern(sqrtf(sq), r, sc);
                xb = xb + make_float3(ox, oy, oz) * (m * k.f * (rbi + Rb[j]));
              });
   if (a.wall_cells)
-    visit_near(a.walls, a.wall_start, a.wall_box.lo, a.wall_box.span, pi4, r, a.reach,
+    visit_near(a.walls, adj_wall_grid(a), pi4, r, a.reach,
                [&](uint32_t, float4, float ox, float oy, float oz, float sq) {
                  if (sq <= 0.0f) return;
                  Kern k = kern(sqrtf(sq), r, sc);

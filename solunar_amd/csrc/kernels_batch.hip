@@ -76,9 +76,12 @@ __device__ __forceinline__ StepParams scene_params(const BatchArgs& b, const Sce
   }
   p.walls = b.walls;
   p.wall_start = b.wall_start + d.wall_base;
+  p.wall_keys = nullptr;  // a batch keeps dense wall tables
+  p.wall_keyed = 0;
   p.wall_box = d.wall_box;
   p.wall_cells = d.wall_cells;
   p.cell_start = b.cell_start + st.slot_base;
+  p.cell_keys = nullptr;
   p.ctl = b.ctl;
   p.pairs = nullptr;
   p.pair_counts = nullptr;
@@ -371,13 +374,13 @@ __global__ __launch_bounds__(BS) void k_b_density(BatchArgs b, const float4* __r
     uint32_t span[3] = {st.box.span[0], st.box.span[1], st.box.span[2]};
     float4 at = P[i];
     float sum = 0.0f;
-    visit_near(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+    visit_near(P, dense_grid(lo, span, prm.cell_start), at, prm.radius, prm.reach,
                [&](uint32_t, float4, float, float, float, float sq) {
                  sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
                });
     float wsum = 0.0f;
     if (prm.wall_cells)
-      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
+      visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach,
                  [&](uint32_t, float4, float, float, float, float sq) {
                    wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
                  });
@@ -414,7 +417,7 @@ __global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __
     float density = vi.w;
     float own = scaled[i];
     float px = 0.0f, py = 0.0f, pz = 0.0f;
-    visit_near(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+    visit_near(PB, dense_grid(lo, span, prm.cell_start), at, prm.radius, prm.reach,
                [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
                  if (sq <= 0.0f) return;
                  float distance = sqrtf(sq);
@@ -434,7 +437,7 @@ __global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __
       float mirrored = own + pressure / (prm.rest * prm.rest);
       float awx = 0.0f, awy = 0.0f, awz = 0.0f;
       auto walls = [&](auto&& f) {
-        visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
+        visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach, f);
       };
       walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
         float gx, gy, gz;

@@ -380,12 +380,15 @@ __global__ __launch_bounds__(256) void k_extent_bodies(const float4* __restrict_
 
 // Any occupied cell of the fluid's last grid inside each body's cell range
 // (lo, hi from the host): the grid's table and box are the last sort's.
-__global__ __launch_bounds__(256) void k_extent_wet(const uint32_t* __restrict__ start, const Control* ctl,
+__global__ __launch_bounds__(256) void k_extent_wet(const uint32_t* __restrict__ start,
+                                                    const uint64_t* __restrict__ keys, const Control* ctl,
                                                     const WetQuery* __restrict__ q, uint32_t nq,
                                                     BodyExtent* __restrict__ e) {
   uint32_t b = blockIdx.y;
   if (b >= nq) return;
   const WetQuery& w = q[b];
+  const GridView g{{ctl->lo[0], ctl->lo[1], ctl->lo[2]}, {ctl->span[0], ctl->span[1], ctl->span[2]}, start, keys,
+                   keys ? ctl->cells : 0u};
   int64_t lo[3], n[3];
   uint64_t volume = 1;
   for (int a = 0; a < 3; ++a) {
@@ -400,8 +403,8 @@ __global__ __launch_bounds__(256) void k_extent_wet(const uint32_t* __restrict__
        k += (uint64_t)gridDim.x * blockDim.x) {
     int64_t z = (int64_t)(k % n[2]), rest = (int64_t)(k / n[2]);
     int64_t y = rest % n[1], x = rest / n[1];
-    int64_t slot = cell_slot(lo[0] + x, lo[1] + y, lo[2] + z, ctl->lo, ctl->span);
-    if (slot >= 0 && start[slot + 1] > start[slot]) {
+    uint32_t s, f;
+    if (grid_range(g, lo[0] + x, lo[1] + y, lo[2] + z, s, f)) {
       e[w.body].wet = 1;
       return;
     }
@@ -892,10 +895,10 @@ void launch_extent(hipStream_t s, const float4* MS, const float4* MV, const Body
   hipLaunchKernelGGL(k_extent_bodies, nbodies, 256, 0, s, MS, MV, bodies, e);
 }
 
-void launch_extent_wet(hipStream_t s, const uint32_t* start, const Control* ctl, const WetQuery* q, uint32_t nq,
-                       BodyExtent* e) {
+void launch_extent_wet(hipStream_t s, const uint32_t* start, const uint64_t* keys, const Control* ctl,
+                       const WetQuery* q, uint32_t nq, BodyExtent* e) {
   if (nq == 0) return;
-  hipLaunchKernelGGL(k_extent_wet, dim3(64, nq), 256, 0, s, start, ctl, q, nq, e);
+  hipLaunchKernelGGL(k_extent_wet, dim3(64, nq), 256, 0, s, start, keys, ctl, q, nq, e);
 }
 
 void launch_lift_box(hipStream_t s, const float4* probes, const BodyState* bodies, uint32_t nbodies,

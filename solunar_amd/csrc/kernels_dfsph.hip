@@ -222,16 +222,23 @@ __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restri
         const uint32_t z0 = (uint32_t)((int64_t)cz - lo[2]) / depth * depth;
         za = z0 > 0 ? z0 - 1 : 0;
       }
+      // lists: the table may be searched; compact: always dense
+      const GridView grid = fluid_grid(prm);
       for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
           for (int dx = -1; dx <= 1; ++dx) {
-            int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
-            if (c < 0) continue;
             const uint32_t code = FORMAT == kCompact ? (uint32_t)((dx + 1) * 4 + (dy + 1))
                                                      : (uint32_t)((dx + 1) * 3 + (dy + 1));
-            uint32_t first = 0;
-            if constexpr (FORMAT == kCompact) first = prm.cell_start[c - ((int64_t)cz + dz - lo[2]) + za];
-            uint32_t s0 = prm.cell_start[c], e0 = prm.cell_start[c + 1];
+            uint32_t first = 0, s0, e0;
+            if constexpr (FORMAT == kCompact) {
+              int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
+              if (c < 0) continue;
+              first = prm.cell_start[c - ((int64_t)cz + dz - lo[2]) + za];
+              s0 = prm.cell_start[c];
+              e0 = prm.cell_start[c + 1];
+            } else {
+              if (!grid_range(grid, (int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, s0, e0)) continue;
+            }
             for (uint32_t j = s0; j < e0; ++j) {
               float4 pj = P[j];
               float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restri
     }
     const uint32_t wall0 = FORMAT == kLists ? nf : 0u;
     if (prm.wall_cells)
-      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
+      visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach,
                  [&](uint32_t w, float4, float ox, float oy, float oz, float sq) {
                    float distance = sqrtf(sq);
                    wsum += kernel_value(distance, prm.radius, prm.scale);

@@ -9,6 +9,8 @@
 // sort_unstable on (cell, id, index) does, and within a cell each particle's
 // slot is the number of cell-mates with a smaller id (ids are unique), which
 // does not depend on the order the counting atomics resolved in.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "device.hpp"
 #include "launch.hpp"
 
@@ -125,6 +127,22 @@ __global__ void k_grid_setup(Control* ctl, uint32_t n, uint64_t capacity) {
     uint64_t prod = vol * (uint64_t)s;
     if (s != 0 && prod / (uint64_t)s != vol) big = true;
     vol = big ? 0xffffffffffffffffull : prod;
+  }
+  if (capacity == kSearchedTable) {
+    // the searched table: any box whose slots fit a 63-bit key
+    if (big || vol >= (1ull << 63)) {
+      ctl->flags |= kFlagOverflow;
+      ctl->abort = 1;
+      ctl->need_volume = 0xffffffffffffffffull;
+      return;
+    }
+    for (int a = 0; a < 3; ++a) {
+      ctl->lo[a] = bmin[a];
+      ctl->span[a] = span[a];
+    }
+    ctl->volume = 0;
+    ctl->need_volume = vol;  // the box, for the host to go dense again
+    return;
   }
   if (big || vol > capacity || vol >= 0xffffffffull) {
     ctl->flags |= kFlagOverflow;
@@ -290,21 +308,94 @@ __global__ __launch_bounds__(256) void k_rank_gather(const float4* __restrict__ 
   if (src) src[s + rank] = me.x;
 }
 
+// ---- the searched table (src/fluid/mod.rs:214-236 without the dense index) --
+//
+// A sort on a searched table: every particle's cell slot in the box as a
+// 64-bit key (ascending keys are the reference's (cx, cy, cz) order), a
+// stable radix sort of (key, index), the run heads of the sorted keys and
+// their exclusive scan (the occupied cells, with their first positions),
+// then the gather, which ranks each particle among its cell-mates by id as
+// the dense sort's gather does.
+
+__global__ __launch_bounds__(256) void k_searched_keys(const float4* __restrict__ P, uint32_t n, float radius,
+                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
+                                                       const Control* ctl) {
+  if (ctl->abort) return;
+  const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+  const uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+  const float4 p = P[i];
+  keys[i] = (uint64_t)cell_slot(cell_coord(p.x, radius), cell_coord(p.y, radius), cell_coord(p.z, radius), lo, span);
+  values[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_searched_heads(const uint64_t* __restrict__ keys, uint32_t n,
+                                                        uint32_t* __restrict__ heads, const Control* ctl) {
+  if (ctl->abort) return;
+  const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  heads[q] = q == 0 || keys[q] != keys[q - 1] ? 1u : 0u;
+}
+
+// With `scan` the exclusive scan of the heads (scan[n] = occupied cells):
+// each cell's key and first position, and start[cells] = n.
+__global__ __launch_bounds__(256) void k_searched_cells(const uint64_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ heads,
+                                                        const uint32_t* __restrict__ scan, uint32_t n,
+                                                        uint64_t* __restrict__ cell_keys,
+                                                        uint32_t* __restrict__ start, Control* ctl) {
+  if (ctl->abort) return;
+  const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
+  if (q < n && heads[q]) {
+    cell_keys[scan[q]] = keys[q];
+    start[scan[q]] = q;
+  }
+  if (q == 0) {
+    start[scan[n]] = n;
+    ctl->cells = scan[n];
+  }
+}
+
+// Sorted position q: its cell's range from the scan, its slot the cell's
+// start plus the number of cell-mates with a smaller id. `src` (optional)
+// keeps each sorted slot's index before the sort.
+__global__ __launch_bounds__(256) void k_searched_gather(const float4* __restrict__ PA, const float4* __restrict__ VA,
+                                                         uint32_t n, const uint32_t* __restrict__ values,
+                                                         const uint32_t* __restrict__ heads,
+                                                         const uint32_t* __restrict__ scan,
+                                                         const uint32_t* __restrict__ start, float4* __restrict__ PB,
+                                                         float4* __restrict__ VB, const Control* ctl,
+                                                         uint32_t* __restrict__ src) {
+  if (ctl->abort) return;
+  const uint32_t q = xcd_block() * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const uint32_t cell = scan[q] + heads[q] - 1u;
+  const uint32_t s = start[cell], e = start[cell + 1];
+  const uint32_t me = values[q];
+  const float4 p = PA[me];
+  const uint32_t id = id_of(p);
+  uint32_t rank = 0;
+  for (uint32_t k = s; k < e; ++k) rank += id_of(PA[values[k]]) < id ? 1u : 0u;
+  PB[s + rank] = p;
+  VB[s + rank] = VA[me];
+  if (src) src[s + rank] = me;
+}
+
 // ---- CSR neighbour lists (readback / list density only) ----------------
 
 // Visits, in the reference's order, every particle of `P` within reach of
-// `at` through the dense table, calling f(j) for each.
+// `at` through the cell table, calling f(j) for each (near, mod.rs:264-277:
+// |x_j - x_i|^2 < r^2).
 template <class F>
-__device__ __forceinline__ void for_each_near(const float4* __restrict__ P, const uint32_t* __restrict__ start,
-                                              const int32_t lo[3], const uint32_t span[3], float4 at, float radius,
+__device__ __forceinline__ void for_each_near(const float4* __restrict__ P, const GridView& g, float4 at, float radius,
                                               float reach, F&& f) {
   int32_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
   for (int dz = -1; dz <= 1; ++dz)
     for (int dy = -1; dy <= 1; ++dy)
       for (int dx = -1; dx <= 1; ++dx) {
-        int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
-        if (c < 0) continue;
-        uint32_t s = start[c], e = start[c + 1];
+        uint32_t s, e;
+        if (!grid_range(g, (int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, s, e)) continue;
         for (uint32_t j = s; j < e; ++j) {
           float4 o = P[j];
           float ox = o.x - at.x, oy = o.y - at.y, oz = o.z - at.z;
@@ -322,12 +413,9 @@ __global__ __launch_bounds__(256) void k_list_count(const float4* __restrict__ l
   uint32_t cnt = 0;
   float4 at = lp[i];
   if (kind == 0) {
-    int32_t lo[3] = {prm.ctl->lo[0], prm.ctl->lo[1], prm.ctl->lo[2]};
-    uint32_t span[3] = {prm.ctl->span[0], prm.ctl->span[1], prm.ctl->span[2]};
-    for_each_near(lp, prm.cell_start, lo, span, at, prm.radius, prm.reach, [&](uint32_t) { ++cnt; });
+    for_each_near(lp, fluid_grid(prm), at, prm.radius, prm.reach, [&](uint32_t) { ++cnt; });
   } else if (prm.wall_cells) {
-    for_each_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
-                  [&](uint32_t) { ++cnt; });
+    for_each_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach, [&](uint32_t) { ++cnt; });
   }
   counts[i] = cnt;
 }
@@ -340,12 +428,9 @@ __global__ __launch_bounds__(256) void k_list_emit(const float4* __restrict__ lp
   uint32_t at_k = list_start[i];
   float4 at = lp[i];
   if (kind == 0) {
-    int32_t lo[3] = {prm.ctl->lo[0], prm.ctl->lo[1], prm.ctl->lo[2]};
-    uint32_t span[3] = {prm.ctl->span[0], prm.ctl->span[1], prm.ctl->span[2]};
-    for_each_near(lp, prm.cell_start, lo, span, at, prm.radius, prm.reach, [&](uint32_t j) { items[at_k++] = j; });
+    for_each_near(lp, fluid_grid(prm), at, prm.radius, prm.reach, [&](uint32_t j) { items[at_k++] = j; });
   } else if (prm.wall_cells) {
-    for_each_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
-                  [&](uint32_t j) { items[at_k++] = j; });
+    for_each_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach, [&](uint32_t j) { items[at_k++] = j; });
   }
 }
 
@@ -404,6 +489,28 @@ void launch_bounds(hipStream_t s, const float4* P, uint32_t n, float radius, Con
 
 void launch_grid_setup(hipStream_t s, Control* ctl, uint32_t n, uint64_t capacity) {
   hipLaunchKernelGGL(k_grid_setup, 1, 64, 0, s, ctl, n, capacity);
+}
+
+hipError_t searched_sort_bytes(uint32_t n, size_t* bytes) {
+  *bytes = 0;
+  return rocprim::radix_sort_pairs(nullptr, *bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                   (const uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 64);
+}
+
+hipError_t launch_searched_sort(hipStream_t s, const SearchedSort& t, const float4* PA, const float4* VA, uint32_t n,
+                                float radius, float4* PB, float4* VB, Control* ctl, uint32_t* src) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = blocks_for(n, 256);
+  hipLaunchKernelGGL(k_searched_keys, blocks, 256, 0, s, PA, n, radius, t.keys_in, t.values_in, ctl);
+  size_t bytes = t.temp_bytes;
+  hipError_t e = rocprim::radix_sort_pairs(t.temp, bytes, t.keys_in, t.keys, t.values_in, t.values, n, 0, 64, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_searched_heads, blocks, 256, 0, s, t.keys, n, t.heads, ctl);
+  launch_scan(s, t.heads, t.scan, t.tile_sums, t.total, nullptr, n, 0, ctl);
+  hipLaunchKernelGGL(k_searched_cells, blocks, 256, 0, s, t.keys, t.heads, t.scan, n, t.cell_keys, t.start, ctl);
+  hipLaunchKernelGGL(k_searched_gather, blocks, 256, 0, s, PA, VA, n, t.values, t.heads, t.scan, t.start, PB, VB,
+                     ctl, src);
+  return hipGetLastError();
 }
 
 void launch_count(hipStream_t s, const float4* P, uint32_t n, float radius, uint32_t* counts, uint2* slot_rank,

@@ -21,17 +21,15 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
   bool live = i < prm.n;
   float comp = 0.0f;
   if (live) {
-    int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
-    uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
     float4 at = P[i];
     float sum = 0.0f;
-    visit_near(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+    visit_near(P, fluid_grid(prm), at, prm.radius, prm.reach,
                [&](uint32_t, float4, float, float, float, float sq) {
                  sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
                });
     float wsum = 0.0f;
     if (prm.wall_cells)
-      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
+      visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach,
                  [&](uint32_t, float4, float, float, float, float sq) {
                    wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
                  });
@@ -73,13 +71,10 @@ template <bool MOVERS, bool RECORD>
 __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                            const float* __restrict__ scaled, const StepParams& prm, uint32_t i,
                                            float4 at, float4 vi, float& ax, float& ay, float& az) {
-  const Control* ctl = prm.ctl;
-  int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
-  uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
   float density = vi.w;
   float own = scaled[i];
   float px = 0.0f, py = 0.0f, pz = 0.0f;
-  visit_near(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+  visit_near(PB, fluid_grid(prm), at, prm.radius, prm.reach,
              [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
                if (sq <= 0.0f) return;
                float distance = sqrtf(sq);
@@ -101,7 +96,7 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
   }
   auto walls = [&](auto&& f) {
     if (prm.wall_cells)
-      visit_near(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
+      visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach, f);
   };
   float awx = 0.0f, awy = 0.0f, awz = 0.0f;
   if (prm.wall_cells) {

@@ -14,6 +14,27 @@ void launch_substep_begin(hipStream_t s, Control* ctl);
 void launch_fastest(hipStream_t s, const float4* V, uint32_t n, Control* ctl);
 void launch_bounds(hipStream_t s, const float4* P, uint32_t n, float radius, Control* ctl);
 void launch_grid_setup(hipStream_t s, Control* ctl, uint32_t n, uint64_t capacity);
+// capacity kSearchedTable: the box goes to a searched table (any size whose
+// slots fit 63 bits)
+constexpr uint64_t kSearchedTable = 0xffffffffffffffffull;
+// The buffers of a sort on a searched table (n particles; scan n + 1).
+struct SearchedSort {
+  uint64_t* keys_in;
+  uint64_t* keys;
+  uint32_t* values_in;
+  uint32_t* values;
+  uint32_t* heads;
+  uint32_t* scan;
+  uint32_t* tile_sums;  // the scan's, (n + 2048) / 2048 + 1
+  uint32_t* total;
+  uint64_t* cell_keys;
+  uint32_t* start;  // n + 1
+  void* temp;
+  size_t temp_bytes;
+};
+hipError_t searched_sort_bytes(uint32_t n, size_t* bytes);
+hipError_t launch_searched_sort(hipStream_t s, const SearchedSort& t, const float4* PA, const float4* VA, uint32_t n,
+                                float radius, float4* PB, float4* VB, Control* ctl, uint32_t* src = nullptr);
 void launch_count(hipStream_t s, const float4* P, uint32_t n, float radius, uint32_t* counts, uint2* slot_rank,
                   Control* ctl);
 void launch_scan(hipStream_t s, uint32_t* in, uint32_t* out, uint32_t* tile_sums, uint32_t* total,

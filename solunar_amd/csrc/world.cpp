@@ -20,7 +20,10 @@ namespace {
 
 constexpr float kPi = 3.14159265358979323846f;  // std::f32::consts::PI
 constexpr uint64_t kMaxCells = uint64_t(1) << 31;
-constexpr uint64_t kMaxWallCells = uint64_t(1) << 30;
+// A dense table is always allowed this many cells; a searched table's cell
+// key (the slot in the box) stays below 2^63.
+constexpr uint64_t kDenseFloor = uint64_t(1) << 26;
+constexpr uint64_t kMaxKeyedCells = uint64_t(1) << 63;
 
 // Kernel::value (src/fluid/mod.rs:140-146) on the host.
 float kernel_value_host(float distance, float radius, float scale) {
@@ -703,19 +706,25 @@ int32_t World::settle_walls() {
     volume = 1;
     for (int a = 0; a < 3; ++a) {
       uint64_t s = (uint64_t)((int64_t)hi[a] - (int64_t)lo[a] + 1);
+      if (s > 0xffffffffull || volume * s / s != volume || volume * s >= kMaxKeyedCells) {
+        detail_ = "walls spread over more cells than a 64-bit cell key addresses";
+        return SFL_UNSUPPORTED;
+      }
       box.lo[a] = lo[a];
-      box.span[a] = (uint32_t)std::min<uint64_t>(s, 0xffffffffu);
-      volume = (s > kMaxWallCells || volume * s > kMaxWallCells) ? kMaxWallCells + 1 : volume * s;
-    }
-    if (volume > kMaxWallCells) {
-      detail_ = "walls span more cells than the device wall table holds";
-      return SFL_UNSUPPORTED;
+      box.span[a] = (uint32_t)s;
+      volume *= s;
     }
   }
+  // dense while the box holds at most 64 cells a wall particle and a floor
+  // (the reference keeps its dense index up to 64 per occupied cell plus
+  // 4096, mod.rs:38-41, 214-217); past that the occupied cells are searched
+  const bool searched = volume > std::max<uint64_t>(64 * (uint64_t)n + 4096, kDenseFloor);
   std::vector<float> pos(3 * n);
   std::vector<uint32_t> own(n);
   std::vector<float4> dev(n);
-  std::vector<uint32_t> start(volume + 1, 0);
+  std::vector<uint32_t> start;
+  std::vector<uint64_t> keys;  // searched: the occupied cells' slots, ascending
+  if (!searched) start.assign(volume + 1, 0);
   for (size_t k = 0; k < n; ++k) {
     const float* p = wall_pos_.data() + 3 * order[k].index;
     pos[3 * k] = p[0];
@@ -726,23 +735,38 @@ int32_t World::settle_walls() {
     uint64_t slot = ((uint64_t)(order[k].c[0] - box.lo[0]) * box.span[1] + (uint64_t)(order[k].c[1] - box.lo[1])) *
                         box.span[2] +
                     (uint64_t)(order[k].c[2] - box.lo[2]);
-    start[slot + 1] += 1;
+    if (!searched) {
+      start[slot + 1] += 1;
+    } else if (keys.empty() || keys.back() != slot) {
+      keys.push_back(slot);
+      start.push_back((uint32_t)k);
+    }
   }
-  for (uint64_t c = 0; c < volume; ++c) start[c + 1] += start[c];
+  if (!searched)
+    for (uint64_t c = 0; c < volume; ++c) start[c + 1] += start[c];
+  else
+    start.push_back((uint32_t)n);
   if (n) {
     SFL_HIP(walls_d_.reserve(n, 0, stream_), "hipMalloc(walls)");
-    SFL_HIP(wall_start_d_.reserve(volume + 1, 0, stream_), "hipMalloc(wall table)");
+    SFL_HIP(wall_start_d_.reserve(start.size(), 0, stream_), "hipMalloc(wall table)");
     SFL_HIP(hipMemcpyAsync(walls_d_.ptr, dev.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream_),
             "upload walls");
-    SFL_HIP(hipMemcpyAsync(wall_start_d_.ptr, start.data(), (volume + 1) * sizeof(uint32_t),
+    SFL_HIP(hipMemcpyAsync(wall_start_d_.ptr, start.data(), start.size() * sizeof(uint32_t),
                            hipMemcpyHostToDevice, stream_),
             "upload wall table");
+    if (searched) {
+      SFL_HIP(wall_keys_d_.reserve(keys.size(), 0, stream_), "hipMalloc(wall table)");
+      SFL_HIP(hipMemcpyAsync(wall_keys_d_.ptr, keys.data(), keys.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                             stream_),
+              "upload wall table");
+    }
     SFL_HIP(hipStreamSynchronize(stream_), "upload walls");
   }
   wall_pos_.swap(pos);
   wall_owner_.swap(own);
   wall_box_ = box;
-  wall_cells_ = (uint32_t)volume;
+  wall_keyed_ = searched ? (uint32_t)keys.size() : 0u;
+  wall_cells_ = searched ? wall_keyed_ : (uint32_t)volume;
   // union of the containers (boundary.rs:240-247)
   has_bounds_ = !containers_.empty();
   for (size_t i = 0; i < containers_.size(); ++i)
@@ -789,9 +813,12 @@ StepParams World::params(float dt) const {
   }
   p.walls = walls_d_.ptr;
   p.wall_start = wall_start_d_.ptr;
+  p.wall_keys = wall_keyed_ ? wall_keys_d_.ptr : nullptr;
+  p.wall_keyed = wall_keyed_;
   p.wall_box = wall_box_;
   p.wall_cells = wall_owner_.empty() ? 0u : wall_cells_;
-  p.cell_start = cell_start_.ptr;
+  p.cell_start = searched_ ? sstart_.ptr : cell_start_.ptr;
+  p.cell_keys = searched_ ? cell_keys_.ptr : nullptr;
   p.ctl = ctl_;
   p.pairs = pairs_.ptr;
   p.pair_counts = pair_counts_.ptr;
@@ -849,15 +876,47 @@ int32_t World::ensure_pairs(uint32_t cap) {
   return SFL_OK;
 }
 
+uint64_t World::dense_limit() const { return std::max<uint64_t>(64 * (uint64_t)n_ + 4096, kDenseFloor); }
+
+// The searched table's buffers for the current particle count; the sorts
+// from here on go through it.
+int32_t World::use_searched() {
+  const size_t n = std::max<size_t>(n_, 1);
+  SFL_HIP(skeys_in_.reserve(n, 0, stream_), "hipMalloc(searched table)");
+  SFL_HIP(skeys_.reserve(n, 0, stream_), "hipMalloc(searched table)");
+  SFL_HIP(svals_in_.reserve(n, 0, stream_), "hipMalloc(searched table)");
+  SFL_HIP(svals_.reserve(n, 0, stream_), "hipMalloc(searched table)");
+  SFL_HIP(heads_.reserve(n, 0, stream_), "hipMalloc(searched table)");
+  SFL_HIP(head_scan_.reserve(n + 1, 0, stream_), "hipMalloc(searched table)");
+  SFL_HIP(cell_keys_.reserve(n, 0, stream_), "hipMalloc(searched table)");
+  SFL_HIP(sstart_.reserve(n + 1, 0, stream_), "hipMalloc(searched table)");
+  SFL_HIP(tile_sums_.reserve((n + 2048) / 2048 + 1, 0, stream_), "hipMalloc(tile sums)");
+  size_t temp = 0;
+  SFL_HIP(searched_sort_bytes((uint32_t)n, &temp), "radix sort size");
+  SFL_HIP(ssort_temp_.reserve(std::max<size_t>(temp, 1), 0, stream_), "hipMalloc(radix sort)");
+  searched_ = true;
+  return SFL_OK;
+}
+
 int32_t World::ensure_cells(uint64_t volume) {
   if (volume <= cell_cap_ && counts_.ptr) return SFL_OK;
   if (no_alloc_) {
     detail_ = "the fluid spreads over more cells than the allocation-free step's scratch holds";
     return SFL_OUT_OF_RANGE;
   }
-  if (volume > kMaxCells) {
-    detail_ = "the fluid spread over more cells than the device cell table holds";
+  if (volume == kSearchedTable) {
+    detail_ = "the fluid spread over more cells than a 63-bit cell key addresses";
     return SFL_OUT_OF_RANGE;
+  }
+  if (volume > dense_limit() || volume > kMaxCells) {
+    // too thin for a dense table: search the occupied cells, as the
+    // reference does (mod.rs:214-217). The DFSPH stop sum runs over the
+    // tiles of the dense box, so a DFSPH world keeps a dense table.
+    if (settings_.solver == SFL_WCSPH) return use_searched();
+    if (volume > kMaxCells) {
+      detail_ = "a DFSPH world spread over more cells than the dense cell table holds";
+      return SFL_OUT_OF_RANGE;
+    }
   }
   uint64_t cap = std::min<uint64_t>(volume + volume / 4 + 4096, kMaxCells);
   counts_.release();
@@ -880,7 +939,13 @@ int32_t World::ensure_cells(uint64_t volume) {
 }
 
 // A first table size: the containers' cells with a margin, or a small one.
+// A world on the searched table goes back to a dense one once its last box
+// is well within the dense limit.
 int32_t World::ensure_cells_for_bounds() {
+  if (searched_) {
+    if (last_box_ > dense_limit() / 2) return use_searched();
+    searched_ = false;
+  }
   uint64_t want = std::max<uint64_t>(4096, 2 * (uint64_t)n_);
   if (has_bounds_) {
     uint64_t v = 1;
@@ -892,7 +957,7 @@ int32_t World::ensure_cells_for_bounds() {
     }
     want = std::max(want, v);
   }
-  return ensure_cells(std::min<uint64_t>(want, kMaxCells));
+  return ensure_cells(std::min<uint64_t>(want, std::min<uint64_t>(dense_limit(), kMaxCells)));
 }
 
 int32_t World::read_control() {
@@ -900,6 +965,8 @@ int32_t World::read_control() {
           "read control");
   SFL_HIP(hipStreamSynchronize(stream_), "step");
   collect_profile();
+  // a searched sort leaves its box's volume here
+  if (searched_ && ctl_host_->need_volume != kSearchedTable) last_box_ = ctl_host_->need_volume;
   return SFL_OK;
 }
 
@@ -958,7 +1025,7 @@ void World::collect_profile() {
 
 // update_neighbors' sort, gathers and grid build (src/fluid/mod.rs:540-566)
 // from A into B.
-void World::enqueue_sort() {
+int32_t World::enqueue_sort() {
   uint32_t n = (uint32_t)n_;
   grid_any_ = n > 0;
   grid_dropped_ = false;
@@ -966,8 +1033,18 @@ void World::enqueue_sort() {
   launch_substep_begin(stream_, ctl_);
   phase_begin(SFL_PHASE_BOUNDS);
   launch_bounds(stream_, pa_.ptr, n, radius_, ctl_);
-  launch_grid_setup(stream_, ctl_, n, cell_cap_);
+  launch_grid_setup(stream_, ctl_, n, searched_ ? kSearchedTable : cell_cap_);
   phase_end(SFL_PHASE_BOUNDS);
+  if (searched_) {
+    phase_begin(SFL_PHASE_SORT);
+    const SearchedSort t{skeys_in_.ptr, skeys_.ptr, svals_in_.ptr, svals_.ptr, heads_.ptr, head_scan_.ptr,
+                         tile_sums_.ptr, scan_total_.ptr, cell_keys_.ptr, sstart_.ptr, ssort_temp_.ptr,
+                         ssort_temp_.cap};
+    hipError_t e = launch_searched_sort(stream_, t, pa_.ptr, va_.ptr, n, radius_, pb_.ptr, vb_.ptr, ctl_);
+    phase_end(SFL_PHASE_SORT);
+    if (e != hipSuccess) return device_error("radix sort (searched cell table)", e);
+    return SFL_OK;
+  }
   phase_begin(SFL_PHASE_SORT);
   launch_count(stream_, pa_.ptr, n, radius_, counts_.ptr, slot_rank_.ptr, ctl_);
   launch_scan(stream_, counts_.ptr, cell_start_.ptr, tile_sums_.ptr, scan_total_.ptr, &ctl_->volume, cell_cap_, 1,
@@ -977,6 +1054,7 @@ void World::enqueue_sort() {
   phase_begin(SFL_PHASE_REORDER);
   launch_rank_gather(stream_, pa_.ptr, va_.ptr, n, radius_, scatter_.ptr, cell_start_.ptr, pb_.ptr, vb_.ptr, ctl_);
   phase_end(SFL_PHASE_REORDER);
+  return SFL_OK;
 }
 
 // wcsph_substep (src/fluid/step.rs:144-224), substep `index` of the step;
@@ -991,7 +1069,8 @@ int32_t World::enqueue_wcsph_substep(const StepParams& base, uint32_t index, flo
     if (st != SFL_OK) return st;
     prm.movers = mover_view(prm.dt);
   }
-  enqueue_sort();
+  int32_t st = enqueue_sort();
+  if (st != SFL_OK) return st;
   phase_begin(SFL_PHASE_DENSITY);
   launch_wcsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm);
   phase_end(SFL_PHASE_DENSITY);
@@ -1016,7 +1095,8 @@ int32_t World::enqueue_wcsph_substep(const StepParams& base, uint32_t index, flo
 // dfsph_substep (src/fluid/dfsph.rs:79-103)
 int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   const sfl_settings& s = settings_;
-  enqueue_sort();
+  int32_t st = enqueue_sort();
+  if (st != SFL_OK) return st;
   phase_begin(SFL_PHASE_DENSITY);
   SFL_HIP(launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm), "density pass");
   phase_end(SFL_PHASE_DENSITY);
@@ -1124,6 +1204,10 @@ int32_t World::step_no_alloc(float dt, sfl_report* out) {
   if (keep_forces_ && forces_.cap < 3 * n_) {
     detail_ = "the kept forces need a larger buffer than the world holds";
     return SFL_UNSUPPORTED;
+  }
+  if (searched_) {
+    detail_ = "the fluid spreads over more cells than the allocation-free step's scratch holds";
+    return SFL_OUT_OF_RANGE;
   }
   if (n_ == 0 || !(dt > 0.0f) || !std::isfinite(dt)) {
     // no device work: the refusals and the empty step of World::step
@@ -1450,7 +1534,8 @@ int32_t World::update_neighbors() {
   if (st != SFL_OK) return st;
   for (;;) {
     launch_step_begin(stream_, ctl_);
-    enqueue_sort();
+    st = enqueue_sort();
+    if (st != SFL_OK) return st;
     st = read_control();
     if (st != SFL_OK) return st;
     if (ctl_host_->flags & kFlagOverflow) {
@@ -1677,7 +1762,8 @@ size_t World::device_bytes() const {
   return sizeof(Control) + b(pa_) + b(va_) + b(pb_) + b(vb_) + b(aux_) + b(kappa_) + b(errors_) + b(forces_) +
          b(slot_rank_) + b(scatter_) + b(counts_) + b(cell_start_) + b(tile_sums_) + b(scan_total_) + b(pairs_) +
          b(masks_) + b(own_slot_) + b(near_) + b(factor_) +
-         b(pair_counts_) + b(stop_sum_) + b(stop_count_) + b(walls_d_) + b(wall_start_d_) + b(fstart_) + b(fitems_) +
+         b(pair_counts_) + b(stop_sum_) + b(stop_count_) + b(walls_d_) + b(wall_start_d_) + b(wall_keys_d_) + b(skeys_in_) + b(skeys_) + b(cell_keys_) +
+         b(svals_in_) + b(svals_) + b(heads_) + b(head_scan_) + b(sstart_) + b(ssort_temp_) + b(fstart_) + b(fitems_) +
          b(wstart_) + b(witems_) + b(list_counts_) + b(flow_flags_) + b(flow_start_) + b(flow_tiles_) +
          b(flow_total_) + b(body_local_) + b(probe_local_) + b(bodies_d_) + b(ms_[0]) + b(ms_[1]) + b(mv_[0]) +
          b(mv_[1]) + b(mp_) + b(mkeys_in_) + b(mkeys_) + b(mperm_in_) + b(mperm_) + b(msort_temp_) + b(mgrid_) +

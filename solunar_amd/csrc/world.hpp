@@ -184,7 +184,7 @@ class World {
   int32_t settle_walls();
   int32_t wcsph(float dt, sfl_report& r);
   int32_t dfsph(float dt, sfl_report& r);
-  void enqueue_sort();
+  int32_t enqueue_sort();
   int32_t enqueue_wcsph_substep(const StepParams& prm, uint32_t index, float small);
   int32_t enqueue_dfsph_substep(const StepParams& prm);
   int32_t emit_lists();
@@ -258,6 +258,17 @@ class World {
   DeviceBuffer<uint2> slot_rank_, scatter_;
   DeviceBuffer<uint32_t> counts_, cell_start_, tile_sums_, scan_total_;
   uint64_t cell_cap_ = 0;
+  // The searched fluid table (the reference's Grid without its dense index,
+  // src/fluid/mod.rs:214-236), used while the particles' cell box is more
+  // than dense_limit() cells: each sort radix-sorts the cells' 64-bit slots
+  // and keeps the occupied ones (cell_keys_) with their ranges (sstart_).
+  bool searched_ = false;
+  uint64_t last_box_ = 0;  // cells of the box of the last searched sort
+  DeviceBuffer<uint64_t> skeys_in_, skeys_, cell_keys_;
+  DeviceBuffer<uint32_t> svals_in_, svals_, heads_, head_scan_, sstart_;
+  DeviceBuffer<uint8_t> ssort_temp_;
+  uint64_t dense_limit() const;
+  int32_t use_searched();
   DeviceBuffer<uint2> pairs_;
   DeviceBuffer<uint32_t> pair_counts_;
   uint32_t pair_cap_ = 0, pair_stride_ = 0;
@@ -282,6 +293,8 @@ class World {
   }
   DeviceBuffer<float4> walls_d_;
   DeviceBuffer<uint32_t> wall_start_d_;
+  DeviceBuffer<uint64_t> wall_keys_d_;  // searched wall table: occupied cells' slots
+  uint32_t wall_keyed_ = 0;             // occupied wall cells when searched, else 0
   CellBox wall_box_{};
   uint32_t wall_cells_ = 0;
 
