@@ -420,7 +420,8 @@ __device__ __forceinline__ GridView wall_grid(const StepParams& prm) {
                   prm.wall_keyed};
 }
 
-// The range [s, e) of cell (x, y, z); false when the cell holds nothing.
+// The range [s, e) of cell (x, y, z); false for a cell outside the box or
+// not in a searched table (an empty range may come back true).
 __device__ __forceinline__ bool grid_range(const GridView& g, int64_t x, int64_t y, int64_t z, uint32_t& s,
                                            uint32_t& e) {
   const int64_t c = cell_slot(x, y, z, g.lo, g.span);
@@ -428,7 +429,7 @@ __device__ __forceinline__ bool grid_range(const GridView& g, int64_t x, int64_t
   if (!g.keys) {
     s = g.start[c];
     e = g.start[c + 1];
-    return s < e;
+    return true;
   }
   uint32_t a = 0, b = g.cells;
   while (a < b) {
@@ -465,6 +466,52 @@ __device__ __forceinline__ void visit_near(const float4* __restrict__ P, const G
           if (sq < reach) f(j, pj, ox, oy, oz, sq);
         }
       }
+}
+
+// visit_near over a dense table given as its parts: the form the hottest
+// kernels use when no table of the step is searched (a GridView built from
+// the parameters holds its fields in scalar registers for the whole kernel,
+// which spills them there).
+template <class F>
+__device__ __forceinline__ void visit_dense(const float4* __restrict__ P, const uint32_t* __restrict__ start,
+                                            const int32_t lo[3], const uint32_t span[3], float4 at, float radius,
+                                            float reach, F&& f) {
+  int32_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
+        if (c < 0) continue;
+        uint32_t s = start[c], e = start[c + 1];
+        for (uint32_t j = s; j < e; ++j) {
+          float4 pj = P[j];
+          float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+          float sq = dot3(ox, oy, oz, ox, oy, oz);
+          if (sq < reach) f(j, pj, ox, oy, oz, sq);
+        }
+      }
+}
+
+// The water's neighbours of `at` in the last sort's table, and the walls';
+// SEARCH = false for a step whose tables are all dense.
+template <bool SEARCH, class F>
+__device__ __forceinline__ void visit_fluid(const float4* __restrict__ P, const StepParams& prm, float4 at, F&& f) {
+  if constexpr (SEARCH) {
+    visit_near(P, fluid_grid(prm), at, prm.radius, prm.reach, f);
+  } else {
+    const Control* ctl = prm.ctl;
+    const int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+    const uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+    visit_dense(P, prm.cell_start, lo, span, at, prm.radius, prm.reach, f);
+  }
+}
+
+template <bool SEARCH, class F>
+__device__ __forceinline__ void visit_walls(const StepParams& prm, float4 at, F&& f) {
+  if constexpr (SEARCH)
+    visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach, f);
+  else
+    visit_dense(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
 }
 
 // First index in keys[0, n) whose key is not below `target`.

@@ -24,12 +24,20 @@
 
 namespace sfl {
 
-__device__ __forceinline__ GridView adj_wall_grid(const AdjointArgs& a) {
-  return GridView{{a.wall_box.lo[0], a.wall_box.lo[1], a.wall_box.lo[2]},
-                  {a.wall_box.span[0], a.wall_box.span[1], a.wall_box.span[2]},
-                  a.wall_start,
-                  a.wall_keys,
-                  a.wall_keyed};
+// The walls near `at`: through the searched table when KEYED, else the
+// dense one given as its parts (see visit_dense).
+template <bool KEYED, class F>
+__device__ __forceinline__ void adj_walls(const AdjointArgs& a, float4 at, F&& f) {
+  if constexpr (KEYED) {
+    const GridView g{{a.wall_box.lo[0], a.wall_box.lo[1], a.wall_box.lo[2]},
+                     {a.wall_box.span[0], a.wall_box.span[1], a.wall_box.span[2]},
+                     a.wall_start,
+                     a.wall_keys,
+                     a.wall_keyed};
+    visit_near(a.walls, g, at, a.radius, a.reach, f);
+  } else {
+    visit_dense(a.walls, a.wall_start, a.wall_box.lo, a.wall_box.span, at, a.radius, a.reach, f);
+  }
 }
 
 namespace {
@@ -127,6 +135,7 @@ __global__ __launch_bounds__(256) void k_adj_advance(AdjointArgs a, const float4
 // wall-viscosity terms, writes rho_bar (through s = p / rho^2 and the
 // wall's p / rho0^2 as well), and folds the viscosity coefficient's
 // adjoint times d nu / d alpha into nu_acc[block] (FP64).
+template <bool KEYED>
 __global__ __launch_bounds__(256) void k_adj_force(AdjointArgs a, const float4* __restrict__ Ab,
                                                    float4* __restrict__ Xb, float4* __restrict__ Vb,
                                                    float* __restrict__ Rb, double* __restrict__ nu_acc) {
@@ -143,42 +152,40 @@ __global__ __launch_bounds__(256) void k_adj_force(AdjointArgs a, const float4* 
     float3 ai = f3(Ab[i]);
     float3 xb = f3(Xb[i]), vb = f3(Vb[i]);
     float sbar = 0.0f, pbar = 0.0f, rbar = 0.0f, nsum = 0.0f;
-    visit_near(a.P, dense_grid(box.lo, box.span, a.cell_start), pi4, r, a.reach,
-               [&](uint32_t j, float4 pj, float ox, float oy, float oz, float sq) {
-                 if (sq <= 0.0f) return;
-                 float d = sqrtf(sq);
-                 Kern k = kern(d, r, sc);
-                 float3 o = make_float3(ox, oy, oz);
-                 float3 g = o * k.f;
-                 float4 vj4 = a.V[j];
-                 float3 dv = vi - f3(vj4);
-                 float u = dot(dv, o);
-                 float t = fminf(u, 0.0f);
-                 float den = sq + a.soften;
-                 float rsum = rho + vj4.w;
-                 float cd = -a.nu * 2.0f / (den * rsum);
-                 float D = cd * t;
-                 float K = m * (si + a.scaled[j] + D);
-                 float3 delta = f3(Ab[j]) - ai;
-                 float G = dot(g, delta);
-                 float od = dot(o, delta);
-                 sbar += m * G;
-                 rbar += m * G * (-D / rsum);
-                 float3 dx = o * (m * G * (-2.0f * D / den)) + (delta * k.f + o * (k.fpd * od)) * K;
-                 if (u < 0.0f) {
-                   vb = vb + o * (m * cd * G);
-                   dx = dx + dv * (m * G * cd);
-                 }
-                 xb = xb + dx;
-                 nsum += m * dot(g, ai) * (2.0f * t / (den * rsum));
-               });
+    visit_dense(a.P, a.cell_start, box.lo, box.span, pi4, r, a.reach,
+                [&](uint32_t j, float4 pj, float ox, float oy, float oz, float sq) {
+                  if (sq <= 0.0f) return;
+                  float d = sqrtf(sq);
+                  Kern k = kern(d, r, sc);
+                  float3 o = make_float3(ox, oy, oz);
+                  float3 g = o * k.f;
+                  float4 vj4 = a.V[j];
+                  float3 dv = vi - f3(vj4);
+                  float u = dot(dv, o);
+                  float t = fminf(u, 0.0f);
+                  float den = sq + a.soften;
+                  float rsum = rho + vj4.w;
+                  float cd = -a.nu * 2.0f / (den * rsum);
+                  float D = cd * t;
+                  float K = m * (si + a.scaled[j] + D);
+                  float3 delta = f3(Ab[j]) - ai;
+                  float G = dot(g, delta);
+                  float od = dot(o, delta);
+                  sbar += m * G;
+                  rbar += m * G * (-D / rsum);
+                  float3 dx = o * (m * G * (-2.0f * D / den)) + (delta * k.f + o * (k.fpd * od)) * K;
+                  if (u < 0.0f) {
+                    vb = vb + o * (m * cd * G);
+                    dx = dx + dv * (m * G * cd);
+                  }
+                  xb = xb + dx;
+                  nsum += m * dot(g, ai) * (2.0f * t / (den * rsum));
+                });
     if (a.wall_cells) {
       float p, dp;
       tait(rho, a.rest, a.stiffness, p, dp);
       float S = si + p / (a.rest * a.rest);
-      auto walls = [&](auto&& f) {
-        visit_near(a.walls, adj_wall_grid(a), pi4, r, a.reach, f);
-      };
+      auto walls = [&](auto&& f) { adj_walls<KEYED>(a, pi4, f); };
       // forward: away, normal, the velocity into the wall
       float3 away = make_float3(0.0f, 0.0f, 0.0f);
       walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
@@ -251,6 +258,7 @@ __global__ __launch_bounds__(256) void k_adj_force(AdjointArgs a, const float4* 
 // R3: the density sum. x_bar_i += m sum_j g_ij (rho_bar_i + rho_bar_j) +
 // m sum_w g_iw rho_bar_i, then both adjoints go back through the sort to
 // the previous substep's order.
+template <bool KEYED>
 __global__ __launch_bounds__(256) void k_adj_density(AdjointArgs a, const float* __restrict__ Rb,
                                                      const float4* __restrict__ Xb, const float4* __restrict__ Vb,
                                                      float4* __restrict__ XbPrev, float4* __restrict__ VbPrev) {
@@ -261,19 +269,18 @@ __global__ __launch_bounds__(256) void k_adj_density(AdjointArgs a, const float*
   float4 pi4 = a.P[i];
   float rbi = Rb[i];
   float3 xb = f3(Xb[i]);
-  visit_near(a.P, dense_grid(box.lo, box.span, a.cell_start), pi4, r, a.reach,
-             [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-               if (sq <= 0.0f) return;
-               Kern k = kern(sqrtf(sq), r, sc);
-               xb = xb + make_float3(ox, oy, oz) * (m * k.f * (rbi + Rb[j]));
-             });
+  visit_dense(a.P, a.cell_start, box.lo, box.span, pi4, r, a.reach,
+              [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+                if (sq <= 0.0f) return;
+                Kern k = kern(sqrtf(sq), r, sc);
+                xb = xb + make_float3(ox, oy, oz) * (m * k.f * (rbi + Rb[j]));
+              });
   if (a.wall_cells)
-    visit_near(a.walls, adj_wall_grid(a), pi4, r, a.reach,
-               [&](uint32_t, float4, float ox, float oy, float oz, float sq) {
-                 if (sq <= 0.0f) return;
-                 Kern k = kern(sqrtf(sq), r, sc);
-                 xb = xb + make_float3(ox, oy, oz) * (m * k.f * rbi);
-               });
+    adj_walls<KEYED>(a, pi4, [&](uint32_t, float4, float ox, float oy, float oz, float sq) {
+      if (sq <= 0.0f) return;
+      Kern k = kern(sqrtf(sq), r, sc);
+      xb = xb + make_float3(ox, oy, oz) * (m * k.f * rbi);
+    });
   uint32_t dst = a.src[i];
   XbPrev[dst] = make_float4(xb.x, xb.y, xb.z, 0.0f);
   VbPrev[dst] = Vb[i];
@@ -321,11 +328,19 @@ void launch_adj_advance(hipStream_t s, const AdjointArgs& a, const float4* XbOut
 uint32_t adj_force_blocks(uint32_t n) { return blocks_for(n, 256); }
 void launch_adj_force(hipStream_t s, const AdjointArgs& a, const float4* Ab, float4* Xb, float4* Vb, float* Rb,
                       double* nu_acc) {
-  if (a.n) hipLaunchKernelGGL(k_adj_force, blocks_for(a.n, 256), 256, 0, s, a, Ab, Xb, Vb, Rb, nu_acc);
+  if (!a.n) return;
+  if (a.wall_keys)
+    hipLaunchKernelGGL(k_adj_force<true>, blocks_for(a.n, 256), 256, 0, s, a, Ab, Xb, Vb, Rb, nu_acc);
+  else
+    hipLaunchKernelGGL(k_adj_force<false>, blocks_for(a.n, 256), 256, 0, s, a, Ab, Xb, Vb, Rb, nu_acc);
 }
 void launch_adj_density(hipStream_t s, const AdjointArgs& a, const float* Rb, const float4* Xb, const float4* Vb,
                         float4* XbPrev, float4* VbPrev) {
-  if (a.n) hipLaunchKernelGGL(k_adj_density, blocks_for(a.n, 256), 256, 0, s, a, Rb, Xb, Vb, XbPrev, VbPrev);
+  if (!a.n) return;
+  if (a.wall_keys)
+    hipLaunchKernelGGL(k_adj_density<true>, blocks_for(a.n, 256), 256, 0, s, a, Rb, Xb, Vb, XbPrev, VbPrev);
+  else
+    hipLaunchKernelGGL(k_adj_density<false>, blocks_for(a.n, 256), 256, 0, s, a, Rb, Xb, Vb, XbPrev, VbPrev);
 }
 void launch_adj_compare(hipStream_t s, uint32_t n, const float4* P, const float4* V, const float4* P2,
                         const float4* V2, uint32_t* count) {

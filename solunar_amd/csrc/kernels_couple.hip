@@ -404,7 +404,7 @@ __global__ __launch_bounds__(256) void k_extent_wet(const uint32_t* __restrict__
     int64_t z = (int64_t)(k % n[2]), rest = (int64_t)(k / n[2]);
     int64_t y = rest % n[1], x = rest / n[1];
     uint32_t s, f;
-    if (grid_range(g, lo[0] + x, lo[1] + y, lo[2] + z, s, f)) {
+    if (grid_range(g, lo[0] + x, lo[1] + y, lo[2] + z, s, f) && s < f) {
       e[w.body].wet = 1;
       return;
     }

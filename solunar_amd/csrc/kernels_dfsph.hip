@@ -168,8 +168,8 @@ __device__ __forceinline__ void store_group(const StepParams& prm, uint32_t i, u
 // than 20 fluid + wall neighbours. A particle with more listed neighbours
 // than the lists hold, or a neighbour past its cell mask's width, flags the
 // world; later launches of the substep then do nothing and the host regrows
-// or widens.
-template <int FORMAT>
+// or widens. KEYED: the walls' table is searched.
+template <int FORMAT, bool KEYED>
 __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restrict__ P, float4* __restrict__ V,
                                                        float* __restrict__ alpha, StepParams prm) {
   Control* ctl = prm.ctl;
@@ -222,23 +222,17 @@ __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restri
         const uint32_t z0 = (uint32_t)((int64_t)cz - lo[2]) / depth * depth;
         za = z0 > 0 ? z0 - 1 : 0;
       }
-      // lists: the table may be searched; compact: always dense
-      const GridView grid = fluid_grid(prm);
+      // a DFSPH world's water table is always dense
       for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
           for (int dx = -1; dx <= 1; ++dx) {
+            int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
+            if (c < 0) continue;
             const uint32_t code = FORMAT == kCompact ? (uint32_t)((dx + 1) * 4 + (dy + 1))
                                                      : (uint32_t)((dx + 1) * 3 + (dy + 1));
-            uint32_t first = 0, s0, e0;
-            if constexpr (FORMAT == kCompact) {
-              int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
-              if (c < 0) continue;
-              first = prm.cell_start[c - ((int64_t)cz + dz - lo[2]) + za];
-              s0 = prm.cell_start[c];
-              e0 = prm.cell_start[c + 1];
-            } else {
-              if (!grid_range(grid, (int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, s0, e0)) continue;
-            }
+            uint32_t first = 0;
+            if constexpr (FORMAT == kCompact) first = prm.cell_start[c - ((int64_t)cz + dz - lo[2]) + za];
+            uint32_t s0 = prm.cell_start[c], e0 = prm.cell_start[c + 1];
             for (uint32_t j = s0; j < e0; ++j) {
               float4 pj = P[j];
               float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
@@ -299,7 +293,7 @@ __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restri
     }
     const uint32_t wall0 = FORMAT == kLists ? nf : 0u;
     if (prm.wall_cells)
-      visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach,
+      visit_walls<KEYED>(prm, at,
                  [&](uint32_t w, float4, float ox, float oy, float oz, float sq) {
                    float distance = sqrtf(sq);
                    wsum += kernel_value(distance, prm.radius, prm.scale);
@@ -1648,9 +1642,25 @@ static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)(
       hipLaunchKernelGGL(KERNEL<kLists>, __VA_ARGS__);                            \
   } while (0)
 
+// The same with a second template argument after the format.
+#define SFL_FORMAT_LAUNCH_WITH(KERNEL, format, ARG, ...)                          \
+  do {                                                                            \
+    if ((format) == kMasks16)                                                     \
+      hipLaunchKernelGGL((KERNEL<kMasks16, ARG>), __VA_ARGS__);                   \
+    else if ((format) == kMasks32)                                                \
+      hipLaunchKernelGGL((KERNEL<kMasks32, ARG>), __VA_ARGS__);                   \
+    else if ((format) == kCompact)                                                \
+      hipLaunchKernelGGL((KERNEL<kCompact, ARG>), __VA_ARGS__);                   \
+    else                                                                          \
+      hipLaunchKernelGGL((KERNEL<kLists, ARG>), __VA_ARGS__);                     \
+  } while (0)
+
 hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm) {
   if (prm.n == 0) return hipSuccess;
-  SFL_FORMAT_LAUNCH(k_dfsph_density, prm.format, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
+  if (prm.wall_keys)
+    SFL_FORMAT_LAUNCH_WITH(k_dfsph_density, prm.format, true, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
+  else
+    SFL_FORMAT_LAUNCH_WITH(k_dfsph_density, prm.format, false, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
   return hipGetLastError();
 }
 

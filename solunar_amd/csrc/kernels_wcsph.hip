@@ -12,7 +12,8 @@ namespace sfl {
 // compute_densities (src/fluid/mod.rs:621-675) for every particle, then the
 // pressure over density squared of step.rs:151-172. rho goes to V.w, p/rho^2
 // to `scaled`. Also folds max(rho/rest - 1, 0) for the report (step.rs:84-88).
-template <bool MOVERS>
+// SEARCH: a table (water's or walls') may be searched rather than dense.
+template <bool MOVERS, bool SEARCH>
 __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict__ P, float4* __restrict__ V,
                                                        float* __restrict__ scaled, StepParams prm) {
   Control* ctl = prm.ctl;
@@ -23,16 +24,14 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
   if (live) {
     float4 at = P[i];
     float sum = 0.0f;
-    visit_near(P, fluid_grid(prm), at, prm.radius, prm.reach,
-               [&](uint32_t, float4, float, float, float, float sq) {
-                 sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
-               });
+    visit_fluid<SEARCH>(P, prm, at, [&](uint32_t, float4, float, float, float, float sq) {
+      sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+    });
     float wsum = 0.0f;
     if (prm.wall_cells)
-      visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach,
-                 [&](uint32_t, float4, float, float, float, float sq) {
-                   wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
-                 });
+      visit_walls<SEARCH>(prm, at, [&](uint32_t, float4, float, float, float, float sq) {
+        wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+      });
     float density;
     if (MOVERS) {
       // the coupled bodies' points (mod.rs:648-656); their count sizes the
@@ -67,27 +66,26 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
 // walls, the coupled bodies' points (mover_push, src/fluid/couple.rs:
 // 368-408), then + wall_viscous (step.rs:262-301) + gravity. With RECORD,
 // each body-point pair's damping on its body goes to the pair records.
-template <bool MOVERS, bool RECORD>
+template <bool MOVERS, bool RECORD, bool SEARCH>
 __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                            const float* __restrict__ scaled, const StepParams& prm, uint32_t i,
                                            float4 at, float4 vi, float& ax, float& ay, float& az) {
   float density = vi.w;
   float own = scaled[i];
   float px = 0.0f, py = 0.0f, pz = 0.0f;
-  visit_near(PB, fluid_grid(prm), at, prm.radius, prm.reach,
-             [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-               if (sq <= 0.0f) return;
-               float distance = sqrtf(sq);
-               float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
-               float gx = ox * f, gy = oy * f, gz = oz * f;
-               float4 vj = VB[j];
-               float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
-               float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
-               float k = prm.mass * (own + scaled[j] + damp);
-               px = px - gx * k;
-               py = py - gy * k;
-               pz = pz - gz * k;
-             });
+  visit_fluid<SEARCH>(PB, prm, at, [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+    if (sq <= 0.0f) return;
+    float distance = sqrtf(sq);
+    float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
+    float gx = ox * f, gy = oy * f, gz = oz * f;
+    float4 vj = VB[j];
+    float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
+    float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
+    float k = prm.mass * (own + scaled[j] + damp);
+    px = px - gx * k;
+    py = py - gy * k;
+    pz = pz - gz * k;
+  });
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
   float mirrored = 0.0f;
   if (MOVERS || prm.wall_cells || prm.terrain.count) {
@@ -96,7 +94,7 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
   }
   auto walls = [&](auto&& f) {
     if (prm.wall_cells)
-      visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach, f);
+      visit_walls<SEARCH>(prm, at, f);
   };
   float awx = 0.0f, awy = 0.0f, awz = 0.0f;
   if (prm.wall_cells) {
@@ -169,7 +167,7 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
 // (step.rs:305-339), for every particle. Reads the sorted state PB/VB, writes
 // PA/VA; `forces` (optional) keeps the push, `clamps` (optional) the
 // clamped components of the advance (bits x, y, z).
-template <bool MOVERS>
+template <bool MOVERS, bool SEARCH>
 __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                      const float* __restrict__ scaled, StepParams prm,
                                                      float4* __restrict__ PA, float4* __restrict__ VA,
@@ -185,7 +183,7 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
     float4 vi = VB[i];
     float density = vi.w;
     float ax, ay, az;
-    wcsph_push<MOVERS, MOVERS>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
+    wcsph_push<MOVERS, MOVERS, SEARCH>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
     if (forces) {
       forces[3 * i] = ax;
       forces[3 * i + 1] = ay;
@@ -214,7 +212,7 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
 // one, undoing what the fused pass wrote past it: the reference stops its
 // advance there (step.rs:333-335), leaving later particles at their old
 // position with the pushed, unclamped velocity. Recomputes that velocity.
-template <bool MOVERS>
+template <bool MOVERS, bool SEARCH>
 __global__ __launch_bounds__(256) void k_wcsph_repair(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                       const float* __restrict__ scaled, StepParams prm,
                                                       float4* __restrict__ PA, float4* __restrict__ VA) {
@@ -230,39 +228,60 @@ __global__ __launch_bounds__(256) void k_wcsph_repair(const float4* __restrict__
   float4 vi = VB[i];
   float density = vi.w;
   float ax, ay, az;
-  wcsph_push<MOVERS, false>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
+  wcsph_push<MOVERS, false, SEARCH>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
   PA[i] = at;
   VA[i] = make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density);
 }
 
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
+// Whether a table of the substep is searched (the kernels' dense-only
+// instantiations skip the check in the neighbour walk).
+static inline bool searched(const StepParams& prm) { return prm.cell_keys || prm.wall_keys; }
+
 void launch_wcsph_density(hipStream_t s, const float4* P, float4* V, float* scaled, const StepParams& prm) {
   if (prm.n == 0) return;
-  if (prm.movers.count)
-    hipLaunchKernelGGL(k_wcsph_density<true>, blocks_for(prm.n, 256), 256, 0, s, P, V, scaled, prm);
+  const uint32_t blocks = blocks_for(prm.n, 256);
+  const bool m = prm.movers.count != 0, k = searched(prm);
+  if (m && k)
+    hipLaunchKernelGGL((k_wcsph_density<true, true>), blocks, 256, 0, s, P, V, scaled, prm);
+  else if (m)
+    hipLaunchKernelGGL((k_wcsph_density<true, false>), blocks, 256, 0, s, P, V, scaled, prm);
+  else if (k)
+    hipLaunchKernelGGL((k_wcsph_density<false, true>), blocks, 256, 0, s, P, V, scaled, prm);
   else
-    hipLaunchKernelGGL(k_wcsph_density<false>, blocks_for(prm.n, 256), 256, 0, s, P, V, scaled, prm);
+    hipLaunchKernelGGL((k_wcsph_density<false, false>), blocks, 256, 0, s, P, V, scaled, prm);
 }
 
 void launch_wcsph_force(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
                         const StepParams& prm, float4* PA, float4* VA, float* forces, uint8_t* clamps) {
   if (prm.n == 0) return;
-  if (prm.movers.count)
-    hipLaunchKernelGGL(k_wcsph_force<true>, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
-                       clamps);
+  const uint32_t blocks = blocks_for(prm.n, 256);
+  const bool m = prm.movers.count != 0, k = searched(prm);
+  if (m && k)
+    hipLaunchKernelGGL((k_wcsph_force<true, true>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces, clamps);
+  else if (m)
+    hipLaunchKernelGGL((k_wcsph_force<true, false>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces, clamps);
+  else if (k)
+    hipLaunchKernelGGL((k_wcsph_force<false, true>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces, clamps);
   else
-    hipLaunchKernelGGL(k_wcsph_force<false>, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
+    hipLaunchKernelGGL((k_wcsph_force<false, false>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
                        clamps);
 }
 
 void launch_wcsph_repair(hipStream_t s, const float4* PB, const float4* VB, const float* scaled,
                          const StepParams& prm, float4* PA, float4* VA) {
   if (prm.n == 0) return;
-  if (prm.movers.count)
-    hipLaunchKernelGGL(k_wcsph_repair<true>, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA);
+  const uint32_t blocks = blocks_for(prm.n, 256);
+  const bool m = prm.movers.count != 0, k = searched(prm);
+  if (m && k)
+    hipLaunchKernelGGL((k_wcsph_repair<true, true>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA);
+  else if (m)
+    hipLaunchKernelGGL((k_wcsph_repair<true, false>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA);
+  else if (k)
+    hipLaunchKernelGGL((k_wcsph_repair<false, true>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA);
   else
-    hipLaunchKernelGGL(k_wcsph_repair<false>, blocks_for(prm.n, 256), 256, 0, s, PB, VB, scaled, prm, PA, VA);
+    hipLaunchKernelGGL((k_wcsph_repair<false, false>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA);
 }
 
 }  // namespace sfl
