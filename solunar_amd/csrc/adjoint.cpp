@@ -32,9 +32,24 @@ Tape::~Tape() {
   if (w_) {
     DeviceGuard g(w_->device_);
     if (w_->stream_) (void)hipStreamSynchronize(w_->stream_);
+    if (replay_stream_) (void)hipStreamSynchronize(replay_stream_);
   }
   if (ctl_) (void)hipFree(ctl_);
   if (ctl_host_) (void)hipHostFree(ctl_host_);
+  if (ready_) (void)hipEventDestroy(ready_);
+  for (hipEvent_t e : reversed_)
+    if (e) (void)hipEventDestroy(e);
+  if (replay_stream_) (void)hipStreamDestroy(replay_stream_);
+}
+
+// The replay stream (default priority, like the world's) and the events that
+// order it against the world's stream.
+int32_t Tape::ensure_streams() {
+  if (!replay_stream_) SFL_HIP(hipStreamCreateWithFlags(&replay_stream_, hipStreamNonBlocking), "replay stream");
+  if (!ready_) SFL_HIP(hipEventCreateWithFlags(&ready_, hipEventDisableTiming), "replay event");
+  for (hipEvent_t& e : reversed_)
+    if (!e) SFL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "replay event");
+  return SFL_OK;
 }
 
 // `steps` calls of FluidWorld::step (src/fluid/step.rs:58-90) on the world,
@@ -124,18 +139,26 @@ size_t Tape::device_bytes() const {
   size_t total = sizeof(Control);
   for (auto& c : ckp_) total += b(*c);
   for (auto& c : ckv_) total += b(*c);
-  for (auto& sl : slots_)
-    total += b(sl->P) + b(sl->V) + b(sl->scaled) + b(sl->clamps) + b(sl->src) + b(sl->cells);
-  return total + b(boxes_) + b(ra_p_) + b(ra_v_) + b(xb_) + b(vb_) + b(xb2_) + b(vb2_) + b(ab_) + b(rb_) +
+  for (const auto& set : slots_)
+    for (auto& sl : set) total += b(sl->P) + b(sl->V) + b(sl->scaled) + b(sl->clamps) + b(sl->src) + b(sl->cells);
+  return total + b(boxes_[0]) + b(boxes_[1]) + b(ra_p_) + b(ra_v_) + b(xb_) + b(vb_) + b(xb2_) + b(vb2_) + b(ab_) + b(rb_) +
          b(bar_in_) + b(bar_out_) + b(nu_acc_) + b(counts_) + b(tile_sums_) + b(scan_total_) + b(slot_rank_) +
          b(scatter_);
 }
 
-int32_t Tape::ensure_slots(uint32_t count) {
-  hipStream_t s = w_->stream_;
-  while (slots_.size() < count) slots_.emplace_back(new Slot());
+// Slot set `set` (written only by the replay stream) for `count` substeps.
+// Growing a buffer frees the old one at once, on the host's clock, while the
+// reverse kernels of an earlier segment may still read this set: growth
+// waits for the world's stream first.
+int32_t Tape::ensure_slots(uint32_t set, uint32_t count) {
+  hipStream_t s = replay_stream_;
+  std::vector<std::unique_ptr<Slot>>& slots = slots_[set];
+  bool grows = boxes_[set].ptr && boxes_[set].cap < count;
+  for (size_t k = 0; k < slots.size() && k < count; ++k) grows |= slots[k]->cells.ptr && slots[k]->cells.cap < cell_cap_ + 1;
+  if (grows) SFL_HIP(hipStreamSynchronize(w_->stream_), "replay");
+  while (slots.size() < count) slots.emplace_back(new Slot());
   for (uint32_t k = 0; k < count; ++k) {
-    Slot& sl = *slots_[k];
+    Slot& sl = *slots[k];
     SFL_HIP(sl.P.reserve(n_, 0, s), "hipMalloc(tape)");
     SFL_HIP(sl.V.reserve(n_, 0, s), "hipMalloc(tape)");
     SFL_HIP(sl.scaled.reserve(n_, 0, s), "hipMalloc(tape)");
@@ -146,22 +169,23 @@ int32_t Tape::ensure_slots(uint32_t count) {
       SFL_HIP(sl.cells.reserve(cell_cap_ + 1, 0, s), "hipMalloc(tape cells)");
     }
   }
-  SFL_HIP(boxes_.reserve(std::max<uint32_t>(count, 1), 0, s), "hipMalloc(tape boxes)");
+  SFL_HIP(boxes_[set].reserve(std::max<uint32_t>(count, 1), 0, s), "hipMalloc(tape boxes)");
   return SFL_OK;
 }
 
-// Replays segment c from its checkpoint with the world's kernels, keeping
-// every substep in a slot; `count` gets the number of substeps.
-int32_t Tape::replay(uint32_t c, uint32_t* count) {
+// Replays segment c from its checkpoint with the world's kernels on the
+// replay stream, keeping every substep in a slot of set `set`; `count` gets
+// the number of substeps. Waits for the replay (its check reads the result).
+int32_t Tape::replay(uint32_t c, uint32_t set, uint32_t* count) {
   World& w = *w_;
-  hipStream_t s = w.stream_;
+  hipStream_t s = replay_stream_;
   uint32_t first = c * every_, last = std::min(steps_, first + every_);
   uint32_t total = 0;
   for (uint32_t k = first; k < last; ++k) total += substeps_[k];
   *count = total;
   uint32_t n = (uint32_t)n_;
   for (int attempt = 0;; ++attempt) {
-    int32_t st = ensure_slots(total);
+    int32_t st = ensure_slots(set, total);
     if (st != SFL_OK) return st;
     SFL_HIP(hipMemcpyAsync(ra_p_.ptr, ckp_[c]->ptr, n_ * sizeof(float4), hipMemcpyDeviceToDevice, s), "replay");
     SFL_HIP(hipMemcpyAsync(ra_v_.ptr, ckv_[c]->ptr, n_ * sizeof(float4), hipMemcpyDeviceToDevice, s), "replay");
@@ -171,7 +195,7 @@ int32_t Tape::replay(uint32_t c, uint32_t* count) {
       StepParams prm = w.params(small_[k]);
       prm.ctl = ctl_;
       for (uint32_t j = 0; j < substeps_[k]; ++j, ++slot) {
-        Slot& sl = *slots_[slot];
+        Slot& sl = *slots_[set][slot];
         prm.cell_start = sl.cells.ptr;
         prm.cell_keys = nullptr;  // the tape's tables are dense
         launch_substep_begin(s, ctl_);
@@ -182,7 +206,7 @@ int32_t Tape::replay(uint32_t c, uint32_t* count) {
         launch_scatter(s, ra_p_.ptr, n, slot_rank_.ptr, sl.cells.ptr, scatter_.ptr, ctl_);
         launch_rank_gather(s, ra_p_.ptr, ra_v_.ptr, n, w.radius_, scatter_.ptr, sl.cells.ptr, sl.P.ptr, sl.V.ptr, ctl_,
                            sl.src.ptr);
-        launch_adj_keep_box(s, ctl_, boxes_.ptr + slot);
+        launch_adj_keep_box(s, ctl_, boxes_[set].ptr + slot);
         launch_wcsph_density(s, sl.P.ptr, sl.V.ptr, sl.scaled.ptr, prm);
         launch_wcsph_force(s, sl.P.ptr, sl.V.ptr, sl.scaled.ptr, prm, ra_p_.ptr, ra_v_.ptr, nullptr, sl.clamps.ptr);
       }
@@ -223,10 +247,10 @@ int32_t Tape::replay(uint32_t c, uint32_t* count) {
   }
 }
 
-// The adjoint kernels over segment c's substeps, last to first. On entry
-// xb_/vb_ hold the adjoints of the segment's final state (in its order); on
-// exit, of its first state.
-int32_t Tape::reverse(uint32_t c, uint32_t count) {
+// The adjoint kernels over segment c's substeps (slot set `which`), last to
+// first, on the world's stream. On entry xb_/vb_ hold the adjoints of the
+// segment's final state (in its order); on exit, of its first state.
+int32_t Tape::reverse(uint32_t c, uint32_t which, uint32_t count) {
   World& w = *w_;
   hipStream_t s = w.stream_;
   const sfl_settings& set = w.settings_;
@@ -236,7 +260,7 @@ int32_t Tape::reverse(uint32_t c, uint32_t count) {
   for (uint32_t k = first; k < last; ++k)
     for (uint32_t j = 0; j < substeps_[k]; ++j) step_of.push_back(k);
   for (int slot = (int)count - 1; slot >= 0; --slot) {
-    Slot& sl = *slots_[slot];
+    Slot& sl = *slots_[which][slot];
     float small = small_[step_of[slot]];
     StepParams prm = w.params(small);
     float r = w.radius_, cs = set.sound_speed;
@@ -260,7 +284,7 @@ int32_t Tape::reverse(uint32_t c, uint32_t count) {
     a.clamps = sl.clamps.ptr;
     a.src = sl.src.ptr;
     a.cell_start = sl.cells.ptr;
-    a.box = boxes_.ptr + slot;
+    a.box = boxes_[which].ptr + slot;
     a.walls = prm.walls;
     a.wall_start = prm.wall_start;
     a.wall_keys = prm.wall_keys;
@@ -314,13 +338,38 @@ int32_t Tape::adjoint(const float* xbar, const float* vbar, float* xbar0, float*
   SFL_HIP(hipMemcpyAsync(bar_in_.ptr, xbar, 3 * n_ * sizeof(float), hipMemcpyHostToDevice, s), "upload");
   SFL_HIP(hipMemcpyAsync(bar_in_.ptr + 3 * n_, vbar, 3 * n_ * sizeof(float), hipMemcpyHostToDevice, s), "upload");
   launch_adj_load(s, (uint32_t)n_, w.pa_.ptr, bar_in_.ptr, bar_in_.ptr + 3 * n_, xb_.ptr, vb_.ptr);
-  uint32_t segments = (uint32_t)ckp_.size();
-  for (int c = (int)segments - 1; c >= 0; --c) {
-    uint32_t count = 0;
-    int32_t st = replay((uint32_t)c, &count);
-    if (st != SFL_OK) return st;
-    st = reverse((uint32_t)c, count);
-    if (st != SFL_OK) return st;
+  // Segment c's reverse kernels (world stream, set c % 2) run beside the
+  // replay of segment c - 1 (replay stream, the other set). A replay waits
+  // for the set-up above and for the reverse that last read its set; a
+  // reverse waits for its replay. On an error, both streams drain first.
+  int32_t st = ensure_streams();
+  if (st != SFL_OK) return st;
+  hipStream_t r = replay_stream_;
+  auto fail = [&](int32_t code) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(r);
+    return code;
+  };
+  SFL_HIP(hipEventRecord(ready_, s), "replay event");
+  SFL_HIP(hipStreamWaitEvent(r, ready_, 0), "replay event");
+  const uint32_t segments = (uint32_t)ckp_.size();
+  uint32_t count[2] = {0, 0};
+  uint32_t c = segments - 1;
+  st = replay(c, c % 2, &count[c % 2]);
+  if (st != SFL_OK) return fail(st);
+  for (;;) {
+    const uint32_t set = c % 2;
+    SFL_HIP(hipEventRecord(ready_, r), "replay event");
+    SFL_HIP(hipStreamWaitEvent(s, ready_, 0), "replay event");
+    st = reverse(c, set, count[set]);
+    if (st != SFL_OK) return fail(st);
+    SFL_HIP(hipEventRecord(reversed_[set], s), "replay event");
+    if (c == 0) break;
+    --c;
+    // set c % 2 was last read by the reverse of segment c + 2
+    SFL_HIP(hipStreamWaitEvent(r, reversed_[c % 2], 0), "replay event");
+    st = replay(c, c % 2, &count[c % 2]);
+    if (st != SFL_OK) return fail(st);
   }
   launch_adj_store(s, (uint32_t)n_, ckp_[0]->ptr, xb_.ptr, vb_.ptr, bar_out_.ptr, bar_out_.ptr + 3 * n_);
   std::vector<double> acc(blocks);

@@ -3,7 +3,8 @@
 // the reverse pass (each segment replayed from its checkpoint with the same
 // kernels, keeping per substep the sorted state, cell table, sort
 // permutation and clamp flags, then the hand-written adjoint kernels run
-// back through it).
+// back through it). The replay of the segment before runs on a stream of
+// its own beside the reverse kernels of the current one.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -67,9 +68,10 @@ class Tape {
  private:
   Tape() = default;
   int32_t device_error(const char* what, hipError_t e);
-  int32_t replay(uint32_t segment, uint32_t* count);
-  int32_t ensure_slots(uint32_t count);
-  int32_t reverse(uint32_t segment, uint32_t count);
+  int32_t replay(uint32_t segment, uint32_t set, uint32_t* count);
+  int32_t ensure_slots(uint32_t set, uint32_t count);
+  int32_t reverse(uint32_t segment, uint32_t set, uint32_t count);
+  int32_t ensure_streams();
 
   struct Slot {
     DeviceBuffer<float4> P, V;
@@ -86,8 +88,13 @@ class Tape {
   std::vector<uint32_t> substeps_;  // per step
   std::vector<float> small_;        // substep length per step
   std::vector<std::unique_ptr<DeviceBuffer<float4>>> ckp_, ckv_;  // checkpoints (x, v) per segment
-  std::vector<std::unique_ptr<Slot>> slots_;
-  DeviceBuffer<CellBox> boxes_;
+  // Two slot sets: segment c replays into set c % 2 on the replay stream
+  // while the reverse kernels read the other set on the world's stream.
+  std::vector<std::unique_ptr<Slot>> slots_[2];
+  DeviceBuffer<CellBox> boxes_[2];
+  hipStream_t replay_stream_ = nullptr;
+  hipEvent_t ready_ = nullptr;        // world stream: set-up done / replay stream: a replay done
+  hipEvent_t reversed_[2] = {};       // reverse kernels done reading set k
   DeviceBuffer<float4> ra_p_, ra_v_;            // replay state
   DeviceBuffer<float4> xb_, vb_, xb2_, vb2_, ab_;  // adjoints
   DeviceBuffer<float> rb_, bar_in_, bar_out_;
