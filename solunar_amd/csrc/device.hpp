@@ -397,11 +397,6 @@ struct GridView {
   uint32_t cells;
 };
 
-__device__ __forceinline__ GridView dense_grid(const int32_t lo[3], const uint32_t span[3],
-                                               const uint32_t* start) {
-  return GridView{{lo[0], lo[1], lo[2]}, {span[0], span[1], span[2]}, start, nullptr, 0u};
-}
-
 // The fluid's table of the last sort (box in the control block).
 __device__ __forceinline__ GridView fluid_grid(const StepParams& prm) {
   const Control* ctl = prm.ctl;
@@ -476,20 +471,39 @@ template <class F>
 __device__ __forceinline__ void visit_dense(const float4* __restrict__ P, const uint32_t* __restrict__ start,
                                             const int32_t lo[3], const uint32_t span[3], float4 at, float radius,
                                             float reach, F&& f) {
-  int32_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        int64_t c = cell_slot((int64_t)cx + dx, (int64_t)cy + dy, (int64_t)cz + dz, lo, span);
-        if (c < 0) continue;
-        uint32_t s = start[c], e = start[c + 1];
-        for (uint32_t j = s; j < e; ++j) {
-          float4 pj = P[j];
-          float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
-          float sq = dot3(ox, oy, oz, ox, oy, oz);
-          if (sq < reach) f(j, pj, ox, oy, oz, sq);
-        }
+  const int32_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
+  // cell k of the 27 in the reference's order (dz, dy, dx); an empty range
+  // outside the box
+  auto range = [&](int k, uint32_t& s, uint32_t& e) {
+    const int64_t c = cell_slot((int64_t)cx + (k % 3 - 1), (int64_t)cy + ((k / 3) % 3 - 1),
+                                (int64_t)cz + (k / 9 - 1), lo, span);
+    s = c < 0 ? 0u : start[c];
+    e = c < 0 ? 0u : start[c + 1];
+  };
+  uint32_t s, e;
+  range(0, s, e);
+#pragma unroll
+  for (int k = 0; k < 27; ++k) {
+    // the next cell's range loads while this one is walked
+    uint32_t s2 = 0, e2 = 0;
+    if (k + 1 < 27) range(k + 1, s2, e2);
+    // four candidates' loads in flight at once (the last one repeated past
+    // the range), then their tests in order
+    for (uint32_t j = s; j < e; j += 4) {
+      float4 q[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) q[u] = P[min(j + u, e - 1u)];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        if (j + u >= e) break;
+        float ox = at.x - q[u].x, oy = at.y - q[u].y, oz = at.z - q[u].z;
+        float sq = dot3(ox, oy, oz, ox, oy, oz);
+        if (sq < reach) f(j + u, q[u], ox, oy, oz, sq);
       }
+    }
+    s = s2;
+    e = e2;
+  }
 }
 
 // The water's neighbours of `at` in the last sort's table, and the walls';

@@ -374,16 +374,16 @@ __global__ __launch_bounds__(BS) void k_b_density(BatchArgs b, const float4* __r
     uint32_t span[3] = {st.box.span[0], st.box.span[1], st.box.span[2]};
     float4 at = P[i];
     float sum = 0.0f;
-    visit_near(P, dense_grid(lo, span, prm.cell_start), at, prm.radius, prm.reach,
-               [&](uint32_t, float4, float, float, float, float sq) {
-                 sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
-               });
+    visit_dense(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+                [&](uint32_t, float4, float, float, float, float sq) {
+                  sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+                });
     float wsum = 0.0f;
-    if (prm.wall_cells)
-      visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach,
-                 [&](uint32_t, float4, float, float, float, float sq) {
-                   wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
-                 });
+    if (prm.wall_cells)  // a batch's walls are dense (searched ones are refused)
+      visit_dense(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
+                  [&](uint32_t, float4, float, float, float, float sq) {
+                    wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+                  });
     float density = prm.mass * (sum + wsum);
     float pressure = tait_pressure(density, prm.rest, prm.stiffness);
     V[i].w = density;
@@ -417,27 +417,27 @@ __global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __
     float density = vi.w;
     float own = scaled[i];
     float px = 0.0f, py = 0.0f, pz = 0.0f;
-    visit_near(PB, dense_grid(lo, span, prm.cell_start), at, prm.radius, prm.reach,
-               [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-                 if (sq <= 0.0f) return;
-                 float distance = sqrtf(sq);
-                 float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
-                 float gx = ox * f, gy = oy * f, gz = oz * f;
-                 float4 vj = VB[j];
-                 float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
-                 float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
-                 float kk = prm.mass * (own + scaled[j] + damp);
-                 px = px - gx * kk;
-                 py = py - gy * kk;
-                 pz = pz - gz * kk;
-               });
+    visit_dense(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+                [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+                  if (sq <= 0.0f) return;
+                  float distance = sqrtf(sq);
+                  float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
+                  float gx = ox * f, gy = oy * f, gz = oz * f;
+                  float4 vj = VB[j];
+                  float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
+                  float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
+                  float kk = prm.mass * (own + scaled[j] + damp);
+                  px = px - gx * kk;
+                  py = py - gy * kk;
+                  pz = pz - gz * kk;
+                });
     float wx = 0.0f, wy = 0.0f, wz = 0.0f;
     if (prm.wall_cells) {
       float pressure = tait_pressure(density, prm.rest, prm.stiffness);
       float mirrored = own + pressure / (prm.rest * prm.rest);
       float awx = 0.0f, awy = 0.0f, awz = 0.0f;
       auto walls = [&](auto&& f) {
-        visit_near(prm.walls, wall_grid(prm), at, prm.radius, prm.reach, f);
+        visit_dense(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach, f);
       };
       walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
         float gx, gy, gz;
