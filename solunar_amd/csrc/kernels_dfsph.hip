@@ -233,17 +233,25 @@ __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restri
             uint32_t first = 0;
             if constexpr (FORMAT == kCompact) first = prm.cell_start[c - ((int64_t)cz + dz - lo[2]) + za];
             uint32_t s0 = prm.cell_start[c], e0 = prm.cell_start[c + 1];
-            for (uint32_t j = s0; j < e0; ++j) {
-              float4 pj = P[j];
-              float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
-              float sq = dot3(ox, oy, oz, ox, oy, oz);
-              if (sq < prm.reach) {
-                if constexpr (FORMAT == kLists) {
-                  fluid(j | code << kListIndexBits, ox, oy, oz, sq);
-                } else {
-                  const uint32_t off = j - first;
-                  wide = wide || off > kNearMask;
-                  fluid(code << kNearBits | (off & kNearMask), ox, oy, oz, sq);
+            // four candidates' loads in flight at once, tested in order
+            for (uint32_t j0 = s0; j0 < e0; j0 += 4) {
+              float4 q[4];
+#pragma unroll
+              for (uint32_t u = 0; u < 4; ++u) q[u] = P[min(j0 + u, e0 - 1u)];
+#pragma unroll
+              for (uint32_t u = 0; u < 4; ++u) {
+                if (j0 + u >= e0) break;
+                const uint32_t j = j0 + u;
+                float ox = at.x - q[u].x, oy = at.y - q[u].y, oz = at.z - q[u].z;
+                float sq = dot3(ox, oy, oz, ox, oy, oz);
+                if (sq < prm.reach) {
+                  if constexpr (FORMAT == kLists) {
+                    fluid(j | code << kListIndexBits, ox, oy, oz, sq);
+                  } else {
+                    const uint32_t off = j - first;
+                    wide = wide || off > kNearMask;
+                    fluid(code << kNearBits | (off & kNearMask), ox, oy, oz, sq);
+                  }
                 }
               }
             }

@@ -146,3 +146,31 @@ def test_the_batch_and_the_tape_refuse_a_searched_world():
     with pytest.raises(PhysicsError) as e:
         Tape(w, 1e-3, 2, every=1)
     assert e.value.code == UNSUPPORTED
+
+
+def test_coupled_bodies_over_two_far_pools_push_as_the_oracle_does():
+    # a fast ball over the far pool asks the searched table whether water
+    # is in its reach (the sound speed of the step depends on it); a slow
+    # box sits in the near pool
+    from solunar_amd.types import BodyPose, Shape
+
+    tick = 1.0 / 60.0
+
+    def build(w):
+        for x0 in (0.0, FAR):
+            w.add_box_container((x0, 0.0, x0), (x0 + 1.0, 1.2, x0 + 1.0))
+            w.fill_box((x0, 0.0, x0), (x0 + 1.0, 0.6, x0 + 1.0))
+        w.couple((1, 0), Shape.sphere(0.1))
+        w.couple((2, 0), Shape.cuboid((0.1, 0.08, 0.1)))
+
+    gpu, cpu = pair(FluidSettings(), build)
+    speed = 24.0
+    for k in range(4):
+        poses = [BodyPose((1, 0), (FAR + 0.5, 0.9 - speed * tick * k, FAR + 0.5), linear_velocity=(0.0, -speed, 0.0)),
+                 BodyPose((2, 0), (0.5, 0.55, 0.5), linear_velocity=(0.1, 0.0, 0.0))]
+        g = gpu.step_with_bodies(tick, poses)
+        c = cpu.step_with_bodies(tick, poses)
+        assert g[0] == c[0], k
+        assert g[1] == c[1], k
+    assert g[0].sound_speed == np.float32(4 * speed)
+    assert_same(gpu, cpu)
