@@ -479,7 +479,11 @@ int32_t World::lifts(float dt, const sfl_body_pose* poses, size_t count, sfl_bod
   };
   int32_t st = pull();
   if (st != SFL_OK) return st;
-  uint64_t near = 0, cols = 0, sort = 0;
+  // A body's columns go to a table of its column box (one slot a column)
+  // while the box holds at most 8 columns a water particle plus 65536 and
+  // the step's tables stay under kMaxColumns; a sparser box sorts its
+  // water's column keys instead (lift.rs:73-84 sorts them too).
+  uint64_t near = 0, cols = 0, sort = 0, sorted_most = 0;
   for (uint32_t b = 0; b < nb; ++b) {
     LiftState& l = ls[b];
     near += l.near;
@@ -490,29 +494,58 @@ int32_t World::lifts(float dt, const sfl_body_pose* poses, size_t count, sfl_bod
       span[0] = (int64_t)cell_of(l.high[0], column) - l.col_lo[0] + 1;
       span[1] = (int64_t)cell_of(l.high[2], column) - l.col_lo[1] + 1;
     }
-    uint64_t here = (uint64_t)span[0] * (uint64_t)span[1];
-    if (span[0] > (int64_t)kMaxColumns || span[1] > (int64_t)kMaxColumns || cols + here > kMaxColumns) {
-      detail_ = "a coupled body's water spreads over more columns than the lift reads";
-      return SFL_UNSUPPORTED;
-    }
-    l.col_span[0] = (uint32_t)span[0];
-    l.col_span[1] = (uint32_t)span[1];
-    l.column_offset = (uint32_t)cols;
-    cols += here;
+    const uint64_t here = (uint64_t)span[0] * (uint64_t)span[1];
+    const bool table = lift_tables_ && span[0] <= (int64_t)kMaxColumns && span[1] <= (int64_t)kMaxColumns &&
+                       here <= 8 * (uint64_t)l.near + 65536 && cols + here <= kMaxColumns;
+    l.sorted = table ? 0u : 1u;
+    l.tops = 0;
     uint64_t size = 1;
-    while (size < here) size <<= 1;
+    if (table) {
+      l.col_span[0] = (uint32_t)span[0];
+      l.col_span[1] = (uint32_t)span[1];
+      l.column_offset = (uint32_t)cols;
+      cols += here;
+      while (size < here) size <<= 1;
+    } else {
+      // at most one top a water particle
+      l.col_span[0] = l.col_span[1] = 0;
+      l.column_offset = 0;
+      sorted_most = std::max<uint64_t>(sorted_most, l.near);
+      while (size < l.near) size <<= 1;
+    }
     l.sort_offset = (uint32_t)sort;
     l.sort_size = (uint32_t)size;
     sort += size;
+    if (sort >= (uint64_t(1) << 32)) {
+      detail_ = "the coupled bodies read more water columns than the lift's 32-bit offsets address";
+      return SFL_UNSUPPORTED;
+    }
   }
   SFL_HIP(lift_near_.reserve(std::max<uint64_t>(near, 1), 0, stream_), "hipMalloc(lift water)");
-  SFL_HIP(lift_columns_.reserve(cols, 0, stream_), "hipMalloc(lift columns)");
+  SFL_HIP(lift_columns_.reserve(std::max<uint64_t>(cols, 1), 0, stream_), "hipMalloc(lift columns)");
   SFL_HIP(lift_sort_.reserve(sort, 0, stream_), "hipMalloc(lift levels)");
-  SFL_HIP(hipMemsetAsync(lift_columns_.ptr, 0, cols * sizeof(unsigned long long), stream_), "clear lift columns");
+  if (cols)
+    SFL_HIP(hipMemsetAsync(lift_columns_.ptr, 0, cols * sizeof(unsigned long long), stream_), "clear lift columns");
+  LiftSort lsort{};
+  if (sorted_most) {
+    const uint64_t room = std::max<uint64_t>(near, 1);
+    SFL_HIP(lift_ckeys_.reserve(room, 0, stream_), "hipMalloc(lift column keys)");
+    SFL_HIP(lift_ckeys2_.reserve(room, 0, stream_), "hipMalloc(lift column keys)");
+    SFL_HIP(lift_ukeys_.reserve(room, 0, stream_), "hipMalloc(lift column keys)");
+    SFL_HIP(lift_cvals_.reserve(room, 0, stream_), "hipMalloc(lift column tops)");
+    SFL_HIP(lift_cvals2_.reserve(room, 0, stream_), "hipMalloc(lift column tops)");
+    size_t temp = 0;
+    SFL_HIP(lift_sort_bytes((uint32_t)sorted_most, &temp), "lift column sort size");
+    SFL_HIP(lift_sort_temp_.reserve(std::max<size_t>(temp, 1), 0, stream_), "hipMalloc(lift column sort)");
+    lsort = LiftSort{lift_ckeys_.ptr, lift_cvals_.ptr, lift_ckeys2_.ptr, lift_cvals2_.ptr, lift_ukeys_.ptr,
+                     lift_sort_temp_.ptr, lift_sort_temp_.cap};
+  }
   st = push();
   if (st != SFL_OK) return st;
   launch_lift_near(stream_, pa_.ptr, (uint32_t)n_, lift_.ptr, nb, lift_boxes_.ptr, lift_offsets_.ptr, lift_near_.ptr,
-                   lift_columns_.ptr, column, up);
+                   lift_columns_.ptr, lsort.keys, lsort.values, column, up);
+  for (uint32_t b = 0; b < nb && sorted_most; ++b)
+    if (ls[b].sorted) SFL_HIP(launch_lift_tops(stream_, lsort, lift_.ptr, b, ls[b], lift_sort_.ptr), "lift column sort");
   launch_lift_level(stream_, pa_.ptr, va_.ptr, lift_.ptr, bodies_d_.ptr, nb, lift_near_.ptr, lift_columns_.ptr,
                     lift_sort_.ptr, probe_arm_.ptr, probe_vel_.ptr, settings_.spacing / 2.0f, up);
   st = pull();

@@ -51,6 +51,8 @@ struct LiftState {
   int32_t col_lo[2];             // first column (x, z) of the box
   uint32_t col_span[2];
   uint32_t column_offset;  // into the column table
+  uint32_t sorted;         // 1: the tops come from a sort of the water's column keys, not the table
+  uint32_t tops;           // sorted: the number of columns (written on the device)
   uint32_t sort_offset, sort_size;
   float level;
   float flow[3];
@@ -101,7 +103,24 @@ void launch_lift_count(hipStream_t s, const float4* P, uint32_t n, LiftState* ls
                        uint32_t* counts, uint32_t* offsets, uint32_t* tiles, uint32_t* total);
 void launch_lift_near(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies,
                       const float4* boxes, const uint32_t* offsets, uint32_t* near, unsigned long long* columns,
-                      float column, const float up[3]);
+                      uint64_t* column_keys, uint32_t* column_tops, float column, const float up[3]);
+// The buffers of the sorted column path (bodies whose column box is too
+// sparse for a table): the near water's column keys and heights, their
+// sorted copies, the unique keys, and rocPRIM's temporary storage.
+struct LiftSort {
+  uint64_t* keys;
+  uint32_t* values;
+  uint64_t* keys_sorted;
+  uint32_t* values_sorted;
+  uint64_t* unique;
+  void* temp;
+  size_t temp_bytes;
+};
+hipError_t lift_sort_bytes(uint32_t most, size_t* bytes);
+// Body b's column tops into sortbuf + sort_offset and their count into its
+// LiftState (the host's copy `l` gives the offsets and the count of water).
+hipError_t launch_lift_tops(hipStream_t s, const LiftSort& t, LiftState* ls, uint32_t b, const LiftState& l,
+                            uint32_t* sortbuf);
 void launch_lift_level(hipStream_t s, const float4* P, const float4* V, LiftState* ls, const BodyState* bodies,
                        uint32_t nbodies, const uint32_t* near, const unsigned long long* columns, uint32_t* sortbuf,
                        const float4* arm, const float4* vel, float half_spacing, const float up[3]);

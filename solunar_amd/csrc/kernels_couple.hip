@@ -10,6 +10,7 @@
 // the library does (device.hpp). Sums the reference folds in order are
 // folded in order by one lane; only maxima, minima and counts use atomics.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
 
 #include "couple.hpp"
 #include "device.hpp"
@@ -557,7 +558,9 @@ __global__ __launch_bounds__(1024) void k_lift_near(const float4* __restrict__ P
                                                     const LiftState* __restrict__ ls, const float4* __restrict__ boxes,
                                                     uint32_t chunks, const uint32_t* __restrict__ offsets,
                                                     uint32_t* __restrict__ near, unsigned long long* __restrict__ columns,
-                                                    float column, float upx, float upy, float upz) {
+                                                    uint64_t* __restrict__ column_keys,
+                                                    uint32_t* __restrict__ column_tops, float column, float upx,
+                                                    float upy, float upz) {
   const uint32_t b = blockIdx.x, c = blockIdx.y;
   const LiftState& l = ls[b];
   const size_t at = (size_t)b * chunks + c;
@@ -579,10 +582,16 @@ __global__ __launch_bounds__(1024) void k_lift_near(const float4* __restrict__ P
   uint32_t offset = offsets[at];
   for (uint32_t w = 0; w < wave; ++w) offset += wave_sums[w];
   near[offset + before] = i;
-  unsigned long long* table = columns + l.column_offset;
   int64_t kx = (int64_t)cell_coord(p.x, column) - l.col_lo[0];
   int64_t kz = (int64_t)cell_coord(p.z, column) - l.col_lo[1];
   float height = dot3(p.x, p.y, p.z, upx, upy, upz);
+  if (l.sorted) {
+    // the column as (x, z) in the reference's key order, for the sort
+    column_keys[offset + before] = (uint64_t)(uint32_t)kx << 32 | (uint32_t)kz;
+    column_tops[offset + before] = total_key(height);
+    return;
+  }
+  unsigned long long* table = columns + l.column_offset;
   atomicMax(&table[kx * l.col_span[1] + kz], (1ull << 32) | total_key(height));
 }
 
@@ -609,7 +618,8 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
   if (threadIdx.x == 0) base_s = 0;
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint64_t first = 0; first < cols; first += 1024) {
+  // a sorted body's tops are already in place
+  for (uint64_t first = 0; !l.sorted && first < cols; first += 1024) {
     uint64_t c = first + threadIdx.x;
     unsigned long long t = c < cols ? table[c] : 0ull;
     bool full = t != 0ull;
@@ -628,7 +638,7 @@ __global__ __launch_bounds__(1024) void k_lift_level(const float4* __restrict__ 
     }
     __syncthreads();
   }
-  const uint32_t count = base_s;
+  const uint32_t count = l.sorted ? l.tops : base_s;
   uint32_t size = 1;  // a power of two, at most the buffer's sort_size
   while (size < count) size <<= 1;
   for (uint32_t k = count + threadIdx.x; k < size; k += 1024) keys[k] = 0xffffffffu;
@@ -922,11 +932,40 @@ void launch_lift_count(hipStream_t s, const float4* P, uint32_t n, LiftState* ls
 
 void launch_lift_near(hipStream_t s, const float4* P, uint32_t n, LiftState* ls, uint32_t nbodies,
                       const float4* boxes, const uint32_t* offsets, uint32_t* near, unsigned long long* columns,
-                      float column, const float up[3]) {
+                      uint64_t* column_keys, uint32_t* column_tops, float column, const float up[3]) {
   if (nbodies == 0 || n == 0) return;
   const uint32_t chunks = blocks_for(n, kLiftChunk);
   hipLaunchKernelGGL(k_lift_near, dim3(nbodies, chunks), 1024, 0, s, P, n, ls, boxes, chunks, offsets, near, columns,
-                     column, up[0], up[1], up[2]);
+                     column_keys, column_tops, column, up[0], up[1], up[2]);
+}
+
+hipError_t lift_sort_bytes(uint32_t most, size_t* bytes) {
+  size_t a = 0, b = 0;
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, a, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                           (const uint32_t*)nullptr, (uint32_t*)nullptr, most, 0, 64);
+  if (e != hipSuccess) return e;
+  e = rocprim::reduce_by_key(nullptr, b, (const uint64_t*)nullptr, (const uint32_t*)nullptr, (size_t)most,
+                             (uint64_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, rocprim::maximum<uint32_t>(),
+                             rocprim::equal_to<uint64_t>());
+  *bytes = std::max(a, b);
+  return e;
+}
+
+// lift.rs:73-84 for one body: its water's (column, height) pairs sorted by
+// column, then the highest of each column (total_cmp order is the order of
+// total_key), in column order; the level pass sorts them by height.
+hipError_t launch_lift_tops(hipStream_t s, const LiftSort& t, LiftState* ls, uint32_t b, const LiftState& l,
+                            uint32_t* sortbuf) {
+  if (l.near == 0) return hipSuccess;
+  const size_t o = l.near_offset;
+  size_t bytes = t.temp_bytes;
+  hipError_t e = rocprim::radix_sort_pairs(t.temp, bytes, t.keys + o, t.keys_sorted + o, t.values + o,
+                                           t.values_sorted + o, l.near, 0, 64, s);
+  if (e != hipSuccess) return e;
+  bytes = t.temp_bytes;
+  return rocprim::reduce_by_key(t.temp, bytes, t.keys_sorted + o, t.values_sorted + o, (size_t)l.near, t.unique + o,
+                                sortbuf + l.sort_offset, &ls[b].tops, rocprim::maximum<uint32_t>(),
+                                rocprim::equal_to<uint64_t>(), s);
 }
 
 void launch_lift_level(hipStream_t s, const float4* P, const float4* V, LiftState* ls, const BodyState* bodies,

@@ -197,6 +197,10 @@ int32_t World::init_device() {
   // count; a small value makes coupled steps outgrow their room (tests)
   const char* slack = std::getenv("SOLUNAR_BODY_PAIR_SLACK");
   if (slack) pair_slack_ = (uint32_t)std::min<long>(std::max(std::atol(slack), 0L), 1L << 30);
+  // SOLUNAR_LIFT_COLUMN_TABLES=0 reads every body's columns by sorting, the
+  // path a body too spread out for a column table takes (tests)
+  const char* tables = std::getenv("SOLUNAR_LIFT_COLUMN_TABLES");
+  if (tables && std::atoi(tables) == 0) lift_tables_ = false;
   if (solve_grid_ <= 0) {
     detail_ = "occupancy query for the persistent solve failed";
     return SFL_DEVICE_ERROR;
@@ -1772,6 +1776,7 @@ size_t World::device_bytes() const {
          b(pair_torque_) + b(pair_temp_) +
          b(extent_) + b(wet_) + b(lift_) + b(probe_arm_) + b(probe_vel_) + b(lift_near_) + b(lift_sort_) +
          b(lift_columns_) + b(lift_counts_) + b(lift_offsets_) + b(lift_tiles_) + b(lift_boxes_) +
+         b(lift_ckeys_) + b(lift_ckeys2_) + b(lift_ukeys_) + b(lift_cvals_) + b(lift_cvals2_) + b(lift_sort_temp_) +
          b(terrain_desc_) + b(terrain_dist_) + b(far_d_) + b(damping_d_) + b(sites_d_) + b(slopes_);
 }
 

@@ -372,6 +372,11 @@ class World {
   DeviceBuffer<uint32_t> lift_near_, lift_sort_, lift_counts_, lift_offsets_, lift_tiles_;
   DeviceBuffer<float4> lift_boxes_;
   DeviceBuffer<unsigned long long> lift_columns_;
+  // the sorted column path (bodies whose column box is too sparse for the table)
+  DeviceBuffer<uint64_t> lift_ckeys_, lift_ckeys2_, lift_ukeys_;
+  DeviceBuffer<uint32_t> lift_cvals_, lift_cvals2_;
+  DeviceBuffer<unsigned char> lift_sort_temp_;
+  bool lift_tables_ = true;  // false: every body takes the sorted path (tests)
 
   // measurement
   bool profiling_ = false;

@@ -235,3 +235,12 @@ def test_steps_that_outgrow_the_pair_room_carry_on_as_the_oracle_does(monkeypatc
             mids += mid
         assert gpu.state_hash() == cpu.state_hash()
     assert firsts >= 1 and mids >= 1
+
+
+def test_lifts_read_by_sorting_columns_push_as_the_oracle_does(monkeypatch):
+    # a body whose column box is too sparse for a column table sorts its
+    # water's column keys instead; here every body takes that path
+    monkeypatch.setenv("SOLUNAR_LIFT_COLUMN_TABLES", "0")
+    print("case: sorted lift columns", flush=True)
+    test_a_turning_box_and_a_kinematic_ball_push_the_water_as_the_oracle_does()
+    test_many_bodies_across_many_chunks()
