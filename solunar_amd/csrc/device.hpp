@@ -506,6 +506,59 @@ __device__ __forceinline__ void visit_dense(const float4* __restrict__ P, const 
   }
 }
 
+// visit_dense in two phases, for pair terms much dearer than the distance
+// test: the walk queues each lane's accepted candidates in `queue` (LDS,
+// slot k of a lane at queue[k * stride]) and f runs over the queued ones
+// whenever a lane's queue could fill within the next batch, and at the end.
+// The pair terms then run over accepted pairs only, most lanes busy, instead
+// of over every candidate any lane accepts. f sees the same (j, pj, o, sq)
+// in the same order as from visit_dense.
+template <uint32_t Q, class F>
+__device__ __forceinline__ void visit_dense_queued(const float4* __restrict__ P, const uint32_t* __restrict__ start,
+                                                   const int32_t lo[3], const uint32_t span[3], float4 at,
+                                                   float radius, float reach, uint32_t* queue, uint32_t stride,
+                                                   F&& f) {
+  static_assert(Q >= 8, "a queue holds at least two batches");
+  uint32_t n = 0;
+  auto flush = [&]() {
+    for (uint32_t k = 0; k < n; ++k) {
+      const uint32_t j = queue[k * stride];
+      const float4 pj = P[j];
+      const float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
+      f(j, pj, ox, oy, oz, dot3(ox, oy, oz, ox, oy, oz));
+    }
+    n = 0;
+  };
+  const int32_t cx = cell_coord(at.x, radius), cy = cell_coord(at.y, radius), cz = cell_coord(at.z, radius);
+  auto range = [&](int k, uint32_t& s, uint32_t& e) {
+    const int64_t c = cell_slot((int64_t)cx + (k % 3 - 1), (int64_t)cy + ((k / 3) % 3 - 1),
+                                (int64_t)cz + (k / 9 - 1), lo, span);
+    s = c < 0 ? 0u : start[c];
+    e = c < 0 ? 0u : start[c + 1];
+  };
+  uint32_t s, e;
+  range(0, s, e);
+  for (int k = 0; k < 27; ++k) {
+    uint32_t s2 = 0, e2 = 0;
+    if (k + 1 < 27) range(k + 1, s2, e2);
+    for (uint32_t j = s; j < e; j += 4) {
+      if (__any(n + 4 > Q)) flush();
+      float4 q[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) q[u] = P[min(j + u, e - 1u)];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        if (j + u >= e) break;
+        const float ox = at.x - q[u].x, oy = at.y - q[u].y, oz = at.z - q[u].z;
+        if (dot3(ox, oy, oz, ox, oy, oz) < reach) queue[n++ * stride] = j + u;
+      }
+    }
+    s = s2;
+    e = e2;
+  }
+  flush();
+}
+
 // The water's neighbours of `at` in the last sort's table, and the walls';
 // SEARCH = false for a step whose tables are all dense.
 template <bool SEARCH, class F>

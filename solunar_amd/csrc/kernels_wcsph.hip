@@ -9,6 +9,14 @@
 
 namespace sfl {
 
+// Queue slots a lane for the force pass's two-phase neighbour walk, and the
+// smallest launch that takes it: at 1M particles the force pass runs 362
+// instead of 388 us (16 slots; 8, 12, 24 and 32 gave 377, 367, 377 and 435);
+// at 128k and below the walk is bound by load latency and the queue's LDS
+// round trips cost more than the pair terms it saves (113 against 109 us).
+constexpr uint32_t kForceQueue = 16;
+constexpr uint32_t kForceQueueFrom = 262144;
+
 // compute_densities (src/fluid/mod.rs:621-675) for every particle, then the
 // pressure over density squared of step.rs:151-172. rho goes to V.w, p/rho^2
 // to `scaled`. Also folds max(rho/rest - 1, 0) for the report (step.rs:84-88).
@@ -66,14 +74,15 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
 // walls, the coupled bodies' points (mover_push, src/fluid/couple.rs:
 // 368-408), then + wall_viscous (step.rs:262-301) + gravity. With RECORD,
 // each body-point pair's damping on its body goes to the pair records.
-template <bool MOVERS, bool RECORD, bool SEARCH>
+template <bool MOVERS, bool RECORD, bool SEARCH, uint32_t QUEUE = 0>
 __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                            const float* __restrict__ scaled, const StepParams& prm, uint32_t i,
-                                           float4 at, float4 vi, float& ax, float& ay, float& az) {
+                                           float4 at, float4 vi, float& ax, float& ay, float& az,
+                                           uint32_t* queue = nullptr) {
   float density = vi.w;
   float own = scaled[i];
   float px = 0.0f, py = 0.0f, pz = 0.0f;
-  visit_fluid<SEARCH>(PB, prm, at, [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+  auto pair = [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
     if (sq <= 0.0f) return;
     float distance = sqrtf(sq);
     float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
@@ -85,7 +94,16 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
     px = px - gx * k;
     py = py - gy * k;
     pz = pz - gz * k;
-  });
+  };
+  if constexpr (QUEUE > 0 && !SEARCH) {
+    const Control* ctl = prm.ctl;
+    const int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
+    const uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
+    visit_dense_queued<QUEUE>(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach, queue + threadIdx.x,
+                              blockDim.x, pair);
+  } else {
+    visit_fluid<SEARCH>(PB, prm, at, pair);
+  }
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
   float mirrored = 0.0f;
   if (MOVERS || prm.wall_cells || prm.terrain.count) {
@@ -166,14 +184,16 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
 // (step.rs:262-301), v += push * dt (step.rs:212-222) and advance
 // (step.rs:305-339), for every particle. Reads the sorted state PB/VB, writes
 // PA/VA; `forces` (optional) keeps the push, `clamps` (optional) the
-// clamped components of the advance (bits x, y, z).
-template <bool MOVERS, bool SEARCH>
+// clamped components of the advance (bits x, y, z). QUEUE: slots a lane of
+// the two-phase walk (0: the one-phase walk).
+template <bool MOVERS, bool SEARCH, uint32_t QUEUE>
 __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ PB, const float4* __restrict__ VB,
                                                      const float* __restrict__ scaled, StepParams prm,
                                                      float4* __restrict__ PA, float4* __restrict__ VA,
                                                      float* __restrict__ forces, uint8_t* __restrict__ clamps) {
   Control* ctl = prm.ctl;
   if (ctl->abort) return;
+  __shared__ uint32_t queue[QUEUE ? QUEUE * 256 : 1];
   uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   bool live = i < prm.n;
   float speed = 0.0f;
@@ -183,7 +203,7 @@ __global__ __launch_bounds__(256) void k_wcsph_force(const float4* __restrict__ 
     float4 vi = VB[i];
     float density = vi.w;
     float ax, ay, az;
-    wcsph_push<MOVERS, MOVERS, SEARCH>(PB, VB, scaled, prm, i, at, vi, ax, ay, az);
+    wcsph_push<MOVERS, MOVERS, SEARCH, QUEUE>(PB, VB, scaled, prm, i, at, vi, ax, ay, az, queue);
     if (forces) {
       forces[3 * i] = ax;
       forces[3 * i + 1] = ay;
@@ -257,15 +277,23 @@ void launch_wcsph_force(hipStream_t s, const float4* PB, const float4* VB, const
                         const StepParams& prm, float4* PA, float4* VA, float* forces, uint8_t* clamps) {
   if (prm.n == 0) return;
   const uint32_t blocks = blocks_for(prm.n, 256);
-  const bool m = prm.movers.count != 0, k = searched(prm);
+  const bool m = prm.movers.count != 0, k = searched(prm), q = !k && prm.n >= kForceQueueFrom;
   if (m && k)
-    hipLaunchKernelGGL((k_wcsph_force<true, true>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces, clamps);
+    hipLaunchKernelGGL((k_wcsph_force<true, true, 0>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces, clamps);
+  else if (m && q)
+    hipLaunchKernelGGL((k_wcsph_force<true, false, kForceQueue>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA,
+                       forces, clamps);
   else if (m)
-    hipLaunchKernelGGL((k_wcsph_force<true, false>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces, clamps);
+    hipLaunchKernelGGL((k_wcsph_force<true, false, 0>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
+                       clamps);
   else if (k)
-    hipLaunchKernelGGL((k_wcsph_force<false, true>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces, clamps);
+    hipLaunchKernelGGL((k_wcsph_force<false, true, 0>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
+                       clamps);
+  else if (q)
+    hipLaunchKernelGGL((k_wcsph_force<false, false, kForceQueue>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA,
+                       forces, clamps);
   else
-    hipLaunchKernelGGL((k_wcsph_force<false, false>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
+    hipLaunchKernelGGL((k_wcsph_force<false, false, 0>), blocks, 256, 0, s, PB, VB, scaled, prm, PA, VA, forces,
                        clamps);
 }
 

@@ -244,3 +244,21 @@ def test_lifts_read_by_sorting_columns_push_as_the_oracle_does(monkeypatch):
     print("case: sorted lift columns", flush=True)
     test_a_turning_box_and_a_kinematic_ball_push_the_water_as_the_oracle_does()
     test_many_bodies_across_many_chunks()
+
+
+def test_bodies_in_a_large_pool_push_as_the_oracle_does():
+    # ~330k particles: past 262,144 the force pass with body points takes
+    # its two-phase walk for the water pairs
+    print("case: 2 bodies, 330k water", flush=True)
+    s = FluidSettings(spacing=0.01)
+    gpu, cpu = pair(s, size=(1.0, 1.0, 0.8), water=(1.0, 0.42, 0.8))
+    assert len(gpu) > 262144
+    for w in (gpu, cpu):
+        w.couple((1, 0), Shape.sphere(0.08))
+        w.couple((2, 0), Shape.cuboid((0.1, 0.06, 0.08)))
+    for k in range(2):
+        poses = [BodyPose((1, 0), (0.3, 0.42 - 0.02 * k, 0.4), linear_velocity=(0.0, -1.2, 0.0)),
+                 BodyPose((2, 0), (0.7, 0.4, 0.4), (0.0, 0.0998334, 0.0, 0.9950042), (0.2, 0.0, 0.0),
+                          (0.0, 0.5, 0.0))]
+        same(gpu, cpu, poses, k)
+    assert gpu.state_hash() == cpu.state_hash()

@@ -95,6 +95,25 @@ def test_one_wcsph_step_density_and_force_match():
     assert np.array_equal(gpu.velocities(), cpu.velocities())
 
 
+def test_a_large_wcsph_dam_break_steps_as_the_oracle_does():
+    # ~300k particles: past 262,144 the force pass takes its two-phase walk
+    s = replace(scenes.settings_dam_small(), spacing=0.0075)
+    gpu, cpu = pair(s)
+    for w in (gpu, cpu):
+        n = scenes.dam_break(w)
+    assert n > 262144
+    gpu.keep_forces(True)
+    cpu.keep_forces(True)
+    for k in range(3):
+        rg, rc = gpu.step(1e-4), cpu.step(1e-4)
+        assert rg == rc, (k, rg, rc)
+        assert np.array_equal(gpu.forces(), cpu.forces()), k
+    assert np.array_equal(gpu.ids(), cpu.ids())
+    assert np.array_equal(gpu.positions(), cpu.positions())
+    assert np.array_equal(gpu.velocities(), cpu.velocities())
+    assert np.array_equal(gpu.densities(), cpu.densities())
+
+
 def test_wcsph_dam_break_100_steps_roll_out_identically():
     gpu, cpu = pair(scenes.settings_dam_small())
     for w in (gpu, cpu):
