@@ -916,7 +916,10 @@ int32_t World::ensure_cells(uint64_t volume) {
     // too thin for a dense table: search the occupied cells, as the
     // reference does (mod.rs:214-217). The DFSPH stop sum runs over the
     // tiles of the dense box, so a DFSPH world keeps a dense table.
-    if (settings_.solver == SFL_WCSPH) return use_searched();
+    if (settings_.solver == SFL_WCSPH) {
+      last_box_ = volume;  // the box that sent it there, until a searched sort reports its own
+      return use_searched();
+    }
     if (volume > kMaxCells) {
       detail_ = "a DFSPH world spread over more cells than the dense cell table holds";
       return SFL_OUT_OF_RANGE;
