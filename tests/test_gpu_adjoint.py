@@ -95,6 +95,25 @@ def test_device_adjoint_matches_the_fp64_reference_on_a_small_scene():
     assert np.allclose(gv2, 2.0 * gv, rtol=1e-5, atol=0)
 
 
+def test_segments_of_several_substeps_reverse_as_the_fp64_reference_does():
+    # CFL substeps (dt 2e-3 at spacing 0.02) in segments of 3 steps: the
+    # replays fill the two slot sets with many substeps each while the
+    # reverse kernels of the segment after read the other set
+    x0, v0 = small_scene()
+    steps, dt, every = 12, 2e-3, 3
+    print("case: adjoint with substeps", flush=True)
+    ow, x32 = oracle_run(small_settings(0.05), SMALL_BOX, x0, v0, steps, dt, record=True, precision=32)
+    w, tape, xT = device_run(small_settings(0.05), SMALL_BOX, x0, v0, steps, dt, every)
+    assert np.array_equal(xT, x32), "the device rollout is the reference's f32 rollout"
+    assert tape.substeps() > 2 * steps
+    seed = np.random.default_rng(5).normal(0.0, 1e-3, size=x0.shape).astype(np.float32)
+    gx64, gv64, ga64 = ow.adjoint(seed)
+    gx, gv, ga = tape.adjoint(seed)
+    assert rel_norm(gv, gv64) < 1e-4
+    assert rel_norm(gx, gx64) < 1e-4
+    assert abs(ga - ga64) / abs(ga64) < 1e-4
+
+
 def test_the_rollout_harness_backpropagates_through_the_tape():
     x0, v0 = small_scene()
     steps, dt = 30, 1e-4
