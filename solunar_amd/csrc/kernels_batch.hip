@@ -393,6 +393,9 @@ __global__ __launch_bounds__(BS) void k_b_density(BatchArgs b, const float4* __r
   wave_max_to(&st.comp_bits, comp);
 }
 
+// Queue slots a lane of the force pass's two-phase walk.
+constexpr uint32_t kBatchQueue = 16;
+
 // The fused force, velocity and advance pass of the single world
 // (k_wcsph_force) for the scenes that step in this slot.
 template <uint32_t BS>
@@ -406,6 +409,7 @@ __global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __
   uint32_t i = batch_index<BS>(b, map, s, live);
   SceneState& st = b.state[s];
   if (!scene_active(b, st)) return;
+  __shared__ uint32_t queue[kBatchQueue * BS];
   float speed = 0.0f;
   uint32_t held = 0;
   if (live) {
@@ -417,7 +421,9 @@ __global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __
     float density = vi.w;
     float own = scaled[i];
     float px = 0.0f, py = 0.0f, pz = 0.0f;
-    visit_dense(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach,
+    // the single world's two-phase walk (visit_dense_queued): the launch
+    // covers every scene, so it is as full as a large world's
+    visit_dense_queued<kBatchQueue>(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach, queue + threadIdx.x, BS,
                 [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
                   if (sq <= 0.0f) return;
                   float distance = sqrtf(sq);
