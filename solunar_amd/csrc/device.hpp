@@ -382,6 +382,26 @@ __device__ __forceinline__ float tait_pressure(float density, float rest, float 
 
 namespace sfl {
 
+// A rank in cell `slot` for every live lane, distinct within the cell, from
+// one atomic per run of consecutive lanes in the same cell: particles come in
+// last substep's sorted order, so such runs are long. Every lane of the wave
+// calls it (dead lanes end runs and take no rank). The sort ranks cell-mates
+// by id afterwards, so which lane gets which rank does not matter.
+__device__ __forceinline__ uint32_t wave_rank(uint32_t* counts, uint32_t slot, bool live) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t before = (uint32_t)__shfl_up((int)slot, 1u, 64);
+  const bool head = lane == 0 || before != slot || !live;
+  const uint64_t heads = __ballot(head);
+  const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane
+  const uint32_t first = 63u - (uint32_t)__clzll(heads & upto);
+  const uint64_t later = heads & ~upto;
+  const uint32_t next = later ? (uint32_t)__ffsll((long long)later) - 1u : 64u;
+  uint32_t base = 0;
+  if (live && head) base = atomicAdd(&counts[slot], next - lane);
+  base = (uint32_t)__shfl((int)base, (int)first, 64);
+  return base + (lane - first);
+}
+
 // A cell table (Grid, src/fluid/mod.rs:168-236) over a box of cells, slot
 // ((x - lo.x) * span.y + (y - lo.y)) * span.z + (z - lo.z). Dense (keys
 // null): start[slot] .. start[slot + 1] is the cell's range. Searched:

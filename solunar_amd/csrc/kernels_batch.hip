@@ -313,10 +313,10 @@ __global__ __launch_bounds__(BS) void k_b_count(BatchArgs b, const float4* __res
   bool live;
   uint32_t i = batch_index<BS>(b, map, s, live);
   const SceneState& st = b.state[s];
-  if (!live || !scene_active(b, st)) return;
-  uint32_t slot = scene_slot(st, P[i], b.radius);
-  uint32_t rank = atomicAdd(&counts[slot], 1u);
-  slot_rank[i] = make_uint2(slot, rank);
+  if (!scene_active(b, st)) return;  // the same for the whole block
+  const uint32_t slot = live ? scene_slot(st, P[i], b.radius) : 0xffffffffu;
+  const uint32_t rank = wave_rank(counts, slot, live);
+  if (live) slot_rank[i] = make_uint2(slot, rank);
 }
 
 template <uint32_t BS>

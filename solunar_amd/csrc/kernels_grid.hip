@@ -162,15 +162,17 @@ __global__ __launch_bounds__(256) void k_count(const float4* __restrict__ P, uin
                                                uint32_t* __restrict__ counts, uint2* __restrict__ slot_rank,
                                                Control* ctl) {
   if (ctl->abort) return;
-  uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+  const bool live = i < n;
   int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
   uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
-  float4 p = P[i];
-  int64_t c = cell_slot(cell_coord(p.x, radius), cell_coord(p.y, radius), cell_coord(p.z, radius), lo, span);
-  uint32_t slot = (uint32_t)c;
-  uint32_t rank = atomicAdd(&counts[slot], 1u);
-  slot_rank[i] = make_uint2(slot, rank);
+  uint32_t slot = 0xffffffffu;
+  if (live) {
+    float4 p = P[i];
+    slot = (uint32_t)cell_slot(cell_coord(p.x, radius), cell_coord(p.y, radius), cell_coord(p.z, radius), lo, span);
+  }
+  const uint32_t rank = wave_rank(counts, slot, live);
+  if (live) slot_rank[i] = make_uint2(slot, rank);
 }
 
 // ---- exclusive scan: out[k] = sum(in[0..k)), out[len] = total -----------
