@@ -191,19 +191,32 @@ __global__ __launch_bounds__(BS) void k_b_bounds(BatchArgs b, const float4* __re
   uint32_t i = batch_index<BS>(b, map, s, live);
   SceneState& st = b.state[s];
   if (!scene_active(b, st)) return;
-  int32_t mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
-  int32_t mx[3] = {(int32_t)0x80000000u, (int32_t)0x80000000u, (int32_t)0x80000000u};
+  // the extreme coordinates' cells are the extreme cells (k_bounds); a NaN
+  // coordinate's cell is 0
+  float fmn[3] = {INFINITY, INFINITY, INFINITY};
+  float fmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool nan[3] = {false, false, false};
   if (live) {
-    float4 p = P[i];
-    mn[0] = mx[0] = cell_coord(p.x, b.radius);
-    mn[1] = mx[1] = cell_coord(p.y, b.radius);
-    mn[2] = mx[2] = cell_coord(p.z, b.radius);
-  }
-  for (int a = 0; a < 3; ++a)
-    for (int off = 32; off >= 1; off >>= 1) {
-      mn[a] = min(mn[a], __shfl_xor(mn[a], off, 64));
-      mx[a] = max(mx[a], __shfl_xor(mx[a], off, 64));
+    const float4 p = P[i];
+    const float v[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a) {
+      fmn[a] = fmx[a] = v[a];
+      nan[a] = !(v[a] == v[a]);
     }
+  }
+  int32_t mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      fmn[a] = fminf(fmn[a], __shfl_xor(fmn[a], off, 64));
+      fmx[a] = fmaxf(fmx[a], __shfl_xor(fmx[a], off, 64));
+    }
+    mn[a] = cell_coord(fmn[a], b.radius);
+    mx[a] = cell_coord(fmx[a], b.radius);
+    if (__any(nan[a])) {
+      mn[a] = min(mn[a], 0);
+      mx[a] = max(mx[a], 0);
+    }
+  }
   // the block's waves fold in LDS, then one lane per block updates the
   // scene (every lane of a block belongs to one scene)
   constexpr uint32_t kWaves = BS / 64;

@@ -68,28 +68,50 @@ __global__ __launch_bounds__(256) void k_fastest(const float4* __restrict__ V, u
 __global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ P, uint32_t n, float radius,
                                                 Control* ctl) {
   if (ctl->abort) return;
-  int32_t mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff};
-  int32_t mx[3] = {(int32_t)0x80000000u, (int32_t)0x80000000u, (int32_t)0x80000000u};
+  // cell_coord is monotonic in the coordinate (a correctly rounded division
+  // by radius > 0, floor, saturation), so the cells of the extreme
+  // coordinates are the extreme cells; a NaN coordinate's cell is 0
+  float mn[3] = {INFINITY, INFINITY, INFINITY};
+  float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool nan[3] = {false, false, false};
   for (uint32_t i = xcd_block() * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float4 p = P[i];
-    int32_t c[3] = {cell_coord(p.x, radius), cell_coord(p.y, radius), cell_coord(p.z, radius)};
+    const float4 p = P[i];
+    const float v[3] = {p.x, p.y, p.z};
     for (int a = 0; a < 3; ++a) {
-      mn[a] = min(mn[a], c[a]);
-      mx[a] = max(mx[a], c[a]);
+      mn[a] = fminf(mn[a], v[a]);
+      mx[a] = fmaxf(mx[a], v[a]);
+      nan[a] = nan[a] || !(v[a] == v[a]);
     }
   }
+  // waves, then the block's 4 waves in LDS: one set of atomics a block
+  __shared__ int32_t part[4][6];
+  const uint32_t w = threadIdx.x >> 6;
   for (int a = 0; a < 3; ++a) {
     for (int off = 32; off >= 1; off >>= 1) {
-      mn[a] = min(mn[a], __shfl_xor(mn[a], off, 64));
-      mx[a] = max(mx[a], __shfl_xor(mx[a], off, 64));
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], off, 64));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, 64));
+    }
+    // no particle: +inf and -inf give the identities of min and max
+    int32_t lo = cell_coord(mn[a], radius), hi = cell_coord(mx[a], radius);
+    if (__any(nan[a])) {
+      lo = min(lo, 0);
+      hi = max(hi, 0);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      part[w][a] = lo;
+      part[w][3 + a] = hi;
     }
   }
-  if ((threadIdx.x & 63) == 0) {
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    int32_t v = part[0][a];
+    for (uint32_t k = 1; k < 4; ++k) v = a < 3 ? min(v, part[k][a]) : max(v, part[k][a]);
     Stripe& s = stripe_of(ctl);
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&s.bmin[a], mn[a]);
-      atomicMax(&s.bmax[a], mx[a]);
-    }
+    if (a < 3)
+      atomicMin(&s.bmin[a], v);
+    else
+      atomicMax(&s.bmax[a - 3], v);
   }
 }
 
@@ -484,7 +506,7 @@ void launch_fastest(hipStream_t s, const float4* V, uint32_t n, Control* ctl) {
 
 void launch_bounds(hipStream_t s, const float4* P, uint32_t n, float radius, Control* ctl) {
   uint32_t blocks = blocks_for(n, 256);
-  if (blocks > 2048) blocks = 2048;
+  if (blocks > 1024) blocks = 1024;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_bounds, blocks, 256, 0, s, P, n, radius, ctl);
 }
