@@ -526,26 +526,48 @@ __device__ __forceinline__ void visit_dense(const float4* __restrict__ P, const 
   }
 }
 
+// A fluid neighbour's data a WCSPH force pass gathers: velocity (density in
+// w) and p / rho^2. Nothing: a pass that loads them itself (fewer registers).
+struct Neighbour {
+  float4 v;
+  float s;
+};
+struct Nothing {};
+
 // visit_dense in two phases, for pair terms much dearer than the distance
 // test: the walk queues each lane's accepted candidates in `queue` (LDS,
-// slot k of a lane at queue[k * stride]) and f runs over the queued ones
-// whenever a lane's queue could fill within the next batch, and at the end.
-// The pair terms then run over accepted pairs only, most lanes busy, instead
-// of over every candidate any lane accepts. f sees the same (j, pj, o, sq)
-// in the same order as from visit_dense.
-template <uint32_t Q, class F>
+// slot k of a lane at queue[k * stride]) and the terms run over the queued
+// ones whenever a lane's queue could fill within the next batch, and at the
+// end. The pair terms then run over accepted pairs only, most lanes busy,
+// instead of over every candidate any lane accepts. The queued pairs go four
+// at a time: their indices, positions and gather(j) (the neighbour's other
+// data) load together, then f(j, pj, o, sq, g) runs for each in order, so f
+// sees the same pairs in the same order as from visit_dense.
+template <uint32_t Q, class Gather, class F>
 __device__ __forceinline__ void visit_dense_queued(const float4* __restrict__ P, const uint32_t* __restrict__ start,
                                                    const int32_t lo[3], const uint32_t span[3], float4 at,
                                                    float radius, float reach, uint32_t* queue, uint32_t stride,
-                                                   F&& f) {
+                                                   Gather&& gather, F&& f) {
   static_assert(Q >= 8, "a queue holds at least two batches");
   uint32_t n = 0;
   auto flush = [&]() {
-    for (uint32_t k = 0; k < n; ++k) {
-      const uint32_t j = queue[k * stride];
-      const float4 pj = P[j];
-      const float ox = at.x - pj.x, oy = at.y - pj.y, oz = at.z - pj.z;
-      f(j, pj, ox, oy, oz, dot3(ox, oy, oz, ox, oy, oz));
+    for (uint32_t k = 0; k < n; k += 4) {
+      uint32_t jj[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) jj[u] = queue[min(k + u, n - 1u) * stride];
+      float4 pj[4];
+      decltype(gather(0u)) g[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        pj[u] = P[jj[u]];
+        g[u] = gather(jj[u]);
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        if (k + u >= n) break;
+        const float ox = at.x - pj[u].x, oy = at.y - pj[u].y, oz = at.z - pj[u].z;
+        f(jj[u], pj[u], ox, oy, oz, dot3(ox, oy, oz, ox, oy, oz), g[u]);
+      }
     }
     n = 0;
   };

@@ -436,20 +436,23 @@ __global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __
     float px = 0.0f, py = 0.0f, pz = 0.0f;
     // the single world's two-phase walk (visit_dense_queued): the launch
     // covers every scene, so it is as full as a large world's
-    visit_dense_queued<kBatchQueue>(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach, queue + threadIdx.x, BS,
-                [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
-                  if (sq <= 0.0f) return;
-                  float distance = sqrtf(sq);
-                  float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
-                  float gx = ox * f, gy = oy * f, gz = oz * f;
-                  float4 vj = VB[j];
-                  float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
-                  float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
-                  float kk = prm.mass * (own + scaled[j] + damp);
-                  px = px - gx * kk;
-                  py = py - gy * kk;
-                  pz = pz - gz * kk;
-                });
+    visit_dense_queued<kBatchQueue>(
+        PB, prm.cell_start, lo, span, at, prm.radius, prm.reach, queue + threadIdx.x, BS,
+        // gathered inside the term: the kernel's 64 registers hold no more
+        [](uint32_t) { return Nothing{}; },
+        [&](uint32_t j, float4, float ox, float oy, float oz, float sq, Nothing) {
+          if (sq <= 0.0f) return;
+          float distance = sqrtf(sq);
+          float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
+          float gx = ox * f, gy = oy * f, gz = oz * f;
+          const float4 vj = VB[j];
+          float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
+          float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
+          float kk = prm.mass * (own + scaled[j] + damp);
+          px = px - gx * kk;
+          py = py - gy * kk;
+          pz = pz - gz * kk;
+        });
     float wx = 0.0f, wy = 0.0f, wz = 0.0f;
     if (prm.wall_cells) {
       float pressure = tait_pressure(density, prm.rest, prm.stiffness);

@@ -82,15 +82,16 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
   float density = vi.w;
   float own = scaled[i];
   float px = 0.0f, py = 0.0f, pz = 0.0f;
-  auto pair = [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+  // one fluid pair's push, from the neighbour's velocity (density in w) and
+  // p / rho^2
+  auto pair = [&](float ox, float oy, float oz, float sq, float4 vj, float sj) {
     if (sq <= 0.0f) return;
     float distance = sqrtf(sq);
     float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
     float gx = ox * f, gy = oy * f, gz = oz * f;
-    float4 vj = VB[j];
     float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
     float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
-    float k = prm.mass * (own + scaled[j] + damp);
+    float k = prm.mass * (own + sj + damp);
     px = px - gx * k;
     py = py - gy * k;
     pz = pz - gz * k;
@@ -99,10 +100,17 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
     const Control* ctl = prm.ctl;
     const int32_t lo[3] = {ctl->lo[0], ctl->lo[1], ctl->lo[2]};
     const uint32_t span[3] = {ctl->span[0], ctl->span[1], ctl->span[2]};
-    visit_dense_queued<QUEUE>(PB, prm.cell_start, lo, span, at, prm.radius, prm.reach, queue + threadIdx.x,
-                              blockDim.x, pair);
+    visit_dense_queued<QUEUE>(
+        PB, prm.cell_start, lo, span, at, prm.radius, prm.reach, queue + threadIdx.x, blockDim.x,
+        [&](uint32_t j) { return Neighbour{VB[j], scaled[j]}; },
+        [&](uint32_t, float4, float ox, float oy, float oz, float sq, Neighbour nb) {
+          pair(ox, oy, oz, sq, nb.v, nb.s);
+        });
   } else {
-    visit_fluid<SEARCH>(PB, prm, at, pair);
+    visit_fluid<SEARCH>(PB, prm, at, [&](uint32_t j, float4, float ox, float oy, float oz, float sq) {
+      if (sq <= 0.0f) return;
+      pair(ox, oy, oz, sq, VB[j], scaled[j]);
+    });
   }
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
   float mirrored = 0.0f;
