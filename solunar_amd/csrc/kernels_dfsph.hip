@@ -1085,9 +1085,10 @@ __device__ __forceinline__ void viscous_finish(const StepParams& prm, uint32_t i
 // The viscosity pass for particle i of a tile, inside the density solve's
 // launch: neighbours from the tile's LDS (A = (x, y, z, vx), C = (vy, vz,
 // density)) or from global memory; the same values as k_dfsph_viscous.
-template <int FORMAT>
-__device__ __forceinline__ void viscous_one(const SolveArgs& a, uint32_t i, const Lds* l, const float4* C,
-                                            bool staged) {
+// STAGED: the neighbours come from the tile's LDS (a loop of its own, so
+// that its loads are LDS reads rather than generic ones chosen at run time).
+template <int FORMAT, bool STAGED>
+__device__ __forceinline__ void viscous_walk(const SolveArgs& a, uint32_t i, const Lds* l, const float4* C) {
   const StepParams& prm = a.prm;
   const float4 at = a.P[i];
   const float4 vi = a.Vin[i];
@@ -1107,7 +1108,7 @@ __device__ __forceinline__ void viscous_one(const SolveArgs& a, uint32_t i, cons
     for (int u = 0; u < 4; ++u) {
       const bool live = k + u < nf;
       j[u] = live ? entry_index<FORMAT>(l, e[u].r) : i;
-      if (staged) {
+      if constexpr (STAGED) {
         const int32_t s = batch_slot<FORMAT>(*l, e[u], live);
         const float4 q = l->A[s], c = C[s];
         pj[u] = make_float3(q.x, q.y, q.z);
@@ -1133,6 +1134,15 @@ __device__ __forceinline__ void viscous_one(const SolveArgs& a, uint32_t i, cons
     load_batch<FORMAT>(prm, i, k + 20, nf, b2);
   }
   viscous_finish<FORMAT>(prm, i, at, vi, px, py, pz, nf, nw, a.V, a.forces);
+}
+
+template <int FORMAT>
+__device__ __forceinline__ void viscous_one(const SolveArgs& a, uint32_t i, const Lds* l, const float4* C,
+                                            bool staged) {
+  if (staged)
+    viscous_walk<FORMAT, true>(a, i, l, C);
+  else
+    viscous_walk<FORMAT, false>(a, i, l, C);
 }
 
 // ---- the stop sum over solve tiles ------------------------------------------
