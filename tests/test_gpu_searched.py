@@ -174,3 +174,19 @@ def test_coupled_bodies_over_two_far_pools_push_as_the_oracle_does():
         assert g[1] == c[1], k
     assert g[0].sound_speed == np.float32(4 * speed)
     assert_same(gpu, cpu)
+
+
+def test_a_large_world_on_the_searched_table_steps_as_the_oracle_does():
+    # ~155k particles in two blocks 600 m apart: the radix sort and the
+    # binary-searched walks at a size where every kernel runs many blocks
+    def build(w):
+        w.add_box_container((0.0, 0.0, 0.0), (0.5, 0.6, 0.5))
+        w.fill_box_jittered((0.0, 0.0, 0.0), (0.3, 0.45, 0.3), 0.05, 13)
+        w.fill_box_jittered((FAR, FAR, FAR), (FAR + 0.3, FAR + 0.3, FAR + 0.3), 0.05, 17)
+
+    gpu, cpu = pair(FluidSettings(spacing=0.0075), build)
+    assert len(gpu) > 150000
+    for k in range(2):
+        rg, rc = gpu.step(1e-4), cpu.step(1e-4)
+        assert rg == rc, (k, rg, rc)
+    assert_same(gpu, cpu)
