@@ -70,6 +70,44 @@ __device__ __forceinline__ void list_entries(const StepParams& prm, uint32_t k, 
   }
 }
 
+// Data that other blocks of the persistent solve write during the launch
+// (velocities, stiffnesses, tile sums) is read by the tiled kernel with
+// device-scope (sc1) buffer loads. Each XCD keeps its own L2: a plain load
+// can return a line this XCD's L2 still holds from before another XCD wrote
+// it back, which the barrier's agent-scope acquire would otherwise have to
+// drop by invalidating the whole L2 under every block of the XCD. A
+// device-scope load sees every store written back before the barrier and
+// leaves the L2 alone.
+constexpr int kCoherentLoad = 16;  // cache policy bits: device scope
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t whole(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, kRawBufferWord3);
+}
+
+__device__ __forceinline__ float4 coherent4(const float4* base, uint32_t i) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(whole(base), i * 16u, 0, kCoherentLoad);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
+__device__ __forceinline__ double coherent_f64(const double* base, uint32_t i) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(whole(base), i * 8u, 0, kCoherentLoad);
+  return __hiloint2double((int)v[1], (int)v[0]);
+}
+
+__device__ __forceinline__ uint32_t coherent_u32(const uint32_t* base, uint32_t i) {
+  return __builtin_amdgcn_raw_buffer_load_b32(whole(base), i * 4u, 0, kCoherentLoad);
+}
+
+// A velocity or stiffness of particle i: device-scope in the tiled kernel
+// (COHERENT), a plain load in the untiled ones, whose barrier acquires.
+template <bool COHERENT>
+__device__ __forceinline__ float4 shared4(const float4* base, uint32_t i) {
+  if constexpr (COHERENT)
+    return coherent4(base, i);
+  else
+    return base[i];
+}
+
 __device__ __forceinline__ bool list_aborted(const Control* ctl) {
   return ctl->abort != 0 || (ctl->flags & (kFlagPairs | kFlagWide)) != 0;
 }
@@ -338,16 +376,21 @@ constexpr uint64_t kBarrierTicks = 200000000ull;  // 2 s
 
 // Grid-wide barrier of the persistent solve (every block resident, the grid
 // a multiple of 8). Every wave waits until its own stores have reached the
-// L2 (a block barrier alone does not); then thread 0 of the block arrives at
-// its XCD group's count (blockIdx % 8) with a release, and the last block of
-// a group arrives at the group count; the last group bumps the generation.
-// The other blocks poll the generation with relaxed loads (an acquire per
-// poll would invalidate the L2 the blocks still working gather from) and
-// acquire once when it moves. One release and one acquire per block, and
+// L2 (a block barrier alone does not); then thread 0 of the block writes
+// its XCD's L2 back (one agent-scope release fence) and arrives at its XCD
+// group's count (blockIdx % 8), and the last block of a group arrives at
+// the group count; the last group bumps the generation. The arrivals need
+// no release of their own: each block's stores were in memory before it
+// arrived. The other blocks poll the generation with relaxed loads and,
+// with `acquire`, take one agent-scope acquire when it moves (an acquire per
+// poll would invalidate the L2 the blocks still working gather from). The
+// tiled kernel passes acquire = false: it reads what other blocks wrote
+// with device-scope loads (coherent4), so its L2 is never invalidated.
 // 32-deep instead of 256-deep atomic chains, each word on its own cache
-// line: 7.6 instead of 29 microseconds per barrier over 256 blocks. Returns
-// false if it waited past the limit, having flagged the world.
-__device__ bool grid_barrier(Control* ctl) {
+// line: 5.4 (7.1 with the acquire) instead of 29 microseconds per barrier
+// over 256 blocks. Returns false if it waited past the limit, having
+// flagged the world.
+__device__ bool grid_barrier(Control* ctl, bool acquire) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -356,17 +399,18 @@ __device__ bool grid_barrier(Control* ctl) {
     int ok = 1;
     const uint32_t group = blockIdx.x % kXcds, per_group = gridDim.x / kXcds;
     uint32_t* mine = &ctl->bar_xcd[group][0];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     uint32_t gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t arrived = __hip_atomic_fetch_add(mine, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t arrived = __hip_atomic_fetch_add(mine, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     bool last = false;
     if (arrived == per_group - 1) {
       __hip_atomic_store(mine, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t groups = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t groups = __hip_atomic_fetch_add(&ctl->bar_count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       last = groups == kXcds - 1;
     }
     if (last) {
       __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       uint64_t start = wall_clock64();
       while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
@@ -378,7 +422,7 @@ __device__ bool grid_barrier(Control* ctl) {
         }
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (acquire) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     ok_s = ok;
   }
   __syncthreads();
@@ -532,7 +576,7 @@ __device__ __forceinline__ float fluid_rate(const SolveArgs& a, uint32_t i, floa
       for (int u = 0; u < 4; ++u) {
         const uint32_t j = k + u < nf ? entry_index<FORMAT>(l, cur[u].r) : i;
         pj[u] = g.P[j];
-        vj[u] = g.V[j];
+        vj[u] = coherent4(g.V, j);
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -717,7 +761,7 @@ template <int FORMAT, bool DEEP = true>
 __device__ __forceinline__ float rate_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, bool staged,
                                          int64_t sysz, int64_t szz) {
   float4 at = a.P[i];
-  float4 vi = a.V[i];
+  float4 vi = shared4<!DEEP>(a.V, i);
   uint32_t counts = a.prm.pair_counts[i];
   uint32_t nf = counts & 0xffffu, nw = counts >> 16;
   float rate;
@@ -735,7 +779,7 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
                                          int64_t sysz, int64_t szz) {
   const StepParams& prm = a.prm;
   const float mass = prm.mass;
-  float4 at = a.kappa[i];  // x as P holds it, own stiffness in w
+  float4 at = shared4<!DEEP>(a.kappa, i);  // x as P holds it, own stiffness in w
   float own = at.w;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
   uint32_t counts = prm.pair_counts[i];
@@ -799,7 +843,7 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
       for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
       fluid_entries<FORMAT>(prm, i, k + 4, nxt);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) pj[u] = g.K[k + u < nf ? entry_index<FORMAT>(l, cur[u].r) : i];
+      for (int u = 0; u < 4; ++u) pj[u] = coherent4(g.K, k + u < nf ? entry_index<FORMAT>(l, cur[u].r) : i);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         if (k + u < nf) {
@@ -862,7 +906,7 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
     cz = cz - (live ? gz * kw : 0.0f);
   });
   const float dt = prm.dt;
-  float4 vi = a.V[i];
+  float4 vi = shared4<!DEEP>(a.V, i);
   vi.x = vi.x + cx * dt;
   vi.y = vi.y + cy * dt;
   vi.z = vi.z + cz * dt;
@@ -1216,8 +1260,8 @@ __device__ __forceinline__ double fold_tiles_lds(const SolveArgs& a, uint32_t ti
   for (uint32_t base = 0; base < tiles; base += room) {
     const uint32_t m = min(room, tiles - base);
     for (uint32_t k = t; k < m; k += kSolveThreads) {
-      L[k] = a.tile_sum[base + k];
-      C[k] = a.tile_count[base + k];
+      L[k] = coherent_f64(a.tile_sum, base + k);
+      C[k] = coherent_u32(a.tile_count, base + k);
     }
     __syncthreads();
     for (uint32_t k = lane; k < m; k += 64u) {
@@ -1341,7 +1385,8 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
           }
         } else if (rate) {
           const float4 p0 = a.P[j0], p1 = a.P[j1], p2 = a.P[j2], p3 = a.P[j3];
-          const float4 v0 = a.V[j0], v1 = a.V[j1], v2 = a.V[j2], v3 = a.V[j3];
+          const float4 v0 = coherent4(a.V, j0), v1 = coherent4(a.V, j1), v2 = coherent4(a.V, j2),
+                       v3 = coherent4(a.V, j3);
           if (h0) {
             A[to0 + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
             B[to0 + s] = make_float2(v0.y, v0.z);
@@ -1359,7 +1404,8 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
             B[to1 + sb] = make_float2(v3.y, v3.z);
           }
         } else {
-          const float4 q0 = a.kappa[j0], q1 = a.kappa[j1], q2 = a.kappa[j2], q3 = a.kappa[j3];
+          const float4 q0 = coherent4(a.kappa, j0), q1 = coherent4(a.kappa, j1), q2 = coherent4(a.kappa, j2),
+                       q3 = coherent4(a.kappa, j3);
           if (h0) Q[to0 + s] = q0;
           if (h1) Q[to1 + s] = q1;
           if (h2) Q[to0 + sb] = q2;
@@ -1454,7 +1500,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
         __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       tiled_pass<FORMAT>(a, sh, kViscousPass, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
       ++pass;
-      if (!grid_barrier(ctl)) return;
+      if (!grid_barrier(ctl, !tiled)) return;
     }
   }
   for (;;) {
@@ -1465,15 +1511,15 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
         __hip_atomic_store(&ctl->tile_next[(pass + 1) & 1u][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if constexpr (TILED) tiled_pass<FORMAT>(a, sh, kRatePass, pass & 1u, lds, tstate, grab_s, g, wsum, wcnt);
       ++pass;
-      if (!grid_barrier(ctl)) return;
+      if (!grid_barrier(ctl, !tiled)) return;
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
         uint32_t i = c * kChunk + tc;
         if (i < prm.n) a.errors[i] = rate_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz);
       }
-      if (!grid_barrier(ctl)) return;
+      if (!grid_barrier(ctl, !tiled)) return;
       tile_sums_from_errors(a, sh, tstate[0], wsum, wcnt);
-      if (!grid_barrier(ctl)) return;
+      if (!grid_barrier(ctl, !tiled)) return;
     }
     // every wave folds the tile sums the same way
     uint32_t loaded = 0;
@@ -1505,7 +1551,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
         if (i < prm.n) push_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz);
       }
     }
-    if (!grid_barrier(ctl)) return;
+    if (!grid_barrier(ctl, !tiled)) return;
     ++it;
   }
   // every block is past the last pass: leave both queues at zero for the
