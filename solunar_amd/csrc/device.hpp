@@ -94,6 +94,18 @@ struct Control {
   // density), for parity logs
   double residual_log[kResidualLogCap];
   uint32_t loaded_log[kResidualLogCap];
+  // the flow solve (cleared once a substep by its tile set-up, and again at
+  // the start of the density solve's launch): each XCD group's task queue,
+  // its rate tasks finished per iteration parity, the groups finished per
+  // parity, each iteration's decision per parity, and the decisions
+  // committed in iteration order; one cache line each
+  alignas(128) uint32_t flow_next[8][32];
+  alignas(128) uint32_t flow_rates[2][8][32];
+  alignas(128) uint32_t flow_groups[2][32];
+  alignas(128) uint32_t flow_decision[2][32];
+  alignas(128) uint32_t flow_resolved;
+  // the slabs of the groups as ranges of the listed tiles (flow_list), set per substep
+  alignas(128) uint32_t flow_begin[9];
 };
 
 __device__ __forceinline__ Stripe& stripe_of(Control* ctl) { return ctl->stripes[blockIdx.x & (kStripes - 1)]; }

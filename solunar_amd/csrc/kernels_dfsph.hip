@@ -79,6 +79,8 @@ __device__ __forceinline__ void list_entries(const StepParams& prm, uint32_t k, 
 // device-scope load sees every store written back before the barrier and
 // leaves the L2 alone.
 constexpr int kCoherentLoad = 16;  // cache policy bits: device scope
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t whole(const void* base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, kRawBufferWord3);
@@ -447,7 +449,28 @@ struct SolveArgs {
   uint32_t lds_cap;     // particles one block stages in LDS; 0: no tiles
   const float4* Vin;    // tiled density solve: velocities before the viscosity pass, which it runs first
   float* forces;        // that pass's accelerations, or null
+  // the flow solve (tiled, no grid barriers between passes; null V2: the
+  // passes meet at grid barriers): the second velocity buffer, per tile its
+  // passes finished this launch and its particles; tile_sum and tile_count
+  // then hold two sets of tile_cap, one per iteration parity
+  float4* V2;
+  uint32_t* tile_done;
+  uint32_t* tile_work;
+  uint32_t* flow_list;  // the non-empty tiles in order
+  uint32_t through;     // stores other blocks read go through the L2 (the flow solve)
 };
+
+// A store another block reads in the same launch: through the L2 to memory
+// (device scope) in the flow solve, where no L2 write-back follows each
+// task, a plain store otherwise (the grid barrier writes the L2 back).
+__device__ __forceinline__ void put4(float4* base, uint32_t i, float4 v, bool through) {
+  if (through) {
+    const u32x4 bits = {__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+    __builtin_amdgcn_raw_buffer_store_b128(bits, whole(base), i * 16u, 0, kCoherentLoad);
+  } else {
+    base[i] = v;
+  }
+}
 
 // ---- one particle of a pass ------------------------------------------------
 //
@@ -753,7 +776,7 @@ __device__ __forceinline__ float finish_rate(const SolveArgs& a, uint32_t i, flo
     target = r / dt;
     error = dt * r / prm.rest;
   }
-  a.kappa[i] = make_float4(at.x, at.y, at.z, target * fabsf(fa));
+  put4(a.kappa, i, make_float4(at.x, at.y, at.z, target * fabsf(fa)), a.through != 0);
   return error;
 }
 
@@ -761,7 +784,7 @@ template <int FORMAT, bool DEEP = true>
 __device__ __forceinline__ float rate_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, bool staged,
                                          int64_t sysz, int64_t szz) {
   float4 at = a.P[i];
-  float4 vi = shared4<!DEEP>(a.V, i);
+  float4 vi = shared4<!DEEP>(g.V, i);
   uint32_t counts = a.prm.pair_counts[i];
   uint32_t nf = counts & 0xffffu, nw = counts >> 16;
   float rate;
@@ -774,9 +797,10 @@ __device__ __forceinline__ float rate_one(const SolveArgs& a, uint32_t i, const 
   return finish_rate<FORMAT>(a, i, at, vi, rate, nf, nw);
 }
 
+// Writes the pushed velocity to `vout` (g.V itself but in the flow solve).
 template <int FORMAT, bool DEEP = true>
 __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const Glob& g, const Lds* l, bool staged,
-                                         int64_t sysz, int64_t szz) {
+                                         int64_t sysz, int64_t szz, float4* vout) {
   const StepParams& prm = a.prm;
   const float mass = prm.mass;
   float4 at = shared4<!DEEP>(a.kappa, i);  // x as P holds it, own stiffness in w
@@ -906,11 +930,11 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
     cz = cz - (live ? gz * kw : 0.0f);
   });
   const float dt = prm.dt;
-  float4 vi = shared4<!DEEP>(a.V, i);
+  float4 vi = shared4<!DEEP>(g.V, i);
   vi.x = vi.x + cx * dt;
   vi.y = vi.y + cy * dt;
   vi.z = vi.z + cz * dt;
-  a.V[i] = vi;
+  put4(vout, i, vi, a.through != 0);
 }
 
 // ---- LDS tiles --------------------------------------------------------------
@@ -1306,6 +1330,77 @@ __device__ void tile_sums_from_errors(const SolveArgs& a, const TileShape& sh, T
 
 constexpr int kRatePass = 0, kPushPass = 1, kViscousPass = 2;
 
+// Stages tile ts's neighbourhood for a pass into the block's LDS: wave w
+// stages columns w and w + 16 (when below 24), lanes on consecutive
+// particles, two rows of 64 of each column per trip: eight loads in flight
+// (past a column's end they read particle 0 and are not stored). The rate
+// pass stages (x, y, z, vx | vy, vz) from P and g.V, the push pass (x,
+// stiffness) from g.K, the viscosity pass (x, y, z, vx | vy, vz, density)
+// from P and Vin.
+__device__ __forceinline__ void stage_tile(const SolveArgs& a, const Glob& g, int mode, const TileState& ts,
+                                           uint32_t stride, float4* A, float2* B, float4* Q, float4* C) {
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t c0 = wave, c1 = wave + 16u;
+  const bool second = c1 < kStagedColumns;
+  const uint32_t f0 = ts.ss[c0], n0 = ts.len[c0];
+  const uint32_t f1 = second ? ts.ss[c1] : 0u, n1 = second ? ts.len[c1] : 0u;
+  const uint32_t to0 = c0 * stride, to1 = c1 * stride;
+  const uint32_t rows = max(n0, n1);
+  for (uint32_t s = lane; s < rows; s += 128) {
+    const uint32_t sb = s + 64;
+    const bool h0 = s < n0, h1 = s < n1, h2 = sb < n0, h3 = sb < n1;
+    const uint32_t j0 = h0 ? f0 + s : 0u, j1 = h1 ? f1 + s : 0u;
+    const uint32_t j2 = h2 ? f0 + sb : 0u, j3 = h3 ? f1 + sb : 0u;
+    if (mode == kViscousPass) {
+      const float4 p0 = a.P[j0], p1 = a.P[j1], p2 = a.P[j2], p3 = a.P[j3];
+      const float4 v0 = a.Vin[j0], v1 = a.Vin[j1], v2 = a.Vin[j2], v3 = a.Vin[j3];
+      if (h0) {
+        A[to0 + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
+        C[to0 + s] = make_float4(v0.y, v0.z, v0.w, 0.0f);
+      }
+      if (h1) {
+        A[to1 + s] = make_float4(p1.x, p1.y, p1.z, v1.x);
+        C[to1 + s] = make_float4(v1.y, v1.z, v1.w, 0.0f);
+      }
+      if (h2) {
+        A[to0 + sb] = make_float4(p2.x, p2.y, p2.z, v2.x);
+        C[to0 + sb] = make_float4(v2.y, v2.z, v2.w, 0.0f);
+      }
+      if (h3) {
+        A[to1 + sb] = make_float4(p3.x, p3.y, p3.z, v3.x);
+        C[to1 + sb] = make_float4(v3.y, v3.z, v3.w, 0.0f);
+      }
+    } else if (mode == kRatePass) {
+      const float4 p0 = a.P[j0], p1 = a.P[j1], p2 = a.P[j2], p3 = a.P[j3];
+      const float4 v0 = coherent4(g.V, j0), v1 = coherent4(g.V, j1), v2 = coherent4(g.V, j2),
+                   v3 = coherent4(g.V, j3);
+      if (h0) {
+        A[to0 + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
+        B[to0 + s] = make_float2(v0.y, v0.z);
+      }
+      if (h1) {
+        A[to1 + s] = make_float4(p1.x, p1.y, p1.z, v1.x);
+        B[to1 + s] = make_float2(v1.y, v1.z);
+      }
+      if (h2) {
+        A[to0 + sb] = make_float4(p2.x, p2.y, p2.z, v2.x);
+        B[to0 + sb] = make_float2(v2.y, v2.z);
+      }
+      if (h3) {
+        A[to1 + sb] = make_float4(p3.x, p3.y, p3.z, v3.x);
+        B[to1 + sb] = make_float2(v3.y, v3.z);
+      }
+    } else {
+      const float4 q0 = coherent4(g.K, j0), q1 = coherent4(g.K, j1), q2 = coherent4(g.K, j2),
+                   q3 = coherent4(g.K, j3);
+      if (h0) Q[to0 + s] = q0;
+      if (h1) Q[to1 + s] = q1;
+      if (h2) Q[to0 + sb] = q2;
+      if (h3) Q[to1 + sb] = q3;
+    }
+  }
+}
+
 // One pass over the tiles (rate, push or viscosity pass), queued
 // per XCD group with stealing (tile_next[parity]). Each round the block
 // takes a tile: stages its neighbourhood in LDS, then every particle of it
@@ -1348,71 +1443,7 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
     const uint32_t stride = __builtin_amdgcn_readfirstlane(ts.stride);
     const uint32_t count = ts.ppre[kProcessedColumns];
     const bool fits = kStagedColumns * stride <= cap;
-    if (fits && count) {
-      // wave w stages columns w and w + 16 (when below 24), lanes on
-      // consecutive particles, two rows of 64 of each column per trip:
-      // eight loads in flight (past a column's end they read particle 0
-      // and are not stored)
-      const uint32_t c0 = wave, c1 = wave + 16u;
-      const bool second = c1 < kStagedColumns;
-      const uint32_t f0 = ts.ss[c0], n0 = ts.len[c0];
-      const uint32_t f1 = second ? ts.ss[c1] : 0u, n1 = second ? ts.len[c1] : 0u;
-      const uint32_t to0 = c0 * stride, to1 = c1 * stride;
-      const uint32_t rows = max(n0, n1);
-      for (uint32_t s = lane; s < rows; s += 128) {
-        const uint32_t sb = s + 64;
-        const bool h0 = s < n0, h1 = s < n1, h2 = sb < n0, h3 = sb < n1;
-        const uint32_t j0 = h0 ? f0 + s : 0u, j1 = h1 ? f1 + s : 0u;
-        const uint32_t j2 = h2 ? f0 + sb : 0u, j3 = h3 ? f1 + sb : 0u;
-        if (viscous) {
-          const float4 p0 = a.P[j0], p1 = a.P[j1], p2 = a.P[j2], p3 = a.P[j3];
-          const float4 v0 = a.Vin[j0], v1 = a.Vin[j1], v2 = a.Vin[j2], v3 = a.Vin[j3];
-          if (h0) {
-            A[to0 + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
-            C[to0 + s] = make_float4(v0.y, v0.z, v0.w, 0.0f);
-          }
-          if (h1) {
-            A[to1 + s] = make_float4(p1.x, p1.y, p1.z, v1.x);
-            C[to1 + s] = make_float4(v1.y, v1.z, v1.w, 0.0f);
-          }
-          if (h2) {
-            A[to0 + sb] = make_float4(p2.x, p2.y, p2.z, v2.x);
-            C[to0 + sb] = make_float4(v2.y, v2.z, v2.w, 0.0f);
-          }
-          if (h3) {
-            A[to1 + sb] = make_float4(p3.x, p3.y, p3.z, v3.x);
-            C[to1 + sb] = make_float4(v3.y, v3.z, v3.w, 0.0f);
-          }
-        } else if (rate) {
-          const float4 p0 = a.P[j0], p1 = a.P[j1], p2 = a.P[j2], p3 = a.P[j3];
-          const float4 v0 = coherent4(a.V, j0), v1 = coherent4(a.V, j1), v2 = coherent4(a.V, j2),
-                       v3 = coherent4(a.V, j3);
-          if (h0) {
-            A[to0 + s] = make_float4(p0.x, p0.y, p0.z, v0.x);
-            B[to0 + s] = make_float2(v0.y, v0.z);
-          }
-          if (h1) {
-            A[to1 + s] = make_float4(p1.x, p1.y, p1.z, v1.x);
-            B[to1 + s] = make_float2(v1.y, v1.z);
-          }
-          if (h2) {
-            A[to0 + sb] = make_float4(p2.x, p2.y, p2.z, v2.x);
-            B[to0 + sb] = make_float2(v2.y, v2.z);
-          }
-          if (h3) {
-            A[to1 + sb] = make_float4(p3.x, p3.y, p3.z, v3.x);
-            B[to1 + sb] = make_float2(v3.y, v3.z);
-          }
-        } else {
-          const float4 q0 = coherent4(a.kappa, j0), q1 = coherent4(a.kappa, j1), q2 = coherent4(a.kappa, j2),
-                       q3 = coherent4(a.kappa, j3);
-          if (h0) Q[to0 + s] = q0;
-          if (h1) Q[to1 + s] = q1;
-          if (h2) Q[to0 + sb] = q2;
-          if (h3) Q[to1 + sb] = q3;
-        }
-      }
-    }
+    if (fits && count) stage_tile(a, g, mode, ts, stride, A, B, Q, C);
     __syncthreads();
     // the last wave (whose particles, if any, come last) takes the next
     // tile and loads its ranges while the others sum
@@ -1435,7 +1466,7 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
       } else if (viscous) {
         viscous_one<FORMAT>(a, i, &l, C, fits);
       } else {
-        push_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
+        push_one<FORMAT, false>(a, i, g, &l, fits, 0, 0, a.V);
       }
     }
     if (rate) {
@@ -1449,6 +1480,552 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
     __syncthreads();
     tsp = &tstate[nxt];
     any = grab_s[nxt] != kNoTile;
+  }
+}
+
+// ---- the dataflow solve -------------------------------------------------------
+//
+// The tiled solve without grid barriers between its passes. Pass q of tile
+// T is a task: q = 2k is iteration k's rate pass, q = 2k + 1 its push. A
+// task reads only its tile's neighbourhood, so it waits only for the tiles
+// around it: (q, T) starts once every tile within one tile of T in x, y and
+// z has finished pass q - 1 (tile_done >= q). The velocities are double
+// buffered: iteration k's rate pass reads V_k (V when k is even, V2 when
+// odd) and its push writes V_k+1 into the other buffer, so the push may run
+// before iteration k's stop decision: if iteration k stops, V_k is intact.
+// The push of iteration k + 1 overwrites V_k, so it waits for the decision
+// of iteration k. Each decision is taken by the block that publishes the
+// last rate task of its iteration: it folds the tile sums in the same fixed
+// shape and commits the decision in iteration order (flow_resolved).
+//
+// Each XCD group has one queue over its slab of tiles: index j is pass
+// j / len, tile begin + j % len; the slabs are cut at equal particle counts,
+// and a group runs its slab's tasks alone. No block waits on a task that no
+// running block will take: a task waits only on tasks of lower passes, every
+// queue hands its tasks out in pass order, and each queue's own blocks take
+// them; past the stop decision, tasks of later passes are dropped, not
+// waited for.
+//
+// Stores that other blocks read (stiffnesses, velocities, tile sums) go
+// through the L2 to memory (device scope, put4), and a task is published
+// only after every wave of its block has waited for its stores, so no L2
+// write-back is needed between tasks; other blocks' data is read with
+// device-scope loads. Empty tiles are marked finished for every pass at the
+// start and skipped.
+//
+// The last wave of a block (the 64 particles of every 1024 that come last,
+// usually none) runs the control: it publishes the previous task, takes the
+// next, loads its ranges, waits for its neighbours, and folds when its
+// publication completes an iteration.
+
+constexpr uint32_t kTaskDone = 0xffffffffu;  // tile_done of an empty tile
+constexpr uint32_t kResolvedStop = 0x80000000u;
+
+// Particles of a tile (its eight processed column ranges).
+__device__ __forceinline__ uint32_t tile_particles(const TileShape& sh, uint32_t tile, const uint32_t* start) {
+  const uint32_t tx = tile / (sh.nty * sh.ntz), ty = (tile / sh.ntz) % sh.nty, tzi = tile % sh.ntz;
+  const uint32_t z0 = tzi * sh.tz, z1 = min(z0 + sh.tz, sh.sz);
+  uint32_t n = 0;
+  for (uint32_t p = 0; p < kProcessedColumns; ++p) {
+    const uint64_t x = 4 * tx + (p >> 1), y = 2 * ty + (p & 1);
+    if (x < sh.sx && y < sh.sy) {
+      const uint64_t base = (x * sh.sy + y) * sh.sz;
+      n += start[base + z1] - start[base + z0];
+    }
+  }
+  return n;
+}
+
+// The flow solve's tiles for a substep, one block of 1024 threads, before
+// the divergence solve (the cell table holds until the advance): the
+// particles of every tile; the non-empty tiles listed in order (flow_list);
+// the slabs of the XCD groups cut at equal particle counts, as ranges of
+// that list (flow_begin: slab v is list entries begin[v] .. begin[v + 1]);
+// empty tiles marked finished for every pass (tile_done = kTaskDone, their
+// tile sums zero in both sets), the others not started; the control words
+// cleared. Thread t works the contiguous tiles of segment t.
+__global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
+  Control* ctl = a.prm.ctl;
+  if (list_aborted(ctl)) return;
+  const TileShape sh = tile_shape(ctl);
+  if (sh.count > a.tile_cap) return;  // the solve flags it
+  __shared__ uint32_t work_pre[kSolveThreads + 1], live_pre[kSolveThreads + 1];
+  const uint32_t t = threadIdx.x, count = sh.count;
+  const uint32_t per = (count + kSolveThreads - 1) / kSolveThreads;
+  const uint32_t lo = min(t * per, count), hi = min(lo + per, count);
+  uint32_t work = 0, live = 0;
+  for (uint32_t k = lo; k < hi; ++k) {
+    const uint32_t n = tile_particles(sh, k, a.prm.cell_start);
+    a.tile_work[k] = n;
+    a.tile_done[k] = n ? 0u : kTaskDone;
+    if (!n) {
+      a.tile_sum[k] = 0.0;
+      a.tile_count[k] = 0u;
+      a.tile_sum[a.tile_cap + k] = 0.0;
+      a.tile_count[a.tile_cap + k] = 0u;
+    }
+    work += n;
+    live += n ? 1u : 0u;
+  }
+  work_pre[t] = work;
+  live_pre[t] = live;
+  __syncthreads();
+  // exclusive scans of both by one wave, 16 segments a lane
+  if (t < 64) {
+    uint32_t rw = 0, rl = 0;
+    for (uint32_t k = t * 16; k < t * 16 + 16; ++k) {
+      rw += work_pre[k];
+      rl += live_pre[k];
+    }
+    uint32_t iw = rw, il = rl;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t uw = (uint32_t)__shfl_up((int)iw, d, 64), ul = (uint32_t)__shfl_up((int)il, d, 64);
+      if ((int)t >= d) {
+        iw += uw;
+        il += ul;
+      }
+    }
+    uint32_t aw = iw - rw, al = il - rl;
+    for (uint32_t k = t * 16; k < t * 16 + 16; ++k) {
+      const uint32_t vw = work_pre[k], vl = live_pre[k];
+      work_pre[k] = aw;
+      live_pre[k] = al;
+      aw += vw;
+      al += vl;
+    }
+    if (t == 63) {
+      work_pre[kSolveThreads] = iw;
+      live_pre[kSolveThreads] = il;
+    }
+  }
+  __syncthreads();
+  const uint64_t total = work_pre[kSolveThreads];
+  const uint32_t pw = work_pre[t], pl = live_pre[t];
+  // slab v starts at the first tile whose preceding tiles hold at least
+  // ceil(total * v / 8) particles (bt); its list entries start after the
+  // non-empty tiles before it
+  __shared__ uint32_t bt[kXcds + 1], bl[kXcds + 1], wave_n[kSolveWaves];
+  if (t == 0) {
+    bt[0] = 0u;
+    bl[0] = 0u;
+    bt[kXcds] = count;
+    bl[kXcds] = live_pre[kSolveThreads];
+  }
+  for (uint32_t v = 1; v < kXcds; ++v) {
+    const uint64_t target = (total * v + kXcds - 1) / kXcds;
+    if (target == 0) {
+      if (t == 0) {
+        bt[v] = 0u;
+        bl[v] = 0u;
+      }
+    } else if (pw < target && target <= (uint64_t)pw + work) {
+      uint64_t run = pw;
+      uint32_t k = lo, listed = pl;
+      while (run < target) {
+        const uint32_t n = a.tile_work[k++];
+        run += n;
+        listed += n ? 1u : 0u;
+      }
+      bt[v] = k;
+      bl[v] = listed;
+    }
+  }
+  __syncthreads();
+  if (t <= kXcds) ctl->flow_begin[t] = bl[t];
+  // Each slab's non-empty tiles listed y-major (tile row ty, then x, then
+  // z): a task of pass q + 1 then needs the tasks of pass q a row or so
+  // earlier in its slab's order (and the neighbouring slabs' at about the
+  // same place in theirs), so a pass can start while the last one is still
+  // finishing, where x-major order would make it wait for a whole plane.
+  const uint32_t plane = sh.nty * sh.ntz, lane = t & 63u, wave = t >> 6;
+  for (uint32_t v = 0; v < kXcds; ++v) {
+    if (bt[v] >= bt[v + 1]) continue;
+    const uint32_t x0 = bt[v] / plane, x1 = (bt[v + 1] - 1) / plane, nx = x1 - x0 + 1;
+    const uint32_t candidates = sh.nty * nx * sh.ntz;
+    uint32_t at = bl[v];
+    for (uint32_t c0 = 0; c0 < candidates; c0 += kSolveThreads) {
+      const uint32_t e = c0 + t;
+      uint32_t idx = 0;
+      bool keep = false;
+      if (e < candidates) {
+        const uint32_t ty = e / (nx * sh.ntz), tx = x0 + (e / sh.ntz) % nx, tz = e % sh.ntz;
+        idx = (tx * sh.nty + ty) * sh.ntz + tz;
+        keep = idx >= bt[v] && idx < bt[v + 1] && a.tile_work[idx] != 0u;
+      }
+      const uint64_t m = __ballot(keep);
+      const uint32_t before = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) wave_n[wave] = (uint32_t)__popcll(m);
+      __syncthreads();
+      uint32_t off = at, all = 0;
+      for (uint32_t w = 0; w < kSolveWaves; ++w) {
+        if (w < wave) off += wave_n[w];
+        all += wave_n[w];
+      }
+      if (keep) a.flow_list[off + before] = idx;
+      at += all;
+      __syncthreads();
+    }
+  }
+  if (t < 8 * 32) (&ctl->flow_next[0][0])[t] = 0u;
+  if (t < 2 * 8 * 32) (&ctl->flow_rates[0][0][0])[t] = 0u;
+  if (t < 2 * 32) {
+    (&ctl->flow_groups[0][0])[t] = 0u;
+    (&ctl->flow_decision[0][0])[t] = 0u;
+  }
+  if (t == 0) ctl->flow_resolved = 0u;
+}
+
+// A task's neighbour wait, by the control wave: every tile within one of T
+// has finished pass q - 1, and the push of iteration k >= 1 finds iteration
+// k - 1 committed as going on. Returns 1 when the task may start, 0 to wait,
+// -1 when the solve stopped before it (the task and every later one are
+// dropped).
+__device__ __forceinline__ int flow_ready(const SolveArgs& a, const TileShape& sh, uint32_t q, uint32_t tile,
+                                          uint32_t lane) {
+  Control* ctl = a.prm.ctl;
+  uint32_t r = 0;
+  if (lane == 0) r = __hip_atomic_load(&ctl->flow_resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  r = __builtin_amdgcn_readfirstlane(r);
+  if (r & kResolvedStop) {
+    if (q >= 2u * (r & ~kResolvedStop) + 1u) return -1;
+  } else if ((q & 1u) && q >= 3u && r < (q >> 1)) {
+    return 0;
+  }
+  if (q == 0) return 1;
+  bool ok = true;
+  if (lane < 27) {
+    const uint32_t tx = tile / (sh.nty * sh.ntz), ty = (tile / sh.ntz) % sh.nty, tzi = tile % sh.ntz;
+    const int64_t x = (int64_t)tx + (int)(lane / 9) - 1, y = (int64_t)ty + (int)((lane / 3) % 3) - 1,
+                  z = (int64_t)tzi + (int)(lane % 3) - 1;
+    if (x >= 0 && y >= 0 && z >= 0 && x < sh.ntx && y < sh.nty && z < sh.ntz) {
+      const uint32_t u = (uint32_t)((x * sh.nty + y) * sh.ntz + z);
+      ok = __hip_atomic_load(&a.tile_done[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= q;
+    }
+  }
+  return __ballot(!ok) == 0 ? 1 : 0;
+}
+
+// The control wave takes the block's next task from its group's queue: its
+// pass and tile, kNoTile as the tile when the queue is done.
+__device__ __forceinline__ uint2 flow_take(const SolveArgs& a, const uint32_t* begin, uint32_t lane) {
+  Control* ctl = a.prm.ctl;
+  const uint32_t v = blockIdx.x % kXcds;
+  const uint32_t b = begin[v], len = begin[v + 1] - b;
+  if (len == 0) return make_uint2(0u, kNoTile);
+  uint32_t j = 0;
+  if (lane == 0) j = __hip_atomic_fetch_add(&ctl->flow_next[v][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  j = __builtin_amdgcn_readfirstlane(j);
+  const uint32_t q = j / len;
+  if (q > 2u * a.max_iterations) return make_uint2(0u, kNoTile);
+  return make_uint2(q, a.flow_list[b + j % len]);
+}
+
+// Waits until the task may start: 1, or -1 when the solve stopped before it,
+// 0 when the wait gave up (the world flagged). A block waits only once it
+// has published every task it finished, so no wait hangs on a task that is
+// done but unpublished.
+__device__ int flow_wait(const SolveArgs& a, const TileShape& sh, uint2 task, uint32_t lane) {
+  const uint64_t t0 = wall_clock64();
+  for (;;) {
+    const int ready = flow_ready(a, sh, task.x, task.y, lane);
+    if (ready != 0) return ready;
+    __builtin_amdgcn_s_sleep(2);
+    if (wall_clock64() - t0 > kBarrierTicks) {
+      if (lane == 0) atomicOr(&a.prm.ctl->flags, kFlagBarrier);
+      return 0;
+    }
+  }
+}
+
+// Commits decided iterations in order, by lane 0 of the control wave:
+// flow_resolved = m (iterations 0 .. m - 1 went on) advances while the
+// decision of iteration m is in (flow_decision[m % 2] = (m + 1) << 1 | stop).
+__device__ void flow_commit(Control* ctl) {
+  uint32_t r = __hip_atomic_load(&ctl->flow_resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (;;) {
+    if (r & kResolvedStop) return;
+    const uint32_t d = __hip_atomic_load(&ctl->flow_decision[r & 1u][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((d >> 1) != r + 1) return;
+    const uint32_t nv = (d & 1u) ? (kResolvedStop | r) : r + 1;
+    if (__hip_atomic_compare_exchange_strong(&ctl->flow_resolved, &r, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      r = nv;
+  }
+}
+
+// Publishes a finished task, by the control wave once every wave of the block
+// has waited for its stores. A rate task also counts toward its iteration;
+// the publication that completes it folds the tile sums (lane l adds sums l,
+// l + 64, ... in order, then the lanes by halving: the fixed shape), decides
+// and commits.
+__device__ void flow_publish(const SolveArgs& a, const TileShape& sh, uint2 task, const uint32_t* begin,
+                             uint32_t groups, uint32_t log_base, uint32_t lane) {
+  Control* ctl = a.prm.ctl;
+  const uint32_t q = task.x, tile = task.y;
+  bool fold = false;
+  if (lane == 0) {
+    __hip_atomic_store(&a.tile_done[tile], q + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!(q & 1u)) {
+      const uint32_t par = (q >> 1) & 1u, v = blockIdx.x % kXcds;
+      const uint32_t c = __hip_atomic_fetch_add(&ctl->flow_rates[par][v][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (c + 1u == begin[v + 1] - begin[v]) {
+        // the group's count starts again for iteration k + 2, which no task
+        // reaches before this iteration's decision
+        __hip_atomic_store(&ctl->flow_rates[par][v][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t g = __hip_atomic_fetch_add(&ctl->flow_groups[par][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g + 1u == groups) {
+          __hip_atomic_store(&ctl->flow_groups[par][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          fold = true;
+        }
+      }
+    }
+  }
+  fold = __shfl((int)fold, 0, 64) != 0;
+  if (!fold) return;
+  const uint32_t k = q >> 1, par = k & 1u;
+  const double* sums = a.tile_sum + (size_t)par * a.tile_cap;
+  const uint32_t* counts = a.tile_count + (size_t)par * a.tile_cap;
+  double s = 0.0;
+  uint32_t cn = 0;
+  uint32_t m = lane;
+  for (; m + 7u * 64u < sh.count; m += 8u * 64u) {
+    double v[8];
+    uint32_t w[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      v[u] = coherent_f64(sums, m + u * 64u);
+      w[u] = coherent_u32(counts, m + u * 64u);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s += v[u];
+      cn += w[u];
+    }
+  }
+  for (; m < sh.count; m += 64u) {
+    s += coherent_f64(sums, m);
+    cn += coherent_u32(counts, m);
+  }
+  wave_fold(s, cn);
+  if (lane == 0) {
+    const double average = s / (double)(cn > 1u ? cn : 1u);
+    const uint32_t at = log_base + k;
+    if (at < (uint32_t)kResidualLogCap) {
+      ctl->residual_log[at] = a.which == 0 ? average : -average;
+      ctl->loaded_log[at] = cn;
+    }
+    const bool settled = k >= a.min_iterations && average <= (double)a.tolerance;
+    const uint32_t stop = (settled || k >= a.max_iterations) ? 1u : 0u;
+    __builtin_amdgcn_s_waitcnt(0);
+    __hip_atomic_store(&ctl->flow_decision[par][0], (k + 1u) << 1 | stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    flow_commit(ctl);
+  }
+}
+
+// The flow passes of one block. Each round: stage the task's neighbourhood,
+// then the control wave publishes the last round's task (every wave has
+// waited for its stores), takes the next task and checks once whether it may
+// start, while the other waves sum. A next task not found ready is waited
+// for at the start of its round, after the last task is published.
+template <int FORMAT>
+__device__ void flow_passes(const SolveArgs& a, const TileShape& sh, float4* lds, TileState (&tstate)[3],
+                            uint2* task_s, int* ready_s, double* wsum, uint32_t* wcnt, const uint32_t* begin,
+                            uint32_t log_base) {
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+  constexpr uint32_t kControl = kSolveWaves - 1;
+  const uint32_t cap = a.lds_cap;
+  float4* A = lds;
+  float2* B = reinterpret_cast<float2*>(lds + cap);
+  float4* Q = lds;
+  uint32_t groups = 0;
+  for (uint32_t v = 0; v < kXcds; ++v) groups += begin[v + 1] > begin[v] ? 1u : 0u;
+  if (wave == kControl) {
+    uint2 first = flow_take(a, begin, lane);
+    if (first.y != kNoTile && flow_wait(a, sh, first, lane) <= 0) first.y = kNoTile;
+    tile_scan(tile_load(sh, first.y, first.y != kNoTile, a.prm.cell_start, lane), tstate[0], lane);
+    if (lane == 0) {
+      task_s[0] = first;
+      ready_s[0] = 1;
+    }
+  }
+  __syncthreads();
+  uint2 pending = make_uint2(0u, kNoTile);  // the last round's task, not yet published
+  for (uint32_t r = 0;; ++r) {
+    const uint32_t cur = r & 1u, nxt = cur ^ 1u;
+    uint2 task = task_s[cur];
+    if (task.y == kNoTile) break;
+    if (!ready_s[cur]) {
+      // publish, then wait
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (wave == kControl) {
+        if (pending.y != kNoTile) flow_publish(a, sh, pending, begin, groups, log_base, lane);
+        if (flow_wait(a, sh, task, lane) <= 0 && lane == 0) task_s[cur].y = kNoTile;
+      }
+      pending.y = kNoTile;
+      __syncthreads();
+      task = task_s[cur];
+      if (task.y == kNoTile) break;
+    }
+    const uint32_t q = task.x, k = q >> 1;
+    const bool rate = !(q & 1u);
+    float4* vcur = (k & 1u) ? a.V2 : a.V;
+    float4* vnext = (k & 1u) ? a.V : a.V2;
+    const Glob g{a.P, vcur, a.kappa};
+    const TileState& ts = tstate[cur];
+    const uint32_t stride = __builtin_amdgcn_readfirstlane(ts.stride);
+    const uint32_t count = ts.ppre[kProcessedColumns];
+    const bool fits = kStagedColumns * stride <= cap;
+    if (fits && count) stage_tile(a, g, rate ? kRatePass : kPushPass, ts, stride, A, B, Q, nullptr);
+    // this round's staging and the last round's stores (tile sum included)
+    // complete in every wave before the last round's task is published
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    uint2 next = make_uint2(0u, kNoTile);
+    int next_ready = 1;
+    TileLoad nl{0u, 0u, kNoTile};
+    if (wave == kControl) {
+      if (pending.y != kNoTile) flow_publish(a, sh, pending, begin, groups, log_base, lane);
+      next = flow_take(a, begin, lane);
+      if (next.y != kNoTile) {
+        next_ready = flow_ready(a, sh, next.x, next.y, lane);
+        if (next_ready < 0) next.y = kNoTile;
+      }
+      nl = tile_load(sh, next.y, next.y != kNoTile, a.prm.cell_start, lane);
+    }
+    double acc = 0.0;
+    uint32_t cnt = 0;
+    for (uint32_t p0 = t; p0 < count; p0 += kSolveThreads) {
+      uint32_t p;
+      const uint32_t i = processed_index(ts, p0, p);
+      const Lds l{A, B, Q, &ts, p, 4u * (p >> 1) + (p & 1u), stride};
+      if (rate) {
+        const float e = rate_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
+        acc += (double)e;
+        cnt += e > 0.0f ? 1u : 0u;
+      } else {
+        push_one<FORMAT, false>(a, i, g, &l, fits, 0, 0, vnext);
+      }
+    }
+    if (rate) {
+      wave_fold(acc, cnt);
+      if (lane == 0) {
+        wsum[wave] = acc;
+        wcnt[wave] = cnt;
+      }
+    }
+    if (wave == kControl) {
+      tile_scan(nl, tstate[nxt], lane);
+      if (lane == 0) {
+        task_s[nxt] = next;
+        ready_s[nxt] = next_ready > 0 ? 1 : 0;
+      }
+    }
+    __syncthreads();
+    if (rate && wave == 0) {
+      // the tile's error sum, through the L2, in its iteration's set
+      double v = lane < kSolveWaves ? wsum[lane] : 0.0;
+      uint32_t c = lane < kSolveWaves ? wcnt[lane] : 0u;
+      for (int h = kSolveWaves / 2; h >= 1; h >>= 1) {
+        v = v + __shfl_down(v, h, 64);
+        c += (uint32_t)__shfl_down((int)c, h, 64);
+      }
+      if (lane == 0) {
+        const uint32_t at = (k & 1u) * a.tile_cap + task.y;
+        const uint64_t bits = (uint64_t)__double_as_longlong(v);
+        const u32x2 w = {(uint32_t)bits, (uint32_t)(bits >> 32)};
+        __builtin_amdgcn_raw_buffer_store_b64(w, whole(a.tile_sum), at * 8u, 0, kCoherentLoad);
+        __builtin_amdgcn_raw_buffer_store_b32(c, whole(a.tile_count), at * 4u, 0, kCoherentLoad);
+      }
+    }
+    pending = task;
+  }
+  // the last task: its stores complete, then it is published
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (wave == kControl && pending.y != kNoTile) flow_publish(a, sh, pending, begin, groups, log_base, lane);
+}
+
+// The flow solve (tiled, passes waiting tile by tile instead of at grid
+// barriers) as a kernel of its own, so that its control code does not take
+// registers from the barrier kernel's passes. The density solve's launch
+// runs the viscosity pass first (tiles queued per pass as in the barrier
+// kernel, then one grid barrier), after clearing the flow state the
+// divergence solve left.
+template <int FORMAT>
+__global__ __launch_bounds__(1024) void k_dfsph_flow(SolveArgs a) {
+  extern __shared__ float4 lds[];
+  const StepParams& prm = a.prm;
+  Control* ctl = prm.ctl;
+  if (list_aborted(ctl)) return;
+  const uint32_t t = threadIdx.x, rank = blockIdx.x / kXcds;
+  __shared__ double wsum[kSolveWaves];
+  __shared__ uint32_t wcnt[kSolveWaves];
+  __shared__ TileState tstate[3];
+  __shared__ uint32_t grab_s[2];
+  __shared__ TileShape shape_s;
+  __shared__ uint32_t begin_s[kXcds + 1];
+  __shared__ uint2 task_s[2];
+  __shared__ int ready_s[2];
+  __shared__ uint32_t log_base_s;
+  if (t == 0) shape_s = tile_shape(ctl);
+  __syncthreads();
+  const TileShape& sh = shape_s;
+  if (sh.count > a.tile_cap) {
+    if (blockIdx.x == 0 && t == 0) atomicOr(&ctl->flags, kFlagTiles);
+    return;
+  }
+  if (a.Vin) {
+    // the divergence solve's flow passes leave their tiles and control
+    // words behind; this launch starts them again
+    const uint32_t listed = ctl->flow_begin[kXcds];
+    for (uint32_t c = blockIdx.x * kSolveThreads + t; c < listed; c += gridDim.x * kSolveThreads)
+      a.tile_done[a.flow_list[c]] = 0u;
+    if (blockIdx.x == 0) {
+      if (t < 8 * 32) (&ctl->flow_next[0][0])[t] = 0u;
+      if (t < 2 * 8 * 32) (&ctl->flow_rates[0][0][0])[t] = 0u;
+      if (t < 2 * 32) {
+        (&ctl->flow_groups[0][0])[t] = 0u;
+        (&ctl->flow_decision[0][0])[t] = 0u;
+      }
+      if (t == 0) ctl->flow_resolved = 0u;
+    }
+    // the viscosity pass, as the barrier kernel runs it
+    if (t < 64) {
+      const uint32_t group = blockIdx.x % kXcds;
+      const bool have = rank < fixed_tiles(sh, group);
+      const uint32_t tile = group_begin(sh, group) + rank;
+      tile_scan(tile_load(sh, tile, have, prm.cell_start, t), tstate[2], t);
+    }
+    if (blockIdx.x == 0 && t < kXcds)
+      __hip_atomic_store(&ctl->tile_next[1][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const Glob g{a.P, a.V, a.kappa};
+    tiled_pass<FORMAT>(a, sh, kViscousPass, 0u, lds, tstate, grab_s, g, wsum, wcnt);
+    if (!grid_barrier(ctl, false)) return;
+  }
+  if (t <= kXcds) begin_s[t] = ctl->flow_begin[t];
+  if (t == 0) log_base_s = ctl->log_len;
+  __syncthreads();
+  const uint32_t log_base = log_base_s;
+  flow_passes<FORMAT>(a, sh, lds, tstate, task_s, ready_s, wsum, wcnt, begin_s, log_base);
+  if (!grid_barrier(ctl, false)) return;
+  const uint32_t r = __hip_atomic_load(&ctl->flow_resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!(r & kResolvedStop)) return;  // a wait gave up; the world is flagged
+  const uint32_t stopped = r & ~kResolvedStop;
+  // iteration `stopped`'s velocities are in V2 when it is odd
+  if (stopped & 1u)
+    for (uint32_t i = blockIdx.x * kSolveThreads + t; i < prm.n; i += gridDim.x * kSolveThreads)
+      a.V[i] = coherent4(a.V2, i);
+  if (blockIdx.x == 0 && t < 2 * kXcds)
+    __hip_atomic_store(&ctl->tile_next[t / kXcds][t % kXcds][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (blockIdx.x == 0 && t == 0) {
+    ctl->iterations[a.which] = stopped;
+    ctl->iterations_step[a.which] += stopped;
+    ctl->rates_step += stopped + 1;
+    ctl->log_len = min(log_base + stopped + 1, (uint32_t)kResidualLogCap);
   }
 }
 
@@ -1548,7 +2125,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_solve(SolveArgs a) {
     } else {
       for (uint32_t c = c_begin + rank + half * per_xcd; c < c_end; c += 2 * per_xcd) {
         uint32_t i = c * kChunk + tc;
-        if (i < prm.n) push_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz);
+        if (i < prm.n) push_one<FORMAT, !TILED>(a, i, g, nullptr, false, sysz, szz, a.V);
       }
     }
     if (!grid_barrier(ctl, !tiled)) return;
@@ -1745,11 +2322,13 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
   int per = 0;
   // the smallest occupancy of the solve kernels
-  const void* kernels[5] = {reinterpret_cast<const void*>(k_dfsph_solve<kMasks16, false>),
+  const void* kernels[7] = {reinterpret_cast<const void*>(k_dfsph_solve<kMasks16, false>),
                             reinterpret_cast<const void*>(k_dfsph_solve<kMasks32, false>),
                             reinterpret_cast<const void*>(k_dfsph_solve<kLists, false>),
                             reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>),
-                            reinterpret_cast<const void*>(k_dfsph_solve<kCompact, true>)};
+                            reinterpret_cast<const void*>(k_dfsph_solve<kCompact, true>),
+                            reinterpret_cast<const void*>(k_dfsph_flow<kLists>),
+                            reinterpret_cast<const void*>(k_dfsph_flow<kCompact>)};
   per = 1 << 30;
   for (const void* k : kernels) {
     int p = 0;
@@ -1761,8 +2340,10 @@ int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes) {
   per = per < cap ? per : cap;
   if (tiles && per == 1) {
     // both tiled kernels (list and compact formats) get the same room
-    const void* tiled[2] = {reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>),
-                            reinterpret_cast<const void*>(k_dfsph_solve<kCompact, true>)};
+    const void* tiled[4] = {reinterpret_cast<const void*>(k_dfsph_solve<kLists, true>),
+                            reinterpret_cast<const void*>(k_dfsph_solve<kCompact, true>),
+                            reinterpret_cast<const void*>(k_dfsph_flow<kLists>),
+                            reinterpret_cast<const void*>(k_dfsph_flow<kCompact>)};
     int64_t room = (int64_t)lds;
     for (const void* k : tiled) {
       hipFuncAttributes attr;
@@ -1792,7 +2373,8 @@ bool dfsph_solve_tiled(const StepParams& prm, uint32_t tile_bytes) {
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
                               float4* kappa, float* errors, double* tile_sum, uint32_t* tile_count,
                               uint32_t tile_cap, int which, float tolerance, uint32_t min_it, uint32_t max_it,
-                              int grid, uint32_t tile_bytes, const float4* vin, float* forces) {
+                              int grid, uint32_t tile_bytes, const float4* vin, float* forces,
+                              const FlowBuffers& flow) {
   if (prm.n == 0) return hipSuccess;
   SolveArgs a;
   a.prm = prm;
@@ -1817,6 +2399,14 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   a.Vin = vin;
   a.forces = forces;
   a.lds_cap = tiled ? tile_bytes / kTileBytes : 0u;
+  const bool flowing = tiled && flow.v2 && flow.tile_done && flow.tile_work && flow.tile_list;
+  a.V2 = flowing ? flow.v2 : nullptr;
+  a.tile_done = flow.tile_done;
+  a.tile_work = flow.tile_work;
+  a.flow_list = flow.tile_list;
+  a.through = flowing ? 1u : 0u;
+  // the flow solve's tiles, once a substep, before its divergence solve
+  if (flowing && which == 0) hipLaunchKernelGGL(k_flow_tiles, dim3(1), dim3(kSolveThreads), 0, s, a);
   // a multiple of 8 blocks so every XCD gets the same share; at least one
   // chunk per XCD group or the small grid of a small world
   uint32_t blocks = (uint32_t)grid;
@@ -1831,7 +2421,11 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   // guard.
   const void* fn;
   size_t lds = 0;
-  if (tiled && prm.format == kCompact) {
+  if (flowing) {
+    fn = prm.format == kCompact ? reinterpret_cast<const void*>(k_dfsph_flow<kCompact>)
+                                : reinterpret_cast<const void*>(k_dfsph_flow<kLists>);
+    lds = tile_bytes;
+  } else if (tiled && prm.format == kCompact) {
     fn = reinterpret_cast<const void*>(k_dfsph_solve<kCompact, true>);
     lds = tile_bytes;
   } else if (tiled) {

@@ -69,10 +69,21 @@ void launch_flow_totals(hipStream_t s, const float4* V, uint32_t n, float rest, 
 // DFSPH (kernels_dfsph.hip)
 hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
 int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes);
+// The flow solve's buffers: a second velocity buffer (n), and per solve tile
+// its passes finished, its particles and the list of non-empty tiles
+// (tile_cap each; tile_sum and tile_count then hold 2 * tile_cap). All null:
+// the tiled solve's passes meet at grid barriers instead.
+struct FlowBuffers {
+  float4* v2 = nullptr;
+  uint32_t* tile_done = nullptr;
+  uint32_t* tile_work = nullptr;
+  uint32_t* tile_list = nullptr;
+};
 hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4* P, float4* V, const float* alpha,
                               float4* kappa, float* errors, double* tile_sum, uint32_t* tile_count,
                               uint32_t tile_cap, int which, float tolerance, uint32_t min_it, uint32_t max_it,
-                              int grid, uint32_t tile_bytes, const float4* vin = nullptr, float* forces = nullptr);
+                              int grid, uint32_t tile_bytes, const float4* vin = nullptr, float* forces = nullptr,
+                              const FlowBuffers& flow = FlowBuffers());
 // Whether the solve runs on LDS tiles (then its density solve also runs the
 // viscosity pass, given the velocities before it).
 bool dfsph_solve_tiled(const StepParams& prm, uint32_t tile_bytes);

@@ -192,6 +192,10 @@ int32_t World::init_device() {
   format_ = solve_lds_ ? kCompact : kLists;
   if (fmt) format_ = std::min(std::max(std::atoi(fmt), (int)kMasks16), (int)kCompact);
   if (format_ == kCompact && !solve_lds_) format_ = kLists;  // compact lists need the tiled solve
+  // SOLUNAR_SOLVE_FLOW=0 makes the tiled solve's passes meet at grid
+  // barriers instead of waiting tile by tile on their neighbours
+  const char* flow = std::getenv("SOLUNAR_SOLVE_FLOW");
+  flow_ = !(flow && std::atoi(flow) == 0);
   // SOLUNAR_BODY_PAIR_SLACK (default 65536) is the smallest room for
   // body-point pair records and the slack a step adds to the last step's
   // count; a small value makes coupled steps outgrow their room (tests)
@@ -938,8 +942,13 @@ int32_t World::ensure_cells(uint64_t volume) {
   // most max(1, 2V/3) (sy = 1 and sx or sz >= 2 halves a factor; sy >= 2
   // gives ceil(sy/2) <= 2sy/3)
   const size_t stop = 2 * cap / 3 + 2;
-  SFL_HIP(stop_sum_.reserve(stop, 0, stream_), "hipMalloc(stop sums)");
-  SFL_HIP(stop_count_.reserve(stop, 0, stream_), "hipMalloc(stop counts)");
+  SFL_HIP(stop_sum_.reserve(2 * stop, 0, stream_), "hipMalloc(stop sums)");
+  SFL_HIP(stop_count_.reserve(2 * stop, 0, stream_), "hipMalloc(stop counts)");
+  if (flow_ && format_ == kCompact) {
+    SFL_HIP(flow_done_.reserve(stop, 0, stream_), "hipMalloc(flow tiles)");
+    SFL_HIP(flow_work_.reserve(stop, 0, stream_), "hipMalloc(flow tiles)");
+    SFL_HIP(flow_list_.reserve(stop, 0, stream_), "hipMalloc(flow tiles)");
+  }
   SFL_HIP(hipStreamSynchronize(stream_), "sync");
   cell_cap_ = cap;
   return SFL_OK;
@@ -1110,7 +1119,7 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   phase_begin(SFL_PHASE_SOLVE);
   SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, stop_sum_.ptr,
                              stop_count_.ptr, stop_cap(), 0, s.divergence_tolerance, s.min_iterations, s.max_iterations,
-                             solve_grid_, solve_lds_),
+                             solve_grid_, solve_lds_, nullptr, nullptr, flow_buffers(va_.ptr)),
           "cooperative launch (divergence solve)");
   phase_end(SFL_PHASE_SOLVE);
   // the tiled density solve runs the viscosity pass (VB -> VA) itself
@@ -1124,7 +1133,7 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, va_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, stop_sum_.ptr,
                              stop_count_.ptr, stop_cap(), 1, s.density_tolerance, s.min_iterations, s.max_iterations,
                              solve_grid_, solve_lds_, fused ? vb_.ptr : nullptr,
-                             fused && keep_forces_ ? forces_.ptr : nullptr),
+                             fused && keep_forces_ ? forces_.ptr : nullptr, flow_buffers(vb_.ptr)),
           "cooperative launch (density solve)");
   phase_end(SFL_PHASE_SOLVE);
   phase_begin(SFL_PHASE_ADVANCE);
@@ -1769,7 +1778,7 @@ size_t World::device_bytes() const {
   return sizeof(Control) + b(pa_) + b(va_) + b(pb_) + b(vb_) + b(aux_) + b(kappa_) + b(errors_) + b(forces_) +
          b(slot_rank_) + b(scatter_) + b(counts_) + b(cell_start_) + b(tile_sums_) + b(scan_total_) + b(pairs_) +
          b(masks_) + b(own_slot_) + b(near_) + b(factor_) +
-         b(pair_counts_) + b(stop_sum_) + b(stop_count_) + b(walls_d_) + b(wall_start_d_) + b(wall_keys_d_) + b(skeys_in_) + b(skeys_) + b(cell_keys_) +
+         b(pair_counts_) + b(stop_sum_) + b(stop_count_) + b(flow_done_) + b(flow_work_) + b(flow_list_) + b(walls_d_) + b(wall_start_d_) + b(wall_keys_d_) + b(skeys_in_) + b(skeys_) + b(cell_keys_) +
          b(svals_in_) + b(svals_) + b(heads_) + b(head_scan_) + b(sstart_) + b(ssort_temp_) + b(fstart_) + b(fitems_) +
          b(wstart_) + b(witems_) + b(list_counts_) + b(flow_flags_) + b(flow_start_) + b(flow_tiles_) +
          b(flow_total_) + b(body_local_) + b(probe_local_) + b(bodies_d_) + b(ms_[0]) + b(ms_[1]) + b(mv_[0]) +

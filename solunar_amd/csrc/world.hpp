@@ -14,6 +14,7 @@
 #include "../../include/solunar_fluid.h"
 #include "couple.hpp"
 #include "device.hpp"
+#include "launch.hpp"
 
 namespace sfl {
 
@@ -286,10 +287,26 @@ class World {
   // keep flags and their scan (drains and the domain), free-slot flags and
   // their scan (emitters)
   DeviceBuffer<uint32_t> flow_flags_, flow_start_, flow_tiles_, flow_total_;
+  // (two sets of stop_cap(): the flow solve keeps one per iteration parity)
   DeviceBuffer<double> stop_sum_;
   DeviceBuffer<uint32_t> stop_count_;
   uint32_t stop_cap() const {
-    return (uint32_t)std::min<size_t>(std::min(stop_sum_.cap, stop_count_.cap), 0xffffffffu);
+    return (uint32_t)std::min<size_t>(std::min(stop_sum_.cap, stop_count_.cap) / 2, 0xffffffffu);
+  }
+  // the flow solve's per tile passes, particles and list of non-empty
+  // tiles (stop_cap() each); flow_ is false when SOLUNAR_SOLVE_FLOW=0 makes
+  // the tiled solve's passes meet at grid barriers
+  DeviceBuffer<uint32_t> flow_done_, flow_work_, flow_list_;
+  bool flow_ = true;
+  FlowBuffers flow_buffers(float4* v2) const {
+    FlowBuffers f;
+    if (flow_ && flow_done_.cap >= stop_cap() && flow_work_.cap >= stop_cap() && flow_list_.cap >= stop_cap()) {
+      f.v2 = v2;
+      f.tile_done = flow_done_.ptr;
+      f.tile_work = flow_work_.ptr;
+      f.tile_list = flow_list_.ptr;
+    }
+    return f;
   }
   DeviceBuffer<float4> walls_d_;
   DeviceBuffer<uint32_t> wall_start_d_;
