@@ -104,6 +104,9 @@ struct Control {
   alignas(128) uint32_t flow_groups[2][32];
   alignas(128) uint32_t flow_decision[2][32];
   alignas(128) uint32_t flow_resolved;
+  // m + 1 when the push of iteration m waits for iteration m's own decision
+  // (its error is already near the tolerance), else 0
+  alignas(128) uint32_t flow_cautious;
   // the slabs of the groups as ranges of the listed tiles (flow_list), set per substep
   alignas(128) uint32_t flow_begin[9];
 };

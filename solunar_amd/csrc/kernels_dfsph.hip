@@ -1519,6 +1519,7 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
 // publication completes an iteration.
 
 constexpr uint32_t kTaskDone = 0xffffffffu;  // tile_done of an empty tile
+constexpr uint32_t kTaskWeight = 512;       // a task's cost beside its particles, in particles
 constexpr uint32_t kResolvedStop = 0x80000000u;
 
 // Particles of a tile (its eight processed column ranges).
@@ -1556,7 +1557,6 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
   uint32_t work = 0, live = 0;
   for (uint32_t k = lo; k < hi; ++k) {
     const uint32_t n = tile_particles(sh, k, a.prm.cell_start);
-    a.tile_work[k] = n;
     a.tile_done[k] = n ? 0u : kTaskDone;
     if (!n) {
       a.tile_sum[k] = 0.0;
@@ -1564,7 +1564,11 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
       a.tile_sum[a.tile_cap + k] = 0.0;
       a.tile_count[a.tile_cap + k] = 0u;
     }
-    work += n;
+    // a task costs about as much as 512 particles besides its particles:
+    // slabs of sparse tiles take more tasks for the same particles
+    const uint32_t w = n ? n + kTaskWeight : 0u;
+    a.tile_work[k] = w;
+    work += w;
     live += n ? 1u : 0u;
   }
   work_pre[t] = work;
@@ -1601,9 +1605,9 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
   __syncthreads();
   const uint64_t total = work_pre[kSolveThreads];
   const uint32_t pw = work_pre[t], pl = live_pre[t];
-  // slab v starts at the first tile whose preceding tiles hold at least
-  // ceil(total * v / 8) particles (bt); its list entries start after the
-  // non-empty tiles before it
+  // slab v starts at the first tile whose preceding tiles weigh at least
+  // ceil(total * v / 8) (bt); its list entries start after the non-empty
+  // tiles before it
   __shared__ uint32_t bt[kXcds + 1], bl[kXcds + 1], wave_n[kSolveWaves];
   if (t == 0) {
     bt[0] = 0u;
@@ -1672,7 +1676,10 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
     (&ctl->flow_groups[0][0])[t] = 0u;
     (&ctl->flow_decision[0][0])[t] = 0u;
   }
-  if (t == 0) ctl->flow_resolved = 0u;
+  if (t == 0) {
+    ctl->flow_resolved = 0u;
+    ctl->flow_cautious = 0u;
+  }
 }
 
 // A task's neighbour wait, by the control wave: every tile within one of T
@@ -1690,6 +1697,12 @@ __device__ __forceinline__ int flow_ready(const SolveArgs& a, const TileShape& s
     if (q >= 2u * (r & ~kResolvedStop) + 1u) return -1;
   } else if ((q & 1u) && q >= 3u && r < (q >> 1)) {
     return 0;
+  } else if (q & 1u) {
+    // a push the last decision marked as likely the last waits for its own
+    uint32_t c = 0;
+    if (lane == 0) c = __hip_atomic_load(&ctl->flow_cautious, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    c = __builtin_amdgcn_readfirstlane(c);
+    if (c == (q >> 1) + 1u && r < c) return 0;
   }
   if (q == 0) return 1;
   bool ok = true;
@@ -1817,6 +1830,10 @@ __device__ void flow_publish(const SolveArgs& a, const TileShape& sh, uint2 task
     }
     const bool settled = k >= a.min_iterations && average <= (double)a.tolerance;
     const uint32_t stop = (settled || k >= a.max_iterations) ? 1u : 0u;
+    // the next iteration may well be the last: its push waits for its own
+    // decision instead of running ahead of it
+    if (!stop && k + 1u >= a.min_iterations && average <= 2.0 * (double)a.tolerance)
+      __hip_atomic_store(&ctl->flow_cautious, k + 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
     __hip_atomic_store(&ctl->flow_decision[par][0], (k + 1u) << 1 | stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
@@ -1990,7 +2007,10 @@ __global__ __launch_bounds__(1024) void k_dfsph_flow(SolveArgs a) {
         (&ctl->flow_groups[0][0])[t] = 0u;
         (&ctl->flow_decision[0][0])[t] = 0u;
       }
-      if (t == 0) ctl->flow_resolved = 0u;
+      if (t == 0) {
+        ctl->flow_resolved = 0u;
+        ctl->flow_cautious = 0u;
+      }
     }
     // the viscosity pass, as the barrier kernel runs it
     if (t < 64) {
