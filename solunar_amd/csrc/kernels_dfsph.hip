@@ -679,14 +679,18 @@ __device__ __forceinline__ int32_t batch_slot(const Lds& l, const Ent& e, bool l
 // The rate pass's fluid sum over the tile's LDS: batches of four entries,
 // three in flight from global memory, each batch's LDS reads issued
 // together before its terms are added in list order.
+// `first`, `second`: the particle's first two batches, already loaded.
 template <int FORMAT>
 __device__ __forceinline__ float fluid_rate_lds(const SolveArgs& a, uint32_t i, float4 at, float4 vi, uint32_t nf,
-                                                const Lds& l) {
+                                                const Lds& l, const Ent (&first)[4], const Ent (&second)[4]) {
   const StepParams& prm = a.prm;
   float rate = 0.0f;
   Ent b0[4], b1[4], b2[4];
-  load_batch<FORMAT>(prm, i, 0, nf, b0);
-  load_batch<FORMAT>(prm, i, 4, nf, b1);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    b0[u] = first[u];
+    b1[u] = second[u];
+  }
   load_batch<FORMAT>(prm, i, 8, nf, b2);
   auto sum = [&](const Ent(&e)[4], uint32_t k) {
     float4 pv[4];
@@ -789,12 +793,81 @@ __device__ __forceinline__ float rate_one(const SolveArgs& a, uint32_t i, const 
   uint32_t nf = counts & 0xffffu, nw = counts >> 16;
   float rate;
   if constexpr (FORMAT == kLists || FORMAT == kCompact) {
-    rate = staged ? fluid_rate_lds<FORMAT>(a, i, at, vi, nf, *l)
-                  : fluid_rate<FORMAT, DEEP>(a, i, at, vi, nf, g, l, sysz, szz);
+    if (staged) {
+      Ent b0[4], b1[4];
+      load_batch<FORMAT>(a.prm, i, 0, nf, b0);
+      load_batch<FORMAT>(a.prm, i, 4, nf, b1);
+      rate = fluid_rate_lds<FORMAT>(a, i, at, vi, nf, *l, b0, b1);
+    } else {
+      rate = fluid_rate<FORMAT, DEEP>(a, i, at, vi, nf, g, l, sysz, szz);
+    }
   } else {
     rate = fluid_rate<FORMAT>(a, i, at, vi, nf, g, l, sysz, szz);
   }
   return finish_rate<FORMAT>(a, i, at, vi, rate, nf, nw);
+}
+
+// The push pass's fluid sum over the tile's LDS (at = (x, own stiffness)),
+// as the rate pass's; `first`, `second`: the first two batches, loaded.
+template <int FORMAT>
+__device__ __forceinline__ void push_fluid_lds(const SolveArgs& a, uint32_t i, float4 at, uint32_t nf, const Lds& l,
+                                               const Ent (&first)[4], const Ent (&second)[4], float& cx, float& cy,
+                                               float& cz) {
+  const StepParams& prm = a.prm;
+  const float mass = prm.mass, own = at.w;
+  Ent b0[4], b1[4], b2[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    b0[u] = first[u];
+    b1[u] = second[u];
+  }
+  load_batch<FORMAT>(prm, i, 8, nf, b2);
+  auto sum = [&](const Ent(&e)[4], uint32_t k) {
+    float4 q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] = l.Q[batch_slot<FORMAT>(l, e[u], k + u < nf)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float f = e[u].f;
+      const float gx = (at.x - q[u].x) * f, gy = (at.y - q[u].y) * f, gz = (at.z - q[u].z) * f;
+      const float k2 = mass * (own + q[u].w);
+      // past the list: subtract +0, the identity of IEEE subtraction
+      const bool live = k + u < nf;
+      cx = cx - (live ? gx * k2 : 0.0f);
+      cy = cy - (live ? gy * k2 : 0.0f);
+      cz = cz - (live ? gz * k2 : 0.0f);
+    }
+  };
+  for (uint32_t k = 0; k < nf; k += 12) {
+    sum(b0, k);
+    load_batch<FORMAT>(prm, i, k + 12, nf, b0);
+    if (k + 4 >= nf) break;
+    sum(b1, k + 4);
+    load_batch<FORMAT>(prm, i, k + 16, nf, b1);
+    if (k + 8 >= nf) break;
+    sum(b2, k + 8);
+    load_batch<FORMAT>(prm, i, k + 20, nf, b2);
+  }
+}
+
+// The push's walls and velocity update of particle i after its fluid sum.
+template <int FORMAT>
+__device__ __forceinline__ void push_finish(const SolveArgs& a, uint32_t i, float4 at, uint32_t nf, uint32_t nw,
+                                            float4 vi, float cx, float cy, float cz, float4* vout) {
+  const StepParams& prm = a.prm;
+  const uint32_t w0 = FORMAT == kLists ? nf : 0u;
+  const float kw = prm.mass * at.w;
+  walk_walls(prm, i, w0, w0 + nw, [&](float f, float4 pw, bool live) {
+    float gx = (at.x - pw.x) * f, gy = (at.y - pw.y) * f, gz = (at.z - pw.z) * f;
+    cx = cx - (live ? gx * kw : 0.0f);
+    cy = cy - (live ? gy * kw : 0.0f);
+    cz = cz - (live ? gz * kw : 0.0f);
+  });
+  const float dt = prm.dt;
+  vi.x = vi.x + cx * dt;
+  vi.y = vi.y + cy * dt;
+  vi.z = vi.z + cz * dt;
+  put4(vout, i, vi, a.through != 0);
 }
 
 // Writes the pushed velocity to `vout` (g.V itself but in the flow solve).
@@ -827,36 +900,10 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
       }
     });
   } else if (staged) {
-    Ent b0[4], b1[4], b2[4];
+    Ent b0[4], b1[4];
     load_batch<FORMAT>(prm, i, 0, nf, b0);
     load_batch<FORMAT>(prm, i, 4, nf, b1);
-    load_batch<FORMAT>(prm, i, 8, nf, b2);
-    auto sum = [&](const Ent(&e)[4], uint32_t k) {
-      float4 q[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) q[u] = l->Q[batch_slot<FORMAT>(*l, e[u], k + u < nf)];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float f = e[u].f;
-        const float gx = (at.x - q[u].x) * f, gy = (at.y - q[u].y) * f, gz = (at.z - q[u].z) * f;
-        const float k2 = mass * (own + q[u].w);
-        // past the list: subtract +0, the identity of IEEE subtraction
-        const bool live = k + u < nf;
-        cx = cx - (live ? gx * k2 : 0.0f);
-        cy = cy - (live ? gy * k2 : 0.0f);
-        cz = cz - (live ? gz * k2 : 0.0f);
-      }
-    };
-    for (uint32_t k = 0; k < nf; k += 12) {
-      sum(b0, k);
-      load_batch<FORMAT>(prm, i, k + 12, nf, b0);
-      if (k + 4 >= nf) break;
-      sum(b1, k + 4);
-      load_batch<FORMAT>(prm, i, k + 16, nf, b1);
-      if (k + 8 >= nf) break;
-      sum(b2, k + 8);
-      load_batch<FORMAT>(prm, i, k + 20, nf, b2);
-    }
+    push_fluid_lds<FORMAT>(a, i, at, nf, *l, b0, b1, cx, cy, cz);
   } else if constexpr (!DEEP) {
     Ent nxt[4];
     fluid_entries<FORMAT>(prm, i, 0, nxt);
@@ -921,20 +968,7 @@ __device__ __forceinline__ void push_one(const SolveArgs& a, uint32_t i, const G
       }
     }
   }
-  const uint32_t w0 = FORMAT == kLists ? nf : 0u;
-  const float kw = mass * own;
-  walk_walls(prm, i, w0, w0 + nw, [&](float f, float4 pw, bool live) {
-    float gx = (at.x - pw.x) * f, gy = (at.y - pw.y) * f, gz = (at.z - pw.z) * f;
-    cx = cx - (live ? gx * kw : 0.0f);
-    cy = cy - (live ? gy * kw : 0.0f);
-    cz = cz - (live ? gz * kw : 0.0f);
-  });
-  const float dt = prm.dt;
-  float4 vi = shared4<!DEEP>(g.V, i);
-  vi.x = vi.x + cx * dt;
-  vi.y = vi.y + cy * dt;
-  vi.z = vi.z + cz * dt;
-  put4(vout, i, vi, a.through != 0);
+  push_finish<FORMAT>(a, i, at, nf, nw, shared4<!DEEP>(g.V, i), cx, cy, cz, vout);
 }
 
 // ---- LDS tiles --------------------------------------------------------------
