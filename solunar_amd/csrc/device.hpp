@@ -61,6 +61,20 @@ struct alignas(64) Stripe {
   uint32_t pad[6];
 };
 
+// One flow solve's control words: each XCD group's task queue, its rate
+// tasks finished per iteration parity, the groups finished per parity, each
+// iteration's decision per parity, the decisions committed in iteration
+// order, and the iteration whose push waits for its own decision (m + 1,
+// or 0); one cache line each.
+struct FlowWords {
+  alignas(128) uint32_t next[8][32];
+  alignas(128) uint32_t rates[2][8][32];
+  alignas(128) uint32_t groups[2][32];
+  alignas(128) uint32_t decision[2][32];
+  alignas(128) uint32_t resolved;
+  alignas(128) uint32_t cautious;
+};
+
 // Device-resident control block of one world: cell box, report reductions,
 // solve counters and the persistent kernels' barrier.
 struct Control {
@@ -94,19 +108,9 @@ struct Control {
   // density), for parity logs
   double residual_log[kResidualLogCap];
   uint32_t loaded_log[kResidualLogCap];
-  // the flow solve (cleared once a substep by its tile set-up, and again at
-  // the start of the density solve's launch): each XCD group's task queue,
-  // its rate tasks finished per iteration parity, the groups finished per
-  // parity, each iteration's decision per parity, and the decisions
-  // committed in iteration order; one cache line each
-  alignas(128) uint32_t flow_next[8][32];
-  alignas(128) uint32_t flow_rates[2][8][32];
-  alignas(128) uint32_t flow_groups[2][32];
-  alignas(128) uint32_t flow_decision[2][32];
-  alignas(128) uint32_t flow_resolved;
-  // m + 1 when the push of iteration m waits for iteration m's own decision
-  // (its error is already near the tolerance), else 0
-  alignas(128) uint32_t flow_cautious;
+  // the flow solves of a substep (0 divergence, 1 density), both set up
+  // once a substep by the tile set-up
+  FlowWords flow[2];
   // the slabs of the groups as ranges of the listed tiles (flow_list), set per substep
   alignas(128) uint32_t flow_begin[9];
 };

@@ -29,6 +29,7 @@
 // and all blocks take the same decision with no further barrier.
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "device.hpp"
 #include "launch.hpp"
@@ -451,8 +452,9 @@ struct SolveArgs {
   float* forces;        // that pass's accelerations, or null
   // the flow solve (tiled, no grid barriers between passes; null V2: the
   // passes meet at grid barriers): the second velocity buffer, per tile its
-  // passes finished this launch and its particles; tile_sum and tile_count
-  // then hold two sets of tile_cap, one per iteration parity
+  // passes finished (a half of tile_cap for each solve of the substep) and
+  // its weight; tile_sum and tile_count then hold two sets of tile_cap, one
+  // per iteration parity
   float4* V2;
   uint32_t* tile_done;
   uint32_t* tile_work;
@@ -1136,7 +1138,7 @@ __device__ __forceinline__ void viscous_pair(const StepParams& prm, float4 at, f
 template <int FORMAT>
 __device__ __forceinline__ void viscous_finish(const StepParams& prm, uint32_t i, float4 at, float4 vi, float px,
                                                float py, float pz, uint32_t nf, uint32_t nw, float4* VA,
-                                               float* forces) {
+                                               float* forces, bool through = false) {
   const float density = vi.w;
   const uint32_t w0 = FORMAT == kLists ? nf : 0u;
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
@@ -1181,7 +1183,7 @@ __device__ __forceinline__ void viscous_finish(const StepParams& prm, uint32_t i
     forces[3 * i + 1] = ay;
     forces[3 * i + 2] = az;
   }
-  VA[i] = make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density);
+  put4(VA, i, make_float4(vi.x + ax * prm.dt, vi.y + ay * prm.dt, vi.z + az * prm.dt, density), through);
 }
 
 // The viscosity pass for particle i of a tile, inside the density solve's
@@ -1235,7 +1237,7 @@ __device__ __forceinline__ void viscous_walk(const SolveArgs& a, uint32_t i, con
     sum(b2, k + 8);
     load_batch<FORMAT>(prm, i, k + 20, nf, b2);
   }
-  viscous_finish<FORMAT>(prm, i, at, vi, px, py, pz, nf, nw, a.V, a.forces);
+  viscous_finish<FORMAT>(prm, i, at, vi, px, py, pz, nf, nw, a.V, a.forces, a.through != 0);
 }
 
 template <int FORMAT>
@@ -1519,33 +1521,36 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
 
 // ---- the dataflow solve -------------------------------------------------------
 //
-// The tiled solve without grid barriers between its passes. Pass q of tile
-// T is a task: q = 2k is iteration k's rate pass, q = 2k + 1 its push. A
-// task reads only its tile's neighbourhood, so it waits only for the tiles
-// around it: (q, T) starts once every tile within one tile of T in x, y and
-// z has finished pass q - 1 (tile_done >= q). The velocities are double
-// buffered: iteration k's rate pass reads V_k (V when k is even, V2 when
-// odd) and its push writes V_k+1 into the other buffer, so the push may run
-// before iteration k's stop decision: if iteration k stops, V_k is intact.
-// The push of iteration k + 1 overwrites V_k, so it waits for the decision
-// of iteration k. Each decision is taken by the block that publishes the
-// last rate task of its iteration: it folds the tile sums in the same fixed
-// shape and commits the decision in iteration order (flow_resolved).
+// The tiled solve without grid barriers between its passes. Pass p of tile
+// T is a task. The density solve's launch starts with the viscosity pass
+// (p = 0) and its Jacobi passes follow at p = 1 + q; the divergence solve's
+// Jacobi passes are p = q. Jacobi pass q = 2k is iteration k's rate pass,
+// q = 2k + 1 its push. A task reads only its tile's neighbourhood, so it
+// waits only for the tiles around it: (p, T) starts once every tile within
+// one tile of T in x, y and z has finished pass p - 1 (tile_done >= p). The
+// velocities are double buffered: iteration k's rate pass reads V_k (V when
+// k is even, V2 when odd) and its push writes V_k+1 into the other buffer,
+// so the push may run before iteration k's stop decision: if iteration k
+// stops, V_k is intact. The push of iteration k + 1 overwrites V_k, so it
+// waits for the decision of iteration k. Each decision is taken by the block
+// that publishes the last rate task of its iteration: it folds the tile
+// sums in the same fixed shape and commits the decision in iteration order
+// (FlowWords::resolved).
 //
 // Each XCD group has one queue over its slab of tiles: index j is pass
-// j / len, tile begin + j % len; the slabs are cut at equal particle counts,
-// and a group runs its slab's tasks alone. No block waits on a task that no
+// j / len, tile begin + j % len; the slabs are cut at equal weights, and a
+// group runs its slab's tasks alone. No block waits on a task that no
 // running block will take: a task waits only on tasks of lower passes, every
 // queue hands its tasks out in pass order, and each queue's own blocks take
 // them; past the stop decision, tasks of later passes are dropped, not
 // waited for.
 //
-// Stores that other blocks read (stiffnesses, velocities, tile sums) go
+// Stores that other blocks read (velocities, stiffnesses, tile sums) go
 // through the L2 to memory (device scope, put4), and a task is published
 // only after every wave of its block has waited for its stores, so no L2
 // write-back is needed between tasks; other blocks' data is read with
-// device-scope loads. Empty tiles are marked finished for every pass at the
-// start and skipped.
+// device-scope loads. Empty tiles are marked finished for every pass and
+// skipped.
 //
 // The last wave of a block (the 64 particles of every 1024 that come last,
 // usually none) runs the control: it publishes the previous task, takes the
@@ -1555,6 +1560,16 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
 constexpr uint32_t kTaskDone = 0xffffffffu;  // tile_done of an empty tile
 constexpr uint32_t kTaskWeight = 512;       // a task's cost beside its particles, in particles
 constexpr uint32_t kResolvedStop = 0x80000000u;
+
+__device__ __forceinline__ FlowWords& flow_words(const SolveArgs& a) { return a.prm.ctl->flow[a.which]; }
+
+// Each solve's half of tile_done.
+__device__ __forceinline__ uint32_t* done_of(const SolveArgs& a) {
+  return a.tile_done + (size_t)a.which * a.tile_cap;
+}
+
+// Passes before the Jacobi passes: the density solve's viscosity pass.
+__device__ __forceinline__ uint32_t flow_base(const SolveArgs& a) { return a.Vin ? 1u : 0u; }
 
 // Particles of a tile (its eight processed column ranges).
 __device__ __forceinline__ uint32_t tile_particles(const TileShape& sh, uint32_t tile, const uint32_t* start) {
@@ -1576,9 +1591,9 @@ __device__ __forceinline__ uint32_t tile_particles(const TileShape& sh, uint32_t
 // particles of every tile; the non-empty tiles listed in order (flow_list);
 // the slabs of the XCD groups cut at equal particle counts, as ranges of
 // that list (flow_begin: slab v is list entries begin[v] .. begin[v + 1]);
-// empty tiles marked finished for every pass (tile_done = kTaskDone, their
-// tile sums zero in both sets), the others not started; the control words
-// cleared. Thread t works the contiguous tiles of segment t.
+// empty tiles marked finished for every pass (tile_done = kTaskDone, in
+// the divergence and the density solve's halves; their tile sums zero in
+// both sets), the others not started; both solves' control words cleared. Thread t works the contiguous tiles of segment t.
 __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
   Control* ctl = a.prm.ctl;
   if (list_aborted(ctl)) return;
@@ -1592,6 +1607,7 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
   for (uint32_t k = lo; k < hi; ++k) {
     const uint32_t n = tile_particles(sh, k, a.prm.cell_start);
     a.tile_done[k] = n ? 0u : kTaskDone;
+    a.tile_done[a.tile_cap + k] = n ? 0u : kTaskDone;
     if (!n) {
       a.tile_sum[k] = 0.0;
       a.tile_count[k] = 0u;
@@ -1704,41 +1720,38 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
       __syncthreads();
     }
   }
-  if (t < 8 * 32) (&ctl->flow_next[0][0])[t] = 0u;
-  if (t < 2 * 8 * 32) (&ctl->flow_rates[0][0][0])[t] = 0u;
-  if (t < 2 * 32) {
-    (&ctl->flow_groups[0][0])[t] = 0u;
-    (&ctl->flow_decision[0][0])[t] = 0u;
-  }
-  if (t == 0) {
-    ctl->flow_resolved = 0u;
-    ctl->flow_cautious = 0u;
-  }
+  // both solves' control words
+  uint32_t* words = reinterpret_cast<uint32_t*>(&ctl->flow[0]);
+  for (uint32_t k = t; k < 2 * sizeof(FlowWords) / 4; k += kSolveThreads) words[k] = 0u;
 }
 
 // A task's neighbour wait, by the control wave: every tile within one of T
-// has finished pass q - 1, and the push of iteration k >= 1 finds iteration
-// k - 1 committed as going on. Returns 1 when the task may start, 0 to wait,
-// -1 when the solve stopped before it (the task and every later one are
-// dropped).
-__device__ __forceinline__ int flow_ready(const SolveArgs& a, const TileShape& sh, uint32_t q, uint32_t tile,
+// has finished pass p - 1, and the push of iteration k >= 1 finds iteration
+// k - 1 committed as going on (and its own iteration decided when the last
+// decision marked it as likely the last). Returns 1 when the task may start,
+// 0 to wait, -1 when the solve stopped before it (the task and every later
+// one are dropped).
+__device__ __forceinline__ int flow_ready(const SolveArgs& a, const TileShape& sh, uint32_t p, uint32_t tile,
                                           uint32_t lane) {
-  Control* ctl = a.prm.ctl;
-  uint32_t r = 0;
-  if (lane == 0) r = __hip_atomic_load(&ctl->flow_resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  r = __builtin_amdgcn_readfirstlane(r);
-  if (r & kResolvedStop) {
-    if (q >= 2u * (r & ~kResolvedStop) + 1u) return -1;
-  } else if ((q & 1u) && q >= 3u && r < (q >> 1)) {
-    return 0;
-  } else if (q & 1u) {
-    // a push the last decision marked as likely the last waits for its own
-    uint32_t c = 0;
-    if (lane == 0) c = __hip_atomic_load(&ctl->flow_cautious, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    c = __builtin_amdgcn_readfirstlane(c);
-    if (c == (q >> 1) + 1u && r < c) return 0;
+  const uint32_t base = flow_base(a);
+  if (p >= base) {
+    const uint32_t q = p - base;
+    uint32_t r = 0;
+    if (lane == 0) r = __hip_atomic_load(&flow_words(a).resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r = __builtin_amdgcn_readfirstlane(r);
+    if (r & kResolvedStop) {
+      if (q >= 2u * (r & ~kResolvedStop) + 1u) return -1;
+    } else if ((q & 1u) && q >= 3u && r < (q >> 1)) {
+      return 0;
+    } else if (q & 1u) {
+      uint32_t c = 0;
+      if (lane == 0) c = __hip_atomic_load(&flow_words(a).cautious, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      c = __builtin_amdgcn_readfirstlane(c);
+      if (c == (q >> 1) + 1u && r < c) return 0;
+    }
   }
-  if (q == 0) return 1;
+  if (p == 0) return 1;
+  const uint32_t* done = done_of(a);
   bool ok = true;
   if (lane < 27) {
     const uint32_t tx = tile / (sh.nty * sh.ntz), ty = (tile / sh.ntz) % sh.nty, tzi = tile % sh.ntz;
@@ -1746,7 +1759,7 @@ __device__ __forceinline__ int flow_ready(const SolveArgs& a, const TileShape& s
                   z = (int64_t)tzi + (int)(lane % 3) - 1;
     if (x >= 0 && y >= 0 && z >= 0 && x < sh.ntx && y < sh.nty && z < sh.ntz) {
       const uint32_t u = (uint32_t)((x * sh.nty + y) * sh.ntz + z);
-      ok = __hip_atomic_load(&a.tile_done[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= q;
+      ok = __hip_atomic_load(&done[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= p;
     }
   }
   return __ballot(!ok) == 0 ? 1 : 0;
@@ -1755,16 +1768,15 @@ __device__ __forceinline__ int flow_ready(const SolveArgs& a, const TileShape& s
 // The control wave takes the block's next task from its group's queue: its
 // pass and tile, kNoTile as the tile when the queue is done.
 __device__ __forceinline__ uint2 flow_take(const SolveArgs& a, const uint32_t* begin, uint32_t lane) {
-  Control* ctl = a.prm.ctl;
   const uint32_t v = blockIdx.x % kXcds;
   const uint32_t b = begin[v], len = begin[v + 1] - b;
   if (len == 0) return make_uint2(0u, kNoTile);
   uint32_t j = 0;
-  if (lane == 0) j = __hip_atomic_fetch_add(&ctl->flow_next[v][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0) j = __hip_atomic_fetch_add(&flow_words(a).next[v][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   j = __builtin_amdgcn_readfirstlane(j);
-  const uint32_t q = j / len;
-  if (q > 2u * a.max_iterations) return make_uint2(0u, kNoTile);
-  return make_uint2(q, a.flow_list[b + j % len]);
+  const uint32_t p = j / len;
+  if (p > 2u * a.max_iterations + flow_base(a)) return make_uint2(0u, kNoTile);
+  return make_uint2(p, a.flow_list[b + j % len]);
 }
 
 // Waits until the task may start: 1, or -1 when the solve stopped before it,
@@ -1785,16 +1797,16 @@ __device__ int flow_wait(const SolveArgs& a, const TileShape& sh, uint2 task, ui
 }
 
 // Commits decided iterations in order, by lane 0 of the control wave:
-// flow_resolved = m (iterations 0 .. m - 1 went on) advances while the
-// decision of iteration m is in (flow_decision[m % 2] = (m + 1) << 1 | stop).
-__device__ void flow_commit(Control* ctl) {
-  uint32_t r = __hip_atomic_load(&ctl->flow_resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// resolved = m (iterations 0 .. m - 1 went on) advances while the decision
+// of iteration m is in (decision[m % 2] = (m + 1) << 1 | stop).
+__device__ void flow_commit(FlowWords& fw) {
+  uint32_t r = __hip_atomic_load(&fw.resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (;;) {
     if (r & kResolvedStop) return;
-    const uint32_t d = __hip_atomic_load(&ctl->flow_decision[r & 1u][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t d = __hip_atomic_load(&fw.decision[r & 1u][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if ((d >> 1) != r + 1) return;
     const uint32_t nv = (d & 1u) ? (kResolvedStop | r) : r + 1;
-    if (__hip_atomic_compare_exchange_strong(&ctl->flow_resolved, &r, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+    if (__hip_atomic_compare_exchange_strong(&fw.resolved, &r, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT))
       r = nv;
   }
@@ -1808,21 +1820,22 @@ __device__ void flow_commit(Control* ctl) {
 __device__ void flow_publish(const SolveArgs& a, const TileShape& sh, uint2 task, const uint32_t* begin,
                              uint32_t groups, uint32_t log_base, uint32_t lane) {
   Control* ctl = a.prm.ctl;
-  const uint32_t q = task.x, tile = task.y;
+  FlowWords& fw = flow_words(a);
+  const uint32_t p = task.x, tile = task.y, base = flow_base(a);
   bool fold = false;
   if (lane == 0) {
-    __hip_atomic_store(&a.tile_done[tile], q + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (!(q & 1u)) {
-      const uint32_t par = (q >> 1) & 1u, v = blockIdx.x % kXcds;
-      const uint32_t c = __hip_atomic_fetch_add(&ctl->flow_rates[par][v][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&done_of(a)[tile], p + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (p >= base && !((p - base) & 1u)) {
+      const uint32_t par = ((p - base) >> 1) & 1u, v = blockIdx.x % kXcds;
+      const uint32_t c = __hip_atomic_fetch_add(&fw.rates[par][v][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (c + 1u == begin[v + 1] - begin[v]) {
         // the group's count starts again for iteration k + 2, which no task
         // reaches before this iteration's decision
-        __hip_atomic_store(&ctl->flow_rates[par][v][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&fw.rates[par][v][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_s_waitcnt(0);
-        const uint32_t g = __hip_atomic_fetch_add(&ctl->flow_groups[par][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t g = __hip_atomic_fetch_add(&fw.groups[par][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (g + 1u == groups) {
-          __hip_atomic_store(&ctl->flow_groups[par][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&fw.groups[par][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           fold = true;
         }
       }
@@ -1830,7 +1843,7 @@ __device__ void flow_publish(const SolveArgs& a, const TileShape& sh, uint2 task
   }
   fold = __shfl((int)fold, 0, 64) != 0;
   if (!fold) return;
-  const uint32_t k = q >> 1, par = k & 1u;
+  const uint32_t k = (p - base) >> 1, par = k & 1u;
   const double* sums = a.tile_sum + (size_t)par * a.tile_cap;
   const uint32_t* counts = a.tile_count + (size_t)par * a.tile_cap;
   double s = 0.0;
@@ -1867,11 +1880,11 @@ __device__ void flow_publish(const SolveArgs& a, const TileShape& sh, uint2 task
     // the next iteration may well be the last: its push waits for its own
     // decision instead of running ahead of it
     if (!stop && k + 1u >= a.min_iterations && average <= 2.0 * (double)a.tolerance)
-      __hip_atomic_store(&ctl->flow_cautious, k + 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&fw.cautious, k + 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
-    __hip_atomic_store(&ctl->flow_decision[par][0], (k + 1u) << 1 | stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&fw.decision[par][0], (k + 1u) << 1 | stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
-    flow_commit(ctl);
+    flow_commit(fw);
   }
 }
 
@@ -1886,10 +1899,7 @@ __device__ void flow_passes(const SolveArgs& a, const TileShape& sh, float4* lds
                             uint32_t log_base) {
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   constexpr uint32_t kControl = kSolveWaves - 1;
-  const uint32_t cap = a.lds_cap;
-  float4* A = lds;
-  float2* B = reinterpret_cast<float2*>(lds + cap);
-  float4* Q = lds;
+  const uint32_t base = flow_base(a);
   uint32_t groups = 0;
   for (uint32_t v = 0; v < kXcds; ++v) groups += begin[v + 1] > begin[v] ? 1u : 0u;
   if (wave == kControl) {
@@ -1903,10 +1913,15 @@ __device__ void flow_passes(const SolveArgs& a, const TileShape& sh, float4* lds
   }
   __syncthreads();
   uint2 pending = make_uint2(0u, kNoTile);  // the last round's task, not yet published
-  for (uint32_t r = 0;; ++r) {
+  uint32_t r = 0;
+  // One round; false when the block has no task left. The viscosity rounds
+  // (VISCOUS) and the Jacobi rounds are separate loops, so that neither's
+  // registers crowd the other's.
+  auto round = [&](auto viscous_tag) -> bool {
+    constexpr bool viscous = decltype(viscous_tag)::value;
     const uint32_t cur = r & 1u, nxt = cur ^ 1u;
     uint2 task = task_s[cur];
-    if (task.y == kNoTile) break;
+    if (task.y == kNoTile) return false;
     if (!ready_s[cur]) {
       // publish, then wait
       __builtin_amdgcn_s_waitcnt(0);
@@ -1918,18 +1933,27 @@ __device__ void flow_passes(const SolveArgs& a, const TileShape& sh, float4* lds
       pending.y = kNoTile;
       __syncthreads();
       task = task_s[cur];
-      if (task.y == kNoTile) break;
+      if (task.y == kNoTile) return false;
     }
-    const uint32_t q = task.x, k = q >> 1;
-    const bool rate = !(q & 1u);
+    const uint32_t p = task.x;
+    const uint32_t q = viscous ? 0u : p - base, k = q >> 1;
+    const bool rate = !viscous && !(q & 1u);
     float4* vcur = (k & 1u) ? a.V2 : a.V;
     float4* vnext = (k & 1u) ? a.V : a.V2;
     const Glob g{a.P, vcur, a.kappa};
+    // the LDS: A (float4) then B (float2) for the rate pass, Q (float4) for
+    // the push pass, A then C (float4: vy, vz, density) for the viscosity
+    // pass, which stages 32 bytes a particle instead of 24
+    const uint32_t cap = viscous ? a.lds_cap * kTileBytes / 32u : a.lds_cap;
+    float4* A = lds;
+    float2* B = reinterpret_cast<float2*>(lds + cap);
+    float4* Q = lds;
+    float4* C = lds + cap;
     const TileState& ts = tstate[cur];
     const uint32_t stride = __builtin_amdgcn_readfirstlane(ts.stride);
     const uint32_t count = ts.ppre[kProcessedColumns];
     const bool fits = kStagedColumns * stride <= cap;
-    if (fits && count) stage_tile(a, g, rate ? kRatePass : kPushPass, ts, stride, A, B, Q, nullptr);
+    if (fits && count) stage_tile(a, g, viscous ? kViscousPass : rate ? kRatePass : kPushPass, ts, stride, A, B, Q, C);
     // this round's staging and the last round's stores (tile sum included)
     // complete in every wave before the last round's task is published
     __builtin_amdgcn_s_waitcnt(0);
@@ -1949,10 +1973,12 @@ __device__ void flow_passes(const SolveArgs& a, const TileShape& sh, float4* lds
     double acc = 0.0;
     uint32_t cnt = 0;
     for (uint32_t p0 = t; p0 < count; p0 += kSolveThreads) {
-      uint32_t p;
-      const uint32_t i = processed_index(ts, p0, p);
-      const Lds l{A, B, Q, &ts, p, 4u * (p >> 1) + (p & 1u), stride};
-      if (rate) {
+      uint32_t pc;
+      const uint32_t i = processed_index(ts, p0, pc);
+      const Lds l{A, B, Q, &ts, pc, 4u * (pc >> 1) + (pc & 1u), stride};
+      if constexpr (viscous) {
+        viscous_one<FORMAT>(a, i, &l, C, fits);
+      } else if (rate) {
         const float e = rate_one<FORMAT, false>(a, i, g, &l, fits, 0, 0);
         acc += (double)e;
         cnt += e > 0.0f ? 1u : 0u;
@@ -1992,6 +2018,13 @@ __device__ void flow_passes(const SolveArgs& a, const TileShape& sh, float4* lds
       }
     }
     pending = task;
+    ++r;
+    return true;
+  };
+  // the viscosity rounds while the next task is one, then the Jacobi rounds
+  while (task_s[r & 1u].y != kNoTile && task_s[r & 1u].x < base && round(std::true_type{})) {
+  }
+  while (round(std::false_type{})) {
   }
   // the last task: its stores complete, then it is published
   __builtin_amdgcn_s_waitcnt(0);
@@ -2001,80 +2034,44 @@ __device__ void flow_passes(const SolveArgs& a, const TileShape& sh, float4* lds
 
 // The flow solve (tiled, passes waiting tile by tile instead of at grid
 // barriers) as a kernel of its own, so that its control code does not take
-// registers from the barrier kernel's passes. The density solve's launch
-// runs the viscosity pass first (tiles queued per pass as in the barrier
-// kernel, then one grid barrier), after clearing the flow state the
-// divergence solve left.
+// registers from the barrier kernel's passes. One grid barrier at the end;
+// the iteration the solve stopped at left its velocities in V2 when odd, and
+// they are copied to V.
 template <int FORMAT>
 __global__ __launch_bounds__(1024) void k_dfsph_flow(SolveArgs a) {
   extern __shared__ float4 lds[];
   const StepParams& prm = a.prm;
   Control* ctl = prm.ctl;
   if (list_aborted(ctl)) return;
-  const uint32_t t = threadIdx.x, rank = blockIdx.x / kXcds;
+  const uint32_t t = threadIdx.x;
   __shared__ double wsum[kSolveWaves];
   __shared__ uint32_t wcnt[kSolveWaves];
   __shared__ TileState tstate[3];
-  __shared__ uint32_t grab_s[2];
   __shared__ TileShape shape_s;
   __shared__ uint32_t begin_s[kXcds + 1];
   __shared__ uint2 task_s[2];
   __shared__ int ready_s[2];
   __shared__ uint32_t log_base_s;
-  if (t == 0) shape_s = tile_shape(ctl);
+  if (t == 0) {
+    shape_s = tile_shape(ctl);
+    log_base_s = ctl->log_len;
+  }
+  if (t <= kXcds) begin_s[t] = ctl->flow_begin[t];
   __syncthreads();
   const TileShape& sh = shape_s;
   if (sh.count > a.tile_cap) {
     if (blockIdx.x == 0 && t == 0) atomicOr(&ctl->flags, kFlagTiles);
     return;
   }
-  if (a.Vin) {
-    // the divergence solve's flow passes leave their tiles and control
-    // words behind; this launch starts them again
-    const uint32_t listed = ctl->flow_begin[kXcds];
-    for (uint32_t c = blockIdx.x * kSolveThreads + t; c < listed; c += gridDim.x * kSolveThreads)
-      a.tile_done[a.flow_list[c]] = 0u;
-    if (blockIdx.x == 0) {
-      if (t < 8 * 32) (&ctl->flow_next[0][0])[t] = 0u;
-      if (t < 2 * 8 * 32) (&ctl->flow_rates[0][0][0])[t] = 0u;
-      if (t < 2 * 32) {
-        (&ctl->flow_groups[0][0])[t] = 0u;
-        (&ctl->flow_decision[0][0])[t] = 0u;
-      }
-      if (t == 0) {
-        ctl->flow_resolved = 0u;
-        ctl->flow_cautious = 0u;
-      }
-    }
-    // the viscosity pass, as the barrier kernel runs it
-    if (t < 64) {
-      const uint32_t group = blockIdx.x % kXcds;
-      const bool have = rank < fixed_tiles(sh, group);
-      const uint32_t tile = group_begin(sh, group) + rank;
-      tile_scan(tile_load(sh, tile, have, prm.cell_start, t), tstate[2], t);
-    }
-    if (blockIdx.x == 0 && t < kXcds)
-      __hip_atomic_store(&ctl->tile_next[1][t][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const Glob g{a.P, a.V, a.kappa};
-    tiled_pass<FORMAT>(a, sh, kViscousPass, 0u, lds, tstate, grab_s, g, wsum, wcnt);
-    if (!grid_barrier(ctl, false)) return;
-  }
-  if (t <= kXcds) begin_s[t] = ctl->flow_begin[t];
-  if (t == 0) log_base_s = ctl->log_len;
-  __syncthreads();
   const uint32_t log_base = log_base_s;
   flow_passes<FORMAT>(a, sh, lds, tstate, task_s, ready_s, wsum, wcnt, begin_s, log_base);
   if (!grid_barrier(ctl, false)) return;
-  const uint32_t r = __hip_atomic_load(&ctl->flow_resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t r = __hip_atomic_load(&flow_words(a).resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (!(r & kResolvedStop)) return;  // a wait gave up; the world is flagged
   const uint32_t stopped = r & ~kResolvedStop;
-  // iteration `stopped`'s velocities are in V2 when it is odd
   if (stopped & 1u)
     for (uint32_t i = blockIdx.x * kSolveThreads + t; i < prm.n; i += gridDim.x * kSolveThreads)
       a.V[i] = coherent4(a.V2, i);
-  if (blockIdx.x == 0 && t < 2 * kXcds)
-    __hip_atomic_store(&ctl->tile_next[t / kXcds][t % kXcds][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (blockIdx.x == 0 && t == 0) {
     ctl->iterations[a.which] = stopped;
     ctl->iterations_step[a.which] += stopped;

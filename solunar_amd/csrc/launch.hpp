@@ -70,9 +70,10 @@ void launch_flow_totals(hipStream_t s, const float4* V, uint32_t n, float rest, 
 hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
 int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes);
 // The flow solve's buffers: a second velocity buffer (n), and per solve tile
-// its passes finished, its particles and the list of non-empty tiles
-// (tile_cap each; tile_sum and tile_count then hold 2 * tile_cap). All null:
-// the tiled solve's passes meet at grid barriers instead.
+// its passes finished (2 * tile_cap: a half for each solve of the substep),
+// its weight and the list of non-empty tiles (tile_cap each; tile_sum and
+// tile_count then hold 2 * tile_cap). All null: the tiled solve's passes
+// meet at grid barriers instead.
 struct FlowBuffers {
   float4* v2 = nullptr;
   uint32_t* tile_done = nullptr;

@@ -945,7 +945,7 @@ int32_t World::ensure_cells(uint64_t volume) {
   SFL_HIP(stop_sum_.reserve(2 * stop, 0, stream_), "hipMalloc(stop sums)");
   SFL_HIP(stop_count_.reserve(2 * stop, 0, stream_), "hipMalloc(stop counts)");
   if (flow_ && format_ == kCompact) {
-    SFL_HIP(flow_done_.reserve(stop, 0, stream_), "hipMalloc(flow tiles)");
+    SFL_HIP(flow_done_.reserve(2 * stop, 0, stream_), "hipMalloc(flow tiles)");  // one half a solve
     SFL_HIP(flow_work_.reserve(stop, 0, stream_), "hipMalloc(flow tiles)");
     SFL_HIP(flow_list_.reserve(stop, 0, stream_), "hipMalloc(flow tiles)");
   }

@@ -293,14 +293,16 @@ class World {
   uint32_t stop_cap() const {
     return (uint32_t)std::min<size_t>(std::min(stop_sum_.cap, stop_count_.cap) / 2, 0xffffffffu);
   }
-  // the flow solve's per tile passes, particles and list of non-empty
-  // tiles (stop_cap() each); flow_ is false when SOLUNAR_SOLVE_FLOW=0 makes
+  // the flow solve's per tile passes (stop_cap() for each of the two
+  // solves), weights and list of non-empty tiles (stop_cap() each); flow_ is
+  // false when SOLUNAR_SOLVE_FLOW=0 makes
   // the tiled solve's passes meet at grid barriers
   DeviceBuffer<uint32_t> flow_done_, flow_work_, flow_list_;
   bool flow_ = true;
   FlowBuffers flow_buffers(float4* v2) const {
     FlowBuffers f;
-    if (flow_ && flow_done_.cap >= stop_cap() && flow_work_.cap >= stop_cap() && flow_list_.cap >= stop_cap()) {
+    if (flow_ && flow_done_.cap >= 2 * (size_t)stop_cap() && flow_work_.cap >= stop_cap() &&
+        flow_list_.cap >= stop_cap()) {
       f.v2 = v2;
       f.tile_done = flow_done_.ptr;
       f.tile_work = flow_work_.ptr;
