@@ -1658,7 +1658,7 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
   // slab v starts at the first tile whose preceding tiles weigh at least
   // ceil(total * v / 8) (bt); its list entries start after the non-empty
   // tiles before it
-  __shared__ uint32_t bt[kXcds + 1], bl[kXcds + 1], wave_n[kSolveWaves];
+  __shared__ uint32_t bt[kXcds + 1], bl[kXcds + 1];
   if (t == 0) {
     bt[0] = 0u;
     bl[0] = 0u;
@@ -1691,14 +1691,15 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
   // earlier in its slab's order (and the neighbouring slabs' at about the
   // same place in theirs), so a pass can start while the last one is still
   // finishing, where x-major order would make it wait for a whole plane.
+  // Wave v lists slab v, 64 candidates at a time.
   const uint32_t plane = sh.nty * sh.ntz, lane = t & 63u, wave = t >> 6;
-  for (uint32_t v = 0; v < kXcds; ++v) {
-    if (bt[v] >= bt[v + 1]) continue;
+  if (wave < kXcds && bt[wave] < bt[wave + 1]) {
+    const uint32_t v = wave;
     const uint32_t x0 = bt[v] / plane, x1 = (bt[v + 1] - 1) / plane, nx = x1 - x0 + 1;
     const uint32_t candidates = sh.nty * nx * sh.ntz;
     uint32_t at = bl[v];
-    for (uint32_t c0 = 0; c0 < candidates; c0 += kSolveThreads) {
-      const uint32_t e = c0 + t;
+    for (uint32_t c0 = 0; c0 < candidates; c0 += 64u) {
+      const uint32_t e = c0 + lane;
       uint32_t idx = 0;
       bool keep = false;
       if (e < candidates) {
@@ -1707,17 +1708,8 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
         keep = idx >= bt[v] && idx < bt[v + 1] && a.tile_work[idx] != 0u;
       }
       const uint64_t m = __ballot(keep);
-      const uint32_t before = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-      if (lane == 0) wave_n[wave] = (uint32_t)__popcll(m);
-      __syncthreads();
-      uint32_t off = at, all = 0;
-      for (uint32_t w = 0; w < kSolveWaves; ++w) {
-        if (w < wave) off += wave_n[w];
-        all += wave_n[w];
-      }
-      if (keep) a.flow_list[off + before] = idx;
-      at += all;
-      __syncthreads();
+      if (keep) a.flow_list[at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = idx;
+      at += (uint32_t)__popcll(m);
     }
   }
   // both solves' control words
@@ -2456,8 +2448,6 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   a.tile_work = flow.tile_work;
   a.flow_list = flow.tile_list;
   a.through = flowing ? 1u : 0u;
-  // the flow solve's tiles, once a substep, before its divergence solve
-  if (flowing && which == 0) hipLaunchKernelGGL(k_flow_tiles, dim3(1), dim3(kSolveThreads), 0, s, a);
   // a multiple of 8 blocks so every XCD gets the same share; at least one
   // chunk per XCD group or the small grid of a small world
   uint32_t blocks = (uint32_t)grid;
@@ -2499,6 +2489,21 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   if (cooperative)
     return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(kSolveThreads), args, (unsigned int)lds, s);
   return hipLaunchKernel(fn, dim3(blocks), dim3(kSolveThreads), args, lds, s);
+}
+
+hipError_t launch_flow_tiles(hipStream_t s, const StepParams& prm, double* tile_sum, uint32_t* tile_count,
+                             uint32_t tile_cap, const FlowBuffers& flow) {
+  if (prm.n == 0 || !flow.tile_done || !flow.tile_work || !flow.tile_list) return hipSuccess;
+  SolveArgs a{};
+  a.prm = prm;
+  a.tile_sum = tile_sum;
+  a.tile_count = tile_count;
+  a.tile_cap = tile_cap;
+  a.tile_done = flow.tile_done;
+  a.tile_work = flow.tile_work;
+  a.flow_list = flow.tile_list;
+  hipLaunchKernelGGL(k_flow_tiles, dim3(1), dim3(kSolveThreads), 0, s, a);
+  return hipGetLastError();
 }
 
 void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,

@@ -85,6 +85,10 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
                               uint32_t tile_cap, int which, float tolerance, uint32_t min_it, uint32_t max_it,
                               int grid, uint32_t tile_bytes, const float4* vin = nullptr, float* forces = nullptr,
                               const FlowBuffers& flow = FlowBuffers());
+// The flow solve's tiles for a substep (after the sort, before the
+// divergence solve; it needs nothing the density pass writes).
+hipError_t launch_flow_tiles(hipStream_t s, const StepParams& prm, double* tile_sum, uint32_t* tile_count,
+                             uint32_t tile_cap, const FlowBuffers& flow);
 // Whether the solve runs on LDS tiles (then its density solve also runs the
 // viscosity pass, given the velocities before it).
 bool dfsph_solve_tiled(const StepParams& prm, uint32_t tile_bytes);
