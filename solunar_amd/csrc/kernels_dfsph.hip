@@ -392,8 +392,10 @@ constexpr uint64_t kBarrierTicks = 200000000ull;  // 2 s
 // 32-deep instead of 256-deep atomic chains, each word on its own cache
 // line: 5.4 (7.1 with the acquire) instead of 29 microseconds per barrier
 // over 256 blocks. Returns false if it waited past the limit, having
-// flagged the world.
-__device__ bool grid_barrier(Control* ctl, bool acquire) {
+// flagged the world; the limit counts from the last change of `heartbeat`
+// when one is given (a block that has no work in the flow solve waits for
+// the whole solve, which moves the word at every stop decision).
+__device__ bool grid_barrier(Control* ctl, bool acquire, const uint32_t* heartbeat = nullptr) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -416,8 +418,16 @@ __device__ bool grid_barrier(Control* ctl, bool acquire) {
       __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       uint64_t start = wall_clock64();
+      uint32_t beat = heartbeat ? __hip_atomic_load(heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
       while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
         __builtin_amdgcn_s_sleep(1);
+        if (heartbeat) {
+          const uint32_t now = __hip_atomic_load(heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (now != beat) {
+            beat = now;
+            start = wall_clock64();
+          }
+        }
         if (wall_clock64() - start > kBarrierTicks) {
           ok = 0;
           atomicOr(&ctl->flags, kFlagBarrier);
@@ -2057,7 +2067,7 @@ __global__ __launch_bounds__(1024) void k_dfsph_flow(SolveArgs a) {
   }
   const uint32_t log_base = log_base_s;
   flow_passes<FORMAT>(a, sh, lds, tstate, task_s, ready_s, wsum, wcnt, begin_s, log_base);
-  if (!grid_barrier(ctl, false)) return;
+  if (!grid_barrier(ctl, false, &flow_words(a).resolved)) return;
   const uint32_t r = __hip_atomic_load(&flow_words(a).resolved, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (!(r & kResolvedStop)) return;  // a wait gave up; the world is flagged
   const uint32_t stopped = r & ~kResolvedStop;
