@@ -214,11 +214,7 @@ int32_t World::init_device() {
 
 World::~World() {
   DeviceGuard g(device_);
-  if (side_) (void)hipStreamSynchronize(side_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  if (side_begin_) (void)hipEventDestroy(side_begin_);
-  if (side_end_) (void)hipEventDestroy(side_end_);
-  if (side_) (void)hipStreamDestroy(side_);
   for (auto& p : pending_) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -1117,29 +1113,16 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   const sfl_settings& s = settings_;
   int32_t st = enqueue_sort();
   if (st != SFL_OK) return st;
-  // the flow solve's tile set-up needs only the sort: it runs on the side
-  // stream beside the density pass, and the solve waits for it
-  const FlowBuffers flow = flow_buffers(va_.ptr);
-  const bool flowing = dfsph_solve_tiled(prm, solve_lds_) && flow.v2;
-  if (flowing) {
-    if (!side_) {
-      SFL_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "hipStreamCreate");
-      SFL_HIP(hipEventCreateWithFlags(&side_begin_, hipEventDisableTiming), "hipEventCreate");
-      SFL_HIP(hipEventCreateWithFlags(&side_end_, hipEventDisableTiming), "hipEventCreate");
-    }
-    SFL_HIP(hipEventRecord(side_begin_, stream_), "hipEventRecord");
-    SFL_HIP(hipStreamWaitEvent(side_, side_begin_, 0), "hipStreamWaitEvent");
-    SFL_HIP(launch_flow_tiles(side_, prm, stop_sum_.ptr, stop_count_.ptr, stop_cap(), flow), "flow tiles");
-    SFL_HIP(hipEventRecord(side_end_, side_), "hipEventRecord");
-  }
   phase_begin(SFL_PHASE_DENSITY);
-  const hipError_t dens = launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm);
+  SFL_HIP(launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm), "density pass");
   phase_end(SFL_PHASE_DENSITY);
-  // the wait goes in whatever the density launch did, so no later sync of
-  // the main stream leaves the side stream's work running
-  if (flowing) SFL_HIP(hipStreamWaitEvent(stream_, side_end_, 0), "hipStreamWaitEvent");
-  SFL_HIP(dens, "density pass");
   phase_begin(SFL_PHASE_SOLVE);
+  // the flow solve's tile set-up, once a substep, on the world's stream: a
+  // second stream would take one of the process's few hardware queues from
+  // every other stream (a tape's replay stream among them)
+  const FlowBuffers flow = flow_buffers(va_.ptr);
+  if (dfsph_solve_tiled(prm, solve_lds_) && flow.v2)
+    SFL_HIP(launch_flow_tiles(stream_, prm, stop_sum_.ptr, stop_count_.ptr, stop_cap(), flow), "flow tiles");
   SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, stop_sum_.ptr,
                              stop_count_.ptr, stop_cap(), 0, s.divergence_tolerance, s.min_iterations, s.max_iterations,
                              solve_grid_, solve_lds_, nullptr, nullptr, flow_buffers(va_.ptr)),

@@ -248,11 +248,7 @@ class World {
 
   // device
   int device_ = 0;
-  hipStream_t stream_ = nullptr;
-  // a second stream for the flow solve's tile set-up beside the density
-  // pass, and the events that order it after the sort and before the solve
-  hipStream_t side_ = nullptr;
-  hipEvent_t side_begin_ = nullptr, side_end_ = nullptr;
+  hipStream_t stream_ = nullptr;  // the world's only stream
   Control* ctl_ = nullptr;       // device
   Control* ctl_host_ = nullptr;  // pinned host mirror
   int solve_grid_ = 0;
