@@ -246,6 +246,7 @@ BatchArgs Batch::args() const {
   a.n = (uint32_t)n_;
   a.scenes = (uint32_t)desc_.size();
   a.radius = radius_;
+  a.rinv = 1.0f / radius_;
   a.reach = reach_;
   a.scale = scale_;
   a.mass = mass_;

@@ -70,7 +70,7 @@ struct SceneState {
 // Everything a batch kernel reads besides the particle arrays.
 struct BatchArgs {
   uint32_t n, scenes;
-  float radius, reach, scale, mass, rest, stiffness, soften, half_spacing;
+  float radius, rinv, reach, scale, mass, rest, stiffness, soften, half_spacing;
   float gx, gy, gz;
   float sound_speed, viscosity, courant;
   uint32_t max_substeps;

@@ -56,6 +56,7 @@ __device__ __forceinline__ StepParams scene_params(const BatchArgs& b, const Sce
   p.terrain = TerrainView{};
   p.n = b.n;
   p.radius = b.radius;
+  p.rinv = b.rinv;
   p.reach = b.reach;
   p.scale = b.scale;
   p.mass = b.mass;
@@ -389,13 +390,13 @@ __global__ __launch_bounds__(BS) void k_b_density(BatchArgs b, const float4* __r
     float sum = 0.0f;
     visit_dense(P, prm.cell_start, lo, span, at, prm.radius, prm.reach,
                 [&](uint32_t, float4, float, float, float, float sq) {
-                  sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+                  sum += kernel_value(sqrtf(sq), prm.radius, prm.rinv, prm.scale);
                 });
     float wsum = 0.0f;
     if (prm.wall_cells)  // a batch's walls are dense (searched ones are refused)
       visit_dense(prm.walls, prm.wall_start, prm.wall_box.lo, prm.wall_box.span, at, prm.radius, prm.reach,
                   [&](uint32_t, float4, float, float, float, float sq) {
-                    wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+                    wsum += kernel_value(sqrtf(sq), prm.radius, prm.rinv, prm.scale);
                   });
     float density = prm.mass * (sum + wsum);
     float pressure = tait_pressure(density, prm.rest, prm.stiffness);
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __
         [&](uint32_t j, float4, float ox, float oy, float oz, float sq, Nothing) {
           if (sq <= 0.0f) return;
           float distance = sqrtf(sq);
-          float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
+          float f = kernel_slope(distance, prm.radius, prm.rinv, prm.scale) / distance;
           float gx = ox * f, gy = oy * f, gz = oz * f;
           const float4 vj = VB[j];
           float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
@@ -463,7 +464,7 @@ __global__ __launch_bounds__(BS, 8) void k_b_force(BatchArgs b, const float4* __
       };
       walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
         float gx, gy, gz;
-        kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+        kernel_gradient(ox, oy, oz, sq, prm.radius, prm.rinv, prm.scale, gx, gy, gz);
         float kk = prm.mass * mirrored;
         px = px - gx * kk;
         py = py - gy * kk;

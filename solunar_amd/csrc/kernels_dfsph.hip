@@ -130,7 +130,7 @@ struct MaskOf<kMasks32> {
 // density pass evaluates, so the same bits.
 __device__ __forceinline__ float pair_factor(float sq, const StepParams& prm) {
   float distance = sqrtf(sq);
-  return distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.scale) / distance;
+  return distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.rinv, prm.scale) / distance;
 }
 
 // Walks particle i's fluid neighbours from its masks in the reference's
@@ -233,8 +233,8 @@ __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restri
     float4 gfac = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     auto fluid = [&](uint32_t j, float ox, float oy, float oz, float sq) {
       float distance = sqrtf(sq);
-      sum += kernel_value(distance, prm.radius, prm.scale);
-      float f = distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.scale) / distance;
+      sum += kernel_value(distance, prm.radius, prm.rinv, prm.scale);
+      float f = distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.rinv, prm.scale) / distance;
       float gx = ox * f, gy = oy * f, gz = oz * f;
       float qx = gx * prm.mass, qy = gy * prm.mass, qz = gz * prm.mass;
       sx = sx + qx;
@@ -345,8 +345,8 @@ __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restri
       visit_walls<KEYED>(prm, at,
                  [&](uint32_t w, float4, float ox, float oy, float oz, float sq) {
                    float distance = sqrtf(sq);
-                   wsum += kernel_value(distance, prm.radius, prm.scale);
-                   float f = distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.scale) / distance;
+                   wsum += kernel_value(distance, prm.radius, prm.rinv, prm.scale);
+                   float f = distance <= 0.0f ? 0.0f : kernel_slope(distance, prm.radius, prm.rinv, prm.scale) / distance;
                    float gx = ox * f, gy = oy * f, gz = oz * f;
                    sx = sx + gx * prm.mass;
                    sy = sy + gy * prm.mass;

@@ -472,13 +472,13 @@ __global__ __launch_bounds__(256) void k_list_density(const float4* __restrict__
   for (uint32_t k = fstart[i]; k < fstart[i + 1]; ++k) {
     float4 o = P[fitems[k]];
     float ox = o.x - at.x, oy = o.y - at.y, oz = o.z - at.z;
-    sum += kernel_value(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), prm.radius, prm.scale);
+    sum += kernel_value(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), prm.radius, prm.rinv, prm.scale);
   }
   float wsum = 0.0f;
   for (uint32_t k = wstart[i]; k < wstart[i + 1]; ++k) {
     float4 o = prm.walls[witems[k]];
     float ox = o.x - at.x, oy = o.y - at.y, oz = o.z - at.z;
-    wsum += kernel_value(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), prm.radius, prm.scale);
+    wsum += kernel_value(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), prm.radius, prm.rinv, prm.scale);
   }
   float density = prm.mass * (sum + wsum);
   if (prm.terrain.count) {

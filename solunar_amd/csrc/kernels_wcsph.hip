@@ -33,12 +33,12 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
     float4 at = P[i];
     float sum = 0.0f;
     visit_fluid<SEARCH>(P, prm, at, [&](uint32_t, float4, float, float, float, float sq) {
-      sum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+      sum += kernel_value(sqrtf(sq), prm.radius, prm.rinv, prm.scale);
     });
     float wsum = 0.0f;
     if (prm.wall_cells)
       visit_walls<SEARCH>(prm, at, [&](uint32_t, float4, float, float, float, float sq) {
-        wsum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+        wsum += kernel_value(sqrtf(sq), prm.radius, prm.rinv, prm.scale);
       });
     float density;
     if (MOVERS) {
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_wcsph_density(const float4* __restrict_
       float msum = 0.0f;
       uint32_t count = 0;
       visit_movers(prm.movers, at, prm.radius, prm.reach, [&](uint32_t, float4, float, float, float, float sq) {
-        msum += kernel_value(sqrtf(sq), prm.radius, prm.scale);
+        msum += kernel_value(sqrtf(sq), prm.radius, prm.rinv, prm.scale);
         ++count;
       });
       prm.movers.pairs.counts[i] = count;
@@ -87,7 +87,7 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
   auto pair = [&](float ox, float oy, float oz, float sq, float4 vj, float sj) {
     if (sq <= 0.0f) return;
     float distance = sqrtf(sq);
-    float f = kernel_slope(distance, prm.radius, prm.scale) / distance;
+    float f = kernel_slope(distance, prm.radius, prm.rinv, prm.scale) / distance;
     float gx = ox * f, gy = oy * f, gz = oz * f;
     float toward = rmin(dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, ox, oy, oz), 0.0f);
     float damp = -prm.nu * toward / (sq + prm.soften) * 2.0f / (density + vj.w);
@@ -126,7 +126,7 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
   if (prm.wall_cells) {
     walls([&](uint32_t, float4, float ox, float oy, float oz, float sq) {
       float gx, gy, gz;
-      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.rinv, prm.scale, gx, gy, gz);
       float k = prm.mass * mirrored;
       px = px - gx * k;
       py = py - gy * k;
@@ -152,7 +152,7 @@ __device__ __forceinline__ void wcsph_push(const float4* __restrict__ PB, const 
       float toward = dot3(vi.x - v.x, vi.y - v.y, vi.z - v.z, ox, oy, oz);
       float damp = toward < 0.0f ? -mv.damping * toward / (sq + prm.soften) / density : 0.0f;
       float gx, gy, gz;
-      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.scale, gx, gy, gz);
+      kernel_gradient(ox, oy, oz, sq, prm.radius, prm.rinv, prm.scale, gx, gy, gz);
       gx = gx * prm.mass;
       gy = gy * prm.mass;
       gz = gz * prm.mass;

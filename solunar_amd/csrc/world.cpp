@@ -799,6 +799,7 @@ StepParams World::params(float dt) const {
   StepParams p{};
   p.n = (uint32_t)n_;
   p.radius = radius_;
+  p.rinv = 1.0f / radius_;
   p.reach = reach_;
   p.scale = scale_;
   p.mass = mass_;
