@@ -210,11 +210,20 @@ __device__ __forceinline__ void store_group(const StepParams& prm, uint32_t i, u
 // than the lists hold, or a neighbour past its cell mask's width, flags the
 // world; later launches of the substep then do nothing and the host regrows
 // or widens. KEYED: the walls' table is searched.
+template <uint32_t THREADS>
+__device__ void flow_tiles_block(const StepParams& prm, const FlowSetup& f);
+
 template <int FORMAT, bool KEYED>
 __global__ __launch_bounds__(256, 8) void k_dfsph_density(const float4* __restrict__ P, float4* __restrict__ V,
-                                                       float* __restrict__ alpha, StepParams prm) {
+                                                       float* __restrict__ alpha, StepParams prm, FlowSetup setup) {
   Control* ctl = prm.ctl;
   if (ctl->abort) return;
+  // the flow solve's tile set-up: the launch's logical last block, one past
+  // the particles' blocks
+  if (setup.tile_done && xcd_block() == gridDim.x - 1) {
+    flow_tiles_block<256>(prm, setup);
+    return;
+  }
   uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
   bool live = i < prm.n;
   float comp = 0.0f;
@@ -1596,48 +1605,54 @@ __device__ __forceinline__ uint32_t tile_particles(const TileShape& sh, uint32_t
   return n;
 }
 
-// The flow solve's tiles for a substep, one block of 1024 threads, before
-// the divergence solve (the cell table holds until the advance): the
-// particles of every tile; the non-empty tiles listed in order (flow_list);
-// the slabs of the XCD groups cut at equal particle counts, as ranges of
+// The flow solve's tiles for a substep, by one block of THREADS threads (the
+// density launch's last block, beside the density pass: the cell table holds
+// until the advance, and nothing here reads what the density pass writes):
+// the particles of every tile; the non-empty tiles listed in order
+// (flow_list); the slabs of the XCD groups cut at equal weights, as ranges of
 // that list (flow_begin: slab v is list entries begin[v] .. begin[v + 1]);
-// empty tiles marked finished for every pass (tile_done = kTaskDone, in
-// the divergence and the density solve's halves; their tile sums zero in
-// both sets), the others not started; both solves' control words cleared. Thread t works the contiguous tiles of segment t.
-__global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
-  Control* ctl = a.prm.ctl;
-  if (list_aborted(ctl)) return;
+// empty tiles marked finished for every pass (tile_done = kTaskDone, in the
+// divergence and the density solve's halves; their tile sums zero in both
+// sets), the others not started; both solves' control words cleared. Thread
+// t works the contiguous tiles of segment t. A substep whose density pass
+// then flags its lists reruns, this with it.
+template <uint32_t THREADS>
+__device__ void flow_tiles_block(const StepParams& prm, const FlowSetup& f) {
+  static_assert(THREADS % 64 == 0 && THREADS >= 64 * kXcds / 2, "whole waves, at most two slabs a wave");
+  Control* ctl = prm.ctl;
+  if (ctl->abort) return;
   const TileShape sh = tile_shape(ctl);
-  if (sh.count > a.tile_cap) return;  // the solve flags it
-  __shared__ uint32_t work_pre[kSolveThreads + 1], live_pre[kSolveThreads + 1];
+  if (sh.count > f.tile_cap) return;  // the solve flags it
+  __shared__ uint32_t work_pre[THREADS + 1], live_pre[THREADS + 1];
   const uint32_t t = threadIdx.x, count = sh.count;
-  const uint32_t per = (count + kSolveThreads - 1) / kSolveThreads;
+  const uint32_t per = (count + THREADS - 1) / THREADS;
   const uint32_t lo = min(t * per, count), hi = min(lo + per, count);
   uint32_t work = 0, live = 0;
   for (uint32_t k = lo; k < hi; ++k) {
-    const uint32_t n = tile_particles(sh, k, a.prm.cell_start);
-    a.tile_done[k] = n ? 0u : kTaskDone;
-    a.tile_done[a.tile_cap + k] = n ? 0u : kTaskDone;
+    const uint32_t n = tile_particles(sh, k, prm.cell_start);
+    f.tile_done[k] = n ? 0u : kTaskDone;
+    f.tile_done[f.tile_cap + k] = n ? 0u : kTaskDone;
     if (!n) {
-      a.tile_sum[k] = 0.0;
-      a.tile_count[k] = 0u;
-      a.tile_sum[a.tile_cap + k] = 0.0;
-      a.tile_count[a.tile_cap + k] = 0u;
+      f.tile_sum[k] = 0.0;
+      f.tile_count[k] = 0u;
+      f.tile_sum[f.tile_cap + k] = 0.0;
+      f.tile_count[f.tile_cap + k] = 0u;
     }
     // a task costs about as much as 512 particles besides its particles:
     // slabs of sparse tiles take more tasks for the same particles
     const uint32_t w = n ? n + kTaskWeight : 0u;
-    a.tile_work[k] = w;
+    f.tile_work[k] = w;
     work += w;
     live += n ? 1u : 0u;
   }
   work_pre[t] = work;
   live_pre[t] = live;
   __syncthreads();
-  // exclusive scans of both by one wave, 16 segments a lane
+  // exclusive scans of both by one wave, THREADS / 64 segments a lane
+  constexpr uint32_t kSeg = THREADS / 64;
   if (t < 64) {
     uint32_t rw = 0, rl = 0;
-    for (uint32_t k = t * 16; k < t * 16 + 16; ++k) {
+    for (uint32_t k = t * kSeg; k < t * kSeg + kSeg; ++k) {
       rw += work_pre[k];
       rl += live_pre[k];
     }
@@ -1650,7 +1665,7 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
       }
     }
     uint32_t aw = iw - rw, al = il - rl;
-    for (uint32_t k = t * 16; k < t * 16 + 16; ++k) {
+    for (uint32_t k = t * kSeg; k < t * kSeg + kSeg; ++k) {
       const uint32_t vw = work_pre[k], vl = live_pre[k];
       work_pre[k] = aw;
       live_pre[k] = al;
@@ -1658,12 +1673,12 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
       al += vl;
     }
     if (t == 63) {
-      work_pre[kSolveThreads] = iw;
-      live_pre[kSolveThreads] = il;
+      work_pre[THREADS] = iw;
+      live_pre[THREADS] = il;
     }
   }
   __syncthreads();
-  const uint64_t total = work_pre[kSolveThreads];
+  const uint64_t total = work_pre[THREADS];
   const uint32_t pw = work_pre[t], pl = live_pre[t];
   // slab v starts at the first tile whose preceding tiles weigh at least
   // ceil(total * v / 8) (bt); its list entries start after the non-empty
@@ -1673,7 +1688,7 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
     bt[0] = 0u;
     bl[0] = 0u;
     bt[kXcds] = count;
-    bl[kXcds] = live_pre[kSolveThreads];
+    bl[kXcds] = live_pre[THREADS];
   }
   for (uint32_t v = 1; v < kXcds; ++v) {
     const uint64_t target = (total * v + kXcds - 1) / kXcds;
@@ -1686,7 +1701,7 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
       uint64_t run = pw;
       uint32_t k = lo, listed = pl;
       while (run < target) {
-        const uint32_t n = a.tile_work[k++];
+        const uint32_t n = f.tile_work[k++];
         run += n;
         listed += n ? 1u : 0u;
       }
@@ -1701,10 +1716,10 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
   // earlier in its slab's order (and the neighbouring slabs' at about the
   // same place in theirs), so a pass can start while the last one is still
   // finishing, where x-major order would make it wait for a whole plane.
-  // Wave v lists slab v, 64 candidates at a time.
+  // Wave w lists slabs w, w + THREADS / 64, ..., 64 candidates at a time.
   const uint32_t plane = sh.nty * sh.ntz, lane = t & 63u, wave = t >> 6;
-  if (wave < kXcds && bt[wave] < bt[wave + 1]) {
-    const uint32_t v = wave;
+  for (uint32_t v = wave; v < kXcds; v += THREADS / 64) {
+    if (bt[v] >= bt[v + 1]) continue;
     const uint32_t x0 = bt[v] / plane, x1 = (bt[v + 1] - 1) / plane, nx = x1 - x0 + 1;
     const uint32_t candidates = sh.nty * nx * sh.ntz;
     uint32_t at = bl[v];
@@ -1715,16 +1730,16 @@ __global__ __launch_bounds__(1024) void k_flow_tiles(SolveArgs a) {
       if (e < candidates) {
         const uint32_t ty = e / (nx * sh.ntz), tx = x0 + (e / sh.ntz) % nx, tz = e % sh.ntz;
         idx = (tx * sh.nty + ty) * sh.ntz + tz;
-        keep = idx >= bt[v] && idx < bt[v + 1] && a.tile_work[idx] != 0u;
+        keep = idx >= bt[v] && idx < bt[v + 1] && f.tile_work[idx] != 0u;
       }
       const uint64_t m = __ballot(keep);
-      if (keep) a.flow_list[at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = idx;
+      if (keep) f.tile_list[at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = idx;
       at += (uint32_t)__popcll(m);
     }
   }
   // both solves' control words
   uint32_t* words = reinterpret_cast<uint32_t*>(&ctl->flow[0]);
-  for (uint32_t k = t; k < 2 * sizeof(FlowWords) / 4; k += kSolveThreads) words[k] = 0u;
+  for (uint32_t k = t; k < 2 * sizeof(FlowWords) / 4; k += THREADS) words[k] = 0u;
 }
 
 // A task's neighbour wait, by the control wave: every tile within one of T
@@ -2349,12 +2364,16 @@ static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)(
       hipLaunchKernelGGL((KERNEL<kLists, ARG>), __VA_ARGS__);                     \
   } while (0)
 
-hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm) {
+hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm,
+                                const FlowSetup& setup) {
   if (prm.n == 0) return hipSuccess;
+  const bool ready = setup.tile_done && setup.tile_work && setup.tile_list && setup.tile_sum && setup.tile_count;
+  const FlowSetup f = ready ? setup : FlowSetup();
+  const uint32_t blocks = blocks_for(prm.n, 256) + (ready ? 1u : 0u);
   if (prm.wall_keys)
-    SFL_FORMAT_LAUNCH_WITH(k_dfsph_density, prm.format, true, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
+    SFL_FORMAT_LAUNCH_WITH(k_dfsph_density, prm.format, true, blocks, 256, 0, s, P, V, alpha, prm, f);
   else
-    SFL_FORMAT_LAUNCH_WITH(k_dfsph_density, prm.format, false, blocks_for(prm.n, 256), 256, 0, s, P, V, alpha, prm);
+    SFL_FORMAT_LAUNCH_WITH(k_dfsph_density, prm.format, false, blocks, 256, 0, s, P, V, alpha, prm, f);
   return hipGetLastError();
 }
 
@@ -2499,21 +2518,6 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
   if (cooperative)
     return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(kSolveThreads), args, (unsigned int)lds, s);
   return hipLaunchKernel(fn, dim3(blocks), dim3(kSolveThreads), args, lds, s);
-}
-
-hipError_t launch_flow_tiles(hipStream_t s, const StepParams& prm, double* tile_sum, uint32_t* tile_count,
-                             uint32_t tile_cap, const FlowBuffers& flow) {
-  if (prm.n == 0 || !flow.tile_done || !flow.tile_work || !flow.tile_list) return hipSuccess;
-  SolveArgs a{};
-  a.prm = prm;
-  a.tile_sum = tile_sum;
-  a.tile_count = tile_count;
-  a.tile_cap = tile_cap;
-  a.tile_done = flow.tile_done;
-  a.tile_work = flow.tile_work;
-  a.flow_list = flow.tile_list;
-  hipLaunchKernelGGL(k_flow_tiles, dim3(1), dim3(kSolveThreads), 0, s, a);
-  return hipGetLastError();
 }
 
 void launch_dfsph_viscous(hipStream_t s, const float4* PB, const float4* VB, const StepParams& prm, float4* VA,

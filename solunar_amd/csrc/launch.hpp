@@ -67,7 +67,20 @@ void launch_emit_append(hipStream_t s, const uint32_t* free, const uint32_t* sta
 void launch_flow_totals(hipStream_t s, const float4* V, uint32_t n, float rest, Control* ctl);
 
 // DFSPH (kernels_dfsph.hip)
-hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm);
+// The flow solve's per-substep tile bookkeeping: both solves' tile sums and
+// counts (tile_cap each), pass counters, weights and list.
+struct FlowSetup {
+  double* tile_sum = nullptr;
+  uint32_t* tile_count = nullptr;
+  uint32_t tile_cap = 0;
+  uint32_t* tile_done = nullptr;
+  uint32_t* tile_work = nullptr;
+  uint32_t* tile_list = nullptr;
+};
+// The DFSPH density pass; with a set-up (tile_done set), one more block of
+// the launch sets the flow solve's tiles up for the substep beside it.
+hipError_t launch_dfsph_density(hipStream_t s, const float4* P, float4* V, float* alpha, const StepParams& prm,
+                                const FlowSetup& setup = FlowSetup());
 int solve_grid_limit(int blocks_per_cu, bool tiles, uint32_t* tile_bytes);
 // The flow solve's buffers: a second velocity buffer (n), and per solve tile
 // its passes finished (2 * tile_cap: a half for each solve of the substep),
@@ -85,10 +98,6 @@ hipError_t launch_dfsph_solve(hipStream_t s, const StepParams& prm, const float4
                               uint32_t tile_cap, int which, float tolerance, uint32_t min_it, uint32_t max_it,
                               int grid, uint32_t tile_bytes, const float4* vin = nullptr, float* forces = nullptr,
                               const FlowBuffers& flow = FlowBuffers());
-// The flow solve's tiles for a substep (after the sort, before the
-// divergence solve; it needs nothing the density pass writes).
-hipError_t launch_flow_tiles(hipStream_t s, const StepParams& prm, double* tile_sum, uint32_t* tile_count,
-                             uint32_t tile_cap, const FlowBuffers& flow);
 // Whether the solve runs on LDS tiles (then its density solve also runs the
 // viscosity pass, given the velocities before it).
 bool dfsph_solve_tiled(const StepParams& prm, uint32_t tile_bytes);

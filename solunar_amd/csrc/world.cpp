@@ -1114,16 +1114,24 @@ int32_t World::enqueue_dfsph_substep(const StepParams& prm) {
   const sfl_settings& s = settings_;
   int32_t st = enqueue_sort();
   if (st != SFL_OK) return st;
+  // the flow solve's tile set-up, once a substep, runs as one more block of
+  // the density launch (a stream of its own would take one of the process's
+  // few hardware queues from every other stream, a tape's replay stream
+  // among them)
+  const FlowBuffers flow = flow_buffers(va_.ptr);
+  FlowSetup setup;
+  if (dfsph_solve_tiled(prm, solve_lds_) && flow.v2) {
+    setup.tile_sum = stop_sum_.ptr;
+    setup.tile_count = stop_count_.ptr;
+    setup.tile_cap = stop_cap();
+    setup.tile_done = flow.tile_done;
+    setup.tile_work = flow.tile_work;
+    setup.tile_list = flow.tile_list;
+  }
   phase_begin(SFL_PHASE_DENSITY);
-  SFL_HIP(launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm), "density pass");
+  SFL_HIP(launch_dfsph_density(stream_, pb_.ptr, vb_.ptr, aux_.ptr, prm, setup), "density pass");
   phase_end(SFL_PHASE_DENSITY);
   phase_begin(SFL_PHASE_SOLVE);
-  // the flow solve's tile set-up, once a substep, on the world's stream: a
-  // second stream would take one of the process's few hardware queues from
-  // every other stream (a tape's replay stream among them)
-  const FlowBuffers flow = flow_buffers(va_.ptr);
-  if (dfsph_solve_tiled(prm, solve_lds_) && flow.v2)
-    SFL_HIP(launch_flow_tiles(stream_, prm, stop_sum_.ptr, stop_count_.ptr, stop_cap(), flow), "flow tiles");
   SFL_HIP(launch_dfsph_solve(stream_, prm, pb_.ptr, vb_.ptr, aux_.ptr, kappa_.ptr, errors_.ptr, stop_sum_.ptr,
                              stop_count_.ptr, stop_cap(), 0, s.divergence_tolerance, s.min_iterations, s.max_iterations,
                              solve_grid_, solve_lds_, nullptr, nullptr, flow_buffers(va_.ptr)),
