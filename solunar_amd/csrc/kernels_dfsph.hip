@@ -1577,7 +1577,7 @@ __device__ void tiled_pass(const SolveArgs& a, const TileShape& sh, int mode, ui
 // publication completes an iteration.
 
 constexpr uint32_t kTaskDone = 0xffffffffu;  // tile_done of an empty tile
-constexpr uint32_t kTaskWeight = 512;       // a task's cost beside its particles, in particles
+constexpr uint32_t kTaskWeight = 768;       // a task's cost beside its particles, in particles (measured)
 constexpr uint32_t kResolvedStop = 0x80000000u;
 
 __device__ __forceinline__ FlowWords& flow_words(const SolveArgs& a) { return a.prm.ctl->flow[a.which]; }
@@ -1638,7 +1638,7 @@ __device__ void flow_tiles_block(const StepParams& prm, const FlowSetup& f) {
       f.tile_sum[f.tile_cap + k] = 0.0;
       f.tile_count[f.tile_cap + k] = 0u;
     }
-    // a task costs about as much as 512 particles besides its particles:
+    // a task costs about as much as kTaskWeight particles besides its particles:
     // slabs of sparse tiles take more tasks for the same particles
     const uint32_t w = n ? n + kTaskWeight : 0u;
     f.tile_work[k] = w;
@@ -1894,9 +1894,11 @@ __device__ void flow_publish(const SolveArgs& a, const TileShape& sh, uint2 task
     }
     const bool settled = k >= a.min_iterations && average <= (double)a.tolerance;
     const uint32_t stop = (settled || k >= a.max_iterations) ? 1u : 0u;
-    // the next iteration may well be the last: its push waits for its own
-    // decision instead of running ahead of it
-    if (!stop && k + 1u >= a.min_iterations && average <= 2.0 * (double)a.tolerance)
+    // the tolerance is met and only the least number of iterations held the
+    // stop back: the next iteration is very likely the last, so its push
+    // waits for its own decision instead of running ahead of it (nearer
+    // the tolerance alone, every push running ahead measured faster)
+    if (!stop && k + 1u >= a.min_iterations && average <= (double)a.tolerance)
       __hip_atomic_store(&fw.cautious, k + 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
     __hip_atomic_store(&fw.decision[par][0], (k + 1u) << 1 | stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
